@@ -1,0 +1,196 @@
+/*
+ * radar_graph.h -- C ABI of the MI355X-native RADAR-Graph traversal engine.
+ *
+ * The reference (CMUAbstract/RADAR-Graph) is five OpenMP copies of Ligra plus
+ * GAP's triangle counting.  Its operator interface is Ligra's C++ template API
+ * (edgeMap / vertexMap / vertexSubset with app functors) and each app's
+ * Compute(graph&, commandLine, pvector<uintE>& new_ids).  Arbitrary C++
+ * functors cannot cross a C ABI, so every functor of the reference maps to a
+ * named device operator (rg_op) and every Compute body maps to one entry point
+ * below.  All signatures use plain pointers and sizes; host pointers unless a
+ * name says "dev".  Every function returns RG_OK (0) or an error code, with a
+ * message in rg_last_error(); nothing here aborts the process.
+ *
+ * Reference citations are relative to the reference tree's src/ directory;
+ * PNO = push-new-orig, PP = push-pull, DS = degree-sorting, HD = hubdup,
+ * RD = RADAR.
+ */
+#ifndef RADAR_GRAPH_H
+#define RADAR_GRAPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------- */
+enum {
+  RG_OK = 0,
+  RG_EINVAL = 1,      /* bad argument (reference: abort/assert paths)          */
+  RG_ENOMEM = 2,      /* host or device allocation failed                      */
+  RG_EDEVICE = 3,     /* HIP runtime error                                      */
+  RG_EIO = 4,         /* file could not be opened / read / written              */
+  RG_EFORMAT = 5,     /* malformed input ("Bad input file", PNO IO.h:235-249)   */
+  RG_EUNSUPPORTED = 6 /* operation not defined for this graph (e.g. directed TC)*/
+};
+
+/* Thread-local text of the last error on the calling thread. */
+const char* rg_last_error(void);
+/* Library version string. */
+const char* rg_version(void);
+
+/* ---- host graphs ---------------------------------------------------------
+ * A CSR in host memory: uint64 offsets[n+1], uint32 neighbours[m]; for
+ * directed graphs also the in-edge CSR (CSC).  Neighbour lists are sorted,
+ * deduplicated and free of self-loops, which is what Ligra's text reader does
+ * (PNO Ligra/ligra/IO.h:364-421 directed, 432-467 symmetric) and what GAP's
+ * SquishCSR does (PNO GAP/src/builder.h:134-163).
+ */
+typedef struct rg_hgraph rg_hgraph;
+
+enum { RG_FMT_ADJ = 0,        /* Ligra AdjacencyGraph text (IO.h:207-475)      */
+       RG_FMT_ADJ_BINARY = 1, /* Ligra .config/.adj/.idx binary (IO.h:477-626) */
+       RG_FMT_EL = 2,         /* GAP edge list text (reader.h:50-57)           */
+       RG_FMT_SG = 3 };       /* GAP serialized graph (reader.h:251-304)       */
+
+/* GAP generator (GAP/src/generator.h:64-124): 2^scale vertices,
+ * degree*2^scale generated edges, R-MAT (A,B,C = 0.57,0.19,0.19) unless
+ * `uniform`; seed kRandSeed = 27491095 (util.h:22).  Raw edge list, before
+ * deduplication.  rg_generate_edge_count gives the array length. */
+int64_t rg_generate_edge_count(int scale, int degree);
+int rg_generate_edges(int scale, int degree, int uniform, uint32_t* src, uint32_t* dst);
+
+/* Generated graph, built with the reader's dedup semantics.  symmetric=1
+ * adds both directions (GAP -g / -u always symmetrize, command_line.h:77-78). */
+int rg_hgraph_generate(int scale, int degree, int uniform, int symmetric, rg_hgraph** out);
+/* Graph from an edge list (duplicates and self-loops removed). */
+int rg_hgraph_from_edges(int64_t n, int64_t nedges, const uint32_t* src,
+                         const uint32_t* dst, int symmetric, rg_hgraph** out);
+/* Graph from an existing CSR.  dedup=1 applies the reader's sort/unique/
+ * self-loop removal; dedup=0 takes the lists as they are (binary reader). */
+int rg_hgraph_from_csr(int64_t n, int64_t m, const uint64_t* offsets,
+                       const uint32_t* nbrs, int symmetric, int dedup, rg_hgraph** out);
+int rg_hgraph_read(const char* path, int format, int symmetric, rg_hgraph** out);
+int rg_hgraph_write(const rg_hgraph* g, const char* path, int format);
+int rg_hgraph_info(const rg_hgraph* g, int64_t* n, int64_t* m, int* symmetric);
+/* Borrowed views (valid until rg_hgraph_free).  in_edges=1 selects the CSC
+ * of a directed graph (for symmetric graphs both views are the CSR). */
+const uint64_t* rg_hgraph_offsets(const rg_hgraph* g, int in_edges);
+const uint32_t* rg_hgraph_neighbors(const rg_hgraph* g, int in_edges);
+void rg_hgraph_free(rg_hgraph* g);
+
+/* ---- source pickers (host) ----------------------------------------------- */
+/* GAP SourcePicker (benchmark.h:47-68): mt19937(27491095) +
+ * uniform_int_distribution<int32>(0,n-1), rejecting out-degree 0. */
+int rg_pick_sources_gap(const rg_hgraph* g, int count, uint32_t* out);
+/* BC's picker (BC-manyiter-v2.C:306-325): mt19937(27491095) +
+ * uniform_int_distribution<long>(0,n-1), rejecting out-degree 0 in the
+ * orientation BC sees for that source (directed graphs alternate, because BC
+ * transposes once per source).  Original ids. */
+int rg_pick_sources_bc(const rg_hgraph* g, int count, uint32_t* out);
+/* Ligra hashInt(ulong) (PNO Ligra/ligra/utils.h:347-355), used by Radii
+ * (Radii.C:120-127) as hashInt(i) % n. */
+uint64_t rg_hash_int(uint64_t a);
+
+/* ---- device context ------------------------------------------------------ */
+typedef struct rg_ctx rg_ctx;
+int rg_device_count(int* count);
+/* One context = one device + one HIP stream + preallocated work pools. */
+int rg_ctx_create(int device, rg_ctx** out);
+int rg_ctx_destroy(rg_ctx* ctx);
+int rg_ctx_sync(rg_ctx* ctx);
+
+/* ---- device graphs ------------------------------------------------------- */
+typedef struct rg_graph rg_graph;
+/* Upload a host graph: uint64 offsets, uint32 neighbours, plus the CSC for a
+ * directed graph (Ligra asymmetricVertex keeps both, vertex.h:244-362). */
+int rg_graph_upload(rg_ctx* ctx, const rg_hgraph* hg, rg_graph** out);
+int rg_graph_info(const rg_graph* g, int64_t* n, int64_t* m, int* symmetric);
+/* Copy the current orientation's CSR (in_edges=0) or CSC back to the host.
+ * offsets has n+1 entries, nbrs m entries. */
+int rg_graph_download(const rg_graph* g, int in_edges, uint64_t* offsets, uint32_t* nbrs);
+/* Ligra graph::transpose() (graph.h:78-86): swap CSR and CSC of a directed
+ * graph; no-op for symmetric graphs. */
+int rg_graph_transpose(rg_graph* g);
+int rg_graph_free(rg_graph* g);
+
+/* DegreeSorting on the GPU (PNO/DS Ligra/ligra/IO.h:729-944): order vertices
+ * by (degree, id) descending -- std::greater on (degree, id) pairs, IO.h:755 --
+ * and rebuild CSR (+CSC) with relabelled neighbours.  by_outdeg=1 sorts by
+ * out-degree (-preprocess 0), 0 by in-degree (-preprocess 1; symmetric graphs
+ * use their degree either way).  sort_neighbors=1 is the strict PNO copy
+ * (lists sorted), 0 the relaxed DS/RD copy (IO.h:725-726: original order,
+ * relabelled).  new_ids (host, n entries, nullable) receives new_ids[orig].
+ * The result is a new device graph; `g` is left unchanged. */
+int rg_graph_degree_sort(rg_graph* g, int by_outdeg, int sort_neighbors,
+                         rg_graph** out, uint32_t* new_ids);
+
+/* ---- apps ------------------------------------------------------------------
+ * Variants (the five reference trees):
+ *   RG_PUSH     push-only edgeMap, dense-forward (PNO; also DS on a sorted graph)
+ *   RG_PULL     Ligra direction switching with pull dense rounds (PP)
+ *   RG_HUBDUP   push with private hub copies + merge (HD), hubs = top-k
+ *               in-degree vertices found by a degree threshold
+ *   RG_RADAR    HUBDUP with hubs = ids 0..k-1 (RD, expects a degree-sorted graph)
+ * Hub placement: RG_HUBS_LDS (one copy per workgroup in LDS) or
+ * RG_HUBS_L2 (one copy per XCD in global memory, L2-resident).
+ */
+enum { RG_PUSH = 0, RG_PULL = 1, RG_HUBDUP = 2, RG_RADAR = 3 };
+enum { RG_HUBS_LDS = 0, RG_HUBS_L2 = 1 };
+
+typedef struct rg_opts {
+  int variant;          /* RG_PUSH ...                                        */
+  int hub_placement;    /* RG_HUBS_LDS / RG_HUBS_L2                           */
+  int64_t hubs;         /* k; <= 0 selects the budget default                 */
+  int64_t threshold;    /* sparse/dense switch; -1 = m/20 (ligra.h:269)       */
+  int wg_size;          /* workgroup size for edge kernels; 0 = default       */
+  int heavy_degree;     /* wave-per-vertex cut-over degree; 0 = default       */
+} rg_opts;
+
+/* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */
+void rg_opts_default(rg_opts* o);
+
+/* PageRank (PNO Ligra/apps/PageRank.C:92-141; HD PageRank.C; RD PageRank.C):
+ * ranks start at 1/n; per iteration p_next[d] += p_curr[s]/outdeg(s) over all
+ * arcs, p_next = 0.85 p_next + 0.15/n, L1 = sum |p_curr - p_next|, stop when
+ * L1 < 1e-7 or after maxiters.  iters_out gets the reference's printed
+ * "Num Iters" (maxiters+1 when not converged), l1_out the last L1, ranks_out
+ * (host, n, nullable) the last computed ranks.  ms_out (nullable) gets the
+ * device time of the iteration loop in milliseconds. */
+int rg_pagerank(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
+                int64_t* iters_out, double* l1_out, double* ms_out);
+
+/* BFS (PNO Ligra/apps/BFS.C:213-258): parents via first successful CAS,
+ * parents[src] = src, UINT32_MAX when unreached.  iters_out = rounds
+ * (printed "Num. Iters"), reached_out = tree size, edges_out = sum of
+ * out-degrees of reached vertices (the traversed-edge count for TEPS). */
+int rg_bfs(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
+           int64_t* iters_out, int64_t* reached_out, int64_t* edges_out, double* ms_out);
+
+/* Radii (PNO Ligra/apps/Radii.C:98-157): bit-parallel BFS from up to 64
+ * sources (source i owns bit i); radii[v] = last round in which v's visited
+ * word changed, -1 if unreached.  rounds_out = printed "Iters". */
+int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o,
+             int32_t* radii_out, int64_t* rounds_out, double* ms_out);
+
+/* BC (PNO Ligra/apps/BC-manyiter-v2.C:280-398), sources given in the graph's
+ * own id space, processed in order with the reference's semantics: forward
+ * path counting by level, backward dependency accumulation on the transposed
+ * graph (the transpose is not undone between sources), Dependencies carried
+ * over between sources (zeroed once), final (dep - 1/sigma)/(1/sigma).
+ * combined_out = sum of forward rounds (printed "Combined Iters"). */
+int rg_bc(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o,
+          double* deps_out, int64_t* combined_out, double* ms_out);
+
+/* Triangle counting (PNO GAP/src/tc_local.cc:48-86): u > v > w ordered merge
+ * counting over a symmetric graph with sorted lists; total and per-vertex
+ * counts (each triangle adds 1 to all three corners).  per_vertex nullable. */
+int rg_tc(rg_graph* g, const rg_opts* o, uint64_t* total_out, uint64_t* per_vertex,
+          double* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RADAR_GRAPH_H */

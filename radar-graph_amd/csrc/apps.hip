@@ -1,0 +1,974 @@
+// The reference's app functors as device operators, the host-side edgeMap
+// dispatcher (Ligra's sparse/dense switch) and the app loops (each app's
+// Compute body) of the radar_graph library.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.h"
+#include "engine.h"
+
+namespace rg {
+
+// ---------------------------------------------------------------------------
+// Operators.  Each mirrors one reference functor; citations are to
+// push-new-orig/Ligra/apps/ unless noted.
+
+// PR_F (PageRank.C:29-43): p_next[d] += p_curr[s]/outdeg(s).  contrib[s]
+// holds p_curr[s]/outdeg(s), computed once per vertex by the vertex phase
+// (the same IEEE quotient the reference recomputes per arc).
+struct PrOp {
+  using SrcT = double;
+  using HubT = double;
+  static constexpr bool kOutput = false;
+  const double* contrib;
+  double* acc;
+  const int64_t* done;
+  __device__ bool skip() const { return *done != 0; }
+  __device__ double load_src(uint32_t s) const { return contrib[s]; }
+  __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool update(uint32_t, double sv, uint32_t d) const {
+    unsafeAtomicAdd(acc + d, sv);
+    return false;
+  }
+  __device__ double hub_identity() const { return 0.0; }
+  __device__ void hub_update(double* slot, uint32_t, double sv, uint32_t) const { atomicAdd(slot, sv); }
+  __device__ bool hub_flush(uint32_t d, double v) const {
+    unsafeAtomicAdd(acc + d, v);
+    return false;
+  }
+};
+
+// BFS_F (BFS.C:169-190): first CAS on Parents[d] wins and d joins the next
+// frontier.  Hub copies keep one candidate parent per hub per workgroup
+// (hubdup BFS.C:40-47, 51-69).
+struct BfsOp {
+  using SrcT = uint32_t;
+  using HubT = uint32_t;
+  static constexpr bool kOutput = true;
+  uint32_t* parents;
+  __device__ bool skip() const { return false; }
+  __device__ uint32_t load_src(uint32_t s) const { return s; }
+  __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool update(uint32_t s, uint32_t, uint32_t d) const {
+    if (parents[d] != kNone) return false;
+    return atomicCAS(parents + d, kNone, s) == kNone;
+  }
+  __device__ uint32_t hub_identity() const { return kNone; }
+  __device__ void hub_update(uint32_t* slot, uint32_t s, uint32_t, uint32_t) const {
+    if (*slot == kNone) atomicCAS(slot, kNone, s);
+  }
+  __device__ bool hub_flush(uint32_t d, uint32_t cand) const {
+    if (parents[d] != kNone) return false;
+    return atomicCAS(parents + d, kNone, cand) == kNone;
+  }
+};
+
+// Radii_F (Radii.C:35-64): OR the source's visited word into NextVisited[d]
+// when it adds bits; the first such d per round records radii[d] = round.
+struct RadiiOp {
+  using SrcT = unsigned long long;
+  using HubT = unsigned long long;
+  static constexpr bool kOutput = true;
+  const unsigned long long* vis;
+  unsigned long long* nvis;
+  int32_t* radii;
+  int32_t round;
+  __device__ bool skip() const { return false; }
+  __device__ unsigned long long load_src(uint32_t s) const { return vis[s]; }
+  __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool claim(uint32_t d) const {
+    int32_t old = radii[d];
+    if (old == round) return false;
+    return atomicCAS(radii + d, old, round) == old;
+  }
+  __device__ bool update(uint32_t, unsigned long long sv, uint32_t d) const {
+    unsigned long long vd = vis[d];
+    unsigned long long tw = vd | sv;
+    if (vd == tw) return false;
+    atomicOr(nvis + d, tw);
+    return claim(d);
+  }
+  __device__ unsigned long long hub_identity() const { return 0ull; }
+  __device__ void hub_update(unsigned long long* slot, uint32_t, unsigned long long sv,
+                             uint32_t d) const {
+    unsigned long long vd = vis[d];
+    unsigned long long tw = vd | sv;
+    if (vd != tw) atomicOr(slot, tw);
+  }
+  __device__ bool hub_flush(uint32_t d, unsigned long long v) const {
+    atomicOr(nvis + d, v);
+    return claim(d);
+  }
+};
+
+// BC_F (BC-manyiter-v2.C:189-213): NumPaths[d] += NumPaths[s] for unvisited
+// d; d joins when the add saw 0.
+struct BcFwdOp {
+  using SrcT = double;
+  using HubT = double;
+  static constexpr bool kOutput = true;
+  double* np;
+  const uint8_t* visited;
+  __device__ bool skip() const { return false; }
+  __device__ double load_src(uint32_t s) const { return np[s]; }
+  __device__ bool cond(uint32_t d) const { return visited[d] == 0; }
+  __device__ bool update(uint32_t, double sv, uint32_t d) const {
+    return unsafeAtomicAdd(np + d, sv) == 0.0;
+  }
+  __device__ double hub_identity() const { return 0.0; }
+  __device__ void hub_update(double* slot, uint32_t, double sv, uint32_t) const { atomicAdd(slot, sv); }
+  __device__ bool hub_flush(uint32_t d, double v) const { return unsafeAtomicAdd(np + d, v) == 0.0; }
+};
+
+// BC_Back_F (BC-manyiter-v2.C:215-236): Dependencies[d] += Dependencies[s].
+struct BcBackOp {
+  using SrcT = double;
+  using HubT = double;
+  static constexpr bool kOutput = false;
+  double* deps;
+  const uint8_t* visited;
+  __device__ bool skip() const { return false; }
+  __device__ double load_src(uint32_t s) const { return deps[s]; }
+  __device__ bool cond(uint32_t d) const { return visited[d] == 0; }
+  __device__ bool update(uint32_t, double sv, uint32_t d) const {
+    unsafeAtomicAdd(deps + d, sv);
+    return false;
+  }
+  __device__ double hub_identity() const { return 0.0; }
+  __device__ void hub_update(double* slot, uint32_t, double sv, uint32_t) const { atomicAdd(slot, sv); }
+  __device__ bool hub_flush(uint32_t d, double v) const {
+    unsafeAtomicAdd(deps + d, v);
+    return false;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Small vertex kernels.
+
+__global__ void k_scatter_u8(const uint32_t* ids, int64_t count, uint8_t* dst, uint8_t val) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[ids[i]] = val;
+}
+
+__global__ void k_degsum(const uint32_t* ids, int64_t count, const uint64_t* off, int64_t* out) {
+  uint64_t s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = ids[i];
+    s += off[v + 1] - off[v];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (__lane_id() == 0 && s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
+}
+
+template <class T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <class T>
+__global__ void k_set(T* p, int64_t i, T v) {
+  p[i] = v;
+}
+
+static int grid_for(rg_ctx* ctx, int64_t n, int block = 256) {
+  int64_t g = (n + block - 1) / block;
+  int64_t cap = (int64_t)ctx->num_cus * 16;
+  return (int)std::max<int64_t>(1, std::min(g, cap));
+}
+
+template <class T>
+static int fill(rg_ctx* ctx, T* p, int64_t n, T v) {
+  if (n <= 0) return RG_OK;
+  hipLaunchKernelGGL(k_fill<T>, grid_for(ctx, n), 256, 0, ctx->stream, p, n, v);
+  RG_HIP(hipGetLastError());
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Engine configuration and the edgeMap dispatcher.
+
+struct Engine {
+  rg_ctx* ctx;
+  rg_graph* g;
+  int64_t threshold;   // Ligra: m/20 by default (ligra.h:269)
+  int block;           // 256 or 1024
+  int heavy_deg;
+  int hub_mode;        // HUB_NONE / HUB_BITMAP / HUB_PREFIX
+  int64_t hub_k;
+  DevBuf* heavy;       // heavy work queue
+  int64_t* heavy_count;
+};
+
+template <class Op, int MODE, int HUB, int BLOCK>
+static int launch_edge(const Engine& E, const Op& op, const EdgeArgs& a, size_t hub_bytes) {
+  constexpr int WPB = BLOCK / kWave;
+  auto kl = edge_light_kernel<Op, MODE, HUB, BLOCK>;
+  auto kh = edge_heavy_kernel<Op, HUB, BLOCK>;
+  static size_t attr_set = 0;
+  if (hub_bytes > 0 && hub_bytes > attr_set) {
+    RG_HIP(hipFuncSetAttribute((const void*)kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hub_bytes));
+    RG_HIP(hipFuncSetAttribute((const void*)kh, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hub_bytes));
+    attr_set = hub_bytes;
+  }
+  const int64_t nchunks = (a.count + kWave - 1) / kWave;
+  int64_t want = (nchunks + WPB - 1) / WPB;
+  // Persistent grid: enough workgroups to fill every CU; with hub copies, as
+  // many as the LDS allows per CU (each extra copy costs one merge).
+  int64_t per_cu;
+  if (HUB != HUB_NONE) {
+    size_t static_lds = WPB * ((kWave + 1) * 4 + kWave * (8 + 4 + 8));
+    per_cu = std::max<int64_t>(1, (int64_t)(E.ctx->lds_per_cu / (static_lds + hub_bytes)));
+    per_cu = std::min<int64_t>(per_cu, std::max(1, 32 / WPB));
+  } else {
+    per_cu = std::max(1, 64 / WPB);
+  }
+  int64_t cap = (int64_t)E.ctx->num_cus * per_cu;
+  int grid_l = (int)std::max<int64_t>(1, std::min(want, cap));
+  hipLaunchKernelGGL(kl, grid_l, BLOCK, hub_bytes, E.ctx->stream, op, a);
+  RG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(kh, (int)cap, BLOCK, hub_bytes, E.ctx->stream, op, a);
+  RG_HIP(hipGetLastError());
+  return RG_OK;
+}
+
+template <class Op, int MODE, int HUB>
+static int launch_block(const Engine& E, const Op& op, const EdgeArgs& a, size_t hub_bytes) {
+  if (E.block >= 1024) return launch_edge<Op, MODE, HUB, 1024>(E, op, a, hub_bytes);
+  return launch_edge<Op, MODE, HUB, 256>(E, op, a, hub_bytes);
+}
+
+template <class Op, int MODE>
+static int launch_hub(const Engine& E, const Op& op, EdgeArgs a) {
+  size_t hub_bytes = (size_t)a.hub_k * sizeof(typename Op::HubT);
+  switch (E.hub_mode) {
+    case HUB_BITMAP: return launch_block<Op, MODE, HUB_BITMAP>(E, op, a, hub_bytes);
+    case HUB_PREFIX: return launch_block<Op, MODE, HUB_PREFIX>(E, op, a, hub_bytes);
+    default: a.hub_k = 0; return launch_block<Op, MODE, HUB_NONE>(E, op, a, 0);
+  }
+}
+
+// Ligra edgeMapData (ligra.h:264-302) on the device: chooses the dense
+// (edgeMapDenseForward) or sparse (edgeMapSparse) traversal from |F| plus
+// the frontier's out-degree sum, runs the push, and (for operators with
+// output) appends the next frontier to out_ids with its size and degree sum
+// in out_info[0..1] (which the caller zeroes).  A frontier with `all` set is
+// every vertex (PageRank's frontier, PageRank.C:112-115).
+template <class Op>
+static int edge_map(const Engine& E, const Op& op, Frontier& F, int64_t f_degsum,
+                    uint32_t* out_ids, int64_t* out_info) {
+  rg_graph* g = E.g;
+  if (!F.all && F.size == 0) return RG_OK;
+  RG_HIP(hipMemsetAsync(E.heavy_count, 0, sizeof(int64_t), E.ctx->stream));
+  EdgeArgs a{};
+  a.off = g->off;
+  a.nbr = g->nbr;
+  a.out_ids = out_ids;
+  a.out_info = out_info;
+  a.heavy = E.heavy->as<uint64_t>();
+  a.heavy_count = E.heavy_count;
+  a.heavy_deg = E.heavy_deg;
+  a.hub_k = E.hub_k;
+  a.hub_bits = g->hub_bits;
+  a.hub_rank = g->hub_rank;
+  a.hub_ids = g->hub_ids;
+  if (F.all) {
+    a.count = g->n;
+    return launch_hub<Op, SRC_ALL>(E, op, a);
+  }
+  if (f_degsum == 0) return RG_OK;   // ligra.h:287
+  if (F.size + f_degsum > E.threshold) {
+    const int grid = grid_for(E.ctx, F.size);
+    hipLaunchKernelGGL(k_scatter_u8, grid, 256, 0, E.ctx->stream, F.ids, F.size, F.dense, (uint8_t)1);
+    a.count = g->n;
+    a.dense = F.dense;
+    RG_TRY((launch_hub<Op, SRC_DENSE>(E, op, a)));
+    hipLaunchKernelGGL(k_scatter_u8, grid, 256, 0, E.ctx->stream, F.ids, F.size, F.dense, (uint8_t)0);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+  }
+  a.count = F.size;
+  a.ids = F.ids;
+  return launch_hub<Op, SRC_LIST>(E, op, a);
+}
+
+// Sum of out-degrees of a frontier list (device), read back.
+static int frontier_degsum(rg_ctx* ctx, const rg_graph* g, const uint32_t* ids, int64_t count,
+                           int64_t* out) {
+  int64_t* d = ctx->counts.as<int64_t>() + 8;
+  RG_HIP(hipMemsetAsync(d, 0, sizeof(int64_t), ctx->stream));
+  if (count > 0)
+    hipLaunchKernelGGL(k_degsum, grid_for(ctx, count), 256, 0, ctx->stream, ids, count, g->off, d);
+  RG_HIP(hipGetLastError());
+  return ctx_read_count(ctx, d, out, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Hub metadata.
+
+__global__ void k_in_degree(const uint64_t* in_off, int64_t n, uint32_t* deg) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    deg[i] = (uint32_t)(in_off[i + 1] - in_off[i]);
+}
+
+// HUBDUP's identifyHubs (hubdup Ligra/ligra/IO.h:726-804): hubs are the
+// vertices whose in-degree exceeds the k-th largest in-degree (0-based),
+// so at most k of them.  Stored as a bitmap, a per-32-bit-word rank prefix
+// and a slot -> vertex table (slots in vertex-id order).
+static int build_hub_bitmap(rg_graph* g, int64_t k) {
+  rg_ctx* ctx = g->ctx;
+  if (g->hub_k_req == k && g->hub_bits) return RG_OK;
+  const int64_t n = g->n;
+  std::vector<uint32_t> deg(n);
+  DevBuf tmp;
+  RG_TRY(tmp.ensure(sizeof(uint32_t) * n));
+  hipLaunchKernelGGL(k_in_degree, grid_for(ctx, n), 256, 0, ctx->stream, g->in_off, n, tmp.as<uint32_t>());
+  RG_HIP(hipMemcpyAsync(deg.data(), tmp.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(hipStreamSynchronize(ctx->stream));
+  tmp.release();
+  uint32_t thresh;
+  if (k >= n) {
+    thresh = 0;   // every vertex with an in-edge
+  } else {
+    std::vector<uint32_t> s(deg);
+    std::nth_element(s.begin(), s.begin() + k, s.end(), std::greater<uint32_t>());
+    thresh = s[k];
+  }
+  const int64_t nw = (n + 31) / 32;
+  std::vector<uint32_t> bits(nw, 0), rank(nw, 0), ids;
+  for (int64_t v = 0; v < n; ++v)
+    if (deg[v] > thresh) {
+      bits[v >> 5] |= 1u << (v & 31);
+      ids.push_back((uint32_t)v);
+    }
+  uint32_t acc = 0;
+  for (int64_t w = 0; w < nw; ++w) {
+    rank[w] = acc;
+    acc += (uint32_t)__builtin_popcount(bits[w]);
+  }
+  if (g->hub_bits) (void)hipFree(g->hub_bits);
+  if (g->hub_rank) (void)hipFree(g->hub_rank);
+  if (g->hub_ids) (void)hipFree(g->hub_ids);
+  g->hub_bits = g->hub_rank = g->hub_ids = nullptr;
+  RG_HIP(hipMalloc(&g->hub_bits, sizeof(uint32_t) * nw));
+  RG_HIP(hipMalloc(&g->hub_rank, sizeof(uint32_t) * nw));
+  RG_HIP(hipMalloc(&g->hub_ids, sizeof(uint32_t) * std::max<size_t>(1, ids.size())));
+  RG_HIP(hipMemcpy(g->hub_bits, bits.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
+  RG_HIP(hipMemcpy(g->hub_rank, rank.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
+  if (!ids.empty())
+    RG_HIP(hipMemcpy(g->hub_ids, ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+  g->hub_k = (int64_t)ids.size();
+  g->hub_k_req = k;
+  return RG_OK;
+}
+
+// Resolves variant/opts into an Engine.  elem = bytes of one hub copy entry.
+static int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E) {
+  rg_ctx* ctx = g->ctx;
+  E.ctx = ctx;
+  E.g = g;
+  E.threshold = (o && o->threshold >= 0) ? o->threshold : (int64_t)(int32_t)(g->m / 20);
+  E.block = (o && o->wg_size >= 1024) ? 1024 : 256;
+  E.heavy_deg = (o && o->heavy_degree > 0) ? o->heavy_degree : 1024;
+  E.hub_mode = HUB_NONE;
+  E.hub_k = 0;
+  const int variant = o ? o->variant : RG_PUSH;
+  if (variant == RG_HUBDUP || variant == RG_RADAR) {
+    const int wpb = E.block / kWave;
+    size_t static_lds = wpb * ((kWave + 1) * 4 + kWave * (8 + 4 + 8));
+    size_t budget = ctx->max_lds_per_block > static_lds ? ctx->max_lds_per_block - static_lds : 0;
+    int64_t kmax = (int64_t)(budget / elem);
+    int64_t k = (o && o->hubs > 0) ? o->hubs : std::min<int64_t>(kmax, (int64_t)(32768 / elem) * 2);
+    k = std::min<int64_t>(k, kmax);
+    k = std::min<int64_t>(k, g->n);
+    if (k > 0) {
+      if (variant == RG_RADAR) {
+        E.hub_mode = HUB_PREFIX;
+        E.hub_k = k;
+      } else {
+        RG_TRY(build_hub_bitmap(g, k));
+        E.hub_mode = HUB_BITMAP;
+        E.hub_k = g->hub_k;
+        if (E.hub_k == 0) E.hub_mode = HUB_NONE;
+      }
+    }
+  }
+  // Heavy queue: at most m/heavy_deg heavy vertices, each split in chunks.
+  int64_t cap = g->m / kHeavyChunk + g->m / std::max(1, E.heavy_deg) + 64;
+  E.heavy = &ctx->scratch[7];
+  RG_TRY(E.heavy->ensure(sizeof(uint64_t) * cap));
+  E.heavy_count = ctx->counts.as<int64_t>() + 0;
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Timing helper (device time of a region on the context stream).
+struct EventTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s;
+  explicit EventTimer(hipStream_t st) : s(st) {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, s);
+  }
+  double stop_ms() {
+    float ms = 0.f;
+    (void)hipEventRecord(b, s);
+    (void)hipEventSynchronize(b);
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+  }
+  ~EventTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// PageRank (PageRank.C:92-141).
+
+__global__ void k_pr_init(const uint64_t* off, int64_t n, double* rank, double* contrib, double* acc,
+                          double one_over_n) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t deg = off[v + 1] - off[v];
+    rank[v] = one_over_n;
+    contrib[v] = deg ? one_over_n / (double)(uint32_t)deg : 0.0;
+    acc[v] = 0.0;
+  }
+}
+
+// Fused vertex phase: PR_Vertex_F (p_next = 0.85 p_next + 0.15/n), the L1
+// term |p_curr - p_next|, PR_Vertex_Reset and the swap, plus the next
+// iteration's per-source quotient.  The multiply and add are rounded
+// separately, as the reference's host code does.
+__global__ __launch_bounds__(256) void k_pr_vertex(const uint64_t* off, int64_t n, double* rank,
+                                                   double* contrib, double* acc, double damping,
+                                                   double added, double* partial,
+                                                   const int64_t* done) {
+  if (*done) return;
+  double diff = 0.0;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    double nw = __dadd_rn(__dmul_rn(damping, acc[v]), added);
+    diff += fabs(rank[v] - nw);
+    rank[v] = nw;
+    uint64_t deg = off[v + 1] - off[v];
+    contrib[v] = deg ? nw / (double)(uint32_t)deg : 0.0;
+    acc[v] = 0.0;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = diff;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Sums the block partials in a fixed order; records L1 for this iteration
+// and, the first time it drops below epsilon, the (1-based) iteration.
+__global__ void k_pr_finish(const double* partial, int nparts, double* l1_hist, int64_t iter,
+                            int64_t* done, double epsilon) {
+  if (*done) return;
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    l1_hist[iter - 1] = red[0];
+    if (red[0] < epsilon) *done = iter;
+  }
+}
+
+int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
+                  int64_t* iters_out, double* l1_out, double* ms_out);
+int pagerank_pull(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
+                  int64_t* iters_out, double* l1_out, double* ms_out);
+
+int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
+                  int64_t* iters_out, double* l1_out, double* ms_out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  Engine E;
+  RG_TRY(make_engine(g, o, sizeof(double), E));
+  DevBuf &b_rank = ctx->scratch[0], &b_contrib = ctx->scratch[1], &b_acc = ctx->scratch[2],
+         &b_part = ctx->scratch[3];
+  RG_TRY(b_rank.ensure(sizeof(double) * n));
+  RG_TRY(b_contrib.ensure(sizeof(double) * n));
+  RG_TRY(b_acc.ensure(sizeof(double) * n));
+  const int vgrid = std::min<int>(grid_for(ctx, n), 1024);
+  RG_TRY(b_part.ensure(sizeof(double) * (vgrid + std::max<int64_t>(maxiters, 1) + 8)));
+  double* rank = b_rank.as<double>();
+  double* contrib = b_contrib.as<double>();
+  double* acc = b_acc.as<double>();
+  double* partial = b_part.as<double>();
+  double* l1_hist = partial + vgrid;
+  int64_t* done = ctx->counts.as<int64_t>() + 16;
+  RG_HIP(hipMemsetAsync(done, 0, sizeof(int64_t), ctx->stream));
+  const double damping = 0.85, epsilon = 0.0000001;
+  const double one_over_n = 1 / (double)n;
+  const double added = (1 - damping) * (1 / (double)n);
+  hipLaunchKernelGGL(k_pr_init, grid_for(ctx, n), 256, 0, ctx->stream, g->off, n, rank, contrib, acc,
+                     one_over_n);
+  RG_HIP(hipGetLastError());
+  Frontier all;
+  all.all = true;
+  PrOp op{contrib, acc, done};
+  EventTimer tm(ctx->stream);
+  for (int64_t it = 1; it <= maxiters; ++it) {
+    RG_TRY(edge_map(E, op, all, g->m, nullptr, nullptr));
+    hipLaunchKernelGGL(k_pr_vertex, vgrid, 256, 0, ctx->stream, g->off, n, rank, contrib, acc, damping,
+                       added, partial, done);
+    hipLaunchKernelGGL(k_pr_finish, 1, 256, 0, ctx->stream, partial, vgrid, l1_hist, it, done, epsilon);
+    RG_HIP(hipGetLastError());
+  }
+  double ms = tm.stop_ms();
+  int64_t dn = 0;
+  RG_TRY(ctx_read_count(ctx, done, &dn, 1));
+  int64_t last = dn ? dn : maxiters;
+  double l1 = 0.0;
+  if (last >= 1)
+    RG_HIP(hipMemcpy(&l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
+  if (iters_out) *iters_out = dn ? dn : maxiters + 1;
+  if (l1_out) *l1_out = l1;
+  if (ms_out) *ms_out = ms;
+  if (ranks_out) RG_HIP(hipMemcpy(ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// BFS (BFS.C:213-258).
+
+__global__ void k_bfs_edges(const uint32_t* parents, const uint64_t* off, int64_t n, int64_t* out) {
+  uint64_t s = 0, c = 0;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    if (parents[v] != kNone) {
+      s += off[v + 1] - off[v];
+      c++;
+    }
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o);
+    c += __shfl_xor(c, o);
+  }
+  if (__lane_id() == 0) {
+    atomicAdd((unsigned long long*)&out[0], (unsigned long long)c);
+    atomicAdd((unsigned long long*)&out[1], (unsigned long long)s);
+  }
+}
+
+int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
+             int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev);
+
+int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
+             int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  if ((int64_t)src >= n) return set_error(RG_EINVAL, "rg_bfs: source out of range");
+  Engine E;
+  RG_TRY(make_engine(g, o, sizeof(uint32_t), E));
+  DevBuf& b_par = ctx->scratch[0];
+  uint32_t* parents = parents_dev;
+  if (!parents) {
+    RG_TRY(b_par.ensure(sizeof(uint32_t) * n));
+    parents = b_par.as<uint32_t>();
+  }
+  Frontier fa, fb;
+  RG_TRY(ctx_frontiers(ctx, n, fa, fb));
+  int64_t* info = ctx->counts.as<int64_t>() + 4;   // [count, degsum]
+  RG_TRY(fill<uint32_t>(ctx, parents, n, kNone));
+  hipLaunchKernelGGL(k_set<uint32_t>, 1, 1, 0, ctx->stream, parents, (int64_t)src, src);
+  hipLaunchKernelGGL(k_set<uint32_t>, 1, 1, 0, ctx->stream, fa.ids, (int64_t)0, src);
+  RG_HIP(hipGetLastError());
+  fa.size = 1;
+  int64_t degsum = 0;
+  RG_TRY(frontier_degsum(ctx, g, fa.ids, 1, &degsum));
+  BfsOp op{parents};
+  Frontier* F = &fa;
+  Frontier* N = &fb;
+  int64_t iter = 0;
+  EventTimer tm(ctx->stream);
+  while (F->size > 0) {
+    RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
+    RG_TRY(edge_map(E, op, *F, degsum, N->ids, info));
+    int64_t h[2];
+    RG_TRY(ctx_read_count(ctx, info, h, 2));
+    N->size = h[0];
+    degsum = h[1];
+    std::swap(F, N);
+    ++iter;
+  }
+  double ms = tm.stop_ms();
+  int64_t* cnt = ctx->counts.as<int64_t>() + 10;
+  RG_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(int64_t), ctx->stream));
+  hipLaunchKernelGGL(k_bfs_edges, grid_for(ctx, n), 256, 0, ctx->stream, parents, g->off, n, cnt);
+  int64_t h[2];
+  RG_TRY(ctx_read_count(ctx, cnt, h, 2));
+  if (iters_out) *iters_out = iter;
+  if (reached_out) *reached_out = h[0];
+  if (edges_out) *edges_out = h[1];
+  if (ms_out) *ms_out = ms;
+  if (parents_out) RG_HIP(hipMemcpy(parents_out, parents, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Radii (Radii.C:98-157).
+
+__global__ void k_radii_sync(const uint32_t* ids, int64_t count, unsigned long long* vis,
+                             const unsigned long long* nvis) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = ids[i];
+    vis[v] = nvis[v];
+  }
+}
+
+__global__ void k_radii_start(const uint32_t* starts, int k, int32_t* radii, unsigned long long* nvis) {
+  // Radii.C:120-127 in start order (a repeated start keeps its last bit).
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int i = 0; i < k; ++i) {
+      radii[starts[i]] = 0;
+      nvis[starts[i]] = 1ull << i;
+    }
+}
+
+int radii_push(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o, int32_t* radii_out,
+               int64_t* rounds_out, double* ms_out);
+
+int radii_push(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o, int32_t* radii_out,
+               int64_t* rounds_out, double* ms_out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  if (nstarts < 0 || nstarts > 64) return set_error(RG_EINVAL, "rg_radii: 0..64 starts");
+  for (int i = 0; i < nstarts; ++i)
+    if ((int64_t)starts[i] >= n) return set_error(RG_EINVAL, "rg_radii: start out of range");
+  Engine E;
+  RG_TRY(make_engine(g, o, sizeof(uint64_t), E));
+  DevBuf &b_rad = ctx->scratch[0], &b_vis = ctx->scratch[1], &b_nvis = ctx->scratch[2];
+  RG_TRY(b_rad.ensure(sizeof(int32_t) * n));
+  RG_TRY(b_vis.ensure(sizeof(uint64_t) * n));
+  RG_TRY(b_nvis.ensure(sizeof(uint64_t) * n));
+  int32_t* radii = b_rad.as<int32_t>();
+  auto* vis = b_vis.as<unsigned long long>();
+  auto* nvis = b_nvis.as<unsigned long long>();
+  RG_TRY(fill<int32_t>(ctx, radii, n, -1));
+  RG_HIP(hipMemsetAsync(vis, 0, sizeof(uint64_t) * n, ctx->stream));
+  RG_HIP(hipMemsetAsync(nvis, 0, sizeof(uint64_t) * n, ctx->stream));
+  Frontier fa, fb;
+  RG_TRY(ctx_frontiers(ctx, n, fa, fb));
+  if (nstarts > 0) {
+    RG_HIP(hipMemcpy(fa.ids, starts, sizeof(uint32_t) * nstarts, hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(k_radii_start, 1, 64, 0, ctx->stream, fa.ids, nstarts, radii, nvis);
+  fa.size = nstarts;
+  int64_t degsum = 0;
+  RG_TRY(frontier_degsum(ctx, g, fa.ids, nstarts, &degsum));
+  int64_t* info = ctx->counts.as<int64_t>() + 4;
+  Frontier* F = &fa;
+  Frontier* N = &fb;
+  int32_t round = 0;
+  EventTimer tm(ctx->stream);
+  while (F->size > 0) {
+    round++;
+    hipLaunchKernelGGL(k_radii_sync, grid_for(ctx, F->size), 256, 0, ctx->stream, F->ids, F->size, vis,
+                       nvis);
+    RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
+    RadiiOp op{vis, nvis, radii, round};
+    RG_TRY(edge_map(E, op, *F, degsum, N->ids, info));
+    int64_t h[2];
+    RG_TRY(ctx_read_count(ctx, info, h, 2));
+    N->size = h[0];
+    degsum = h[1];
+    std::swap(F, N);
+  }
+  double ms = tm.stop_ms();
+  if (rounds_out) *rounds_out = round;
+  if (ms_out) *ms_out = ms;
+  if (radii_out) RG_HIP(hipMemcpy(radii_out, radii, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// BC (BC-manyiter-v2.C:280-398).
+
+__global__ void k_bc_start(double* np, uint8_t* visited, uint32_t start) {
+  np[start] = 1.0;
+  visited[start] = 1;
+}
+
+__global__ void k_mark(const uint32_t* ids, int64_t count, uint8_t* visited) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    visited[ids[i]] = 1;
+}
+
+// BC_Back_Vertex_F (BC-manyiter-v2.C:250-258).
+__global__ void k_bc_back_vertex(const uint32_t* ids, int64_t count, uint8_t* visited, double* deps,
+                                 const double* inv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = ids[i];
+    visited[v] = 1;
+    deps[v] += inv[v];
+  }
+}
+
+__global__ void k_invert(double* np, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    np[i] = 1 / np[i];
+}
+
+__global__ void k_bc_final(double* deps, const double* inv, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    deps[i] = (deps[i] - inv[i]) / inv[i];
+}
+
+int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
+            int64_t* combined_out, double* ms_out);
+
+int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
+            int64_t* combined_out, double* ms_out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  for (int i = 0; i < nsources; ++i)
+    if ((int64_t)sources[i] >= n) return set_error(RG_EINVAL, "rg_bc: source out of range");
+  DevBuf &b_np = ctx->scratch[0], &b_dep = ctx->scratch[1], &b_vis = ctx->scratch[2],
+         &b_lv = ctx->scratch[3];
+  RG_TRY(b_np.ensure(sizeof(double) * n));
+  RG_TRY(b_dep.ensure(sizeof(double) * n));
+  RG_TRY(b_vis.ensure((size_t)n));
+  RG_TRY(b_lv.ensure(sizeof(uint32_t) * (n + 1)));
+  double* np = b_np.as<double>();
+  double* deps = b_dep.as<double>();
+  uint8_t* visited = b_vis.as<uint8_t>();
+  uint32_t* lv = b_lv.as<uint32_t>();
+  Frontier F, unused;   // F views the level lists stored in lv
+  RG_TRY(ctx_frontiers(ctx, n, F, unused));
+  RG_HIP(hipMemsetAsync(deps, 0, sizeof(double) * n, ctx->stream));
+  int64_t* info = ctx->counts.as<int64_t>() + 4;
+  int64_t combined = 0;
+  const bool was_transposed = g->transposed;
+  EventTimer tm(ctx->stream);
+  for (int it = 0; it < nsources; ++it) {
+    Engine E;
+    RG_TRY(make_engine(g, o, sizeof(double), E));
+    uint32_t start = sources[it];
+    RG_HIP(hipMemsetAsync(np, 0, sizeof(double) * n, ctx->stream));
+    RG_HIP(hipMemsetAsync(visited, 0, (size_t)n, ctx->stream));
+    hipLaunchKernelGGL(k_bc_start, 1, 1, 0, ctx->stream, np, visited, start);
+    hipLaunchKernelGGL(k_set<uint32_t>, 1, 1, 0, ctx->stream, lv, (int64_t)0, start);
+    std::vector<int64_t> lstart = {0, 1};
+    int64_t degsum = 0;
+    RG_TRY(frontier_degsum(ctx, g, lv, 1, &degsum));
+    int64_t round = 0;
+    BcFwdOp fop{np, visited};
+    while (true) {
+      int64_t fb = lstart[lstart.size() - 2], fe = lstart.back();
+      if (fe == fb) break;
+      round++;
+      F.ids = lv + fb;
+      F.size = fe - fb;
+      RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
+      RG_TRY(edge_map(E, fop, F, degsum, lv + fe, info));
+      int64_t h[2];
+      RG_TRY(ctx_read_count(ctx, info, h, 2));
+      if (h[0] > 0)
+        hipLaunchKernelGGL(k_mark, grid_for(ctx, h[0]), 256, 0, ctx->stream, lv + fe, h[0], visited);
+      degsum = h[1];
+      lstart.push_back(fe + h[0]);
+    }
+    combined += round;
+    hipLaunchKernelGGL(k_invert, grid_for(ctx, n), 256, 0, ctx->stream, np, n);
+    RG_HIP(hipMemsetAsync(visited, 0, (size_t)n, ctx->stream));
+    // Frontier = Levels[round-1]; level r is lv[lstart[r], lstart[r+1]).
+    auto level = [&](int64_t r) { F.ids = lv + lstart[r]; F.size = lstart[r + 1] - lstart[r]; };
+    level(round - 1);
+    if (F.size > 0)
+      hipLaunchKernelGGL(k_bc_back_vertex, grid_for(ctx, F.size), 256, 0, ctx->stream, F.ids, F.size,
+                         visited, deps, np);
+    RG_TRY(rg_graph_transpose(g));
+    Engine EB;
+    RG_TRY(make_engine(g, o, sizeof(double), EB));
+    BcBackOp bop{deps, visited};
+    for (int64_t r = round - 2; r >= 0; --r) {
+      int64_t ds = 0;
+      RG_TRY(frontier_degsum(ctx, g, F.ids, F.size, &ds));
+      RG_TRY(edge_map(EB, bop, F, ds, nullptr, nullptr));
+      level(r);
+      hipLaunchKernelGGL(k_bc_back_vertex, grid_for(ctx, F.size), 256, 0, ctx->stream, F.ids, F.size,
+                         visited, deps, np);
+    }
+    hipLaunchKernelGGL(k_bc_final, grid_for(ctx, n), 256, 0, ctx->stream, deps, np, n);
+    RG_HIP(hipGetLastError());
+  }
+  double ms = tm.stop_ms();
+  (void)was_transposed;
+  if (combined_out) *combined_out = combined;
+  if (ms_out) *ms_out = ms;
+  if (deps_out) RG_HIP(hipMemcpy(deps_out, deps, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Pull PageRank of the push-pull tree: threshold 0 makes every iteration an
+// edgeMapDense (push-pull Ligra/apps/PageRank.C:122, ligra.h:61-88), which
+// for each vertex v sums p_curr[s]/outdeg(s) over its in-neighbours with the
+// non-atomic update.  Light vertices (in-degree <= 64) sum sequentially in
+// in-list order -- the reference's order -- one thread each; heavier ones are
+// reduced by a wavefront each.
+
+constexpr uint32_t kPullLight = 64;
+
+__global__ void k_pr_pull_light(const uint64_t* in_off, const uint32_t* in_nbr, int64_t n,
+                                const double* contrib, double* acc, uint32_t* heavy,
+                                int64_t* heavy_count, const int64_t* done) {
+  if (*done) return;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t b = in_off[v], e = in_off[v + 1];
+    if (e - b > kPullLight) {
+      int64_t slot = atomicAdd((unsigned long long*)heavy_count, 1ull);
+      heavy[slot] = (uint32_t)v;
+      continue;
+    }
+    double s = acc[v];
+    for (uint64_t j = b; j < e; ++j) s += contrib[in_nbr[j]];
+    acc[v] = s;
+  }
+}
+
+__global__ void k_pr_pull_heavy(const uint64_t* in_off, const uint32_t* in_nbr, const double* contrib,
+                                double* acc, const uint32_t* heavy, const int64_t* heavy_count,
+                                const int64_t* done) {
+  if (*done) return;
+  const int lane = (int)__lane_id();
+  const int64_t items = *heavy_count;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  for (int64_t it = wid; it < items; it += nw) {
+    uint32_t v = heavy[it];
+    double s = 0.0;
+    for (uint64_t j = in_off[v] + lane; j < in_off[v + 1]; j += kWave) s += contrib[in_nbr[j]];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) acc[v] += s;
+  }
+}
+
+int pagerank_pull(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
+                  int64_t* iters_out, double* l1_out, double* ms_out) {
+  (void)o;
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  DevBuf &b_rank = ctx->scratch[0], &b_contrib = ctx->scratch[1], &b_acc = ctx->scratch[2],
+         &b_part = ctx->scratch[3], &b_heavy = ctx->scratch[4];
+  RG_TRY(b_rank.ensure(sizeof(double) * n));
+  RG_TRY(b_contrib.ensure(sizeof(double) * n));
+  RG_TRY(b_acc.ensure(sizeof(double) * n));
+  RG_TRY(b_heavy.ensure(sizeof(uint32_t) * n));
+  const int vgrid = std::min<int>(grid_for(ctx, n), 1024);
+  RG_TRY(b_part.ensure(sizeof(double) * (vgrid + std::max<int64_t>(maxiters, 1) + 8)));
+  double* rank = b_rank.as<double>();
+  double* contrib = b_contrib.as<double>();
+  double* acc = b_acc.as<double>();
+  double* partial = b_part.as<double>();
+  double* l1_hist = partial + vgrid;
+  int64_t* done = ctx->counts.as<int64_t>() + 16;
+  int64_t* hcount = ctx->counts.as<int64_t>() + 17;
+  RG_HIP(hipMemsetAsync(done, 0, sizeof(int64_t), ctx->stream));
+  const double damping = 0.85, epsilon = 0.0000001;
+  const double one_over_n = 1 / (double)n;
+  const double added = (1 - damping) * (1 / (double)n);
+  hipLaunchKernelGGL(k_pr_init, grid_for(ctx, n), 256, 0, ctx->stream, g->off, n, rank, contrib, acc,
+                     one_over_n);
+  RG_HIP(hipGetLastError());
+  EventTimer tm(ctx->stream);
+  for (int64_t it = 1; it <= maxiters; ++it) {
+    RG_HIP(hipMemsetAsync(hcount, 0, sizeof(int64_t), ctx->stream));
+    hipLaunchKernelGGL(k_pr_pull_light, grid_for(ctx, n), 256, 0, ctx->stream, g->in_off, g->in_nbr, n,
+                       contrib, acc, b_heavy.as<uint32_t>(), hcount, done);
+    hipLaunchKernelGGL(k_pr_pull_heavy, ctx->num_cus * 8, 256, 0, ctx->stream, g->in_off, g->in_nbr,
+                       contrib, acc, b_heavy.as<uint32_t>(), hcount, done);
+    hipLaunchKernelGGL(k_pr_vertex, vgrid, 256, 0, ctx->stream, g->off, n, rank, contrib, acc, damping,
+                       added, partial, done);
+    hipLaunchKernelGGL(k_pr_finish, 1, 256, 0, ctx->stream, partial, vgrid, l1_hist, it, done, epsilon);
+    RG_HIP(hipGetLastError());
+  }
+  double ms = tm.stop_ms();
+  int64_t dn = 0;
+  RG_TRY(ctx_read_count(ctx, done, &dn, 1));
+  int64_t last = dn ? dn : maxiters;
+  double l1 = 0.0;
+  if (last >= 1) RG_HIP(hipMemcpy(&l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
+  if (iters_out) *iters_out = dn ? dn : maxiters + 1;
+  if (l1_out) *l1_out = l1;
+  if (ms_out) *ms_out = ms;
+  if (ranks_out) RG_HIP(hipMemcpy(ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+}  // namespace rg
+
+using rg::set_error;
+
+extern "C" {
+
+void rg_opts_default(rg_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->variant = RG_PUSH;
+  o->hub_placement = RG_HUBS_LDS;
+  o->hubs = 0;
+  o->threshold = -1;
+  o->wg_size = 0;
+  o->heavy_degree = 0;
+}
+
+int rg_pagerank(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out, int64_t* iters_out,
+                double* l1_out, double* ms_out) {
+  if (!g || maxiters < 0) return set_error(RG_EINVAL, "rg_pagerank: bad arguments");
+  RG_HIP(hipSetDevice(g->ctx->device));
+  if (o && o->variant == RG_PULL)
+    return rg::pagerank_pull(g, maxiters, o, ranks_out, iters_out, l1_out, ms_out);
+  return rg::pagerank_push(g, maxiters, o, ranks_out, iters_out, l1_out, ms_out);
+}
+
+int rg_bfs(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
+           int64_t* reached_out, int64_t* edges_out, double* ms_out) {
+  if (!g) return set_error(RG_EINVAL, "rg_bfs: null graph");
+  RG_HIP(hipSetDevice(g->ctx->device));
+  return rg::bfs_push(g, src, o, parents_out, iters_out, reached_out, edges_out, ms_out, nullptr);
+}
+
+int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o, int32_t* radii_out,
+             int64_t* rounds_out, double* ms_out) {
+  if (!g || (nstarts > 0 && !starts)) return set_error(RG_EINVAL, "rg_radii: bad arguments");
+  RG_HIP(hipSetDevice(g->ctx->device));
+  return rg::radii_push(g, starts, nstarts, o, radii_out, rounds_out, ms_out);
+}
+
+int rg_bc(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
+          int64_t* combined_out, double* ms_out) {
+  if (!g || nsources < 0 || (nsources > 0 && !sources)) return set_error(RG_EINVAL, "rg_bc: bad arguments");
+  RG_HIP(hipSetDevice(g->ctx->device));
+  return rg::bc_push(g, sources, nsources, o, deps_out, combined_out, ms_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// Internal device-side structures of the radar_graph library (HIP, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rg_common.h"
+
+namespace rg {
+
+constexpr uint32_t kNone = 0xffffffffu;   // UINT_E_MAX of Ligra (parallel.h:121)
+constexpr int kWave = 64;                 // CDNA wavefront width
+
+#define RG_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t _e = (call);                                                       \
+    if (_e != hipSuccess)                                                         \
+      return ::rg::set_error(RG_EDEVICE, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define RG_TRY(call)            \
+  do {                          \
+    int _rc = (call);           \
+    if (_rc != RG_OK) return _rc; \
+  } while (0)
+
+// A device buffer that grows on demand and is reused across calls, so the
+// per-round loops allocate nothing after warm-up.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want);
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace rg
+
+struct rg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+  int num_xcds = 8;
+  size_t lds_per_cu = 160 * 1024;
+  size_t max_lds_per_block = 64 * 1024;
+  // Pinned host word for small device->host readbacks (frontier sizes).
+  int64_t* h_counts = nullptr;
+  // Scratch pools reused by the engine (grown on demand).
+  rg::DevBuf counts;      // int64 counters (queue tails, sums)
+  rg::DevBuf scratch[8];  // generic pools for apps / kernels
+  rg::DevBuf fr_ids[2];   // frontier id lists (uint32[n] each)
+  rg::DevBuf fr_dense;    // frontier byte map (uint8[n]); all zero between uses
+};
+
+struct rg_graph {
+  rg_ctx* ctx = nullptr;
+  int64_t n = 0, m = 0;
+  bool symmetric = false;
+  // Current orientation: out-CSR and in-CSR (the two swap on transpose).
+  uint64_t* off = nullptr;
+  uint32_t* nbr = nullptr;
+  uint64_t* in_off = nullptr;
+  uint32_t* in_nbr = nullptr;
+  bool transposed = false;
+  // Hub metadata (HUBDUP): bitmap of hubs + per-word rank prefix, for the
+  // in-degree ordering of the current orientation.
+  uint32_t* hub_bits = nullptr;
+  uint32_t* hub_rank = nullptr;
+  uint32_t* hub_ids = nullptr;   // slot -> vertex id
+  int64_t hub_k = 0;             // hubs selected
+  int64_t hub_k_req = -1;        // k the metadata was built for
+};
+
+namespace rg {
+
+// Device frontier (Ligra vertexSubset, vertexSubset.h:113-229): a sparse id
+// list (always kept) and, for dense rounds, a byte map over all n.  A view:
+// the storage belongs to the context pools (ctx_frontiers).
+struct Frontier {
+  uint32_t* ids = nullptr;  // sparse list
+  uint8_t* dense = nullptr; // byte map, all zero outside a dense round
+  int64_t size = 0;         // host copy of the number of vertices
+  bool all = false;         // the subset is every vertex (ids not materialised)
+};
+
+// Points a and b at the context's two frontier lists and its byte map,
+// growing them to n entries (the byte map is zeroed when it grows).
+int ctx_frontiers(rg_ctx* ctx, int64_t n, Frontier& a, Frontier& b);
+
+int ctx_read_count(rg_ctx* ctx, const int64_t* dev, int64_t* host, int nwords = 1);
+
+}  // namespace rg

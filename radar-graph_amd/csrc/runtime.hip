@@ -1,0 +1,205 @@
+// Context, device graph and buffer management of the radar_graph library.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "device.h"
+
+namespace rg {
+
+int DevBuf::ensure(size_t want) {
+  if (want <= bytes && p) return RG_OK;
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  size_t sz = std::max<size_t>(want, 256);
+  hipError_t e = hipMalloc(&p, sz);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return set_error(RG_ENOMEM, "device allocation of " + std::to_string(sz) + " bytes failed");
+  }
+  bytes = sz;
+  return RG_OK;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+int ctx_frontiers(rg_ctx* ctx, int64_t n, Frontier& a, Frontier& b) {
+  size_t cnt = (size_t)std::max<int64_t>(n, 64);
+  RG_TRY(ctx->fr_ids[0].ensure(sizeof(uint32_t) * cnt));
+  RG_TRY(ctx->fr_ids[1].ensure(sizeof(uint32_t) * cnt));
+  if (ctx->fr_dense.bytes < cnt || !ctx->fr_dense.p) {
+    RG_TRY(ctx->fr_dense.ensure(cnt));
+    RG_HIP(hipMemsetAsync(ctx->fr_dense.p, 0, ctx->fr_dense.bytes, ctx->stream));
+  }
+  a = Frontier{};
+  b = Frontier{};
+  a.ids = ctx->fr_ids[0].as<uint32_t>();
+  b.ids = ctx->fr_ids[1].as<uint32_t>();
+  a.dense = b.dense = ctx->fr_dense.as<uint8_t>();
+  return RG_OK;
+}
+
+int ctx_read_count(rg_ctx* ctx, const int64_t* dev, int64_t* host, int nwords) {
+  RG_HIP(hipMemcpyAsync(ctx->h_counts, dev, sizeof(int64_t) * nwords, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  RG_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(host, ctx->h_counts, sizeof(int64_t) * nwords);
+  return RG_OK;
+}
+
+}  // namespace rg
+
+using rg::set_error;
+
+extern "C" {
+
+const char* rg_version(void) { return "radar_graph 0.1 (gfx950)"; }
+
+int rg_device_count(int* count) {
+  if (!count) return set_error(RG_EINVAL, "rg_device_count: null");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    c = 0;
+  }
+  *count = c;
+  return RG_OK;
+}
+
+int rg_ctx_create(int device, rg_ctx** out) {
+  if (!out) return set_error(RG_EINVAL, "rg_ctx_create: null out");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess || c == 0) {
+    (void)hipGetLastError();
+    return set_error(RG_EDEVICE, "rg_ctx_create: no HIP device available");
+  }
+  if (device < 0 || device >= c) return set_error(RG_EINVAL, "rg_ctx_create: bad device index");
+  RG_HIP(hipSetDevice(device));
+  std::unique_ptr<rg_ctx> ctx(new rg_ctx);
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  RG_HIP(hipGetDeviceProperties(&prop, device));
+  ctx->num_cus = prop.multiProcessorCount;
+  ctx->max_lds_per_block = prop.sharedMemPerBlock;
+  if (prop.maxSharedMemoryPerMultiProcessor > 0)
+    ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+  RG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  RG_HIP(hipHostMalloc((void**)&ctx->h_counts, 64 * sizeof(int64_t), hipHostMallocDefault));
+  RG_TRY(ctx->counts.ensure(64 * sizeof(int64_t)));
+  *out = ctx.release();
+  return RG_OK;
+}
+
+int rg_ctx_destroy(rg_ctx* ctx) {
+  if (!ctx) return RG_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->counts.release();
+  for (auto& b : ctx->scratch) b.release();
+  ctx->fr_ids[0].release();
+  ctx->fr_ids[1].release();
+  ctx->fr_dense.release();
+  if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RG_OK;
+}
+
+int rg_ctx_sync(rg_ctx* ctx) {
+  if (!ctx) return set_error(RG_EINVAL, "rg_ctx_sync: null");
+  RG_HIP(hipStreamSynchronize(ctx->stream));
+  return RG_OK;
+}
+
+static int alloc_copy(void** dst, const void* src, size_t bytes, hipStream_t s) {
+  hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 8));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *dst = nullptr;
+    return set_error(RG_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  if (bytes) RG_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s));
+  return RG_OK;
+}
+
+int rg_graph_upload(rg_ctx* ctx, const rg_hgraph* hg, rg_graph** out) {
+  if (!ctx || !hg || !out) return set_error(RG_EINVAL, "rg_graph_upload: null argument");
+  RG_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<rg_graph> g(new rg_graph);
+  g->ctx = ctx;
+  g->n = hg->n;
+  g->m = hg->m;
+  g->symmetric = hg->symmetric;
+  int rc = alloc_copy((void**)&g->off, hg->off.data(), sizeof(uint64_t) * (hg->n + 1), ctx->stream);
+  if (!rc) rc = alloc_copy((void**)&g->nbr, hg->nbr.data(), sizeof(uint32_t) * hg->m, ctx->stream);
+  if (!rc && !hg->symmetric) {
+    rc = alloc_copy((void**)&g->in_off, hg->in_off.data(), sizeof(uint64_t) * (hg->n + 1), ctx->stream);
+    if (!rc) rc = alloc_copy((void**)&g->in_nbr, hg->in_nbr.data(), sizeof(uint32_t) * hg->m, ctx->stream);
+  } else if (!rc) {
+    g->in_off = g->off;
+    g->in_nbr = g->nbr;
+  }
+  if (rc) {
+    rg_graph_free(g.release());
+    return rc;
+  }
+  RG_HIP(hipStreamSynchronize(ctx->stream));
+  *out = g.release();
+  return RG_OK;
+}
+
+int rg_graph_info(const rg_graph* g, int64_t* n, int64_t* m, int* symmetric) {
+  if (!g) return set_error(RG_EINVAL, "rg_graph_info: null graph");
+  if (n) *n = g->n;
+  if (m) *m = g->m;
+  if (symmetric) *symmetric = g->symmetric ? 1 : 0;
+  return RG_OK;
+}
+
+int rg_graph_download(const rg_graph* g, int in_edges, uint64_t* offsets, uint32_t* nbrs) {
+  if (!g || !offsets || (g->m > 0 && !nbrs)) return set_error(RG_EINVAL, "rg_graph_download: null argument");
+  RG_HIP(hipSetDevice(g->ctx->device));
+  const uint64_t* o = in_edges ? g->in_off : g->off;
+  const uint32_t* b = in_edges ? g->in_nbr : g->nbr;
+  RG_HIP(hipMemcpyAsync(offsets, o, sizeof(uint64_t) * (g->n + 1), hipMemcpyDeviceToHost, g->ctx->stream));
+  if (g->m) RG_HIP(hipMemcpyAsync(nbrs, b, sizeof(uint32_t) * g->m, hipMemcpyDeviceToHost, g->ctx->stream));
+  RG_HIP(hipStreamSynchronize(g->ctx->stream));
+  return RG_OK;
+}
+
+int rg_graph_transpose(rg_graph* g) {
+  if (!g) return set_error(RG_EINVAL, "rg_graph_transpose: null graph");
+  if (g->symmetric) return RG_OK;
+  std::swap(g->off, g->in_off);
+  std::swap(g->nbr, g->in_nbr);
+  g->transposed = !g->transposed;
+  g->hub_k_req = -1;   // hub metadata follows the in-degrees of the orientation
+  return RG_OK;
+}
+
+int rg_graph_free(rg_graph* g) {
+  if (!g) return RG_OK;
+  if (g->ctx) (void)hipSetDevice(g->ctx->device);
+  if (g->in_off && g->in_off != g->off) (void)hipFree(g->in_off);
+  if (g->in_nbr && g->in_nbr != g->nbr) (void)hipFree(g->in_nbr);
+  if (g->off) (void)hipFree(g->off);
+  if (g->nbr) (void)hipFree(g->nbr);
+  if (g->hub_bits) (void)hipFree(g->hub_bits);
+  if (g->hub_rank) (void)hipFree(g->hub_rank);
+  if (g->hub_ids) (void)hipFree(g->hub_ids);
+  delete g;
+  return RG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,376 @@
+"""Python binding of the radar_graph C ABI (include/radar_graph.h).
+
+Loads ``librg.so`` from this directory with ctypes; importing fails loudly
+when the library has not been built (there is no CPU fallback).  The classes
+mirror the reference's objects: :class:`HostGraph` is a Ligra/GAP graph in
+host memory, :class:`DeviceGraph` the device CSR that the apps (the
+reference's ``Compute`` bodies: PageRank, BFS, Radii, BC, triangle
+counting) run on.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librg.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"radar_graph: {LIB_PATH} is missing; build it first "
+        "(python -c 'import __graft_entry__ as g; g.build()' or make -C radar-graph_amd)")
+
+lib = C.CDLL(LIB_PATH)
+
+RG_OK, RG_EINVAL, RG_ENOMEM, RG_EDEVICE, RG_EIO, RG_EFORMAT, RG_EUNSUPPORTED = range(7)
+FMT_ADJ, FMT_ADJ_BINARY, FMT_EL, FMT_SG = range(4)
+PUSH, PULL, HUBDUP, RADAR = range(4)
+HUBS_LDS, HUBS_L2 = range(2)
+VARIANTS = {"push": PUSH, "pull": PULL, "hubdup": HUBDUP, "radar": RADAR}
+
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class RGError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"radar_graph error {code}: {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    """rg_opts: variant, hub placement, hub count k, sparse/dense threshold,
+    workgroup size and the wave-per-vertex cut-over degree."""
+    _fields_ = [("variant", C.c_int), ("hub_placement", C.c_int), ("hubs", C.c_int64),
+                ("threshold", C.c_int64), ("wg_size", C.c_int), ("heavy_degree", C.c_int)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("rg_last_error", C.c_char_p)
+_sig("rg_version", C.c_char_p)
+_sig("rg_generate_edge_count", C.c_int64, C.c_int, C.c_int)
+_sig("rg_generate_edges", C.c_int, C.c_int, C.c_int, C.c_int, _u32p, _u32p)
+_sig("rg_hgraph_generate", C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp))
+_sig("rg_hgraph_from_edges", C.c_int, C.c_int64, C.c_int64, _u32p, _u32p, C.c_int, C.POINTER(_vp))
+_sig("rg_hgraph_from_csr", C.c_int, C.c_int64, C.c_int64, _u64p, _u32p, C.c_int, C.c_int, C.POINTER(_vp))
+_sig("rg_hgraph_read", C.c_int, C.c_char_p, C.c_int, C.c_int, C.POINTER(_vp))
+_sig("rg_hgraph_write", C.c_int, _vp, C.c_char_p, C.c_int)
+_sig("rg_hgraph_info", C.c_int, _vp, _i64p, _i64p, C.POINTER(C.c_int))
+_sig("rg_hgraph_offsets", _u64p, _vp, C.c_int)
+_sig("rg_hgraph_neighbors", _u32p, _vp, C.c_int)
+_sig("rg_hgraph_free", None, _vp)
+_sig("rg_pick_sources_gap", C.c_int, _vp, C.c_int, _u32p)
+_sig("rg_pick_sources_bc", C.c_int, _vp, C.c_int, _u32p)
+_sig("rg_hash_int", C.c_uint64, C.c_uint64)
+_sig("rg_device_count", C.c_int, C.POINTER(C.c_int))
+_sig("rg_ctx_create", C.c_int, C.c_int, C.POINTER(_vp))
+_sig("rg_ctx_destroy", C.c_int, _vp)
+_sig("rg_ctx_sync", C.c_int, _vp)
+_sig("rg_graph_upload", C.c_int, _vp, _vp, C.POINTER(_vp))
+_sig("rg_graph_info", C.c_int, _vp, _i64p, _i64p, C.POINTER(C.c_int))
+_sig("rg_graph_download", C.c_int, _vp, C.c_int, _u64p, _u32p)
+_sig("rg_graph_transpose", C.c_int, _vp)
+_sig("rg_graph_free", C.c_int, _vp)
+_sig("rg_opts_default", None, C.POINTER(Opts))
+_sig("rg_pagerank", C.c_int, _vp, C.c_int64, C.POINTER(Opts), _f64p, _i64p, _f64p, _f64p)
+_sig("rg_bfs", C.c_int, _vp, C.c_uint32, C.POINTER(Opts), _u32p, _i64p, _i64p, _i64p, _f64p)
+_sig("rg_radii", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _i32p, _i64p, _f64p)
+_sig("rg_bc", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _f64p, _i64p, _f64p)
+
+_OPTIONAL = {}
+
+
+def _opt_sig(name, res, *args):
+    if hasattr(lib, name):
+        _OPTIONAL[name] = _sig(name, res, *args)
+
+
+_opt_sig("rg_graph_degree_sort", C.c_int, _vp, C.c_int, C.c_int, C.POINTER(_vp), _u32p)
+_opt_sig("rg_tc", C.c_int, _vp, C.POINTER(Opts), _u64p, _u64p, _f64p)
+
+
+def exported_symbols():
+    """Names the loaded library exports among those the header declares."""
+    return sorted(n for n in dir(lib) if n.startswith("rg_"))
+
+
+def _check(rc: int):
+    if rc != RG_OK:
+        raise RGError(rc, lib.rg_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def opts(variant="push", hubs=0, threshold=-1, wg_size=0, heavy_degree=0, hub_placement=HUBS_LDS) -> Opts:
+    o = Opts()
+    lib.rg_opts_default(C.byref(o))
+    o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    o.hubs = int(hubs)
+    o.threshold = int(threshold)
+    o.wg_size = int(wg_size)
+    o.heavy_degree = int(heavy_degree)
+    o.hub_placement = int(hub_placement)
+    return o
+
+
+def generate_edges(scale: int, degree: int, uniform: bool = False):
+    ne = lib.rg_generate_edge_count(scale, degree)
+    if ne < 0:
+        raise RGError(RG_EINVAL, "bad scale/degree")
+    s = np.empty(ne, np.uint32)
+    d = np.empty(ne, np.uint32)
+    _check(lib.rg_generate_edges(scale, degree, int(uniform), _ptr(s, _u32p), _ptr(d, _u32p)))
+    return s, d
+
+
+def hash_int(a: int) -> int:
+    return int(lib.rg_hash_int(C.c_uint64(a)))
+
+
+def radii_starts(n: int, k: int = 64):
+    """Radii's sources: hashInt(i) % n for i < min(n, 64) (Radii.C:117-127)."""
+    return np.array([hash_int(i) % n for i in range(min(n, k))], np.uint32)
+
+
+class HostGraph:
+    """Host CSR with the reference readers' dedup semantics."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def generate(cls, scale, degree=16, uniform=False, symmetric=True):
+        h = _vp()
+        _check(lib.rg_hgraph_generate(scale, degree, int(uniform), int(symmetric), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, n, src, dst, symmetric=True):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        h = _vp()
+        _check(lib.rg_hgraph_from_edges(n, len(src), _ptr(src, _u32p), _ptr(dst, _u32p),
+                                        int(symmetric), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, offsets, nbrs, symmetric=True, dedup=False):
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        nbrs = np.ascontiguousarray(nbrs, np.uint32)
+        h = _vp()
+        _check(lib.rg_hgraph_from_csr(len(offsets) - 1, len(nbrs), _ptr(offsets, _u64p),
+                                      _ptr(nbrs, _u32p), int(symmetric), int(dedup), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read(cls, path, fmt=FMT_ADJ, symmetric=True):
+        h = _vp()
+        _check(lib.rg_hgraph_read(str(path).encode(), fmt, int(symmetric), C.byref(h)))
+        return cls(h)
+
+    def write(self, path, fmt=FMT_ADJ):
+        _check(lib.rg_hgraph_write(self._h, str(path).encode(), fmt))
+
+    def info(self):
+        n, m, s = C.c_int64(), C.c_int64(), C.c_int()
+        _check(lib.rg_hgraph_info(self._h, C.byref(n), C.byref(m), C.byref(s)))
+        return n.value, m.value, bool(s.value)
+
+    @property
+    def n(self):
+        return self.info()[0]
+
+    @property
+    def m(self):
+        return self.info()[1]
+
+    @property
+    def symmetric(self):
+        return self.info()[2]
+
+    def offsets(self, in_edges=False) -> np.ndarray:
+        n = self.n
+        p = lib.rg_hgraph_offsets(self._h, int(in_edges))
+        return np.ctypeslib.as_array(p, shape=(n + 1,)).copy()
+
+    def neighbors(self, in_edges=False) -> np.ndarray:
+        m = self.m
+        if m == 0:
+            return np.zeros(0, np.uint32)
+        p = lib.rg_hgraph_neighbors(self._h, int(in_edges))
+        return np.ctypeslib.as_array(p, shape=(m,)).copy()
+
+    def csr(self, in_edges=False):
+        return self.offsets(in_edges), self.neighbors(in_edges)
+
+    def pick_sources(self, count):
+        out = np.empty(count, np.uint32)
+        _check(lib.rg_pick_sources_gap(self._h, count, _ptr(out, _u32p)))
+        return out
+
+    def pick_bc_sources(self, count):
+        out = np.empty(count, np.uint32)
+        _check(lib.rg_pick_sources_bc(self._h, count, _ptr(out, _u32p)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib.rg_hgraph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    c = C.c_int()
+    _check(lib.rg_device_count(C.byref(c)))
+    return c.value
+
+
+class Context:
+    """One device, one stream, reusable work pools (rg_ctx)."""
+
+    def __init__(self, device=0):
+        self._h = _vp()
+        _check(lib.rg_ctx_create(device, C.byref(self._h)))
+
+    def sync(self):
+        _check(lib.rg_ctx_sync(self._h))
+
+    def close(self):
+        if self._h:
+            lib.rg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class PageRankResult:
+    ranks: np.ndarray
+    iters: int
+    l1: float
+    ms: float
+
+
+@dataclass
+class BFSResult:
+    parents: np.ndarray
+    iters: int
+    reached: int
+    edges: int
+    ms: float
+
+
+class DeviceGraph:
+    """Device CSR (+CSC) and the app entry points."""
+
+    def __init__(self, ctx: Context, hg: HostGraph = None, handle=None):
+        self.ctx = ctx
+        if handle is not None:
+            self._h = handle
+        else:
+            self._h = _vp()
+            _check(lib.rg_graph_upload(ctx._h, hg._h, C.byref(self._h)))
+        n, m, s = C.c_int64(), C.c_int64(), C.c_int()
+        _check(lib.rg_graph_info(self._h, C.byref(n), C.byref(m), C.byref(s)))
+        self.n, self.m, self.symmetric = n.value, m.value, bool(s.value)
+
+    def download(self, in_edges=False):
+        off = np.empty(self.n + 1, np.uint64)
+        nb = np.empty(max(self.m, 1), np.uint32)
+        _check(lib.rg_graph_download(self._h, int(in_edges), _ptr(off, _u64p), _ptr(nb, _u32p)))
+        return off, nb[: self.m]
+
+    def transpose(self):
+        _check(lib.rg_graph_transpose(self._h))
+
+    def degree_sort(self, by_outdeg=False, sort_neighbors=False):
+        """GPU DegreeSorting; returns (new DeviceGraph, new_ids)."""
+        f = _OPTIONAL.get("rg_graph_degree_sort")
+        if f is None:
+            raise RGError(RG_EUNSUPPORTED, "rg_graph_degree_sort not in this build")
+        ids = np.empty(self.n, np.uint32)
+        h = _vp()
+        _check(f(self._h, int(by_outdeg), int(sort_neighbors), C.byref(h), _ptr(ids, _u32p)))
+        return DeviceGraph(self.ctx, handle=h), ids
+
+    def pagerank(self, maxiters=20, o: Opts = None, want_ranks=True) -> PageRankResult:
+        o = o or opts()
+        ranks = np.empty(self.n, np.float64) if want_ranks else None
+        it, l1, ms = C.c_int64(), C.c_double(), C.c_double()
+        _check(lib.rg_pagerank(self._h, maxiters, C.byref(o),
+                               _ptr(ranks, _f64p) if want_ranks else None,
+                               C.byref(it), C.byref(l1), C.byref(ms)))
+        return PageRankResult(ranks, it.value, l1.value, ms.value)
+
+    def bfs(self, src, o: Opts = None, want_parents=True) -> BFSResult:
+        o = o or opts()
+        par = np.empty(self.n, np.uint32) if want_parents else None
+        it, r, e, ms = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        _check(lib.rg_bfs(self._h, int(src), C.byref(o), _ptr(par, _u32p) if want_parents else None,
+                          C.byref(it), C.byref(r), C.byref(e), C.byref(ms)))
+        return BFSResult(par, it.value, r.value, e.value, ms.value)
+
+    def radii(self, starts, o: Opts = None):
+        o = o or opts()
+        starts = np.ascontiguousarray(starts, np.uint32)
+        rad = np.empty(self.n, np.int32)
+        rounds, ms = C.c_int64(), C.c_double()
+        _check(lib.rg_radii(self._h, _ptr(starts, _u32p), len(starts), C.byref(o), _ptr(rad, _i32p),
+                            C.byref(rounds), C.byref(ms)))
+        return rad, rounds.value, ms.value
+
+    def bc(self, sources, o: Opts = None):
+        o = o or opts()
+        sources = np.ascontiguousarray(sources, np.uint32)
+        deps = np.empty(self.n, np.float64)
+        comb, ms = C.c_int64(), C.c_double()
+        _check(lib.rg_bc(self._h, _ptr(sources, _u32p), len(sources), C.byref(o), _ptr(deps, _f64p),
+                         C.byref(comb), C.byref(ms)))
+        return deps, comb.value, ms.value
+
+    def tc(self, o: Opts = None, per_vertex=False):
+        f = _OPTIONAL.get("rg_tc")
+        if f is None:
+            raise RGError(RG_EUNSUPPORTED, "rg_tc not in this build")
+        o = o or opts()
+        tot, ms = C.c_uint64(), C.c_double()
+        pv = np.empty(self.n, np.uint64) if per_vertex else None
+        _check(f(self._h, C.byref(o), C.byref(tot), _ptr(pv, _u64p) if per_vertex else None, C.byref(ms)))
+        return (tot.value, pv, ms.value) if per_vertex else (tot.value, ms.value)
+
+    def close(self):
+        if self._h:
+            lib.rg_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

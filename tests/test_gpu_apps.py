@@ -1,0 +1,91 @@
+"""GPU parity: every app through the C ABI against the oracle on seeded
+graphs.  Integers (BFS levels, Radii) bit-exact; BFS parents a valid
+shortest-path tree; FP64 (PageRank, BC) within 1e-10 relative L-inf, the
+only difference being atomic summation order."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP_TOL = 1e-10   # relative L-inf bound for FP64 results
+VARIANTS = ["push", "hubdup", "radar"]
+
+
+def rel_linf(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nan_a, nan_b = np.isnan(a), np.isnan(b)
+    assert np.array_equal(nan_a, nan_b), "NaN positions differ"
+    a, b = a[~nan_a], b[~nan_b]
+    if a.size == 0:
+        return 0.0
+    assert np.array_equal(np.isinf(a), np.isinf(b))
+    fin = np.isfinite(a)
+    assert np.array_equal(a[~fin], b[~fin])
+    a, b = a[fin], b[fin]
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12s"])
+@pytest.mark.parametrize("variant", VARIANTS + ["pull"])
+def test_pagerank(rg, orc, ctx, graphs, gname, variant):
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    print(f"pagerank {gname} {variant}", flush=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    res = dg.pagerank(20, rg.opts(variant, hubs=256))
+    if variant == "pull":
+        ioff, inbr = hg.csr(in_edges=True)
+        ref, it, l1 = orc.pagerank_pull(off, ioff, inbr, 20)
+    else:
+        ref, it, l1 = orc.pagerank(off, nbr, 20)
+    assert res.iters == it
+    assert rel_linf(res.ranks, ref) < FP_TOL
+    assert abs(res.l1 - l1) <= 1e-9 * max(abs(l1), 1e-300)
+
+
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d"])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_bfs(rg, orc, ctx, graphs, gname, variant):
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    srcs = hg.pick_sources(4)
+    print(f"bfs {gname} {variant} {srcs}", flush=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    for s in srcs:
+        r = dg.bfs(int(s), rg.opts(variant, hubs=128))
+        lv, ecc = orc.bfs_levels(off, nbr, int(s))
+        assert orc.check_parents(off, nbr, int(s), lv, r.parents) == 0
+        assert r.reached == int((lv >= 0).sum())
+        assert r.iters == ecc + 1
+        assert r.edges == int(np.diff(off)[lv >= 0].sum())
+
+
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d"])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_radii(rg, orc, ctx, graphs, gname, variant):
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    starts = rg.radii_starts(hg.n)
+    print(f"radii {gname} {variant}", flush=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    rad, rounds, _ = dg.radii(starts, rg.opts(variant, hubs=128))
+    ref, rr = orc.radii(off, nbr, starts)
+    assert rounds == rr
+    assert np.array_equal(rad, ref)
+
+
+@pytest.mark.parametrize("gname", ["rmat10s", "rmat12d"])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_bc(rg, orc, ctx, graphs, gname, variant):
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    ioff, inbr = hg.csr(in_edges=True)
+    srcs = hg.pick_bc_sources(4)
+    print(f"bc {gname} {variant} {srcs}", flush=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    deps, comb, _ = dg.bc(srcs, rg.opts(variant, hubs=128))
+    ref, rc = orc.bc(off, nbr, ioff, inbr, srcs)
+    assert comb == rc
+    assert rel_linf(deps, ref) < FP_TOL

@@ -1,0 +1,164 @@
+"""Benchmark: PageRank / BFS GTEPS of push-only, HUBDUP, DegreeSorting and
+RADAR on one MI355X, on the directed R-MAT graph (GAP generator, scale 24,
+edge factor 16, a/b/c = 0.57/0.19/0.19, seed 27491095, self-loops and
+duplicates removed; int32 ids, uint64 offsets, FP64 ranks).
+
+One "step" is one PageRank run of 20 iterations with the RADAR engine
+(degree-sorted graph, hub copies) -- the headline; its GTEPS is
+20 * m / time.  The other variants, BFS from 64 GAP-picked sources, and
+the CPU oracle baseline are reported alongside.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S]
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "radar-graph_amd"), os.path.join(ROOT, "oracle")]
+
+PEAK_HBM_GBS = 8000.0   # MI355X HBM3E peak, GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--bfs-sources", type=int, default=64)
+    ap.add_argument("--hubs", type=int, default=0, help="LDS hub slots (0 = LDS budget)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is described in DESIGN.md and not measured", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import radar_graph as rg
+
+    t0 = time.time()
+    hg = rg.HostGraph.generate(args.scale, 16, uniform=False, symmetric=False)
+    n, m, _ = hg.info()
+    t_gen = time.time() - t0
+    log(f"graph: rmat{args.scale} directed n={n} m={m} (host build {t_gen:.1f}s)")
+    ctx = rg.Context(0)
+    dg = rg.DeviceGraph(ctx, hg)
+    t0 = time.time()
+    sg, new_ids = dg.degree_sort(by_outdeg=False, sort_neighbors=False)
+    ctx.sync()
+    t_sort = time.time() - t0
+    log(f"degree sort (GPU): {t_sort * 1e3:.1f} ms")
+
+    it = args.iters
+    variants = {
+        "push": (dg, rg.opts("push")),
+        "hubdup": (dg, rg.opts("hubdup", hubs=args.hubs)),
+        "degree_sorting": (sg, rg.opts("push")),
+        "radar": (sg, rg.opts("radar", hubs=args.hubs)),
+        "pull": (dg, rg.opts("pull")),
+    }
+    pr = {}
+    for name, (g, o) in variants.items():
+        g.pagerank(it, o, want_ranks=False)
+        ms = sorted(g.pagerank(it, o, want_ranks=False).ms for _ in range(3))[1]
+        pr[name] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2)}
+        log(f"pagerank {name:15s} {ms:8.2f} ms  {pr[name]['gteps']:7.2f} GTEPS")
+
+    # Headline: RADAR PageRank, K timed steps after W warmups.
+    g, o = variants["radar"]
+    for _ in range(args.warmup):
+        g.pagerank(it, o, want_ranks=False)
+    ctx.sync()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = g.pagerank(it, o, want_ranks=False)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    launches, kms = ctx.kernel_time()
+    ctx.timing(False)
+    ms_step = wall * 1e3 / args.steps
+    value = args.steps * it * m / wall / 1e9
+    # Algorithmic bytes of one edge-map launch (one PageRank iteration's push):
+    # per arc 4 B column + 16 B read-modify-write of the FP64 accumulator,
+    # per vertex 8 B offset + 8 B contribution.
+    bytes_launch = 20 * m + 16 * n
+    avg_launch_ms = kms / max(launches, 1)
+    achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_pagerank_radar_rmat24.json")
+    if os.path.exists(tpath) and args.scale == 24:
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    # BFS from 64 sources per variant.
+    srcs = hg.pick_sources(args.bfs_sources)
+    bfs = {}
+    for name, g2, o2, mapped in [("push", dg, rg.opts("push"), False),
+                                 ("hubdup", dg, rg.opts("hubdup", hubs=args.hubs), False),
+                                 ("degree_sorting", sg, rg.opts("push"), True),
+                                 ("radar", sg, rg.opts("radar", hubs=args.hubs), True)]:
+        tot_e, tot_ms = 0, 0.0
+        for s in srcs:
+            s2 = int(new_ids[s]) if mapped else int(s)
+            r = g2.bfs(s2, o2, want_parents=False)
+            tot_e += r.edges
+            tot_ms += r.ms
+        bfs[name] = {"gteps": round(tot_e / tot_ms / 1e6, 2), "ms_per_source": round(tot_ms / len(srcs), 3)}
+        log(f"bfs {name:15s} {bfs[name]['ms_per_source']:8.3f} ms/src {bfs[name]['gteps']:7.2f} GTEPS")
+
+    cpu = None
+    if not args.no_cpu:
+        import pyoracle
+        off, nbr = hg.csr()
+        cores = min(16, os.cpu_count() or 1)
+        cpu_iters = 2
+        t0 = time.perf_counter()
+        pyoracle.pagerank(off, nbr, cpu_iters, nthreads=cores)
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(cpu_iters * m / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
+               "sample": f"oracle push PageRank (OpenMP, CAS-loop FP64 adds as the reference), "
+                         f"{cpu_iters} iterations on the same rmat{args.scale} graph, {tc:.1f} s"}
+        log(f"cpu baseline: {cpu['value']} GTEPS on {cores} threads")
+
+    out = {
+        "metric": "GTEPS per app on RMAT-26 + batched BFS queries/s, p50/p99 vs one-at-a-time",
+        "value": round(value, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (GAP R-MAT generator, seed 27491095)",
+        "config": {"workload": f"PageRank {it} iterations, RADAR (degree-sorted + LDS hub copies), "
+                               f"directed rmat{args.scale} edge factor 16",
+                   "n": n, "m": m, "iterations": it, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "edge_light_kernel<PrOp,SRC_ALL,HUB_PREFIX>",
+                     "launches": launches, "avg_launch_ms": round(avg_launch_ms, 4),
+                     "bytes_per_launch": bytes_launch},
+        "cpu_baseline": cpu,
+        "pagerank": pr,
+        "bfs": bfs,
+        "preprocess_ms": {"host_graph_build": round(t_gen * 1e3, 1), "gpu_degree_sort": round(t_sort * 1e3, 2)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -101,6 +101,14 @@ int rg_device_count(int* count);
 int rg_ctx_create(int device, rg_ctx** out);
 int rg_ctx_destroy(rg_ctx* ctx);
 int rg_ctx_sync(rg_ctx* ctx);
+/* Instrumentation (the reference times whole Compute calls with its Timer,
+ * PNO Ligra/ligra/timer.h; this adds per-launch device time): with timing
+ * enabled, every edge-map kernel launch is bracketed by HIP events on the
+ * context stream.  rg_ctx_timing(ctx, 1) also resets the record;
+ * rg_ctx_kernel_time returns the number of timed launches and their summed
+ * device time since then. */
+int rg_ctx_timing(rg_ctx* ctx, int enable);
+int rg_ctx_kernel_time(rg_ctx* ctx, int64_t* launches, double* total_ms);
 
 /* ---- device graphs ------------------------------------------------------- */
 typedef struct rg_graph rg_graph;
@@ -143,10 +151,13 @@ enum { RG_HUBS_LDS = 0, RG_HUBS_L2 = 1 };
 typedef struct rg_opts {
   int variant;          /* RG_PUSH ...                                        */
   int hub_placement;    /* RG_HUBS_LDS / RG_HUBS_L2                           */
-  int64_t hubs;         /* k; <= 0 selects the budget default                 */
+  int64_t hubs;         /* k of the LDS tier; <= 0 selects the LDS budget     */
   int64_t threshold;    /* sparse/dense switch; -1 = m/20 (ligra.h:269)       */
   int wg_size;          /* workgroup size for edge kernels; 0 = default       */
   int heavy_degree;     /* wave-per-vertex cut-over degree; 0 = default       */
+  int64_t l2_hubs;      /* HUBDUP/RADAR FP64 accumulators: hub slots [hubs,    */
+                        /* l2_hubs) get one copy per XCD (L2-resident); 0 =    */
+                        /* half an XCD L2 (256K), < 0 = off                    */
 } rg_opts;
 
 /* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */

@@ -24,6 +24,7 @@ struct PrOp {
   using SrcT = double;
   using HubT = double;
   static constexpr bool kOutput = false;
+  static constexpr bool kL2 = true;
   const double* contrib;
   double* acc;
   const int64_t* done;
@@ -40,6 +41,12 @@ struct PrOp {
     unsafeAtomicAdd(acc + d, v);
     return false;
   }
+  // The per-XCD copy is touched by one XCD only: a workgroup-scope atomic
+  // executes in that XCD's L2 instead of being forwarded to memory.
+  __device__ void l2_update(double* slot, double sv) const {
+    __hip_atomic_fetch_add(slot, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ double hub_combine(double x, double y) const { return x + y; }
 };
 
 // BFS_F (BFS.C:169-190): first CAS on Parents[d] wins and d joins the next
@@ -49,6 +56,9 @@ struct BfsOp {
   using SrcT = uint32_t;
   using HubT = uint32_t;
   static constexpr bool kOutput = true;
+  static constexpr bool kL2 = false;
+  __device__ void l2_update(uint32_t*, uint32_t) const {}
+  __device__ uint32_t hub_combine(uint32_t x, uint32_t) const { return x; }
   uint32_t* parents;
   __device__ bool skip() const { return false; }
   __device__ uint32_t load_src(uint32_t s) const { return s; }
@@ -73,6 +83,9 @@ struct RadiiOp {
   using SrcT = unsigned long long;
   using HubT = unsigned long long;
   static constexpr bool kOutput = true;
+  static constexpr bool kL2 = false;
+  __device__ void l2_update(unsigned long long*, unsigned long long) const {}
+  __device__ unsigned long long hub_combine(unsigned long long x, unsigned long long y) const { return x | y; }
   const unsigned long long* vis;
   unsigned long long* nvis;
   int32_t* radii;
@@ -111,6 +124,9 @@ struct BcFwdOp {
   using SrcT = double;
   using HubT = double;
   static constexpr bool kOutput = true;
+  static constexpr bool kL2 = false;
+  __device__ void l2_update(double*, double) const {}
+  __device__ double hub_combine(double x, double y) const { return x + y; }
   double* np;
   const uint8_t* visited;
   __device__ bool skip() const { return false; }
@@ -129,6 +145,13 @@ struct BcBackOp {
   using SrcT = double;
   using HubT = double;
   static constexpr bool kOutput = false;
+  static constexpr bool kL2 = true;
+  // The per-XCD copy is touched by one XCD only: a workgroup-scope atomic
+  // executes in that XCD's L2 instead of being forwarded to memory.
+  __device__ void l2_update(double* slot, double sv) const {
+    __hip_atomic_fetch_add(slot, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ double hub_combine(double x, double y) const { return x + y; }
   double* deps;
   const uint8_t* visited;
   __device__ bool skip() const { return false; }
@@ -202,10 +225,27 @@ struct Engine {
   int block;           // 256 or 1024
   int heavy_deg;
   int hub_mode;        // HUB_NONE / HUB_BITMAP / HUB_PREFIX
-  int64_t hub_k;
+  int64_t hub_k;       // LDS slots
+  int64_t hub_k2;      // LDS + per-XCD slots
+  void* l2_copies;
+  int l2_ncopies;
   DevBuf* heavy;       // heavy work queue
   int64_t* heavy_count;
 };
+
+// With timing on, records the start event of one edge-map launch on the
+// context stream and returns the event pair (the caller records [1]).
+static hipEvent_t* timing_begin(rg_ctx* ctx) {
+  if (!ctx->timing) return nullptr;
+  if (ctx->ev_used == ctx->ev_pairs.size()) {
+    rg_ctx::EvPair p;
+    if (hipEventCreate(&p.e[0]) != hipSuccess || hipEventCreate(&p.e[1]) != hipSuccess) return nullptr;
+    ctx->ev_pairs.push_back(p);
+  }
+  auto& p = ctx->ev_pairs[ctx->ev_used++];
+  if (hipEventRecord(p.e[0], ctx->stream) != hipSuccess) return nullptr;
+  return p.e;
+}
 
 template <class Op, int MODE, int HUB, int BLOCK>
 static int launch_edge(const Engine& E, const Op& op, const EdgeArgs& a, size_t hub_bytes) {
@@ -232,10 +272,21 @@ static int launch_edge(const Engine& E, const Op& op, const EdgeArgs& a, size_t 
   }
   int64_t cap = (int64_t)E.ctx->num_cus * per_cu;
   int grid_l = (int)std::max<int64_t>(1, std::min(want, cap));
+  if (a.static_heavy) grid_l = (int)cap;   // the heavy phase needs the full grid
+  hipEvent_t* ev = timing_begin(E.ctx);
   hipLaunchKernelGGL(kl, grid_l, BLOCK, hub_bytes, E.ctx->stream, op, a);
   RG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(kh, (int)cap, BLOCK, hub_bytes, E.ctx->stream, op, a);
-  RG_HIP(hipGetLastError());
+  if (ev) RG_HIP(hipEventRecord(ev[1], E.ctx->stream));
+  if (!a.static_heavy) {
+    hipLaunchKernelGGL(kh, (int)cap, BLOCK, hub_bytes, E.ctx->stream, op, a);
+    RG_HIP(hipGetLastError());
+  }
+  if (Op::kL2 && HUB != HUB_NONE && a.hub_k2 > a.hub_k) {
+    int64_t span = a.hub_k2;
+    int g = (int)std::max<int64_t>(1, std::min<int64_t>((span + 255) / 256, (int64_t)E.ctx->num_cus * 8));
+    hipLaunchKernelGGL((k_l2_merge<Op, HUB>), g, 256, 0, E.ctx->stream, op, a);
+    RG_HIP(hipGetLastError());
+  }
   return RG_OK;
 }
 
@@ -251,8 +302,28 @@ static int launch_hub(const Engine& E, const Op& op, EdgeArgs a) {
   switch (E.hub_mode) {
     case HUB_BITMAP: return launch_block<Op, MODE, HUB_BITMAP>(E, op, a, hub_bytes);
     case HUB_PREFIX: return launch_block<Op, MODE, HUB_PREFIX>(E, op, a, hub_bytes);
-    default: a.hub_k = 0; return launch_block<Op, MODE, HUB_NONE>(E, op, a, 0);
+    default: a.hub_k = a.hub_k2 = 0; return launch_block<Op, MODE, HUB_NONE>(E, op, a, 0);
   }
+}
+
+// Builds (once per out-CSR and heavy degree) the chunk list of the vertices
+// whose degree exceeds heavy_deg, for all-vertices frontiers.
+static int static_heavy_list(rg_graph* g, int heavy_deg) {
+  if (g->heavy_items && g->heavy_items_deg == heavy_deg && g->heavy_items_off == g->off) return RG_OK;
+  rg_ctx* ctx = g->ctx;
+  if (g->heavy_items) (void)hipFree(g->heavy_items);
+  g->heavy_items = nullptr;
+  int64_t cap = g->m / kHeavyChunk + g->m / std::max(1, heavy_deg) + 64;
+  RG_HIP(hipMalloc(&g->heavy_items, sizeof(uint64_t) * cap));
+  int64_t* cnt = ctx->counts.as<int64_t>() + 24;
+  RG_HIP(hipMemsetAsync(cnt, 0, sizeof(int64_t), ctx->stream));
+  hipLaunchKernelGGL(k_heavy_list, grid_for(ctx, g->n), 256, 0, ctx->stream, g->off, g->n, heavy_deg,
+                     g->heavy_items, cnt);
+  RG_HIP(hipGetLastError());
+  RG_TRY(ctx_read_count(ctx, cnt, &g->heavy_items_count, 1));
+  g->heavy_items_deg = heavy_deg;
+  g->heavy_items_off = g->off;
+  return RG_OK;
 }
 
 // Ligra edgeMapData (ligra.h:264-302) on the device: chooses the dense
@@ -276,11 +347,18 @@ static int edge_map(const Engine& E, const Op& op, Frontier& F, int64_t f_degsum
   a.heavy_count = E.heavy_count;
   a.heavy_deg = E.heavy_deg;
   a.hub_k = E.hub_k;
+  a.hub_k2 = Op::kL2 ? E.hub_k2 : E.hub_k;
   a.hub_bits = g->hub_bits;
   a.hub_rank = g->hub_rank;
+  a.hub_slot_of = g->hub_slot_of;
   a.hub_ids = g->hub_ids;
+  a.l2_copies = E.l2_copies;
+  a.l2_ncopies = E.l2_ncopies;
   if (F.all) {
     a.count = g->n;
+    RG_TRY(static_heavy_list(g, E.heavy_deg));
+    a.static_heavy = g->heavy_items;
+    a.static_heavy_count = g->heavy_items_count;
     return launch_hub<Op, SRC_ALL>(E, op, a);
   }
   if (f_degsum == 0) return RG_OK;   // ligra.h:287
@@ -321,8 +399,10 @@ __global__ void k_in_degree(const uint64_t* in_off, int64_t n, uint32_t* deg) {
 
 // HUBDUP's identifyHubs (hubdup Ligra/ligra/IO.h:726-804): hubs are the
 // vertices whose in-degree exceeds the k-th largest in-degree (0-based),
-// so at most k of them.  Stored as a bitmap, a per-32-bit-word rank prefix
-// and a slot -> vertex table (slots in vertex-id order).
+// so at most k of them.  Stored as a bitmap with a per-32-bit-word rank
+// prefix (id-order rank of a hub), a rank -> slot table and a slot -> vertex
+// table; slots are ordered by descending in-degree so the first slots (the
+// LDS tier) are the heaviest hubs.
 static int build_hub_bitmap(rg_graph* g, int64_t k) {
   rg_ctx* ctx = g->ctx;
   if (g->hub_k_req == k && g->hub_bits) return RG_OK;
@@ -354,17 +434,34 @@ static int build_hub_bitmap(rg_graph* g, int64_t k) {
     rank[w] = acc;
     acc += (uint32_t)__builtin_popcount(bits[w]);
   }
+  // Slots by (in-degree, id) descending; slot_of maps id-order rank -> slot.
+  std::vector<uint32_t> by_deg(ids.size());
+  for (size_t i = 0; i < ids.size(); ++i) by_deg[i] = (uint32_t)i;
+  std::sort(by_deg.begin(), by_deg.end(), [&](uint32_t x, uint32_t y) {
+    uint32_t dx = deg[ids[x]], dy = deg[ids[y]];
+    return dx != dy ? dx > dy : ids[x] > ids[y];
+  });
+  std::vector<uint32_t> slot_of(ids.size()), slot_ids(ids.size());
+  for (size_t s = 0; s < by_deg.size(); ++s) {
+    slot_of[by_deg[s]] = (uint32_t)s;
+    slot_ids[s] = ids[by_deg[s]];
+  }
   if (g->hub_bits) (void)hipFree(g->hub_bits);
   if (g->hub_rank) (void)hipFree(g->hub_rank);
   if (g->hub_ids) (void)hipFree(g->hub_ids);
-  g->hub_bits = g->hub_rank = g->hub_ids = nullptr;
+  if (g->hub_slot_of) (void)hipFree(g->hub_slot_of);
+  g->hub_bits = g->hub_rank = g->hub_ids = g->hub_slot_of = nullptr;
+  const size_t nh = std::max<size_t>(1, ids.size());
   RG_HIP(hipMalloc(&g->hub_bits, sizeof(uint32_t) * nw));
   RG_HIP(hipMalloc(&g->hub_rank, sizeof(uint32_t) * nw));
-  RG_HIP(hipMalloc(&g->hub_ids, sizeof(uint32_t) * std::max<size_t>(1, ids.size())));
+  RG_HIP(hipMalloc(&g->hub_ids, sizeof(uint32_t) * nh));
+  RG_HIP(hipMalloc(&g->hub_slot_of, sizeof(uint32_t) * nh));
   RG_HIP(hipMemcpy(g->hub_bits, bits.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
   RG_HIP(hipMemcpy(g->hub_rank, rank.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
-  if (!ids.empty())
-    RG_HIP(hipMemcpy(g->hub_ids, ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+  if (!ids.empty()) {
+    RG_HIP(hipMemcpy(g->hub_ids, slot_ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+    RG_HIP(hipMemcpy(g->hub_slot_of, slot_of.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+  }
   g->hub_k = (int64_t)ids.size();
   g->hub_k_req = k;
   return RG_OK;
@@ -379,26 +476,45 @@ static int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E) {
   E.block = (o && o->wg_size >= 1024) ? 1024 : 256;
   E.heavy_deg = (o && o->heavy_degree > 0) ? o->heavy_degree : 1024;
   E.hub_mode = HUB_NONE;
-  E.hub_k = 0;
+  E.hub_k = E.hub_k2 = 0;
+  E.l2_copies = nullptr;
+  E.l2_ncopies = 0;
   const int variant = o ? o->variant : RG_PUSH;
   if (variant == RG_HUBDUP || variant == RG_RADAR) {
     const int wpb = E.block / kWave;
-    size_t static_lds = wpb * ((kWave + 1) * 4 + kWave * (8 + 4 + 8));
+    size_t static_lds = wpb * ((kWave + 1) * 4 + kWave * (8 + 4 + 8) + kQueue * 4);
     size_t budget = ctx->max_lds_per_block > static_lds ? ctx->max_lds_per_block - static_lds : 0;
     int64_t kmax = (int64_t)(budget / elem);
-    int64_t k = (o && o->hubs > 0) ? o->hubs : std::min<int64_t>(kmax, (int64_t)(32768 / elem) * 2);
-    k = std::min<int64_t>(k, kmax);
+    // LDS tier: the requested k or the LDS budget (the reference sizes its
+    // copies from the cache budget llcCap, hubdup IO.h:749-755).
+    int64_t k = (o && o->hubs > 0) ? std::min<int64_t>(o->hubs, kmax) : kmax;
     k = std::min<int64_t>(k, g->n);
+    // Per-XCD tier: up to half of a 4 MB XCD L2 per copy by default.
+    int64_t k2 = k;
+    if (elem == sizeof(double)) {
+      int64_t l2 = (o && o->l2_hubs != 0) ? o->l2_hubs : (int64_t)((2u << 20) / elem);
+      if (l2 > k) k2 = std::min<int64_t>(l2, g->n);
+    }
     if (k > 0) {
       if (variant == RG_RADAR) {
         E.hub_mode = HUB_PREFIX;
         E.hub_k = k;
+        E.hub_k2 = k2;
       } else {
-        RG_TRY(build_hub_bitmap(g, k));
+        RG_TRY(build_hub_bitmap(g, k2));
         E.hub_mode = HUB_BITMAP;
-        E.hub_k = g->hub_k;
-        if (E.hub_k == 0) E.hub_mode = HUB_NONE;
+        E.hub_k2 = g->hub_k;
+        E.hub_k = std::min<int64_t>(k, g->hub_k);
+        if (E.hub_k2 == 0) E.hub_mode = HUB_NONE;
       }
+    }
+    if (E.hub_mode != HUB_NONE && E.hub_k2 > E.hub_k) {
+      E.l2_ncopies = ctx->num_xcds;
+      size_t bytes = (size_t)E.l2_ncopies * (size_t)E.hub_k2 * elem;
+      DevBuf& b = ctx->scratch[6];
+      RG_TRY(b.ensure(bytes));
+      RG_HIP(hipMemsetAsync(b.p, 0, bytes, ctx->stream));
+      E.l2_copies = b.p;
     }
   }
   // Heavy queue: at most m/heavy_deg heavy vertices, each split in chunks.

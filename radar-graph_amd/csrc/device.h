@@ -32,6 +32,10 @@ constexpr int kWave = 64;                 // CDNA wavefront width
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   int ensure(size_t want);
   void release();
   template <class T> T* as() const { return static_cast<T*>(p); }
@@ -53,6 +57,11 @@ struct rg_ctx {
   rg::DevBuf scratch[8];  // generic pools for apps / kernels
   rg::DevBuf fr_ids[2];   // frontier id lists (uint32[n] each)
   rg::DevBuf fr_dense;    // frontier byte map (uint8[n]); all zero between uses
+  // Optional HIP-event timing of the edge-map launches (rg_ctx_timing).
+  bool timing = false;
+  struct EvPair { hipEvent_t e[2]; };
+  std::vector<EvPair> ev_pairs;
+  size_t ev_used = 0;
 };
 
 struct rg_graph {
@@ -69,9 +78,16 @@ struct rg_graph {
   // in-degree ordering of the current orientation.
   uint32_t* hub_bits = nullptr;
   uint32_t* hub_rank = nullptr;
-  uint32_t* hub_ids = nullptr;   // slot -> vertex id
+  uint32_t* hub_ids = nullptr;   // slot -> vertex
+  uint32_t* hub_slot_of = nullptr; // id-order hub rank -> slot id
   int64_t hub_k = 0;             // hubs selected
   int64_t hub_k_req = -1;        // k the metadata was built for
+  // Static heavy-chunk list for all-vertices frontiers (PageRank), built
+  // for one (out-CSR, heavy degree) pair.
+  uint64_t* heavy_items = nullptr;
+  int64_t heavy_items_count = 0;
+  int heavy_items_deg = -1;
+  const uint64_t* heavy_items_off = nullptr;
 };
 
 namespace rg {

@@ -4,21 +4,27 @@
 // functor's updateAtomic inlined as an "operator" struct.
 //
 // Work decomposition (one persistent grid per launch):
-//   * light kernel: a wavefront takes 64 consecutive frontier entries; each
+//   * light phase: a wavefront takes 64 consecutive frontier entries; each
 //     lane owns one vertex; the wave scans the 64 degrees and then walks the
 //     concatenated edge ranges 64 edges at a time, each lane finding its
 //     source by a 6-step binary search over the scanned degrees in LDS.
 //     Consecutive lanes read consecutive column entries (coalesced).
-//   * vertices with degree > heavy_deg are not walked there; their edge
-//     range is cut into chunks of kHeavyChunk edges that the heavy kernel
-//     processes one wavefront per chunk.
+//   * heavy phase: vertices with degree > heavy_deg are cut into chunks of
+//     kHeavyChunk edges, one wavefront per chunk (lanes stride the chunk).
+//     For the all-vertices frontier the chunk list is static and cached per
+//     graph, so both phases run in one launch; for list / byte-map frontiers
+//     the light phase enqueues chunks and a second launch drains them.
 //   * hub duplication (HUBDUP / RADAR): destinations that are hubs are
 //     updated in a per-workgroup LDS copy instead of with a global atomic;
 //     the copy is merged into global state when the workgroup ends.
+//   * next frontier: each wavefront stages activated vertices in its own LDS
+//     queue and flushes them in batches (one global atomic per batch), so
+//     frontier output does not serialise on a single counter.
 //
 // Operators provide:
 //   using SrcT, HubT;
 //   static constexpr bool kOutput;   // produces a next frontier
+//   bool skip()                      // whole launch is a no-op (converged)
 //   SrcT load_src(uint32_t s)        // per-source value, loaded once
 //   bool cond(uint32_t d)            // Ligra cond
 //   bool update(uint32_t s, SrcT, uint32_t d)   // updateAtomic; true = d joins
@@ -36,6 +42,7 @@
 namespace rg {
 
 constexpr int kHeavyChunk = 2048;
+constexpr int kQueue = 256;   // per-wave LDS staging queue (vertices)
 
 enum SrcMode { SRC_ALL = 0, SRC_LIST = 1, SRC_DENSE = 2 };
 enum HubMode { HUB_NONE = 0, HUB_BITMAP = 1, HUB_PREFIX = 2 };
@@ -49,49 +56,86 @@ struct EdgeArgs {
   // next frontier (operators with kOutput)
   uint32_t* out_ids;
   int64_t* out_info;      // [0] = count, [1] = sum of out-degrees of the appended vertices
-  // heavy work queue
-  uint64_t* heavy;        // (vertex << 32 | chunk)
+  // heavy work: dynamic queue (list / byte-map frontiers) or a static list
+  // (all-vertices frontier), items are (vertex << 32 | chunk)
+  uint64_t* heavy;
   int64_t* heavy_count;
+  const uint64_t* static_heavy;
+  int64_t static_heavy_count;
   int heavy_deg;
-  // hubs
-  int64_t hub_k;                  // number of hub slots
+  // hubs.  Slots are ordered by descending in-degree; slots < hub_k live
+  // in the workgroup's LDS copy, slots in [hub_k, hub_k2) in the per-XCD
+  // copy l2_copies[xcd * hub_k2 + slot] (operators with kL2 only).
+  int64_t hub_k;                  // LDS slots
+  int64_t hub_k2;                 // LDS + per-XCD slots
   const uint32_t* hub_bits;       // HUB_BITMAP: bit per vertex
   const uint32_t* hub_rank;       // HUB_BITMAP: hubs before word w
+  const uint32_t* hub_slot_of;    // HUB_BITMAP: id-order rank -> slot
   const uint32_t* hub_ids;        // HUB_BITMAP: slot -> vertex
+  void* l2_copies;
+  int l2_ncopies;
 };
+
+// XCD of the executing wave (HW_REG_XCC_ID); picks the per-XCD hub copy.
+__device__ __forceinline__ int xcd_id() {
+  return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   return (uint64_t(1) << __lane_id()) - 1;
 }
 
-// Appends d to the next frontier for every lane with `act`; one atomic per
-// wavefront.  Must be called by all lanes of the wave (uniform control flow).
-__device__ __forceinline__ void wave_append(const EdgeArgs& a, bool act, uint32_t d) {
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-wavefront staging queue for the next frontier.  fill is wave-uniform.
+struct WaveQueue {
+  uint32_t* buf;   // kQueue entries in LDS
+  int fill;
+  uint64_t deg;    // this lane's share of the appended out-degrees
+};
+
+__device__ __forceinline__ void wq_flush(WaveQueue& q, const EdgeArgs& a) {
+  if (q.fill == 0) return;
+  wave_sync_lds();
+  unsigned long long base = 0;
+  if (__lane_id() == 0) base = atomicAdd((unsigned long long*)&a.out_info[0], (unsigned long long)q.fill);
+  base = __shfl(base, 0);
+  for (int i = (int)__lane_id(); i < q.fill; i += kWave) a.out_ids[base + i] = q.buf[i];
+  wave_sync_lds();
+  q.fill = 0;
+}
+
+// Appends d for every lane with `act`.  Called by all lanes of the wave.
+__device__ __forceinline__ void wq_push(WaveQueue& q, const EdgeArgs& a, bool act, uint32_t d) {
   uint64_t mask = __ballot(act);
   if (mask == 0) return;
-  uint32_t dd = 0;
-  if (act) dd = (uint32_t)(a.off[d + 1] - a.off[d]);
-  // Sum of appended degrees (drives the next round's sparse/dense choice).
-  uint64_t degsum = dd;
-  for (int o = 32; o > 0; o >>= 1) degsum += __shfl_xor(degsum, o);
-  int leader = __ffsll((long long)mask) - 1;
-  int64_t base = 0;
-  if ((int)__lane_id() == leader) {
-    base = atomicAdd((unsigned long long*)&a.out_info[0], (unsigned long long)__popcll(mask));
-    atomicAdd((unsigned long long*)&a.out_info[1], (unsigned long long)degsum);
+  if (act) {
+    q.buf[q.fill + __popcll(mask & lanemask_lt())] = d;
+    q.deg += a.off[d + 1] - a.off[d];
   }
-  base = __shfl(base, leader);
-  if (act) a.out_ids[base + __popcll(mask & lanemask_lt())] = d;
+  q.fill += __popcll(mask);
+  if (q.fill > kQueue - kWave) wq_flush(q, a);
+}
+
+__device__ __forceinline__ void wq_finish(WaveQueue& q, const EdgeArgs& a) {
+  wq_flush(q, a);
+  uint64_t d = q.deg;
+  for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+  if (__lane_id() == 0 && d) atomicAdd((unsigned long long*)&a.out_info[1], (unsigned long long)d);
 }
 
 template <int HUB>
 __device__ __forceinline__ int64_t hub_slot(const EdgeArgs& a, uint32_t d) {
-  if (HUB == HUB_PREFIX) return (int64_t)d < a.hub_k ? (int64_t)d : -1;
+  if (HUB == HUB_PREFIX) return (int64_t)d < a.hub_k2 ? (int64_t)d : -1;
   if (HUB == HUB_BITMAP) {
     uint32_t w = a.hub_bits[d >> 5];
     uint32_t bit = 1u << (d & 31);
     if (!(w & bit)) return -1;
-    return (int64_t)a.hub_rank[d >> 5] + __popc(w & (bit - 1));
+    return (int64_t)a.hub_slot_of[a.hub_rank[d >> 5] + __popc(w & (bit - 1))];
   }
   return -1;
 }
@@ -104,28 +148,42 @@ __device__ __forceinline__ uint32_t hub_vertex(const EdgeArgs& a, int64_t slot) 
 // One edge: the cond / hub / atomic path.  Returns whether d joins the next
 // frontier through a global update.
 template <class Op, int HUB>
-__device__ __forceinline__ bool do_edge(Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub,
-                                        uint32_t s, typename Op::SrcT sv, uint32_t d) {
+__device__ __forceinline__ bool do_edge(const Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub,
+                                        typename Op::HubT* l2_hub, uint32_t s, typename Op::SrcT sv,
+                                        uint32_t d) {
   if (!op.cond(d)) return false;
   if (HUB != HUB_NONE) {
     int64_t slot = hub_slot<HUB>(a, d);
-    if (slot >= 0) {
+    if (slot >= 0 && slot < a.hub_k) {
       op.hub_update(&lds_hub[slot], s, sv, d);
+      return false;
+    }
+    if (Op::kL2 && slot >= 0) {
+      op.l2_update(&l2_hub[slot], sv);
       return false;
     }
   }
   return op.update(s, sv, d);
 }
 
+template <class Op, int HUB>
+__device__ __forceinline__ typename Op::HubT* l2_copy(const EdgeArgs& a) {
+  if (!Op::kL2 || HUB == HUB_NONE || a.hub_k2 <= a.hub_k) return nullptr;
+  return reinterpret_cast<typename Op::HubT*>(a.l2_copies) + (int64_t)(xcd_id() % a.l2_ncopies) * a.hub_k2;
+}
+
 template <class Op, int HUB, int BLOCK>
-__device__ __forceinline__ void hub_init(Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub) {
+__device__ __forceinline__ void hub_init(const Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub) {
   if (HUB == HUB_NONE) return;
   for (int64_t i = threadIdx.x; i < a.hub_k; i += BLOCK) lds_hub[i] = op.hub_identity();
   __syncthreads();
 }
 
+// Merges the workgroup's LDS copy: into the XCD's L2 copy when there is one
+// (the per-XCD merge kernel then covers every slot), else into global state.
 template <class Op, int HUB, int BLOCK>
-__device__ __forceinline__ void hub_merge(Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub) {
+__device__ __forceinline__ void hub_merge(const Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub,
+                                          typename Op::HubT* l2_hub, WaveQueue& q) {
   if (HUB == HUB_NONE) return;
   __syncthreads();
   const typename Op::HubT ident = op.hub_identity();
@@ -136,15 +194,48 @@ __device__ __forceinline__ void hub_merge(Op& op, const EdgeArgs& a, typename Op
     if (slot < a.hub_k) {
       typename Op::HubT v = lds_hub[slot];
       if (!(v == ident)) {
-        d = hub_vertex<HUB>(a, slot);
-        act = op.hub_flush(d, v);
+        if (Op::kL2 && l2_hub) {
+          op.l2_update(&l2_hub[slot], v);
+        } else {
+          d = hub_vertex<HUB>(a, slot);
+          act = op.hub_flush(d, v);
+        }
       }
     }
-    if (Op::kOutput) wave_append(a, act, d);
+    if (Op::kOutput) wq_push(q, a, act, d);
   }
 }
 
-// Light kernel.  Grid-strides over chunks of 64 entries, one chunk per wave.
+// Processes heavy items [first, end) with stride `step`, one per wave.
+template <class Op, int HUB>
+__device__ __forceinline__ void heavy_items(const Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub,
+                                            typename Op::HubT* l2_hub, WaveQueue& q,
+                                            const uint64_t* items, int64_t first, int64_t end,
+                                            int64_t step) {
+  const int lane = (int)__lane_id();
+  for (int64_t it = first; it < end; it += step) {
+    uint64_t item = items[it];
+    uint32_t v = (uint32_t)(item >> 32);
+    uint64_t k = item & 0xffffffffu;
+    uint64_t beg = a.off[v] + k * kHeavyChunk;
+    uint64_t stop = a.off[v + 1];
+    if (stop > beg + kHeavyChunk) stop = beg + kHeavyChunk;
+    typename Op::SrcT sv = op.load_src(v);
+    for (uint64_t e0 = beg; e0 < stop; e0 += kWave) {
+      uint64_t e = e0 + lane;
+      bool act = false;
+      uint32_t d = 0;
+      if (e < stop) {
+        d = a.nbr[e];
+        act = do_edge<Op, HUB>(op, a, lds_hub, l2_hub, v, sv, d);
+      }
+      if (Op::kOutput) wq_push(q, a, act, d);
+    }
+  }
+}
+
+// Light (+ static heavy) kernel.  Grid-strides over chunks of 64 entries,
+// one chunk per wave.
 template <class Op, int MODE, int HUB, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
   constexpr int WPB = BLOCK / kWave;
@@ -152,16 +243,20 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
   __shared__ uint64_t s_start[WPB][kWave];
   __shared__ uint32_t s_src[WPB][kWave];
   __shared__ typename Op::SrcT s_val[WPB][kWave];
+  __shared__ uint32_t s_q[Op::kOutput ? WPB : 1][Op::kOutput ? kQueue : 1];
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   typename Op::HubT* lds_hub = reinterpret_cast<typename Op::HubT*>(dyn_lds);
   if (op.skip()) return;
   hub_init<Op, HUB, BLOCK>(op, a, lds_hub);
+  typename Op::HubT* l2_hub = l2_copy<Op, HUB>(a);
 
   const int lane = (int)__lane_id();
   const int w = threadIdx.x / kWave;
+  WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0};
   const int64_t nchunks = (a.count + kWave - 1) / kWave;
   const int64_t wave_id = (int64_t)blockIdx.x * WPB + w;
   const int64_t nwaves = (int64_t)gridDim.x * WPB;
+  const bool static_heavy = a.static_heavy != nullptr;
 
   for (int64_t c = wave_id; c < nchunks; c += nwaves) {
     int64_t i = c * kWave + lane;
@@ -178,9 +273,11 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
       start = a.off[v];
       uint64_t dd = a.off[v + 1] - start;
       if (dd > (uint64_t)a.heavy_deg) {
-        int64_t nch = (int64_t)((dd + kHeavyChunk - 1) / kHeavyChunk);
-        int64_t base = atomicAdd((unsigned long long*)a.heavy_count, (unsigned long long)nch);
-        for (int64_t k = 0; k < nch; ++k) a.heavy[base + k] = ((uint64_t)v << 32) | (uint64_t)k;
+        if (!static_heavy) {
+          int64_t nch = (int64_t)((dd + kHeavyChunk - 1) / kHeavyChunk);
+          int64_t base = atomicAdd((unsigned long long*)a.heavy_count, (unsigned long long)nch);
+          for (int64_t k = 0; k < nch; ++k) a.heavy[base + k] = ((uint64_t)v << 32) | (uint64_t)k;
+        }
         deg = 0;
       } else {
         deg = (uint32_t)dd;
@@ -195,66 +292,91 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
     uint32_t total = __shfl(inc, kWave - 1);
     if (total == 0) continue;
     s_pre[w][lane] = inc - deg;
-    if (lane == 0) s_pre[w][kWave] = total;
     s_start[w][lane] = start;
     s_src[w][lane] = v;
     if (deg) s_val[w][lane] = op.load_src(v);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    wave_sync_lds();
     for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
       uint32_t t = t0 + lane;
-      bool valid = t < total;
       bool act = false;
       uint32_t d = 0;
-      if (valid) {
+      if (t < total) {
         int j = 0;
 #pragma unroll
         for (int step = 32; step > 0; step >>= 1)
           if (s_pre[w][j + step] <= t) j += step;
         uint64_t e = s_start[w][j] + (t - s_pre[w][j]);
         d = a.nbr[e];
-        act = do_edge<Op, HUB>(op, a, lds_hub, s_src[w][j], s_val[w][j], d);
+        act = do_edge<Op, HUB>(op, a, lds_hub, l2_hub, s_src[w][j], s_val[w][j], d);
       }
-      if (Op::kOutput) wave_append(a, act, d);
+      if (Op::kOutput) wq_push(q, a, act, d);
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    wave_sync_lds();
   }
-  hub_merge<Op, HUB, BLOCK>(op, a, lds_hub);
+  if (static_heavy)
+    heavy_items<Op, HUB>(op, a, lds_hub, l2_hub, q, a.static_heavy, wave_id, a.static_heavy_count,
+                         nwaves);
+  hub_merge<Op, HUB, BLOCK>(op, a, lds_hub, l2_hub, q);
+  if (Op::kOutput) wq_finish(q, a);
 }
 
-// Heavy kernel: one wavefront per (vertex, chunk) item of the heavy queue.
+// Heavy kernel: drains the dynamic heavy queue, one wavefront per item.
 template <class Op, int HUB, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void edge_heavy_kernel(Op op, EdgeArgs a) {
   constexpr int WPB = BLOCK / kWave;
+  __shared__ uint32_t s_q[Op::kOutput ? WPB : 1][Op::kOutput ? kQueue : 1];
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   typename Op::HubT* lds_hub = reinterpret_cast<typename Op::HubT*>(dyn_lds);
   if (op.skip()) return;
-  hub_init<Op, HUB, BLOCK>(op, a, lds_hub);
-  const int lane = (int)__lane_id();
   const int64_t items = *a.heavy_count;
-  const int64_t wave_id = (int64_t)blockIdx.x * WPB + threadIdx.x / kWave;
+  if (items == 0) return;
+  hub_init<Op, HUB, BLOCK>(op, a, lds_hub);
+  typename Op::HubT* l2_hub = l2_copy<Op, HUB>(a);
+  const int w = threadIdx.x / kWave;
+  WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0};
+  const int64_t wave_id = (int64_t)blockIdx.x * WPB + w;
   const int64_t nwaves = (int64_t)gridDim.x * WPB;
-  for (int64_t it = wave_id; it < items; it += nwaves) {
-    uint64_t item = a.heavy[it];
-    uint32_t v = (uint32_t)(item >> 32);
-    uint64_t k = item & 0xffffffffu;
-    uint64_t beg = a.off[v] + k * kHeavyChunk;
-    uint64_t end = a.off[v + 1];
-    if (end > beg + kHeavyChunk) end = beg + kHeavyChunk;
-    typename Op::SrcT sv = op.load_src(v);
-    for (uint64_t e0 = beg; e0 < end; e0 += kWave) {
-      uint64_t e = e0 + lane;
-      bool act = false;
-      uint32_t d = 0;
-      if (e < end) {
-        d = a.nbr[e];
-        act = do_edge<Op, HUB>(op, a, lds_hub, v, sv, d);
+  heavy_items<Op, HUB>(op, a, lds_hub, l2_hub, q, a.heavy, wave_id, items, nwaves);
+  hub_merge<Op, HUB, BLOCK>(op, a, lds_hub, l2_hub, q);
+  if (Op::kOutput) wq_finish(q, a);
+}
+
+// Merges the per-XCD copies of slots [0, hub_k2) into global state and
+// clears them (operators with kL2; runs after the edge launch).
+template <class Op, int HUB>
+__global__ void k_l2_merge(Op op, EdgeArgs a) {
+  if (op.skip()) return;
+  typename Op::HubT* base = reinterpret_cast<typename Op::HubT*>(a.l2_copies);
+  const typename Op::HubT ident = op.hub_identity();
+  for (int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; slot < a.hub_k2;
+       slot += (int64_t)gridDim.x * blockDim.x) {
+    typename Op::HubT s = ident;
+    bool any = false;
+    for (int x = 0; x < a.l2_ncopies; ++x) {
+      typename Op::HubT v = base[(int64_t)x * a.hub_k2 + slot];
+      if (!(v == ident)) {
+        s = any ? op.hub_combine(s, v) : v;
+        any = true;
+        base[(int64_t)x * a.hub_k2 + slot] = ident;
       }
-      if (Op::kOutput) wave_append(a, act, d);
+    }
+    if (any) op.hub_flush(hub_vertex<HUB>(a, slot), s);
+  }
+}
+
+// Static heavy list for the all-vertices frontier: every vertex with degree
+// > heavy_deg contributes ceil(deg / kHeavyChunk) items.
+__global__ void k_heavy_list(const uint64_t* off, int64_t n, int heavy_deg, uint64_t* items,
+                             int64_t* count) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t dd = off[v + 1] - off[v];
+    if (dd > (uint64_t)heavy_deg) {
+      int64_t nch = (int64_t)((dd + kHeavyChunk - 1) / kHeavyChunk);
+      int64_t base = atomicAdd((unsigned long long*)count, (unsigned long long)nch);
+      for (int64_t k = 0; k < nch; ++k) items[base + k] = ((uint64_t)v << 32) | (uint64_t)k;
     }
   }
-  hub_merge<Op, HUB, BLOCK>(op, a, lds_hub);
 }
 
 }  // namespace rg

@@ -110,9 +110,35 @@ int rg_ctx_destroy(rg_ctx* ctx) {
   ctx->fr_ids[0].release();
   ctx->fr_ids[1].release();
   ctx->fr_dense.release();
+  for (auto& p : ctx->ev_pairs) {
+    (void)hipEventDestroy(p.e[0]);
+    (void)hipEventDestroy(p.e[1]);
+  }
   if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  return RG_OK;
+}
+
+int rg_ctx_timing(rg_ctx* ctx, int enable) {
+  if (!ctx) return set_error(RG_EINVAL, "rg_ctx_timing: null");
+  RG_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->timing = enable != 0;
+  ctx->ev_used = 0;
+  return RG_OK;
+}
+
+int rg_ctx_kernel_time(rg_ctx* ctx, int64_t* launches, double* total_ms) {
+  if (!ctx) return set_error(RG_EINVAL, "rg_ctx_kernel_time: null");
+  RG_HIP(hipStreamSynchronize(ctx->stream));
+  double t = 0.0;
+  for (size_t i = 0; i < ctx->ev_used; ++i) {
+    float ms = 0.f;
+    RG_HIP(hipEventElapsedTime(&ms, ctx->ev_pairs[i].e[0], ctx->ev_pairs[i].e[1]));
+    t += ms;
+  }
+  if (launches) *launches = (int64_t)ctx->ev_used;
+  if (total_ms) *total_ms = t;
   return RG_OK;
 }
 
@@ -198,6 +224,8 @@ int rg_graph_free(rg_graph* g) {
   if (g->hub_bits) (void)hipFree(g->hub_bits);
   if (g->hub_rank) (void)hipFree(g->hub_rank);
   if (g->hub_ids) (void)hipFree(g->hub_ids);
+  if (g->hub_slot_of) (void)hipFree(g->hub_slot_of);
+  if (g->heavy_items) (void)hipFree(g->heavy_items);
   delete g;
   return RG_OK;
 }

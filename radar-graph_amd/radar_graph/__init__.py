@@ -50,7 +50,8 @@ class Opts(C.Structure):
     """rg_opts: variant, hub placement, hub count k, sparse/dense threshold,
     workgroup size and the wave-per-vertex cut-over degree."""
     _fields_ = [("variant", C.c_int), ("hub_placement", C.c_int), ("hubs", C.c_int64),
-                ("threshold", C.c_int64), ("wg_size", C.c_int), ("heavy_degree", C.c_int)]
+                ("threshold", C.c_int64), ("wg_size", C.c_int), ("heavy_degree", C.c_int),
+                ("l2_hubs", C.c_int64)]
 
 
 def _sig(name, res, *args):
@@ -80,6 +81,8 @@ _sig("rg_device_count", C.c_int, C.POINTER(C.c_int))
 _sig("rg_ctx_create", C.c_int, C.c_int, C.POINTER(_vp))
 _sig("rg_ctx_destroy", C.c_int, _vp)
 _sig("rg_ctx_sync", C.c_int, _vp)
+_sig("rg_ctx_timing", C.c_int, _vp, C.c_int)
+_sig("rg_ctx_kernel_time", C.c_int, _vp, _i64p, _f64p)
 _sig("rg_graph_upload", C.c_int, _vp, _vp, C.POINTER(_vp))
 _sig("rg_graph_info", C.c_int, _vp, _i64p, _i64p, C.POINTER(C.c_int))
 _sig("rg_graph_download", C.c_int, _vp, C.c_int, _u64p, _u32p)
@@ -117,7 +120,8 @@ def _ptr(a: np.ndarray, t):
     return a.ctypes.data_as(t)
 
 
-def opts(variant="push", hubs=0, threshold=-1, wg_size=0, heavy_degree=0, hub_placement=HUBS_LDS) -> Opts:
+def opts(variant="push", hubs=0, threshold=-1, wg_size=0, heavy_degree=0, hub_placement=HUBS_LDS,
+         l2_hubs=0) -> Opts:
     o = Opts()
     lib.rg_opts_default(C.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -126,6 +130,7 @@ def opts(variant="push", hubs=0, threshold=-1, wg_size=0, heavy_degree=0, hub_pl
     o.wg_size = int(wg_size)
     o.heavy_degree = int(heavy_degree)
     o.hub_placement = int(hub_placement)
+    o.l2_hubs = int(l2_hubs)
     return o
 
 
@@ -256,6 +261,16 @@ class Context:
 
     def sync(self):
         _check(lib.rg_ctx_sync(self._h))
+
+    def timing(self, enable=True):
+        """Bracket every edge-map launch with HIP events (resets the record)."""
+        _check(lib.rg_ctx_timing(self._h, int(enable)))
+
+    def kernel_time(self):
+        """(timed edge-map launches, their summed device ms) since timing()."""
+        n, ms = C.c_int64(), C.c_double()
+        _check(lib.rg_ctx_kernel_time(self._h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     def close(self):
         if self._h:

@@ -142,10 +142,16 @@ int rg_graph_degree_sort(rg_graph* g, int by_outdeg, int sort_neighbors,
  *   RG_HUBDUP   push with private hub copies + merge (HD), hubs = top-k
  *               in-degree vertices found by a degree threshold
  *   RG_RADAR    HUBDUP with hubs = ids 0..k-1 (RD, expects a degree-sorted graph)
+ *   RG_BLOCKED  PageRank only: the all-vertices push without global atomics --
+ *               destinations in LDS-resident blocks (hub blocks split over
+ *               several private copies merged by the vertex phase), in-edges
+ *               ordered by source segment so reads stay in each XCD's L2.
+ *               On a degree-sorted graph this is RADAR's reorder + duplicate
+ *               scheme mapped onto LDS and L2.
  * Hub placement: RG_HUBS_LDS (one copy per workgroup in LDS) or
  * RG_HUBS_L2 (one copy per XCD in global memory, L2-resident).
  */
-enum { RG_PUSH = 0, RG_PULL = 1, RG_HUBDUP = 2, RG_RADAR = 3 };
+enum { RG_PUSH = 0, RG_PULL = 1, RG_HUBDUP = 2, RG_RADAR = 3, RG_BLOCKED = 4 };
 enum { RG_HUBS_LDS = 0, RG_HUBS_L2 = 1 };
 
 typedef struct rg_opts {
@@ -158,6 +164,11 @@ typedef struct rg_opts {
   int64_t l2_hubs;      /* HUBDUP/RADAR FP64 accumulators: hub slots [hubs,    */
                         /* l2_hubs) get one copy per XCD (L2-resident); 0 =    */
                         /* half an XCD L2 (256K), < 0 = off                    */
+  int dst_block_bits;   /* RG_BLOCKED: 2^bits destinations per LDS block       */
+                        /* (10..14; 0 = 14)                                    */
+  int src_segment_bits; /* RG_BLOCKED: 2^bits sources per L2 segment (1..24;   */
+                        /* 0 = order by source id)                             */
+  int unroll;           /* RG_BLOCKED: edges in flight per lane (1 or 4)       */
 } rg_opts;
 
 /* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */

@@ -233,20 +233,6 @@ struct Engine {
   int64_t* heavy_count;
 };
 
-// With timing on, records the start event of one edge-map launch on the
-// context stream and returns the event pair (the caller records [1]).
-static hipEvent_t* timing_begin(rg_ctx* ctx) {
-  if (!ctx->timing) return nullptr;
-  if (ctx->ev_used == ctx->ev_pairs.size()) {
-    rg_ctx::EvPair p;
-    if (hipEventCreate(&p.e[0]) != hipSuccess || hipEventCreate(&p.e[1]) != hipSuccess) return nullptr;
-    ctx->ev_pairs.push_back(p);
-  }
-  auto& p = ctx->ev_pairs[ctx->ev_used++];
-  if (hipEventRecord(p.e[0], ctx->stream) != hipSuccess) return nullptr;
-  return p.e;
-}
-
 template <class Op, int MODE, int HUB, int BLOCK>
 static int launch_edge(const Engine& E, const Op& op, const EdgeArgs& a, size_t hub_bytes) {
   constexpr int WPB = BLOCK / kWave;
@@ -615,6 +601,8 @@ int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks
                   int64_t* iters_out, double* l1_out, double* ms_out);
 int pagerank_pull(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
                   int64_t* iters_out, double* l1_out, double* ms_out);
+int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
+                     int64_t* iters_out, double* l1_out, double* ms_out);
 
 int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
                   int64_t* iters_out, double* l1_out, double* ms_out) {
@@ -1063,6 +1051,8 @@ int rg_pagerank(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_o
   RG_HIP(hipSetDevice(g->ctx->device));
   if (o && o->variant == RG_PULL)
     return rg::pagerank_pull(g, maxiters, o, ranks_out, iters_out, l1_out, ms_out);
+  if (o && o->variant == RG_BLOCKED)
+    return rg::pagerank_blocked(g, maxiters, o, ranks_out, iters_out, l1_out, ms_out);
   return rg::pagerank_push(g, maxiters, o, ranks_out, iters_out, l1_out, ms_out);
 }
 

@@ -1,0 +1,368 @@
+// Blocked PageRank engine: the all-vertices push of PageRank.C:122
+// (p_next[d] += p_curr[s]/outdeg(s) over every arc) reorganised for MI355X
+// memory, with no global atomics.
+//
+// On this part every global atomic is performed at the memory side (the L2
+// forwards it), so a push that scatters FP64 atomics over an n-vertex
+// accumulator is bound by memory-side atomic throughput, and a gather over
+// in-edges by 128-byte line fetches for 8-byte reads.  The blocked layout
+// keeps both sides on chip:
+//   * destinations are cut into blocks of 2^kVB vertices whose FP64
+//     accumulators fit in one workgroup's LDS; a workgroup ("unit") owns an
+//     edge range of one block and accumulates with LDS atomics;
+//   * a block with more than kUnitEdges in-edges (the hub blocks of a
+//     degree-sorted graph) is split over several units, each with its own
+//     private LDS copy, merged by the vertex phase -- the reference's hub
+//     duplication (hubdup PageRank.C:45-73) at workgroup granularity;
+//   * within a block, edges are ordered by source segment (2^kSB sources),
+//     so the contrib[s] reads of the units running at the same time fall in
+//     few segments and hit in each XCD's L2, which keeps its own copy.
+// Per arc the kernel streams a 4-byte source id and a 2-byte local
+// destination; per unit it writes its 2^kVB partial sums.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
+#include "device.h"
+
+namespace rg {
+
+constexpr int kMaxVB = 14;                       // at most 16K destinations per block (128 KB LDS)
+constexpr int64_t kUnitEdges = int64_t(1) << 19; // edges per unit (work split)
+constexpr int kUnitThreads = 1024;
+
+struct BlockedCSC {
+  const uint64_t* built_for = nullptr;   // in_off of the orientation it was built from
+  int vb = 0, sb = 0;                    // destination-block / source-segment bits
+  int64_t nblocks = 0, nunits = 0;
+  uint32_t* src = nullptr;        // m, sorted by (block, segment, local dst)
+  uint16_t* vl = nullptr;         // m, local destination
+  uint64_t* unit_lo = nullptr;    // nunits + 1 edge boundaries
+  uint32_t* block_unit0 = nullptr;// nblocks + 1: first unit of each block
+  double* part = nullptr;         // nunits * kBlockV partial sums
+  void release() {
+    if (src) (void)hipFree(src);
+    if (vl) (void)hipFree(vl);
+    if (unit_lo) (void)hipFree(unit_lo);
+    if (block_unit0) (void)hipFree(block_unit0);
+    if (part) (void)hipFree(part);
+    src = nullptr; vl = nullptr; unit_lo = nullptr; block_unit0 = nullptr; part = nullptr;
+    built_for = nullptr;
+  }
+};
+
+void blocked_free(void* p) {
+  BlockedCSC* b = static_cast<BlockedCSC*>(p);
+  if (b) {
+    b->release();
+    delete b;
+  }
+}
+
+namespace {
+
+// One wavefront per destination: key = block << bs | segment << vb | local,
+// with bs = vb + (32 - sb) so the segment field (32 - sb bits) never
+// overlaps the block field; the key needs 64 - sb bits.
+__global__ void k_blk_keys(const uint64_t* in_off, const uint32_t* in_nbr, int64_t n, int vb, int sb,
+                           int bs, uint64_t* keys, uint32_t* vals) {
+  const int lane = (int)__lane_id();
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  const int64_t lmask = (int64_t(1) << vb) - 1;
+  for (int64_t v = wid; v < n; v += nw) {
+    const uint64_t b = in_off[v], e = in_off[v + 1];
+    const uint64_t hi = ((uint64_t)(v >> vb) << bs) | (uint64_t)(v & lmask);
+    for (uint64_t j = b + lane; j < e; j += kWave) {
+      uint32_t s = in_nbr[j];
+      keys[j] = hi | ((uint64_t)(s >> sb) << vb);
+      vals[j] = s;
+    }
+  }
+}
+
+__global__ void k_blk_split(const uint64_t* keys, int64_t m, int vb, int bs, uint16_t* vl, uint64_t* block_lo) {
+  const uint64_t lmask = (uint64_t(1) << vb) - 1;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    vl[i] = (uint16_t)(k & lmask);
+    uint64_t blk = k >> bs;
+    if (i == 0 || (keys[i - 1] >> bs) != blk) block_lo[blk] = (uint64_t)i;
+  }
+}
+
+// One workgroup per unit: LDS accumulators for the unit's block, streamed
+// (source, local destination) pairs, partial sums written out at the end.
+// The two streams are read once (nontemporal loads, so they do not evict
+// the L2-resident source segment); four edges per lane are in flight.
+template <bool NT, class T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
+template <int VB, int THREADS, bool NT, int UNROLL>
+__global__ __launch_bounds__(THREADS) void k_blk_pr(const uint32_t* __restrict__ src,
+                                                    const uint16_t* __restrict__ vl,
+                                                    const uint64_t* __restrict__ unit_lo,
+                                                    const double* __restrict__ contrib,
+                                                    double* __restrict__ part, const int64_t* done) {
+  constexpr int BV = 1 << VB;
+  __shared__ double acc[BV];
+  if (*done) return;
+  const int64_t u = blockIdx.x;
+  for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
+  __syncthreads();
+  const uint64_t lo = unit_lo[u], hi = unit_lo[u + 1];
+  uint64_t e = lo + threadIdx.x;
+  if (UNROLL == 4) {
+    for (; e + 3 * THREADS < hi; e += 4 * THREADS) {
+      const uint32_t s0 = ld_stream<NT>(src + e);
+      const uint32_t s1 = ld_stream<NT>(src + e + THREADS);
+      const uint32_t s2 = ld_stream<NT>(src + e + 2 * THREADS);
+      const uint32_t s3 = ld_stream<NT>(src + e + 3 * THREADS);
+      const uint32_t l0 = ld_stream<NT>(vl + e);
+      const uint32_t l1 = ld_stream<NT>(vl + e + THREADS);
+      const uint32_t l2 = ld_stream<NT>(vl + e + 2 * THREADS);
+      const uint32_t l3 = ld_stream<NT>(vl + e + 3 * THREADS);
+      const double c0 = contrib[s0], c1 = contrib[s1], c2 = contrib[s2], c3 = contrib[s3];
+      atomicAdd(&acc[l0], c0);
+      atomicAdd(&acc[l1], c1);
+      atomicAdd(&acc[l2], c2);
+      atomicAdd(&acc[l3], c3);
+    }
+  }
+  for (; e < hi; e += THREADS) {
+    const uint32_t s = ld_stream<NT>(src + e);
+    const uint32_t l = ld_stream<NT>(vl + e);
+    atomicAdd(&acc[l], contrib[s]);
+  }
+  __syncthreads();
+  double* out = part + u * (int64_t)BV;
+  for (int i = threadIdx.x; i < BV; i += THREADS) out[i] = acc[i];
+}
+
+int grid_n(rg_ctx* ctx, int64_t n) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 16));
+}
+
+}  // namespace
+
+// Builds (or returns the cached) blocked layout of g's in-edges with
+// destination blocks of 2^vb vertices and source segments of 2^sb.
+int blocked_build(rg_graph* g, int vb, int sb, BlockedCSC** out) {
+  BlockedCSC* B = static_cast<BlockedCSC*>(g->blocked);
+  if (B && B->built_for == g->in_off && B->vb == vb && B->sb == sb) {
+    *out = B;
+    return RG_OK;
+  }
+  if (g->m > (int64_t)INT32_MAX) return set_error(RG_EUNSUPPORTED, "blocked PageRank: more than 2^31 arcs");
+  rg_ctx* ctx = g->ctx;
+  hipStream_t s = ctx->stream;
+  if (!B) {
+    B = new BlockedCSC;
+    g->blocked = B;
+  }
+  B->release();
+  const int64_t n = g->n, m = g->m;
+  const int64_t nblocks = (n + (int64_t(1) << vb) - 1) >> vb;
+  DevBuf keys, skeys, vals, tmp, blo;
+  RG_TRY(keys.ensure(sizeof(uint64_t) * std::max<int64_t>(m, 1)));
+  RG_TRY(skeys.ensure(sizeof(uint64_t) * std::max<int64_t>(m, 1)));
+  RG_TRY(vals.ensure(sizeof(uint32_t) * std::max<int64_t>(m, 1)));
+  RG_HIP(hipMalloc(&B->src, sizeof(uint32_t) * std::max<int64_t>(m, 1)));
+  RG_HIP(hipMalloc(&B->vl, sizeof(uint16_t) * std::max<int64_t>(m, 1)));
+  RG_TRY(blo.ensure(sizeof(uint64_t) * (nblocks + 1)));
+  RG_HIP(hipMemsetAsync(blo.p, 0xff, sizeof(uint64_t) * (nblocks + 1), s));
+  const int bs = vb + (32 - sb);
+  hipLaunchKernelGGL(k_blk_keys, grid_n(ctx, n), 256, 0, s, g->in_off, g->in_nbr, n, vb, sb, bs,
+                     keys.as<uint64_t>(), vals.as<uint32_t>());
+  const int end_bit = bs + std::max(1, (int)(64 - __builtin_clzll((unsigned long long)nblocks)));
+  size_t tb = 0;
+  RG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.as<uint64_t>(), skeys.as<uint64_t>(),
+                                            vals.as<uint32_t>(), B->src, (int)m, 0, end_bit, s));
+  RG_TRY(tmp.ensure(tb));
+  RG_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, keys.as<uint64_t>(), skeys.as<uint64_t>(),
+                                            vals.as<uint32_t>(), B->src, (int)m, 0, end_bit, s));
+  hipLaunchKernelGGL(k_blk_split, grid_n(ctx, m), 256, 0, s, skeys.as<uint64_t>(), m, vb, bs, B->vl,
+                     blo.as<uint64_t>());
+  RG_HIP(hipGetLastError());
+  std::vector<uint64_t> block_lo(nblocks + 1);
+  RG_HIP(hipMemcpyAsync(block_lo.data(), blo.p, sizeof(uint64_t) * (nblocks + 1), hipMemcpyDeviceToHost, s));
+  RG_HIP(hipStreamSynchronize(s));
+  keys.release();
+  skeys.release();
+  vals.release();
+  tmp.release();
+  // Blocks without edges start where the next non-empty block starts.
+  block_lo[nblocks] = (uint64_t)m;
+  for (int64_t b = nblocks - 1; b >= 0; --b)
+    if (block_lo[b] == ~0ull) block_lo[b] = block_lo[b + 1];
+  std::vector<uint64_t> ulo;
+  std::vector<uint32_t> bu0(nblocks + 1);
+  for (int64_t b = 0; b < nblocks; ++b) {
+    bu0[b] = (uint32_t)ulo.size();
+    uint64_t lo = block_lo[b], hi = block_lo[b + 1];
+    if (hi == lo) {
+      ulo.push_back(lo);   // an empty unit keeps every block's partials defined
+      continue;
+    }
+    for (uint64_t e = lo; e < hi; e += (uint64_t)kUnitEdges) ulo.push_back(e);
+  }
+  bu0[nblocks] = (uint32_t)ulo.size();
+  B->nunits = (int64_t)ulo.size();
+  ulo.push_back((uint64_t)m);
+  B->nblocks = nblocks;
+  RG_HIP(hipMalloc(&B->unit_lo, sizeof(uint64_t) * ulo.size()));
+  RG_HIP(hipMalloc(&B->block_unit0, sizeof(uint32_t) * bu0.size()));
+  RG_HIP(hipMalloc(&B->part, sizeof(double) * ((size_t)B->nunits << vb)));
+  RG_HIP(hipMemcpy(B->unit_lo, ulo.data(), sizeof(uint64_t) * ulo.size(), hipMemcpyHostToDevice));
+  RG_HIP(hipMemcpy(B->block_unit0, bu0.data(), sizeof(uint32_t) * bu0.size(), hipMemcpyHostToDevice));
+  B->built_for = g->in_off;
+  B->vb = vb;
+  B->sb = sb;
+  *out = B;
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PageRank on the blocked layout.  The vertex phase sums the partials of
+// every unit of v's block in unit order (deterministic), then applies
+// PR_Vertex_F / the L1 term / the reset as the push path does.
+
+namespace {
+
+__global__ void k_pr_init_b(const uint64_t* off, int64_t n, double* rank, double* contrib, double one_over_n) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t deg = off[v + 1] - off[v];
+    rank[v] = one_over_n;
+    contrib[v] = deg ? one_over_n / (double)(uint32_t)deg : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pr_vertex_b(const uint64_t* off, int64_t n, double* rank,
+                                                     double* contrib, const double* part,
+                                                     const uint32_t* block_unit0, int vb, double damping,
+                                                     double added, double* partial, const int64_t* done) {
+  if (*done) return;
+  double diff = 0.0;
+  const int64_t lmask = (int64_t(1) << vb) - 1;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = v >> vb;
+    const int64_t l = v & lmask;
+    double a = 0.0;
+    for (uint32_t u = block_unit0[b]; u < block_unit0[b + 1]; ++u) a += part[((int64_t)u << vb) + l];
+    double nw = __dadd_rn(__dmul_rn(damping, a), added);
+    diff += fabs(rank[v] - nw);
+    rank[v] = nw;
+    uint64_t deg = off[v + 1] - off[v];
+    contrib[v] = deg ? nw / (double)(uint32_t)deg : 0.0;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = diff;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_pr_finish_b(const double* partial, int nparts, double* l1_hist, int64_t iter,
+                              int64_t* done, double epsilon) {
+  if (*done) return;
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    l1_hist[iter - 1] = red[0];
+    if (red[0] < epsilon) *done = iter;
+  }
+}
+
+}  // namespace
+
+int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
+                     int64_t* iters_out, double* l1_out, double* ms_out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  const int vb = (o && o->dst_block_bits >= 10 && o->dst_block_bits <= kMaxVB) ? o->dst_block_bits : kMaxVB;
+  // src_segment_bits: 1..24 -> 2^bits sources per segment; 0 (default) or
+  // < 0 = order by source id itself (measured best: units then sweep the
+  // source range in step and their reads share L2 lines).
+  const int sb = (o && o->src_segment_bits > 0 && o->src_segment_bits <= 24) ? o->src_segment_bits : 0;
+  const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
+  BlockedCSC* B = nullptr;
+  RG_TRY(blocked_build(g, vb, sb, &B));
+  void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const double*, double*, const int64_t*);
+  const bool u4 = o && o->unroll == 4;
+#define RG_PICK(VB)                                                                   \
+  kern = threads == 512 ? (u4 ? k_blk_pr<VB, 512, true, 4> : k_blk_pr<VB, 512, true, 1>) \
+                        : (u4 ? k_blk_pr<VB, 1024, true, 4> : k_blk_pr<VB, 1024, true, 1>)
+  if (vb == 14) RG_PICK(14);
+  else if (vb == 13) RG_PICK(13);
+  else if (vb == 12) RG_PICK(12);
+  else if (vb == 11) RG_PICK(11);
+  else RG_PICK(10);
+#undef RG_PICK
+  DevBuf &b_rank = ctx->scratch[0], &b_contrib = ctx->scratch[1], &b_part = ctx->scratch[3];
+  RG_TRY(b_rank.ensure(sizeof(double) * n));
+  RG_TRY(b_contrib.ensure(sizeof(double) * n));
+  const int vgrid = std::min<int>(grid_n(ctx, n), 1024);
+  RG_TRY(b_part.ensure(sizeof(double) * (vgrid + std::max<int64_t>(maxiters, 1) + 8)));
+  double* rank = b_rank.as<double>();
+  double* contrib = b_contrib.as<double>();
+  double* partial = b_part.as<double>();
+  double* l1_hist = partial + vgrid;
+  int64_t* done = ctx->counts.as<int64_t>() + 16;
+  RG_HIP(hipMemsetAsync(done, 0, sizeof(int64_t), ctx->stream));
+  const double damping = 0.85, epsilon = 0.0000001;
+  const double one_over_n = 1 / (double)n;
+  const double added = (1 - damping) * (1 / (double)n);
+  hipLaunchKernelGGL(k_pr_init_b, grid_n(ctx, n), 256, 0, ctx->stream, g->off, n, rank, contrib, one_over_n);
+  RG_HIP(hipGetLastError());
+  hipEvent_t t0, t1;
+  RG_HIP(hipEventCreate(&t0));
+  RG_HIP(hipEventCreate(&t1));
+  RG_HIP(hipEventRecord(t0, ctx->stream));
+  for (int64_t it = 1; it <= maxiters; ++it) {
+    hipEvent_t* ev = timing_begin(ctx);
+    hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, contrib,
+                       B->part, done);
+    if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
+    hipLaunchKernelGGL(k_pr_vertex_b, vgrid, 256, 0, ctx->stream, g->off, n, rank, contrib, B->part,
+                       B->block_unit0, vb, damping, added, partial, done);
+    hipLaunchKernelGGL(k_pr_finish_b, 1, 256, 0, ctx->stream, partial, vgrid, l1_hist, it, done, epsilon);
+    RG_HIP(hipGetLastError());
+  }
+  RG_HIP(hipEventRecord(t1, ctx->stream));
+  RG_HIP(hipEventSynchronize(t1));
+  float ms = 0.f;
+  RG_HIP(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  int64_t dn = 0;
+  RG_TRY(ctx_read_count(ctx, done, &dn, 1));
+  int64_t last = dn ? dn : maxiters;
+  double l1 = 0.0;
+  if (last >= 1) RG_HIP(hipMemcpy(&l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
+  if (iters_out) *iters_out = dn ? dn : maxiters + 1;
+  if (l1_out) *l1_out = l1;
+  if (ms_out) *ms_out = ms;
+  if (ranks_out) RG_HIP(hipMemcpy(ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+}  // namespace rg

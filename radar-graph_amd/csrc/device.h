@@ -88,6 +88,8 @@ struct rg_graph {
   int64_t heavy_items_count = 0;
   int heavy_items_deg = -1;
   const uint64_t* heavy_items_off = nullptr;
+  // Blocked in-edge layout for the blocked PageRank engine (opaque).
+  void* blocked = nullptr;
 };
 
 namespace rg {
@@ -107,5 +109,13 @@ struct Frontier {
 int ctx_frontiers(rg_ctx* ctx, int64_t n, Frontier& a, Frontier& b);
 
 int ctx_read_count(rg_ctx* ctx, const int64_t* dev, int64_t* host, int nwords = 1);
+
+// With timing on (rg_ctx_timing), records the start event of one edge-map
+// launch on the context stream and returns the event pair; the caller
+// records element [1] after the launch.  nullptr when timing is off.
+hipEvent_t* timing_begin(rg_ctx* ctx);
+
+// Cached blocked layout of a graph's in-edges (blocked.hip).
+void blocked_free(void* p);
 
 }  // namespace rg

@@ -49,6 +49,18 @@ int ctx_frontiers(rg_ctx* ctx, int64_t n, Frontier& a, Frontier& b) {
   return RG_OK;
 }
 
+hipEvent_t* timing_begin(rg_ctx* ctx) {
+  if (!ctx->timing) return nullptr;
+  if (ctx->ev_used == ctx->ev_pairs.size()) {
+    rg_ctx::EvPair p;
+    if (hipEventCreate(&p.e[0]) != hipSuccess || hipEventCreate(&p.e[1]) != hipSuccess) return nullptr;
+    ctx->ev_pairs.push_back(p);
+  }
+  auto& p = ctx->ev_pairs[ctx->ev_used++];
+  if (hipEventRecord(p.e[0], ctx->stream) != hipSuccess) return nullptr;
+  return p.e;
+}
+
 int ctx_read_count(rg_ctx* ctx, const int64_t* dev, int64_t* host, int nwords) {
   RG_HIP(hipMemcpyAsync(ctx->h_counts, dev, sizeof(int64_t) * nwords, hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -226,6 +238,7 @@ int rg_graph_free(rg_graph* g) {
   if (g->hub_ids) (void)hipFree(g->hub_ids);
   if (g->hub_slot_of) (void)hipFree(g->hub_slot_of);
   if (g->heavy_items) (void)hipFree(g->heavy_items);
+  rg::blocked_free(g->blocked);
   delete g;
   return RG_OK;
 }

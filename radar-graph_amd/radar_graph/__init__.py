@@ -27,9 +27,9 @@ lib = C.CDLL(LIB_PATH)
 
 RG_OK, RG_EINVAL, RG_ENOMEM, RG_EDEVICE, RG_EIO, RG_EFORMAT, RG_EUNSUPPORTED = range(7)
 FMT_ADJ, FMT_ADJ_BINARY, FMT_EL, FMT_SG = range(4)
-PUSH, PULL, HUBDUP, RADAR = range(4)
+PUSH, PULL, HUBDUP, RADAR, BLOCKED = range(5)
 HUBS_LDS, HUBS_L2 = range(2)
-VARIANTS = {"push": PUSH, "pull": PULL, "hubdup": HUBDUP, "radar": RADAR}
+VARIANTS = {"push": PUSH, "pull": PULL, "hubdup": HUBDUP, "radar": RADAR, "blocked": BLOCKED}
 
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)
@@ -51,7 +51,8 @@ class Opts(C.Structure):
     workgroup size and the wave-per-vertex cut-over degree."""
     _fields_ = [("variant", C.c_int), ("hub_placement", C.c_int), ("hubs", C.c_int64),
                 ("threshold", C.c_int64), ("wg_size", C.c_int), ("heavy_degree", C.c_int),
-                ("l2_hubs", C.c_int64)]
+                ("l2_hubs", C.c_int64), ("dst_block_bits", C.c_int), ("src_segment_bits", C.c_int),
+                ("unroll", C.c_int)]
 
 
 def _sig(name, res, *args):
@@ -121,7 +122,7 @@ def _ptr(a: np.ndarray, t):
 
 
 def opts(variant="push", hubs=0, threshold=-1, wg_size=0, heavy_degree=0, hub_placement=HUBS_LDS,
-         l2_hubs=0) -> Opts:
+         l2_hubs=0, dst_block_bits=0, src_segment_bits=0, unroll=0) -> Opts:
     o = Opts()
     lib.rg_opts_default(C.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -131,6 +132,9 @@ def opts(variant="push", hubs=0, threshold=-1, wg_size=0, heavy_degree=0, hub_pl
     o.heavy_degree = int(heavy_degree)
     o.hub_placement = int(hub_placement)
     o.l2_hubs = int(l2_hubs)
+    o.dst_block_bits = int(dst_block_bits)
+    o.src_segment_bits = int(src_segment_bits)
+    o.unroll = int(unroll)
     return o
 
 

@@ -28,7 +28,7 @@ def rel_linf(a, b):
 
 
 @pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12s"])
-@pytest.mark.parametrize("variant", VARIANTS + ["pull"])
+@pytest.mark.parametrize("variant", VARIANTS + ["pull", "blocked"])
 def test_pagerank(rg, orc, ctx, graphs, gname, variant):
     hg = graphs(gname)
     off, nbr = hg.csr()

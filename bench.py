@@ -3,10 +3,12 @@ RADAR on one MI355X, on the directed R-MAT graph (GAP generator, scale 24,
 edge factor 16, a/b/c = 0.57/0.19/0.19, seed 27491095, self-loops and
 duplicates removed; int32 ids, uint64 offsets, FP64 ranks).
 
-One "step" is one PageRank run of 20 iterations with the RADAR engine
-(degree-sorted graph, hub copies) -- the headline; its GTEPS is
-20 * m / time.  The other variants, BFS from 64 GAP-picked sources, and
-the CPU oracle baseline are reported alongside.
+One "step" is one PageRank run of 20 iterations with RADAR: the
+degree-sorted graph with hub-duplicated accumulators, on the engine the
+autotuner picks for this graph (the blocked LDS engine or the atomic one);
+its GTEPS is 20 * m / time.  Every variant (atomic push-only, HUBDUP,
+DegreeSorting, atomic RADAR, pull, blocked on the unsorted graph), BFS from
+64 GAP-picked sources, and the CPU oracle baseline are reported alongside.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S]
 Prints one JSON line.
@@ -66,6 +68,7 @@ def main():
         "degree_sorting": (sg, rg.opts("push")),
         "radar": (sg, rg.opts("radar", hubs=args.hubs)),
         "pull": (dg, rg.opts("pull")),
+        "blocked_unsorted": (dg, rg.opts("blocked")),
     }
     pr = {}
     for name, (g, o) in variants.items():
@@ -74,8 +77,20 @@ def main():
         pr[name] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2)}
         log(f"pagerank {name:15s} {ms:8.2f} ms  {pr[name]['gteps']:7.2f} GTEPS")
 
+    # RADAR with the autotuned engine for this graph.
+    t0 = time.time()
+    tuned, _ = sg.autotune("pagerank", base=rg.opts("radar", hubs=args.hubs))
+    t_tune = time.time() - t0
+    engine = {0: "push", 1: "pull", 2: "hubdup", 3: "radar-atomic", 4: "blocked"}[tuned.variant]
+    log(f"autotune pagerank: engine={engine} wg={tuned.wg_size} unroll={tuned.unroll} ({t_tune:.1f}s)")
+    g, o = sg, tuned
+    g.pagerank(it, o, want_ranks=False)
+    ms = sorted(g.pagerank(it, o, want_ranks=False).ms for _ in range(3))[1]
+    pr["radar_tuned"] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2), "engine": engine,
+                         "wg_size": tuned.wg_size, "unroll": tuned.unroll}
+    log(f"pagerank radar_tuned      {ms:8.2f} ms  {pr['radar_tuned']['gteps']:7.2f} GTEPS")
+
     # Headline: RADAR PageRank, K timed steps after W warmups.
-    g, o = variants["radar"]
     for _ in range(args.warmup):
         g.pagerank(it, o, want_ranks=False)
     ctx.sync()
@@ -89,9 +104,10 @@ def main():
     ctx.timing(False)
     ms_step = wall * 1e3 / args.steps
     value = args.steps * it * m / wall / 1e9
-    # Algorithmic bytes of one edge-map launch (one PageRank iteration's push):
-    # per arc 4 B column + 16 B read-modify-write of the FP64 accumulator,
-    # per vertex 8 B offset + 8 B contribution.
+    # Algorithmic bytes of one edge-phase launch (one PageRank iteration's
+    # push, the edge terms of the 20 m + 48 n per-iteration model): per arc
+    # 4 B column + 16 B read-modify-write of the FP64 accumulator, per vertex
+    # 8 B offset + 8 B contribution.
     bytes_launch = 20 * m + 16 * n
     avg_launch_ms = kms / max(launches, 1)
     achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
@@ -103,11 +119,15 @@ def main():
 
     # BFS from 64 sources per variant.
     srcs = hg.pick_sources(args.bfs_sources)
+    bfs_tuned, _ = sg.autotune("bfs", sources=new_ids[srcs[:4]], base=rg.opts("push"))
+    log(f"autotune bfs: variant={bfs_tuned.variant} threshold={bfs_tuned.threshold} "
+        f"heavy={bfs_tuned.heavy_degree} hubs={bfs_tuned.hubs}")
     bfs = {}
     for name, g2, o2, mapped in [("push", dg, rg.opts("push"), False),
                                  ("hubdup", dg, rg.opts("hubdup", hubs=args.hubs), False),
                                  ("degree_sorting", sg, rg.opts("push"), True),
-                                 ("radar", sg, rg.opts("radar", hubs=args.hubs), True)]:
+                                 ("radar", sg, rg.opts("radar", hubs=args.hubs), True),
+                                 ("radar_tuned", sg, bfs_tuned, True)]:
         tot_e, tot_ms = 0, 0.0
         for s in srcs:
             s2 = int(new_ids[s]) if mapped else int(s)
@@ -122,10 +142,14 @@ def main():
         import pyoracle
         off, nbr = hg.csr()
         cores = min(16, os.cpu_count() or 1)
-        cpu_iters = 2
         t0 = time.perf_counter()
-        pyoracle.pagerank(off, nbr, cpu_iters, nthreads=cores)
+        pyoracle.pagerank(off, nbr, 1, nthreads=cores)
+        per_iter = time.perf_counter() - t0
+        cpu_iters = int(max(2, min(60, round(12.0 / max(per_iter, 1e-3)))))
+        t0 = time.perf_counter()
+        _, done_iters, _ = pyoracle.pagerank(off, nbr, cpu_iters, nthreads=cores)
         tc = time.perf_counter() - t0
+        cpu_iters = min(done_iters, cpu_iters)
         cpu = {"value": round(cpu_iters * m / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
                "sample": f"oracle push PageRank (OpenMP, CAS-loop FP64 adds as the reference), "
                          f"{cpu_iters} iterations on the same rmat{args.scale} graph, {tc:.1f} s"}
@@ -144,18 +168,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (GAP R-MAT generator, seed 27491095)",
-        "config": {"workload": f"PageRank {it} iterations, RADAR (degree-sorted + LDS hub copies), "
-                               f"directed rmat{args.scale} edge factor 16",
+        "config": {"workload": f"PageRank {it} iterations, RADAR (degree-sorted graph, hub-duplicated "
+                               f"LDS accumulators, autotuned engine: {engine}), directed rmat{args.scale} "
+                               f"edge factor 16",
                    "n": n, "m": m, "iterations": it, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": "edge_light_kernel<PrOp,SRC_ALL,HUB_PREFIX>",
+                     "kernel": "k_blk_pr" if engine == "blocked" else "edge_light_kernel<PrOp,SRC_ALL,HUB_PREFIX>",
                      "launches": launches, "avg_launch_ms": round(avg_launch_ms, 4),
                      "bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,
         "pagerank": pr,
         "bfs": bfs,
-        "preprocess_ms": {"host_graph_build": round(t_gen * 1e3, 1), "gpu_degree_sort": round(t_sort * 1e3, 2)},
+        "preprocess_ms": {"host_graph_build": round(t_gen * 1e3, 1), "gpu_degree_sort": round(t_sort * 1e3, 2),
+                          "autotune": round(t_tune * 1e3, 1)},
     }
     print(json.dumps(out), flush=True)
 

@@ -212,6 +212,17 @@ int rg_bc(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o,
 int rg_tc(rg_graph* g, const rg_opts* o, uint64_t* total_out, uint64_t* per_vertex,
           double* ms_out);
 
+/* ---- autotuner -------------------------------------------------------------
+ * Times candidate configurations of `app` on g and writes the fastest to
+ * *best (the reference fixes these: llcCap = 1<<25 for the hub count, hubdup
+ * Ligra/ligra/IO.h:749-755, and the m/20 threshold, ligra.h:269).
+ * RG_APP_PAGERANK: blocked-engine workgroup shape / unroll vs atomic RADAR.
+ * RG_APP_BFS: sparse/dense threshold x heavy-degree cut-over x hub count,
+ * timed over the given sources.  best_ms (nullable): the winner's time. */
+enum { RG_APP_PAGERANK = 0, RG_APP_BFS = 1 };
+int rg_autotune(rg_graph* g, int app, const uint32_t* sources, int nsources, const rg_opts* base,
+                rg_opts* best, double* best_ms);
+
 #ifdef __cplusplus
 }
 #endif

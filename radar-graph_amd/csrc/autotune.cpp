@@ -1,0 +1,92 @@
+// Per-graph autotuner: times candidate engine configurations on the graph
+// itself and keeps the fastest.  The reference hard-codes these choices
+// (llcCap = 1<<25 for the hub count, hubdup Ligra/ligra/IO.h:749-755; the
+// m/20 sparse/dense threshold, ligra.h:269; OpenMP's dynamic,64 schedule,
+// ligra.h:59); on the GPU they depend on the graph's degree skew and id
+// order, so they are measured instead.
+//
+// PageRank candidates: the blocked engine with each workgroup shape
+// (512 / 1024 threads) and 1 or 4 edges in flight per lane, and the atomic
+// RADAR engine at the LDS hub budget.  BFS candidates: the sparse/dense
+// threshold (m/5, m/20, m/80), the wave-per-vertex cut-over degree and the
+// hub count (none / LDS budget).  Every candidate computes the same result;
+// only time differs.
+#include <cstring>
+#include <vector>
+
+#include "rg_common.h"
+
+namespace {
+
+struct Trial {
+  rg_opts o;
+  double ms;
+};
+
+}  // namespace
+
+extern "C" int rg_autotune(rg_graph* g, int app, const uint32_t* sources, int nsources, const rg_opts* base,
+                           rg_opts* best, double* best_ms) {
+  if (!g || !best) return rg::set_error(RG_EINVAL, "rg_autotune: null argument");
+  int64_t n = 0, m = 0;
+  int sym = 0;
+  RG_TRY_HOST(rg_graph_info(g, &n, &m, &sym));
+  rg_opts b;
+  if (base) b = *base;
+  else rg_opts_default(&b);
+  std::vector<rg_opts> cand;
+  if (app == RG_APP_PAGERANK) {
+    for (int wg : {1024, 512})
+      for (int un : {1, 4}) {
+        rg_opts o = b;
+        o.variant = RG_BLOCKED;
+        o.wg_size = wg;
+        o.unroll = un;
+        cand.push_back(o);
+      }
+    rg_opts r = b;
+    r.variant = RG_RADAR;
+    r.hubs = 0;
+    cand.push_back(r);
+  } else if (app == RG_APP_BFS) {
+    if (!sources || nsources <= 0) return rg::set_error(RG_EINVAL, "rg_autotune: BFS needs sources");
+    for (int64_t div : {5, 20, 80})
+      for (int heavy : {256, 1024, 4096})
+        for (int hv : {RG_PUSH, RG_RADAR}) {
+          rg_opts o = b;
+          o.variant = hv;
+          o.threshold = m / div;
+          o.heavy_degree = heavy;
+          o.hubs = hv == RG_RADAR ? 2048 : 0;
+          cand.push_back(o);
+        }
+  } else {
+    return rg::set_error(RG_EINVAL, "rg_autotune: unknown app");
+  }
+  std::vector<Trial> res;
+  for (const rg_opts& o : cand) {
+    double total = 0.0;
+    if (app == RG_APP_PAGERANK) {
+      int64_t it = 0;
+      double l1 = 0.0, ms = 0.0;
+      // One untimed call builds any cached layout; the second is timed.
+      RG_TRY_HOST(rg_pagerank(g, 3, &o, nullptr, &it, &l1, &ms));
+      RG_TRY_HOST(rg_pagerank(g, 5, &o, nullptr, &it, &l1, &ms));
+      total = ms;
+    } else {
+      for (int i = 0; i < nsources; ++i) {
+        int64_t it = 0, reached = 0, edges = 0;
+        double ms = 0.0;
+        RG_TRY_HOST(rg_bfs(g, sources[i], &o, nullptr, &it, &reached, &edges, &ms));
+        total += ms;
+      }
+    }
+    res.push_back({o, total});
+  }
+  size_t bi = 0;
+  for (size_t i = 1; i < res.size(); ++i)
+    if (res[i].ms < res[bi].ms) bi = i;
+  *best = res[bi].o;
+  if (best_ms) *best_ms = res[bi].ms;
+  return RG_OK;
+}

@@ -11,6 +11,12 @@
 
 #include "radar_graph.h"
 
+#define RG_TRY_HOST(call)       \
+  do {                          \
+    int _rc = (call);           \
+    if (_rc != RG_OK) return _rc; \
+  } while (0)
+
 namespace rg {
 
 // Records `msg` as the calling thread's last error and returns `code`.

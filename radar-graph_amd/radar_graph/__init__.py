@@ -105,6 +105,8 @@ def _opt_sig(name, res, *args):
 
 _opt_sig("rg_graph_degree_sort", C.c_int, _vp, C.c_int, C.c_int, C.POINTER(_vp), _u32p)
 _opt_sig("rg_tc", C.c_int, _vp, C.POINTER(Opts), _u64p, _u64p, _f64p)
+_sig("rg_autotune", C.c_int, _vp, C.c_int, _u32p, C.c_int, C.POINTER(Opts), C.POINTER(Opts), _f64p)
+APP_PAGERANK, APP_BFS = 0, 1
 
 
 def exported_symbols():
@@ -372,6 +374,18 @@ class DeviceGraph:
         _check(lib.rg_bc(self._h, _ptr(sources, _u32p), len(sources), C.byref(o), _ptr(deps, _f64p),
                          C.byref(comb), C.byref(ms)))
         return deps, comb.value, ms.value
+
+    def autotune(self, app="pagerank", sources=None, base: Opts = None):
+        """Fastest engine configuration for `app` on this graph: (Opts, ms)."""
+        best, ms = Opts(), C.c_double()
+        if app == "pagerank":
+            _check(lib.rg_autotune(self._h, APP_PAGERANK, None, 0, C.byref(base) if base else None,
+                                   C.byref(best), C.byref(ms)))
+        else:
+            src = np.ascontiguousarray(sources, np.uint32)
+            _check(lib.rg_autotune(self._h, APP_BFS, _ptr(src, _u32p), len(src),
+                                   C.byref(base) if base else None, C.byref(best), C.byref(ms)))
+        return best, ms.value
 
     def tc(self, o: Opts = None, per_vertex=False):
         f = _OPTIONAL.get("rg_tc")

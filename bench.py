@@ -39,6 +39,10 @@ def main():
     ap.add_argument("--bfs-sources", type=int, default=64)
     ap.add_argument("--hubs", type=int, default=0, help="LDS hub slots (0 = LDS budget)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only (no variant table, no BFS)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run the library's plain 1 GiB stream copy once (k_stream_copy: the "
+                         "known-size kernel that scales FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -48,12 +52,18 @@ def main():
     import numpy as np
     import radar_graph as rg
 
+
     t0 = time.time()
     hg = rg.HostGraph.generate(args.scale, 16, uniform=False, symmetric=False)
     n, m, _ = hg.info()
     t_gen = time.time() - t0
     log(f"graph: rmat{args.scale} directed n={n} m={m} (host build {t_gen:.1f}s)")
     ctx = rg.Context(0)
+    if args.calibrate_copy:
+        gb, cms = ctx.copy_bandwidth(1 << 30, reps=1)
+        log(f"calibration copy (k_stream_copy): 1 GiB read + 1 GiB write, {gb:.0f} GB/s")
+    stream_gbs, _ = ctx.copy_bandwidth(1 << 30, reps=5)
+    log(f"measured stream copy: {stream_gbs:.0f} GB/s (read + write)")
     dg = rg.DeviceGraph(ctx, hg)
     t0 = time.time()
     sg, new_ids = dg.degree_sort(by_outdeg=False, sort_neighbors=False)
@@ -71,6 +81,8 @@ def main():
         "blocked_unsorted": (dg, rg.opts("blocked")),
     }
     pr = {}
+    if args.quick:
+        variants = {}
     for name, (g, o) in variants.items():
         g.pagerank(it, o, want_ranks=False)
         ms = sorted(g.pagerank(it, o, want_ranks=False).ms for _ in range(3))[1]
@@ -119,15 +131,18 @@ def main():
 
     # BFS from 64 sources per variant.
     srcs = hg.pick_sources(args.bfs_sources)
-    bfs_tuned, _ = sg.autotune("bfs", sources=new_ids[srcs[:4]], base=rg.opts("push"))
-    log(f"autotune bfs: variant={bfs_tuned.variant} threshold={bfs_tuned.threshold} "
-        f"heavy={bfs_tuned.heavy_degree} hubs={bfs_tuned.hubs}")
     bfs = {}
-    for name, g2, o2, mapped in [("push", dg, rg.opts("push"), False),
-                                 ("hubdup", dg, rg.opts("hubdup", hubs=args.hubs), False),
-                                 ("degree_sorting", sg, rg.opts("push"), True),
-                                 ("radar", sg, rg.opts("radar", hubs=args.hubs), True),
-                                 ("radar_tuned", sg, bfs_tuned, True)]:
+    bfs_runs = []
+    if not args.quick:
+        bfs_tuned, _ = sg.autotune("bfs", sources=new_ids[srcs[:4]], base=rg.opts("push"))
+        log(f"autotune bfs: variant={bfs_tuned.variant} threshold={bfs_tuned.threshold} "
+            f"heavy={bfs_tuned.heavy_degree} hubs={bfs_tuned.hubs}")
+        bfs_runs = [("push", dg, rg.opts("push"), False),
+                    ("hubdup", dg, rg.opts("hubdup", hubs=args.hubs), False),
+                    ("degree_sorting", sg, rg.opts("push"), True),
+                    ("radar", sg, rg.opts("radar", hubs=args.hubs), True),
+                    ("radar_tuned", sg, bfs_tuned, True)]
+    for name, g2, o2, mapped in bfs_runs:
         tot_e, tot_ms = 0, 0.0
         for s in srcs:
             s2 = int(new_ids[s]) if mapped else int(s)
@@ -174,6 +189,7 @@ def main():
                    "n": n, "m": m, "iterations": it, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "measured_stream_copy_gbs": round(stream_gbs, 1),
                      "kernel": "k_blk_pr" if engine == "blocked" else "edge_light_kernel<PrOp,SRC_ALL,HUB_PREFIX>",
                      "launches": launches, "avg_launch_ms": round(avg_launch_ms, 4),
                      "bytes_per_launch": bytes_launch},

@@ -108,6 +108,12 @@ int rg_ctx_sync(rg_ctx* ctx);
  * rg_ctx_kernel_time returns the number of timed launches and their summed
  * device time since then. */
 int rg_ctx_timing(rg_ctx* ctx, int enable);
+/* Device stream-copy probe: copies `bytes` between two device buffers `reps`
+ * times with a plain 16-byte-per-lane kernel (k_stream_copy); *gbps counts
+ * read + write bytes.  The measured HBM reference beside the 8 TB/s peak,
+ * and the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE
+ * counters.  No counterpart in the reference. */
+int rg_device_copy_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, double* gbps, double* ms_per_copy);
 int rg_ctx_kernel_time(rg_ctx* ctx, int64_t* launches, double* total_ms);
 
 /* ---- device graphs ------------------------------------------------------- */

@@ -154,6 +154,43 @@ int rg_ctx_kernel_time(rg_ctx* ctx, int64_t* launches, double* total_ms) {
   return RG_OK;
 }
 
+// Plain streaming copy, 16 bytes per lane per step: the reference point for
+// HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int rg_device_copy_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, double* gbps, double* ms_per_copy) {
+  if (!ctx || bytes < 16 || reps < 1) return set_error(RG_EINVAL, "rg_device_copy_bandwidth: bad arguments");
+  RG_HIP(hipSetDevice(ctx->device));
+  const int64_t n16 = bytes / 16;
+  rg::DevBuf a, b;
+  RG_TRY(a.ensure((size_t)n16 * 16));
+  RG_TRY(b.ensure((size_t)n16 * 16));
+  RG_HIP(hipMemsetAsync(a.p, 1, (size_t)n16 * 16, ctx->stream));
+  const int grid = ctx->num_cus * 8;
+  hipLaunchKernelGGL(k_stream_copy, grid, 256, 0, ctx->stream, a.as<uint4>(), b.as<uint4>(), n16);
+  hipEvent_t e0, e1;
+  RG_HIP(hipEventCreate(&e0));
+  RG_HIP(hipEventCreate(&e1));
+  RG_HIP(hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_stream_copy, grid, 256, 0, ctx->stream, a.as<uint4>(), b.as<uint4>(), n16);
+  RG_HIP(hipGetLastError());
+  RG_HIP(hipEventRecord(e1, ctx->stream));
+  RG_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  RG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const double per = ms / reps;
+  if (ms_per_copy) *ms_per_copy = per;
+  if (gbps) *gbps = 2.0 * (double)(n16 * 16) / (per * 1e-3) / 1e9;   // read + write
+  return RG_OK;
+}
+
 int rg_ctx_sync(rg_ctx* ctx) {
   if (!ctx) return set_error(RG_EINVAL, "rg_ctx_sync: null");
   RG_HIP(hipStreamSynchronize(ctx->stream));

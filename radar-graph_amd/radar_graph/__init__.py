@@ -84,6 +84,7 @@ _sig("rg_ctx_destroy", C.c_int, _vp)
 _sig("rg_ctx_sync", C.c_int, _vp)
 _sig("rg_ctx_timing", C.c_int, _vp, C.c_int)
 _sig("rg_ctx_kernel_time", C.c_int, _vp, _i64p, _f64p)
+_sig("rg_device_copy_bandwidth", C.c_int, _vp, C.c_int64, C.c_int, _f64p, _f64p)
 _sig("rg_graph_upload", C.c_int, _vp, _vp, C.POINTER(_vp))
 _sig("rg_graph_info", C.c_int, _vp, _i64p, _i64p, C.POINTER(C.c_int))
 _sig("rg_graph_download", C.c_int, _vp, C.c_int, _u64p, _u32p)
@@ -271,6 +272,12 @@ class Context:
     def timing(self, enable=True):
         """Bracket every edge-map launch with HIP events (resets the record)."""
         _check(lib.rg_ctx_timing(self._h, int(enable)))
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10):
+        """(GB/s read+write, ms per copy) of a plain device stream copy."""
+        g, ms = C.c_double(), C.c_double()
+        _check(lib.rg_device_copy_bandwidth(self._h, int(nbytes), int(reps), C.byref(g), C.byref(ms)))
+        return g.value, ms.value
 
     def kernel_time(self):
         """(timed edge-map launches, their summed device ms) since timing()."""

@@ -218,6 +218,51 @@ int rg_bc(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o,
 int rg_tc(rg_graph* g, const rg_opts* o, uint64_t* total_out, uint64_t* per_vertex,
           double* ms_out);
 
+/* ---- engine level: Ligra's vertexSubset / edgeMap / vertexMap -----------------
+ * The reference's operator interface (PNO Ligra/ligra/ligra.h:336-340
+ * edgeMap, 447-479 vertexMap; vertexSubset.h:113-229) with each app functor
+ * as a named operator over device state arrays.  A subset is a device
+ * vertex list (or "all vertices"); edgeMap chooses sparse / dense rounds
+ * with Ligra's rule |F| + sum(outdeg) > threshold (threshold < 0: m/20,
+ * ligra.h:269) and returns the next frontier unless RG_NO_OUTPUT. */
+typedef struct rg_subset rg_subset;
+typedef struct rg_array rg_array;
+enum { RG_NO_OUTPUT = 1, RG_DENSE_FORWARD = 8 };   /* Ligra's flags (ligra.h:51-58) */
+enum { RG_OP_BFS = 0, RG_OP_PAGERANK = 1, RG_OP_RADII = 2, RG_OP_BC_FWD = 3, RG_OP_BC_BACK = 4 };
+enum { RG_VOP_PR_UPDATE = 0, RG_VOP_PR_CONTRIB = 1, RG_VOP_RADII_SYNC = 2, RG_VOP_MARK = 3,
+       RG_VOP_BC_BACK = 4, RG_VOP_INVERT = 5, RG_VOP_BC_FINAL = 6 };
+int rg_subset_create(rg_graph* g, rg_subset** out);                 /* empty subset  */
+int rg_subset_set(rg_subset* s, const uint32_t* ids, int64_t count); /* from host ids */
+int rg_subset_set_all(rg_subset* s);                                 /* every vertex  */
+int rg_subset_size(const rg_subset* s, int64_t* size);               /* numNonzeros   */
+int rg_subset_get(const rg_subset* s, uint32_t* ids);                /* toSparse copy */
+int rg_subset_free(rg_subset* s);                                    /* del()         */
+/* Device state arrays (the apps' malloc'd arrays): elem_bytes 1, 4 or 8. */
+int rg_array_alloc(rg_ctx* ctx, int64_t count, int elem_bytes, rg_array** out);
+int rg_array_write(rg_array* a, const void* host, int64_t first, int64_t count);
+int rg_array_read(const rg_array* a, void* host, int64_t first, int64_t count);
+int rg_array_fill(rg_array* a, uint64_t bits);  /* every element = low elem_bytes of bits */
+int rg_array_free(rg_array* a);
+/* edgeMap.  State per operator:
+ *   RG_OP_BFS      {parents u32}                    BFS_F      (BFS.C:169-190)
+ *   RG_OP_PAGERANK {contrib f64, p_next f64}        PR_F       (PageRank.C:29-43)
+ *   RG_OP_RADII    {visited u64, next u64, radii i32} Radii_F  (Radii.C:35-64; `round`)
+ *   RG_OP_BC_FWD   {num_paths f64, visited u8}      BC_F       (BC-manyiter-v2.C:189-213)
+ *   RG_OP_BC_BACK  {dependencies f64, visited u8}   BC_Back_F  (BC-manyiter-v2.C:215-236)
+ * o (nullable) selects push / HUBDUP / RADAR hub copies. */
+int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int nstate, int64_t round,
+               int64_t threshold, int flags, const rg_opts* o, rg_subset* out);
+/* vertexMap over a subset:
+ *   RG_VOP_PR_UPDATE  {p}: p = a*p + b             PR_Vertex_F (PageRank.C:46-58)
+ *   RG_VOP_PR_CONTRIB {p, contrib}: contrib = p/outdeg, every vertex
+ *   RG_VOP_RADII_SYNC {visited, next}: visited = next   Radii_Vertex_F (Radii.C:67-75)
+ *   RG_VOP_MARK       {visited u8}: visited = 1     BC_Vertex_F (BC-manyiter-v2.C:239-246)
+ *   RG_VOP_BC_BACK    {visited u8, deps, inv}: visited = 1, deps += inv
+ *                                                   BC_Back_Vertex_F (BC-manyiter-v2.C:250-258)
+ *   RG_VOP_INVERT     {x}: x = 1/x, every vertex    (BC-manyiter-v2.C:364-365)
+ *   RG_VOP_BC_FINAL   {deps, inv}: deps = (deps - inv)/inv, every vertex (BC-manyiter-v2.C:387-389) */
+int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int nstate, double a, double b);
+
 /* ---- autotuner -------------------------------------------------------------
  * Times candidate configurations of `app` on g and writes the fastest to
  * *best (the reference fixes these: llcCap = 1<<25 for the hub count, hubdup

@@ -442,11 +442,11 @@ static int build_hub_bitmap(rg_graph* g, int64_t k) {
   RG_HIP(hipMalloc(&g->hub_rank, sizeof(uint32_t) * nw));
   RG_HIP(hipMalloc(&g->hub_ids, sizeof(uint32_t) * nh));
   RG_HIP(hipMalloc(&g->hub_slot_of, sizeof(uint32_t) * nh));
-  RG_HIP(hipMemcpy(g->hub_bits, bits.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
-  RG_HIP(hipMemcpy(g->hub_rank, rank.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
+  RG_TRY(rg::ctx_copy(ctx, g->hub_bits, bits.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
+  RG_TRY(rg::ctx_copy(ctx, g->hub_rank, rank.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
   if (!ids.empty()) {
-    RG_HIP(hipMemcpy(g->hub_ids, slot_ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
-    RG_HIP(hipMemcpy(g->hub_slot_of, slot_of.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+    RG_TRY(rg::ctx_copy(ctx, g->hub_ids, slot_ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+    RG_TRY(rg::ctx_copy(ctx, g->hub_slot_of, slot_of.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
   }
   g->hub_k = (int64_t)ids.size();
   g->hub_k_req = k;
@@ -647,11 +647,11 @@ int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks
   int64_t last = dn ? dn : maxiters;
   double l1 = 0.0;
   if (last >= 1)
-    RG_HIP(hipMemcpy(&l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
+    RG_TRY(rg::ctx_copy(ctx, &l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
   if (iters_out) *iters_out = dn ? dn : maxiters + 1;
   if (l1_out) *l1_out = l1;
   if (ms_out) *ms_out = ms;
-  if (ranks_out) RG_HIP(hipMemcpy(ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (ranks_out) RG_TRY(rg::ctx_copy(ctx, ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 
@@ -727,7 +727,7 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   if (reached_out) *reached_out = h[0];
   if (edges_out) *edges_out = h[1];
   if (ms_out) *ms_out = ms;
-  if (parents_out) RG_HIP(hipMemcpy(parents_out, parents, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  if (parents_out) RG_TRY(rg::ctx_copy(ctx, parents_out, parents, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 
@@ -777,7 +777,7 @@ int radii_push(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* 
   Frontier fa, fb;
   RG_TRY(ctx_frontiers(ctx, n, fa, fb));
   if (nstarts > 0) {
-    RG_HIP(hipMemcpy(fa.ids, starts, sizeof(uint32_t) * nstarts, hipMemcpyHostToDevice));
+    RG_TRY(rg::ctx_copy(ctx, fa.ids, starts, sizeof(uint32_t) * nstarts, hipMemcpyHostToDevice));
   }
   hipLaunchKernelGGL(k_radii_start, 1, 64, 0, ctx->stream, fa.ids, nstarts, radii, nvis);
   fa.size = nstarts;
@@ -804,7 +804,7 @@ int radii_push(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* 
   double ms = tm.stop_ms();
   if (rounds_out) *rounds_out = round;
   if (ms_out) *ms_out = ms;
-  if (radii_out) RG_HIP(hipMemcpy(radii_out, radii, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  if (radii_out) RG_TRY(rg::ctx_copy(ctx, radii_out, radii, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 
@@ -927,7 +927,7 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
   (void)was_transposed;
   if (combined_out) *combined_out = combined;
   if (ms_out) *ms_out = ms;
-  if (deps_out) RG_HIP(hipMemcpy(deps_out, deps, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (deps_out) RG_TRY(rg::ctx_copy(ctx, deps_out, deps, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 
@@ -1020,11 +1020,11 @@ int pagerank_pull(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks
   RG_TRY(ctx_read_count(ctx, done, &dn, 1));
   int64_t last = dn ? dn : maxiters;
   double l1 = 0.0;
-  if (last >= 1) RG_HIP(hipMemcpy(&l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
+  if (last >= 1) RG_TRY(rg::ctx_copy(ctx, &l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
   if (iters_out) *iters_out = dn ? dn : maxiters + 1;
   if (l1_out) *l1_out = l1;
   if (ms_out) *ms_out = ms;
-  if (ranks_out) RG_HIP(hipMemcpy(ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (ranks_out) RG_TRY(rg::ctx_copy(ctx, ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 
@@ -1075,6 +1075,332 @@ int rg_bc(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, 
   if (!g || nsources < 0 || (nsources > 0 && !sources)) return set_error(RG_EINVAL, "rg_bc: bad arguments");
   RG_HIP(hipSetDevice(g->ctx->device));
   return rg::bc_push(g, sources, nsources, o, deps_out, combined_out, ms_out);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Engine-level C ABI: Ligra's vertexSubset / edgeMap / vertexMap with the
+// reference's functors as named operators (PNO Ligra/ligra/ligra.h:336-340,
+// 447-479; vertexSubset.h:113-229).
+
+struct rg_subset {
+  rg_graph* g = nullptr;
+  rg::DevBuf ids;
+  int64_t size = 0;
+  int64_t degsum = -1;   // sum of out-degrees, -1 = unknown
+  bool all = false;
+};
+
+struct rg_array {
+  rg_ctx* ctx = nullptr;
+  rg::DevBuf buf;
+  int64_t count = 0;
+  int elem = 0;
+};
+
+namespace rg {
+
+__global__ void k_bc_back_vertex_all(int64_t n, uint8_t* visited, double* deps, const double* inv) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    visited[v] = 1;
+    deps[v] += inv[v];
+  }
+}
+
+__global__ void k_pr_vertex_update(const uint32_t* ids, int64_t count, bool all, double* p_next, double damping,
+                                   double added) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = all ? (uint32_t)i : ids[i];
+    p_next[v] = __dadd_rn(__dmul_rn(damping, p_next[v]), added);
+  }
+}
+
+__global__ void k_sync_u64(const uint32_t* ids, int64_t count, bool all, unsigned long long* dst,
+                           const unsigned long long* src) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = all ? (uint32_t)i : ids[i];
+    dst[v] = src[v];
+  }
+}
+
+__global__ void k_mark_u8(const uint32_t* ids, int64_t count, bool all, uint8_t* dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[all ? i : ids[i]] = 1;
+}
+
+__global__ void k_bc_back_vertex_l(const uint32_t* ids, int64_t count, uint8_t* visited, double* deps,
+                                   const double* inv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = ids[i];
+    visited[v] = 1;
+    deps[v] += inv[v];
+  }
+}
+
+__global__ void k_pr_contrib(const uint64_t* off, int64_t n, const double* p, double* contrib) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t deg = off[v + 1] - off[v];
+    contrib[v] = deg ? p[v] / (double)(uint32_t)deg : 0.0;
+  }
+}
+
+static int check_state(rg_graph* g, rg_array* const* st, int nstate, std::initializer_list<int> elems,
+                       const char* who) {
+  if (nstate != (int)elems.size()) return set_error(RG_EINVAL, std::string(who) + ": wrong number of state arrays");
+  int i = 0;
+  for (int e : elems) {
+    if (!st[i] || st[i]->elem != e || st[i]->count < g->n)
+      return set_error(RG_EINVAL, std::string(who) + ": state array " + std::to_string(i) +
+                                      " must hold n elements of " + std::to_string(e) + " bytes");
+    ++i;
+  }
+  return RG_OK;
+}
+
+}  // namespace rg
+
+extern "C" {
+
+int rg_subset_create(rg_graph* g, rg_subset** out) {
+  if (!g || !out) return set_error(RG_EINVAL, "rg_subset_create: null argument");
+  std::unique_ptr<rg_subset> s(new rg_subset);
+  s->g = g;
+  RG_TRY(s->ids.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(g->n, 64)));
+  *out = s.release();
+  return RG_OK;
+}
+
+int rg_subset_set(rg_subset* s, const uint32_t* ids, int64_t count) {
+  if (!s || count < 0 || count > s->g->n || (count > 0 && !ids)) return set_error(RG_EINVAL, "rg_subset_set: bad arguments");
+  for (int64_t i = 0; i < count; ++i)
+    if ((int64_t)ids[i] >= s->g->n) return set_error(RG_EINVAL, "rg_subset_set: vertex out of range");
+  if (count) RG_TRY(rg::ctx_copy(s->g->ctx, s->ids.p, ids, sizeof(uint32_t) * count, hipMemcpyHostToDevice));
+  s->size = count;
+  s->all = false;
+  s->degsum = -1;
+  return RG_OK;
+}
+
+int rg_subset_set_all(rg_subset* s) {
+  if (!s) return set_error(RG_EINVAL, "rg_subset_set_all: null");
+  s->all = true;
+  s->size = s->g->n;
+  s->degsum = s->g->m;
+  return RG_OK;
+}
+
+int rg_subset_size(const rg_subset* s, int64_t* size) {
+  if (!s || !size) return set_error(RG_EINVAL, "rg_subset_size: null");
+  *size = s->size;
+  return RG_OK;
+}
+
+int rg_subset_get(const rg_subset* s, uint32_t* ids) {
+  if (!s || !ids) return set_error(RG_EINVAL, "rg_subset_get: null");
+  if (s->all) {
+    for (int64_t i = 0; i < s->size; ++i) ids[i] = (uint32_t)i;
+  } else if (s->size) {
+    RG_TRY(rg::ctx_copy(s->g->ctx, ids, s->ids.p, sizeof(uint32_t) * s->size, hipMemcpyDeviceToHost));
+  }
+  return RG_OK;
+}
+
+int rg_subset_free(rg_subset* s) {
+  delete s;
+  return RG_OK;
+}
+
+int rg_array_alloc(rg_ctx* ctx, int64_t count, int elem_bytes, rg_array** out) {
+  if (!ctx || !out || count < 0 || (elem_bytes != 1 && elem_bytes != 4 && elem_bytes != 8))
+    return set_error(RG_EINVAL, "rg_array_alloc: bad arguments");
+  std::unique_ptr<rg_array> a(new rg_array);
+  a->ctx = ctx;
+  a->count = count;
+  a->elem = elem_bytes;
+  RG_TRY(a->buf.ensure((size_t)std::max<int64_t>(count, 1) * elem_bytes));
+  *out = a.release();
+  return RG_OK;
+}
+
+int rg_array_write(rg_array* a, const void* host, int64_t first, int64_t count) {
+  if (!a || (count > 0 && !host) || first < 0 || first + count > a->count)
+    return set_error(RG_EINVAL, "rg_array_write: bad range");
+  if (count)
+    RG_TRY(rg::ctx_copy(a->ctx, static_cast<char*>(a->buf.p) + first * a->elem, host, (size_t)count * a->elem,
+                     hipMemcpyHostToDevice));
+  return RG_OK;
+}
+
+int rg_array_read(const rg_array* a, void* host, int64_t first, int64_t count) {
+  if (!a || (count > 0 && !host) || first < 0 || first + count > a->count)
+    return set_error(RG_EINVAL, "rg_array_read: bad range");
+  RG_HIP(hipStreamSynchronize(a->ctx->stream));
+  if (count)
+    RG_TRY(rg::ctx_copy(a->ctx, host, static_cast<const char*>(a->buf.p) + first * a->elem, (size_t)count * a->elem,
+                     hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+int rg_array_fill(rg_array* a, uint64_t bits) {
+  if (!a) return set_error(RG_EINVAL, "rg_array_fill: null");
+  rg_ctx* ctx = a->ctx;
+  switch (a->elem) {
+    case 1: return rg::fill<uint8_t>(ctx, a->buf.as<uint8_t>(), a->count, (uint8_t)bits);
+    case 4: return rg::fill<uint32_t>(ctx, a->buf.as<uint32_t>(), a->count, (uint32_t)bits);
+    default: return rg::fill<unsigned long long>(ctx, a->buf.as<unsigned long long>(), a->count, bits);
+  }
+}
+
+int rg_array_free(rg_array* a) {
+  delete a;
+  return RG_OK;
+}
+
+// edgeMap (ligra.h:264-302, 336-340).  Operators and their state arrays:
+//   RG_OP_BFS      [parents u32]                       BFS_F (BFS.C:169-190)
+//   RG_OP_PAGERANK [contrib f64, p_next f64]           PR_F (PageRank.C:29-43); contrib[s] =
+//                                                      p_curr[s]/outdeg(s) (rg_vertexmap RG_VOP_PR_CONTRIB)
+//   RG_OP_RADII    [visited u64, next_visited u64, radii i32]   Radii_F (Radii.C:35-64), `round`
+//   RG_OP_BC_FWD   [num_paths f64, visited u8]         BC_F (BC-manyiter-v2.C:189-213)
+//   RG_OP_BC_BACK  [dependencies f64, visited u8]      BC_Back_F (BC-manyiter-v2.C:215-236)
+int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int nstate, int64_t round,
+               int64_t threshold, int flags, const rg_opts* o, rg_subset* out) {
+  if (!g || !in || in->g->n != g->n) return set_error(RG_EINVAL, "edgeMap: Sizes Don't match");
+  rg_ctx* ctx = g->ctx;
+  RG_HIP(hipSetDevice(ctx->device));
+  rg_opts oo;
+  if (o) oo = *o;
+  else rg_opts_default(&oo);
+  oo.threshold = threshold;
+  if (oo.variant == RG_PULL || oo.variant == RG_BLOCKED) oo.variant = RG_PUSH;
+  rg::Frontier F, unused;
+  RG_TRY(rg::ctx_frontiers(ctx, g->n, F, unused));
+  F.ids = in->ids.as<uint32_t>();
+  F.size = in->size;
+  F.all = in->all;
+  int64_t degsum = in->degsum;
+  if (!in->all && degsum < 0) RG_TRY(rg::frontier_degsum(ctx, g, F.ids, F.size, &degsum));
+  const bool want_out = out && !(flags & RG_NO_OUTPUT);
+  if (want_out && out->g->n != g->n) return set_error(RG_EINVAL, "edgeMap: Sizes Don't match");
+  int64_t* info = ctx->counts.as<int64_t>() + 28;
+  RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
+  uint32_t* out_ids = want_out ? out->ids.as<uint32_t>() : ctx->fr_ids[1].as<uint32_t>();
+  rg::Engine E;
+  switch (op) {
+    case RG_OP_BFS: {
+      RG_TRY(rg::check_state(g, state, nstate, {4}, "edgeMap(BFS_F)"));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(uint32_t), E));
+      RG_TRY(rg::edge_map(E, rg::BfsOp{state[0]->buf.as<uint32_t>()}, F, degsum, out_ids, info));
+      break;
+    }
+    case RG_OP_PAGERANK: {
+      RG_TRY(rg::check_state(g, state, nstate, {8, 8}, "edgeMap(PR_F)"));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E));
+      int64_t* zero = ctx->counts.as<int64_t>() + 30;
+      RG_HIP(hipMemsetAsync(zero, 0, sizeof(int64_t), ctx->stream));
+      rg::PrOp op2{state[0]->buf.as<double>(), state[1]->buf.as<double>(), zero};
+      RG_TRY(rg::edge_map(E, op2, F, degsum, nullptr, nullptr));
+      break;
+    }
+    case RG_OP_RADII: {
+      RG_TRY(rg::check_state(g, state, nstate, {8, 8, 4}, "edgeMap(Radii_F)"));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(uint64_t), E));
+      rg::RadiiOp op2{state[0]->buf.as<unsigned long long>(), state[1]->buf.as<unsigned long long>(),
+                      state[2]->buf.as<int32_t>(), (int32_t)round};
+      RG_TRY(rg::edge_map(E, op2, F, degsum, out_ids, info));
+      break;
+    }
+    case RG_OP_BC_FWD: {
+      RG_TRY(rg::check_state(g, state, nstate, {8, 1}, "edgeMap(BC_F)"));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E));
+      RG_TRY(rg::edge_map(E, rg::BcFwdOp{state[0]->buf.as<double>(), state[1]->buf.as<uint8_t>()}, F, degsum,
+                          out_ids, info));
+      break;
+    }
+    case RG_OP_BC_BACK: {
+      RG_TRY(rg::check_state(g, state, nstate, {8, 1}, "edgeMap(BC_Back_F)"));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E));
+      RG_TRY(rg::edge_map(E, rg::BcBackOp{state[0]->buf.as<double>(), state[1]->buf.as<uint8_t>()}, F, degsum,
+                          nullptr, nullptr));
+      break;
+    }
+    default:
+      return set_error(RG_EINVAL, "edgeMap: unknown operator");
+  }
+  if (want_out) {
+    int64_t h[2] = {0, 0};
+    if (op == RG_OP_BFS || op == RG_OP_RADII || op == RG_OP_BC_FWD) RG_TRY(rg::ctx_read_count(ctx, info, h, 2));
+    out->size = h[0];
+    out->degsum = h[1];
+    out->all = false;
+  } else if (out) {
+    out->size = 0;
+    out->degsum = 0;
+    out->all = false;
+  }
+  return RG_OK;
+}
+
+// vertexMap (ligra.h:464-479) with the reference's vertex functors.
+int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int nstate, double a, double b) {
+  if (!g || !s || s->g->n != g->n) return set_error(RG_EINVAL, "vertexMap: bad subset");
+  rg_ctx* ctx = g->ctx;
+  RG_HIP(hipSetDevice(ctx->device));
+  const int64_t cnt = s->all ? g->n : s->size;
+  if (cnt == 0) return RG_OK;
+  const int grid = rg::grid_for(ctx, cnt);
+  const uint32_t* ids = s->ids.as<uint32_t>();
+  switch (vop) {
+    case RG_VOP_PR_UPDATE:   // PR_Vertex_F (PageRank.C:46-58): p = a*p + b
+      RG_TRY(rg::check_state(g, state, nstate, {8}, "vertexMap(PR_Vertex_F)"));
+      hipLaunchKernelGGL(rg::k_pr_vertex_update, grid, 256, 0, ctx->stream, ids, cnt, s->all,
+                         state[0]->buf.as<double>(), a, b);
+      break;
+    case RG_VOP_PR_CONTRIB:  // contrib[v] = p[v] / outdeg(v), all vertices
+      RG_TRY(rg::check_state(g, state, nstate, {8, 8}, "vertexMap(contrib)"));
+      hipLaunchKernelGGL(rg::k_pr_contrib, rg::grid_for(ctx, g->n), 256, 0, ctx->stream, g->off, g->n,
+                         state[0]->buf.as<double>(), state[1]->buf.as<double>());
+      break;
+    case RG_VOP_RADII_SYNC:  // Radii_Vertex_F (Radii.C:67-75): Visited = NextVisited
+      RG_TRY(rg::check_state(g, state, nstate, {8, 8}, "vertexMap(Radii_Vertex_F)"));
+      hipLaunchKernelGGL(rg::k_sync_u64, grid, 256, 0, ctx->stream, ids, cnt, s->all,
+                         state[0]->buf.as<unsigned long long>(), state[1]->buf.as<unsigned long long>());
+      break;
+    case RG_VOP_MARK:        // BC_Vertex_F (BC-manyiter-v2.C:239-246): Visited = 1
+      RG_TRY(rg::check_state(g, state, nstate, {1}, "vertexMap(BC_Vertex_F)"));
+      hipLaunchKernelGGL(rg::k_mark_u8, grid, 256, 0, ctx->stream, ids, cnt, s->all, state[0]->buf.as<uint8_t>());
+      break;
+    case RG_VOP_BC_BACK:     // BC_Back_Vertex_F (BC-manyiter-v2.C:250-258)
+      RG_TRY(rg::check_state(g, state, nstate, {1, 8, 8}, "vertexMap(BC_Back_Vertex_F)"));
+      if (s->all)
+        hipLaunchKernelGGL(rg::k_bc_back_vertex_all, grid, 256, 0, ctx->stream, g->n, state[0]->buf.as<uint8_t>(),
+                           state[1]->buf.as<double>(), state[2]->buf.as<double>());
+      else
+        hipLaunchKernelGGL(rg::k_bc_back_vertex_l, grid, 256, 0, ctx->stream, ids, cnt, state[0]->buf.as<uint8_t>(),
+                           state[1]->buf.as<double>(), state[2]->buf.as<double>());
+      break;
+    case RG_VOP_INVERT:      // inverseNumPaths = 1/NumPaths, every vertex (BC-manyiter-v2.C:364-365)
+      RG_TRY(rg::check_state(g, state, nstate, {8}, "vertexMap(invert)"));
+      hipLaunchKernelGGL(rg::k_invert, rg::grid_for(ctx, g->n), 256, 0, ctx->stream, state[0]->buf.as<double>(),
+                         g->n);
+      break;
+    case RG_VOP_BC_FINAL:    // Dep = (Dep - inv)/inv, every vertex (BC-manyiter-v2.C:387-389)
+      RG_TRY(rg::check_state(g, state, nstate, {8, 8}, "vertexMap(BC final)"));
+      hipLaunchKernelGGL(rg::k_bc_final, rg::grid_for(ctx, g->n), 256, 0, ctx->stream, state[0]->buf.as<double>(),
+                         state[1]->buf.as<double>(), g->n);
+      break;
+    default:
+      return set_error(RG_EINVAL, "vertexMap: unknown operator");
+  }
+  RG_HIP(hipGetLastError());
+  return RG_OK;
 }
 
 }  // extern "C"

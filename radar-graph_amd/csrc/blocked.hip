@@ -220,8 +220,8 @@ int blocked_build(rg_graph* g, int vb, int sb, BlockedCSC** out) {
   RG_HIP(hipMalloc(&B->unit_lo, sizeof(uint64_t) * ulo.size()));
   RG_HIP(hipMalloc(&B->block_unit0, sizeof(uint32_t) * bu0.size()));
   RG_HIP(hipMalloc(&B->part, sizeof(double) * ((size_t)B->nunits << vb)));
-  RG_HIP(hipMemcpy(B->unit_lo, ulo.data(), sizeof(uint64_t) * ulo.size(), hipMemcpyHostToDevice));
-  RG_HIP(hipMemcpy(B->block_unit0, bu0.data(), sizeof(uint32_t) * bu0.size(), hipMemcpyHostToDevice));
+  RG_TRY(ctx_copy(ctx, B->unit_lo, ulo.data(), sizeof(uint64_t) * ulo.size(), hipMemcpyHostToDevice));
+  RG_TRY(ctx_copy(ctx, B->block_unit0, bu0.data(), sizeof(uint32_t) * bu0.size(), hipMemcpyHostToDevice));
   B->built_for = g->in_off;
   B->vb = vb;
   B->sb = sb;
@@ -357,11 +357,11 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   RG_TRY(ctx_read_count(ctx, done, &dn, 1));
   int64_t last = dn ? dn : maxiters;
   double l1 = 0.0;
-  if (last >= 1) RG_HIP(hipMemcpy(&l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
+  if (last >= 1) RG_TRY(ctx_copy(ctx, &l1, l1_hist + (last - 1), sizeof(double), hipMemcpyDeviceToHost));
   if (iters_out) *iters_out = dn ? dn : maxiters + 1;
   if (l1_out) *l1_out = l1;
   if (ms_out) *ms_out = ms;
-  if (ranks_out) RG_HIP(hipMemcpy(ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (ranks_out) RG_TRY(ctx_copy(ctx, ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 

@@ -110,6 +110,11 @@ int ctx_frontiers(rg_ctx* ctx, int64_t n, Frontier& a, Frontier& b);
 
 int ctx_read_count(rg_ctx* ctx, const int64_t* dev, int64_t* host, int nwords = 1);
 
+// Host<->device copy ordered on the context stream (the stream is
+// non-blocking, so a plain hipMemcpy on the null stream would not wait for
+// work queued on it); returns after the copy has completed.
+int ctx_copy(rg_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+
 // With timing on (rg_ctx_timing), records the start event of one edge-map
 // launch on the context stream and returns the event pair; the caller
 // records element [1] after the launch.  nullptr when timing is off.

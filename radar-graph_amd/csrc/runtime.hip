@@ -61,6 +61,13 @@ hipEvent_t* timing_begin(rg_ctx* ctx) {
   return p.e;
 }
 
+int ctx_copy(rg_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (bytes == 0) return RG_OK;
+  RG_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+  RG_HIP(hipStreamSynchronize(ctx->stream));
+  return RG_OK;
+}
+
 int ctx_read_count(rg_ctx* ctx, const int64_t* dev, int64_t* host, int nwords) {
   RG_HIP(hipMemcpyAsync(ctx->h_counts, dev, sizeof(int64_t) * nwords, hipMemcpyDeviceToHost,
                         ctx->stream));

@@ -197,6 +197,31 @@ int rg_pagerank(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_o
 int rg_bfs(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
            int64_t* iters_out, int64_t* reached_out, int64_t* edges_out, double* ms_out);
 
+/* Summary of one BFS query: iters = rounds with a nonempty frontier (the
+ * printed "Num. Iters", BFS.C:253), reached = tree size (BFS.C:254-257),
+ * edges = sum of out-degrees of the reached vertices (TEPS numerator),
+ * checksum = sum over reached v of rg_level_hash(v, level(v)) mod 2^64, which
+ * identifies the level array.  No counterpart in the reference beyond the
+ * first two (its BFS prints only those). */
+typedef struct rg_bfs_summary {
+  int64_t iters, reached, edges;
+  uint64_t checksum;
+} rg_bfs_summary;
+uint64_t rg_level_hash(uint32_t v, uint32_t level);
+
+/* One BFS query on the single-source engine (BFS.C:213-258 semantics, as
+ * rg_bfs), returning its summary. */
+int rg_bfs_query(rg_graph* g, uint32_t src, const rg_opts* o, rg_bfs_summary* out, double* ms_out);
+
+/* Batched BFS: 1..256 sources traversed together as a bit-parallel frontier
+ * (source i owns bit i of ceil(n_src/64) words per vertex; Radii's trick,
+ * Radii.C:35-75, generalised).  out[i] equals rg_bfs_query(sources[i]) for
+ * every i; levels_out (nullable, nsources * n int32, source-major) receives
+ * each source's level array (-1 = unreached).  o: threshold (sparse/dense
+ * level switch, < 0 = m/20), heavy_degree, wg_size. */
+int rg_bfs_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, int32_t* levels_out,
+                 rg_bfs_summary* out, double* ms_out);
+
 /* Radii (PNO Ligra/apps/Radii.C:98-157): bit-parallel BFS from up to 64
  * sources (source i owns bit i); radii[v] = last round in which v's visited
  * word changed, -1 if unreached.  rounds_out = printed "Iters". */

@@ -69,6 +69,33 @@ def bfs_levels(off, nbr, src):
     return lv, int(ecc)
 
 
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def level_hash(v, level):
+    """splitmix64's output function (Steele, Lea and Flood, OOPSLA 2014;
+    Vigna's public-domain splitmix64.c) applied to (level << 32 | v) + the
+    golden-ratio increment; vectorised over numpy arrays (uint64 wraps)."""
+    with np.errstate(over="ignore"):
+        z = (np.asarray(level, np.uint64) << np.uint64(32)) | np.asarray(v, np.uint64)
+        z = z + _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def bfs_summary(off, level):
+    """(iters, reached, edges, checksum) of a BFS level array: rounds with a
+    nonempty frontier (BFS.C:253), tree size (BFS.C:254-257), out-degree sum
+    of the reached vertices, and the sum of level_hash(v, level) mod 2^64."""
+    level = np.asarray(level)
+    reached = np.nonzero(level >= 0)[0]
+    deg = np.diff(np.asarray(off, np.uint64).astype(np.int64))
+    with np.errstate(over="ignore"):
+        cs = int(np.sum(level_hash(reached, level[reached]), dtype=np.uint64))
+    return (int(level.max()) + 1, len(reached), int(deg[reached].sum()), cs)
+
+
 def check_parents(off, nbr, src, level, parents):
     off, nbr = _c(off, np.uint64), _c(nbr, np.uint32)
     level, parents = _c(level, np.int32), _c(parents, np.uint32)

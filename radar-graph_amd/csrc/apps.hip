@@ -10,6 +10,8 @@
 
 #include "device.h"
 #include "engine.h"
+#include "engine_host.h"
+#include "queries.h"
 
 namespace rg {
 
@@ -189,106 +191,18 @@ __global__ void k_degsum(const uint32_t* ids, int64_t count, const uint64_t* off
   if (__lane_id() == 0 && s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
 }
 
-template <class T>
-__global__ void k_fill(T* p, int64_t n, T v) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
-
-template <class T>
-__global__ void k_set(T* p, int64_t i, T v) {
-  p[i] = v;
-}
-
-static int grid_for(rg_ctx* ctx, int64_t n, int block = 256) {
-  int64_t g = (n + block - 1) / block;
-  int64_t cap = (int64_t)ctx->num_cus * 16;
-  return (int)std::max<int64_t>(1, std::min(g, cap));
-}
-
-template <class T>
-static int fill(rg_ctx* ctx, T* p, int64_t n, T v) {
-  if (n <= 0) return RG_OK;
-  hipLaunchKernelGGL(k_fill<T>, grid_for(ctx, n), 256, 0, ctx->stream, p, n, v);
-  RG_HIP(hipGetLastError());
-  return RG_OK;
-}
-
-// ---------------------------------------------------------------------------
-// Engine configuration and the edgeMap dispatcher.
-
-struct Engine {
-  rg_ctx* ctx;
-  rg_graph* g;
-  int64_t threshold;   // Ligra: m/20 by default (ligra.h:269)
-  int block;           // 256 or 1024
-  int heavy_deg;
-  int hub_mode;        // HUB_NONE / HUB_BITMAP / HUB_PREFIX
-  int64_t hub_k;       // LDS slots
-  int64_t hub_k2;      // LDS + per-XCD slots
-  void* l2_copies;
-  int l2_ncopies;
-  DevBuf* heavy;       // heavy work queue
-  int64_t* heavy_count;
-};
-
-template <class Op, int MODE, int HUB, int BLOCK>
-static int launch_edge(const Engine& E, const Op& op, const EdgeArgs& a, size_t hub_bytes) {
-  constexpr int WPB = BLOCK / kWave;
-  auto kl = edge_light_kernel<Op, MODE, HUB, BLOCK>;
-  auto kh = edge_heavy_kernel<Op, HUB, BLOCK>;
-  static size_t attr_set = 0;
-  if (hub_bytes > 0 && hub_bytes > attr_set) {
-    RG_HIP(hipFuncSetAttribute((const void*)kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hub_bytes));
-    RG_HIP(hipFuncSetAttribute((const void*)kh, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hub_bytes));
-    attr_set = hub_bytes;
-  }
-  const int64_t nchunks = (a.count + kWave - 1) / kWave;
-  int64_t want = (nchunks + WPB - 1) / WPB;
-  // Persistent grid: enough workgroups to fill every CU; with hub copies, as
-  // many as the LDS allows per CU (each extra copy costs one merge).
-  int64_t per_cu;
-  if (HUB != HUB_NONE) {
-    size_t static_lds = WPB * ((kWave + 1) * 4 + kWave * (8 + 4 + 8));
-    per_cu = std::max<int64_t>(1, (int64_t)(E.ctx->lds_per_cu / (static_lds + hub_bytes)));
-    per_cu = std::min<int64_t>(per_cu, std::max(1, 32 / WPB));
-  } else {
-    per_cu = std::max(1, 64 / WPB);
-  }
-  int64_t cap = (int64_t)E.ctx->num_cus * per_cu;
-  int grid_l = (int)std::max<int64_t>(1, std::min(want, cap));
-  if (a.static_heavy) grid_l = (int)cap;   // the heavy phase needs the full grid
-  hipEvent_t* ev = timing_begin(E.ctx);
-  hipLaunchKernelGGL(kl, grid_l, BLOCK, hub_bytes, E.ctx->stream, op, a);
-  RG_HIP(hipGetLastError());
-  if (ev) RG_HIP(hipEventRecord(ev[1], E.ctx->stream));
-  if (!a.static_heavy) {
-    hipLaunchKernelGGL(kh, (int)cap, BLOCK, hub_bytes, E.ctx->stream, op, a);
-    RG_HIP(hipGetLastError());
-  }
-  if (Op::kL2 && HUB != HUB_NONE && a.hub_k2 > a.hub_k) {
-    int64_t span = a.hub_k2;
-    int g = (int)std::max<int64_t>(1, std::min<int64_t>((span + 255) / 256, (int64_t)E.ctx->num_cus * 8));
-    hipLaunchKernelGGL((k_l2_merge<Op, HUB>), g, 256, 0, E.ctx->stream, op, a);
-    RG_HIP(hipGetLastError());
-  }
-  return RG_OK;
-}
-
-template <class Op, int MODE, int HUB>
-static int launch_block(const Engine& E, const Op& op, const EdgeArgs& a, size_t hub_bytes) {
-  if (E.block >= 1024) return launch_edge<Op, MODE, HUB, 1024>(E, op, a, hub_bytes);
-  return launch_edge<Op, MODE, HUB, 256>(E, op, a, hub_bytes);
-}
-
-template <class Op, int MODE>
-static int launch_hub(const Engine& E, const Op& op, EdgeArgs a) {
-  size_t hub_bytes = (size_t)a.hub_k * sizeof(typename Op::HubT);
-  switch (E.hub_mode) {
-    case HUB_BITMAP: return launch_block<Op, MODE, HUB_BITMAP>(E, op, a, hub_bytes);
-    case HUB_PREFIX: return launch_block<Op, MODE, HUB_PREFIX>(E, op, a, hub_bytes);
-    default: a.hub_k = a.hub_k2 = 0; return launch_block<Op, MODE, HUB_NONE>(E, op, a, 0);
+// Static heavy list for the all-vertices frontier: every vertex with degree
+// > heavy_deg contributes ceil(deg / kHeavyChunk) items.
+__global__ void k_heavy_list(const uint64_t* off, int64_t n, int heavy_deg, uint64_t* items,
+                             int64_t* count) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t dd = off[v + 1] - off[v];
+    if (dd > (uint64_t)heavy_deg) {
+      int64_t nch = (int64_t)((dd + kHeavyChunk - 1) / kHeavyChunk);
+      int64_t base = atomicAdd((unsigned long long*)count, (unsigned long long)nch);
+      for (int64_t k = 0; k < nch; ++k) items[base + k] = ((uint64_t)v << 32) | (uint64_t)k;
+    }
   }
 }
 
@@ -454,7 +368,7 @@ static int build_hub_bitmap(rg_graph* g, int64_t k) {
 }
 
 // Resolves variant/opts into an Engine.  elem = bytes of one hub copy entry.
-static int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E) {
+int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E) {
   rg_ctx* ctx = g->ctx;
   E.ctx = ctx;
   E.g = g;
@@ -510,29 +424,6 @@ static int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E) {
   E.heavy_count = ctx->counts.as<int64_t>() + 0;
   return RG_OK;
 }
-
-// ---------------------------------------------------------------------------
-// Timing helper (device time of a region on the context stream).
-struct EventTimer {
-  hipEvent_t a = nullptr, b = nullptr;
-  hipStream_t s;
-  explicit EventTimer(hipStream_t st) : s(st) {
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
-    (void)hipEventRecord(a, s);
-  }
-  double stop_ms() {
-    float ms = 0.f;
-    (void)hipEventRecord(b, s);
-    (void)hipEventSynchronize(b);
-    (void)hipEventElapsedTime(&ms, a, b);
-    return ms;
-  }
-  ~EventTimer() {
-    if (a) (void)hipEventDestroy(a);
-    if (b) (void)hipEventDestroy(b);
-  }
-};
 
 // ---------------------------------------------------------------------------
 // PageRank (PageRank.C:92-141).
@@ -676,11 +567,20 @@ __global__ void k_bfs_edges(const uint32_t* parents, const uint64_t* off, int64_
   }
 }
 
-int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
-             int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev);
+// Sum of level_hash(v, level) over a frontier list whose size is on the
+// device (the level checksum of a BFS query).
+__global__ void k_level_cs(const uint32_t* ids, const int64_t* count, uint32_t level, unsigned long long* out) {
+  const int64_t c = *count;
+  unsigned long long s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < c; i += (int64_t)gridDim.x * blockDim.x)
+    s += level_hash(ids[i], level);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (__lane_id() == 0 && s) atomicAdd(out, s);
+}
 
 int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
-             int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev) {
+             int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev,
+             uint64_t* checksum_out) {
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
   if ((int64_t)src >= n) return set_error(RG_EINVAL, "rg_bfs: source out of range");
@@ -706,10 +606,14 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   Frontier* F = &fa;
   Frontier* N = &fb;
   int64_t iter = 0;
+  unsigned long long* cs = reinterpret_cast<unsigned long long*>(ctx->counts.as<int64_t>() + 12);
+  if (checksum_out) RG_HIP(hipMemsetAsync(cs, 0, sizeof(int64_t), ctx->stream));
   EventTimer tm(ctx->stream);
   while (F->size > 0) {
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
     RG_TRY(edge_map(E, op, *F, degsum, N->ids, info));
+    if (checksum_out)
+      hipLaunchKernelGGL(k_level_cs, ctx->num_cus, 256, 0, ctx->stream, N->ids, info, (uint32_t)(iter + 1), cs);
     int64_t h[2];
     RG_TRY(ctx_read_count(ctx, info, h, 2));
     N->size = h[0];
@@ -727,6 +631,11 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   if (reached_out) *reached_out = h[0];
   if (edges_out) *edges_out = h[1];
   if (ms_out) *ms_out = ms;
+  if (checksum_out) {
+    int64_t c = 0;
+    RG_TRY(ctx_read_count(ctx, reinterpret_cast<int64_t*>(cs), &c, 1));
+    *checksum_out = (uint64_t)c + level_hash(src, 0);
+  }
   if (parents_out) RG_TRY(rg::ctx_copy(ctx, parents_out, parents, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
@@ -1060,7 +969,16 @@ int rg_bfs(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, i
            int64_t* reached_out, int64_t* edges_out, double* ms_out) {
   if (!g) return set_error(RG_EINVAL, "rg_bfs: null graph");
   RG_HIP(hipSetDevice(g->ctx->device));
-  return rg::bfs_push(g, src, o, parents_out, iters_out, reached_out, edges_out, ms_out, nullptr);
+  return rg::bfs_push(g, src, o, parents_out, iters_out, reached_out, edges_out, ms_out, nullptr, nullptr);
+}
+
+int rg_bfs_query(rg_graph* g, uint32_t src, const rg_opts* o, rg_bfs_summary* out, double* ms_out) {
+  if (!g || !out) return rg::set_error(RG_EINVAL, "rg_bfs_query: null argument");
+  RG_HIP(hipSetDevice(g->ctx->device));
+  uint64_t cs = 0;
+  RG_TRY(rg::bfs_push(g, src, o, nullptr, &out->iters, &out->reached, &out->edges, ms_out, nullptr, &cs));
+  out->checksum = cs;
+  return RG_OK;
 }
 
 int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o, int32_t* radii_out,

@@ -1,0 +1,40 @@
+// Destination-blocked in-edge layout shared by the blocked PageRank engine
+// and the dense levels of the bit-parallel multi-source BFS.
+//
+// Destinations are cut into blocks of 2^vb vertices; the in-edges of a block
+// are sorted by source (segment) and stored as a 4-byte source id plus a
+// 2-byte local destination.  A block's edge range is split into units of at
+// most kUnitEdges edges, one workgroup each; a block with several units (the
+// hub blocks of a degree-sorted graph) gets one private LDS copy per unit,
+// merged afterwards.
+#pragma once
+
+#include <cstdint>
+
+#include "device.h"
+
+namespace rg {
+
+constexpr int64_t kUnitEdges = int64_t(1) << 19;
+
+struct BlockedCSC {
+  const uint64_t* built_for = nullptr;   // in_off of the orientation it was built from
+  int vb = 0, sb = 0;                    // destination-block / source-segment bits
+  int64_t nblocks = 0, nunits = 0;
+  int64_t nmulti = 0;                    // blocks with more than one unit
+  uint32_t* src = nullptr;         // m, sorted by (block, segment, local dst)
+  uint16_t* vl = nullptr;          // m, local destination
+  uint64_t* unit_lo = nullptr;     // nunits + 1 edge boundaries
+  uint32_t* block_unit0 = nullptr; // nblocks + 1: first unit of each block
+  uint32_t* unit_block = nullptr;  // nunits: block of each unit
+  uint32_t* multi = nullptr;       // nmulti: the blocks with more than one unit
+  double* part = nullptr;          // nunits << vb partial sums (PageRank only)
+  void release();
+};
+
+// Builds (or returns the cached) blocked layout of g's current in-edges in
+// the cache slot *slot (a member of rg_graph).  want_part allocates the
+// PageRank partial-sum array.
+int blocked_build(rg_graph* g, int vb, int sb, void** slot, bool want_part, BlockedCSC** out);
+
+}  // namespace rg

@@ -26,33 +26,27 @@
 #include <cstdlib>
 #include <vector>
 
+#include "blocked.h"
 #include "device.h"
 
 namespace rg {
 
 constexpr int kMaxVB = 14;                       // at most 16K destinations per block (128 KB LDS)
-constexpr int64_t kUnitEdges = int64_t(1) << 19; // edges per unit (work split)
 constexpr int kUnitThreads = 1024;
 
-struct BlockedCSC {
-  const uint64_t* built_for = nullptr;   // in_off of the orientation it was built from
-  int vb = 0, sb = 0;                    // destination-block / source-segment bits
-  int64_t nblocks = 0, nunits = 0;
-  uint32_t* src = nullptr;        // m, sorted by (block, segment, local dst)
-  uint16_t* vl = nullptr;         // m, local destination
-  uint64_t* unit_lo = nullptr;    // nunits + 1 edge boundaries
-  uint32_t* block_unit0 = nullptr;// nblocks + 1: first unit of each block
-  double* part = nullptr;         // nunits * kBlockV partial sums
-  void release() {
-    if (src) (void)hipFree(src);
-    if (vl) (void)hipFree(vl);
-    if (unit_lo) (void)hipFree(unit_lo);
-    if (block_unit0) (void)hipFree(block_unit0);
-    if (part) (void)hipFree(part);
-    src = nullptr; vl = nullptr; unit_lo = nullptr; block_unit0 = nullptr; part = nullptr;
-    built_for = nullptr;
-  }
-};
+void BlockedCSC::release() {
+  if (src) (void)hipFree(src);
+  if (vl) (void)hipFree(vl);
+  if (unit_lo) (void)hipFree(unit_lo);
+  if (block_unit0) (void)hipFree(block_unit0);
+  if (unit_block) (void)hipFree(unit_block);
+  if (multi) (void)hipFree(multi);
+  if (part) (void)hipFree(part);
+  src = nullptr; vl = nullptr; unit_lo = nullptr; block_unit0 = nullptr; unit_block = nullptr;
+  multi = nullptr; part = nullptr;
+  built_for = nullptr;
+  nmulti = 0;
+}
 
 void blocked_free(void* p) {
   BlockedCSC* b = static_cast<BlockedCSC*>(p);
@@ -152,20 +146,18 @@ int grid_n(rg_ctx* ctx, int64_t n) {
 
 }  // namespace
 
-// Builds (or returns the cached) blocked layout of g's in-edges with
-// destination blocks of 2^vb vertices and source segments of 2^sb.
-int blocked_build(rg_graph* g, int vb, int sb, BlockedCSC** out) {
-  BlockedCSC* B = static_cast<BlockedCSC*>(g->blocked);
-  if (B && B->built_for == g->in_off && B->vb == vb && B->sb == sb) {
+int blocked_build(rg_graph* g, int vb, int sb, void** slot, bool want_part, BlockedCSC** out) {
+  BlockedCSC* B = static_cast<BlockedCSC*>(*slot);
+  if (B && B->built_for == g->in_off && B->vb == vb && B->sb == sb && (B->part || !want_part)) {
     *out = B;
     return RG_OK;
   }
-  if (g->m > (int64_t)INT32_MAX) return set_error(RG_EUNSUPPORTED, "blocked PageRank: more than 2^31 arcs");
+  if (g->m > (int64_t)INT32_MAX) return set_error(RG_EUNSUPPORTED, "blocked layout: more than 2^31 arcs");
   rg_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
   if (!B) {
     B = new BlockedCSC;
-    g->blocked = B;
+    *slot = B;
   }
   B->release();
   const int64_t n = g->n, m = g->m;
@@ -217,11 +209,22 @@ int blocked_build(rg_graph* g, int vb, int sb, BlockedCSC** out) {
   B->nunits = (int64_t)ulo.size();
   ulo.push_back((uint64_t)m);
   B->nblocks = nblocks;
+  std::vector<uint32_t> ub(B->nunits), multi;
+  for (int64_t b = 0; b < nblocks; ++b) {
+    for (uint32_t u = bu0[b]; u < bu0[b + 1]; ++u) ub[u] = (uint32_t)b;
+    if (bu0[b + 1] - bu0[b] > 1) multi.push_back((uint32_t)b);
+  }
+  B->nmulti = (int64_t)multi.size();
   RG_HIP(hipMalloc(&B->unit_lo, sizeof(uint64_t) * ulo.size()));
   RG_HIP(hipMalloc(&B->block_unit0, sizeof(uint32_t) * bu0.size()));
-  RG_HIP(hipMalloc(&B->part, sizeof(double) * ((size_t)B->nunits << vb)));
+  RG_HIP(hipMalloc(&B->unit_block, sizeof(uint32_t) * ub.size()));
+  RG_HIP(hipMalloc(&B->multi, sizeof(uint32_t) * std::max<size_t>(1, multi.size())));
+  if (want_part) RG_HIP(hipMalloc(&B->part, sizeof(double) * ((size_t)B->nunits << vb)));
   RG_TRY(ctx_copy(ctx, B->unit_lo, ulo.data(), sizeof(uint64_t) * ulo.size(), hipMemcpyHostToDevice));
   RG_TRY(ctx_copy(ctx, B->block_unit0, bu0.data(), sizeof(uint32_t) * bu0.size(), hipMemcpyHostToDevice));
+  RG_TRY(ctx_copy(ctx, B->unit_block, ub.data(), sizeof(uint32_t) * ub.size(), hipMemcpyHostToDevice));
+  if (!multi.empty())
+    RG_TRY(ctx_copy(ctx, B->multi, multi.data(), sizeof(uint32_t) * multi.size(), hipMemcpyHostToDevice));
   B->built_for = g->in_off;
   B->vb = vb;
   B->sb = sb;
@@ -305,7 +308,7 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   const int sb = (o && o->src_segment_bits > 0 && o->src_segment_bits <= 24) ? o->src_segment_bits : 0;
   const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
   BlockedCSC* B = nullptr;
-  RG_TRY(blocked_build(g, vb, sb, &B));
+  RG_TRY(blocked_build(g, vb, sb, &g->blocked, true, &B));
   void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const double*, double*, const int64_t*);
   const bool u4 = o && o->unroll == 4;
 #define RG_PICK(VB)                                                                   \

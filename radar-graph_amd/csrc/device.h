@@ -62,6 +62,8 @@ struct rg_ctx {
   struct EvPair { hipEvent_t e[2]; };
   std::vector<EvPair> ev_pairs;
   size_t ev_used = 0;
+  // Multi-source BFS state (opaque, msbfs.hip), grown on demand.
+  void* ms = nullptr;
 };
 
 struct rg_graph {
@@ -88,8 +90,14 @@ struct rg_graph {
   int64_t heavy_items_count = 0;
   int heavy_items_deg = -1;
   const uint64_t* heavy_items_off = nullptr;
-  // Blocked in-edge layout for the blocked PageRank engine (opaque).
+  // Blocked in-edge layouts (opaque BlockedCSC): PageRank's (2^14-vertex
+  // blocks, with partial sums) and the multi-source BFS's (2^12).
   void* blocked = nullptr;
+  void* blocked_ms = nullptr;
+  // A view shares the device arrays and cached layouts of `base` (bound to
+  // another context, e.g. one serving slot) and frees only what it builds.
+  bool view = false;
+  const rg_graph* base = nullptr;
 };
 
 namespace rg {
@@ -122,5 +130,8 @@ hipEvent_t* timing_begin(rg_ctx* ctx);
 
 // Cached blocked layout of a graph's in-edges (blocked.hip).
 void blocked_free(void* p);
+
+// Multi-source BFS state of a context (msbfs.hip).
+void ms_state_free(void* p);
 
 }  // namespace rg

@@ -51,6 +51,7 @@ struct EdgeArgs {
   const uint64_t* off;
   const uint32_t* nbr;
   int64_t count;          // entries: n for SRC_ALL / SRC_DENSE, list size for SRC_LIST
+  const int64_t* count_dev; // SRC_LIST: list size read on the device (count is then a bound)
   const uint32_t* ids;    // SRC_LIST
   const uint8_t* dense;   // SRC_DENSE
   // next frontier (operators with kOutput)
@@ -253,7 +254,8 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
   const int lane = (int)__lane_id();
   const int w = threadIdx.x / kWave;
   WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0};
-  const int64_t nchunks = (a.count + kWave - 1) / kWave;
+  const int64_t count = a.count_dev ? *a.count_dev : a.count;
+  const int64_t nchunks = (count + kWave - 1) / kWave;
   const int64_t wave_id = (int64_t)blockIdx.x * WPB + w;
   const int64_t nwaves = (int64_t)gridDim.x * WPB;
   const bool static_heavy = a.static_heavy != nullptr;
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
     uint32_t v = 0;
     uint32_t deg = 0;
     uint64_t start = 0;
-    bool active = i < a.count;
+    bool active = i < count;
     if (active) {
       if (MODE == SRC_LIST) v = a.ids[i];
       else v = (uint32_t)i;
@@ -361,21 +363,6 @@ __global__ void k_l2_merge(Op op, EdgeArgs a) {
       }
     }
     if (any) op.hub_flush(hub_vertex<HUB>(a, slot), s);
-  }
-}
-
-// Static heavy list for the all-vertices frontier: every vertex with degree
-// > heavy_deg contributes ceil(deg / kHeavyChunk) items.
-__global__ void k_heavy_list(const uint64_t* off, int64_t n, int heavy_deg, uint64_t* items,
-                             int64_t* count) {
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t dd = off[v + 1] - off[v];
-    if (dd > (uint64_t)heavy_deg) {
-      int64_t nch = (int64_t)((dd + kHeavyChunk - 1) / kHeavyChunk);
-      int64_t base = atomicAdd((unsigned long long*)count, (unsigned long long)nch);
-      for (int64_t k = 0; k < nch; ++k) items[base + k] = ((uint64_t)v << 32) | (uint64_t)k;
-    }
   }
 }
 

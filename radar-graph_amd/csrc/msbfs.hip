@@ -1,0 +1,677 @@
+// Bit-parallel multi-source BFS: up to 256 single-source BFS queries
+// traversed together, source i owning bit i of W = 1, 2 or 4 64-bit words
+// per vertex.  This is Radii's visited-word trick (push-new-orig
+// Ligra/apps/Radii.C:35-75: NextVisited[d] |= Visited[s], one word, 64
+// sources) generalised to several words and to per-source results, so a
+// batch of BFS queries shares every edge scan.
+//
+// Per vertex: seen (bits of the sources that reached it), front (bits for
+// which it is on the current level; two buffers, current and next) and, for
+// sparse levels, next (bits arriving this level).  A level is either
+//   * sparse: the push engine (engine.h) over the frontier list with an
+//     operator that ORs front[s] & ~seen[d] into next[d] (global atomics,
+//     few at this frontier size), then a vertex pass over the touched list;
+//   * dense: the destination-blocked in-edge layout (blocked.h): one
+//     workgroup per unit ORs front[s] of active sources into an LDS copy of
+//     its block's words and finalises the block itself -- no global
+//     atomics; blocks split over several units (hubs) are merged by a
+//     second kernel.
+// Ligra's rule picks the level kind (|F| + sum of out-degrees > threshold,
+// ligra.h:269), on the device: every level launches the kernels of both
+// kinds and the ones not chosen exit at once, so the host reads the state
+// back only every few levels.
+//
+// Per source the batch yields the BFS summary of a single-source run:
+// rounds, vertices reached, the out-degree sum of the reached vertices and
+// a checksum of its level array (sum over reached v of mix(v, level) mod
+// 2^64), and optionally the level arrays themselves.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "blocked.h"
+#include "device.h"
+#include "engine.h"
+#include "engine_host.h"
+#include "queries.h"
+
+namespace rg {
+
+constexpr int kMsVB = 12;            // 4096 destinations per dense block
+constexpr int kMsThreads = 512;      // dense-level workgroup
+constexpr int kMsMaxSources = 256;
+constexpr int kMsLevelsPerSync = 4;  // levels launched between host checks
+
+enum { MS_SPARSE = 0, MS_DENSE = 1, MS_DONE = 2 };
+
+// Device control block of one batch.
+struct MsCtl {
+  int64_t mode;
+  int64_t level;          // level of the current frontier
+  int64_t fcount, fdeg;   // current frontier list: size, out-degree sum
+  int64_t ncount, ndeg;   // next frontier list (the engine's out_info)
+  int64_t threshold;
+  int64_t heavy_count;    // engine heavy queue
+  int64_t dense_levels, sparse_levels;
+};
+
+typedef unsigned long long ull;
+
+struct MsArgs {
+  MsCtl* ctl;
+  int64_t n;
+  int nsrc;
+  const uint64_t* off;                 // out-offsets (degrees)
+  // blocked in-edges
+  const uint32_t* src;
+  const uint16_t* vl;
+  const uint64_t* unit_lo;
+  const uint32_t* unit_block;
+  const uint32_t* block_unit0;
+  const uint32_t* multi;
+  // per-vertex words
+  ull* seen;
+  const ull* front_cur;
+  ull* front_nxt;
+  ull* next;
+  const ull* active_cur;               // bit per vertex: front_cur != 0
+  ull* active_nxt;
+  const uint32_t* list_cur;
+  uint32_t* list_nxt;
+  ull* part;                           // nunits << VB words * W (split blocks)
+  // per source
+  ull* lvl_new;
+  ull* edges;
+  ull* cs;
+  int32_t* levels;                     // optional [nsrc][n]
+};
+
+// ---------------------------------------------------------------------------
+// Sparse level: the push engine's operator.
+
+template <int W>
+struct MsWords {
+  ull w[W];
+};
+
+template <int W>
+struct MsPushOp {
+  using SrcT = MsWords<W>;
+  using HubT = uint32_t;
+  static constexpr bool kOutput = true;
+  static constexpr bool kL2 = false;
+  const ull* front;
+  const ull* seen;
+  ull* next;
+  uint32_t* stamp;
+  uint32_t tag;
+  const int64_t* mode;
+  __device__ bool skip() const { return *mode != MS_SPARSE; }
+  __device__ SrcT load_src(uint32_t s) const {
+    SrcT r;
+#pragma unroll
+    for (int w = 0; w < W; ++w) r.w[w] = front[(size_t)s * W + w];
+    return r;
+  }
+  __device__ bool cond(uint32_t) const { return true; }
+  // next[d] |= front[s] & ~seen[d]; d joins the touched list once per level.
+  __device__ bool update(uint32_t, SrcT sv, uint32_t d) const {
+    bool any = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      ull x = sv.w[w] & ~seen[(size_t)d * W + w];
+      if (x) {
+        atomicOr(next + (size_t)d * W + w, x);
+        any = true;
+      }
+    }
+    if (!any || stamp[d] == tag) return false;
+    return atomicExch(stamp + d, tag) != tag;
+  }
+  __device__ uint32_t hub_identity() const { return 0; }
+  __device__ void hub_update(uint32_t*, uint32_t, SrcT, uint32_t) const {}
+  __device__ bool hub_flush(uint32_t, uint32_t) const { return false; }
+  template <class V>
+  __device__ void l2_update(uint32_t*, const V&) const {}
+  __device__ uint32_t hub_combine(uint32_t x, uint32_t) const { return x; }
+};
+
+// ---------------------------------------------------------------------------
+// Per-source statistics staged in LDS, flushed once per workgroup.
+
+template <int W>
+struct MsStats {
+  uint32_t cnt[64 * W];
+  ull deg[64 * W];
+  ull cs[64 * W];
+};
+
+template <int W, int THREADS>
+__device__ __forceinline__ void stats_init(MsStats<W>& st) {
+  for (int i = threadIdx.x; i < 64 * W; i += THREADS) {
+    st.cnt[i] = 0;
+    st.deg[i] = 0;
+    st.cs[i] = 0;
+  }
+}
+
+template <int W, int THREADS>
+__device__ __forceinline__ void stats_flush(const MsArgs& a, MsStats<W>& st) {
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.nsrc; i += THREADS) {
+    if (st.cnt[i]) {
+      atomicAdd(a.lvl_new + i, (ull)st.cnt[i]);
+      atomicAdd(a.edges + i, st.deg[i]);
+      atomicAdd(a.cs + i, st.cs[i]);
+    }
+  }
+}
+
+// Records vertex d, reached at `level` by the sources in nn.
+template <int W>
+__device__ __forceinline__ void stats_add(const MsArgs& a, MsStats<W>& st, uint32_t d, uint32_t level,
+                                          const ull* nn, ull deg) {
+  const ull h = level_hash(d, level);
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    ull bits = nn[w];
+    while (bits) {
+      const int s = w * 64 + (__ffsll((long long)bits) - 1);
+      bits &= bits - 1;
+      atomicAdd(&st.cnt[s], 1u);
+      atomicAdd(&st.deg[s], deg);
+      atomicAdd(&st.cs[s], h);
+      if (a.levels) a.levels[(size_t)s * a.n + d] = (int32_t)level;
+    }
+  }
+}
+
+// Sparse level, vertex pass over the touched list (= the next frontier):
+// nn = next[d] (already disjoint from seen), seen |= nn, front_nxt = nn.
+template <int W>
+__global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
+  __shared__ MsStats<W> st;
+  if (a.ctl->mode != MS_SPARSE) return;
+  stats_init<W, 256>(st);
+  __syncthreads();
+  const int64_t count = a.ctl->ncount;
+  const uint32_t level = (uint32_t)a.ctl->level + 1;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = a.list_nxt[i];
+    ull nn[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const size_t k = (size_t)d * W + w;
+      nn[w] = a.next[k];
+      a.next[k] = 0;
+      if (nn[w]) {
+        a.seen[k] |= nn[w];
+        a.front_nxt[k] = nn[w];
+      }
+    }
+    atomicOr(a.active_nxt + (d >> 6), 1ull << (d & 63));
+    stats_add<W>(a, st, d, level, nn, a.off[d + 1] - a.off[d]);
+  }
+  stats_flush<W, 256>(a, st);
+}
+
+// ---------------------------------------------------------------------------
+// Dense level.
+
+// Finalises destination d (valid lanes) from its OR-accumulated words:
+// seen, front_nxt (every vertex of the block is written), the active bit
+// word (lane 0 of each wave; the 64 lanes hold 64 aligned vertices), the
+// next-frontier list and the statistics.  Called by whole waves.
+template <int W>
+__device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
+                                            int64_t d, bool valid, const ull* accw, uint32_t level) {
+  ull nn[W];
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    nn[w] = 0;
+    if (valid) {
+      const size_t k = (size_t)d * W + w;
+      const ull sn = a.seen[k];
+      nn[w] = accw[w] & ~sn;
+      a.front_nxt[k] = nn[w];
+      if (nn[w]) a.seen[k] = sn | nn[w];
+    }
+    any |= nn[w] != 0;
+  }
+  const ull word = __ballot(any);
+  if (__lane_id() == 0 && d < a.n) a.active_nxt[d >> 6] = word;
+  ull deg = 0;
+  if (any) {
+    deg = a.off[d + 1] - a.off[d];
+    stats_add<W>(a, st, (uint32_t)d, level, nn, deg);
+  }
+  wq_push(q, ea, any, (uint32_t)d);
+}
+
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+template <int W, int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a) {
+  constexpr int BV = 1 << VB;
+  constexpr int NW = THREADS / kWave;
+  __shared__ ull acc[BV * W];
+  __shared__ ull full[BV / 64];
+  __shared__ uint32_t s_q[NW][kQueue];
+  __shared__ MsStats<W> st;
+  if (a.ctl->mode != MS_DENSE) return;
+  const int64_t u = blockIdx.x;
+  const uint32_t b = a.unit_block[u];
+  const int64_t base = (int64_t)b << VB;
+  const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
+  const uint32_t level = (uint32_t)a.ctl->level + 1;
+  for (int i = threadIdx.x; i < BV * W; i += THREADS) acc[i] = 0;
+  stats_init<W, THREADS>(st);
+  // Destinations every source of the batch has reached need no gathers.
+  ull allbits[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int rem = a.nsrc - 64 * w;
+    allbits[w] = rem >= 64 ? ~0ull : rem <= 0 ? 0ull : ((1ull << rem) - 1);
+  }
+  for (int l = threadIdx.x; l < BV; l += THREADS) {
+    const int64_t d = base + l;
+    bool f = true;
+    if (d < a.n) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) f = f && (a.seen[(size_t)d * W + w] & allbits[w]) == allbits[w];
+    }
+    const ull word = __ballot(f);
+    if (__lane_id() == 0) full[l >> 6] = word;
+  }
+  __syncthreads();
+  const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
+  for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
+    const uint32_t s = ld_nt(a.src + e);
+    const uint32_t l = ld_nt(a.vl + e);
+    if ((full[l >> 6] >> (l & 63)) & 1) continue;
+    if (!((a.active_cur[s >> 6] >> (s & 63)) & 1)) continue;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const ull f = a.front_cur[(size_t)s * W + w];
+      if (f) atomicOr(&acc[l * W + w], f);
+    }
+  }
+  __syncthreads();
+  if (!single) {
+    ull* out = a.part + ((size_t)u << VB) * W;
+    for (int i = threadIdx.x; i < BV * W; i += THREADS) out[i] = acc[i];
+    return;
+  }
+  EdgeArgs ea{};
+  ea.off = a.off;
+  ea.out_ids = a.list_nxt;
+  ea.out_info = &a.ctl->ncount;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  for (int l0 = 0; l0 < BV; l0 += THREADS) {
+    const int l = l0 + threadIdx.x;
+    const int64_t d = base + l;
+    ms_finalize<W>(a, st, q, ea, d, d < a.n, &acc[l * W], level);
+  }
+  wq_finish(q, ea);
+  stats_flush<W, THREADS>(a, st);
+}
+
+// Blocks split over several units: OR the units' partial words, finalise.
+// Grid: nmulti * (BV / THREADS) workgroups.
+template <int W, int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
+  constexpr int BV = 1 << VB;
+  constexpr int NW = THREADS / kWave;
+  constexpr int PER = BV / THREADS;
+  __shared__ uint32_t s_q[NW][kQueue];
+  __shared__ MsStats<W> st;
+  if (a.ctl->mode != MS_DENSE) return;
+  const uint32_t b = a.multi[blockIdx.x / PER];
+  const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
+  const int64_t d = ((int64_t)b << VB) + l;
+  const uint32_t level = (uint32_t)a.ctl->level + 1;
+  stats_init<W, THREADS>(st);
+  __syncthreads();
+  ull accw[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) accw[w] = 0;
+  for (uint32_t u = a.block_unit0[b]; u < a.block_unit0[b + 1]; ++u) {
+    const ull* p = a.part + (((size_t)u << VB) + l) * W;
+#pragma unroll
+    for (int w = 0; w < W; ++w) accw[w] |= p[w];
+  }
+  EdgeArgs ea{};
+  ea.off = a.off;
+  ea.out_ids = a.list_nxt;
+  ea.out_info = &a.ctl->ncount;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  ms_finalize<W>(a, st, q, ea, d, d < a.n, accw, level);
+  wq_finish(q, ea);
+  stats_flush<W, THREADS>(a, st);
+}
+
+// ---------------------------------------------------------------------------
+// Level bookkeeping.
+
+// After a level: clear the current frontier's words (the buffer becomes the
+// next level's output) and its active bits.
+template <int W>
+__global__ void k_ms_clear(MsArgs a, int64_t nwords) {
+  if (a.ctl->mode == MS_DONE) return;
+  const int64_t count = a.ctl->fcount;
+  ull* fc = const_cast<ull*>(a.front_cur);
+  ull* ac = const_cast<ull*>(a.active_cur);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = a.list_cur[i];
+#pragma unroll
+    for (int w = 0; w < W; ++w) fc[(size_t)v * W + w] = 0;
+  }
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwords;
+       i += (int64_t)gridDim.x * blockDim.x)
+    ac[i] = 0;
+}
+
+// Closes a level: per-source reached counts and depths, the next frontier
+// becomes current, and the next level's kind is chosen.
+__global__ void k_ms_level_end(MsArgs a, ull* reached, int64_t* depth) {
+  MsCtl* c = a.ctl;
+  if (c->mode == MS_DONE) return;
+  const int64_t level = c->level + 1;
+  for (int s = threadIdx.x; s < a.nsrc; s += blockDim.x) {
+    const ull k = a.lvl_new[s];
+    if (k) {
+      reached[s] += k;
+      depth[s] = level;
+      a.lvl_new[s] = 0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (c->mode == MS_DENSE) c->dense_levels++;
+    else c->sparse_levels++;
+    c->level = level;
+    c->fcount = c->ncount;
+    c->fdeg = c->ndeg;
+    c->ncount = 0;
+    c->ndeg = 0;
+    c->heavy_count = 0;
+    c->mode = c->fcount == 0 ? MS_DONE : (c->fcount + c->fdeg > c->threshold ? MS_DENSE : MS_SPARSE);
+  }
+}
+
+// Batch start: source bits, per-source statistics, the first frontier
+// (unique source vertices, given in list_cur) and the first level's kind.
+__global__ void k_ms_init(MsArgs a, int W, const uint32_t* sources, int64_t nuniq, int64_t threshold,
+                          ull* reached, int64_t* depth) {
+  __shared__ ull degsum;
+  if (threadIdx.x == 0) degsum = 0;
+  __syncthreads();
+  ull* fc = const_cast<ull*>(a.front_cur);
+  ull* ac = const_cast<ull*>(a.active_cur);
+  for (int s = threadIdx.x; s < a.nsrc; s += blockDim.x) {
+    const uint32_t v = sources[s];
+    const ull bit = 1ull << (s & 63);
+    atomicOr(a.seen + (size_t)v * W + (s >> 6), bit);
+    atomicOr(fc + (size_t)v * W + (s >> 6), bit);
+    atomicOr(ac + (v >> 6), 1ull << (v & 63));
+    reached[s] = 1;
+    depth[s] = 0;
+    a.edges[s] = a.off[v + 1] - a.off[v];
+    a.cs[s] = level_hash(v, 0);
+    a.lvl_new[s] = 0;
+    if (a.levels) a.levels[(size_t)s * a.n + v] = 0;
+  }
+  for (int64_t i = threadIdx.x; i < nuniq; i += blockDim.x) {
+    const uint32_t v = a.list_cur[i];
+    atomicAdd(&degsum, (ull)(a.off[v + 1] - a.off[v]));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    MsCtl* c = a.ctl;
+    c->level = 0;
+    c->fcount = nuniq;
+    c->fdeg = (int64_t)degsum;
+    c->ncount = c->ndeg = 0;
+    c->threshold = threshold;
+    c->heavy_count = 0;
+    c->dense_levels = c->sparse_levels = 0;
+    c->mode = c->fcount + c->fdeg > threshold ? MS_DENSE : MS_SPARSE;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+// Per-context state, grown on demand.  Between batches every buffer except
+// seen is all zero (each level clears what it used), so a batch only
+// clears its n * W words of seen; `dirty` forces a full clear after a
+// failed batch.  The word buffers are sized for the widest batch so far
+// (a narrower batch uses their first n * W words).
+struct MsState {
+  DevBuf seen, front[2], next, stamp, active[2], list[2], part, stats, ctl, srcs;
+  int64_t n = 0;
+  int wcap = 0;
+  uint32_t tag = 0;
+  bool dirty = true;
+};
+
+void ms_state_free(void* p) { delete static_cast<MsState*>(p); }
+
+namespace {
+
+template <class T>
+int zero(rg_ctx* ctx, DevBuf& b) {
+  if (b.bytes) RG_TRY(fill<T>(ctx, b.as<T>(), (int64_t)(b.bytes / sizeof(T)), T(0)));
+  return RG_OK;
+}
+
+int ms_prepare(rg_ctx* ctx, MsState& S, int64_t n, int W, int64_t nunits) {
+  if (S.n != n || W > S.wcap) {
+    const int wcap = std::max(W, S.n == n ? S.wcap : 0);
+    const size_t words = (size_t)n * wcap * sizeof(ull);
+    const size_t aw = sizeof(ull) * (size_t)(n / 64 + 1);
+    for (DevBuf* b : {&S.seen, &S.front[0], &S.front[1], &S.next, &S.stamp, &S.active[0], &S.active[1],
+                      &S.list[0], &S.list[1]})
+      b->release();
+    RG_TRY(S.seen.ensure(words));
+    RG_TRY(S.front[0].ensure(words));
+    RG_TRY(S.front[1].ensure(words));
+    RG_TRY(S.next.ensure(words));
+    RG_TRY(S.stamp.ensure(sizeof(uint32_t) * n));
+    RG_TRY(S.active[0].ensure(aw));
+    RG_TRY(S.active[1].ensure(aw));
+    RG_TRY(S.list[0].ensure(sizeof(uint32_t) * n));
+    RG_TRY(S.list[1].ensure(sizeof(uint32_t) * n));
+    S.n = n;
+    S.wcap = wcap;
+    S.dirty = true;
+  }
+  const size_t pb = ((size_t)std::max<int64_t>(nunits, 1) << kMsVB) * W * sizeof(ull);
+  if (S.part.bytes < pb) {
+    S.part.release();
+    RG_TRY(S.part.ensure(pb));
+  }
+  RG_TRY(S.stats.ensure(sizeof(ull) * 5 * kMsMaxSources));
+  RG_TRY(S.ctl.ensure(sizeof(MsCtl)));
+  RG_TRY(S.srcs.ensure(sizeof(uint32_t) * kMsMaxSources));
+  if (S.dirty || S.tag > 0x7fffffffu) {
+    for (DevBuf* b : {&S.front[0], &S.front[1], &S.next, &S.active[0], &S.active[1]}) RG_TRY(zero<ull>(ctx, *b));
+    RG_TRY(zero<uint32_t>(ctx, S.stamp));
+    S.tag = 0;
+    S.dirty = false;
+  }
+  return RG_OK;
+}
+
+template <int W>
+int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32_t* sources, int nsrc,
+           int64_t threshold, int32_t* levels_dev, int64_t* levels_run) {
+  rg_ctx* ctx = g->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = g->n;
+  const int64_t nwords = n / 64 + 1;
+  RG_TRY(fill<ull>(ctx, S.seen.as<ull>(), n * W, 0ull));
+  // Unique source vertices form the first frontier list.
+  std::vector<uint32_t> uniq(sources, sources + nsrc);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  RG_HIP(hipMemcpyAsync(S.srcs.p, sources, sizeof(uint32_t) * nsrc, hipMemcpyHostToDevice, st));
+  RG_HIP(hipMemcpyAsync(S.list[0].p, uniq.data(), sizeof(uint32_t) * uniq.size(), hipMemcpyHostToDevice, st));
+  ull* stats = S.stats.as<ull>();
+  MsArgs a{};
+  a.ctl = S.ctl.as<MsCtl>();
+  a.n = n;
+  a.nsrc = nsrc;
+  a.off = g->off;
+  a.src = B->src;
+  a.vl = B->vl;
+  a.unit_lo = B->unit_lo;
+  a.unit_block = B->unit_block;
+  a.block_unit0 = B->block_unit0;
+  a.multi = B->multi;
+  a.seen = S.seen.as<ull>();
+  a.next = S.next.as<ull>();
+  a.part = S.part.as<ull>();
+  a.lvl_new = stats;
+  a.edges = stats + kMsMaxSources;
+  a.cs = stats + 2 * kMsMaxSources;
+  ull* reached = stats + 3 * kMsMaxSources;
+  int64_t* depth = reinterpret_cast<int64_t*>(stats + 4 * kMsMaxSources);
+  a.levels = levels_dev;
+  int cur = 0;
+  auto bind = [&](int c) {
+    a.front_cur = S.front[c].as<ull>();
+    a.front_nxt = S.front[c ^ 1].as<ull>();
+    a.active_cur = S.active[c].as<ull>();
+    a.active_nxt = S.active[c ^ 1].as<ull>();
+    a.list_cur = S.list[c].as<uint32_t>();
+    a.list_nxt = S.list[c ^ 1].as<uint32_t>();
+  };
+  bind(cur);
+  S.dirty = true;   // until the batch completes
+  hipLaunchKernelGGL(k_ms_init, 1, 256, 0, st, a, W, S.srcs.as<uint32_t>(), (int64_t)uniq.size(), threshold,
+                     reached, depth);
+  RG_HIP(hipGetLastError());
+
+  EdgeArgs ea{};
+  ea.off = g->off;
+  ea.nbr = g->nbr;
+  ea.count = n;                     // bound; the list size is read on the device
+  ea.count_dev = &a.ctl->fcount;
+  ea.out_info = &a.ctl->ncount;
+  ea.heavy = E.heavy->as<uint64_t>();
+  ea.heavy_count = &a.ctl->heavy_count;
+  ea.heavy_deg = E.heavy_deg;
+  const int gfin = std::max(1, ctx->num_cus * 4);
+  const int gclr = grid_for(ctx, std::max<int64_t>(nwords, 1));
+  constexpr int PER = (1 << kMsVB) / kMsThreads;
+  int64_t steps = 0;
+  for (;;) {
+    for (int k = 0; k < kMsLevelsPerSync; ++k) {
+      ++steps;
+      MsPushOp<W> op{a.front_cur, a.seen, a.next, S.stamp.as<uint32_t>(), S.tag + (uint32_t)steps, &a.ctl->mode};
+      ea.ids = a.list_cur;
+      ea.out_ids = a.list_nxt;
+      RG_TRY((launch_hub<MsPushOp<W>, SRC_LIST>(E, op, ea)));
+      hipLaunchKernelGGL(k_ms_sparse_fin<W>, gfin, 256, 0, st, a);
+      hipLaunchKernelGGL((k_ms_dense<W, kMsVB, kMsThreads>), (int)B->nunits, kMsThreads, 0, st, a);
+      if (B->nmulti)
+        hipLaunchKernelGGL((k_ms_merge<W, kMsVB, kMsThreads>), (int)(B->nmulti * PER), kMsThreads, 0, st, a);
+      hipLaunchKernelGGL(k_ms_clear<W>, gclr, 256, 0, st, a, nwords);
+      hipLaunchKernelGGL(k_ms_level_end, 1, 256, 0, st, a, reached, depth);
+      RG_HIP(hipGetLastError());
+      cur ^= 1;
+      bind(cur);
+    }
+    int64_t mode = 0;
+    RG_TRY(ctx_read_count(ctx, &a.ctl->mode, &mode, 1));
+    if (mode == MS_DONE) break;
+    if (steps > n + 8) return set_error(RG_EDEVICE, "multi-source BFS: no convergence");
+  }
+  S.tag += (uint32_t)steps + 1;
+  S.dirty = false;
+  if (levels_run) *levels_run = steps;
+  return RG_OK;
+}
+
+}  // namespace
+
+int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
+              rg_bfs_summary* out, double* ms_out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  if (nsrc < 1 || nsrc > kMsMaxSources)
+    return set_error(RG_EINVAL, "rg_bfs_batch: between 1 and 256 sources per batch");
+  if (!sources || !out) return set_error(RG_EINVAL, "rg_bfs_batch: null argument");
+  for (int i = 0; i < nsrc; ++i)
+    if ((int64_t)sources[i] >= n) return set_error(RG_EINVAL, "rg_bfs_batch: source out of range");
+  if (n >= (int64_t)1 << 32) return set_error(RG_EUNSUPPORTED, "rg_bfs_batch: n >= 2^32");
+  const int W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
+  BlockedCSC* B = nullptr;
+  RG_TRY(blocked_build(g, kMsVB, 0, &g->blocked_ms, false, &B));
+  Engine E;
+  rg_opts po;
+  rg_opts_default(&po);
+  if (o) {
+    po.heavy_degree = o->heavy_degree;
+    po.wg_size = o->wg_size;
+  }
+  po.variant = RG_PUSH;
+  RG_TRY(make_engine(g, &po, sizeof(uint32_t), E));
+  const int64_t threshold = (o && o->threshold >= 0) ? o->threshold : g->m / 20;
+  MsState* S = static_cast<MsState*>(ctx->ms);
+  if (!S) {
+    S = new MsState;
+    ctx->ms = S;
+  }
+  RG_TRY(ms_prepare(ctx, *S, n, W, B->nunits));
+  DevBuf lv;
+  int32_t* levels_dev = nullptr;
+  if (levels_out) {
+    RG_TRY(lv.ensure(sizeof(int32_t) * (size_t)nsrc * n));
+    levels_dev = lv.as<int32_t>();
+    RG_TRY(fill<int32_t>(ctx, levels_dev, (int64_t)nsrc * n, -1));
+  }
+  EventTimer tm(ctx->stream);
+  int64_t steps = 0;
+  int rc = W == 1   ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, &steps)
+           : W == 2 ? ms_run<2>(g, *S, B, E, sources, nsrc, threshold, levels_dev, &steps)
+                    : ms_run<4>(g, *S, B, E, sources, nsrc, threshold, levels_dev, &steps);
+  if (rc != RG_OK) return rc;
+  double ms = tm.stop_ms();
+  std::vector<ull> h(5 * kMsMaxSources);
+  RG_TRY(ctx_copy(ctx, h.data(), S->stats.p, sizeof(ull) * h.size(), hipMemcpyDeviceToHost));
+  for (int s = 0; s < nsrc; ++s) {
+    out[s].iters = (int64_t)h[4 * kMsMaxSources + s] + 1;
+    out[s].reached = (int64_t)h[3 * kMsMaxSources + s];
+    out[s].edges = (int64_t)h[kMsMaxSources + s];
+    out[s].checksum = h[2 * kMsMaxSources + s];
+  }
+  if (ms_out) *ms_out = ms;
+  if (levels_out)
+    RG_TRY(ctx_copy(ctx, levels_out, levels_dev, sizeof(int32_t) * (size_t)nsrc * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+}  // namespace rg
+
+extern "C" {
+
+int rg_bfs_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, int32_t* levels_out,
+                 rg_bfs_summary* out, double* ms_out) {
+  if (!g) return rg::set_error(RG_EINVAL, "rg_bfs_batch: null graph");
+  RG_HIP(hipSetDevice(g->ctx->device));
+  return rg::bfs_batch(g, sources, nsources, o, levels_out, out, ms_out);
+}
+
+uint64_t rg_level_hash(uint32_t v, uint32_t level) { return rg::level_hash(v, level); }
+
+}  // extern "C"

@@ -1,0 +1,34 @@
+// Internal entry points of the query paths (single-source BFS, batched
+// multi-source BFS, BC, PageRank) shared by the C ABI, the autotuner and the
+// serving loop, and the per-level hash of the BFS level checksum.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device.h"
+
+namespace rg {
+
+// splitmix64 finaliser (Steele, Lea, Flood 2014) of (level << 32 | v); a BFS
+// query's checksum is the sum of level_hash(v, level(v)) over the reached
+// vertices, modulo 2^64.
+__host__ __device__ inline uint64_t level_hash(uint32_t v, uint32_t level) {
+  uint64_t z = (((uint64_t)level << 32) | v) + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
+             int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev,
+             uint64_t* checksum_out);
+
+int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
+              rg_bfs_summary* out, double* ms_out);
+
+int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
+            int64_t* combined_out, double* ms_out);
+
+}  // namespace rg

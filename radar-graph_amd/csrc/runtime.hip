@@ -129,6 +129,8 @@ int rg_ctx_destroy(rg_ctx* ctx) {
   ctx->fr_ids[0].release();
   ctx->fr_ids[1].release();
   ctx->fr_dense.release();
+  rg::ms_state_free(ctx->ms);
+  ctx->ms = nullptr;
   for (auto& p : ctx->ev_pairs) {
     (void)hipEventDestroy(p.e[0]);
     (void)hipEventDestroy(p.e[1]);
@@ -273,16 +275,22 @@ int rg_graph_transpose(rg_graph* g) {
 int rg_graph_free(rg_graph* g) {
   if (!g) return RG_OK;
   if (g->ctx) (void)hipSetDevice(g->ctx->device);
-  if (g->in_off && g->in_off != g->off) (void)hipFree(g->in_off);
-  if (g->in_nbr && g->in_nbr != g->nbr) (void)hipFree(g->in_nbr);
-  if (g->off) (void)hipFree(g->off);
-  if (g->nbr) (void)hipFree(g->nbr);
-  if (g->hub_bits) (void)hipFree(g->hub_bits);
-  if (g->hub_rank) (void)hipFree(g->hub_rank);
-  if (g->hub_ids) (void)hipFree(g->hub_ids);
-  if (g->hub_slot_of) (void)hipFree(g->hub_slot_of);
-  if (g->heavy_items) (void)hipFree(g->heavy_items);
-  rg::blocked_free(g->blocked);
+  const rg_graph* b = g->view ? g->base : nullptr;
+  // A view frees only what it built itself (not shared with its base).
+  auto own = [&](const void* p, const void* shared) { return p && (!b || p != shared); };
+  if (!b) {
+    if (g->in_off && g->in_off != g->off) (void)hipFree(g->in_off);
+    if (g->in_nbr && g->in_nbr != g->nbr) (void)hipFree(g->in_nbr);
+    if (g->off) (void)hipFree(g->off);
+    if (g->nbr) (void)hipFree(g->nbr);
+  }
+  if (own(g->hub_bits, b ? b->hub_bits : nullptr)) (void)hipFree(g->hub_bits);
+  if (own(g->hub_rank, b ? b->hub_rank : nullptr)) (void)hipFree(g->hub_rank);
+  if (own(g->hub_ids, b ? b->hub_ids : nullptr)) (void)hipFree(g->hub_ids);
+  if (own(g->hub_slot_of, b ? b->hub_slot_of : nullptr)) (void)hipFree(g->hub_slot_of);
+  if (own(g->heavy_items, b ? b->heavy_items : nullptr)) (void)hipFree(g->heavy_items);
+  if (own(g->blocked, b ? b->blocked : nullptr)) rg::blocked_free(g->blocked);
+  if (own(g->blocked_ms, b ? b->blocked_ms : nullptr)) rg::blocked_free(g->blocked_ms);
   delete g;
   return RG_OK;
 }

@@ -55,6 +55,16 @@ class Opts(C.Structure):
                 ("unroll", C.c_int)]
 
 
+class BfsSummary(C.Structure):
+    """rg_bfs_summary: rounds, vertices reached, out-degree sum of the
+    reached vertices, level-array checksum."""
+    _fields_ = [("iters", C.c_int64), ("reached", C.c_int64), ("edges", C.c_int64),
+                ("checksum", C.c_uint64)]
+
+    def astuple(self):
+        return (self.iters, self.reached, self.edges, self.checksum)
+
+
 def _sig(name, res, *args):
     f = getattr(lib, name)
     f.restype = res
@@ -93,6 +103,9 @@ _sig("rg_graph_free", C.c_int, _vp)
 _sig("rg_opts_default", None, C.POINTER(Opts))
 _sig("rg_pagerank", C.c_int, _vp, C.c_int64, C.POINTER(Opts), _f64p, _i64p, _f64p, _f64p)
 _sig("rg_bfs", C.c_int, _vp, C.c_uint32, C.POINTER(Opts), _u32p, _i64p, _i64p, _i64p, _f64p)
+_sig("rg_level_hash", C.c_uint64, C.c_uint32, C.c_uint32)
+_sig("rg_bfs_query", C.c_int, _vp, C.c_uint32, C.POINTER(Opts), C.POINTER(BfsSummary), _f64p)
+_sig("rg_bfs_batch", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _i32p, C.POINTER(BfsSummary), _f64p)
 _sig("rg_radii", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _i32p, _i64p, _f64p)
 _sig("rg_bc", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _f64p, _i64p, _f64p)
 
@@ -153,6 +166,10 @@ def generate_edges(scale: int, degree: int, uniform: bool = False):
 
 def hash_int(a: int) -> int:
     return int(lib.rg_hash_int(C.c_uint64(a)))
+
+
+def level_hash(v: int, level: int) -> int:
+    return int(lib.rg_level_hash(v, level))
 
 
 def radii_starts(n: int, k: int = 64):
@@ -363,6 +380,25 @@ class DeviceGraph:
         _check(lib.rg_bfs(self._h, int(src), C.byref(o), _ptr(par, _u32p) if want_parents else None,
                           C.byref(it), C.byref(r), C.byref(e), C.byref(ms)))
         return BFSResult(par, it.value, r.value, e.value, ms.value)
+
+    def bfs_query(self, src, o: Opts = None):
+        """One BFS query on the single-source engine: (BfsSummary, ms)."""
+        o = o or opts()
+        out, ms = BfsSummary(), C.c_double()
+        _check(lib.rg_bfs_query(self._h, int(src), C.byref(o), C.byref(out), C.byref(ms)))
+        return out, ms.value
+
+    def bfs_batch(self, sources, o: Opts = None, want_levels=False):
+        """Bit-parallel batch of BFS queries: (list of BfsSummary, levels
+        [nsources, n] int32 or None, ms)."""
+        o = o or opts()
+        src = np.ascontiguousarray(sources, np.uint32)
+        out = (BfsSummary * len(src))()
+        lv = np.empty((len(src), self.n), np.int32) if want_levels else None
+        ms = C.c_double()
+        _check(lib.rg_bfs_batch(self._h, _ptr(src, _u32p), len(src), C.byref(o),
+                                _ptr(lv, _i32p) if want_levels else None, out, C.byref(ms)))
+        return list(out), lv, ms.value
 
     def radii(self, starts, o: Opts = None):
         o = o or opts()

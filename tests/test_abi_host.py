@@ -121,3 +121,32 @@ def test_bc_picker_alternates_orientation(rg):
 
 def test_device_count_without_gpu(rg):
     assert rg.device_count() >= 0
+
+
+# splitmix64 known answers (published outputs of Vigna's splitmix64.c for
+# seeds 1234567 and 0): output k of a generator seeded with x is the
+# finaliser of x + k * 0x9E3779B97F4A7C15, i.e. level_hash(v, level) with
+# (level << 32 | v) = x + (k - 1) * 0x9E3779B97F4A7C15.
+SPLITMIX = [(1234567, [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                       4593380528125082431, 16408922859458223821]),
+            (0, [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC])]
+
+
+@pytest.mark.parametrize("seed,outs", SPLITMIX)
+def test_level_hash_known_answers(rg, orc, seed, outs):
+    g = 0x9E3779B97F4A7C15
+    for k, want in enumerate(outs):
+        x = (seed + k * g) & ((1 << 64) - 1)
+        v, lv = x & 0xFFFFFFFF, x >> 32
+        assert rg.level_hash(v, lv) == want
+        assert int(orc.level_hash(v, lv)) == want
+
+
+def test_bfs_summary_by_hand(orc):
+    # Path 0 -> 1 -> 2 plus an isolated vertex 3: levels 0, 1, 2, -1.
+    off = np.array([0, 1, 2, 2, 2], np.uint64)
+    lv = np.array([0, 1, 2, -1], np.int32)
+    it, reached, edges, cs = orc.bfs_summary(off, lv)
+    assert (it, reached, edges) == (3, 3, 2)
+    want = sum(int(orc.level_hash(v, l)) for v, l in [(0, 0), (1, 1), (2, 2)]) % (1 << 64)
+    assert cs == want

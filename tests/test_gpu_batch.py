@@ -1,0 +1,124 @@
+"""GPU tests of the batched (bit-parallel multi-source) BFS: every source's
+level array equals the oracle's BFS levels bit-for-bit, and every batched
+summary equals the single-source engine's answer for the same query
+(rg_bfs_query) and the summary computed from the oracle levels."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BIG = 1 << 60
+
+
+def pick(n, k, seed):
+    r = np.random.default_rng(seed)
+    return r.integers(0, n, size=k).astype(np.uint32)
+
+
+def check_batch(rg, orc, hg, dg, srcs, o=None):
+    print("batch", dg.n, dg.m, len(srcs), flush=True)
+    off, nb = hg.csr()
+    out, lv, _ = dg.bfs_batch(srcs, o, want_levels=True)
+    for i, s in enumerate(srcs):
+        ref, _ = orc.bfs_levels(off, nb, int(s))
+        assert np.array_equal(lv[i], ref), f"source {s} (slot {i})"
+        assert out[i].astuple() == orc.bfs_summary(off, ref), f"source {s}"
+    return out
+
+
+@pytest.mark.parametrize("name", ["rmat12s", "rmat12d", "er12s", "er12d"])
+@pytest.mark.parametrize("k", [1, 7, 64, 65, 130, 256])
+def test_batch_levels_match_oracle(rg, orc, ctx, graphs, name, k):
+    hg = graphs(name)
+    dg = rg.DeviceGraph(ctx, hg)
+    check_batch(rg, orc, hg, dg, pick(hg.n, k, 100 + k))
+
+
+@pytest.mark.parametrize("threshold", [0, BIG])
+@pytest.mark.parametrize("k", [64, 200])
+def test_batch_all_dense_or_all_sparse(rg, orc, ctx, graphs, threshold, k):
+    # threshold 0 forces every level onto the blocked dense kernel, a huge
+    # one every level onto the push engine.
+    hg = graphs("rmat12d")
+    dg = rg.DeviceGraph(ctx, hg)
+    check_batch(rg, orc, hg, dg, pick(hg.n, k, 7), rg.opts(threshold=threshold))
+
+
+def test_batch_duplicates_and_isolated(rg, orc, ctx, graphs):
+    # Repeated sources share vertices in the first frontier; isolated
+    # sources finish at level 0.
+    hg = graphs("rmat12s")
+    dg = rg.DeviceGraph(ctx, hg)
+    deg = np.diff(hg.offsets())
+    iso = np.nonzero(deg == 0)[0][:3].astype(np.uint32)
+    srcs = np.concatenate([np.full(5, 17, np.uint32), iso, pick(hg.n, 40, 3), np.full(3, 17, np.uint32)])
+    out = check_batch(rg, orc, hg, dg, srcs)
+    for i in range(len(iso)):
+        assert out[5 + i].iters == 1 and out[5 + i].reached == 1
+
+
+def test_batch_equals_single_queries(rg, orc, ctx, graphs):
+    hg = graphs("rmat14d")
+    dg = rg.DeviceGraph(ctx, hg)
+    srcs = hg.pick_sources(96)
+    out, _, _ = dg.bfs_batch(srcs)
+    for i, s in enumerate(srcs):
+        single, _ = dg.bfs_query(int(s))
+        assert out[i].astuple() == single.astuple(), f"source {s}"
+
+
+def test_batch_split_hub_block(rg, orc, ctx):
+    # A star whose hub has 2^20 in-edges: its destination block is split
+    # over several workgroups whose partial words are merged.
+    leaves = 1 << 20
+    src = np.zeros(leaves, np.uint32)
+    dst = np.arange(1, leaves + 1, dtype=np.uint32)
+    # a few extra edges between leaves so levels go past 2
+    src = np.concatenate([src, dst[:1000]])
+    dst = np.concatenate([dst, dst[1000:2000]])
+    hg = rg.HostGraph.from_edges(leaves + 1, src, dst, symmetric=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    srcs = np.array([0, 1, 5, 1500, 999, 77777, leaves], np.uint32)
+    check_batch(rg, orc, hg, dg, srcs, rg.opts(threshold=0))
+    check_batch(rg, orc, hg, dg, srcs)
+
+
+def test_batch_degree_sorted(rg, orc, ctx, graphs):
+    # RADAR layout: hubs are ids 0.., the first destination blocks are split.
+    hg = graphs("rmat16s")
+    dg = rg.DeviceGraph(ctx, hg)
+    sg, ids = dg.degree_sort()
+    srcs = hg.pick_sources(128)
+    out_s, _, _ = sg.bfs_batch(ids[srcs])
+    out_u, _, _ = dg.bfs_batch(srcs)
+    off, nb = hg.csr()
+    for i, s in enumerate(srcs[:16]):
+        ref, _ = orc.bfs_levels(off, nb, int(s))
+        it, reached, edges, _ = orc.bfs_summary(off, ref)
+        assert out_u[i].astuple() == orc.bfs_summary(off, ref)
+        # relabelling permutes vertex ids, so the checksum differs; the
+        # rest of the summary is invariant
+        assert (out_s[i].iters, out_s[i].reached, out_s[i].edges) == (it, reached, edges)
+
+
+def test_batch_repeat_reuses_state(rg, orc, ctx, graphs):
+    # Consecutive batches of different widths on one context: the state
+    # left by a batch is reused, so answers must not depend on the order.
+    hg = graphs("rmat12d")
+    dg = rg.DeviceGraph(ctx, hg)
+    a = pick(hg.n, 150, 11)
+    b = pick(hg.n, 20, 12)
+    first, _, _ = dg.bfs_batch(a)
+    dg.bfs_batch(b)
+    dg.bfs_batch(pick(hg.n, 256, 13))
+    again, _, _ = dg.bfs_batch(a)
+    assert [x.astuple() for x in first] == [x.astuple() for x in again]
+
+
+def test_batch_rejects_bad_input(rg, ctx, graphs):
+    hg = graphs("rmat10s")
+    dg = rg.DeviceGraph(ctx, hg)
+    with pytest.raises(rg.RGError):
+        dg.bfs_batch(np.zeros(257, np.uint32))
+    with pytest.raises(rg.RGError):
+        dg.bfs_batch(np.array([hg.n], np.uint32))

@@ -288,6 +288,57 @@ int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int n
  *   RG_VOP_BC_FINAL   {deps, inv}: deps = (deps - inv)/inv, every vertex (BC-manyiter-v2.C:387-389) */
 int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int nstate, double a, double b);
 
+/* ---- query serving ---------------------------------------------------------
+ * A request loop over a resident graph (no counterpart in the reference,
+ * whose apps run one Compute per process): queries arrive at given times,
+ * `slots` device contexts (one stream each, sharing the graph's arrays)
+ * execute them concurrently, and with batching on, every free slot takes
+ * all BFS queries that have arrived (up to max_batch) as one bit-parallel
+ * batch (rg_bfs_batch) -- continuous batching: queries that arrive while
+ * others run join the next batch on the next free slot.  BC and PageRank
+ * queries run one per slot.  The number of slots is capped by free HBM
+ * (less hbm_reserve) over a per-slot estimate.  one_at_a_time = 1 runs the
+ * same trace on one slot, one query at a time, BFS on the single-source
+ * engine (rg_bfs_query): the baseline.
+ * Answers: BFS an rg_bfs_summary (equal to rg_bfs_query's); BC (one source,
+ * BC-manyiter-v2.C semantics for that source, orientation restored after)
+ * and PageRank (`iters` iterations, PageRank.C) a summary of the result
+ * array: wsum = sum over finite x[v] of x[v] * w(v), w(v) in [0,1) a fixed
+ * hash of v, and the count of non-finite entries. */
+enum { RG_Q_BFS = 0, RG_Q_BC = 1, RG_Q_PAGERANK = 2 };
+typedef struct rg_query {
+  int kind;
+  uint32_t source;       /* BFS, BC */
+  int32_t iters;         /* PageRank (maxiters) */
+  double arrival_ms;     /* offset from the start of the run */
+} rg_query;
+typedef struct rg_query_result {
+  int kind, slot;
+  int64_t batch_id, batch_size;
+  rg_bfs_summary bfs;
+  int64_t iters;         /* PageRank: printed iterations; BC: combined rounds */
+  double l1;             /* PageRank: last L1 */
+  double wsum;
+  int64_t nonfinite;
+  double arrival_ms, start_ms, end_ms, latency_ms;
+} rg_query_result;
+typedef struct rg_serve_config {
+  int slots;             /* concurrent device contexts (<= 0: 2) */
+  int max_batch;         /* BFS queries per batch (1..256, <= 0: 256) */
+  int one_at_a_time;
+  int64_t hbm_reserve;   /* bytes of device memory left free (<= 0: 4 GiB) */
+  rg_opts opts;          /* engine options for every query */
+} rg_serve_config;
+typedef struct rg_serve_stats {
+  int64_t queries, batches;
+  int slots;
+  double makespan_ms, qps, p50_ms, p99_ms, mean_ms, mean_batch;
+  int64_t hbm_budget, slot_bytes;
+} rg_serve_stats;
+void rg_serve_config_default(rg_serve_config* c);
+int rg_serve(rg_graph* g, const rg_serve_config* c, const rg_query* queries, int64_t nqueries,
+             rg_query_result* results, rg_serve_stats* stats);
+
 /* ---- autotuner -------------------------------------------------------------
  * Times candidate configurations of `app` on g and writes the fastest to
  * *best (the reference fixes these: llcCap = 1<<25 for the hub count, hubdup

@@ -208,7 +208,7 @@ __global__ void k_heavy_list(const uint64_t* off, int64_t n, int heavy_deg, uint
 
 // Builds (once per out-CSR and heavy degree) the chunk list of the vertices
 // whose degree exceeds heavy_deg, for all-vertices frontiers.
-static int static_heavy_list(rg_graph* g, int heavy_deg) {
+int static_heavy_list(rg_graph* g, int heavy_deg) {
   if (g->heavy_items && g->heavy_items_deg == heavy_deg && g->heavy_items_off == g->off) return RG_OK;
   rg_ctx* ctx = g->ctx;
   if (g->heavy_items) (void)hipFree(g->heavy_items);
@@ -542,6 +542,7 @@ int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks
   if (iters_out) *iters_out = dn ? dn : maxiters + 1;
   if (l1_out) *l1_out = l1;
   if (ms_out) *ms_out = ms;
+  ctx->last_result = rank;
   if (ranks_out) RG_TRY(rg::ctx_copy(ctx, ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
@@ -836,6 +837,7 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
   (void)was_transposed;
   if (combined_out) *combined_out = combined;
   if (ms_out) *ms_out = ms;
+  ctx->last_result = deps;
   if (deps_out) RG_TRY(rg::ctx_copy(ctx, deps_out, deps, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
@@ -933,6 +935,7 @@ int pagerank_pull(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks
   if (iters_out) *iters_out = dn ? dn : maxiters + 1;
   if (l1_out) *l1_out = l1;
   if (ms_out) *ms_out = ms;
+  ctx->last_result = rank;
   if (ranks_out) RG_TRY(rg::ctx_copy(ctx, ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }

@@ -28,13 +28,13 @@ struct BlockedCSC {
   uint32_t* block_unit0 = nullptr; // nblocks + 1: first unit of each block
   uint32_t* unit_block = nullptr;  // nunits: block of each unit
   uint32_t* multi = nullptr;       // nmulti: the blocks with more than one unit
-  double* part = nullptr;          // nunits << vb partial sums (PageRank only)
   void release();
 };
 
 // Builds (or returns the cached) blocked layout of g's current in-edges in
-// the cache slot *slot (a member of rg_graph).  want_part allocates the
-// PageRank partial-sum array.
-int blocked_build(rg_graph* g, int vb, int sb, void** slot, bool want_part, BlockedCSC** out);
+// the cache slot *slot (a member of rg_graph).  The layout is read-only once
+// built, so contexts sharing the graph may use it at the same time; the
+// per-unit partial results live in each context's scratch.
+int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out);
 
 }  // namespace rg

@@ -41,9 +41,8 @@ void BlockedCSC::release() {
   if (block_unit0) (void)hipFree(block_unit0);
   if (unit_block) (void)hipFree(unit_block);
   if (multi) (void)hipFree(multi);
-  if (part) (void)hipFree(part);
   src = nullptr; vl = nullptr; unit_lo = nullptr; block_unit0 = nullptr; unit_block = nullptr;
-  multi = nullptr; part = nullptr;
+  multi = nullptr;
   built_for = nullptr;
   nmulti = 0;
 }
@@ -146,9 +145,9 @@ int grid_n(rg_ctx* ctx, int64_t n) {
 
 }  // namespace
 
-int blocked_build(rg_graph* g, int vb, int sb, void** slot, bool want_part, BlockedCSC** out) {
+int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out) {
   BlockedCSC* B = static_cast<BlockedCSC*>(*slot);
-  if (B && B->built_for == g->in_off && B->vb == vb && B->sb == sb && (B->part || !want_part)) {
+  if (B && B->built_for == g->in_off && B->vb == vb && B->sb == sb) {
     *out = B;
     return RG_OK;
   }
@@ -219,7 +218,6 @@ int blocked_build(rg_graph* g, int vb, int sb, void** slot, bool want_part, Bloc
   RG_HIP(hipMalloc(&B->block_unit0, sizeof(uint32_t) * bu0.size()));
   RG_HIP(hipMalloc(&B->unit_block, sizeof(uint32_t) * ub.size()));
   RG_HIP(hipMalloc(&B->multi, sizeof(uint32_t) * std::max<size_t>(1, multi.size())));
-  if (want_part) RG_HIP(hipMalloc(&B->part, sizeof(double) * ((size_t)B->nunits << vb)));
   RG_TRY(ctx_copy(ctx, B->unit_lo, ulo.data(), sizeof(uint64_t) * ulo.size(), hipMemcpyHostToDevice));
   RG_TRY(ctx_copy(ctx, B->block_unit0, bu0.data(), sizeof(uint32_t) * bu0.size(), hipMemcpyHostToDevice));
   RG_TRY(ctx_copy(ctx, B->unit_block, ub.data(), sizeof(uint32_t) * ub.size(), hipMemcpyHostToDevice));
@@ -297,18 +295,33 @@ __global__ void k_pr_finish_b(const double* partial, int nparts, double* l1_hist
 
 }  // namespace
 
+namespace {
+// Destination-block bits (default 2^14 vertices) and source-segment bits:
+// 1..24 -> 2^bits sources per segment; 0 (default) or < 0 = order by source
+// id itself (measured best: units then sweep the source range in step and
+// their reads share L2 lines).
+void layout_bits(const rg_opts* o, int* vb, int* sb) {
+  *vb = (o && o->dst_block_bits >= 10 && o->dst_block_bits <= kMaxVB) ? o->dst_block_bits : kMaxVB;
+  *sb = (o && o->src_segment_bits > 0 && o->src_segment_bits <= 24) ? o->src_segment_bits : 0;
+}
+}  // namespace
+
+int pagerank_prepare(rg_graph* g, const rg_opts* o) {
+  int vb, sb;
+  layout_bits(o, &vb, &sb);
+  BlockedCSC* B = nullptr;
+  return blocked_build(g, vb, sb, &g->blocked, &B);
+}
+
 int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
                      int64_t* iters_out, double* l1_out, double* ms_out) {
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
-  const int vb = (o && o->dst_block_bits >= 10 && o->dst_block_bits <= kMaxVB) ? o->dst_block_bits : kMaxVB;
-  // src_segment_bits: 1..24 -> 2^bits sources per segment; 0 (default) or
-  // < 0 = order by source id itself (measured best: units then sweep the
-  // source range in step and their reads share L2 lines).
-  const int sb = (o && o->src_segment_bits > 0 && o->src_segment_bits <= 24) ? o->src_segment_bits : 0;
+  int vb, sb;
+  layout_bits(o, &vb, &sb);
   const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
   BlockedCSC* B = nullptr;
-  RG_TRY(blocked_build(g, vb, sb, &g->blocked, true, &B));
+  RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B));
   void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const double*, double*, const int64_t*);
   const bool u4 = o && o->unroll == 4;
 #define RG_PICK(VB)                                                                   \
@@ -320,9 +333,12 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   else if (vb == 11) RG_PICK(11);
   else RG_PICK(10);
 #undef RG_PICK
-  DevBuf &b_rank = ctx->scratch[0], &b_contrib = ctx->scratch[1], &b_part = ctx->scratch[3];
+  DevBuf &b_rank = ctx->scratch[0], &b_contrib = ctx->scratch[1], &b_part = ctx->scratch[3],
+         &b_unit = ctx->scratch[2];
   RG_TRY(b_rank.ensure(sizeof(double) * n));
   RG_TRY(b_contrib.ensure(sizeof(double) * n));
+  RG_TRY(b_unit.ensure(sizeof(double) * ((size_t)B->nunits << vb)));
+  double* unit_part = b_unit.as<double>();
   const int vgrid = std::min<int>(grid_n(ctx, n), 1024);
   RG_TRY(b_part.ensure(sizeof(double) * (vgrid + std::max<int64_t>(maxiters, 1) + 8)));
   double* rank = b_rank.as<double>();
@@ -343,9 +359,9 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   for (int64_t it = 1; it <= maxiters; ++it) {
     hipEvent_t* ev = timing_begin(ctx);
     hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, contrib,
-                       B->part, done);
+                       unit_part, done);
     if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
-    hipLaunchKernelGGL(k_pr_vertex_b, vgrid, 256, 0, ctx->stream, g->off, n, rank, contrib, B->part,
+    hipLaunchKernelGGL(k_pr_vertex_b, vgrid, 256, 0, ctx->stream, g->off, n, rank, contrib, unit_part,
                        B->block_unit0, vb, damping, added, partial, done);
     hipLaunchKernelGGL(k_pr_finish_b, 1, 256, 0, ctx->stream, partial, vgrid, l1_hist, it, done, epsilon);
     RG_HIP(hipGetLastError());
@@ -364,6 +380,7 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   if (iters_out) *iters_out = dn ? dn : maxiters + 1;
   if (l1_out) *l1_out = l1;
   if (ms_out) *ms_out = ms;
+  ctx->last_result = rank;
   if (ranks_out) RG_TRY(ctx_copy(ctx, ranks_out, rank, sizeof(double) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }

@@ -64,6 +64,9 @@ struct rg_ctx {
   size_t ev_used = 0;
   // Multi-source BFS state (opaque, msbfs.hip), grown on demand.
   void* ms = nullptr;
+  // Device array holding the last PageRank ranks / BC dependencies computed
+  // on this context (n doubles; valid until the next call on the context).
+  const double* last_result = nullptr;
 };
 
 struct rg_graph {

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "device.h"
 #include "engine.h"
@@ -61,11 +62,15 @@ static int launch_edge(const Engine& E, const Op& op, const EdgeArgs& a, size_t 
   constexpr int WPB = BLOCK / kWave;
   auto kl = edge_light_kernel<Op, MODE, HUB, BLOCK>;
   auto kh = edge_heavy_kernel<Op, HUB, BLOCK>;
-  static size_t attr_set = 0;
-  if (hub_bytes > 0 && hub_bytes > attr_set) {
+  // Largest dynamic LDS size set on these kernels so far (contexts may
+  // launch from several host threads).
+  static std::atomic<size_t> attr_set{0};
+  if (hub_bytes > 0 && hub_bytes > attr_set.load()) {
     RG_HIP(hipFuncSetAttribute((const void*)kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hub_bytes));
     RG_HIP(hipFuncSetAttribute((const void*)kh, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hub_bytes));
-    attr_set = hub_bytes;
+    size_t cur = attr_set.load();
+    while (hub_bytes > cur && !attr_set.compare_exchange_weak(cur, hub_bytes)) {
+    }
   }
   const int64_t nchunks = (a.count + kWave - 1) / kWave;
   int64_t want = (nchunks + WPB - 1) / WPB;

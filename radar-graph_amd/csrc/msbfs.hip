@@ -604,6 +604,17 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
 
 }  // namespace
 
+int bfs_batch_prepare(rg_graph* g) {
+  BlockedCSC* B = nullptr;
+  return blocked_build(g, kMsVB, 0, &g->blocked_ms, &B);
+}
+
+int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc) {
+  const int64_t W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
+  const int64_t units = (n >> kMsVB) + m / kUnitEdges + 2;
+  return 4 * n * W * 8 + (units << kMsVB) * W * 8 + n * 12 + n / 4 + 4096;
+}
+
 int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
               rg_bfs_summary* out, double* ms_out) {
   rg_ctx* ctx = g->ctx;
@@ -616,7 +627,7 @@ int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
   if (n >= (int64_t)1 << 32) return set_error(RG_EUNSUPPORTED, "rg_bfs_batch: n >= 2^32");
   const int W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
   BlockedCSC* B = nullptr;
-  RG_TRY(blocked_build(g, kMsVB, 0, &g->blocked_ms, false, &B));
+  RG_TRY(blocked_build(g, kMsVB, 0, &g->blocked_ms, &B));
   Engine E;
   rg_opts po;
   rg_opts_default(&po);

@@ -28,6 +28,17 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
 int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
               rg_bfs_summary* out, double* ms_out);
 
+// Builds the blocked in-edge layout the batches use (once per graph).
+int bfs_batch_prepare(rg_graph* g);
+// Device bytes one context holds for batches of nsrc sources on an
+// (n, m) graph.
+int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc);
+
+// Builds the blocked PageRank layout for these options (once per graph).
+int pagerank_prepare(rg_graph* g, const rg_opts* o);
+// Builds the all-vertices heavy-chunk list of the push engine.
+int static_heavy_list(rg_graph* g, int heavy_deg);
+
 int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
             int64_t* combined_out, double* ms_out);
 

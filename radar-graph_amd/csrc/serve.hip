@@ -1,0 +1,374 @@
+// Query serving over a resident graph: a request loop that admits BFS, BC
+// and PageRank queries by arrival time, runs them on several device
+// contexts ("slots", one stream each) that share the graph's arrays, and
+// coalesces the BFS queries that have arrived into bit-parallel batches
+// (continuous batching: a query that arrives while batches are running
+// joins the batch the next free slot starts).  The reference runs one
+// Compute per process and has no such loop; the per-query semantics are its
+// apps' (BFS.C, BC-manyiter-v2.C for one source, PageRank.C).
+//
+// Host side only: one std::thread per slot pulls work from shared pending
+// queues under a mutex and drives the synchronous query entry points on
+// its own context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "blocked.h"
+#include "device.h"
+#include "engine_host.h"
+#include "queries.h"
+
+namespace rg {
+
+namespace {
+
+constexpr int kSumBlocks = 512;
+
+// Weighted sum of the finite entries (weights w(v) in [0,1) from a fixed
+// hash of v) and count of the non-finite ones: per-block partial sums in a
+// fixed tree order, summed on the host in block order -- deterministic for
+// a given array.
+__global__ __launch_bounds__(256) void k_fp_summary(const double* x, int64_t n, double* part,
+                                                    unsigned long long* nonfinite) {
+  __shared__ double red[256];
+  double s = 0.0;
+  unsigned long long nf = 0;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const double xv = x[v];
+    if (!isfinite(xv)) {
+      nf++;
+    } else {
+      const double w = (double)(level_hash((uint32_t)v, 0xffffffffu) >> 11) * 0x1p-53;
+      s += xv * w;
+    }
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  for (int o = 32; o > 0; o >>= 1) nf += __shfl_xor(nf, o);
+  if (__lane_id() == 0 && nf) atomicAdd(nonfinite, nf);
+}
+
+int fp_summary(rg_ctx* ctx, const double* x, int64_t n, double* wsum, int64_t* nonfinite) {
+  if (!x) return set_error(RG_EDEVICE, "serve: no result array");
+  DevBuf& b = ctx->scratch[5];
+  RG_TRY(b.ensure(sizeof(double) * (kSumBlocks + 1)));
+  double* part = b.as<double>();
+  unsigned long long* nf = reinterpret_cast<unsigned long long*>(part + kSumBlocks);
+  RG_TRY(fill<unsigned long long>(ctx, nf, 1, 0ull));
+  hipLaunchKernelGGL(k_fp_summary, kSumBlocks, 256, 0, ctx->stream, x, n, part, nf);
+  RG_HIP(hipGetLastError());
+  std::vector<double> h(kSumBlocks + 1);
+  RG_TRY(ctx_copy(ctx, h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  double s = 0.0;
+  for (int i = 0; i < kSumBlocks; ++i) s += h[i];
+  uint64_t c;
+  std::memcpy(&c, &h[kSumBlocks], sizeof(c));
+  *wsum = s;
+  *nonfinite = (int64_t)c;
+  return RG_OK;
+}
+
+struct Slot {
+  rg_ctx* ctx = nullptr;
+  rg_graph* view = nullptr;
+};
+
+struct Shared {
+  const rg_query* q;
+  int64_t nq;
+  std::vector<int64_t> order;      // queries by arrival time
+  int64_t next = 0;                // next query (in order) not yet arrived
+  std::deque<int64_t> pending[3];
+  int64_t batches = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::chrono::steady_clock::time_point t0;
+  int err = RG_OK;
+  std::string msg;
+  double ms_now() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+// Picks the next work item: the kind whose oldest pending query arrived
+// first; all pending BFS queries (up to max_batch) when batching.  Returns
+// false when every query has been taken.
+bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64_t>& work, int64_t* batch) {
+  std::unique_lock<std::mutex> lk(S.mu);
+  for (;;) {
+    if (S.err != RG_OK) return false;
+    const double now = S.ms_now();
+    while (S.next < S.nq && S.q[S.order[S.next]].arrival_ms <= now) {
+      const int64_t i = S.order[S.next++];
+      S.pending[S.q[i].kind].push_back(i);
+    }
+    int best = -1;
+    double ba = 0.0;
+    for (int k = 0; k < 3; ++k)
+      if (!S.pending[k].empty()) {
+        const double a = S.q[S.pending[k].front()].arrival_ms;
+        if (best < 0 || a < ba) {
+          best = k;
+          ba = a;
+        }
+      }
+    if (best >= 0) {
+      size_t take = 1;
+      if (best == RG_Q_BFS && !c.one_at_a_time) take = std::min<size_t>(S.pending[best].size(), (size_t)c.max_batch);
+      for (size_t i = 0; i < take; ++i) {
+        work.push_back(S.pending[best].front());
+        S.pending[best].pop_front();
+      }
+      *kind = best;
+      *batch = S.batches++;
+      bool more = false;
+      for (auto& p : S.pending) more = more || !p.empty();
+      if (more) S.cv.notify_all();
+      return true;
+    }
+    if (S.next >= S.nq) return false;
+    S.cv.wait_until(lk, S.t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                   std::chrono::duration<double, std::milli>(S.q[S.order[S.next]].arrival_ms)));
+  }
+}
+
+int run_work(Slot& sl, const rg_serve_config& c, const rg_query* q, int kind, const std::vector<int64_t>& work,
+             rg_query_result* res) {
+  rg_graph* g = sl.view;
+  rg_opts o = c.opts;
+  rg_opts qo = c.opts;   // BFS / BC run on the push engine (optionally with RADAR hub copies)
+  if (qo.variant != RG_RADAR && qo.variant != RG_HUBDUP) qo.variant = RG_PUSH;
+  const size_t k = work.size();
+  std::vector<rg_bfs_summary> sum(k, rg_bfs_summary{0, 0, 0, 0});
+  std::vector<int64_t> iters(k, 0);
+  std::vector<double> l1(k, 0.0), wsum(k, 0.0);
+  std::vector<int64_t> nonfin(k, 0);
+  double ms = 0.0;
+  if (kind == RG_Q_BFS) {
+    if (c.one_at_a_time) {
+      uint64_t cs = 0;
+      RG_TRY(bfs_push(g, q[work[0]].source, &qo, nullptr, &sum[0].iters, &sum[0].reached, &sum[0].edges, &ms,
+                      nullptr, &cs));
+      sum[0].checksum = cs;
+    } else {
+      std::vector<uint32_t> src(k);
+      for (size_t i = 0; i < k; ++i) src[i] = q[work[i]].source;
+      RG_TRY(bfs_batch(g, src.data(), (int)k, &qo, nullptr, sum.data(), &ms));
+    }
+  } else if (kind == RG_Q_BC) {
+    const bool tr = g->transposed;
+    const uint32_t s = q[work[0]].source;
+    RG_TRY(bc_push(g, &s, 1, &qo, nullptr, &iters[0], &ms));
+    if (g->transposed != tr) RG_TRY(rg_graph_transpose(g));
+    RG_TRY(fp_summary(sl.ctx, sl.ctx->last_result, g->n, &wsum[0], &nonfin[0]));
+  } else {
+    const int64_t it = q[work[0]].iters > 0 ? q[work[0]].iters : 10;
+    RG_TRY(rg_pagerank(g, it, &o, nullptr, &iters[0], &l1[0], &ms));
+    RG_TRY(fp_summary(sl.ctx, sl.ctx->last_result, g->n, &wsum[0], &nonfin[0]));
+  }
+  for (size_t i = 0; i < k; ++i) {
+    rg_query_result& R = res[work[i]];
+    R.bfs = sum[i];
+    R.iters = iters[i];
+    R.l1 = l1[i];
+    R.wsum = wsum[i];
+    R.nonfinite = nonfin[i];
+  }
+  return RG_OK;
+}
+
+}  // namespace
+
+}  // namespace rg
+
+using rg::set_error;
+
+extern "C" {
+
+void rg_serve_config_default(rg_serve_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->slots = 2;
+  c->max_batch = 256;
+  c->one_at_a_time = 0;
+  c->hbm_reserve = (int64_t)4 << 30;
+  rg_opts_default(&c->opts);
+}
+
+int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, int64_t nq,
+             rg_query_result* results, rg_serve_stats* stats) {
+  if (!g || !queries || !results || nq < 0) return set_error(RG_EINVAL, "rg_serve: bad arguments");
+  rg_serve_config c;
+  if (cfg) c = *cfg;
+  else rg_serve_config_default(&c);
+  if (c.slots <= 0) c.slots = 2;
+  if (c.max_batch <= 0 || c.max_batch > 256) c.max_batch = 256;
+  if (c.hbm_reserve <= 0) c.hbm_reserve = (int64_t)4 << 30;
+  if (c.one_at_a_time) {
+    c.slots = 1;
+    c.max_batch = 1;
+  }
+  for (int64_t i = 0; i < nq; ++i) {
+    const rg_query& q = queries[i];
+    if (q.kind < RG_Q_BFS || q.kind > RG_Q_PAGERANK) return set_error(RG_EINVAL, "rg_serve: bad query kind");
+    if (q.kind != RG_Q_PAGERANK && (int64_t)q.source >= g->n)
+      return set_error(RG_EINVAL, "rg_serve: query source out of range");
+    if (!(q.arrival_ms >= 0.0)) return set_error(RG_EINVAL, "rg_serve: negative arrival time");
+  }
+  const int dev = g->ctx->device;
+  RG_HIP(hipSetDevice(dev));
+  // Layouts every slot reads, built once on the shared graph.
+  bool any_pr = false, any_bfs = false;
+  for (int64_t i = 0; i < nq; ++i) {
+    any_pr |= queries[i].kind == RG_Q_PAGERANK;
+    any_bfs |= queries[i].kind == RG_Q_BFS;
+  }
+  if (any_bfs && !c.one_at_a_time) RG_TRY(rg::bfs_batch_prepare(g));
+  if (any_pr && c.opts.variant == RG_BLOCKED) RG_TRY(rg::pagerank_prepare(g, &c.opts));
+  if (any_pr && c.opts.variant != RG_BLOCKED && c.opts.variant != RG_PULL)
+    RG_TRY(rg::static_heavy_list(g, c.opts.heavy_degree > 0 ? c.opts.heavy_degree : 1024));
+  RG_HIP(hipDeviceSynchronize());
+  // Admission by device memory: slots = free HBM (less the reserve) over the
+  // bytes one slot can hold (a full-width BFS batch plus the larger of a BC
+  // and a PageRank query).
+  size_t free_b = 0, total_b = 0;
+  RG_HIP(hipMemGetInfo(&free_b, &total_b));
+  const int64_t n = g->n, m = g->m;
+  const int64_t bfs_b = rg::bfs_batch_bytes(n, m, c.max_batch);
+  const int64_t bc_b = n * 30 + (m / 2048 + m / 1024 + 64) * 8;
+  const int64_t pr_b = n * 24 + ((n >> 14) + m / rg::kUnitEdges + 2) * (int64_t)(16384 * 8);
+  const int64_t slot_bytes = bfs_b + std::max(bc_b, pr_b);
+  const int64_t budget = (int64_t)free_b - c.hbm_reserve;
+  if (budget < slot_bytes)
+    return set_error(RG_ENOMEM, "rg_serve: free device memory holds no slot (" + std::to_string(budget) +
+                                    " bytes free beyond the reserve, " + std::to_string(slot_bytes) + " per slot)");
+  const int nslots = (int)std::max<int64_t>(1, std::min<int64_t>(c.slots, budget / slot_bytes));
+
+  std::vector<rg::Slot> slots(nslots);
+  int rc = RG_OK;
+  for (int i = 0; i < nslots && rc == RG_OK; ++i) {
+    rc = rg_ctx_create(dev, &slots[i].ctx);
+    if (rc == RG_OK) {
+      slots[i].view = new rg_graph(*g);
+      slots[i].view->ctx = slots[i].ctx;
+      slots[i].view->view = true;
+      slots[i].view->base = g;
+    }
+  }
+  rg::Shared S;
+  S.q = queries;
+  S.nq = nq;
+  S.order.resize(nq);
+  std::iota(S.order.begin(), S.order.end(), 0);
+  std::stable_sort(S.order.begin(), S.order.end(),
+                   [&](int64_t a, int64_t b) { return queries[a].arrival_ms < queries[b].arrival_ms; });
+  if (rc == RG_OK) {
+    S.t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> th;
+    for (int i = 0; i < nslots; ++i) {
+      th.emplace_back([&, i] {
+        if (hipSetDevice(dev) != hipSuccess) {
+          std::lock_guard<std::mutex> lk(S.mu);
+          S.err = RG_EDEVICE;
+          S.msg = "rg_serve: hipSetDevice failed in a slot thread";
+          S.cv.notify_all();
+          return;
+        }
+        for (;;) {
+          int kind = 0;
+          int64_t batch = 0;
+          std::vector<int64_t> work;
+          if (!rg::take_work(S, c, &kind, work, &batch)) return;
+          const double start = S.ms_now();
+          int r = rg::run_work(slots[i], c, queries, kind, work, results);
+          const double end = S.ms_now();
+          if (r != RG_OK) {
+            std::lock_guard<std::mutex> lk(S.mu);
+            if (S.err == RG_OK) {
+              S.err = r;
+              S.msg = rg_last_error();
+            }
+            S.cv.notify_all();
+            return;
+          }
+          for (int64_t qi : work) {
+            rg_query_result& R = results[qi];
+            R.kind = kind;
+            R.slot = i;
+            R.batch_id = batch;
+            R.batch_size = (int64_t)work.size();
+            R.arrival_ms = queries[qi].arrival_ms;
+            R.start_ms = start;
+            R.end_ms = end;
+            R.latency_ms = end - queries[qi].arrival_ms;
+          }
+        }
+      });
+    }
+    for (auto& t : th) t.join();
+  }
+  for (auto& sl : slots) {
+    if (sl.view) rg_graph_free(sl.view);
+    if (sl.ctx) rg_ctx_destroy(sl.ctx);
+  }
+  if (rc != RG_OK) return rc;
+  if (S.err != RG_OK) return set_error(S.err, S.msg);
+  if (stats) {
+    std::vector<double> lat(nq);
+    double mk = 0.0, tot = 0.0;
+    int64_t nbfs = 0;
+    for (int64_t i = 0; i < nq; ++i) {
+      lat[i] = results[i].latency_ms;
+      mk = std::max(mk, results[i].end_ms);
+      tot += lat[i];
+      nbfs += results[i].kind == RG_Q_BFS;
+    }
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double p) {
+      if (lat.empty()) return 0.0;
+      int64_t r = (int64_t)std::ceil(p * (double)lat.size());
+      return lat[std::min<int64_t>(std::max<int64_t>(r, 1), (int64_t)lat.size()) - 1];
+    };
+    std::memset(stats, 0, sizeof(*stats));
+    stats->queries = nq;
+    stats->batches = S.batches;
+    stats->slots = nslots;
+    stats->makespan_ms = mk;
+    stats->qps = mk > 0 ? (double)nq / (mk * 1e-3) : 0.0;
+    stats->p50_ms = pct(0.50);
+    stats->p99_ms = pct(0.99);
+    stats->mean_ms = nq ? tot / (double)nq : 0.0;
+    int64_t bfs_batches = 0;
+    {
+      std::vector<char> seen((size_t)S.batches, 0);
+      for (int64_t i = 0; i < nq; ++i)
+        if (results[i].kind == RG_Q_BFS && !seen[results[i].batch_id]) {
+          seen[results[i].batch_id] = 1;
+          bfs_batches++;
+        }
+    }
+    stats->mean_batch = bfs_batches ? (double)nbfs / (double)bfs_batches : 0.0;
+    stats->hbm_budget = budget;
+    stats->slot_bytes = slot_bytes;
+  }
+  return RG_OK;
+}
+
+}  // extern "C"

@@ -65,6 +65,35 @@ class BfsSummary(C.Structure):
         return (self.iters, self.reached, self.edges, self.checksum)
 
 
+Q_BFS, Q_BC, Q_PAGERANK = range(3)
+
+
+class Query(C.Structure):
+    _fields_ = [("kind", C.c_int), ("source", C.c_uint32), ("iters", C.c_int32), ("arrival_ms", C.c_double)]
+
+
+class QueryResult(C.Structure):
+    _fields_ = [("kind", C.c_int), ("slot", C.c_int), ("batch_id", C.c_int64), ("batch_size", C.c_int64),
+                ("bfs", BfsSummary), ("iters", C.c_int64), ("l1", C.c_double), ("wsum", C.c_double),
+                ("nonfinite", C.c_int64), ("arrival_ms", C.c_double), ("start_ms", C.c_double),
+                ("end_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("slots", C.c_int), ("max_batch", C.c_int), ("one_at_a_time", C.c_int),
+                ("hbm_reserve", C.c_int64), ("opts", Opts)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("queries", C.c_int64), ("batches", C.c_int64), ("slots", C.c_int),
+                ("makespan_ms", C.c_double), ("qps", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("mean_batch", C.c_double),
+                ("hbm_budget", C.c_int64), ("slot_bytes", C.c_int64)]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _sig(name, res, *args):
     f = getattr(lib, name)
     f.restype = res
@@ -106,6 +135,9 @@ _sig("rg_bfs", C.c_int, _vp, C.c_uint32, C.POINTER(Opts), _u32p, _i64p, _i64p, _
 _sig("rg_level_hash", C.c_uint64, C.c_uint32, C.c_uint32)
 _sig("rg_bfs_query", C.c_int, _vp, C.c_uint32, C.POINTER(Opts), C.POINTER(BfsSummary), _f64p)
 _sig("rg_bfs_batch", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _i32p, C.POINTER(BfsSummary), _f64p)
+_sig("rg_serve_config_default", None, C.POINTER(ServeConfig))
+_sig("rg_serve", C.c_int, _vp, C.POINTER(ServeConfig), C.POINTER(Query), C.c_int64, C.POINTER(QueryResult),
+     C.POINTER(ServeStats))
 _sig("rg_radii", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _i32p, _i64p, _f64p)
 _sig("rg_bc", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _f64p, _i64p, _f64p)
 
@@ -166,6 +198,38 @@ def generate_edges(scale: int, degree: int, uniform: bool = False):
 
 def hash_int(a: int) -> int:
     return int(lib.rg_hash_int(C.c_uint64(a)))
+
+
+def serve_config(slots=2, max_batch=256, one_at_a_time=False, hbm_reserve=0, o: Opts = None) -> ServeConfig:
+    c = ServeConfig()
+    lib.rg_serve_config_default(C.byref(c))
+    c.slots, c.max_batch, c.one_at_a_time = slots, max_batch, int(one_at_a_time)
+    if hbm_reserve:
+        c.hbm_reserve = hbm_reserve
+    if o is not None:
+        c.opts = o
+    return c
+
+
+def poisson_workload(hg: "HostGraph", count: int, rate_qps: float, mix=(0.8, 0.15, 0.05), pr_iters=10,
+                     seed=27491095):
+    """Seeded query trace: exponential inter-arrival times at rate_qps
+    (all at time 0 when rate_qps <= 0), kinds drawn with probabilities mix
+    (BFS, BC, PageRank), sources uniform over vertices with out-degree > 0.
+    Returns a ctypes array of Query."""
+    rng = np.random.default_rng(seed)
+    if rate_qps > 0:
+        t = np.cumsum(rng.exponential(1e3 / rate_qps, size=count))
+    else:
+        t = np.zeros(count)
+    kinds = rng.choice(3, size=count, p=np.asarray(mix, float) / sum(mix))
+    deg = np.diff(hg.offsets().astype(np.int64))
+    cand = np.nonzero(deg > 0)[0]
+    src = cand[rng.integers(0, len(cand), size=count)]
+    q = (Query * count)()
+    for i in range(count):
+        q[i].kind, q[i].source, q[i].iters, q[i].arrival_ms = int(kinds[i]), int(src[i]), pr_iters, float(t[i])
+    return q
 
 
 def level_hash(v: int, level: int) -> int:
@@ -399,6 +463,14 @@ class DeviceGraph:
         _check(lib.rg_bfs_batch(self._h, _ptr(src, _u32p), len(src), C.byref(o),
                                 _ptr(lv, _i32p) if want_levels else None, out, C.byref(ms)))
         return list(out), lv, ms.value
+
+    def serve(self, queries, cfg: ServeConfig = None):
+        """Runs a query trace through the serving loop: (results, stats)."""
+        cfg = cfg or serve_config()
+        res = (QueryResult * len(queries))()
+        st = ServeStats()
+        _check(lib.rg_serve(self._h, C.byref(cfg), queries, len(queries), res, C.byref(st)))
+        return res, st
 
     def radii(self, starts, o: Opts = None):
         o = o or opts()

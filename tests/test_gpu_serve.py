@@ -1,0 +1,107 @@
+"""GPU tests of the serving loop: continuous batching of BFS queries over
+several slots, mixed with BC and PageRank queries.  Every answer in the mix
+equals the same query run alone (BFS summaries bit-exact; BC / PageRank
+result summaries within 1e-10 relative, non-finite counts exact), and the
+alone answers are checked against the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP_TOL = 1e-10
+
+
+def weights(orc, n):
+    v = np.arange(n, dtype=np.uint64)
+    h = orc.level_hash(v, np.full(n, 0xFFFFFFFF, np.uint64))
+    return (h >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+def fp_summary(orc, x):
+    f = np.isfinite(x)
+    return float(np.sum(x[f] * weights(orc, len(x))[f])), int((~f).sum())
+
+
+def close(a, b, tol=FP_TOL):
+    return abs(a - b) <= tol * max(abs(a), abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["rmat14s", "rmat12d"])
+def test_mix_equals_alone(rg, orc, ctx, graphs, name):
+    hg = graphs(name)
+    dg = rg.DeviceGraph(ctx, hg)
+    off, nb = hg.csr()
+    in_off, in_nb = hg.csr(in_edges=True) if not hg.symmetric else (off, nb)
+    q = rg.poisson_workload(hg, 120, 20000.0, seed=5)
+    print("serve", name, len(q), flush=True)
+    res, st = dg.serve(q, rg.serve_config(slots=3))
+    assert st.queries == len(q)
+    sizes = [r.batch_size for r in res if r.kind == rg.Q_BFS]
+    assert max(sizes) > 1, "no BFS queries were coalesced"
+    pr_alone = None
+    for i, (qq, r) in enumerate(zip(q, res)):
+        assert r.kind == qq.kind
+        assert r.latency_ms >= 0 and r.end_ms >= r.start_ms >= r.arrival_ms - 1e-6
+        if qq.kind == rg.Q_BFS:
+            alone, _ = dg.bfs_query(qq.source)
+            assert r.bfs.astuple() == alone.astuple(), f"query {i}"
+            lv, _ = orc.bfs_levels(off, nb, qq.source)
+            assert alone.astuple() == orc.bfs_summary(off, lv)
+        elif qq.kind == rg.Q_BC:
+            deps, comb, _ = dg.bc([qq.source])
+            if not hg.symmetric:
+                dg.transpose()   # BC leaves a directed graph transposed (BC-manyiter-v2.C:217)
+            ws, nf = fp_summary(orc, deps)
+            assert r.nonfinite == nf and close(r.wsum, ws), f"query {i}"
+            ref, _ = orc.bc(off, nb, in_off, in_nb, [qq.source])
+            ws2, nf2 = fp_summary(orc, ref)
+            assert nf == nf2 and close(ws, ws2)
+        else:
+            if pr_alone is None:
+                pr_alone = dg.pagerank(qq.iters)
+                ref, it, _ = orc.pagerank(off, nb, qq.iters)
+                assert pr_alone.iters == it
+                assert np.max(np.abs(pr_alone.ranks - ref)) <= FP_TOL * np.max(np.abs(ref))
+            ws, nf = fp_summary(orc, pr_alone.ranks)
+            assert r.iters == pr_alone.iters and r.nonfinite == nf and close(r.wsum, ws), f"query {i}"
+
+
+def test_slots_overlap_and_batches_grow_under_load(rg, ctx, graphs):
+    hg = graphs("rmat14s")
+    dg = rg.DeviceGraph(ctx, hg)
+    q = rg.poisson_workload(hg, 600, 0.0, mix=(1, 0, 0), seed=9)   # everything arrives at t = 0
+    res, st = dg.serve(q, rg.serve_config(slots=3, max_batch=64))
+    slots = {r.slot for r in res}
+    assert st.slots == 3 and len(slots) > 1
+    # batches of different slots run at the same time
+    iv = sorted({(r.slot, r.start_ms, r.end_ms) for r in res}, key=lambda t: t[1])
+    assert any(a[0] != b[0] and b[1] < a[2] for a, b in zip(iv, iv[1:]))
+    assert st.mean_batch > 8 and max(r.batch_size for r in res) == 64
+
+
+def test_one_at_a_time_same_answers(rg, ctx, graphs):
+    hg = graphs("rmat12s")
+    dg = rg.DeviceGraph(ctx, hg)
+    q = rg.poisson_workload(hg, 80, 5000.0, seed=3)
+    a, sa = dg.serve(q, rg.serve_config(slots=2))
+    b, sb = dg.serve(q, rg.serve_config(one_at_a_time=True))
+    assert sb.slots == 1 and max(r.batch_size for r in b) == 1
+    for x, y in zip(a, b):
+        assert x.kind == y.kind
+        if x.kind == rg.Q_BFS:
+            assert x.bfs.astuple() == y.bfs.astuple()
+        else:
+            assert x.nonfinite == y.nonfinite and close(x.wsum, y.wsum)
+
+
+def test_admission_by_memory(rg, ctx, graphs):
+    hg = graphs("rmat12s")
+    dg = rg.DeviceGraph(ctx, hg)
+    q = rg.poisson_workload(hg, 30, 0.0, seed=4)
+    _, st = dg.serve(q, rg.serve_config(slots=4))
+    free = st.hbm_budget + rg.serve_config().hbm_reserve
+    # leave room for one slot and a half: one slot admitted
+    _, st1 = dg.serve(q, rg.serve_config(slots=4, hbm_reserve=int(free - 1.5 * st.slot_bytes)))
+    assert st1.slots == 1
+    with pytest.raises(rg.RGError):
+        dg.serve(q, rg.serve_config(slots=4, hbm_reserve=int(free - st.slot_bytes // 2)))

@@ -8,7 +8,10 @@ degree-sorted graph with hub-duplicated accumulators, on the engine the
 autotuner picks for this graph (the blocked LDS engine or the atomic one);
 its GTEPS is 20 * m / time.  Every variant (atomic push-only, HUBDUP,
 DegreeSorting, atomic RADAR, pull, blocked on the unsorted graph), BFS from
-64 GAP-picked sources, and the CPU oracle baseline are reported alongside.
+64 GAP-picked sources, batched BFS (256 sources as one bit-parallel batch
+vs one query at a time), the serving loop (symmetric R-MAT, default scale
+22: saturated queries/s and Poisson-arrival p50/p99, batched vs one at a
+time) and the CPU oracle baseline are reported alongside.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S]
 Prints one JSON line.
@@ -29,6 +32,49 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def serve_bench(rg, args, log):
+    """Serving over a resident degree-sorted symmetric R-MAT graph: seeded
+    query traces (BFS only; 80/15/5 BFS/BC/PageRank-10) run saturated (all
+    queries at t = 0) and as Poisson arrivals at 80% of the one-at-a-time
+    capacity, batched (continuous batching, 2 slots) vs one at a time."""
+    t0 = time.time()
+    hg = rg.HostGraph.generate(args.serve_scale, 16, symmetric=True)
+    n, m, _ = hg.info()
+    ctx = rg.Context(0)
+    g0 = rg.DeviceGraph(ctx, hg)
+    g, ids = g0.degree_sort()
+    g0.close()
+    log(f"serving graph: rmat{args.serve_scale} symmetric n={n} m={m} ({time.time() - t0:.1f}s)")
+
+    def trace(count, rate, mix, seed):
+        q = rg.poisson_workload(hg, count, rate, mix=mix, seed=seed)
+        for x in q:
+            x.source = int(ids[x.source])
+        return q
+
+    def row(st):
+        return {"queries_per_s": round(st.qps, 1), "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
+                "mean_batch": round(st.mean_batch, 1), "slots": st.slots}
+
+    out = {"graph": f"rmat{args.serve_scale} symmetric, degree-sorted", "n": n, "m": m}
+    for name, mix, count in [("bfs", (1, 0, 0), args.serve_queries), ("mix_80_15_5", (0.8, 0.15, 0.05), 256)]:
+        q = trace(count, 0.0, mix, 27491095)
+        _, one = g.serve(q, rg.serve_config(one_at_a_time=True))
+        res, bat = g.serve(q, rg.serve_config(slots=2))
+        rate = 0.8 * one.qps
+        q2 = trace(count, rate, mix, 1)
+        _, one_p = g.serve(q2, rg.serve_config(one_at_a_time=True))
+        _, bat_p = g.serve(q2, rg.serve_config(slots=2))
+        out[name] = {"saturated": {"one_at_a_time": row(one), "batched": row(bat),
+                                   "speedup": round(bat.qps / one.qps, 2)},
+                     f"poisson_{rate:.0f}qps": {"one_at_a_time": row(one_p), "batched": row(bat_p)}}
+        log(f"serving {name}: saturated {bat.qps:.0f} vs {one.qps:.0f} q/s one at a time; at {rate:.0f} q/s "
+            f"p50/p99 {bat_p.p50_ms:.2f}/{bat_p.p99_ms:.2f} ms vs {one_p.p50_ms:.2f}/{one_p.p99_ms:.2f} ms")
+    g.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +86,10 @@ def main():
     ap.add_argument("--hubs", type=int, default=0, help="LDS hub slots (0 = LDS budget)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (no variant table, no BFS)")
+    ap.add_argument("--batch", type=int, default=256, help="sources per batched BFS")
+    ap.add_argument("--serve-scale", type=int, default=22, help="R-MAT scale of the serving graph (symmetric)")
+    ap.add_argument("--serve-queries", type=int, default=512)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the library's plain 1 GiB stream copy once (k_stream_copy: the "
                          "known-size kernel that scales FETCH_SIZE / WRITE_SIZE)")
@@ -152,6 +202,35 @@ def main():
         bfs[name] = {"gteps": round(tot_e / tot_ms / 1e6, 2), "ms_per_source": round(tot_ms / len(srcs), 3)}
         log(f"bfs {name:15s} {bfs[name]['ms_per_source']:8.3f} ms/src {bfs[name]['gteps']:7.2f} GTEPS")
 
+    # Batched BFS: the same 256 GAP-picked sources as one bit-parallel batch
+    # on the RADAR graph vs one query at a time (single-source engine).
+    batched = None
+    if not args.quick:
+        bsrc = new_ids[hg.pick_sources(args.batch)]
+        sg.bfs_query(int(bsrc[0]))
+        t0 = time.perf_counter()
+        single = [sg.bfs_query(int(s))[0] for s in bsrc[:64]]
+        t_single = (time.perf_counter() - t0) / 64
+        sg.bfs_batch(bsrc)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out, _, _ = sg.bfs_batch(bsrc)
+        t_batch = (time.perf_counter() - t0) / reps
+        same = all(a.astuple() == b.astuple() for a, b in zip(out[:64], single))
+        batched = {"sources": len(bsrc), "batch_ms": round(t_batch * 1e3, 3),
+                   "queries_per_s": round(len(bsrc) / t_batch, 1),
+                   "one_at_a_time_queries_per_s": round(1.0 / t_single, 1),
+                   "speedup": round(t_single * len(bsrc) / t_batch, 2),
+                   "equivalent_gteps": round(sum(x.edges for x in out) / t_batch / 1e9, 1),
+                   "answers_equal_single": same}
+        log(f"batched bfs: {batched['queries_per_s']} q/s vs {batched['one_at_a_time_queries_per_s']} q/s "
+            f"one at a time ({batched['speedup']}x), equal answers: {same}")
+
+    serving = None
+    if not args.quick and not args.no_serve:
+        serving = serve_bench(rg, args, log)
+
     cpu = None
     if not args.no_cpu:
         import pyoracle
@@ -196,6 +275,8 @@ def main():
         "cpu_baseline": cpu,
         "pagerank": pr,
         "bfs": bfs,
+        "batched_bfs": batched,
+        "serving": serving,
         "preprocess_ms": {"host_graph_build": round(t_gen * 1e3, 1), "gpu_degree_sort": round(t_sort * 1e3, 2),
                           "autotune": round(t_tune * 1e3, 1)},
     }

@@ -233,7 +233,9 @@ int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o,
  * path counting by level, backward dependency accumulation on the transposed
  * graph (the transpose is not undone between sources), Dependencies carried
  * over between sources (zeroed once), final (dep - 1/sigma)/(1/sigma).
- * combined_out = sum of forward rounds (printed "Combined Iters"). */
+ * combined_out = sum of forward rounds (printed "Combined Iters").  With
+ * o->variant RG_RADAR or RG_BLOCKED, levels above the sparse/dense threshold
+ * run on the destination-blocked layout (LDS sums, no global atomics). */
 int rg_bc(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o,
           double* deps_out, int64_t* combined_out, double* ms_out);
 
@@ -327,7 +329,8 @@ typedef struct rg_serve_config {
   int max_batch;         /* BFS queries per batch (1..256, <= 0: 256) */
   int one_at_a_time;
   int64_t hbm_reserve;   /* bytes of device memory left free (<= 0: 4 GiB) */
-  rg_opts opts;          /* engine options for every query */
+  rg_opts opts;          /* engine options for every query (default variant
+                            RG_BLOCKED: blocked PageRank, blocked dense BC levels) */
 } rg_serve_config;
 typedef struct rg_serve_stats {
   int64_t queries, batches;

@@ -780,6 +780,9 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
   int64_t* info = ctx->counts.as<int64_t>() + 4;
   int64_t combined = 0;
   const bool was_transposed = g->transposed;
+  // RADAR and blocked run dense levels on the blocked layout (bcdense.hip);
+  // push and HUBDUP keep the atomic push engine for every level.
+  const bool blocked = o && (o->variant == RG_RADAR || o->variant == RG_BLOCKED);
   EventTimer tm(ctx->stream);
   for (int it = 0; it < nsources; ++it) {
     Engine E;
@@ -801,7 +804,10 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
       F.ids = lv + fb;
       F.size = fe - fb;
       RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
-      RG_TRY(edge_map(E, fop, F, degsum, lv + fe, info));
+      if (blocked && F.size + degsum > E.threshold)
+        RG_TRY(bc_dense_level(g, true, F.ids, F.size, np, np, visited, lv + fe, info));
+      else
+        RG_TRY(edge_map(E, fop, F, degsum, lv + fe, info));
       int64_t h[2];
       RG_TRY(ctx_read_count(ctx, info, h, 2));
       if (h[0] > 0)
@@ -825,7 +831,10 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
     for (int64_t r = round - 2; r >= 0; --r) {
       int64_t ds = 0;
       RG_TRY(frontier_degsum(ctx, g, F.ids, F.size, &ds));
-      RG_TRY(edge_map(EB, bop, F, ds, nullptr, nullptr));
+      if (blocked && ds > 0 && F.size + ds > EB.threshold)
+        RG_TRY(bc_dense_level(g, false, F.ids, F.size, deps, deps, visited, nullptr, nullptr));
+      else
+        RG_TRY(edge_map(EB, bop, F, ds, nullptr, nullptr));
       level(r);
       hipLaunchKernelGGL(k_bc_back_vertex, grid_for(ctx, F.size), 256, 0, ctx->stream, F.ids, F.size,
                          visited, deps, np);

@@ -1,0 +1,202 @@
+// Dense BC levels on the destination-blocked layout (blocked.h): the
+// forward path counting of BC_F (push-new-orig Ligra/apps/BC-manyiter-v2.C:
+// 189-213, NumPaths[d] += NumPaths[s] for unvisited d) and the backward
+// accumulation of BC_Back_F (215-236, Dependencies[d] += Dependencies[s] on
+// the transposed graph), for frontiers large enough that the push engine
+// would scatter an FP64 atomic per arc.  Those atomics execute at the memory
+// side on this part; here one workgroup per unit sums the values of the
+// frontier sources of its block's in-edges in LDS and writes each
+// destination once.  Blocks split over several units (hub blocks) write
+// per-unit partial sums that a second kernel adds in unit order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "blocked.h"
+#include "device.h"
+#include "engine.h"
+#include "engine_host.h"
+#include "queries.h"
+
+namespace rg {
+
+namespace {
+
+constexpr int kBcThreads = 512;
+
+struct BcDenseArgs {
+  int64_t n;
+  const uint64_t* off;                 // out-offsets of the orientation (frontier degrees)
+  const uint32_t* src;
+  const uint16_t* vl;
+  const uint64_t* unit_lo;
+  const uint32_t* unit_block;
+  const uint32_t* block_unit0;
+  const uint32_t* multi;
+  const unsigned long long* active;    // bit per vertex: in the frontier
+  const uint8_t* visited;
+  const double* val;                   // value pushed from each frontier source
+  double* out;                         // forward: set at joining vertices; backward: added
+  double* part;                        // per-unit partial sums of split blocks
+  uint32_t* out_ids;                   // forward: next frontier
+  int64_t* out_info;                   // forward: [count, out-degree sum]
+};
+
+// Destination d of the block, with sum x: forward, d joins the next
+// frontier when a frontier in-neighbour reached it (x != 0: path counts of
+// visited vertices are positive) and NumPaths[d] (0 while unvisited) = x;
+// backward, Dependencies[d] += x.
+template <bool FWD>
+__device__ __forceinline__ void bc_finalize(const BcDenseArgs& a, WaveQueue& q, const EdgeArgs& ea, int64_t d,
+                                            bool valid, double x) {
+  bool join = false;
+  if (valid && x != 0.0) {
+    if (FWD) {
+      a.out[d] = x;
+      join = true;
+    } else {
+      a.out[d] += x;
+    }
+  }
+  if (FWD) wq_push(q, ea, join, (uint32_t)d);
+}
+
+template <bool FWD, int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
+  constexpr int BV = 1 << VB;
+  constexpr int NW = THREADS / kWave;
+  __shared__ double acc[BV];
+  __shared__ unsigned long long skip[BV / 64];
+  __shared__ uint32_t s_q[FWD ? NW : 1][FWD ? kQueue : 1];
+  const int64_t u = blockIdx.x;
+  const uint32_t b = a.unit_block[u];
+  const int64_t base = (int64_t)b << VB;
+  const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
+  for (int l = threadIdx.x; l < BV; l += THREADS) {
+    acc[l] = 0.0;
+    const int64_t d = base + l;
+    const bool sk = d >= a.n || a.visited[d] != 0;
+    const unsigned long long w = __ballot(sk);
+    if (__lane_id() == 0) skip[l >> 6] = w;
+  }
+  __syncthreads();
+  const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
+  for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
+    const uint32_t s = __builtin_nontemporal_load(a.src + e);
+    const uint32_t l = __builtin_nontemporal_load(a.vl + e);
+    if ((skip[l >> 6] >> (l & 63)) & 1) continue;
+    if (!((a.active[s >> 6] >> (s & 63)) & 1)) continue;
+    atomicAdd(&acc[l], a.val[s]);
+  }
+  __syncthreads();
+  if (!single) {
+    double* p = a.part + ((size_t)u << VB);
+    for (int l = threadIdx.x; l < BV; l += THREADS) p[l] = acc[l];
+    return;
+  }
+  EdgeArgs ea{};
+  ea.off = a.off;
+  ea.out_ids = a.out_ids;
+  ea.out_info = a.out_info;
+  WaveQueue q{s_q[FWD ? threadIdx.x / kWave : 0], 0, 0};
+  for (int l0 = 0; l0 < BV; l0 += THREADS) {
+    const int l = l0 + threadIdx.x;
+    const int64_t d = base + l;
+    bc_finalize<FWD>(a, q, ea, d, !((skip[l >> 6] >> (l & 63)) & 1), acc[l]);
+  }
+  if (FWD) wq_finish(q, ea);
+}
+
+// Split blocks: partial sums added in unit order.  Grid: nmulti * BV/THREADS.
+template <bool FWD, int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_bc_merge(BcDenseArgs a) {
+  constexpr int BV = 1 << VB;
+  constexpr int NW = THREADS / kWave;
+  constexpr int PER = BV / THREADS;
+  __shared__ uint32_t s_q[FWD ? NW : 1][FWD ? kQueue : 1];
+  const uint32_t b = a.multi[blockIdx.x / PER];
+  const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
+  const int64_t d = ((int64_t)b << VB) + l;
+  const bool valid = d < a.n && a.visited[d] == 0;
+  double x = 0.0;
+  for (uint32_t u = a.block_unit0[b]; u < a.block_unit0[b + 1]; ++u) x += a.part[((size_t)u << VB) + l];
+  EdgeArgs ea{};
+  ea.off = a.off;
+  ea.out_ids = a.out_ids;
+  ea.out_info = a.out_info;
+  WaveQueue q{s_q[FWD ? threadIdx.x / kWave : 0], 0, 0};
+  bc_finalize<FWD>(a, q, ea, d, valid, x);
+  if (FWD) wq_finish(q, ea);
+}
+
+__global__ void k_set_bits(const uint32_t* ids, int64_t count, unsigned long long* bits) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = ids[i];
+    atomicOr(bits + (v >> 6), 1ull << (v & 63));
+  }
+}
+
+__global__ void k_clear_bits(const uint32_t* ids, int64_t count, unsigned long long* bits) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    bits[ids[i] >> 6] = 0;
+}
+
+}  // namespace
+
+// One dense BC level over the frontier ids[0, count) of g's current
+// orientation.  Forward: out_ids / out_info receive the next frontier (the
+// caller zeroes out_info).  The frontier bitmap and partial sums live in the
+// context's bc_dense buffer; the bitmap is all zero between calls.
+int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, const double* val, double* out,
+                   const uint8_t* visited, uint32_t* out_ids, int64_t* out_info) {
+  rg_ctx* ctx = g->ctx;
+  BlockedCSC* B = nullptr;
+  RG_TRY(query_layout(g, &B));
+  const int64_t nwords = g->n / 64 + 1;
+  const size_t bits_bytes = ((sizeof(unsigned long long) * nwords + 255) / 256) * 256;
+  const size_t part_bytes = sizeof(double) * ((size_t)std::max<int64_t>(B->nunits, 1) << kQueryVB);
+  DevBuf& sb = ctx->bc_dense;
+  if (sb.bytes < bits_bytes + part_bytes) {
+    sb.release();
+    RG_TRY(sb.ensure(bits_bytes + part_bytes));
+    RG_TRY(fill<unsigned long long>(ctx, sb.as<unsigned long long>(), nwords, 0ull));
+  }
+  unsigned long long* bits = sb.as<unsigned long long>();
+  BcDenseArgs a{};
+  a.n = g->n;
+  a.off = g->off;
+  a.src = B->src;
+  a.vl = B->vl;
+  a.unit_lo = B->unit_lo;
+  a.unit_block = B->unit_block;
+  a.block_unit0 = B->block_unit0;
+  a.multi = B->multi;
+  a.active = bits;
+  a.visited = visited;
+  a.val = val;
+  a.out = out;
+  a.part = reinterpret_cast<double*>(sb.as<char>() + bits_bytes);
+  a.out_ids = out_ids;
+  a.out_info = out_info;
+  const int grid = grid_for(ctx, count);
+  hipLaunchKernelGGL(k_set_bits, grid, 256, 0, ctx->stream, ids, count, bits);
+  constexpr int PER = (1 << kQueryVB) / kBcThreads;
+  hipEvent_t* ev = timing_begin(ctx);
+  if (fwd) {
+    hipLaunchKernelGGL((k_bc_dense<true, kQueryVB, kBcThreads>), (int)B->nunits, kBcThreads, 0, ctx->stream, a);
+    if (B->nmulti)
+      hipLaunchKernelGGL((k_bc_merge<true, kQueryVB, kBcThreads>), (int)(B->nmulti * PER), kBcThreads, 0,
+                         ctx->stream, a);
+  } else {
+    hipLaunchKernelGGL((k_bc_dense<false, kQueryVB, kBcThreads>), (int)B->nunits, kBcThreads, 0, ctx->stream, a);
+    if (B->nmulti)
+      hipLaunchKernelGGL((k_bc_merge<false, kQueryVB, kBcThreads>), (int)(B->nmulti * PER), kBcThreads, 0,
+                         ctx->stream, a);
+  }
+  if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
+  hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);
+  RG_HIP(hipGetLastError());
+  return RG_OK;
+}
+
+}  // namespace rg

@@ -16,6 +16,10 @@
 namespace rg {
 
 constexpr int64_t kUnitEdges = int64_t(1) << 19;
+// Destination-block bits of the layout the query kernels share (batched
+// BFS, dense BC levels): 4096 vertices, whose 64-bit words or FP64 values
+// fit in LDS.
+constexpr int kQueryVB = 12;
 
 struct BlockedCSC {
   const uint64_t* built_for = nullptr;   // in_off of the orientation it was built from
@@ -36,5 +40,11 @@ struct BlockedCSC {
 // built, so contexts sharing the graph may use it at the same time; the
 // per-unit partial results live in each context's scratch.
 int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out);
+
+// The query layout (2^kQueryVB-vertex blocks, source order) of g's current
+// orientation, built on first use.
+inline int query_layout(rg_graph* g, BlockedCSC** out) {
+  return blocked_build(g, kQueryVB, 0, g->transposed ? &g->blocked_ms_t : &g->blocked_ms, out);
+}
 
 }  // namespace rg

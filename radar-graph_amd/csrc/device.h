@@ -62,6 +62,8 @@ struct rg_ctx {
   struct EvPair { hipEvent_t e[2]; };
   std::vector<EvPair> ev_pairs;
   size_t ev_used = 0;
+  // Dense BC levels: frontier bitmap (all zero between levels) + partials.
+  rg::DevBuf bc_dense;
   // Multi-source BFS state (opaque, msbfs.hip), grown on demand.
   void* ms = nullptr;
   // Device array holding the last PageRank ranks / BC dependencies computed
@@ -96,7 +98,8 @@ struct rg_graph {
   // Blocked in-edge layouts (opaque BlockedCSC): PageRank's (2^14-vertex
   // blocks, with partial sums) and the multi-source BFS's (2^12).
   void* blocked = nullptr;
-  void* blocked_ms = nullptr;
+  void* blocked_ms = nullptr;     // of the in-edges of the original orientation
+  void* blocked_ms_t = nullptr;   // of the transposed orientation (directed graphs)
   // A view shares the device arrays and cached layouts of `base` (bound to
   // another context, e.g. one serving slot) and frees only what it builds.
   bool view = false;

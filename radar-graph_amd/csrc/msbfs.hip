@@ -39,7 +39,7 @@
 
 namespace rg {
 
-constexpr int kMsVB = 12;            // 4096 destinations per dense block
+constexpr int kMsVB = kQueryVB;      // 4096 destinations per dense block
 constexpr int kMsThreads = 512;      // dense-level workgroup
 constexpr int kMsMaxSources = 256;
 constexpr int kMsLevelsPerSync = 4;  // levels launched between host checks
@@ -606,7 +606,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
 
 int bfs_batch_prepare(rg_graph* g) {
   BlockedCSC* B = nullptr;
-  return blocked_build(g, kMsVB, 0, &g->blocked_ms, &B);
+  return query_layout(g, &B);
 }
 
 int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc) {
@@ -627,7 +627,7 @@ int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
   if (n >= (int64_t)1 << 32) return set_error(RG_EUNSUPPORTED, "rg_bfs_batch: n >= 2^32");
   const int W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
   BlockedCSC* B = nullptr;
-  RG_TRY(blocked_build(g, kMsVB, 0, &g->blocked_ms, &B));
+  RG_TRY(query_layout(g, &B));
   Engine E;
   rg_opts po;
   rg_opts_default(&po);

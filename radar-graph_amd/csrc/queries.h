@@ -39,6 +39,11 @@ int pagerank_prepare(rg_graph* g, const rg_opts* o);
 // Builds the all-vertices heavy-chunk list of the push engine.
 int static_heavy_list(rg_graph* g, int heavy_deg);
 
+// One dense BC level (forward path counts or backward dependencies) on the
+// blocked layout of g's current orientation (bcdense.hip).
+int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, const double* val, double* out,
+                   const uint8_t* visited, uint32_t* out_ids, int64_t* out_info);
+
 int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
             int64_t* combined_out, double* ms_out);
 

@@ -129,6 +129,7 @@ int rg_ctx_destroy(rg_ctx* ctx) {
   ctx->fr_ids[0].release();
   ctx->fr_ids[1].release();
   ctx->fr_dense.release();
+  ctx->bc_dense.release();
   rg::ms_state_free(ctx->ms);
   ctx->ms = nullptr;
   for (auto& p : ctx->ev_pairs) {
@@ -291,6 +292,7 @@ int rg_graph_free(rg_graph* g) {
   if (own(g->heavy_items, b ? b->heavy_items : nullptr)) (void)hipFree(g->heavy_items);
   if (own(g->blocked, b ? b->blocked : nullptr)) rg::blocked_free(g->blocked);
   if (own(g->blocked_ms, b ? b->blocked_ms : nullptr)) rg::blocked_free(g->blocked_ms);
+  if (own(g->blocked_ms_t, b ? b->blocked_ms_t : nullptr)) rg::blocked_free(g->blocked_ms_t);
   delete g;
   return RG_OK;
 }

@@ -153,8 +153,8 @@ int run_work(Slot& sl, const rg_serve_config& c, const rg_query* q, int kind, co
              rg_query_result* res) {
   rg_graph* g = sl.view;
   rg_opts o = c.opts;
-  rg_opts qo = c.opts;   // BFS / BC run on the push engine (optionally with RADAR hub copies)
-  if (qo.variant != RG_RADAR && qo.variant != RG_HUBDUP) qo.variant = RG_PUSH;
+  rg_opts qo = c.opts;   // BFS / BC: the push engine, RADAR hub copies or blocked dense BC levels
+  if (qo.variant == RG_PULL) qo.variant = RG_PUSH;
   const size_t k = work.size();
   std::vector<rg_bfs_summary> sum(k, rg_bfs_summary{0, 0, 0, 0});
   std::vector<int64_t> iters(k, 0);
@@ -210,6 +210,7 @@ void rg_serve_config_default(rg_serve_config* c) {
   c->one_at_a_time = 0;
   c->hbm_reserve = (int64_t)4 << 30;
   rg_opts_default(&c->opts);
+  c->opts.variant = RG_BLOCKED;
 }
 
 int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, int64_t nq,
@@ -235,12 +236,14 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
   const int dev = g->ctx->device;
   RG_HIP(hipSetDevice(dev));
   // Layouts every slot reads, built once on the shared graph.
-  bool any_pr = false, any_bfs = false;
+  bool any_pr = false, any_bfs = false, any_bc = false;
   for (int64_t i = 0; i < nq; ++i) {
     any_pr |= queries[i].kind == RG_Q_PAGERANK;
     any_bfs |= queries[i].kind == RG_Q_BFS;
+    any_bc |= queries[i].kind == RG_Q_BC;
   }
-  if (any_bfs && !c.one_at_a_time) RG_TRY(rg::bfs_batch_prepare(g));
+  const bool bc_blocked = c.opts.variant == RG_RADAR || c.opts.variant == RG_BLOCKED;
+  if ((any_bfs && !c.one_at_a_time) || (any_bc && bc_blocked)) RG_TRY(rg::bfs_batch_prepare(g));
   if (any_pr && c.opts.variant == RG_BLOCKED) RG_TRY(rg::pagerank_prepare(g, &c.opts));
   if (any_pr && c.opts.variant != RG_BLOCKED && c.opts.variant != RG_PULL)
     RG_TRY(rg::static_heavy_list(g, c.opts.heavy_degree > 0 ? c.opts.heavy_degree : 1024));

@@ -79,7 +79,7 @@ def test_radii(rg, orc, ctx, graphs, gname, variant):
 
 
 @pytest.mark.parametrize("gname", ["rmat10s", "rmat12d"])
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS + ["blocked"])
 def test_bc(rg, orc, ctx, graphs, gname, variant):
     hg = graphs(gname)
     off, nbr = hg.csr()
@@ -91,3 +91,36 @@ def test_bc(rg, orc, ctx, graphs, gname, variant):
     ref, rc = orc.bc(off, nbr, ioff, inbr, srcs)
     assert comb == rc
     assert rel_linf(deps, ref) < FP_TOL
+
+
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d", "rmat16s"])
+@pytest.mark.parametrize("threshold", [0, -1])
+def test_bc_blocked_dense_levels(rg, orc, ctx, graphs, gname, threshold):
+    # threshold 0: every forward and backward level on the blocked kernels
+    # (directed graphs use the layouts of both orientations).
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    ioff, inbr = hg.csr(in_edges=True)
+    srcs = hg.pick_bc_sources(3)
+    print(f"bc blocked {gname} {threshold} {srcs}", flush=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    deps, comb, _ = dg.bc(srcs, rg.opts("blocked", threshold=threshold))
+    ref, rc = orc.bc(off, nbr, ioff, inbr, srcs)
+    assert comb == rc
+    assert rel_linf(deps, ref) < FP_TOL
+
+
+def test_bc_blocked_split_hub_block(rg, orc, ctx):
+    # A hub with 2^20 in-edges: its destination block is split over units
+    # whose partial sums the merge kernel adds.
+    leaves = 1 << 20
+    src = np.concatenate([np.zeros(leaves, np.uint32), np.arange(1, 1001, dtype=np.uint32)])
+    dst = np.concatenate([np.arange(1, leaves + 1, dtype=np.uint32), np.arange(1001, 2001, dtype=np.uint32)])
+    hg = rg.HostGraph.from_edges(leaves + 1, src, dst, symmetric=True)
+    off, nbr = hg.csr()
+    dg = rg.DeviceGraph(ctx, hg)
+    for s in ([5], [0], [1500, 3]):
+        deps, comb, _ = dg.bc(np.array(s, np.uint32), rg.opts("blocked", threshold=0))
+        ref, rc = orc.bc(off, nbr, off, nbr, np.array(s, np.uint32))
+        assert comb == rc
+        assert rel_linf(deps, ref) < FP_TOL

@@ -38,14 +38,12 @@ int64_t orc_bfs_levels(int64_t n, const uint64_t* off, const uint32_t* nbr, uint
   return ecc;
 }
 
-static int has_arc(const uint64_t* off, const uint32_t* nbr, uint32_t u, uint32_t v) {
-  for (uint64_t j = off[u]; j < off[u + 1]; ++j)
-    if (nbr[j] == v) return 1;
-  return 0;
-}
-
 int64_t orc_bfs_check_parents(int64_t n, const uint64_t* off, const uint32_t* nbr,
                               uint32_t src, const int32_t* level, const uint32_t* parent) {
+  /* Graph500-style validation: the root is its own parent; every reached
+   * vertex has a parent one level up with an arc parent -> v; unreached
+   * vertices have none.  The arc test is one pass over all arcs (linear in
+   * m whatever the degrees or the order of the lists). */
   if (parent[src] != src) return 1 + (int64_t)src;
   for (int64_t v = 0; v < n; ++v) {
     if (v == (int64_t)src) continue;
@@ -56,9 +54,17 @@ int64_t orc_bfs_check_parents(int64_t n, const uint64_t* off, const uint32_t* nb
     }
     if (p == NONE32 || p >= (uint64_t)n) return 1 + v;
     if (level[p] != level[v] - 1) return 1 + v;
-    if (!has_arc(off, nbr, p, (uint32_t)v)) return 1 + v;
   }
-  return 0;
+  unsigned char* ok = (unsigned char*)calloc((size_t)n + 1, 1);
+  if (!ok) return -1;
+  for (int64_t u = 0; u < n; ++u)
+    for (uint64_t j = off[u]; j < off[u + 1]; ++j)
+      if (parent[nbr[j]] == (uint32_t)u) ok[nbr[j]] = 1;
+  int64_t bad = 0;
+  for (int64_t v = 0; v < n && !bad; ++v)
+    if (v != (int64_t)src && level[v] >= 0 && !ok[v]) bad = 1 + v;
+  free(ok);
+  return bad;
 }
 
 /* ---- PageRank --------------------------------------------------------------

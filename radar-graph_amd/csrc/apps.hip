@@ -610,9 +610,15 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   unsigned long long* cs = reinterpret_cast<unsigned long long*>(ctx->counts.as<int64_t>() + 12);
   if (checksum_out) RG_HIP(hipMemsetAsync(cs, 0, sizeof(int64_t), ctx->stream));
   EventTimer tm(ctx->stream);
+  // RADAR and blocked run dense levels on the blocked layout (each unvisited
+  // vertex scans its in-edges for a frontier parent; no global atomics).
+  const bool blocked = o && (o->variant == RG_RADAR || o->variant == RG_BLOCKED);
   while (F->size > 0) {
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
-    RG_TRY(edge_map(E, op, *F, degsum, N->ids, info));
+    if (blocked && degsum > 0 && F->size + degsum > E.threshold)
+      RG_TRY(bfs_dense_level(g, F->ids, F->size, parents, N->ids, info));
+    else
+      RG_TRY(edge_map(E, op, *F, degsum, N->ids, info));
     if (checksum_out)
       hipLaunchKernelGGL(k_level_cs, ctx->num_cus, 256, 0, ctx->stream, N->ids, info, (uint32_t)(iter + 1), cs);
     int64_t h[2];

@@ -42,6 +42,16 @@ struct BcDenseArgs {
   int64_t* out_info;                   // forward: [count, out-degree sum]
 };
 
+// Whether every bit of the block's skip words is set (all threads read the
+// same LDS words, so the answer is uniform across the workgroup).
+template <int BV>
+__device__ __forceinline__ bool block_done(const unsigned long long* skip) {
+  bool all = true;
+#pragma unroll 4
+  for (int i = 0; i < BV / 64; ++i) all = all && skip[i] == ~0ull;
+  return all;
+}
+
 // Destination d of the block, with sum x: forward, d joins the next
 // frontier when a frontier in-neighbour reached it (x != 0: path counts of
 // visited vertices are positive) and NumPaths[d] (0 while unvisited) = x;
@@ -80,6 +90,9 @@ __global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
     if (__lane_id() == 0) skip[l >> 6] = w;
   }
   __syncthreads();
+  // A block whose destinations are all visited has nothing to do (the merge
+  // of a split block reads no partial of a visited destination).
+  if (block_done<BV>(skip)) return;
   const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
   for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
     const uint32_t s = __builtin_nontemporal_load(a.src + e);
@@ -129,6 +142,83 @@ __global__ __launch_bounds__(THREADS) void k_bc_merge(BcDenseArgs a) {
   if (FWD) wq_finish(q, ea);
 }
 
+// BFS_F (BFS.C:169-190) on a dense level: every unvisited destination of
+// the block takes a frontier in-neighbour as its parent (the smallest-id
+// one seen by the unit, any being a valid BFS parent); split blocks keep
+// the first unit's candidate.  `out` is Parents (uint32).
+template <int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_bfs_dense(BcDenseArgs a, uint32_t* parents, uint32_t* part) {
+  constexpr int BV = 1 << VB;
+  constexpr int NW = THREADS / kWave;
+  __shared__ uint32_t cand[BV];
+  __shared__ unsigned long long skip[BV / 64];
+  __shared__ uint32_t s_q[NW][kQueue];
+  const int64_t u = blockIdx.x;
+  const uint32_t b = a.unit_block[u];
+  const int64_t base = (int64_t)b << VB;
+  const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
+  for (int l = threadIdx.x; l < BV; l += THREADS) {
+    cand[l] = kNone;
+    const int64_t d = base + l;
+    const bool sk = d >= a.n || parents[d] != kNone;
+    const unsigned long long w = __ballot(sk);
+    if (__lane_id() == 0) skip[l >> 6] = w;
+  }
+  __syncthreads();
+  if (block_done<BV>(skip)) return;
+  const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
+  for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
+    const uint32_t s = __builtin_nontemporal_load(a.src + e);
+    const uint32_t l = __builtin_nontemporal_load(a.vl + e);
+    if ((skip[l >> 6] >> (l & 63)) & 1) continue;
+    if (cand[l] <= s) continue;
+    if (!((a.active[s >> 6] >> (s & 63)) & 1)) continue;
+    atomicMin(&cand[l], s);
+  }
+  __syncthreads();
+  if (!single) {
+    uint32_t* p = part + ((size_t)u << VB);
+    for (int l = threadIdx.x; l < BV; l += THREADS) p[l] = cand[l];
+    return;
+  }
+  EdgeArgs ea{};
+  ea.off = a.off;
+  ea.out_ids = a.out_ids;
+  ea.out_info = a.out_info;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  for (int l0 = 0; l0 < BV; l0 += THREADS) {
+    const int l = l0 + threadIdx.x;
+    const uint32_t c = cand[l];
+    const bool join = c != kNone;
+    if (join) parents[base + l] = c;
+    wq_push(q, ea, join, (uint32_t)(base + l));
+  }
+  wq_finish(q, ea);
+}
+
+template <int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_bfs_merge(BcDenseArgs a, uint32_t* parents, const uint32_t* part) {
+  constexpr int BV = 1 << VB;
+  constexpr int NW = THREADS / kWave;
+  constexpr int PER = BV / THREADS;
+  __shared__ uint32_t s_q[NW][kQueue];
+  const uint32_t b = a.multi[blockIdx.x / PER];
+  const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
+  const int64_t d = ((int64_t)b << VB) + l;
+  uint32_t c = kNone;
+  if (d < a.n && parents[d] == kNone)
+    for (uint32_t u = a.block_unit0[b]; u < a.block_unit0[b + 1] && c == kNone; ++u) c = part[((size_t)u << VB) + l];
+  const bool join = c != kNone;
+  if (join) parents[d] = c;
+  EdgeArgs ea{};
+  ea.off = a.off;
+  ea.out_ids = a.out_ids;
+  ea.out_info = a.out_info;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  wq_push(q, ea, join, (uint32_t)d);
+  wq_finish(q, ea);
+}
+
 __global__ void k_set_bits(const uint32_t* ids, int64_t count, unsigned long long* bits) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t v = ids[i];
@@ -143,12 +233,15 @@ __global__ void k_clear_bits(const uint32_t* ids, int64_t count, unsigned long l
 
 }  // namespace
 
-// One dense BC level over the frontier ids[0, count) of g's current
-// orientation.  Forward: out_ids / out_info receive the next frontier (the
-// caller zeroes out_info).  The frontier bitmap and partial sums live in the
+// One dense level over the frontier ids[0, count) of g's current
+// orientation: kind 0 = BC forward, 1 = BC backward, 2 = BFS.  Forward and
+// BFS: out_ids / out_info receive the next frontier (the caller zeroes
+// out_info).  The frontier bitmap and partial sums live in the
 // context's bc_dense buffer; the bitmap is all zero between calls.
-int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, const double* val, double* out,
-                   const uint8_t* visited, uint32_t* out_ids, int64_t* out_info) {
+namespace {
+
+int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const double* val, double* out,
+                const uint8_t* visited, uint32_t* parents, uint32_t* out_ids, int64_t* out_info) {
   rg_ctx* ctx = g->ctx;
   BlockedCSC* B = nullptr;
   RG_TRY(query_layout(g, &B));
@@ -159,8 +252,12 @@ int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, co
   if (sb.bytes < bits_bytes + part_bytes) {
     sb.release();
     RG_TRY(sb.ensure(bits_bytes + part_bytes));
-    RG_TRY(fill<unsigned long long>(ctx, sb.as<unsigned long long>(), nwords, 0ull));
+    ctx->bc_bits_zero = 0;
   }
+  // The partials of a smaller graph may have overwritten words past its own
+  // bitmap: clear the bitmap of this graph unless it is known to be zero.
+  if (ctx->bc_bits_zero < nwords) RG_TRY(fill<unsigned long long>(ctx, sb.as<unsigned long long>(), nwords, 0ull));
+  ctx->bc_bits_zero = nwords;
   unsigned long long* bits = sb.as<unsigned long long>();
   BcDenseArgs a{};
   a.n = g->n;
@@ -182,7 +279,14 @@ int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, co
   hipLaunchKernelGGL(k_set_bits, grid, 256, 0, ctx->stream, ids, count, bits);
   constexpr int PER = (1 << kQueryVB) / kBcThreads;
   hipEvent_t* ev = timing_begin(ctx);
-  if (fwd) {
+  if (kind == 2) {
+    uint32_t* upart = reinterpret_cast<uint32_t*>(a.part);
+    hipLaunchKernelGGL((k_bfs_dense<kQueryVB, kBcThreads>), (int)B->nunits, kBcThreads, 0, ctx->stream, a, parents,
+                       upart);
+    if (B->nmulti)
+      hipLaunchKernelGGL((k_bfs_merge<kQueryVB, kBcThreads>), (int)(B->nmulti * PER), kBcThreads, 0, ctx->stream,
+                         a, parents, upart);
+  } else if (kind == 0) {
     hipLaunchKernelGGL((k_bc_dense<true, kQueryVB, kBcThreads>), (int)B->nunits, kBcThreads, 0, ctx->stream, a);
     if (B->nmulti)
       hipLaunchKernelGGL((k_bc_merge<true, kQueryVB, kBcThreads>), (int)(B->nmulti * PER), kBcThreads, 0,
@@ -197,6 +301,18 @@ int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, co
   hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);
   RG_HIP(hipGetLastError());
   return RG_OK;
+}
+
+}  // namespace
+
+int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, const double* val, double* out,
+                   const uint8_t* visited, uint32_t* out_ids, int64_t* out_info) {
+  return dense_level(g, fwd ? 0 : 1, ids, count, val, out, visited, nullptr, out_ids, out_info);
+}
+
+int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, uint32_t* out_ids,
+                    int64_t* out_info) {
+  return dense_level(g, 2, ids, count, nullptr, nullptr, nullptr, parents, out_ids, out_info);
 }
 
 }  // namespace rg

@@ -62,8 +62,10 @@ struct rg_ctx {
   struct EvPair { hipEvent_t e[2]; };
   std::vector<EvPair> ev_pairs;
   size_t ev_used = 0;
-  // Dense BC levels: frontier bitmap (all zero between levels) + partials.
+  // Dense BFS / BC levels: frontier bitmap + per-unit partials; the first
+  // bc_bits_zero words of the bitmap are all zero between levels.
   rg::DevBuf bc_dense;
+  int64_t bc_bits_zero = 0;
   // Multi-source BFS state (opaque, msbfs.hip), grown on demand.
   void* ms = nullptr;
   // Device array holding the last PageRank ranks / BC dependencies computed

@@ -226,8 +226,19 @@ __global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
 // word (lane 0 of each wave; the 64 lanes hold 64 aligned vertices), the
 // next-frontier list and the statistics.  Called by whole waves.
 template <int W>
+__device__ __forceinline__ void batch_bits(int nsrc, ull* allbits) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int rem = nsrc - 64 * w;
+    allbits[w] = rem >= 64 ? ~0ull : rem <= 0 ? 0ull : ((1ull << rem) - 1);
+  }
+}
+
+template <int W>
 __device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
                                             int64_t d, bool valid, const ull* accw, uint32_t level) {
+  ull allbits[W];
+  batch_bits<W>(a.nsrc, allbits);
   ull nn[W];
   bool any = false;
 #pragma unroll
@@ -235,15 +246,21 @@ __device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, Wav
     nn[w] = 0;
     if (valid) {
       const size_t k = (size_t)d * W + w;
-      const ull sn = a.seen[k];
-      nn[w] = accw[w] & ~sn;
-      a.front_nxt[k] = nn[w];
-      if (nn[w]) a.seen[k] = sn | nn[w];
+      if (accw[w]) {
+        const ull sn = a.seen[k];
+        nn[w] = accw[w] & ~sn & allbits[w];
+        if (nn[w]) {
+          a.front_nxt[k] = nn[w];
+          a.seen[k] = sn | nn[w];
+        }
+      }
     }
     any |= nn[w] != 0;
   }
+  // front_nxt and active_nxt are all zero when a level starts (each level
+  // clears what it used), so only new words are written.
   const ull word = __ballot(any);
-  if (__lane_id() == 0 && d < a.n) a.active_nxt[d >> 6] = word;
+  if (__lane_id() == 0 && word) a.active_nxt[d >> 6] = word;
   ull deg = 0;
   if (any) {
     deg = a.off[d + 1] - a.off[d];
@@ -275,11 +292,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a) {
   stats_init<W, THREADS>(st);
   // Destinations every source of the batch has reached need no gathers.
   ull allbits[W];
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    const int rem = a.nsrc - 64 * w;
-    allbits[w] = rem >= 64 ? ~0ull : rem <= 0 ? 0ull : ((1ull << rem) - 1);
-  }
+  batch_bits<W>(a.nsrc, allbits);
   for (int l = threadIdx.x; l < BV; l += THREADS) {
     const int64_t d = base + l;
     bool f = true;
@@ -291,6 +304,11 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a) {
     if (__lane_id() == 0) full[l >> 6] = word;
   }
   __syncthreads();
+  // Every destination reached by every source: nothing arrives (the merge
+  // of a split block masks its partials with ~seen, which is then empty).
+  bool all = true;
+  for (int i = 0; i < BV / 64; ++i) all = all && full[i] == ~0ull;
+  if (all) return;
   const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
   for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
     const uint32_t s = ld_nt(a.src + e);

@@ -44,6 +44,11 @@ int static_heavy_list(rg_graph* g, int heavy_deg);
 int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, const double* val, double* out,
                    const uint8_t* visited, uint32_t* out_ids, int64_t* out_info);
 
+// One dense BFS level (BFS_F: unvisited vertices with a frontier in-neighbour
+// take one as parent and join the next frontier) on the blocked layout.
+int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, uint32_t* out_ids,
+                    int64_t* out_info);
+
 int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
             int64_t* combined_out, double* ms_out);
 

@@ -48,7 +48,7 @@ def test_pagerank(rg, orc, ctx, graphs, gname, variant):
 
 
 @pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d"])
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS + ["blocked"])
 def test_bfs(rg, orc, ctx, graphs, gname, variant):
     hg = graphs(gname)
     off, nbr = hg.csr()
@@ -124,3 +124,41 @@ def test_bc_blocked_split_hub_block(rg, orc, ctx):
         ref, rc = orc.bc(off, nbr, off, nbr, np.array(s, np.uint32))
         assert comb == rc
         assert rel_linf(deps, ref) < FP_TOL
+
+
+def check_bfs(rg, orc, dg, off, nbr, srcs, o):
+    for s in srcs:
+        r = dg.bfs(int(s), o)
+        lv, ecc = orc.bfs_levels(off, nbr, int(s))
+        assert orc.check_parents(off, nbr, int(s), lv, r.parents) == 0, f"source {s}"
+        assert (r.iters, r.reached) == (ecc + 1, int((lv >= 0).sum()))
+        q, _ = dg.bfs_query(int(s), o)
+        assert q.astuple() == orc.bfs_summary(off, lv)
+
+
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d", "rmat16s"])
+@pytest.mark.parametrize("threshold", [0, -1])
+def test_bfs_blocked_dense_levels(rg, orc, ctx, graphs, gname, threshold):
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    dg = rg.DeviceGraph(ctx, hg)
+    print(f"bfs blocked {gname} {threshold}", flush=True)
+    check_bfs(rg, orc, dg, off, nbr, hg.pick_sources(4), rg.opts("blocked", threshold=threshold))
+
+
+def test_bfs_blocked_split_hub_block(rg, orc, ctx):
+    leaves = 1 << 20
+    src = np.concatenate([np.zeros(leaves, np.uint32), np.arange(1, 1001, dtype=np.uint32)])
+    dst = np.concatenate([np.arange(1, leaves + 1, dtype=np.uint32), np.arange(1001, 2001, dtype=np.uint32)])
+    hg = rg.HostGraph.from_edges(leaves + 1, src, dst, symmetric=True)
+    off, nbr = hg.csr()
+    dg = rg.DeviceGraph(ctx, hg)
+    check_bfs(rg, orc, dg, off, nbr, [5, 0, 1500], rg.opts("blocked", threshold=0))
+
+
+def test_bfs_radar_on_degree_sorted(rg, orc, ctx, graphs):
+    hg = graphs("rmat16s")
+    dg = rg.DeviceGraph(ctx, hg)
+    sg, ids = dg.degree_sort()
+    off, nbr = sg.download()
+    check_bfs(rg, orc, sg, off, nbr, ids[hg.pick_sources(4)], rg.opts("radar"))

@@ -231,3 +231,20 @@ def test_tc_totals(rg, orc, meta, name, scale):
     assert tot == meta["tc"][name]["pno"] == meta["tc"][name]["ds"]
     assert tot == orc.tc_verify(off, nb)
     assert int(pv.sum()) == 3 * tot   # tc_local.cc:84
+
+
+def test_check_parents_catches_planted_errors(orc):
+    # Star 0 - {1..4} plus the path 4 - 5: every planted mistake (wrong
+    # root, parent a level too high, parent without an arc, unreached vertex
+    # with a parent) must be reported; the correct tree passes.
+    src = np.array([0, 0, 0, 0, 4])
+    dst = np.array([1, 2, 3, 4, 5])
+    off, nbr = orc.csr_from_edges(7, src, dst, symmetric=True)
+    lv, ecc = orc.bfs_levels(off, nbr, 0)
+    assert list(lv) == [0, 1, 1, 1, 1, 2, -1] and ecc == 2
+    good = np.array([0, 0, 0, 0, 0, 4, 0xFFFFFFFF], np.uint32)
+    assert orc.check_parents(off, nbr, 0, lv, good) == 0
+    for v, p in [(0, 1), (5, 0), (5, 3), (2, 1), (6, 5)]:
+        bad = good.copy()
+        bad[v] = p
+        assert orc.check_parents(off, nbr, 0, lv, bad) == 1 + v, (v, p)

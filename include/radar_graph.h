@@ -224,7 +224,9 @@ int rg_bfs_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_op
 
 /* Radii (PNO Ligra/apps/Radii.C:98-157): bit-parallel BFS from up to 64
  * sources (source i owns bit i); radii[v] = last round in which v's visited
- * word changed, -1 if unreached.  rounds_out = printed "Iters". */
+ * word changed, -1 if unreached.  rounds_out = printed "Iters".  With
+ * o->variant RG_RADAR or RG_BLOCKED the starts run as one batch of the
+ * bit-parallel BFS (rg_bfs_batch's engine: blocked dense levels). */
 int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o,
              int32_t* radii_out, int64_t* rounds_out, double* ms_out);
 

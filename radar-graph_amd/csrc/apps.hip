@@ -1003,6 +1003,8 @@ int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o,
              int64_t* rounds_out, double* ms_out) {
   if (!g || (nstarts > 0 && !starts)) return set_error(RG_EINVAL, "rg_radii: bad arguments");
   RG_HIP(hipSetDevice(g->ctx->device));
+  if (o && (o->variant == RG_RADAR || o->variant == RG_BLOCKED))
+    return rg::radii_batch(g, starts, nstarts, o, radii_out, rounds_out, ms_out);
   return rg::radii_push(g, starts, nstarts, o, radii_out, rounds_out, ms_out);
 }
 

@@ -86,6 +86,7 @@ struct MsArgs {
   ull* edges;
   ull* cs;
   int32_t* levels;                     // optional [nsrc][n]
+  int32_t* radii;                      // optional [n]: last level a vertex gained a bit (Radii.C)
 };
 
 // ---------------------------------------------------------------------------
@@ -213,6 +214,7 @@ __global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
       }
     }
     atomicOr(a.active_nxt + (d >> 6), 1ull << (d & 63));
+    if (a.radii) a.radii[d] = (int32_t)level;
     stats_add<W>(a, st, d, level, nn, a.off[d + 1] - a.off[d]);
   }
   stats_flush<W, 256>(a, st);
@@ -264,6 +266,7 @@ __device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, Wav
   ull deg = 0;
   if (any) {
     deg = a.off[d + 1] - a.off[d];
+    if (a.radii) a.radii[d] = (int32_t)level;
     stats_add<W>(a, st, (uint32_t)d, level, nn, deg);
   }
   wq_push(q, ea, any, (uint32_t)d);
@@ -446,6 +449,7 @@ __global__ void k_ms_init(MsArgs a, int W, const uint32_t* sources, int64_t nuni
     a.cs[s] = level_hash(v, 0);
     a.lvl_new[s] = 0;
     if (a.levels) a.levels[(size_t)s * a.n + v] = 0;
+    if (a.radii) a.radii[v] = 0;
   }
   for (int64_t i = threadIdx.x; i < nuniq; i += blockDim.x) {
     const uint32_t v = a.list_cur[i];
@@ -531,7 +535,7 @@ int ms_prepare(rg_ctx* ctx, MsState& S, int64_t n, int W, int64_t nunits) {
 
 template <int W>
 int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32_t* sources, int nsrc,
-           int64_t threshold, int32_t* levels_dev, int64_t* levels_run) {
+           int64_t threshold, int32_t* levels_dev, int32_t* radii_dev, int64_t* levels_run) {
   rg_ctx* ctx = g->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = g->n;
@@ -564,6 +568,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   ull* reached = stats + 3 * kMsMaxSources;
   int64_t* depth = reinterpret_cast<int64_t*>(stats + 4 * kMsMaxSources);
   a.levels = levels_dev;
+  a.radii = radii_dev;
   int cur = 0;
   auto bind = [&](int c) {
     a.front_cur = S.front[c].as<ull>();
@@ -633,16 +638,14 @@ int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc) {
   return 4 * n * W * 8 + (units << kMsVB) * W * 8 + n * 12 + n / 4 + 4096;
 }
 
-int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
-              rg_bfs_summary* out, double* ms_out) {
+namespace {
+
+// Runs one batch; per-source summaries into out (nullable), level arrays or
+// radii into the device buffers given (nullable).
+int batch_run(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_dev,
+              int32_t* radii_dev, rg_bfs_summary* out, double* ms_out) {
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
-  if (nsrc < 1 || nsrc > kMsMaxSources)
-    return set_error(RG_EINVAL, "rg_bfs_batch: between 1 and 256 sources per batch");
-  if (!sources || !out) return set_error(RG_EINVAL, "rg_bfs_batch: null argument");
-  for (int i = 0; i < nsrc; ++i)
-    if ((int64_t)sources[i] >= n) return set_error(RG_EINVAL, "rg_bfs_batch: source out of range");
-  if (n >= (int64_t)1 << 32) return set_error(RG_EUNSUPPORTED, "rg_bfs_batch: n >= 2^32");
   const int W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
   BlockedCSC* B = nullptr;
   RG_TRY(query_layout(g, &B));
@@ -662,6 +665,39 @@ int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
     ctx->ms = S;
   }
   RG_TRY(ms_prepare(ctx, *S, n, W, B->nunits));
+  EventTimer tm(ctx->stream);
+  int64_t steps = 0;
+  int rc = W == 1   ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps)
+           : W == 2 ? ms_run<2>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps)
+                    : ms_run<4>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps);
+  if (rc != RG_OK) return rc;
+  double ms = tm.stop_ms();
+  if (out) {
+    std::vector<ull> h(5 * kMsMaxSources);
+    RG_TRY(ctx_copy(ctx, h.data(), S->stats.p, sizeof(ull) * h.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < nsrc; ++s) {
+      out[s].iters = (int64_t)h[4 * kMsMaxSources + s] + 1;
+      out[s].reached = (int64_t)h[3 * kMsMaxSources + s];
+      out[s].edges = (int64_t)h[kMsMaxSources + s];
+      out[s].checksum = h[2 * kMsMaxSources + s];
+    }
+  }
+  if (ms_out) *ms_out = ms;
+  return RG_OK;
+}
+
+}  // namespace
+
+int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
+              rg_bfs_summary* out, double* ms_out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  if (nsrc < 1 || nsrc > kMsMaxSources)
+    return set_error(RG_EINVAL, "rg_bfs_batch: between 1 and 256 sources per batch");
+  if (!sources || !out) return set_error(RG_EINVAL, "rg_bfs_batch: null argument");
+  for (int i = 0; i < nsrc; ++i)
+    if ((int64_t)sources[i] >= n) return set_error(RG_EINVAL, "rg_bfs_batch: source out of range");
+  if (n >= (int64_t)1 << 32) return set_error(RG_EUNSUPPORTED, "rg_bfs_batch: n >= 2^32");
   DevBuf lv;
   int32_t* levels_dev = nullptr;
   if (levels_out) {
@@ -669,24 +705,41 @@ int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
     levels_dev = lv.as<int32_t>();
     RG_TRY(fill<int32_t>(ctx, levels_dev, (int64_t)nsrc * n, -1));
   }
-  EventTimer tm(ctx->stream);
-  int64_t steps = 0;
-  int rc = W == 1   ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, &steps)
-           : W == 2 ? ms_run<2>(g, *S, B, E, sources, nsrc, threshold, levels_dev, &steps)
-                    : ms_run<4>(g, *S, B, E, sources, nsrc, threshold, levels_dev, &steps);
-  if (rc != RG_OK) return rc;
-  double ms = tm.stop_ms();
-  std::vector<ull> h(5 * kMsMaxSources);
-  RG_TRY(ctx_copy(ctx, h.data(), S->stats.p, sizeof(ull) * h.size(), hipMemcpyDeviceToHost));
-  for (int s = 0; s < nsrc; ++s) {
-    out[s].iters = (int64_t)h[4 * kMsMaxSources + s] + 1;
-    out[s].reached = (int64_t)h[3 * kMsMaxSources + s];
-    out[s].edges = (int64_t)h[kMsMaxSources + s];
-    out[s].checksum = h[2 * kMsMaxSources + s];
-  }
-  if (ms_out) *ms_out = ms;
+  RG_TRY(batch_run(g, sources, nsrc, o, levels_dev, nullptr, out, ms_out));
   if (levels_out)
     RG_TRY(ctx_copy(ctx, levels_out, levels_dev, sizeof(int32_t) * (size_t)nsrc * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+// Radii (Radii.C:98-157) as one bit-parallel batch: radii[v] = the last
+// level at which v gained a source bit (max distance from the starts), -1
+// if unreached; rounds = levels with a nonempty frontier.  Only the set of
+// start vertices matters (a repeated start in the reference overwrites its
+// own word, Radii.C:124-126), so starts are deduplicated.
+int radii_batch(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o, int32_t* radii_out,
+                int64_t* rounds_out, double* ms_out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  if (nstarts < 0 || nstarts > 64) return set_error(RG_EINVAL, "rg_radii: 0..64 starts");
+  for (int i = 0; i < nstarts; ++i)
+    if ((int64_t)starts[i] >= n) return set_error(RG_EINVAL, "rg_radii: start out of range");
+  std::vector<uint32_t> uniq(starts, starts + nstarts);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  DevBuf& b_rad = ctx->scratch[0];
+  RG_TRY(b_rad.ensure(sizeof(int32_t) * n));
+  int32_t* radii = b_rad.as<int32_t>();
+  RG_TRY(fill<int32_t>(ctx, radii, n, -1));
+  int64_t rounds = 0;
+  double ms = 0.0;
+  if (!uniq.empty()) {
+    std::vector<rg_bfs_summary> sum(uniq.size());
+    RG_TRY(batch_run(g, uniq.data(), (int)uniq.size(), o, nullptr, radii, sum.data(), &ms));
+    for (const auto& x : sum) rounds = std::max(rounds, x.iters);
+  }
+  if (rounds_out) *rounds_out = rounds;
+  if (ms_out) *ms_out = ms;
+  if (radii_out) RG_TRY(ctx_copy(ctx, radii_out, radii, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 

@@ -28,6 +28,9 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
 int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
               rg_bfs_summary* out, double* ms_out);
 
+int radii_batch(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o, int32_t* radii_out,
+                int64_t* rounds_out, double* ms_out);
+
 // Builds the blocked in-edge layout the batches use (once per graph).
 int bfs_batch_prepare(rg_graph* g);
 // Device bytes one context holds for batches of nsrc sources on an

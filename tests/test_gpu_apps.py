@@ -64,8 +64,8 @@ def test_bfs(rg, orc, ctx, graphs, gname, variant):
         assert r.edges == int(np.diff(off)[lv >= 0].sum())
 
 
-@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d"])
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d", "rmat16s"])
+@pytest.mark.parametrize("variant", VARIANTS + ["blocked"])
 def test_radii(rg, orc, ctx, graphs, gname, variant):
     hg = graphs(gname)
     off, nbr = hg.csr()
@@ -162,3 +162,20 @@ def test_bfs_radar_on_degree_sorted(rg, orc, ctx, graphs):
     sg, ids = dg.degree_sort()
     off, nbr = sg.download()
     check_bfs(rg, orc, sg, off, nbr, ids[hg.pick_sources(4)], rg.opts("radar"))
+
+
+@pytest.mark.parametrize("threshold", [0, -1, 1 << 60])
+def test_radii_batch_edge_cases(rg, orc, ctx, graphs, threshold):
+    # Repeated starts, fewer than 64 starts and no starts, on the blocked
+    # (bit-parallel batch) engine, all levels dense / default / all sparse.
+    hg = graphs("rmat12d")
+    off, nbr = hg.csr()
+    dg = rg.DeviceGraph(ctx, hg)
+    o = rg.opts("blocked", threshold=threshold)
+    base = rg.radii_starts(hg.n)
+    for starts in (np.concatenate([base[:10], base[:3]]), base[:1], base):
+        rad, rounds, _ = dg.radii(starts, o)
+        ref, rr = orc.radii(off, nbr, starts)
+        assert rounds == rr and np.array_equal(rad, ref)
+    rad, rounds, _ = dg.radii(np.zeros(0, np.uint32), o)
+    assert rounds == 0 and (rad == -1).all()

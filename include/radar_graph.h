@@ -190,6 +190,18 @@ void rg_opts_default(rg_opts* o);
 int rg_pagerank(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
                 int64_t* iters_out, double* l1_out, double* ms_out);
 
+/* PageRankDelta (PNO Ligra/apps/PageRankDelta.C:102-154): p = 0, Delta =
+ * 1/n, every vertex active; per round nghSum[d] += Delta[s]/outdeg(s) over
+ * the arcs of active s (edgeMap with Ligra's m/20 switch), then the
+ * vertexFilter (round 1: Delta = 0.85 nghSum + 0.15/n, p += Delta, Delta -=
+ * 1/n; later rounds: Delta = 0.85 nghSum, p += Delta when kept) keeps the
+ * vertices with |Delta| > 0.01 p; stop when sum |Delta| < 1e-7 or after
+ * maxiters.  iters_out = printed "Num Iters" (maxiters+1 when not
+ * converged), l1_out = "L1_Norm", ranks_out (nullable) = p.  RG_BLOCKED
+ * runs dense rounds on the blocked PageRank layout. */
+int rg_pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
+                      int64_t* iters_out, double* l1_out, double* ms_out);
+
 /* BFS (PNO Ligra/apps/BFS.C:213-258): parents via first successful CAS,
  * parents[src] = src, UINT32_MAX when unreached.  iters_out = rounds
  * (printed "Num. Iters"), reached_out = tree size, edges_out = sum of

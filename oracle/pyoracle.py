@@ -42,6 +42,7 @@ def _s(name, res, *args):
 _s("orc_bfs_levels", C.c_int64, C.c_int64, _u64p, _u32p, C.c_uint32, _i32p)
 _s("orc_bfs_check_parents", C.c_int64, C.c_int64, _u64p, _u32p, C.c_uint32, _i32p, _u32p)
 _s("orc_pagerank", None, C.c_int64, _u64p, _u32p, C.c_int64, C.c_int, _f64p, _i64p, _f64p)
+_s("orc_pagerank_delta", None, C.c_int64, _u64p, _u32p, C.c_int64, _f64p, _i64p, _f64p)
 _s("orc_pagerank_pull", None, C.c_int64, _u64p, _u64p, _u32p, C.c_int64, _f64p, _i64p, _f64p)
 _s("orc_radii", None, C.c_int64, _u64p, _u32p, _u32p, C.c_int, _i32p, _i64p)
 _s("orc_bc", None, C.c_int64, _u64p, _u32p, _u64p, _u32p, _u32p, C.c_int, _f64p, _i64p)
@@ -110,6 +111,15 @@ def pagerank(off, nbr, maxiters, nthreads=1):
     it, l1 = C.c_int64(), C.c_double()
     _lib.orc_pagerank(n, _p(off, _u64p), _p(nbr, _u32p), maxiters, nthreads, _p(r, _f64p),
                       C.byref(it), C.byref(l1))
+    return r, it.value, l1.value
+
+
+def pagerank_delta(off, nbr, maxiters):
+    off, nbr = _c(off, np.uint64), _c(nbr, np.uint32)
+    n = len(off) - 1
+    r = np.empty(n, np.float64)
+    it, l1 = C.c_int64(), C.c_double()
+    _lib.orc_pagerank_delta(n, _p(off, _u64p), _p(nbr, _u32p), maxiters, _p(r, _f64p), C.byref(it), C.byref(l1))
     return r, it.value, l1.value
 
 

@@ -144,6 +144,66 @@ void orc_pagerank(int64_t n, const uint64_t* off, const uint32_t* nbr, int64_t m
   free(p_next);
 }
 
+/* ---- PageRankDelta -----------------------------------------------------------
+ * push-new-orig Ligra/apps/PageRankDelta.C:102-154: p = 0, Delta = 1/n,
+ * every vertex active; per round nghSum[d] += Delta[s]/outdeg(s) over the
+ * arcs of active s (PR_F, 30-48), then PR_Vertex_F_FirstRound (51-64) in
+ * round 1 and PR_Vertex_F (66-78) afterwards decide the next active set
+ * (|Delta| > 0.01 p), L1 = sum |Delta| (plusReduce), stop when L1 < 1e-7.
+ * Sources ascending, neighbours in list order: a one-thread run's order. */
+void orc_pagerank_delta(int64_t n, const uint64_t* off, const uint32_t* nbr, int64_t maxiters,
+                        double* ranks, int64_t* iters, double* l1) {
+  const double damping = 0.85, epsilon = 0.0000001, epsilon2 = 0.01;
+  const double one_over_n = 1 / (double)n;
+  const double added = (1 - damping) * one_over_n;
+  double* p = (double*)malloc(sizeof(double) * (size_t)n);
+  double* delta = (double*)malloc(sizeof(double) * (size_t)n);
+  double* ngh = (double*)malloc(sizeof(double) * (size_t)n);
+  unsigned char* fr = (unsigned char*)malloc((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    p[i] = 0.0;
+    delta[i] = one_over_n;
+    ngh[i] = 0.0;
+    fr[i] = 1;
+  }
+  int64_t round = 0;
+  double L1 = 0.0;
+  while (round++ < maxiters) {
+    for (int64_t s = 0; s < n; ++s) {
+      if (!fr[s]) continue;
+      uint64_t deg = off[s + 1] - off[s];
+      for (uint64_t j = off[s]; j < off[s + 1]; ++j) ngh[nbr[j]] = ngh[nbr[j]] + delta[s] / (double)(uint32_t)deg;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      if (round == 1) {
+        delta[i] = damping * ngh[i] + added;
+        p[i] += delta[i];
+        delta[i] -= one_over_n;
+        fr[i] = fabs(delta[i]) > epsilon2 * p[i];
+      } else {
+        delta[i] = ngh[i] * damping;
+        if (fabs(delta[i]) > epsilon2 * p[i]) {
+          p[i] += delta[i];
+          fr[i] = 1;
+        } else {
+          fr[i] = 0;
+        }
+      }
+    }
+    for (int64_t i = 0; i < n; ++i) ngh[i] = fabs(delta[i]);
+    L1 = blocked_sum(ngh, n);
+    if (L1 < epsilon) break;
+    for (int64_t i = 0; i < n; ++i) ngh[i] = 0.0;
+  }
+  memcpy(ranks, p, sizeof(double) * (size_t)n);
+  *iters = round;
+  *l1 = L1;
+  free(p);
+  free(delta);
+  free(ngh);
+  free(fr);
+}
+
 void orc_pagerank_pull(int64_t n, const uint64_t* off, const uint64_t* in_off,
                        const uint32_t* in_nbr, int64_t maxiters, double* ranks,
                        int64_t* iters, double* l1) {

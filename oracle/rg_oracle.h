@@ -39,6 +39,11 @@ int64_t orc_bfs_check_parents(int64_t n, const uint64_t* off, const uint32_t* nb
 void orc_pagerank(int64_t n, const uint64_t* off, const uint32_t* nbr, int64_t maxiters,
                   int nthreads, double* ranks, int64_t* iters, double* l1);
 
+/* PageRankDelta (push-new-orig Ligra/apps/PageRankDelta.C:102-154): final
+ * ranks p, printed "Num Iters" and L1_Norm. */
+void orc_pagerank_delta(int64_t n, const uint64_t* off, const uint32_t* nbr, int64_t maxiters,
+                        double* ranks, int64_t* iters, double* l1);
+
 /* Pull PageRank of the push-pull tree (push-pull Ligra/apps/PageRank.C:122,
  * threshold 0 => edgeMapDense every iteration, ligra.h:61-88): p_next[v] sums
  * p_curr[s]/outdeg(s) over in-neighbours s in in-list order. */

@@ -548,6 +548,173 @@ int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks
 }
 
 // ---------------------------------------------------------------------------
+// PageRankDelta (PageRankDelta.C:102-154).
+
+__global__ void k_prd_init(const uint64_t* off, int64_t n, double* p, double* delta, double* ngh, double* contrib,
+                           double one_over_n) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t deg = off[v + 1] - off[v];
+    p[v] = 0.0;
+    delta[v] = one_over_n;
+    ngh[v] = 0.0;
+    contrib[v] = deg ? one_over_n / (double)(uint32_t)deg : 0.0;
+  }
+}
+
+// Fused vertexFilter of one round: PR_Vertex_F_FirstRound (round 1,
+// PageRankDelta.C:51-64) or PR_Vertex_F (66-78), the L1 term |Delta|, the
+// reset of nghSum (80-88) and the next round's quotient Delta[s]/outdeg(s)
+// for active s (0 for inactive ones, so an all-vertices pass adds nothing
+// for them).  Active vertices are appended to out_ids with their out-degree
+// sum.  nghSum comes from `ngh` or, after a blocked pass, from the unit
+// partials of v's block added in unit order.  Multiplies and adds are
+// rounded separately, as the reference's host code computes them.
+__global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t n, double* p, double* delta,
+                                                    double* ngh, double* contrib, const double* part,
+                                                    const uint32_t* block_unit0, int vb, int first,
+                                                    double damping, double added, double one_over_n,
+                                                    double eps2, double* partial, uint32_t* out_ids,
+                                                    int64_t* out_info) {
+  __shared__ uint32_t s_q[4][kQueue];
+  __shared__ double red[256];
+  EdgeArgs ea{};
+  ea.off = off;
+  ea.out_ids = out_ids;
+  ea.out_info = out_info;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  double l1 = 0.0;
+  const int64_t lmask = (int64_t(1) << vb) - 1;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    bool act = false;
+    if (v < n) {
+      double ns;
+      if (part) {
+        ns = 0.0;
+        const int64_t b = v >> vb;
+        for (uint32_t u = block_unit0[b]; u < block_unit0[b + 1]; ++u) ns += part[((int64_t)u << vb) + (v & lmask)];
+      } else {
+        ns = ngh[v];
+        ngh[v] = 0.0;
+      }
+      double pv = p[v], dl;
+      if (first) {
+        dl = __dadd_rn(__dmul_rn(damping, ns), added);
+        pv = __dadd_rn(pv, dl);
+        dl = __dsub_rn(dl, one_over_n);
+        act = fabs(dl) > __dmul_rn(eps2, pv);
+      } else {
+        dl = __dmul_rn(ns, damping);
+        act = fabs(dl) > __dmul_rn(eps2, pv);
+        if (act) pv = __dadd_rn(pv, dl);
+      }
+      p[v] = pv;
+      delta[v] = dl;
+      l1 += fabs(dl);
+      const uint64_t deg = off[v + 1] - off[v];
+      contrib[v] = (act && deg) ? dl / (double)(uint32_t)deg : 0.0;
+    }
+    wq_push(q, ea, act, (uint32_t)v);
+  }
+  wq_finish(q, ea);
+  red[threadIdx.x] = l1;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_prd_l1(const double* partial, int nparts, double* out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out, int64_t* iters_out,
+                   double* l1_out, double* ms_out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  if (n == 0) return set_error(RG_EINVAL, "rg_pagerank_delta: empty graph");
+  Engine E;
+  RG_TRY(make_engine(g, o, sizeof(double), E));
+  const bool blocked = o && o->variant == RG_BLOCKED;
+  DevBuf &b_p = ctx->scratch[0], &b_delta = ctx->scratch[1], &b_ngh = ctx->scratch[2], &b_part = ctx->scratch[3],
+         &b_contrib = ctx->scratch[4], &b_unit = ctx->scratch[5];
+  RG_TRY(b_p.ensure(sizeof(double) * n));
+  RG_TRY(b_delta.ensure(sizeof(double) * n));
+  RG_TRY(b_ngh.ensure(sizeof(double) * n));
+  RG_TRY(b_contrib.ensure(sizeof(double) * n));
+  const int vgrid = std::min<int>(grid_for(ctx, n), 1024);
+  RG_TRY(b_part.ensure(sizeof(double) * (vgrid + 8)));
+  double* p = b_p.as<double>();
+  double* delta = b_delta.as<double>();
+  double* ngh = b_ngh.as<double>();
+  double* contrib = b_contrib.as<double>();
+  double* partial = b_part.as<double>();
+  double* l1_dev = partial + vgrid;
+  int64_t* zero = ctx->counts.as<int64_t>() + 32;
+  int64_t* info = ctx->counts.as<int64_t>() + 34;   // [count, out-degree sum]
+  RG_HIP(hipMemsetAsync(zero, 0, sizeof(int64_t), ctx->stream));
+  Frontier fa, fb;
+  RG_TRY(ctx_frontiers(ctx, n, fa, fb));
+  const double damping = 0.85, epsilon = 0.0000001, epsilon2 = 0.01;
+  const double one_over_n = 1 / (double)n;
+  const double added = (1 - damping) * one_over_n;
+  hipLaunchKernelGGL(k_prd_init, grid_for(ctx, n), 256, 0, ctx->stream, g->off, n, p, delta, ngh, contrib,
+                     one_over_n);
+  RG_HIP(hipGetLastError());
+  Frontier* F = &fa;
+  Frontier* N = &fb;
+  F->all = true;
+  F->size = n;
+  int64_t degsum = g->m;
+  PrOp op{contrib, ngh, zero};
+  double L1 = 0.0;
+  int64_t round = 0;
+  EventTimer tm(ctx->stream);
+  while (round++ < maxiters) {
+    const double* part = nullptr;
+    const uint32_t* bu0 = nullptr;
+    int vb = 0;
+    if (blocked && (F->all || F->size + degsum > E.threshold)) {
+      RG_TRY(blocked_pr_pass(g, o, contrib, b_unit, zero, &vb, &bu0));
+      part = b_unit.as<double>();
+    } else {
+      RG_TRY(edge_map(E, op, *F, degsum, nullptr, nullptr));
+    }
+    RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
+    hipLaunchKernelGGL(k_prd_vertex, vgrid, 256, 0, ctx->stream, g->off, n, p, delta, ngh, contrib, part, bu0, vb,
+                       round == 1 ? 1 : 0, damping, added, one_over_n, epsilon2, partial, N->ids, info);
+    hipLaunchKernelGGL(k_prd_l1, 1, 256, 0, ctx->stream, partial, vgrid, l1_dev);
+    RG_HIP(hipGetLastError());
+    int64_t h[2];
+    RG_TRY(ctx_read_count(ctx, info, h, 2));
+    RG_TRY(ctx_copy(ctx, &L1, l1_dev, sizeof(double), hipMemcpyDeviceToHost));
+    if (L1 < epsilon) break;
+    N->all = false;
+    N->size = h[0];
+    degsum = h[1];
+    std::swap(F, N);
+  }
+  const double ms = tm.stop_ms();
+  ctx->last_result = p;
+  if (iters_out) *iters_out = round;
+  if (l1_out) *l1_out = L1;
+  if (ms_out) *ms_out = ms;
+  if (ranks_out) RG_TRY(rg::ctx_copy(ctx, ranks_out, p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------
 // BFS (BFS.C:213-258).
 
 __global__ void k_bfs_edges(const uint32_t* parents, const uint64_t* off, int64_t n, int64_t* out) {
@@ -997,6 +1164,13 @@ int rg_bfs_query(rg_graph* g, uint32_t src, const rg_opts* o, rg_bfs_summary* ou
   RG_TRY(rg::bfs_push(g, src, o, nullptr, &out->iters, &out->reached, &out->edges, ms_out, nullptr, &cs));
   out->checksum = cs;
   return RG_OK;
+}
+
+int rg_pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out, int64_t* iters_out,
+                      double* l1_out, double* ms_out) {
+  if (!g || maxiters < 0) return set_error(RG_EINVAL, "rg_pagerank_delta: bad arguments");
+  RG_HIP(hipSetDevice(g->ctx->device));
+  return rg::pagerank_delta(g, maxiters, o, ranks_out, iters_out, l1_out, ms_out);
 }
 
 int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o, int32_t* radii_out,

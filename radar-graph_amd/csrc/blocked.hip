@@ -313,6 +313,40 @@ int pagerank_prepare(rg_graph* g, const rg_opts* o) {
   return blocked_build(g, vb, sb, &g->blocked, &B);
 }
 
+// One edge pass of the blocked engine: per-unit partial sums of contrib[s]
+// over each destination block's in-edges into unit_part (grown as needed).
+// The caller adds a block's partials in unit order: *vb_out and
+// *block_unit0_out describe the layout.
+int blocked_pr_pass(rg_graph* g, const rg_opts* o, const double* contrib, DevBuf& unit_part,
+                    const int64_t* zero, int* vb_out, const uint32_t** block_unit0_out) {
+  rg_ctx* ctx = g->ctx;
+  int vb, sb;
+  layout_bits(o, &vb, &sb);
+  BlockedCSC* B = nullptr;
+  RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B));
+  RG_TRY(unit_part.ensure(sizeof(double) * ((size_t)B->nunits << vb)));
+  const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
+  const bool u4 = o && o->unroll == 4;
+  void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const double*, double*, const int64_t*);
+#define RG_PICK(VB)                                                                   \
+  kern = threads == 512 ? (u4 ? k_blk_pr<VB, 512, true, 4> : k_blk_pr<VB, 512, true, 1>) \
+                        : (u4 ? k_blk_pr<VB, 1024, true, 4> : k_blk_pr<VB, 1024, true, 1>)
+  if (vb == 14) RG_PICK(14);
+  else if (vb == 13) RG_PICK(13);
+  else if (vb == 12) RG_PICK(12);
+  else if (vb == 11) RG_PICK(11);
+  else RG_PICK(10);
+#undef RG_PICK
+  hipEvent_t* ev = timing_begin(ctx);
+  hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, contrib,
+                     unit_part.as<double>(), zero);
+  if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
+  RG_HIP(hipGetLastError());
+  *vb_out = vb;
+  *block_unit0_out = B->block_unit0;
+  return RG_OK;
+}
+
 int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
                      int64_t* iters_out, double* l1_out, double* ms_out) {
   rg_ctx* ctx = g->ctx;

@@ -37,6 +37,11 @@ int bfs_batch_prepare(rg_graph* g);
 // (n, m) graph.
 int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc);
 
+// One blocked PageRank edge pass (blocked.hip): per-unit partial sums of
+// contrib over each destination block's in-edges.
+int blocked_pr_pass(rg_graph* g, const rg_opts* o, const double* contrib, DevBuf& unit_part,
+                    const int64_t* zero, int* vb_out, const uint32_t** block_unit0_out);
+
 // Builds the blocked PageRank layout for these options (once per graph).
 int pagerank_prepare(rg_graph* g, const rg_opts* o);
 // Builds the all-vertices heavy-chunk list of the push engine.

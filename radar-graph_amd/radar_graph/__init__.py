@@ -131,6 +131,7 @@ _sig("rg_graph_transpose", C.c_int, _vp)
 _sig("rg_graph_free", C.c_int, _vp)
 _sig("rg_opts_default", None, C.POINTER(Opts))
 _sig("rg_pagerank", C.c_int, _vp, C.c_int64, C.POINTER(Opts), _f64p, _i64p, _f64p, _f64p)
+_sig("rg_pagerank_delta", C.c_int, _vp, C.c_int64, C.POINTER(Opts), _f64p, _i64p, _f64p, _f64p)
 _sig("rg_bfs", C.c_int, _vp, C.c_uint32, C.POINTER(Opts), _u32p, _i64p, _i64p, _i64p, _f64p)
 _sig("rg_level_hash", C.c_uint64, C.c_uint32, C.c_uint32)
 _sig("rg_bfs_query", C.c_int, _vp, C.c_uint32, C.POINTER(Opts), C.POINTER(BfsSummary), _f64p)
@@ -435,6 +436,14 @@ class DeviceGraph:
         _check(lib.rg_pagerank(self._h, maxiters, C.byref(o),
                                _ptr(ranks, _f64p) if want_ranks else None,
                                C.byref(it), C.byref(l1), C.byref(ms)))
+        return PageRankResult(ranks, it.value, l1.value, ms.value)
+
+    def pagerank_delta(self, maxiters=20, o: Opts = None, want_ranks=True) -> PageRankResult:
+        o = o or opts()
+        ranks = np.empty(self.n, np.float64) if want_ranks else None
+        it, l1, ms = C.c_int64(), C.c_double(), C.c_double()
+        _check(lib.rg_pagerank_delta(self._h, maxiters, C.byref(o), _ptr(ranks, _f64p) if want_ranks else None,
+                                     C.byref(it), C.byref(l1), C.byref(ms)))
         return PageRankResult(ranks, it.value, l1.value, ms.value)
 
     def bfs(self, src, o: Opts = None, want_parents=True) -> BFSResult:

@@ -110,6 +110,13 @@ def ligra_fixtures(tmp, name, hg, sym):
                                    [8, 8], n, [np.float64, np.float64], tmp)
         arrays[f"{var}_pr_ranks"] = a0 if a0.sum() > a1.sum() else a1
         summ[f"{var}_pr"] = {"iters": int(parse(out, "Num Iters")), "l1": float(parse(out, "L1_Norm"))}
+        if var != "pp":
+            # PageRankDelta.C frees p, Delta, nghSum last (153): slot 2 from the end is p.
+            out, (_, _, prd), _ = capture(f"cap_{var}_PageRankDelta",
+                                          flag + ["-rounds", "0", "-maxiters", "20", path],
+                                          [8, 8, 8], n, [np.float64] * 3, tmp)
+            arrays[f"{var}_prd_ranks"] = prd
+            summ[f"{var}_prd"] = {"iters": int(parse(out, "Num Iters")), "l1": float(parse(out, "L1_Norm"))}
         out, (rad,), _ = capture(f"cap_{var}_Radii", flag + ["-rounds", "0", path], [4], n, [np.int32], tmp)
         arrays[f"{var}_radii"] = rad
         summ[f"{var}_radii"] = {"rounds": int(parse(out, "Iters until convergence"))}

@@ -179,3 +179,18 @@ def test_radii_batch_edge_cases(rg, orc, ctx, graphs, threshold):
         assert rounds == rr and np.array_equal(rad, ref)
     rad, rounds, _ = dg.radii(np.zeros(0, np.uint32), o)
     assert rounds == 0 and (rad == -1).all()
+
+
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12s", "rmat16s"])
+@pytest.mark.parametrize("variant", VARIANTS + ["blocked"])
+def test_pagerank_delta(rg, orc, ctx, graphs, gname, variant):
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    print(f"pagerank_delta {gname} {variant}", flush=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    for iters in (20, 3):
+        res = dg.pagerank_delta(iters, rg.opts(variant, hubs=256))
+        ref, it, l1 = orc.pagerank_delta(off, nbr, iters)
+        assert res.iters == it
+        assert rel_linf(res.ranks, ref) < FP_TOL
+        assert abs(res.l1 - l1) <= FP_TOL * max(abs(l1), 1e-300) or (res.l1 < 1e-7 and l1 < 1e-7)

@@ -128,3 +128,16 @@ def test_tc_local_app(orc, golden, relabel):
     assert "Verification:           PASS" in o or re.search(r"Verification:\s+PASS", o)
     total = int(re.search(r"^(\d+) triangles", o, re.M).group(1))
     assert total == golden["tc"]["rmat10s"]["pno"]
+
+
+@pytest.mark.parametrize("name", ["rmat10s", "rmat10d"])
+@pytest.mark.parametrize("pre,var", [("-1", "pno"), ("1", "ds")])
+@pytest.mark.parametrize("variant", ["push", "radar", "blocked"])
+def test_pagerank_delta_app(rg, golden, tmp_path, name, pre, var, variant):
+    _, path, flag = graph_file(rg, tmp_path, name)
+    fx = np.load(os.path.join(GOLD, f"ligra_{name}.npz"))
+    out = str(tmp_path / "prd.bin")
+    o = run("PageRankDelta", flag + ["-rounds", "0", "-maxiters", "20", "-preprocess", pre, "-variant", variant,
+                                     "-k", "128", "-out", out, path])
+    assert int(val(o, "Num Iters")) == golden["ligra"][name][f"{var}_prd"]["iters"]
+    assert same_fp(np.fromfile(out, np.float64), fx[f"{var}_prd_ranks"])

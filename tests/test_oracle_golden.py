@@ -139,6 +139,42 @@ def test_pagerank_bitexact(rg, orc, meta, name, sym):
 
 
 @pytest.mark.parametrize("name,sym", GRAPHS)
+def test_pagerank_delta_bitexact(rg, orc, meta, name, sym):
+    # PageRankDelta.C run by the reference (one thread): final p, printed
+    # Num Iters and L1_Norm; and on the degree-sorted graph (degree-sorting
+    # tree, -preprocess 1).
+    hg = _graph(rg, name)
+    off, nb = hg.csr()
+    fx = load(name)
+    r, it, l1 = orc.pagerank_delta(off, nb, 20)
+    assert np.array_equal(r, fx["pno_prd_ranks"])
+    s = meta["ligra"][name]["pno_prd"]
+    assert it == s["iters"] and abs(l1 - s["l1"]) <= 1e-5 * abs(s["l1"])
+    off2, nb2, _, _ = _relabelled(orc, hg, fx["ds_new_ids"])
+    r2, it2, _ = orc.pagerank_delta(off2, nb2, 20)
+    assert np.array_equal(r2, fx["ds_prd_ranks"])
+    assert it2 == meta["ligra"][name]["ds_prd"]["iters"]
+
+
+def test_pagerank_delta_closed_forms(orc):
+    # Directed cycle: round 1 gives every vertex nghSum = 1/n, so p = 1/n
+    # and Delta = 0.85/n + 0.15/n - 1/n (zero up to rounding): converged in
+    # one round.  A directed star 0 -> {1..7} does not converge in two
+    # rounds, so the count is maxiters + 1 (PageRankDelta.C:131), and after
+    # round 1 the leaves hold 0.85/(8*7) + 0.15/8, the centre 0.15/8.
+    n = 8
+    off = np.arange(n + 1, dtype=np.uint64)
+    nb = ((np.arange(n) + 1) % n).astype(np.uint32)
+    r, it, l1 = orc.pagerank_delta(off, nb, 20)
+    assert np.allclose(r, 1.0 / n, rtol=0, atol=1e-15) and it == 1 and l1 < 1e-7
+    off = np.array([0] + [7] * 8, np.uint64)
+    nb = np.arange(1, 8, dtype=np.uint32)
+    r, it, l1 = orc.pagerank_delta(off, nb, 1)
+    assert it == 2 and l1 > 1e-7
+    assert np.isclose(r[0], 0.15 / 8, rtol=1e-15) and np.allclose(r[1:], 0.85 / 56 + 0.15 / 8, rtol=1e-15)
+
+
+@pytest.mark.parametrize("name,sym", GRAPHS)
 def test_pagerank_pull_bitexact(rg, orc, meta, name, sym):
     hg = _graph(rg, name)
     off, nb = hg.csr()

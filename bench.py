@@ -191,6 +191,7 @@ def main():
                     ("hubdup", dg, rg.opts("hubdup", hubs=args.hubs), False),
                     ("degree_sorting", sg, rg.opts("push"), True),
                     ("radar", sg, rg.opts("radar", hubs=args.hubs), True),
+                    ("push_pull", dg, rg.opts("pull"), False),
                     ("radar_tuned", sg, bfs_tuned, True)]
     for name, g2, o2, mapped in bfs_runs:
         tot_e, tot_ms = 0, 0.0
@@ -200,6 +201,9 @@ def main():
             tot_e += r.edges
             tot_ms += r.ms
         bfs[name] = {"gteps": round(tot_e / tot_ms / 1e6, 2), "ms_per_source": round(tot_ms / len(srcs), 3)}
+        if name == "radar_tuned":
+            bfs[name].update({"engine": {0: "push", 1: "pull", 2: "hubdup", 3: "radar", 4: "blocked"}[o2.variant],
+                              "threshold": o2.threshold, "heavy_degree": o2.heavy_degree})
         log(f"bfs {name:15s} {bfs[name]['ms_per_source']:8.3f} ms/src {bfs[name]['gteps']:7.2f} GTEPS")
 
     # Batched BFS: the same 256 GAP-picked sources as one bit-parallel batch

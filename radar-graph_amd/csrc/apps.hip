@@ -8,6 +8,7 @@
 #include <cstring>
 #include <vector>
 
+#include "blocked.h"
 #include "device.h"
 #include "engine.h"
 #include "engine_host.h"
@@ -776,13 +777,22 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   int64_t iter = 0;
   unsigned long long* cs = reinterpret_cast<unsigned long long*>(ctx->counts.as<int64_t>() + 12);
   if (checksum_out) RG_HIP(hipMemsetAsync(cs, 0, sizeof(int64_t), ctx->stream));
-  EventTimer tm(ctx->stream);
   // RADAR and blocked run dense levels on the blocked layout (each unvisited
   // vertex scans its in-edges for a frontier parent; no global atomics).
   const bool blocked = o && (o->variant == RG_RADAR || o->variant == RG_BLOCKED);
+  // Pull (the push-pull tree): dense levels scan the in-edges of unvisited
+  // vertices with early exit -- direction-optimizing BFS.
+  const bool pull = o && o->variant == RG_PULL;
+  if (blocked) {
+    BlockedCSC* B = nullptr;
+    RG_TRY(query_layout(g, &B));   // built (once per graph) outside the timed loop
+  }
+  EventTimer tm(ctx->stream);
   while (F->size > 0) {
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
-    if (blocked && degsum > 0 && F->size + degsum > E.threshold)
+    if (pull && degsum > 0 && F->size + degsum > E.threshold)
+      RG_TRY(bfs_pull_level(g, F->ids, F->size, parents, o->heavy_degree, N->ids, info));
+    else if (blocked && degsum > 0 && F->size + degsum > E.threshold)
       RG_TRY(bfs_dense_level(g, F->ids, F->size, parents, N->ids, info));
     else
       RG_TRY(edge_map(E, op, *F, degsum, N->ids, info));
@@ -956,6 +966,10 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
   // RADAR and blocked run dense levels on the blocked layout (bcdense.hip);
   // push and HUBDUP keep the atomic push engine for every level.
   const bool blocked = o && (o->variant == RG_RADAR || o->variant == RG_BLOCKED);
+  if (blocked) {
+    BlockedCSC* B = nullptr;
+    RG_TRY(query_layout(g, &B));   // the forward orientation's layout, outside the timed loop
+  }
   EventTimer tm(ctx->stream);
   for (int it = 0; it < nsources; ++it) {
     Engine E;

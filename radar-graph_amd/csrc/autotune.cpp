@@ -9,7 +9,8 @@
 // (512 / 1024 threads) and 1 or 4 edges in flight per lane, and the atomic
 // RADAR engine at the LDS hub budget.  BFS candidates: the sparse/dense
 // threshold (m/5, m/20, m/80), the wave-per-vertex cut-over degree and the
-// hub count (none / LDS budget).  Every candidate computes the same result;
+// dense-level engine (push; RADAR: LDS hub copies + blocked dense levels;
+// pull: bottom-up dense levels).  Every candidate computes the same levels;
 // only time differs.
 #include <cstring>
 #include <vector>
@@ -51,8 +52,8 @@ extern "C" int rg_autotune(rg_graph* g, int app, const uint32_t* sources, int ns
   } else if (app == RG_APP_BFS) {
     if (!sources || nsources <= 0) return rg::set_error(RG_EINVAL, "rg_autotune: BFS needs sources");
     for (int64_t div : {5, 20, 80})
-      for (int heavy : {256, 1024, 4096})
-        for (int hv : {RG_PUSH, RG_RADAR}) {
+      for (int heavy : {256, 1024})
+        for (int hv : {RG_PUSH, RG_RADAR, RG_PULL}) {
           rg_opts o = b;
           o.variant = hv;
           o.threshold = m / div;
@@ -74,6 +75,9 @@ extern "C" int rg_autotune(rg_graph* g, int app, const uint32_t* sources, int ns
       RG_TRY_HOST(rg_pagerank(g, 5, &o, nullptr, &it, &l1, &ms));
       total = ms;
     } else {
+      int64_t it = 0, reached = 0, edges = 0;
+      double ms = 0.0;
+      RG_TRY_HOST(rg_bfs(g, sources[0], &o, nullptr, &it, &reached, &edges, &ms));   // untimed warm-up
       for (int i = 0; i < nsources; ++i) {
         int64_t it = 0, reached = 0, edges = 0;
         double ms = 0.0;

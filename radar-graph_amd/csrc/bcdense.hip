@@ -219,6 +219,88 @@ __global__ __launch_bounds__(THREADS) void k_bfs_merge(BcDenseArgs a, uint32_t* 
   wq_finish(q, ea);
 }
 
+// Bottom-up BFS level: the push-pull tree's edgeMapDense (push-pull
+// Ligra/ligra/ligra.h:62-88) with decodeInNghBreakEarly (vertex.h:13-27):
+// every unvisited vertex scans its in-neighbours in order and takes the
+// first one in the frontier as parent, ending the scan (BFS_F's cond turns
+// false).  Light vertices: one thread each; in-degree above heavy_deg: one
+// wavefront each (second kernel), 64 in-neighbours per step, the lowest
+// lane that finds one wins -- the same parent the ordered scan picks.
+__global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, const uint32_t* in_nbr,
+                                                        const uint64_t* off, int64_t n,
+                                                        const unsigned long long* active, uint32_t* parents,
+                                                        uint32_t* heavy, int64_t* heavy_count, int heavy_deg,
+                                                        uint32_t* out_ids, int64_t* out_info) {
+  __shared__ uint32_t s_q[4][kQueue];
+  EdgeArgs ea{};
+  ea.off = off;
+  ea.out_ids = out_ids;
+  ea.out_info = out_info;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    bool join = false;
+    if (v < n && parents[v] == kNone) {
+      const uint64_t b = in_off[v], e = in_off[v + 1];
+      if (e - b > (uint64_t)heavy_deg) {
+        heavy[atomicAdd((unsigned long long*)heavy_count, 1ull)] = (uint32_t)v;
+      } else {
+        for (uint64_t j = b; j < e; ++j) {
+          const uint32_t s = in_nbr[j];
+          if ((active[s >> 6] >> (s & 63)) & 1) {
+            parents[v] = s;
+            join = true;
+            break;
+          }
+        }
+      }
+    }
+    wq_push(q, ea, join, (uint32_t)v);
+  }
+  wq_finish(q, ea);
+}
+
+__global__ __launch_bounds__(256) void k_bfs_pull_heavy(const uint64_t* in_off, const uint32_t* in_nbr,
+                                                        const uint64_t* off, const unsigned long long* active,
+                                                        uint32_t* parents, const uint32_t* heavy,
+                                                        const int64_t* heavy_count, uint32_t* out_ids,
+                                                        int64_t* out_info) {
+  __shared__ uint32_t s_q[4][kQueue];
+  EdgeArgs ea{};
+  ea.off = off;
+  ea.out_ids = out_ids;
+  ea.out_info = out_info;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  const int lane = (int)__lane_id();
+  const int64_t items = *heavy_count;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  for (int64_t it = wave; it < items; it += nwaves) {
+    const uint32_t v = heavy[it];
+    const uint64_t b = in_off[v], e = in_off[v + 1];
+    bool join = false;
+    for (uint64_t j0 = b; j0 < e; j0 += kWave) {
+      const uint64_t j = j0 + lane;
+      uint32_t s = 0;
+      bool f = false;
+      if (j < e) {
+        s = in_nbr[j];
+        f = (active[s >> 6] >> (s & 63)) & 1;
+      }
+      const unsigned long long m = __ballot(f);
+      if (m) {
+        const int first = __ffsll((long long)m) - 1;
+        const uint32_t p = __shfl(s, first);
+        if (lane == 0) parents[v] = p;
+        join = true;
+        break;
+      }
+    }
+    wq_push(q, ea, join && lane == 0, v);
+  }
+  wq_finish(q, ea);
+}
+
 __global__ void k_set_bits(const uint32_t* ids, int64_t count, unsigned long long* bits) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t v = ids[i];
@@ -240,14 +322,11 @@ __global__ void k_clear_bits(const uint32_t* ids, int64_t count, unsigned long l
 // context's bc_dense buffer; the bitmap is all zero between calls.
 namespace {
 
-int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const double* val, double* out,
-                const uint8_t* visited, uint32_t* parents, uint32_t* out_ids, int64_t* out_info) {
-  rg_ctx* ctx = g->ctx;
-  BlockedCSC* B = nullptr;
-  RG_TRY(query_layout(g, &B));
-  const int64_t nwords = g->n / 64 + 1;
+// The context's frontier bitmap for an n-vertex graph followed by
+// part_bytes of scratch; the bitmap is all zero on return.
+int frontier_bitmap(rg_ctx* ctx, int64_t n, size_t part_bytes, unsigned long long** bits, void** part) {
+  const int64_t nwords = n / 64 + 1;
   const size_t bits_bytes = ((sizeof(unsigned long long) * nwords + 255) / 256) * 256;
-  const size_t part_bytes = sizeof(double) * ((size_t)std::max<int64_t>(B->nunits, 1) << kQueryVB);
   DevBuf& sb = ctx->bc_dense;
   if (sb.bytes < bits_bytes + part_bytes) {
     sb.release();
@@ -258,7 +337,20 @@ int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const
   // bitmap: clear the bitmap of this graph unless it is known to be zero.
   if (ctx->bc_bits_zero < nwords) RG_TRY(fill<unsigned long long>(ctx, sb.as<unsigned long long>(), nwords, 0ull));
   ctx->bc_bits_zero = nwords;
-  unsigned long long* bits = sb.as<unsigned long long>();
+  *bits = sb.as<unsigned long long>();
+  *part = sb.as<char>() + bits_bytes;
+  return RG_OK;
+}
+
+int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const double* val, double* out,
+                const uint8_t* visited, uint32_t* parents, uint32_t* out_ids, int64_t* out_info) {
+  rg_ctx* ctx = g->ctx;
+  BlockedCSC* B = nullptr;
+  RG_TRY(query_layout(g, &B));
+  const size_t part_bytes = sizeof(double) * ((size_t)std::max<int64_t>(B->nunits, 1) << kQueryVB);
+  unsigned long long* bits = nullptr;
+  void* part = nullptr;
+  RG_TRY(frontier_bitmap(ctx, g->n, part_bytes, &bits, &part));
   BcDenseArgs a{};
   a.n = g->n;
   a.off = g->off;
@@ -272,7 +364,7 @@ int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const
   a.visited = visited;
   a.val = val;
   a.out = out;
-  a.part = reinterpret_cast<double*>(sb.as<char>() + bits_bytes);
+  a.part = static_cast<double*>(part);
   a.out_ids = out_ids;
   a.out_info = out_info;
   const int grid = grid_for(ctx, count);
@@ -313,6 +405,35 @@ int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, co
 int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, uint32_t* out_ids,
                     int64_t* out_info) {
   return dense_level(g, 2, ids, count, nullptr, nullptr, nullptr, parents, out_ids, out_info);
+}
+
+int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, int heavy_deg,
+                   uint32_t* out_ids, int64_t* out_info) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  if (heavy_deg <= 0) heavy_deg = 256;
+  // Heavy list (vertices above heavy_deg; at most m / heavy_deg) and its
+  // counter after the frontier bitmap.
+  const size_t cap = (size_t)(g->m / heavy_deg + 1);
+  unsigned long long* bits = nullptr;
+  void* part = nullptr;
+  RG_TRY(frontier_bitmap(ctx, n, sizeof(int64_t) * 2 + sizeof(uint32_t) * cap, &bits, &part));
+  int64_t* hcount = static_cast<int64_t*>(part);
+  uint32_t* heavy = reinterpret_cast<uint32_t*>(hcount + 2);
+  RG_TRY(fill<int64_t>(ctx, hcount, 1, 0));
+  const int grid = grid_for(ctx, count);
+  hipLaunchKernelGGL(k_set_bits, grid, 256, 0, ctx->stream, ids, count, bits);
+  hipEvent_t* ev = timing_begin(ctx);
+  const int vgrid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 8);
+  hipLaunchKernelGGL(k_bfs_pull_light, std::max(vgrid, 1), 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off, n,
+                     bits, parents, heavy, hcount, heavy_deg, out_ids, out_info);
+  hipLaunchKernelGGL(k_bfs_pull_heavy, ctx->num_cus * 4, 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off, bits,
+                     parents, heavy, hcount, out_ids, out_info);
+  if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
+  hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);
+  RG_HIP(hipGetLastError());
+  // The heavy list overwrote scratch past the bitmap only.
+  return RG_OK;
 }
 
 }  // namespace rg

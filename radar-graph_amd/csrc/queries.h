@@ -57,6 +57,11 @@ int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, co
 int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, uint32_t* out_ids,
                     int64_t* out_info);
 
+// One bottom-up BFS level (the push-pull tree's pull edgeMap with early
+// exit) over g's in-edges (bcdense.hip).
+int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, int heavy_deg,
+                   uint32_t* out_ids, int64_t* out_info);
+
 int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
             int64_t* combined_out, double* ms_out);
 

@@ -48,7 +48,7 @@ def test_pagerank(rg, orc, ctx, graphs, gname, variant):
 
 
 @pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d"])
-@pytest.mark.parametrize("variant", VARIANTS + ["blocked"])
+@pytest.mark.parametrize("variant", VARIANTS + ["blocked", "pull"])
 def test_bfs(rg, orc, ctx, graphs, gname, variant):
     hg = graphs(gname)
     off, nbr = hg.csr()
@@ -138,12 +138,30 @@ def check_bfs(rg, orc, dg, off, nbr, srcs, o):
 
 @pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d", "rmat16s"])
 @pytest.mark.parametrize("threshold", [0, -1])
-def test_bfs_blocked_dense_levels(rg, orc, ctx, graphs, gname, threshold):
+@pytest.mark.parametrize("variant", ["blocked", "pull"])
+def test_bfs_blocked_dense_levels(rg, orc, ctx, graphs, gname, threshold, variant):
     hg = graphs(gname)
     off, nbr = hg.csr()
     dg = rg.DeviceGraph(ctx, hg)
-    print(f"bfs blocked {gname} {threshold}", flush=True)
-    check_bfs(rg, orc, dg, off, nbr, hg.pick_sources(4), rg.opts("blocked", threshold=threshold))
+    print(f"bfs {variant} {gname} {threshold}", flush=True)
+    check_bfs(rg, orc, dg, off, nbr, hg.pick_sources(4), rg.opts(variant, threshold=threshold, heavy_degree=64))
+
+
+def test_bfs_pull_takes_first_frontier_in_neighbour(rg, orc, ctx, graphs):
+    # With every level dense, the bottom-up scan (decodeInNghBreakEarly)
+    # picks the first in-neighbour in the frontier, in in-list order, for
+    # light and heavy vertices alike.
+    hg = graphs("rmat12d")
+    off, nbr = hg.csr()
+    ioff, inb = hg.csr(in_edges=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    for s in hg.pick_sources(3):
+        r = dg.bfs(int(s), rg.opts("pull", threshold=0, heavy_degree=32))
+        lv, _ = orc.bfs_levels(off, nbr, int(s))
+        for v in np.nonzero(lv > 0)[0]:
+            ins = inb[ioff[v]:ioff[v + 1]]
+            first = ins[lv[ins] == lv[v] - 1][0]
+            assert r.parents[v] == first, f"vertex {v}"
 
 
 def test_bfs_blocked_split_hub_block(rg, orc, ctx):
@@ -154,6 +172,7 @@ def test_bfs_blocked_split_hub_block(rg, orc, ctx):
     off, nbr = hg.csr()
     dg = rg.DeviceGraph(ctx, hg)
     check_bfs(rg, orc, dg, off, nbr, [5, 0, 1500], rg.opts("blocked", threshold=0))
+    check_bfs(rg, orc, dg, off, nbr, [5, 0, 1500], rg.opts("pull", threshold=0))
 
 
 def test_bfs_radar_on_degree_sorted(rg, orc, ctx, graphs):

@@ -92,7 +92,10 @@ static int launch_edge(const Engine& E, const Op& op, const EdgeArgs& a, size_t 
   RG_HIP(hipGetLastError());
   if (ev) RG_HIP(hipEventRecord(ev[1], E.ctx->stream));
   if (!a.static_heavy) {
-    hipLaunchKernelGGL(kh, (int)cap, BLOCK, hub_bytes, E.ctx->stream, op, a);
+    // The heavy queue is usually short or empty: a smaller persistent grid
+    // (its waves stride over the items) keeps an empty launch cheap.
+    const int grid_h = (int)std::max<int64_t>(1, std::min<int64_t>(cap, (int64_t)E.ctx->num_cus * 4));
+    hipLaunchKernelGGL(kh, grid_h, BLOCK, hub_bytes, E.ctx->stream, op, a);
     RG_HIP(hipGetLastError());
   }
   if (Op::kL2 && HUB != HUB_NONE && a.hub_k2 > a.hub_k) {

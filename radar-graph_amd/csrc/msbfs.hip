@@ -55,6 +55,7 @@ struct MsCtl {
   int64_t threshold;
   int64_t heavy_count;    // engine heavy queue
   int64_t dense_levels, sparse_levels;
+  int64_t unit_next;      // dense level: next unit to take (dynamic work distribution)
 };
 
 typedef unsigned long long ull;
@@ -278,7 +279,7 @@ __device__ __forceinline__ T ld_nt(const T* p) {
 }
 
 template <int W, int VB, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a) {
+__global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) {
   constexpr int BV = 1 << VB;
   constexpr int NW = THREADS / kWave;
   __shared__ ull acc[BV * W];
@@ -286,59 +287,68 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a) {
   __shared__ uint32_t s_q[NW][kQueue];
   __shared__ MsStats<W> st;
   if (a.ctl->mode != MS_DENSE) return;
-  const int64_t u = blockIdx.x;
-  const uint32_t b = a.unit_block[u];
-  const int64_t base = (int64_t)b << VB;
-  const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
   const uint32_t level = (uint32_t)a.ctl->level + 1;
-  for (int i = threadIdx.x; i < BV * W; i += THREADS) acc[i] = 0;
-  stats_init<W, THREADS>(st);
-  // Destinations every source of the batch has reached need no gathers.
   ull allbits[W];
   batch_bits<W>(a.nsrc, allbits);
-  for (int l = threadIdx.x; l < BV; l += THREADS) {
-    const int64_t d = base + l;
-    bool f = true;
-    if (d < a.n) {
-#pragma unroll
-      for (int w = 0; w < W; ++w) f = f && (a.seen[(size_t)d * W + w] & allbits[w]) == allbits[w];
-    }
-    const ull word = __ballot(f);
-    if (__lane_id() == 0) full[l >> 6] = word;
-  }
-  __syncthreads();
-  // Every destination reached by every source: nothing arrives (the merge
-  // of a split block masks its partials with ~seen, which is then empty).
-  bool all = true;
-  for (int i = 0; i < BV / 64; ++i) all = all && full[i] == ~0ull;
-  if (all) return;
-  const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
-  for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
-    const uint32_t s = ld_nt(a.src + e);
-    const uint32_t l = ld_nt(a.vl + e);
-    if ((full[l >> 6] >> (l & 63)) & 1) continue;
-    if (!((a.active_cur[s >> 6] >> (s & 63)) & 1)) continue;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      const ull f = a.front_cur[(size_t)s * W + w];
-      if (f) atomicOr(&acc[l * W + w], f);
-    }
-  }
-  __syncthreads();
-  if (!single) {
-    ull* out = a.part + ((size_t)u << VB) * W;
-    for (int i = threadIdx.x; i < BV * W; i += THREADS) out[i] = acc[i];
-    return;
-  }
   EdgeArgs ea{};
   ea.off = a.off;
   ea.out_ids = a.list_nxt;
   ea.out_info = &a.ctl->ncount;
   WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
-  for (int l0 = 0; l0 < BV; l0 += THREADS) {
-    const int l = l0 + threadIdx.x;
-    const int64_t d = base + l;
-    ms_finalize<W>(a, st, q, ea, d, d < a.n, &acc[l * W], level);
+  __shared__ int64_t s_unit;
+  stats_init<W, THREADS>(st);
+  // Persistent grid: each workgroup takes the next unit from a counter
+  // until none is left (the conditions below are uniform across it).
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_unit = (int64_t)atomicAdd((unsigned long long*)&a.ctl->unit_next, 1ull);
+    __syncthreads();
+    const int64_t u = s_unit;
+    if (u >= nunits) break;
+    const uint32_t b = a.unit_block[u];
+    const int64_t base = (int64_t)b << VB;
+    const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
+    for (int i = threadIdx.x; i < BV * W; i += THREADS) acc[i] = 0;
+    // Destinations every source of the batch has reached need no gathers.
+    for (int l = threadIdx.x; l < BV; l += THREADS) {
+      const int64_t d = base + l;
+      bool f = true;
+      if (d < a.n) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) f = f && (a.seen[(size_t)d * W + w] & allbits[w]) == allbits[w];
+      }
+      const ull word = __ballot(f);
+      if (__lane_id() == 0) full[l >> 6] = word;
+    }
+    __syncthreads();
+    // Every destination reached by every source: nothing arrives (the merge
+    // of a split block masks its partials with ~seen, which is then empty).
+    bool all = true;
+    for (int i = 0; i < BV / 64; ++i) all = all && full[i] == ~0ull;
+    if (all) continue;
+    const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
+    for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
+      const uint32_t s = ld_nt(a.src + e);
+      const uint32_t l = ld_nt(a.vl + e);
+      if ((full[l >> 6] >> (l & 63)) & 1) continue;
+      if (!((a.active_cur[s >> 6] >> (s & 63)) & 1)) continue;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const ull f = a.front_cur[(size_t)s * W + w];
+        if (f) atomicOr(&acc[l * W + w], f);
+      }
+    }
+    __syncthreads();
+    if (!single) {
+      ull* out = a.part + ((size_t)u << VB) * W;
+      for (int i = threadIdx.x; i < BV * W; i += THREADS) out[i] = acc[i];
+      continue;
+    }
+    for (int l0 = 0; l0 < BV; l0 += THREADS) {
+      const int l = l0 + threadIdx.x;
+      const int64_t d = base + l;
+      ms_finalize<W>(a, st, q, ea, d, d < a.n, &acc[l * W], level);
+    }
   }
   wq_finish(q, ea);
   stats_flush<W, THREADS>(a, st);
@@ -424,6 +434,7 @@ __global__ void k_ms_level_end(MsArgs a, ull* reached, int64_t* depth) {
     c->ncount = 0;
     c->ndeg = 0;
     c->heavy_count = 0;
+    c->unit_next = 0;
     c->mode = c->fcount == 0 ? MS_DONE : (c->fcount + c->fdeg > c->threshold ? MS_DENSE : MS_SPARSE);
   }
 }
@@ -464,6 +475,7 @@ __global__ void k_ms_init(MsArgs a, int W, const uint32_t* sources, int64_t nuni
     c->ncount = c->ndeg = 0;
     c->threshold = threshold;
     c->heavy_count = 0;
+    c->unit_next = 0;
     c->dense_levels = c->sparse_levels = 0;
     c->mode = c->fcount + c->fdeg > threshold ? MS_DENSE : MS_SPARSE;
   }
@@ -587,13 +599,18 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   EdgeArgs ea{};
   ea.off = g->off;
   ea.nbr = g->nbr;
-  ea.count = n;                     // bound; the list size is read on the device
+  // The list size is read on the device; the bound (n) only sizes the grid,
+  // the full persistent one: sparse levels are bound by atomics in flight.
+  ea.count = n;
   ea.count_dev = &a.ctl->fcount;
   ea.out_info = &a.ctl->ncount;
   ea.heavy = E.heavy->as<uint64_t>();
   ea.heavy_count = &a.ctl->heavy_count;
   ea.heavy_deg = E.heavy_deg;
   const int gfin = std::max(1, ctx->num_cus * 4);
+  // Dense level: a persistent grid taking units from a counter (the LDS
+  // copy limits how many run per CU: 4 for W = 1, 2 for W = 2, 1 for W = 4).
+  const int gdense = (int)std::min<int64_t>(B->nunits, (int64_t)ctx->num_cus * 4);
   const int gclr = grid_for(ctx, std::max<int64_t>(nwords, 1));
   constexpr int PER = (1 << kMsVB) / kMsThreads;
   int64_t steps = 0;
@@ -605,7 +622,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
       ea.out_ids = a.list_nxt;
       RG_TRY((launch_hub<MsPushOp<W>, SRC_LIST>(E, op, ea)));
       hipLaunchKernelGGL(k_ms_sparse_fin<W>, gfin, 256, 0, st, a);
-      hipLaunchKernelGGL((k_ms_dense<W, kMsVB, kMsThreads>), (int)B->nunits, kMsThreads, 0, st, a);
+      hipLaunchKernelGGL((k_ms_dense<W, kMsVB, kMsThreads>), gdense, kMsThreads, 0, st, a, B->nunits);
       if (B->nmulti)
         hipLaunchKernelGGL((k_ms_merge<W, kMsVB, kMsThreads>), (int)(B->nmulti * PER), kMsThreads, 0, st, a);
       hipLaunchKernelGGL(k_ms_clear<W>, gclr, 256, 0, st, a, nwords);

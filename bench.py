@@ -1,17 +1,23 @@
-"""Benchmark: PageRank / BFS GTEPS of push-only, HUBDUP, DegreeSorting and
-RADAR on one MI355X, on the directed R-MAT graph (GAP generator, scale 24,
-edge factor 16, a/b/c = 0.57/0.19/0.19, seed 27491095, self-loops and
-duplicates removed; int32 ids, uint64 offsets, FP64 ranks).
+"""Benchmark: GTEPS of push-only vs RADAR PageRank and BFS, batched BFS
+queries/s and the serving loop's p50/p99, on one MI355X.
 
-One "step" is one PageRank run of 20 iterations with RADAR: the
-degree-sorted graph with hub-duplicated accumulators, on the engine the
-autotuner picks for this graph (the blocked LDS engine or the atomic one);
-its GTEPS is 20 * m / time.  Every variant (atomic push-only, HUBDUP,
-DegreeSorting, atomic RADAR, pull, blocked on the unsorted graph), BFS from
-64 GAP-picked sources, batched BFS (256 sources as one bit-parallel batch
-vs one query at a time), the serving loop (symmetric R-MAT, default scale
-22: saturated queries/s and Poisson-arrival p50/p99, batched vs one at a
-time) and the CPU oracle baseline are reported alongside.
+Graphs come from the GAP R-MAT generator (a/b/c = 0.57/0.19/0.19, seed
+27491095, edge factor 16, self-loops and duplicates removed; int32 ids,
+uint64 offsets, FP64 ranks).
+
+* Headline (symmetric R-MAT scale 26, n = 67M, m = 2.1B arcs): one "step"
+  is one PageRank run of 20 iterations with RADAR -- the degree-sorted
+  graph on the engine the autotuner picks (the blocked engine, whose hub
+  blocks are split over private LDS copies); value = 20 m / step time.
+  Beside it on the same graph: push-only PageRank, BFS from 16 GAP-picked
+  sources (push-only vs autotuned RADAR), batched BFS (256 sources as one
+  bit-parallel batch vs one query at a time) and the CPU oracle baseline.
+* Variants table (directed R-MAT scale 24): PageRank and BFS for push-only,
+  HUBDUP, DegreeSorting, RADAR, pull and the blocked engine.
+* Serving (symmetric R-MAT scale 22, degree-sorted): BFS-only and 80/15/5
+  BFS/BC/PageRank-10 traces, saturated and as Poisson arrivals at 80% of
+  the one-at-a-time capacity, batched (continuous batching) vs one at a
+  time.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S]
 Prints one JSON line.
@@ -26,13 +32,66 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "radar-graph_amd"), os.path.join(ROOT, "oracle")]
 
 PEAK_HBM_GBS = 8000.0   # MI355X HBM3E peak, GB/s
+ENGINES = {0: "push", 1: "pull", 2: "hubdup", 3: "radar", 4: "blocked"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def serve_bench(rg, args, log):
+def median_pr(g, it, o, reps=3):
+    g.pagerank(it, o, want_ranks=False)
+    return sorted(g.pagerank(it, o, want_ranks=False).ms for _ in range(reps))[reps // 2]
+
+
+def bfs_gteps(g, srcs, o):
+    g.bfs(int(srcs[0]), o, want_parents=False)
+    te, tm = 0, 0.0
+    for s in srcs:
+        r = g.bfs(int(s), o, want_parents=False)
+        te += r.edges
+        tm += r.ms
+    return {"gteps": round(te / tm / 1e6, 2), "ms_per_source": round(tm / len(srcs), 3)}
+
+
+def variants_table(rg, args):
+    """Every variant on directed R-MAT (the study's second configuration)."""
+    t0 = time.time()
+    hg = rg.HostGraph.generate(args.variants_scale, 16, symmetric=False)
+    n, m, _ = hg.info()
+    ctx = rg.Context(0)
+    dg = rg.DeviceGraph(ctx, hg)
+    sg, ids = dg.degree_sort()
+    log(f"variants graph: rmat{args.variants_scale} directed n={n} m={m} ({time.time() - t0:.1f}s)")
+    it = args.iters
+    out = {"graph": f"rmat{args.variants_scale} directed", "n": n, "m": m, "pagerank": {}, "bfs": {}}
+    pr_runs = [("push", dg, rg.opts("push")), ("hubdup", dg, rg.opts("hubdup", hubs=args.hubs)),
+               ("degree_sorting", sg, rg.opts("push")), ("radar_atomic", sg, rg.opts("radar", hubs=args.hubs)),
+               ("pull", dg, rg.opts("pull")), ("blocked_unsorted", dg, rg.opts("blocked"))]
+    tuned, _ = sg.autotune("pagerank", base=rg.opts("radar", hubs=args.hubs))
+    pr_runs.append(("radar_tuned", sg, tuned))
+    for name, g, o in pr_runs:
+        ms = median_pr(g, it, o)
+        out["pagerank"][name] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2)}
+        log(f"  pagerank {name:16s} {ms:8.2f} ms {it * m / ms / 1e6:8.2f} GTEPS")
+    srcs = hg.pick_sources(args.variants_bfs_sources)
+    bt, _ = sg.autotune("bfs", sources=ids[srcs[:4]], base=rg.opts("push"))
+    bfs_runs = [("push", dg, srcs, rg.opts("push")), ("hubdup", dg, srcs, rg.opts("hubdup", hubs=args.hubs)),
+                ("degree_sorting", sg, ids[srcs], rg.opts("push")),
+                ("radar", sg, ids[srcs], rg.opts("radar", hubs=args.hubs)),
+                ("push_pull", dg, srcs, rg.opts("pull")), ("radar_tuned", sg, ids[srcs], bt)]
+    for name, g, ss, o in bfs_runs:
+        out["bfs"][name] = bfs_gteps(g, ss, o)
+        log(f"  bfs {name:16s} {out['bfs'][name]['ms_per_source']:8.3f} ms/src "
+            f"{out['bfs'][name]['gteps']:8.2f} GTEPS")
+    out["bfs"]["radar_tuned"]["engine"] = ENGINES[bt.variant]
+    for g in (sg, dg):
+        g.close()
+    ctx.close()
+    return out
+
+
+def serve_bench(rg, args):
     """Serving over a resident degree-sorted symmetric R-MAT graph: seeded
     query traces (BFS only; 80/15/5 BFS/BC/PageRank-10) run saturated (all
     queries at t = 0) and as Poisson arrivals at 80% of the one-at-a-time
@@ -60,7 +119,7 @@ def serve_bench(rg, args, log):
     for name, mix, count in [("bfs", (1, 0, 0), args.serve_queries), ("mix_80_15_5", (0.8, 0.15, 0.05), 256)]:
         q = trace(count, 0.0, mix, 27491095)
         _, one = g.serve(q, rg.serve_config(one_at_a_time=True))
-        res, bat = g.serve(q, rg.serve_config(slots=2))
+        _, bat = g.serve(q, rg.serve_config(slots=2))
         rate = 0.8 * one.qps
         q2 = trace(count, rate, mix, 1)
         _, one_p = g.serve(q2, rg.serve_config(one_at_a_time=True))
@@ -80,37 +139,37 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--scale", type=int, default=26, help="R-MAT scale of the headline graph (symmetric)")
     ap.add_argument("--iters", type=int, default=20)
-    ap.add_argument("--bfs-sources", type=int, default=64)
-    ap.add_argument("--hubs", type=int, default=0, help="LDS hub slots (0 = LDS budget)")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--quick", action="store_true", help="headline only (no variant table, no BFS)")
+    ap.add_argument("--bfs-sources", type=int, default=16)
     ap.add_argument("--batch", type=int, default=256, help="sources per batched BFS")
+    ap.add_argument("--hubs", type=int, default=0, help="LDS hub slots (0 = LDS budget)")
+    ap.add_argument("--variants-scale", type=int, default=24, help="directed R-MAT of the variants table (0: skip)")
+    ap.add_argument("--variants-bfs-sources", type=int, default=64)
     ap.add_argument("--serve-scale", type=int, default=22, help="R-MAT scale of the serving graph (symmetric)")
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the library's plain 1 GiB stream copy once (k_stream_copy: the "
                          "known-size kernel that scales FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md and not measured", file=sys.stderr)
+              "the multi-GPU sharding is a design, not measured here", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import radar_graph as rg
 
-
     t0 = time.time()
-    hg = rg.HostGraph.generate(args.scale, 16, uniform=False, symmetric=False)
+    hg = rg.HostGraph.generate(args.scale, 16, uniform=False, symmetric=True)
     n, m, _ = hg.info()
     t_gen = time.time() - t0
-    log(f"graph: rmat{args.scale} directed n={n} m={m} (host build {t_gen:.1f}s)")
+    log(f"graph: rmat{args.scale} symmetric n={n} m={m} (host build {t_gen:.1f}s)")
     ctx = rg.Context(0)
     if args.calibrate_copy:
-        gb, cms = ctx.copy_bandwidth(1 << 30, reps=1)
+        gb, _ = ctx.copy_bandwidth(1 << 30, reps=1)
         log(f"calibration copy (k_stream_copy): 1 GiB read + 1 GiB write, {gb:.0f} GB/s")
     stream_gbs, _ = ctx.copy_bandwidth(1 << 30, reps=5)
     log(f"measured stream copy: {stream_gbs:.0f} GB/s (read + write)")
@@ -120,46 +179,22 @@ def main():
     ctx.sync()
     t_sort = time.time() - t0
     log(f"degree sort (GPU): {t_sort * 1e3:.1f} ms")
-
     it = args.iters
-    variants = {
-        "push": (dg, rg.opts("push")),
-        "hubdup": (dg, rg.opts("hubdup", hubs=args.hubs)),
-        "degree_sorting": (sg, rg.opts("push")),
-        "radar": (sg, rg.opts("radar", hubs=args.hubs)),
-        "pull": (dg, rg.opts("pull")),
-        "blocked_unsorted": (dg, rg.opts("blocked")),
-    }
-    pr = {}
-    if args.quick:
-        variants = {}
-    for name, (g, o) in variants.items():
-        g.pagerank(it, o, want_ranks=False)
-        ms = sorted(g.pagerank(it, o, want_ranks=False).ms for _ in range(3))[1]
-        pr[name] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2)}
-        log(f"pagerank {name:15s} {ms:8.2f} ms  {pr[name]['gteps']:7.2f} GTEPS")
 
-    # RADAR with the autotuned engine for this graph.
+    # RADAR PageRank on the engine the autotuner picks for this graph.
     t0 = time.time()
     tuned, _ = sg.autotune("pagerank", base=rg.opts("radar", hubs=args.hubs))
     t_tune = time.time() - t0
-    engine = {0: "push", 1: "pull", 2: "hubdup", 3: "radar-atomic", 4: "blocked"}[tuned.variant]
+    engine = ENGINES[tuned.variant]
     log(f"autotune pagerank: engine={engine} wg={tuned.wg_size} unroll={tuned.unroll} ({t_tune:.1f}s)")
     g, o = sg, tuned
-    g.pagerank(it, o, want_ranks=False)
-    ms = sorted(g.pagerank(it, o, want_ranks=False).ms for _ in range(3))[1]
-    pr["radar_tuned"] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2), "engine": engine,
-                         "wg_size": tuned.wg_size, "unroll": tuned.unroll}
-    log(f"pagerank radar_tuned      {ms:8.2f} ms  {pr['radar_tuned']['gteps']:7.2f} GTEPS")
-
-    # Headline: RADAR PageRank, K timed steps after W warmups.
     for _ in range(args.warmup):
         g.pagerank(it, o, want_ranks=False)
     ctx.sync()
     ctx.timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = g.pagerank(it, o, want_ranks=False)
+        g.pagerank(it, o, want_ranks=False)
     ctx.sync()
     wall = time.perf_counter() - t0
     launches, kms = ctx.kernel_time()
@@ -174,46 +209,34 @@ def main():
     avg_launch_ms = kms / max(launches, 1)
     achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_pagerank_radar_rmat24.json")
-    if os.path.exists(tpath) and args.scale == 24:
+    tpath = os.path.join(ROOT, "profiles", f"traffic_pagerank_radar_rmat{args.scale}.json")
+    if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-
-    # BFS from 64 sources per variant.
-    srcs = hg.pick_sources(args.bfs_sources)
-    bfs = {}
-    bfs_runs = []
+    pr = {"radar_tuned": {"ms": round(ms_step, 3), "gteps": round(value, 2), "engine": engine,
+                          "wg_size": tuned.wg_size, "unroll": tuned.unroll}}
+    bfs, batched = {}, None
     if not args.quick:
-        bfs_tuned, _ = sg.autotune("bfs", sources=new_ids[srcs[:4]], base=rg.opts("push"))
-        log(f"autotune bfs: variant={bfs_tuned.variant} threshold={bfs_tuned.threshold} "
-            f"heavy={bfs_tuned.heavy_degree} hubs={bfs_tuned.hubs}")
-        bfs_runs = [("push", dg, rg.opts("push"), False),
-                    ("hubdup", dg, rg.opts("hubdup", hubs=args.hubs), False),
-                    ("degree_sorting", sg, rg.opts("push"), True),
-                    ("radar", sg, rg.opts("radar", hubs=args.hubs), True),
-                    ("push_pull", dg, rg.opts("pull"), False),
-                    ("radar_tuned", sg, bfs_tuned, True)]
-    for name, g2, o2, mapped in bfs_runs:
-        tot_e, tot_ms = 0, 0.0
-        for s in srcs:
-            s2 = int(new_ids[s]) if mapped else int(s)
-            r = g2.bfs(s2, o2, want_parents=False)
-            tot_e += r.edges
-            tot_ms += r.ms
-        bfs[name] = {"gteps": round(tot_e / tot_ms / 1e6, 2), "ms_per_source": round(tot_ms / len(srcs), 3)}
-        if name == "radar_tuned":
-            bfs[name].update({"engine": {0: "push", 1: "pull", 2: "hubdup", 3: "radar", 4: "blocked"}[o2.variant],
-                              "threshold": o2.threshold, "heavy_degree": o2.heavy_degree})
-        log(f"bfs {name:15s} {bfs[name]['ms_per_source']:8.3f} ms/src {bfs[name]['gteps']:7.2f} GTEPS")
-
-    # Batched BFS: the same 256 GAP-picked sources as one bit-parallel batch
-    # on the RADAR graph vs one query at a time (single-source engine).
-    batched = None
-    if not args.quick:
+        ms = median_pr(dg, it, rg.opts("push"), reps=1)
+        pr["push"] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2)}
+        pr["radar_vs_push"] = round(pr["radar_tuned"]["gteps"] / pr["push"]["gteps"], 2)
+        log(f"pagerank push {ms:.1f} ms {pr['push']['gteps']} GTEPS; RADAR {pr['radar_vs_push']}x")
+        # BFS from GAP-picked sources: push-only vs autotuned RADAR.
+        srcs = hg.pick_sources(args.bfs_sources)
+        bt, _ = sg.autotune("bfs", sources=new_ids[srcs[:4]], base=rg.opts("push"))
+        bfs["push"] = bfs_gteps(dg, srcs, rg.opts("push"))
+        bfs["radar_tuned"] = bfs_gteps(sg, new_ids[srcs], bt)
+        bfs["radar_tuned"].update({"engine": ENGINES[bt.variant], "threshold": bt.threshold,
+                                   "heavy_degree": bt.heavy_degree})
+        bfs["radar_vs_push"] = round(bfs["radar_tuned"]["gteps"] / bfs["push"]["gteps"], 2)
+        log(f"bfs push {bfs['push']['gteps']} GTEPS, radar_tuned ({ENGINES[bt.variant]}) "
+            f"{bfs['radar_tuned']['gteps']} GTEPS ({bfs['radar_vs_push']}x)")
+        # Batched BFS: 256 sources as one bit-parallel batch vs the same
+        # queries one at a time on the best single-source engine.
         bsrc = new_ids[hg.pick_sources(args.batch)]
-        sg.bfs_query(int(bsrc[0]))
+        sg.bfs_query(int(bsrc[0]), bt)
         t0 = time.perf_counter()
-        single = [sg.bfs_query(int(s))[0] for s in bsrc[:64]]
+        single = [sg.bfs_query(int(s), bt)[0] for s in bsrc[:64]]
         t_single = (time.perf_counter() - t0) / 64
         sg.bfs_batch(bsrc)
         reps = 3
@@ -225,15 +248,12 @@ def main():
         batched = {"sources": len(bsrc), "batch_ms": round(t_batch * 1e3, 3),
                    "queries_per_s": round(len(bsrc) / t_batch, 1),
                    "one_at_a_time_queries_per_s": round(1.0 / t_single, 1),
+                   "one_at_a_time_engine": ENGINES[bt.variant],
                    "speedup": round(t_single * len(bsrc) / t_batch, 2),
                    "equivalent_gteps": round(sum(x.edges for x in out) / t_batch / 1e9, 1),
                    "answers_equal_single": same}
         log(f"batched bfs: {batched['queries_per_s']} q/s vs {batched['one_at_a_time_queries_per_s']} q/s "
             f"one at a time ({batched['speedup']}x), equal answers: {same}")
-
-    serving = None
-    if not args.quick and not args.no_serve:
-        serving = serve_bench(rg, args, log)
 
     cpu = None
     if not args.no_cpu:
@@ -252,6 +272,18 @@ def main():
                "sample": f"oracle push PageRank (OpenMP, CAS-loop FP64 adds as the reference), "
                          f"{cpu_iters} iterations on the same rmat{args.scale} graph, {tc:.1f} s"}
         log(f"cpu baseline: {cpu['value']} GTEPS on {cores} threads")
+        del off, nbr
+    for x in (sg, dg):
+        x.close()
+    hg.close()
+    ctx.close()
+
+    variants = None
+    if not args.quick and args.variants_scale > 0:
+        variants = variants_table(rg, args)
+    serving = None
+    if not args.quick and not args.no_serve:
+        serving = serve_bench(rg, args)
 
     out = {
         "metric": "GTEPS per app on RMAT-26 + batched BFS queries/s, p50/p99 vs one-at-a-time",
@@ -266,9 +298,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (GAP R-MAT generator, seed 27491095)",
-        "config": {"workload": f"PageRank {it} iterations, RADAR (degree-sorted graph, hub-duplicated "
-                               f"LDS accumulators, autotuned engine: {engine}), directed rmat{args.scale} "
-                               f"edge factor 16",
+        "config": {"workload": f"PageRank {it} iterations, RADAR (degree-sorted graph, autotuned engine: {engine}), "
+                               f"symmetric rmat{args.scale} edge factor 16",
                    "n": n, "m": m, "iterations": it, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
@@ -280,6 +311,7 @@ def main():
         "pagerank": pr,
         "bfs": bfs,
         "batched_bfs": batched,
+        "variants": variants,
         "serving": serving,
         "preprocess_ms": {"host_graph_build": round(t_gen * 1e3, 1), "gpu_degree_sort": round(t_sort * 1e3, 2),
                           "autotune": round(t_tune * 1e3, 1)},

@@ -14,6 +14,9 @@ uint64 offsets, FP64 ranks).
   bit-parallel batch vs one query at a time) and the CPU oracle baseline.
 * Variants table (directed R-MAT scale 24): PageRank and BFS for push-only,
   HUBDUP, DegreeSorting, RADAR, pull and the blocked engine.
+* Radii (64 starts) and BC (one source) on symmetric R-MAT scale 25 and an
+  Erdos-Renyi control of the same size: push-only, HUBDUP, RADAR.
+* Triangle counting on the degree-sorted symmetric R-MAT scale 22.
 * Serving (symmetric R-MAT scale 22, degree-sorted): BFS-only and 80/15/5
   BFS/BC/PageRank-10 traces, saturated and as Poisson arrivals at 80% of
   the one-at-a-time capacity, batched (continuous batching) vs one at a
@@ -91,6 +94,74 @@ def variants_table(rg, args):
     return out
 
 
+def radii_bc_study(rg, args):
+    """Radii (64 hashInt starts) and BC (one GAP-picked source) on symmetric
+    R-MAT and on an Erdos-Renyi control with the same n and about the same
+    m (uniform degrees: hub copies should gain nothing there)."""
+    out = {}
+    rm = None
+    for kind in ("rmat", "er"):
+        t0 = time.time()
+        if kind == "rmat":
+            hg = rg.HostGraph.generate(args.radii_scale, 16, symmetric=True)
+        else:
+            deg = max(1, int(round(rm / 2 / (1 << args.radii_scale))))
+            hg = rg.HostGraph.generate(args.radii_scale, deg, uniform=True, symmetric=True)
+        n, m, _ = hg.info()
+        rm = rm or m
+        ctx = rg.Context(0)
+        dg = rg.DeviceGraph(ctx, hg)
+        sg, ids = dg.degree_sort()
+        log(f"{kind}{args.radii_scale} symmetric n={n} m={m} ({time.time() - t0:.1f}s)")
+        starts = rg.radii_starts(n)
+        row = {"n": n, "m": m, "radii_ms": {}, "bc_ms": {}}
+        for name, g, st, o in [("push", dg, starts, rg.opts("push")),
+                               ("hubdup", dg, starts, rg.opts("hubdup", hubs=args.hubs)),
+                               ("radar", sg, ids[starts], rg.opts("radar", hubs=args.hubs))]:
+            g.radii(st, o)
+            _, rounds, ms = g.radii(st, o)
+            row["radii_ms"][name] = round(ms, 3)
+            row["radii_rounds"] = rounds
+        src = hg.pick_bc_sources(1)
+        for name, g, ss, o in [("push", dg, src, rg.opts("push")),
+                               ("hubdup", dg, src, rg.opts("hubdup", hubs=args.hubs)),
+                               ("radar", sg, ids[src], rg.opts("radar", hubs=args.hubs))]:
+            g.bc(ss, o)
+            _, _, ms = g.bc(ss, o)
+            row["bc_ms"][name] = round(ms, 3)
+        log(f"  radii ms {row['radii_ms']}  bc ms {row['bc_ms']}")
+        out[f"{kind}{args.radii_scale}"] = row
+        for x in (sg, dg):
+            x.close()
+        hg.close()
+        ctx.close()
+    return out
+
+
+def tc_study(rg, args):
+    """Triangle counting on the degree-sorted symmetric R-MAT (oriented by
+    id after the relabel: low degree -> high degree), reorder time reported
+    apart from count time."""
+    hg = rg.HostGraph.generate(args.tc_scale, 16, symmetric=True)
+    n, m, _ = hg.info()
+    ctx = rg.Context(0)
+    dg = rg.DeviceGraph(ctx, hg)
+    t0 = time.time()
+    sg, _ = dg.degree_sort(sort_neighbors=True)
+    ctx.sync()
+    reorder = (time.time() - t0) * 1e3
+    sg.tc()
+    total, ms = sg.tc()
+    out = {"graph": f"rmat{args.tc_scale} symmetric, degree-sorted", "n": n, "m": m, "triangles": total,
+           "count_ms": round(ms, 3), "reorder_ms": round(reorder, 2)}
+    log(f"tc rmat{args.tc_scale}: {total} triangles in {ms:.1f} ms (reorder {reorder:.1f} ms)")
+    for x in (sg, dg):
+        x.close()
+    hg.close()
+    ctx.close()
+    return out
+
+
 def serve_bench(rg, args):
     """Serving over a resident degree-sorted symmetric R-MAT graph: seeded
     query traces (BFS only; 80/15/5 BFS/BC/PageRank-10) run saturated (all
@@ -146,6 +217,8 @@ def main():
     ap.add_argument("--hubs", type=int, default=0, help="LDS hub slots (0 = LDS budget)")
     ap.add_argument("--variants-scale", type=int, default=24, help="directed R-MAT of the variants table (0: skip)")
     ap.add_argument("--variants-bfs-sources", type=int, default=64)
+    ap.add_argument("--radii-scale", type=int, default=25, help="Radii/BC study: R-MAT and ER scale (0: skip)")
+    ap.add_argument("--tc-scale", type=int, default=22, help="triangle counting R-MAT scale (0: skip)")
     ap.add_argument("--serve-scale", type=int, default=22, help="R-MAT scale of the serving graph (symmetric)")
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true")
@@ -281,6 +354,12 @@ def main():
     variants = None
     if not args.quick and args.variants_scale > 0:
         variants = variants_table(rg, args)
+    study = None
+    if not args.quick and args.radii_scale > 0:
+        study = radii_bc_study(rg, args)
+    tc = None
+    if not args.quick and args.tc_scale > 0:
+        tc = tc_study(rg, args)
     serving = None
     if not args.quick and not args.no_serve:
         serving = serve_bench(rg, args)
@@ -312,6 +391,8 @@ def main():
         "bfs": bfs,
         "batched_bfs": batched,
         "variants": variants,
+        "radii_bc": study,
+        "tc": tc,
         "serving": serving,
         "preprocess_ms": {"host_graph_build": round(t_gen * 1e3, 1), "gpu_degree_sort": round(t_sort * 1e3, 2),
                           "autotune": round(t_tune * 1e3, 1)},

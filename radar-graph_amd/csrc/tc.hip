@@ -8,13 +8,15 @@
 // degree, RelabelByDegree) the prefixes are the "oriented low-to-high
 // degree" lists and stay short.
 //
-// Mapping: one wavefront per vertex u; its lanes take the arcs (u, v) of
-// N+(u) and each merges N+(u) with N+(v).
+// Mapping: one wavefront per vertex u.  With per-vertex counts its lanes take
+// the arcs (u, v) of N+(u) and each merges N+(u) with N+(v); for the total
+// alone the (v, w) work is flattened over the lanes (k_tc_flat).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "device.h"
+#include "engine.h"
 
 namespace rg {
 namespace {
@@ -73,6 +75,99 @@ __global__ __launch_bounds__(256) void k_tc(const uint64_t* off, const uint32_t*
   }
 }
 
+// Totals only: the same count with the (v, w) work of a vertex u spread
+// evenly over the wavefront.  N+(u) (ascending) is staged in the wave's LDS
+// when it fits; for each group of 64 v = N+(u)[i] the wave scans |N+(v)|,
+// then every lane takes one w of the concatenated N+(v) lists and
+// binary-searches it in N+(u)[0, i) (a common neighbour w < v must sit below
+// v's position in N+(u)).  N+(u)[0] is skipped: nothing in N+(u) lies below it.
+constexpr int kTcLds = 1024;
+
+template <bool LDS>
+__device__ __forceinline__ bool tc_find(const uint32_t* nu_lds, const uint32_t* nu_glb, uint32_t len, uint32_t x) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t y = LDS ? nu_lds[mid] : nu_glb[mid];
+    if (y < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < len && (LDS ? nu_lds[lo] : nu_glb[lo]) == x;
+}
+
+template <bool LDS>
+__device__ __forceinline__ unsigned long long tc_vertex(const uint64_t* off, const uint32_t* nbr, uint64_t ub,
+                                                        uint32_t du, const uint32_t* nu, uint32_t* pre,
+                                                        uint64_t* vbeg, uint32_t* vidx) {
+  const int lane = (int)__lane_id();
+  const uint32_t* nu_glb = nbr + ub;
+  unsigned long long c = 0;
+  for (uint32_t i0 = 1; i0 < du; i0 += kWave) {
+    const uint32_t i = i0 + lane;
+    uint32_t dv = 0;
+    uint64_t vb = 0;
+    if (i < du) {
+      const uint32_t v = LDS ? nu[i] : nu_glb[i];
+      vb = off[v];
+      dv = (uint32_t)prefix_below(nbr, vb, off[v + 1], v);
+    }
+    uint32_t inc = dv;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t t = __shfl_up(inc, o);
+      if (lane >= o) inc += t;
+    }
+    const uint32_t total = __shfl(inc, kWave - 1);
+    pre[lane] = inc - dv;
+    vbeg[lane] = vb;
+    vidx[lane] = i;
+    if (lane == 0) pre[kWave] = total;
+    wave_sync_lds();
+    for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
+      const uint32_t t = t0 + lane;
+      if (t < total) {
+        int j = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (pre[j + step] <= t) j += step;
+        const uint32_t w = nbr[vbeg[j] + (t - pre[j])];
+        if (tc_find<LDS>(nu, nu_glb, vidx[j], w)) ++c;
+      }
+    }
+    wave_sync_lds();
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(256) void k_tc_flat(const uint64_t* off, const uint32_t* nbr, int64_t n,
+                                                 unsigned long long* total) {
+  constexpr int WPB = 256 / kWave;
+  __shared__ uint32_t s_nu[WPB][kTcLds];
+  __shared__ uint32_t s_pre[WPB][kWave + 1];
+  __shared__ uint64_t s_vb[WPB][kWave];
+  __shared__ uint32_t s_vi[WPB][kWave];
+  const int lane = (int)__lane_id();
+  const int w = threadIdx.x / kWave;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  unsigned long long c = 0;
+  for (int64_t u = wid; u < n; u += nw) {
+    const uint64_t ub = off[u], ue = off[u + 1];
+    uint64_t du = 0;
+    if (lane == 0) du = prefix_below(nbr, ub, ue, (uint32_t)u);
+    du = __shfl(du, 0);
+    if (du < 2) continue;
+    if (du <= (uint64_t)kTcLds) {
+      for (uint64_t i = lane; i < du; i += kWave) s_nu[w][i] = nbr[ub + i];
+      wave_sync_lds();
+      c += tc_vertex<true>(off, nbr, ub, (uint32_t)du, s_nu[w], s_pre[w], s_vb[w], s_vi[w]);
+    } else {
+      c += tc_vertex<false>(off, nbr, ub, (uint32_t)du, nullptr, s_pre[w], s_vb[w], s_vi[w]);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane == 0 && c) atomicAdd(total, c);
+}
+
 }  // namespace
 }  // namespace rg
 
@@ -100,8 +195,10 @@ extern "C" int rg_tc(rg_graph* g, const rg_opts* o, uint64_t* total_out, uint64_
   RG_HIP(hipEventCreate(&b));
   RG_HIP(hipEventRecord(a, s));
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 3) / 4, (int64_t)ctx->num_cus * 32));
-  hipLaunchKernelGGL(rg::k_tc, grid, 256, 0, s, g->off, g->nbr, n, d_total,
-                     per_vertex ? tri.as<unsigned long long>() : nullptr);
+  if (per_vertex)
+    hipLaunchKernelGGL(rg::k_tc, grid, 256, 0, s, g->off, g->nbr, n, d_total, tri.as<unsigned long long>());
+  else
+    hipLaunchKernelGGL(rg::k_tc_flat, grid, 256, 0, s, g->off, g->nbr, n, d_total);
   RG_HIP(hipGetLastError());
   RG_HIP(hipEventRecord(b, s));
   RG_HIP(hipEventSynchronize(b));

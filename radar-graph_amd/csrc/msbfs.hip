@@ -40,7 +40,11 @@
 namespace rg {
 
 constexpr int kMsVB = kQueryVB;      // 4096 destinations per dense block
-constexpr int kMsThreads = 512;      // dense-level workgroup
+// Dense-level workgroup: the LDS copy of a 4096-vertex block (W words per
+// vertex) allows 4 / 2 / 1 workgroups per CU for W = 1 / 2 / 4, so wider
+// batches use wider workgroups to keep as many gathers in flight.
+template <int W>
+constexpr int ms_threads() { return W == 1 ? 512 : 1024; }
 constexpr int kMsMaxSources = 256;
 constexpr int kMsLevelsPerSync = 4;  // levels launched between host checks
 
@@ -612,6 +616,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   // copy limits how many run per CU: 4 for W = 1, 2 for W = 2, 1 for W = 4).
   const int gdense = (int)std::min<int64_t>(B->nunits, (int64_t)ctx->num_cus * 4);
   const int gclr = grid_for(ctx, std::max<int64_t>(nwords, 1));
+  constexpr int kMsThreads = ms_threads<W>();
   constexpr int PER = (1 << kMsVB) / kMsThreads;
   int64_t steps = 0;
   for (;;) {

@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--sections", default="headline,variants,radii,tc,serve",
+                    help="comma list of sections to run (headline,variants,radii,tc,serve)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the library's plain 1 GiB stream copy once (k_stream_copy: the "
                          "known-size kernel that scales FETCH_SIZE / WRITE_SIZE)")
@@ -234,6 +236,21 @@ def main():
         sys.exit(2)
 
     import radar_graph as rg
+
+    sections = set(args.sections.split(","))
+    if "headline" not in sections:
+        # Sections on their own (e.g. for counter collection): no headline line.
+        out = {}
+        if "variants" in sections and args.variants_scale > 0:
+            out["variants"] = variants_table(rg, args)
+        if "radii" in sections and args.radii_scale > 0:
+            out["radii_bc"] = radii_bc_study(rg, args)
+        if "tc" in sections and args.tc_scale > 0:
+            out["tc"] = tc_study(rg, args)
+        if "serve" in sections and not args.no_serve:
+            out["serving"] = serve_bench(rg, args)
+        print(json.dumps(out), flush=True)
+        return
 
     t0 = time.time()
     hg = rg.HostGraph.generate(args.scale, 16, uniform=False, symmetric=True)
@@ -352,16 +369,16 @@ def main():
     ctx.close()
 
     variants = None
-    if not args.quick and args.variants_scale > 0:
+    if not args.quick and "variants" in sections and args.variants_scale > 0:
         variants = variants_table(rg, args)
     study = None
-    if not args.quick and args.radii_scale > 0:
+    if not args.quick and "radii" in sections and args.radii_scale > 0:
         study = radii_bc_study(rg, args)
     tc = None
-    if not args.quick and args.tc_scale > 0:
+    if not args.quick and "tc" in sections and args.tc_scale > 0:
         tc = tc_study(rg, args)
     serving = None
-    if not args.quick and not args.no_serve:
+    if not args.quick and "serve" in sections and not args.no_serve:
         serving = serve_bench(rg, args)
 
     out = {

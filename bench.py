@@ -77,6 +77,16 @@ def variants_table(rg, args):
         ms = median_pr(g, it, o)
         out["pagerank"][name] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2)}
         log(f"  pagerank {name:16s} {ms:8.2f} ms {it * m / ms / 1e6:8.2f} GTEPS")
+    # Hub count k swept from the LDS budget to the L2 budget: LDS copies of
+    # k hubs per workgroup alone, then 16K in LDS plus k2 per-XCD copies.
+    sweep = []
+    for variant, g in (("hubdup", dg), ("radar", sg)):
+        for k, k2 in ((1024, 1), (4096, 1), (16384, 1), (16384, 65536), (16384, 262144), (16384, 1048576)):
+            ms = median_pr(g, it, rg.opts(variant, hubs=k, l2_hubs=k2), reps=1)
+            sweep.append({"variant": variant, "lds_hubs": k, "l2_hubs": max(k, k2), "ms": round(ms, 3),
+                          "gteps": round(it * m / ms / 1e6, 2)})
+            log(f"  pagerank {variant} k={k} k2={max(k, k2)}: {it * m / ms / 1e6:8.2f} GTEPS")
+    out["hub_sweep"] = sweep
     srcs = hg.pick_sources(args.variants_bfs_sources)
     bt, _ = sg.autotune("bfs", sources=ids[srcs[:4]], base=rg.opts("push"))
     bfs_runs = [("push", dg, srcs, rg.opts("push")), ("hubdup", dg, srcs, rg.opts("hubdup", hubs=args.hubs)),

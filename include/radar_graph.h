@@ -53,7 +53,11 @@ typedef struct rg_hgraph rg_hgraph;
 enum { RG_FMT_ADJ = 0,        /* Ligra AdjacencyGraph text (IO.h:207-475)      */
        RG_FMT_ADJ_BINARY = 1, /* Ligra .config/.adj/.idx binary (IO.h:477-626) */
        RG_FMT_EL = 2,         /* GAP edge list text (reader.h:50-57)           */
-       RG_FMT_SG = 3 };       /* GAP serialized graph (reader.h:251-304)       */
+       RG_FMT_SG = 3,         /* GAP serialized graph (reader.h:251-304)       */
+       RG_FMT_WEL = 4,        /* GAP weighted edge list, weights dropped (reader.h:59-67) */
+       RG_FMT_GR = 5,         /* DIMACS "a u v w" arcs (reader.h:69-83)          */
+       RG_FMT_METIS = 6,      /* METIS .graph, 1-based (reader.h:85-141)         */
+       RG_FMT_MTX = 7 };      /* Matrix Market coordinate (reader.h:143-217)     */
 
 /* GAP generator (GAP/src/generator.h:64-124): 2^scale vertices,
  * degree*2^scale generated edges, R-MAT (A,B,C = 0.57,0.19,0.19) unless
@@ -73,6 +77,9 @@ int rg_hgraph_from_edges(int64_t n, int64_t nedges, const uint32_t* src,
 int rg_hgraph_from_csr(int64_t n, int64_t m, const uint64_t* offsets,
                        const uint32_t* nbrs, int symmetric, int dedup, rg_hgraph** out);
 int rg_hgraph_read(const char* path, int format, int symmetric, rg_hgraph** out);
+/* Format by file suffix, as GAP's Reader dispatches (reader.h:219-249):
+ * .el .wel .sg .gr .graph .mtx; anything else is Ligra text (RG_FMT_ADJ). */
+int rg_format_of(const char* path);
 int rg_hgraph_write(const rg_hgraph* g, const char* path, int format);
 int rg_hgraph_info(const rg_hgraph* g, int64_t* n, int64_t* m, int* symmetric);
 /* Borrowed views (valid until rg_hgraph_free).  in_edges=1 selects the CSC

@@ -284,8 +284,9 @@ template <class vertex>
 void Compute(graph<vertex>& G, commandLine P, pvector<uintE>& new_ids);
 
 // parallel_main (ligra.h:559-654): read the graph (-s symmetric, -b Ligra
-// binary; otherwise AdjacencyGraph text; files ending in .el/.sg use the GAP
-// readers), optionally degree-sort it on the GPU (-preprocess 0 out-degree,
+// binary; otherwise AdjacencyGraph text; files ending in .el/.wel/.sg/.gr/
+// .graph/.mtx use the GAP readers), optionally degree-sort it on the GPU
+// (-preprocess 0 out-degree,
 // 1 in-degree; the DS/RADAR default is 1), then run Compute once plus
 // -rounds more times, transposing a directed graph back between rounds.
 // Extensions: -variant push|hubdup|radar|blocked|pull, -k <hub slots>,
@@ -305,9 +306,7 @@ inline int ligra_main(int argc, char** argv, int default_preprocess = -1) {
                : variant == "pull" ? RG_PULL : RG_PUSH;
   eo.hubs = P.getOptionLongValue("-k", 0);
   std::string f(iFile);
-  int fmt = binary ? RG_FMT_ADJ_BINARY : RG_FMT_ADJ;
-  if (f.size() > 3 && f.substr(f.size() - 3) == ".el") fmt = RG_FMT_EL;
-  if (f.size() > 3 && f.substr(f.size() - 3) == ".sg") fmt = RG_FMT_SG;
+  int fmt = binary ? RG_FMT_ADJ_BINARY : rg_format_of(f.c_str());
   Timer t;
   t.Start();
   graph<symmetricVertex> G;

@@ -1,7 +1,7 @@
 // Triangle counting with GAP's command line (push-new-orig GAP/src/
 // tc_local.cc:151-163, command_line.h:28-139): -g <scale> R-MAT or
 // -u <scale> uniform (both symmetrised), -k <degree> (default 16),
-// -f <file> (.el / .sg / Ligra .adj), -s symmetrize, -n <trials>, -v verify,
+// -f <file> (.el / .wel / .sg / .gr / .graph / .mtx / Ligra .adj), -s symmetrize, -n <trials>, -v verify,
 // -a analysis (prints the count).  -r relabels by degree first (the
 // degree-sorting and RADAR trees, tc_local.cc:116 / 158); the relabel is
 // timed separately from the count.
@@ -63,9 +63,7 @@ int main(int argc, char** argv) {
   if (scale != -1) symmetrize = true;
   rg_hgraph* hg = nullptr;
   if (!file.empty()) {
-    int fmt = RG_FMT_ADJ;
-    if (file.size() > 3 && file.substr(file.size() - 3) == ".el") fmt = RG_FMT_EL;
-    if (file.size() > 3 && file.substr(file.size() - 3) == ".sg") fmt = RG_FMT_SG;
+    int fmt = rg_format_of(file.c_str());
     check(rg_hgraph_read(file.c_str(), fmt, symmetrize ? 1 : 0, &hg), "read");
   } else {
     check(rg_hgraph_generate(scale, degree, uniform ? 1 : 0, 1, &hg), "generate");

@@ -430,6 +430,208 @@ static int read_sg(const char* path, rg_hgraph** out) {
   return rc;
 }
 
+// ---- GAP's other text readers (reader.h:59-217) -----------------------
+// Each yields an edge list that the builder turns into a CSR with
+// num_nodes = max id + 1 (builder.h:209-210); weights are read and dropped
+// (the path's graphs are unweighted).
+
+// Splits buf into lines and each line into whitespace-separated tokens.
+struct TextLines {
+  const std::string& buf;
+  size_t pos = 0;
+  explicit TextLines(const std::string& b) : buf(b) {}
+  bool next(std::string& line) {
+    if (pos >= buf.size()) return false;
+    size_t e = buf.find('\n', pos);
+    if (e == std::string::npos) e = buf.size();
+    line.assign(buf, pos, e - pos);
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    pos = e + 1;
+    return true;
+  }
+};
+
+static void tokens(const std::string& line, std::vector<std::string>& t) {
+  t.clear();
+  size_t i = 0;
+  while (i < line.size()) {
+    while (i < line.size() && std::isspace((unsigned char)line[i])) ++i;
+    if (i >= line.size()) break;
+    size_t st = i;
+    while (i < line.size() && !std::isspace((unsigned char)line[i])) ++i;
+    t.emplace_back(line, st, i - st);
+  }
+}
+
+static bool to_id(const std::string& tok, int64_t base, uint32_t& out) {
+  if (tok.empty()) return false;
+  uint64_t v = 0;
+  for (char c : tok) {
+    if (c < '0' || c > '9') return false;
+    v = v * 10 + (uint64_t)(c - '0');
+    if (v > (uint64_t)INT32_MAX + 1) return false;
+  }
+  if ((int64_t)v < base) return false;
+  out = (uint32_t)((int64_t)v - base);
+  return true;
+}
+
+struct EdgeAcc {
+  std::vector<uint32_t> s, d;
+  uint64_t maxid = 0;
+  bool any = false;
+  void add(uint32_t a, uint32_t b) {
+    s.push_back(a);
+    d.push_back(b);
+    maxid = std::max<uint64_t>(maxid, std::max(a, b));
+    any = true;
+  }
+  int build(int symmetric, rg_hgraph** out) {
+    return rg_hgraph_from_edges(any ? (int64_t)maxid + 1 : 0, (int64_t)s.size(), s.data(), d.data(), symmetric, out);
+  }
+};
+
+// .wel: "u v w" triples (ReadInWEL, reader.h:59-67).
+static int read_wel(const std::string& buf, int symmetric, rg_hgraph** out) {
+  std::vector<std::pair<size_t, size_t>> w;
+  split_words(buf, w);
+  if (w.size() % 3) return set_error(RG_EFORMAT, "weighted edge list: not u v w triples");
+  EdgeAcc el;
+  for (size_t i = 0; i < w.size(); i += 3) {
+    uint32_t a, b;
+    if (!to_id(buf.substr(w[i].first, w[i].second), 0, a) || !to_id(buf.substr(w[i + 1].first, w[i + 1].second), 0, b))
+      return set_error(RG_EFORMAT, "Bad input file");
+    el.add(a, b);
+  }
+  return el.build(symmetric, out);
+}
+
+// .gr (DIMACS): "a u v w" arc lines, ids kept as written; other lines
+// ignored (ReadInGR, reader.h:69-83).
+static int read_gr(const std::string& buf, int symmetric, rg_hgraph** out) {
+  TextLines lines(buf);
+  std::string line;
+  std::vector<std::string> t;
+  EdgeAcc el;
+  while (lines.next(line)) {
+    if (line.empty() || line[0] != 'a') continue;
+    tokens(line, t);
+    uint32_t a, b;
+    if (t.size() < 3 || t[0] != "a" || !to_id(t[1], 0, a) || !to_id(t[2], 0, b))
+      return set_error(RG_EFORMAT, "Bad input file");
+    el.add(a, b);
+  }
+  return el.build(symmetric, out);
+}
+
+// .graph (METIS): header "n m [fmt]" (fmt 1: weighted), then one line per
+// vertex listing its 1-based neighbours (with weights for fmt 1); '%' lines
+// are comments (ReadInMetis, reader.h:85-141).
+static int read_metis(const std::string& buf, int symmetric, rg_hgraph** out) {
+  TextLines lines(buf);
+  std::string line;
+  std::vector<std::string> t;
+  bool header = false, weights = false;
+  int64_t num_nodes = 0, u = 0;
+  EdgeAcc el;
+  while (lines.next(line)) {
+    if (!line.empty() && line[0] == '%') continue;
+    if (!header) {
+      tokens(line, t);
+      if (t.size() < 2) return set_error(RG_EFORMAT, "METIS header needs n and m");
+      num_nodes = std::atoll(t[0].c_str());
+      if (t.size() >= 3) {
+        const int fmt = std::atoi(t[2].c_str());
+        if (fmt == 1) weights = true;
+        else if (fmt != 0 && fmt != 100)
+          return set_error(RG_EFORMAT, "Do not support METIS fmt type: " + t[2]);
+      }
+      header = true;
+      continue;
+    }
+    if (u >= num_nodes) break;
+    tokens(line, t);
+    const size_t step = weights ? 2 : 1;
+    if (weights && t.size() % 2) return set_error(RG_EFORMAT, "METIS: neighbour without weight");
+    for (size_t i = 0; i < t.size(); i += step) {
+      uint32_t v;
+      if (!to_id(t[i], 1, v)) return set_error(RG_EFORMAT, "Bad input file");
+      el.add((uint32_t)u, v);
+    }
+    ++u;
+  }
+  if (!header) return set_error(RG_EFORMAT, "METIS header missing");
+  return el.build(symmetric, out);
+}
+
+// .mtx: "%%MatrixMarket matrix coordinate <field> <symmetry>", '%'
+// comments, "rows cols nonzeros", then 1-based "u v [w]" entries;
+// symmetric matrices add (v, u) (ReadInMTX, reader.h:143-217).
+static int read_mtx(const std::string& buf, int symmetric, rg_hgraph** out) {
+  TextLines lines(buf);
+  std::string line;
+  std::vector<std::string> t;
+  if (!lines.next(line)) return set_error(RG_EFORMAT, ".mtx file did not start with %%MatrixMarket");
+  tokens(line, t);
+  if (t.size() < 5 || t[0] != "%%MatrixMarket") return set_error(RG_EFORMAT, ".mtx file did not start with %%MatrixMarket");
+  if (t[1] != "matrix" || t[2] != "coordinate")
+    return set_error(RG_EFORMAT, "only allow matrix coordinate format for .mtx");
+  if (t[3] == "complex") return set_error(RG_EFORMAT, "do not support complex weights for .mtx");
+  bool weights;
+  if (t[3] == "pattern") weights = false;
+  else if (t[3] == "real" || t[3] == "double" || t[3] == "integer") weights = true;
+  else return set_error(RG_EFORMAT, "unrecognized field type for .mtx");
+  bool undirected;
+  if (t[4] == "symmetric") undirected = true;
+  else if (t[4] == "general" || t[4] == "skew-symmetric") undirected = false;
+  else return set_error(RG_EFORMAT, "unsupported symmetry type for .mtx");
+  bool sized = false;
+  EdgeAcc el;
+  while (lines.next(line)) {
+    if (!line.empty() && line[0] == '%') continue;
+    tokens(line, t);
+    if (t.empty()) continue;
+    if (!sized) {
+      if (t.size() < 3 || t[0] != t[1]) return set_error(RG_EFORMAT, "matrix must be square for .mtx");
+      sized = true;
+      continue;
+    }
+    uint32_t a, b;
+    if (t.size() < (weights ? 3u : 2u) || !to_id(t[0], 1, a) || !to_id(t[1], 1, b))
+      return set_error(RG_EFORMAT, "Bad input file");
+    el.add(a, b);
+    if (undirected) el.add(b, a);
+  }
+  return el.build(symmetric, out);
+}
+
+static int read_text_format(const char* path, int format, int symmetric, rg_hgraph** out) {
+  std::string buf;
+  if (!read_file(path, buf)) return set_error(RG_EIO, std::string("cannot read ") + path);
+  switch (format) {
+    case RG_FMT_WEL: return read_wel(buf, symmetric, out);
+    case RG_FMT_GR: return read_gr(buf, symmetric, out);
+    case RG_FMT_METIS: return read_metis(buf, symmetric, out);
+    default: return read_mtx(buf, symmetric, out);
+  }
+}
+
+int rg_format_of(const char* path) {
+  if (!path) return RG_FMT_ADJ;
+  const std::string f(path);
+  auto ends = [&](const char* suf) {
+    const size_t k = std::strlen(suf);
+    return f.size() > k && f.compare(f.size() - k, k, suf) == 0;
+  };
+  if (ends(".el")) return RG_FMT_EL;
+  if (ends(".wel")) return RG_FMT_WEL;
+  if (ends(".sg")) return RG_FMT_SG;
+  if (ends(".gr")) return RG_FMT_GR;
+  if (ends(".graph")) return RG_FMT_METIS;
+  if (ends(".mtx")) return RG_FMT_MTX;
+  return RG_FMT_ADJ;
+}
+
 int rg_hgraph_read(const char* path, int format, int symmetric, rg_hgraph** out) {
   if (!path || !out) return set_error(RG_EINVAL, "rg_hgraph_read: null argument");
   try {
@@ -438,6 +640,10 @@ int rg_hgraph_read(const char* path, int format, int symmetric, rg_hgraph** out)
       case RG_FMT_ADJ_BINARY: return read_adj_binary(path, symmetric, out);
       case RG_FMT_EL: return read_el(path, symmetric, out);
       case RG_FMT_SG: return read_sg(path, out);
+      case RG_FMT_WEL:
+      case RG_FMT_GR:
+      case RG_FMT_METIS:
+      case RG_FMT_MTX: return read_text_format(path, format, symmetric, out);
       default: return set_error(RG_EINVAL, "rg_hgraph_read: unknown format");
     }
   } catch (const std::bad_alloc&) {

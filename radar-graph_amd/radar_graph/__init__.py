@@ -26,7 +26,7 @@ if not os.path.exists(LIB_PATH):
 lib = C.CDLL(LIB_PATH)
 
 RG_OK, RG_EINVAL, RG_ENOMEM, RG_EDEVICE, RG_EIO, RG_EFORMAT, RG_EUNSUPPORTED = range(7)
-FMT_ADJ, FMT_ADJ_BINARY, FMT_EL, FMT_SG = range(4)
+FMT_ADJ, FMT_ADJ_BINARY, FMT_EL, FMT_SG, FMT_WEL, FMT_GR, FMT_METIS, FMT_MTX = range(8)
 PUSH, PULL, HUBDUP, RADAR, BLOCKED = range(5)
 HUBS_LDS, HUBS_L2 = range(2)
 VARIANTS = {"push": PUSH, "pull": PULL, "hubdup": HUBDUP, "radar": RADAR, "blocked": BLOCKED}

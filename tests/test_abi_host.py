@@ -150,3 +150,92 @@ def test_bfs_summary_by_hand(orc):
     assert (it, reached, edges) == (3, 3, 2)
     want = sum(int(orc.level_hash(v, l)) for v, l in [(0, 0), (1, 1), (2, 2)]) % (1 << 64)
     assert cs == want
+
+
+# ---- GAP's other text formats (reader.h:59-217) ----------------------------------
+# A small graph written in each format must read back as the same CSR as its
+# edge list through the reference's builder rules (dedup, no self-loops,
+# num_nodes = max id + 1); ids are 1-based in METIS and Matrix Market,
+# as written in DIMACS .gr.
+
+EDGES = [(0, 1), (1, 2), (2, 0), (2, 3), (3, 3), (1, 2), (4, 0)]
+
+
+def _expect(orc, edges, sym, shift=0):
+    src = np.array([u + shift for u, _ in edges])
+    dst = np.array([v + shift for _, v in edges])
+    n = int(max(src.max(), dst.max())) + 1
+    return orc.csr_from_edges(n, src, dst, sym)
+
+
+@pytest.mark.parametrize("sym", [True, False])
+def test_read_wel(rg, orc, tmp_path, sym):
+    p = tmp_path / "g.wel"
+    p.write_text("".join(f"{u} {v} {3 * u + v}\n" for u, v in EDGES))
+    hg = rg.HostGraph.read(str(p), rg.FMT_WEL, symmetric=sym)
+    off, nb = _expect(orc, EDGES, sym)
+    assert np.array_equal(hg.offsets(), off) and np.array_equal(hg.neighbors(), nb)
+
+
+@pytest.mark.parametrize("sym", [True, False])
+def test_read_dimacs_gr(rg, orc, tmp_path, sym):
+    p = tmp_path / "g.gr"
+    body = "c comment line\np sp 6 7\n" + "".join(f"a {u + 1} {v + 1} 7\n" for u, v in EDGES)
+    p.write_text(body)
+    hg = rg.HostGraph.read(str(p), rg.FMT_GR, symmetric=sym)
+    off, nb = _expect(orc, EDGES, sym, shift=1)   # DIMACS ids kept as written
+    assert np.array_equal(hg.offsets(), off) and np.array_equal(hg.neighbors(), nb)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_read_metis(rg, orc, tmp_path, weighted):
+    n = 5
+    adj = [[] for _ in range(n)]
+    for u, v in EDGES:
+        adj[u].append(v)
+    lines = ["% a METIS file", f"{n} {len(EDGES)}" + (" 1" if weighted else "")]
+    for u in range(n):
+        if weighted:
+            lines.append(" ".join(f"{v + 1} 9" for v in adj[u]))
+        else:
+            lines.append(" ".join(str(v + 1) for v in adj[u]))
+    p = tmp_path / "g.graph"
+    p.write_text("\n".join(lines) + "\n")
+    hg = rg.HostGraph.read(str(p), rg.FMT_METIS, symmetric=False)
+    off, nb = _expect(orc, EDGES, False)
+    assert np.array_equal(hg.offsets(), off) and np.array_equal(hg.neighbors(), nb)
+
+
+@pytest.mark.parametrize("field,symmetry", [("pattern", "general"), ("real", "general"), ("integer", "symmetric")])
+def test_read_matrix_market(rg, orc, tmp_path, field, symmetry):
+    rows = [f"%%MatrixMarket matrix coordinate {field} {symmetry}", "% comment", f"5 5 {len(EDGES)}"]
+    for u, v in EDGES:
+        rows.append(f"{u + 1} {v + 1}" + ("" if field == "pattern" else " 2.5"))
+    p = tmp_path / "g.mtx"
+    p.write_text("\n".join(rows) + "\n")
+    hg = rg.HostGraph.read(str(p), rg.FMT_MTX, symmetric=False)
+    edges = EDGES + ([(v, u) for u, v in EDGES] if symmetry == "symmetric" else [])
+    off, nb = _expect(orc, edges, False)
+    assert np.array_equal(hg.offsets(), off) and np.array_equal(hg.neighbors(), nb)
+
+
+def test_read_formats_reject_bad_headers(rg, tmp_path):
+    bad = {"a.mtx": "%%MatrixMarket matrix array real general\n3 3\n",
+           "b.mtx": "%%MatrixMarket matrix coordinate complex general\n1 1 1\n1 1 1 1\n",
+           "c.mtx": "%%MatrixMarket matrix coordinate pattern general\n3 4 1\n1 2\n",
+           "d.graph": "3 2 11\n2\n1\n\n",
+           "e.wel": "0 1\n"}
+    fmts = {"mtx": rg.FMT_MTX, "graph": rg.FMT_METIS, "wel": rg.FMT_WEL}
+    for name, text in bad.items():
+        p = tmp_path / name
+        p.write_text(text)
+        with pytest.raises(rg.RGError):
+            rg.HostGraph.read(str(p), fmts[name.split(".")[1]], symmetric=False)
+
+
+def test_format_of_suffix(rg):
+    f = rg.lib.rg_format_of
+    f.restype = rg.C.c_int
+    f.argtypes = [rg.C.c_char_p]
+    assert [f(x) for x in (b"g.el", b"g.wel", b"g.sg", b"g.gr", b"g.graph", b"g.mtx", b"g.adj")] == \
+        [rg.FMT_EL, rg.FMT_WEL, rg.FMT_SG, rg.FMT_GR, rg.FMT_METIS, rg.FMT_MTX, rg.FMT_ADJ]

@@ -260,6 +260,30 @@ int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o,
 int rg_bc(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o,
           double* deps_out, int64_t* combined_out, double* ms_out);
 
+/* Summary of one single-source BC query: rounds = forward rounds (the
+ * "Combined Iters" of rg_bc with that one source), reached = vertices the
+ * source reaches, wsum = sum over the finite dependencies dep[v] of
+ * dep[v] * w(v) with w(v) = (rg_level_hash(v, 0xffffffff) >> 11) * 2^-53,
+ * nonfinite = count of non-finite dependencies (the unreached vertices are
+ * NaN in BC-manyiter-v2.C:387-389's final step).  No counterpart in the
+ * reference, which prints no per-source result. */
+typedef struct rg_bc_summary {
+  int64_t rounds, reached;
+  double wsum;
+  int64_t nonfinite;
+} rg_bc_summary;
+
+/* Batched BC: 1..64 independent single-source queries answered together;
+ * row i of deps_out (nullable, nsources * n doubles, source-major) equals
+ * rg_bc(g, &sources[i], 1, ...)'s deps_out within FP64 rounding, out[i] its
+ * summary.  The forward levels run on the bit-parallel BFS of rg_bfs_batch
+ * (source i owns bit i); path counts and dependencies are summed per vertex
+ * by one wavefront with a lane per source, with no atomics but for
+ * vertices of degree above 4096.  Sources deeper than 63 levels are
+ * answered by single-source runs.  No counterpart in the reference. */
+int rg_bc_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
+                rg_bc_summary* out, double* ms_out);
+
 /* Triangle counting (PNO GAP/src/tc_local.cc:48-86): u > v > w ordered merge
  * counting over a symmetric graph with sorted lists; total and per-vertex
  * counts (each triangle adds 1 to all three corners).  per_vertex nullable. */
@@ -318,16 +342,18 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
  * execute them concurrently, and with batching on, every free slot takes
  * all BFS queries that have arrived (up to max_batch) as one bit-parallel
  * batch (rg_bfs_batch) -- continuous batching: queries that arrive while
- * others run join the next batch on the next free slot.  BC and PageRank
- * queries run one per slot.  The number of slots is capped by free HBM
- * (less hbm_reserve) over a per-slot estimate.  one_at_a_time = 1 runs the
- * same trace on one slot, one query at a time, BFS on the single-source
- * engine (rg_bfs_query): the baseline.
+ * others run join the next batch on the next free slot; BC queries the
+ * same way in batches of up to 64 (rg_bc_batch).  PageRank queries run one
+ * per slot.  The number of slots is capped by free HBM (less hbm_reserve)
+ * over a per-slot estimate.  one_at_a_time = 1 runs the same trace on one
+ * slot, one query at a time, BFS on the single-source engine
+ * (rg_bfs_query), BC on rg_bc with one source: the baseline.
  * Answers: BFS an rg_bfs_summary (equal to rg_bfs_query's); BC (one source,
  * BC-manyiter-v2.C semantics for that source, orientation restored after)
  * and PageRank (`iters` iterations, PageRank.C) a summary of the result
- * array: wsum = sum over finite x[v] of x[v] * w(v), w(v) in [0,1) a fixed
- * hash of v, and the count of non-finite entries. */
+ * array: wsum = sum over finite x[v] of x[v] * w(v), w(v) =
+ * (rg_level_hash(v, 0xffffffff) >> 11) * 2^-53, and the count of
+ * non-finite entries (rg_bc_summary's fields). */
 enum { RG_Q_BFS = 0, RG_Q_BC = 1, RG_Q_PAGERANK = 2 };
 typedef struct rg_query {
   int kind;

@@ -68,6 +68,8 @@ struct rg_ctx {
   int64_t bc_bits_zero = 0;
   // Multi-source BFS state (opaque, msbfs.hip), grown on demand.
   void* ms = nullptr;
+  // Batched BC state (opaque, bcbatch.hip), grown on demand.
+  void* bcb = nullptr;
   // Device array holding the last PageRank ranks / BC dependencies computed
   // on this context (n doubles; valid until the next call on the context).
   const double* last_result = nullptr;
@@ -141,5 +143,6 @@ void blocked_free(void* p);
 
 // Multi-source BFS state of a context (msbfs.hip).
 void ms_state_free(void* p);
+void bcb_state_free(void* p);
 
 }  // namespace rg

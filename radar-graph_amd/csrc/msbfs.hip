@@ -49,6 +49,7 @@ constexpr int kMsMaxSources = 256;
 constexpr int kMsLevelsPerSync = 4;  // levels launched between host checks
 
 enum { MS_SPARSE = 0, MS_DENSE = 1, MS_DONE = 2 };
+static_assert(MS_DONE == kMsDone, "level hooks test the mode against kMsDone");
 
 // Device control block of one batch.
 struct MsCtl {
@@ -551,7 +552,7 @@ int ms_prepare(rg_ctx* ctx, MsState& S, int64_t n, int W, int64_t nunits) {
 
 template <int W>
 int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32_t* sources, int nsrc,
-           int64_t threshold, int32_t* levels_dev, int32_t* radii_dev, int64_t* levels_run) {
+           int64_t threshold, int32_t* levels_dev, int32_t* radii_dev, int64_t* levels_run, MsLevelHook* hook) {
   rg_ctx* ctx = g->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = g->n;
@@ -630,6 +631,11 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
       hipLaunchKernelGGL((k_ms_dense<W, kMsVB, kMsThreads>), gdense, kMsThreads, 0, st, a, B->nunits);
       if (B->nmulti)
         hipLaunchKernelGGL((k_ms_merge<W, kMsVB, kMsThreads>), (int)(B->nmulti * PER), kMsThreads, 0, st, a);
+      if (hook) {
+        const MsLevelView view{&a.ctl->mode, &a.ctl->level, &a.ctl->ncount, a.list_nxt,
+                               a.front_cur,  a.front_nxt,   a.active_cur};
+        RG_TRY(hook->level(view));
+      }
       hipLaunchKernelGGL(k_ms_clear<W>, gclr, 256, 0, st, a, nwords);
       hipLaunchKernelGGL(k_ms_level_end, 1, 256, 0, st, a, reached, depth);
       RG_HIP(hipGetLastError());
@@ -665,7 +671,7 @@ namespace {
 // Runs one batch; per-source summaries into out (nullable), level arrays or
 // radii into the device buffers given (nullable).
 int batch_run(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_dev,
-              int32_t* radii_dev, rg_bfs_summary* out, double* ms_out) {
+              int32_t* radii_dev, rg_bfs_summary* out, double* ms_out, MsLevelHook* hook = nullptr) {
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
   const int W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
@@ -689,9 +695,9 @@ int batch_run(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
   RG_TRY(ms_prepare(ctx, *S, n, W, B->nunits));
   EventTimer tm(ctx->stream);
   int64_t steps = 0;
-  int rc = W == 1   ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps)
-           : W == 2 ? ms_run<2>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps)
-                    : ms_run<4>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps);
+  int rc = W == 1   ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook)
+           : W == 2 ? ms_run<2>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook)
+                    : ms_run<4>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook);
   if (rc != RG_OK) return rc;
   double ms = tm.stop_ms();
   if (out) {
@@ -731,6 +737,12 @@ int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
   if (levels_out)
     RG_TRY(ctx_copy(ctx, levels_out, levels_dev, sizeof(int32_t) * (size_t)nsrc * n, hipMemcpyDeviceToHost));
   return RG_OK;
+}
+
+int bfs_batch_hooked(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, MsLevelHook* hook,
+                     rg_bfs_summary* out) {
+  if (nsrc < 1 || nsrc > 64) return set_error(RG_EINVAL, "hooked batch: between 1 and 64 sources");
+  return batch_run(g, sources, nsrc, o, nullptr, nullptr, out, nullptr, hook);
 }
 
 // Radii (Radii.C:98-157) as one bit-parallel batch: radii[v] = the last

@@ -21,6 +21,11 @@ __host__ __device__ inline uint64_t level_hash(uint32_t v, uint32_t level) {
   return z ^ (z >> 31);
 }
 
+// Weight of vertex v in a result summary: 53 hash bits as a double in [0, 1).
+__host__ __device__ inline double summary_weight(uint32_t v) {
+  return (double)(level_hash(v, 0xffffffffu) >> 11) * 0x1p-53;
+}
+
 int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
              int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev,
              uint64_t* checksum_out);
@@ -28,6 +33,40 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
 int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
               rg_bfs_summary* out, double* ms_out);
 
+// Per-level view of a running bit-parallel batch (device pointers), handed
+// to a level hook after a level's kernels are launched and before its
+// frontier words are cleared.  Levels are launched ahead of the host's
+// checks, so the hook's kernels read *mode on the device and do nothing
+// once it is kMsDone.
+constexpr int64_t kMsDone = 2;
+struct MsLevelView {
+  const int64_t* mode;
+  const int64_t* level;                  // level of the current frontier
+  const int64_t* ncount;                 // size of the next frontier list
+  const uint32_t* list_nxt;              // vertices some source reaches first at level + 1
+  const unsigned long long* front_cur;   // per vertex: sources for which it is at level
+  const unsigned long long* front_nxt;   // per vertex: sources for which it is at level + 1
+  const unsigned long long* active_cur;  // bit per vertex: front_cur != 0
+};
+struct MsLevelHook {
+  virtual ~MsLevelHook() = default;
+  virtual int level(const MsLevelView& v) = 0;
+};
+
+// A one-word batch (1..64 sources) with a hook called at every level.
+int bfs_batch_hooked(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, MsLevelHook* hook,
+                     rg_bfs_summary* out);
+
+// Batched single-source BC (bcbatch.hip): up to 64 sources, each answered
+// as rg_bc with that one source.
+int bc_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_out,
+             rg_bc_summary* out, double* ms_out);
+// Device bytes one context holds for BC batches on an (n, m) graph.
+int64_t bc_batch_bytes(int64_t n, int64_t m);
+
+// Weighted sum of the finite entries of x (weights in [0, 1) from a fixed
+// hash of the vertex id) and the count of non-finite ones (serve.hip).
+int fp_summary(rg_ctx* ctx, const double* x, int64_t n, double* wsum, int64_t* nonfinite);
 int radii_batch(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o, int32_t* radii_out,
                 int64_t* rounds_out, double* ms_out);
 

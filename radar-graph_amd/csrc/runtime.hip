@@ -132,6 +132,8 @@ int rg_ctx_destroy(rg_ctx* ctx) {
   ctx->bc_dense.release();
   rg::ms_state_free(ctx->ms);
   ctx->ms = nullptr;
+  rg::bcb_state_free(ctx->bcb);
+  ctx->bcb = nullptr;
   for (auto& p : ctx->ev_pairs) {
     (void)hipEventDestroy(p.e[0]);
     (void)hipEventDestroy(p.e[1]);

@@ -1,9 +1,10 @@
 // Query serving over a resident graph: a request loop that admits BFS, BC
 // and PageRank queries by arrival time, runs them on several device
 // contexts ("slots", one stream each) that share the graph's arrays, and
-// coalesces the BFS queries that have arrived into bit-parallel batches
-// (continuous batching: a query that arrives while batches are running
-// joins the batch the next free slot starts).  The reference runs one
+// coalesces the BFS queries that have arrived into bit-parallel batches of
+// up to 256 and the BC queries into batches of up to 64 (continuous
+// batching: a query that arrives while batches are running joins the batch
+// the next free slot starts).  The reference runs one
 // Compute per process and has no such loop; the per-query semantics are its
 // apps' (BFS.C, BC-manyiter-v2.C for one source, PageRank.C).
 //
@@ -32,9 +33,8 @@
 
 namespace rg {
 
-namespace {
-
 constexpr int kSumBlocks = 512;
+constexpr int kBcBatch = 64;   // BC queries per batch (one word of source bits)
 
 // Weighted sum of the finite entries (weights w(v) in [0,1) from a fixed
 // hash of v) and count of the non-finite ones: per-block partial sums in a
@@ -50,8 +50,7 @@ __global__ __launch_bounds__(256) void k_fp_summary(const double* x, int64_t n, 
     if (!isfinite(xv)) {
       nf++;
     } else {
-      const double w = (double)(level_hash((uint32_t)v, 0xffffffffu) >> 11) * 0x1p-53;
-      s += xv * w;
+      s += xv * summary_weight((uint32_t)v);
     }
   }
   red[threadIdx.x] = s;
@@ -84,6 +83,8 @@ int fp_summary(rg_ctx* ctx, const double* x, int64_t n, double* wsum, int64_t* n
   *nonfinite = (int64_t)c;
   return RG_OK;
 }
+
+namespace {
 
 struct Slot {
   rg_ctx* ctx = nullptr;
@@ -132,6 +133,8 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
     if (best >= 0) {
       size_t take = 1;
       if (best == RG_Q_BFS && !c.one_at_a_time) take = std::min<size_t>(S.pending[best].size(), (size_t)c.max_batch);
+      if (best == RG_Q_BC && !c.one_at_a_time)
+        take = std::min<size_t>(S.pending[best].size(), (size_t)std::min(c.max_batch, kBcBatch));
       for (size_t i = 0; i < take; ++i) {
         work.push_back(S.pending[best].front());
         S.pending[best].pop_front();
@@ -171,6 +174,16 @@ int run_work(Slot& sl, const rg_serve_config& c, const rg_query* q, int kind, co
       std::vector<uint32_t> src(k);
       for (size_t i = 0; i < k; ++i) src[i] = q[work[i]].source;
       RG_TRY(bfs_batch(g, src.data(), (int)k, &qo, nullptr, sum.data(), &ms));
+    }
+  } else if (kind == RG_Q_BC && !c.one_at_a_time) {
+    std::vector<uint32_t> src(k);
+    for (size_t i = 0; i < k; ++i) src[i] = q[work[i]].source;
+    std::vector<rg_bc_summary> bs(k);
+    RG_TRY(bc_batch(g, src.data(), (int)k, &qo, nullptr, bs.data(), &ms));
+    for (size_t i = 0; i < k; ++i) {
+      iters[i] = bs[i].rounds;
+      wsum[i] = bs[i].wsum;
+      nonfin[i] = bs[i].nonfinite;
     }
   } else if (kind == RG_Q_BC) {
     const bool tr = g->transposed;
@@ -243,7 +256,7 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
     any_bc |= queries[i].kind == RG_Q_BC;
   }
   const bool bc_blocked = c.opts.variant == RG_RADAR || c.opts.variant == RG_BLOCKED;
-  if ((any_bfs && !c.one_at_a_time) || (any_bc && bc_blocked)) RG_TRY(rg::bfs_batch_prepare(g));
+  if (((any_bfs || any_bc) && !c.one_at_a_time) || (any_bc && bc_blocked)) RG_TRY(rg::bfs_batch_prepare(g));
   if (any_pr && c.opts.variant == RG_BLOCKED) RG_TRY(rg::pagerank_prepare(g, &c.opts));
   if (any_pr && c.opts.variant != RG_BLOCKED && c.opts.variant != RG_PULL)
     RG_TRY(rg::static_heavy_list(g, c.opts.heavy_degree > 0 ? c.opts.heavy_degree : 1024));
@@ -255,7 +268,8 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
   RG_HIP(hipMemGetInfo(&free_b, &total_b));
   const int64_t n = g->n, m = g->m;
   const int64_t bfs_b = rg::bfs_batch_bytes(n, m, c.max_batch);
-  const int64_t bc_b = n * 30 + (m / 2048 + m / 1024 + 64) * 8;
+  const int64_t bc_b = std::max<int64_t>(n * 30 + (m / 2048 + m / 1024 + 64) * 8,
+                                         c.one_at_a_time ? 0 : rg::bc_batch_bytes(n, m));
   const int64_t pr_b = n * 24 + ((n >> 14) + m / rg::kUnitEdges + 2) * (int64_t)(16384 * 8);
   const int64_t slot_bytes = bfs_b + std::max(bc_b, pr_b);
   const int64_t budget = (int64_t)free_b - c.hbm_reserve;

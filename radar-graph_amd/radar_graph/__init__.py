@@ -65,6 +65,13 @@ class BfsSummary(C.Structure):
         return (self.iters, self.reached, self.edges, self.checksum)
 
 
+class BcSummary(C.Structure):
+    """rg_bc_summary: forward rounds, vertices reached, weighted sum of the
+    finite dependencies, count of the non-finite ones."""
+    _fields_ = [("rounds", C.c_int64), ("reached", C.c_int64), ("wsum", C.c_double),
+                ("nonfinite", C.c_int64)]
+
+
 Q_BFS, Q_BC, Q_PAGERANK = range(3)
 
 
@@ -141,6 +148,7 @@ _sig("rg_serve", C.c_int, _vp, C.POINTER(ServeConfig), C.POINTER(Query), C.c_int
      C.POINTER(ServeStats))
 _sig("rg_radii", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _i32p, _i64p, _f64p)
 _sig("rg_bc", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _f64p, _i64p, _f64p)
+_sig("rg_bc_batch", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _f64p, C.POINTER(BcSummary), _f64p)
 
 _OPTIONAL = {}
 
@@ -235,6 +243,18 @@ def poisson_workload(hg: "HostGraph", count: int, rate_qps: float, mix=(0.8, 0.1
 
 def level_hash(v: int, level: int) -> int:
     return int(lib.rg_level_hash(v, level))
+
+
+def summary_weights(n: int) -> np.ndarray:
+    """w(v) = (rg_level_hash(v, 0xffffffff) >> 11) * 2^-53 for v < n: the
+    weights of the BC / PageRank result summaries (rg_bc_summary.wsum)."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(0xffffffff) << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+        z = z + np.uint64(0x9e3779b97f4a7c15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
 
 
 def radii_starts(n: int, k: int = 64):
@@ -498,6 +518,18 @@ class DeviceGraph:
         _check(lib.rg_bc(self._h, _ptr(sources, _u32p), len(sources), C.byref(o), _ptr(deps, _f64p),
                          C.byref(comb), C.byref(ms)))
         return deps, comb.value, ms.value
+
+    def bc_batch(self, sources, o: Opts = None, want_deps=False):
+        """Batch of 1..64 single-source BC queries: (list of BcSummary,
+        dependencies [nsources, n] float64 or None, ms)."""
+        o = o or opts()
+        src = np.ascontiguousarray(sources, np.uint32)
+        out = (BcSummary * len(src))()
+        deps = np.empty((len(src), self.n), np.float64) if want_deps else None
+        ms = C.c_double()
+        _check(lib.rg_bc_batch(self._h, _ptr(src, _u32p), len(src), C.byref(o),
+                               _ptr(deps, _f64p) if want_deps else None, out, C.byref(ms)))
+        return list(out), deps, ms.value
 
     def autotune(self, app="pagerank", sources=None, base: Opts = None):
         """Fastest engine configuration for `app` on this graph: (Opts, ms)."""

@@ -105,3 +105,21 @@ def test_admission_by_memory(rg, ctx, graphs):
     assert st1.slots == 1
     with pytest.raises(rg.RGError):
         dg.serve(q, rg.serve_config(slots=4, hbm_reserve=int(free - st.slot_bytes // 2)))
+
+
+def test_bc_queries_batched(rg, ctx, graphs):
+    # BC queries that have arrived are answered in batches of up to 64; each
+    # answer equals the same query answered alone.
+    hg = graphs("rmat12d")
+    dg = rg.DeviceGraph(ctx, hg)
+    q = rg.poisson_workload(hg, 150, 0.0, mix=(0.2, 0.8, 0.0), seed=12)
+    a, _ = dg.serve(q, rg.serve_config(slots=2))
+    b, _ = dg.serve(q, rg.serve_config(one_at_a_time=True))
+    sizes = [r.batch_size for r in a if r.kind == rg.Q_BC]
+    assert max(sizes) > 1 and max(sizes) <= 64
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert x.kind == y.kind
+        if x.kind == rg.Q_BC:
+            assert (x.iters, x.nonfinite) == (y.iters, y.nonfinite) and close(x.wsum, y.wsum), f"query {i}"
+        else:
+            assert x.bfs.astuple() == y.bfs.astuple()

@@ -139,7 +139,19 @@ def radii_bc_study(rg, args):
             g.bc(ss, o)
             _, _, ms = g.bc(ss, o)
             row["bc_ms"][name] = round(ms, 3)
-        log(f"  radii ms {row['radii_ms']}  bc ms {row['bc_ms']}")
+        # 64 BC queries: one batch (rg_bc_batch) vs one single-source RADAR
+        # run per source on the same degree-sorted graph.
+        src64 = ids[hg.pick_bc_sources(64)]
+        o = rg.opts("radar", hubs=args.hubs)
+        sg.bc_batch(src64, o)
+        _, _, bms = sg.bc_batch(src64, o)
+        single = 0.0
+        for s in src64:
+            single += sg.bc([s], o)[2]
+        row["bc_64_queries"] = {"batch_ms": round(bms, 3), "single_ms": round(single, 3),
+                                "speedup": round(single / bms, 2)}
+        log(f"  radii ms {row['radii_ms']}  bc ms {row['bc_ms']}  64 BC queries: batch {bms:.1f} ms vs "
+            f"{single:.1f} ms one at a time")
         out[f"{kind}{args.radii_scale}"] = row
         for x in (sg, dg):
             x.close()
@@ -174,7 +186,7 @@ def tc_study(rg, args):
 
 def serve_bench(rg, args):
     """Serving over a resident degree-sorted symmetric R-MAT graph: seeded
-    query traces (BFS only; 80/15/5 BFS/BC/PageRank-10) run saturated (all
+    query traces (BFS only; BC only; 80/15/5 BFS/BC/PageRank-10) run saturated (all
     queries at t = 0) and as Poisson arrivals at 80% of the one-at-a-time
     capacity, batched (continuous batching, 2 slots) vs one at a time."""
     t0 = time.time()
@@ -197,7 +209,8 @@ def serve_bench(rg, args):
                 "mean_batch": round(st.mean_batch, 1), "slots": st.slots}
 
     out = {"graph": f"rmat{args.serve_scale} symmetric, degree-sorted", "n": n, "m": m}
-    for name, mix, count in [("bfs", (1, 0, 0), args.serve_queries), ("mix_80_15_5", (0.8, 0.15, 0.05), 256)]:
+    for name, mix, count in [("bfs", (1, 0, 0), args.serve_queries), ("bc", (0, 1, 0), 128),
+                             ("mix_80_15_5", (0.8, 0.15, 0.05), 256)]:
         q = trace(count, 0.0, mix, 27491095)
         _, one = g.serve(q, rg.serve_config(one_at_a_time=True))
         _, bat = g.serve(q, rg.serve_config(slots=2))

@@ -343,7 +343,8 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
  * all BFS queries that have arrived (up to max_batch) as one bit-parallel
  * batch (rg_bfs_batch) -- continuous batching: queries that arrive while
  * others run join the next batch on the next free slot; BC queries the
- * same way in batches of up to 64 (rg_bc_batch).  PageRank queries run one
+ * same way in batches of up to 64 (rg_bc_batch) once three or more wait,
+ * one per slot on the single-source engine otherwise.  PageRank queries run one
  * per slot.  The number of slots is capped by free HBM (less hbm_reserve)
  * over a per-slot estimate.  one_at_a_time = 1 runs the same trace on one
  * slot, one query at a time, BFS on the single-source engine

@@ -19,14 +19,18 @@
 // v at level r from source s,
 //     D_s(v) = 1/sigma_s(v) + sum of D_s(w) over the out-neighbours w at level r + 1 from s
 // and the dependency is (D - 1/sigma) / (1/sigma) (BC-manyiter-v2.C:387-389),
-// NaN for the vertices the source does not reach (1/0 there).  D replaces
-// sigma in place: a (vertex, source) entry is read at one level and written
-// at another, so no pass reads an entry that it writes.
+// NaN for the vertices the source does not reach (1/0 there).  The same
+// gather as the forward one, over out-edges: a lane per edge tests the
+// head's level-(r + 1) word against v's level-r word, then a lane per source
+// reads the heads' rows.  D replaces sigma in place: a (vertex, source)
+// entry is read at one level and written at another, so no pass reads an
+// entry that it writes.
 //
-// Per vertex: sigma / D [64] f64, level [64] u8 (0xFF = unreached) and a
-// 64-bit mask of the levels at which some source reaches it.  A vertex of
-// degree above kBcbHeavy has its edges split into chunks of kBcbChunk, one
-// queue item each, whose sums meet by FP64 atomics.
+// Per vertex: sigma / D [64] f64, and per level L a 64-bit word of the
+// sources for which the vertex is at level L (the bit-parallel BFS's
+// frontier word of that level, kept).  A vertex of degree above kBcbHeavy
+// has its edges split into chunks of kBcbChunk, one queue item each, whose
+// sums meet by FP64 atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,9 +46,15 @@
 namespace rg {
 
 constexpr int kBcbMaxSources = 64;
-constexpr int64_t kBcbMaxLevel = 63;   // bits of the level mask
+constexpr int kBcbMaxLevels = 64;      // levels 0..63 kept; deeper sources take single-source runs
+constexpr int kBcbLevelsInit = 32;     // level words allocated at first (grown on demand)
 constexpr uint64_t kBcbHeavy = 4096;   // degree above which edges are split into chunks
 constexpr uint64_t kBcbChunk = 4096;
+constexpr int kBcbQ = 4;               // hits whose rows are loaded together
+// Vertices per wave in the light kernels.  One: the gathers are bound by
+// the bytes of the rows they read, and more vertices per wave (more load
+// chains in flight, more registers) measured slower at 2 and 4.
+constexpr int kBcbV = 1;
 constexpr int kBcbThreads = 256;
 constexpr int kBcbWaves = kBcbThreads / kWave;
 
@@ -54,22 +64,20 @@ struct BcbCtl {
   int64_t items;     // heavy chunk items queued at this level
   int64_t hverts;    // heavy vertices of this backward level
   int64_t count;     // vertices of this backward level
-  int64_t overflow;  // a source reached a level beyond kBcbMaxLevel
+  int64_t overflow;  // a source reached a level beyond the level words allocated
 };
 
 struct BcbArgs {
   BcbCtl* ctl;
   int64_t n;
   int nsrc;
+  int lcap;              // levels with words allocated
   double* sig;           // [n][64] path counts, then D
-  uint8_t* lev;          // [n][64] level from each source, 0xFF = unreached
-  ull* lvmask;           // [n] levels at which some source reaches the vertex
+  ull* lw;               // [lcap][n] per level: sources for which the vertex is at that level
   ull* items;            // heavy chunk queue: (vertex or heavy slot) << 32 | chunk
   uint32_t* hv;          // backward: heavy vertices of the level
   double* acc_h;         // backward: [heavy slot][64] sums
   uint32_t* list;        // backward: vertices of the level
-  ull* bm_cur;           // backward: bitmap of the level's vertices (written by select)
-  const ull* bm_child;   // backward: bitmap of level r + 1
   double* wsum;          // [64] weighted sums of the dependencies
   ull* nonfin;           // [64] non-finite dependencies among the reached vertices
   double* deps;          // optional [nsrc][n]
@@ -87,80 +95,79 @@ __device__ __forceinline__ int64_t bcb_wave() {
 }
 __device__ __forceinline__ int64_t bcb_waves() { return (int64_t)gridDim.x * kBcbWaves; }
 
-// Sum of sig[u][lane] over the in-edges [e0, e1) whose tail u is on the
-// current frontier of source `lane`, for the lanes in nn (the sources that
-// reach the head first at the next level).  Called by whole waves.
-__device__ __forceinline__ double fwd_sum(const uint32_t* in_nbr, uint64_t e0, uint64_t e1, ull nn,
-                                          const ull* active, const ull* front, const double* sig, int lane) {
-  double acc = 0.0;
-  for (uint64_t e = e0; e < e1; e += kWave) {
-    const uint64_t k = e + lane;
-    uint32_t u = 0;
-    ull f = 0;
-    if (k < e1) {
-      u = in_nbr[k];
-      if ((active[u >> 6] >> (u & 63)) & 1) f = front[u] & nn;
+// For V edge ranges at once (V vertices, or V parts of one): acc[t] +=
+// rows[u][lane] over the edges [e0[t], e1[t]) of the neighbour list nbr
+// whose neighbour u has bit `lane` of mask[t] in its word words[u].
+// FILTER: test the bitmap `active` (words[u] != 0) before loading the word.
+// Per step a lane takes one edge of each range (V independent load chains
+// in flight); the hits of each range are then taken four at a time with a
+// lane per source, each reading its own word of the neighbour's 64-word
+// row.  Called by whole waves with wave-uniform ranges and masks.
+template <int V, bool FILTER>
+__device__ __forceinline__ void gather_rows(const uint32_t* nbr, const uint64_t* e0, const uint64_t* e1,
+                                            const ull* mask, const ull* active, const ull* words,
+                                            const double* rows, int lane, double* acc) {
+  uint64_t len = 0;
+#pragma unroll
+  for (int t = 0; t < V; ++t) len = std::max<uint64_t>(len, e1[t] - e0[t]);
+  for (uint64_t o = 0; o < len; o += kWave) {
+    uint32_t u[V];
+    bool in[V];
+#pragma unroll
+    for (int t = 0; t < V; ++t) {
+      const uint64_t k = e0[t] + o + lane;
+      in[t] = k < e1[t];
+      u[t] = in[t] ? nbr[k] : 0u;
     }
-    ull hit = __ballot(f != 0);
-    while (hit) {
-      // Four tails per step, their row loads issued together.
-      uint32_t uu[4];
-      bool on[4];
+    if (FILTER) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const bool have = hit != 0;
-        const int j = have ? __ffsll((long long)hit) - 1 : 0;
-        hit &= hit - 1;
-        const ull fj = bcb_rl64(f, j);
-        uu[t] = bcb_rl32(u, j);
-        on[t] = have && ((fj >> lane) & 1);
+      for (int t = 0; t < V; ++t) in[t] = in[t] && ((active[u[t] >> 6] >> (u[t] & 63)) & 1);
+    }
+    ull f[V];
+#pragma unroll
+    for (int t = 0; t < V; ++t) f[t] = in[t] ? (words[u[t]] & mask[t]) : 0ull;
+#pragma unroll
+    for (int t = 0; t < V; ++t) {
+      ull hit = __ballot(f[t] != 0);
+      while (hit) {
+        uint32_t uu[kBcbQ];
+        bool on[kBcbQ];
+#pragma unroll
+        for (int q = 0; q < kBcbQ; ++q) {
+          const bool have = hit != 0;
+          const int j = have ? __ffsll((long long)hit) - 1 : 0;
+          hit &= hit - 1;
+          const ull fj = bcb_rl64(f[t], j);
+          uu[q] = bcb_rl32(u[t], j);
+          on[q] = have && ((fj >> lane) & 1);
+        }
+        double x[kBcbQ];
+#pragma unroll
+        for (int q = 0; q < kBcbQ; ++q) x[q] = on[q] ? rows[(size_t)uu[q] * 64 + lane] : 0.0;
+#pragma unroll
+        for (int q = 0; q < kBcbQ; ++q) acc[t] += x[q];
       }
-      double x[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) x[t] = on[t] ? sig[(size_t)uu[t] * 64 + lane] : 0.0;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) acc += x[t];
     }
   }
-  return acc;
 }
 
-// Sum of D[w][lane] over the out-edges [e0, e1) whose head w is at level rc
-// from source `lane`, for the lanes where `mine`.  bm: vertices at level rc
-// from some source.  Called by whole waves.
-__device__ __forceinline__ double bwd_sum(const uint32_t* nbr, uint64_t e0, uint64_t e1, uint32_t rc, const ull* bm,
-                                          const uint8_t* lev, const double* D, bool mine, int lane) {
-  double acc = 0.0;
-  for (uint64_t e = e0; e < e1; e += kWave) {
-    const uint64_t k = e + lane;
-    uint32_t w = 0;
-    bool c = false;
-    if (k < e1) {
-      w = nbr[k];
-      c = (bm[w >> 6] >> (w & 63)) & 1;
-    }
-    ull hit = __ballot(c);
-    while (hit) {
-      uint32_t ww[4];
-      bool have[4];
+// A heavy chunk [lo, hi) as four ranges gathered together.
+template <bool FILTER>
+__device__ __forceinline__ double gather_chunk(const uint32_t* nbr, uint64_t lo, uint64_t hi, ull mask,
+                                               const ull* active, const ull* words, const double* rows, int lane) {
+  constexpr uint64_t kPart = kBcbChunk / 4;
+  uint64_t e0[4], e1[4];
+  ull mk[4];
+  double acc[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        have[t] = hit != 0;
-        const int j = have[t] ? __ffsll((long long)hit) - 1 : 0;
-        hit &= hit - 1;
-        ww[t] = bcb_rl32(w, j);
-      }
-      uint32_t l[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) l[t] = (have[t] && mine) ? lev[(size_t)ww[t] * 64 + lane] : 0xFFu;
-      double x[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) x[t] = l[t] == rc ? D[(size_t)ww[t] * 64 + lane] : 0.0;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) acc += x[t];
-    }
+  for (int t = 0; t < 4; ++t) {
+    e0[t] = std::min<uint64_t>(lo + t * kPart, hi);
+    e1[t] = std::min<uint64_t>(lo + (t + 1) * kPart, hi);
+    mk[t] = mask;
+    acc[t] = 0.0;
   }
-  return acc;
+  gather_rows<4, FILTER>(nbr, e0, e1, mk, active, words, rows, lane, acc);
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -172,44 +179,62 @@ __global__ void k_bcb_reset(BcbCtl* c) {
   c->count = 0;
 }
 
-// Wave per vertex of the next frontier: its level bytes and level-mask bit,
-// then its path counts (vertices of in-degree above kBcbHeavy: zeroed here,
-// summed by k_bcf_heavy's chunk items).
-__global__ __launch_bounds__(kBcbThreads) void k_bcf_light(BcbArgs b, MsLevelView v, const uint64_t* in_off,
-                                                           const uint32_t* in_nbr) {
+// The next level's words: lw[L + 1] = front_nxt (every vertex written).
+__global__ __launch_bounds__(kBcbThreads) void k_bcf_words(BcbArgs b, MsLevelView v) {
   if (*v.mode == kMsDone) return;
-  const int64_t count = *v.ncount;
   const int64_t L1 = *v.level + 1;
-  if (L1 > kBcbMaxLevel) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && count > 0) b.ctl->overflow = 1;
+  if (L1 >= b.lcap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *v.ncount > 0) b.ctl->overflow = 1;
     return;
   }
+  ull* dst = b.lw + L1 * b.n;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < b.n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = v.front_nxt[i];
+}
+
+// kBcbV vertices of the next frontier per wave: their path counts
+// (vertices of in-degree above kBcbHeavy: zeroed here, summed by
+// k_bcf_heavy's chunk items).
+
+__global__ __launch_bounds__(kBcbThreads) void k_bcf_light(BcbArgs b, MsLevelView v, const uint64_t* in_off,
+                                                           const uint32_t* in_nbr) {
+  if (*v.mode == kMsDone || *v.level + 1 >= b.lcap) return;
+  const int64_t count = *v.ncount;
   const int lane = (int)__lane_id();
-  for (int64_t i = bcb_wave(); i < count; i += bcb_waves()) {
-    const uint32_t d = v.list_nxt[i];
-    const ull nn = v.front_nxt[d];
-    const bool mine = (nn >> lane) & 1;
-    const size_t row = (size_t)d * 64 + lane;
-    if (mine) b.lev[row] = (uint8_t)L1;
-    if (lane == 0) b.lvmask[d] |= 1ull << L1;
-    const uint64_t e0 = in_off[d], e1 = in_off[d + 1];
-    if (e1 - e0 > kBcbHeavy) {
-      if (mine) b.sig[row] = 0.0;
-      const uint64_t nch = (e1 - e0 + kBcbChunk - 1) / kBcbChunk;
-      ull base = 0;
-      if (lane == 0) base = atomicAdd((ull*)&b.ctl->items, (ull)nch);
-      base = bcb_rl64(base, 0);
-      for (uint64_t c = lane; c < nch; c += kWave) b.items[base + c] = ((ull)d << 32) | c;
-      continue;
+  for (int64_t i0 = bcb_wave() * kBcbV; i0 < count; i0 += bcb_waves() * kBcbV) {
+    uint32_t d[kBcbV];
+    ull nn[kBcbV];
+    uint64_t e0[kBcbV], e1[kBcbV];
+    double acc[kBcbV];
+#pragma unroll
+    for (int t = 0; t < kBcbV; ++t) {
+      const bool valid = i0 + t < count;
+      d[t] = valid ? v.list_nxt[i0 + t] : 0u;
+      nn[t] = valid ? v.front_nxt[d[t]] : 0ull;
+      e0[t] = valid ? in_off[d[t]] : 0;
+      e1[t] = valid ? in_off[d[t] + 1] : 0;
+      acc[t] = 0.0;
+      if (e1[t] - e0[t] > kBcbHeavy) {
+        if ((nn[t] >> lane) & 1) b.sig[(size_t)d[t] * 64 + lane] = 0.0;
+        const uint64_t nch = (e1[t] - e0[t] + kBcbChunk - 1) / kBcbChunk;
+        ull base = 0;
+        if (lane == 0) base = atomicAdd((ull*)&b.ctl->items, (ull)nch);
+        base = bcb_rl64(base, 0);
+        for (uint64_t c = lane; c < nch; c += kWave) b.items[base + c] = ((ull)d[t] << 32) | c;
+        nn[t] = 0;          // nothing more to write for this vertex here
+        e1[t] = e0[t];
+      }
     }
-    const double s = fwd_sum(in_nbr, e0, e1, nn, v.active_cur, v.front_cur, b.sig, lane);
-    if (mine) b.sig[row] = s;
+    gather_rows<kBcbV, true>(in_nbr, e0, e1, nn, v.active_cur, v.front_cur, b.sig, lane, acc);
+#pragma unroll
+    for (int t = 0; t < kBcbV; ++t)
+      if ((nn[t] >> lane) & 1) b.sig[(size_t)d[t] * 64 + lane] = acc[t];
   }
 }
 
 __global__ __launch_bounds__(kBcbThreads) void k_bcf_heavy(BcbArgs b, MsLevelView v, const uint64_t* in_off,
                                                            const uint32_t* in_nbr) {
-  if (*v.mode == kMsDone || *v.level + 1 > kBcbMaxLevel) return;
+  if (*v.mode == kMsDone || *v.level + 1 >= b.lcap) return;
   const int64_t count = b.ctl->items;
   const int lane = (int)__lane_id();
   for (int64_t i = bcb_wave(); i < count; i += bcb_waves()) {
@@ -219,18 +244,17 @@ __global__ __launch_bounds__(kBcbThreads) void k_bcf_heavy(BcbArgs b, MsLevelVie
     const ull nn = v.front_nxt[d];
     const uint64_t lo = in_off[d] + c * kBcbChunk;
     const uint64_t hi = std::min<uint64_t>(in_off[d + 1], lo + kBcbChunk);
-    const double s = fwd_sum(in_nbr, lo, hi, nn, v.active_cur, v.front_cur, b.sig, lane);
+    const double s = gather_chunk<true>(in_nbr, lo, hi, nn, v.active_cur, v.front_cur, b.sig, lane);
     if (((nn >> lane) & 1) && s != 0.0) unsafeAtomicAdd(b.sig + (size_t)d * 64 + lane, s);
   }
 }
 
-// Sources at level 0; per-source sums cleared.
+// Sources at level 0 (lw[0] cleared before); per-source sums cleared.
 __global__ void k_bcb_init(BcbArgs b, const uint32_t* sources) {
   for (int s = threadIdx.x; s < b.nsrc; s += blockDim.x) {
     const uint32_t v = sources[s];
     b.sig[(size_t)v * 64 + s] = 1.0;
-    b.lev[(size_t)v * 64 + s] = 0;
-    atomicOr(b.lvmask + v, 1ull);
+    atomicOr(b.lw + v, 1ull << s);
   }
   for (int s = threadIdx.x; s < kBcbMaxSources; s += blockDim.x) {
     b.wsum[s] = 0.0;
@@ -247,20 +271,30 @@ __global__ void k_bcb_init(BcbArgs b, const uint32_t* sources) {
 // ---------------------------------------------------------------------------
 // Backward.
 
-// Vertices at level r from some source: the list and the bitmap.
+// Vertices at level r from some source.  Each wave scans a contiguous
+// range of 64-vertex words twice: counting, one reservation in the list,
+// writing.
 __global__ __launch_bounds__(kBcbThreads) void k_bcb_select(BcbArgs b, uint32_t r) {
   const int lane = (int)__lane_id();
-  for (int64_t base = bcb_wave() * kWave; base < b.n; base += bcb_waves() * kWave) {
-    const int64_t v = base + lane;
-    const bool at = v < b.n && ((b.lvmask[v] >> r) & 1);
+  const ull* lr = b.lw + (int64_t)r * b.n;
+  const int64_t nwords = (b.n + kWave - 1) / kWave;
+  const int64_t per = (nwords + bcb_waves() - 1) / bcb_waves();
+  const int64_t w0 = bcb_wave() * per, w1 = std::min<int64_t>(w0 + per, nwords);
+  ull total = 0;
+  for (int64_t w = w0; w < w1; ++w) {
+    const int64_t v = w * kWave + lane;
+    total += __popcll(__ballot(v < b.n && lr[v] != 0));
+  }
+  if (!total) return;
+  ull pos = 0;
+  if (lane == 0) pos = atomicAdd((ull*)&b.ctl->count, total);
+  pos = bcb_rl64(pos, 0);
+  for (int64_t w = w0; w < w1; ++w) {
+    const int64_t v = w * kWave + lane;
+    const bool at = v < b.n && lr[v] != 0;
     const ull word = __ballot(at);
-    if (lane == 0) b.bm_cur[base >> 6] = word;
-    if (word) {
-      ull pos = 0;
-      if (lane == 0) pos = atomicAdd((ull*)&b.ctl->count, (ull)__popcll(word));
-      pos = bcb_rl64(pos, 0);
-      if (at) b.list[pos + __popcll(word & ((1ull << lane) - 1))] = (uint32_t)v;
-    }
+    if (at) b.list[pos + __popcll(word & ((1ull << lane) - 1))] = (uint32_t)v;
+    pos += __popcll(word);
   }
 }
 
@@ -289,9 +323,10 @@ __device__ __forceinline__ void bcb_flush(const BcbArgs& b, double* s_w, ull* s_
   }
 }
 
-// Wave per vertex of level r.  CHILDREN = false at the deepest level (no
-// vertex has children there).  Vertices of out-degree above kBcbHeavy take a
-// heavy slot (its row of sums zeroed here) and queue their chunks.
+// kBcbV vertices of level r per wave.  CHILDREN = false at the deepest
+// level (no vertex has children there).  Vertices of out-degree above
+// kBcbHeavy take a heavy slot (its row of sums zeroed here) and queue their
+// chunks.
 template <bool CHILDREN>
 __global__ __launch_bounds__(kBcbThreads) void k_bcb_light(BcbArgs b, const uint64_t* off, const uint32_t* nbr,
                                                            uint32_t r) {
@@ -303,17 +338,26 @@ __global__ __launch_bounds__(kBcbThreads) void k_bcb_light(BcbArgs b, const uint
   }
   __syncthreads();
   const int lane = (int)__lane_id();
+  const ull* lr = b.lw + (int64_t)r * b.n;
+  const ull* lc = lr + b.n;   // level r + 1 (read only when CHILDREN)
   double wl = 0.0;
   ull nfl = 0;
   const int64_t count = b.ctl->count;
-  for (int64_t i = bcb_wave(); i < count; i += bcb_waves()) {
-    const uint32_t v = b.list[i];
-    const bool mine = b.lev[(size_t)v * 64 + lane] == r;
-    double acc = 0.0;
-    if (CHILDREN) {
-      const uint64_t e0 = off[v], e1 = off[v + 1];
-      if (e1 - e0 > kBcbHeavy) {
-        const uint64_t nch = (e1 - e0 + kBcbChunk - 1) / kBcbChunk;
+  for (int64_t i0 = bcb_wave() * kBcbV; i0 < count; i0 += bcb_waves() * kBcbV) {
+    uint32_t v[kBcbV];
+    ull mask[kBcbV];
+    uint64_t e0[kBcbV], e1[kBcbV];
+    double acc[kBcbV];
+#pragma unroll
+    for (int t = 0; t < kBcbV; ++t) {
+      const bool valid = i0 + t < count;
+      v[t] = valid ? b.list[i0 + t] : 0u;
+      mask[t] = valid ? lr[v[t]] : 0ull;
+      e0[t] = valid && CHILDREN ? off[v[t]] : 0;
+      e1[t] = valid && CHILDREN ? off[v[t] + 1] : 0;
+      acc[t] = 0.0;
+      if (CHILDREN && e1[t] - e0[t] > kBcbHeavy) {
+        const uint64_t nch = (e1[t] - e0[t] + kBcbChunk - 1) / kBcbChunk;
         ull h = 0, base = 0;
         if (lane == 0) {
           h = atomicAdd((ull*)&b.ctl->hverts, 1ull);
@@ -321,14 +365,17 @@ __global__ __launch_bounds__(kBcbThreads) void k_bcb_light(BcbArgs b, const uint
         }
         h = bcb_rl64(h, 0);
         base = bcb_rl64(base, 0);
-        if (lane == 0) b.hv[h] = v;
+        if (lane == 0) b.hv[h] = v[t];
         b.acc_h[h * 64 + lane] = 0.0;
         for (uint64_t c = lane; c < nch; c += kWave) b.items[base + c] = (h << 32) | c;
-        continue;
+        mask[t] = 0;        // finished by k_bcb_heavy_fin
+        e1[t] = e0[t];
       }
-      acc = bwd_sum(nbr, e0, e1, r + 1, b.bm_child, b.lev, b.sig, mine, lane);
     }
-    if (mine) bwd_finish(b, v, lane, acc, wl, nfl);
+    if (CHILDREN) gather_rows<kBcbV, false>(nbr, e0, e1, mask, nullptr, lc, b.sig, lane, acc);
+#pragma unroll
+    for (int t = 0; t < kBcbV; ++t)
+      if ((mask[t] >> lane) & 1) bwd_finish(b, v[t], lane, acc[t], wl, nfl);
   }
   bcb_flush(b, s_w, s_nf, wl, nfl);
 }
@@ -337,16 +384,17 @@ __global__ __launch_bounds__(kBcbThreads) void k_bcb_heavy(BcbArgs b, const uint
                                                            uint32_t r) {
   const int64_t count = b.ctl->items;
   const int lane = (int)__lane_id();
+  const ull* lr = b.lw + (int64_t)r * b.n;
   for (int64_t i = bcb_wave(); i < count; i += bcb_waves()) {
     const ull it = b.items[i];
     const ull h = it >> 32;
     const uint64_t c = it & 0xffffffffull;
     const uint32_t v = b.hv[h];
-    const bool mine = b.lev[(size_t)v * 64 + lane] == r;
+    const ull mask = lr[v];
     const uint64_t lo = off[v] + c * kBcbChunk;
     const uint64_t hi = std::min<uint64_t>(off[v + 1], lo + kBcbChunk);
-    const double s = bwd_sum(nbr, lo, hi, r + 1, b.bm_child, b.lev, b.sig, mine, lane);
-    if (mine && s != 0.0) unsafeAtomicAdd(b.acc_h + h * 64 + lane, s);
+    const double s = gather_chunk<false>(nbr, lo, hi, mask, nullptr, lr + b.n, b.sig, lane);
+    if (((mask >> lane) & 1) && s != 0.0) unsafeAtomicAdd(b.acc_h + h * 64 + lane, s);
   }
 }
 
@@ -359,12 +407,13 @@ __global__ __launch_bounds__(kBcbThreads) void k_bcb_heavy_fin(BcbArgs b, uint32
   }
   __syncthreads();
   const int lane = (int)__lane_id();
+  const ull* lr = b.lw + (int64_t)r * b.n;
   double wl = 0.0;
   ull nfl = 0;
   const int64_t count = b.ctl->hverts;
   for (int64_t h = bcb_wave(); h < count; h += bcb_waves()) {
     const uint32_t v = b.hv[h];
-    if (b.lev[(size_t)v * 64 + lane] == r) bwd_finish(b, v, lane, b.acc_h[h * 64 + lane], wl, nfl);
+    if ((lr[v] >> lane) & 1) bwd_finish(b, v, lane, b.acc_h[h * 64 + lane], wl, nfl);
   }
   bcb_flush(b, s_w, s_nf, wl, nfl);
 }
@@ -373,8 +422,9 @@ __global__ __launch_bounds__(kBcbThreads) void k_bcb_heavy_fin(BcbArgs b, uint32
 // Host side.
 
 struct BcbState {
-  DevBuf sig, lev, lvmask, items, hv, acc_h, list, bm[2], ctl, stats, srcs;
+  DevBuf sig, lw, items, hv, acc_h, list, ctl, stats, srcs;
   int64_t n = -1, m = -1;
+  int lcap = 0;
 };
 
 void bcb_state_free(void* p) { delete static_cast<BcbState*>(p); }
@@ -384,25 +434,26 @@ namespace {
 int64_t heavy_cap(int64_t m) { return m / (int64_t)kBcbHeavy + 1; }
 int64_t item_cap(int64_t m) { return 2 * heavy_cap(m) + 64; }
 
-int bcb_prepare(rg_ctx* ctx, BcbState& S, int64_t n, int64_t m) {
-  if (S.n == n && S.m == m) return RG_OK;
-  for (DevBuf* b : {&S.sig, &S.lev, &S.lvmask, &S.items, &S.hv, &S.acc_h, &S.list, &S.bm[0], &S.bm[1]}) b->release();
-  const size_t words = (size_t)(n / 64 + 1);
-  RG_TRY(S.sig.ensure(sizeof(double) * 64 * (size_t)n));
-  RG_TRY(S.lev.ensure((size_t)64 * n));
-  RG_TRY(S.lvmask.ensure(sizeof(ull) * (size_t)n));
-  RG_TRY(S.items.ensure(sizeof(ull) * (size_t)item_cap(m)));
-  RG_TRY(S.hv.ensure(sizeof(uint32_t) * (size_t)heavy_cap(m)));
-  RG_TRY(S.acc_h.ensure(sizeof(double) * 64 * (size_t)heavy_cap(m)));
-  RG_TRY(S.list.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(n, 1)));
-  RG_TRY(S.bm[0].ensure(sizeof(ull) * words));
-  RG_TRY(S.bm[1].ensure(sizeof(ull) * words));
-  RG_TRY(S.ctl.ensure(sizeof(BcbCtl)));
-  RG_TRY(S.stats.ensure(sizeof(double) * 2 * kBcbMaxSources));
-  RG_TRY(S.srcs.ensure(sizeof(uint32_t) * kBcbMaxSources));
-  S.n = n;
-  S.m = m;
-  (void)ctx;
+int bcb_prepare(BcbState& S, int64_t n, int64_t m, int lcap) {
+  if (S.n != n || S.m != m) {
+    for (DevBuf* b : {&S.sig, &S.lw, &S.items, &S.hv, &S.acc_h, &S.list}) b->release();
+    RG_TRY(S.sig.ensure(sizeof(double) * 64 * (size_t)n));
+    RG_TRY(S.items.ensure(sizeof(ull) * (size_t)item_cap(m)));
+    RG_TRY(S.hv.ensure(sizeof(uint32_t) * (size_t)heavy_cap(m)));
+    RG_TRY(S.acc_h.ensure(sizeof(double) * 64 * (size_t)heavy_cap(m)));
+    RG_TRY(S.list.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(n, 1)));
+    RG_TRY(S.ctl.ensure(sizeof(BcbCtl)));
+    RG_TRY(S.stats.ensure(sizeof(double) * 2 * kBcbMaxSources));
+    RG_TRY(S.srcs.ensure(sizeof(uint32_t) * kBcbMaxSources));
+    S.n = n;
+    S.m = m;
+    S.lcap = 0;
+  }
+  if (lcap > S.lcap) {
+    S.lw.release();
+    RG_TRY(S.lw.ensure(sizeof(ull) * (size_t)lcap * (size_t)std::max<int64_t>(n, 1)));
+    S.lcap = lcap;
+  }
   return RG_OK;
 }
 
@@ -411,10 +462,11 @@ struct ForwardHook : MsLevelHook {
   BcbArgs b;
   const uint64_t* in_off;
   const uint32_t* in_nbr;
-  int grid;
+  int grid, gwords;
   int level(const MsLevelView& v) override {
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(k_bcb_reset, 1, 1, 0, st, b.ctl);
+    hipLaunchKernelGGL(k_bcf_words, gwords, kBcbThreads, 0, st, b, v);
     hipLaunchKernelGGL(k_bcf_light, grid, kBcbThreads, 0, st, b, v, in_off, in_nbr);
     hipLaunchKernelGGL(k_bcf_heavy, grid, kBcbThreads, 0, st, b, v, in_off, in_nbr);
     RG_HIP(hipGetLastError());
@@ -422,7 +474,68 @@ struct ForwardHook : MsLevelHook {
   }
 };
 
-// Sources whose levels exceed the level mask: one single-source run each
+// One batch: forward on the bit-parallel BFS with the path-count hook, then
+// the backward levels.  *overflow: a source went deeper than the level
+// words allocated (nothing of the backward ran).
+int bcb_run(rg_graph* g, BcbState& S, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_dev,
+            std::vector<rg_bfs_summary>& fw, bool* overflow) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  hipStream_t st = ctx->stream;
+  const int grid = ctx->num_cus * 8;
+  if (deps_dev) RG_TRY(fill<double>(ctx, deps_dev, (int64_t)nsrc * n, std::nan("")));
+  RG_TRY(fill<ull>(ctx, S.lw.as<ull>(), n, 0ull));   // level 0's words
+  RG_HIP(hipMemcpyAsync(S.srcs.p, sources, sizeof(uint32_t) * nsrc, hipMemcpyHostToDevice, st));
+  BcbArgs b{};
+  b.ctl = S.ctl.as<BcbCtl>();
+  b.n = n;
+  b.nsrc = nsrc;
+  b.lcap = S.lcap;
+  b.sig = S.sig.as<double>();
+  b.lw = S.lw.as<ull>();
+  b.items = S.items.as<ull>();
+  b.hv = S.hv.as<uint32_t>();
+  b.acc_h = S.acc_h.as<double>();
+  b.list = S.list.as<uint32_t>();
+  b.wsum = S.stats.as<double>();
+  b.nonfin = reinterpret_cast<ull*>(S.stats.as<double>() + kBcbMaxSources);
+  b.deps = deps_dev;
+  hipLaunchKernelGGL(k_bcb_init, 1, 64, 0, st, b, S.srcs.as<uint32_t>());
+  RG_HIP(hipGetLastError());
+
+  ForwardHook hook;
+  hook.ctx = ctx;
+  hook.b = b;
+  hook.in_off = g->in_off;
+  hook.in_nbr = g->in_nbr;
+  hook.grid = grid;
+  hook.gwords = grid_for(ctx, n, kBcbThreads);
+  RG_TRY(bfs_batch_hooked(g, sources, nsrc, o, &hook, fw.data()));
+  int64_t ovf = 0;
+  RG_TRY(ctx_read_count(ctx, &b.ctl->overflow, &ovf, 1));
+  *overflow = ovf != 0;
+  if (*overflow) return RG_OK;
+  int64_t maxd = 0;
+  for (const auto& x : fw) maxd = std::max(maxd, x.iters - 1);
+
+  // Backward, deepest level first.
+  const int gsel = (int)std::max<int64_t>(1, std::min<int64_t>((n / 64 + kBcbWaves) / kBcbWaves, grid));
+  for (int64_t r = maxd; r >= 0; --r) {
+    hipLaunchKernelGGL(k_bcb_reset, 1, 1, 0, st, b.ctl);
+    hipLaunchKernelGGL(k_bcb_select, gsel, kBcbThreads, 0, st, b, (uint32_t)r);
+    if (r == maxd) {
+      hipLaunchKernelGGL(k_bcb_light<false>, grid, kBcbThreads, 0, st, b, g->off, g->nbr, (uint32_t)r);
+    } else {
+      hipLaunchKernelGGL(k_bcb_light<true>, grid, kBcbThreads, 0, st, b, g->off, g->nbr, (uint32_t)r);
+      hipLaunchKernelGGL(k_bcb_heavy, grid, kBcbThreads, 0, st, b, g->off, g->nbr, (uint32_t)r);
+      hipLaunchKernelGGL(k_bcb_heavy_fin, grid, kBcbThreads, 0, st, b, (uint32_t)r);
+    }
+    RG_HIP(hipGetLastError());
+  }
+  return RG_OK;
+}
+
+// Sources deeper than the level words can hold: one single-source run each
 // (rg_bc with that source, the orientation restored after it).
 int bc_batch_single(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_out,
                     const std::vector<rg_bfs_summary>& fw, rg_bc_summary* out) {
@@ -448,7 +561,7 @@ int bc_batch_single(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opt
 }  // namespace
 
 int64_t bc_batch_bytes(int64_t n, int64_t m) {
-  return n * (64 * 8 + 64 + 8 + 4) + 2 * (n / 64 + 1) * 8 + item_cap(m) * 8 + heavy_cap(m) * (4 + 64 * 8) + 4096;
+  return n * (64 * 8 + 8 * kBcbLevelsInit + 4) + item_cap(m) * 8 + heavy_cap(m) * (4 + 64 * 8) + 4096;
 }
 
 int bc_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_out,
@@ -466,71 +579,31 @@ int bc_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, d
     S = new BcbState;
     ctx->bcb = S;
   }
-  RG_TRY(bcb_prepare(ctx, *S, n, g->m));
+  RG_TRY(bcb_prepare(*S, n, g->m, kBcbLevelsInit));
   DevBuf dep;
   double* deps_dev = nullptr;
   if (deps_out) {
     RG_TRY(dep.ensure(sizeof(double) * (size_t)nsrc * n));
     deps_dev = dep.as<double>();
   }
-  hipStream_t st = ctx->stream;
-  const int grid = ctx->num_cus * 8;
-  EventTimer tm(st);
-  if (deps_dev) RG_TRY(fill<double>(ctx, deps_dev, (int64_t)nsrc * n, std::nan("")));
-  RG_TRY(fill<ull>(ctx, S->lev.as<ull>(), n * 8, ~0ull));
-  RG_TRY(fill<ull>(ctx, S->lvmask.as<ull>(), n, 0ull));
-  RG_HIP(hipMemcpyAsync(S->srcs.p, sources, sizeof(uint32_t) * nsrc, hipMemcpyHostToDevice, st));
-  BcbArgs b{};
-  b.ctl = S->ctl.as<BcbCtl>();
-  b.n = n;
-  b.nsrc = nsrc;
-  b.sig = S->sig.as<double>();
-  b.lev = S->lev.as<uint8_t>();
-  b.lvmask = S->lvmask.as<ull>();
-  b.items = S->items.as<ull>();
-  b.hv = S->hv.as<uint32_t>();
-  b.acc_h = S->acc_h.as<double>();
-  b.list = S->list.as<uint32_t>();
-  b.wsum = S->stats.as<double>();
-  b.nonfin = reinterpret_cast<ull*>(S->stats.as<double>() + kBcbMaxSources);
-  b.deps = deps_dev;
-  hipLaunchKernelGGL(k_bcb_init, 1, 64, 0, st, b, S->srcs.as<uint32_t>());
-  RG_HIP(hipGetLastError());
-
-  // Forward: the batch BFS with the path-count hook at every level.
-  ForwardHook hook;
-  hook.ctx = ctx;
-  hook.b = b;
-  hook.in_off = g->in_off;
-  hook.in_nbr = g->in_nbr;
-  hook.grid = grid;
+  EventTimer tm(ctx->stream);
   std::vector<rg_bfs_summary> fw(nsrc);
-  RG_TRY(bfs_batch_hooked(g, sources, nsrc, o, &hook, fw.data()));
-  int64_t overflow = 0;
-  RG_TRY(ctx_read_count(ctx, &b.ctl->overflow, &overflow, 1));
+  bool overflow = false;
+  RG_TRY(bcb_run(g, *S, sources, nsrc, o, deps_dev, fw, &overflow));
   if (overflow) {
-    RG_TRY(bc_batch_single(g, sources, nsrc, o, deps_out, fw, out));
-    if (ms_out) *ms_out = tm.stop_ms();
-    return RG_OK;
-  }
-  int64_t maxd = 0;
-  for (const auto& x : fw) maxd = std::max(maxd, x.iters - 1);
-
-  // Backward, deepest level first.
-  const int gsel = (int)std::max<int64_t>(1, std::min<int64_t>((n / 64 + kBcbWaves) / kBcbWaves, grid));
-  for (int64_t r = maxd; r >= 0; --r) {
-    b.bm_cur = S->bm[r & 1].as<ull>();
-    b.bm_child = S->bm[(r + 1) & 1].as<ull>();
-    hipLaunchKernelGGL(k_bcb_reset, 1, 1, 0, st, b.ctl);
-    hipLaunchKernelGGL(k_bcb_select, gsel, kBcbThreads, 0, st, b, (uint32_t)r);
-    if (r == maxd) {
-      hipLaunchKernelGGL(k_bcb_light<false>, grid, kBcbThreads, 0, st, b, g->off, g->nbr, (uint32_t)r);
-    } else {
-      hipLaunchKernelGGL(k_bcb_light<true>, grid, kBcbThreads, 0, st, b, g->off, g->nbr, (uint32_t)r);
-      hipLaunchKernelGGL(k_bcb_heavy, grid, kBcbThreads, 0, st, b, g->off, g->nbr, (uint32_t)r);
-      hipLaunchKernelGGL(k_bcb_heavy_fin, grid, kBcbThreads, 0, st, b, (uint32_t)r);
+    // The forward told the depth: grow the level words and run again, or,
+    // beyond kBcbMaxLevels levels, answer each source on its own.
+    int64_t levels = 0;
+    for (const auto& x : fw) levels = std::max(levels, x.iters);
+    if (levels <= kBcbMaxLevels) {
+      RG_TRY(bcb_prepare(*S, n, g->m, (int)levels));
+      RG_TRY(bcb_run(g, *S, sources, nsrc, o, deps_dev, fw, &overflow));
     }
-    RG_HIP(hipGetLastError());
+    if (overflow) {
+      RG_TRY(bc_batch_single(g, sources, nsrc, o, deps_out, fw, out));
+      if (ms_out) *ms_out = tm.stop_ms();
+      return RG_OK;
+    }
   }
   std::vector<double> h(2 * kBcbMaxSources);
   RG_TRY(ctx_copy(ctx, h.data(), S->stats.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));

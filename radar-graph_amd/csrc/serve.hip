@@ -35,6 +35,10 @@ namespace rg {
 
 constexpr int kSumBlocks = 512;
 constexpr int kBcBatch = 64;   // BC queries per batch (one word of source bits)
+// Fewest waiting BC queries worth a batch: a batch of one or two costs more
+// than as many single-source runs (rg_bc_batch: ~1.7x one single-source run
+// for one source, ~2.3x for two, on an R-MAT of 2^22 vertices).
+constexpr int kBcBatchMin = 3;
 
 // Weighted sum of the finite entries (weights w(v) in [0,1) from a fixed
 // hash of v) and count of the non-finite ones: per-block partial sums in a
@@ -133,7 +137,9 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
     if (best >= 0) {
       size_t take = 1;
       if (best == RG_Q_BFS && !c.one_at_a_time) take = std::min<size_t>(S.pending[best].size(), (size_t)c.max_batch);
-      if (best == RG_Q_BC && !c.one_at_a_time)
+      // BC: a batch once kBcBatchMin queries wait (below that, single-source
+      // runs on free slots answer sooner), else one query.
+      if (best == RG_Q_BC && !c.one_at_a_time && S.pending[best].size() >= (size_t)kBcBatchMin)
         take = std::min<size_t>(S.pending[best].size(), (size_t)std::min(c.max_batch, kBcBatch));
       for (size_t i = 0; i < take; ++i) {
         work.push_back(S.pending[best].front());
@@ -175,7 +181,7 @@ int run_work(Slot& sl, const rg_serve_config& c, const rg_query* q, int kind, co
       for (size_t i = 0; i < k; ++i) src[i] = q[work[i]].source;
       RG_TRY(bfs_batch(g, src.data(), (int)k, &qo, nullptr, sum.data(), &ms));
     }
-  } else if (kind == RG_Q_BC && !c.one_at_a_time) {
+  } else if (kind == RG_Q_BC && k > 1) {
     std::vector<uint32_t> src(k);
     for (size_t i = 0; i < k; ++i) src[i] = q[work[i]].source;
     std::vector<rg_bc_summary> bs(k);

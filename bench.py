@@ -31,6 +31,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "radar-graph_amd"), os.path.join(ROOT, "oracle")]
 
@@ -48,13 +50,27 @@ def median_pr(g, it, o, reps=3):
 
 
 def bfs_gteps(g, srcs, o):
+    """GTEPS over the sources (scanned arcs m_scan / time) and the fraction
+    of the HBM roofline at BFS's algorithmic bytes, 8 m_scan + 24 n_reach
+    per source (4 B column + 4 B state per scanned arc; offset, parent and
+    queue entry per reached vertex)."""
     g.bfs(int(srcs[0]), o, want_parents=False)
-    te, tm = 0, 0.0
+    te, tr, tm = 0, 0, 0.0
     for s in srcs:
         r = g.bfs(int(s), o, want_parents=False)
         te += r.edges
+        tr += r.reached
         tm += r.ms
-    return {"gteps": round(te / tm / 1e6, 2), "ms_per_source": round(tm / len(srcs), 3)}
+    gbs = (8 * te + 24 * tr) / (tm * 1e-3) / 1e9
+    return {"gteps": round(te / tm / 1e6, 2), "ms_per_source": round(tm / len(srcs), 3),
+            "alg_gbs": round(gbs, 1), "roofline_frac": round(gbs / PEAK_HBM_GBS, 4)}
+
+
+def bc_bytes(summary):
+    """BC's algorithmic bytes for one source: forward and backward 24 B per
+    scanned arc each (4 B column + 4 B level + 16 B FP64 read-modify-write)
+    plus 28 B per reached vertex."""
+    return 48 * summary.edges + 28 * summary.reached
 
 
 def variants_table(rg, args):
@@ -139,17 +155,26 @@ def radii_bc_study(rg, args):
             g.bc(ss, o)
             _, _, ms = g.bc(ss, o)
             row["bc_ms"][name] = round(ms, 3)
+            if name == "push":
+                one, _ = dg.bfs_query(int(ss[0]))
+                row["bc_roofline_frac"] = {}
+            row["bc_roofline_frac"][name] = round(bc_bytes(one) / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
         # 64 BC queries: one batch (rg_bc_batch) vs one single-source RADAR
         # run per source on the same degree-sorted graph.
         src64 = ids[hg.pick_bc_sources(64)]
         o = rg.opts("radar", hubs=args.hubs)
         sg.bc_batch(src64, o)
         _, _, bms = sg.bc_batch(src64, o)
-        single = 0.0
+        single, eq = 0.0, 0
         for s in src64:
             single += sg.bc([s], o)[2]
+            eq += bc_bytes(sg.bfs_query(int(s))[0])
+        # Equivalent bandwidth: the single-source algorithmic bytes of the 64
+        # queries over the batch time (the batch shares its arc scans, so
+        # this can exceed the HBM peak).
         row["bc_64_queries"] = {"batch_ms": round(bms, 3), "single_ms": round(single, 3),
-                                "speedup": round(single / bms, 2)}
+                                "speedup": round(single / bms, 2),
+                                "equivalent_gbs": round(eq / (bms * 1e-3) / 1e9, 1)}
         log(f"  radii ms {row['radii_ms']}  bc ms {row['bc_ms']}  64 BC queries: batch {bms:.1f} ms vs "
             f"{single:.1f} ms one at a time")
         out[f"{kind}{args.radii_scale}"] = row
@@ -174,8 +199,21 @@ def tc_study(rg, args):
     reorder = (time.time() - t0) * 1e3
     sg.tc()
     total, ms = sg.tc()
+    # Algorithmic bytes: 8 (n + 1) + 4 m+ + 4 sum over oriented arcs (u, v),
+    # v < u, of d+(u) + d+(v), with d+(u) = |{v in N(u): v < u}|.
+    off, nbr = sg.download()
+    deg = np.diff(off.astype(np.int64))
+    u = np.repeat(np.arange(n, dtype=np.uint32), deg)
+    low = nbr < u
+    dplus = np.bincount(u[low], minlength=n).astype(np.int64)
+    mplus = int(low.sum())
+    scan = int(dplus[u[low]].sum() + dplus[nbr[low]].sum())
+    del off, nbr, u, low
+    tbytes = 8 * (n + 1) + 4 * mplus + 4 * scan
     out = {"graph": f"rmat{args.tc_scale} symmetric, degree-sorted", "n": n, "m": m, "triangles": total,
-           "count_ms": round(ms, 3), "reorder_ms": round(reorder, 2)}
+           "count_ms": round(ms, 3), "reorder_ms": round(reorder, 2),
+           "alg_bytes": tbytes, "compulsory_bytes": 8 * n + 4 * mplus,
+           "roofline_frac": round(tbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
     log(f"tc rmat{args.tc_scale}: {total} triangles in {ms:.1f} ms (reorder {reorder:.1f} ms)")
     for x in (sg, dg):
         x.close()
@@ -326,8 +364,11 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    # Whole iterations (edge and vertex phases) at 20 m + 48 n bytes each.
+    pr_frac = it * (20 * m + 48 * n) / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS
     pr = {"radar_tuned": {"ms": round(ms_step, 3), "gteps": round(value, 2), "engine": engine,
-                          "wg_size": tuned.wg_size, "unroll": tuned.unroll}}
+                          "wg_size": tuned.wg_size, "unroll": tuned.unroll,
+                          "iteration_roofline_frac": round(pr_frac, 4)}}
     bfs, batched = {}, None
     if not args.quick:
         ms = median_pr(dg, it, rg.opts("push"), reps=1)

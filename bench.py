@@ -352,11 +352,14 @@ def main():
     ctx.timing(False)
     ms_step = wall * 1e3 / args.steps
     value = args.steps * it * m / wall / 1e9
-    # Algorithmic bytes of one edge-phase launch (one PageRank iteration's
-    # push, the edge terms of the 20 m + 48 n per-iteration model): per arc
-    # 4 B column + 16 B read-modify-write of the FP64 accumulator, per vertex
-    # 8 B offset + 8 B contribution.
-    bytes_launch = 20 * m + 16 * n
+    # Algorithmic bytes of one timed launch: the blocked engine runs a whole
+    # iteration in it (k_blk_pr_it, plus k_blk_pr_merge for split hub
+    # blocks, timed together), the per-iteration 20 m + 48 n of the push
+    # model: per arc 4 B column + 16 B accumulator read-modify-write, per
+    # vertex 8 B offset + 8 B contribution + 8 B accumulator read + 8 B old
+    # rank + 8 B new rank + 8 B accumulator reset.  The atomic engine's
+    # timed launch is the edge phase only: 20 m + 16 n.
+    bytes_launch = 20 * m + (48 if engine == "blocked" else 16) * n
     avg_launch_ms = kms / max(launches, 1)
     achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
     traffic = None
@@ -464,7 +467,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "measured_stream_copy_gbs": round(stream_gbs, 1),
-                     "kernel": "k_blk_pr" if engine == "blocked" else "edge_light_kernel<PrOp,SRC_ALL,HUB_PREFIX>",
+                     "kernel": "k_blk_pr_it + k_blk_pr_merge" if engine == "blocked" else "edge_light_kernel<PrOp,SRC_ALL,HUB_PREFIX>",
                      "launches": launches, "avg_launch_ms": round(avg_launch_ms, 4),
                      "bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,

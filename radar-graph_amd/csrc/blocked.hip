@@ -98,19 +98,12 @@ __device__ __forceinline__ T ld_stream(const T* p) {
   return *p;
 }
 
-template <int VB, int THREADS, bool NT, int UNROLL>
-__global__ __launch_bounds__(THREADS) void k_blk_pr(const uint32_t* __restrict__ src,
-                                                    const uint16_t* __restrict__ vl,
-                                                    const uint64_t* __restrict__ unit_lo,
-                                                    const double* __restrict__ contrib,
-                                                    double* __restrict__ part, const int64_t* done) {
-  constexpr int BV = 1 << VB;
-  __shared__ double acc[BV];
-  if (*done) return;
-  const int64_t u = blockIdx.x;
-  for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
-  __syncthreads();
-  const uint64_t lo = unit_lo[u], hi = unit_lo[u + 1];
+// Sums contrib[src] into acc[local destination] over the unit's edges
+// [lo, hi) (LDS atomics; four edges per lane in flight with UNROLL = 4).
+template <int THREADS, bool NT, int UNROLL>
+__device__ __forceinline__ void blk_accumulate(const uint32_t* __restrict__ src, const uint16_t* __restrict__ vl,
+                                               uint64_t lo, uint64_t hi, const double* __restrict__ contrib,
+                                               double* acc) {
   uint64_t e = lo + threadIdx.x;
   if (UNROLL == 4) {
     for (; e + 3 * THREADS < hi; e += 4 * THREADS) {
@@ -134,9 +127,151 @@ __global__ __launch_bounds__(THREADS) void k_blk_pr(const uint32_t* __restrict__
     const uint32_t l = ld_stream<NT>(vl + e);
     atomicAdd(&acc[l], contrib[s]);
   }
+}
+
+template <int VB, int THREADS, bool NT, int UNROLL>
+__global__ __launch_bounds__(THREADS) void k_blk_pr(const uint32_t* __restrict__ src,
+                                                    const uint16_t* __restrict__ vl,
+                                                    const uint64_t* __restrict__ unit_lo,
+                                                    const double* __restrict__ contrib,
+                                                    double* __restrict__ part, const int64_t* done) {
+  constexpr int BV = 1 << VB;
+  __shared__ double acc[BV];
+  if (*done) return;
+  const int64_t u = blockIdx.x;
+  for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
+  __syncthreads();
+  blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
   __syncthreads();
   double* out = part + u * (int64_t)BV;
   for (int i = threadIdx.x; i < BV; i += THREADS) out[i] = acc[i];
+}
+
+// PR_Vertex_F (PageRank.C:55) for vertex v from its in-edge sum a, fused
+// with the next iteration's contribution: returns |old - new| (the L1 term).
+__device__ __forceinline__ double pr_finalize(int64_t v, double a, double* rank, const uint64_t* off,
+                                              double* contrib_next, double damping, double added) {
+  const double nw = __dadd_rn(__dmul_rn(damping, a), added);
+  const double d = fabs(rank[v] - nw);
+  rank[v] = nw;
+  const uint64_t deg = off[v + 1] - off[v];
+  contrib_next[v] = deg ? nw / (double)(uint32_t)deg : 0.0;
+  return d;
+}
+
+// Sum of x over the workgroup (THREADS a multiple of 64), valid in thread 0.
+template <int THREADS>
+__device__ __forceinline__ double wg_sum(double x) {
+  __shared__ double red[THREADS / kWave];
+  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  if (__lane_id() == 0) red[threadIdx.x / kWave] = x;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < THREADS / kWave; ++i) t += red[i];
+  return t;
+}
+
+// One whole PageRank iteration on the blocked layout: each unit sums its
+// in-edges into an LDS block; a unit that holds its block alone then applies
+// the vertex step to the block's vertices itself (new rank in place, L1
+// term, next contribution into the other contribution buffer); units of
+// split blocks write partials, which k_blk_pr_merge adds in unit order.
+// diff_part[u] = the unit's L1 terms (0 for split-block units).
+template <int VB, int THREADS, bool NT, int UNROLL>
+__global__ __launch_bounds__(THREADS) void k_blk_pr_it(const uint32_t* __restrict__ src,
+                                                       const uint16_t* __restrict__ vl,
+                                                       const uint64_t* __restrict__ unit_lo,
+                                                       const uint32_t* __restrict__ unit_block,
+                                                       const uint32_t* __restrict__ block_unit0,
+                                                       const double* __restrict__ contrib,
+                                                       double* __restrict__ contrib_next, double* rank,
+                                                       const uint64_t* __restrict__ off, int64_t n,
+                                                       double* __restrict__ part, double* diff_part,
+                                                       const int64_t* done, double damping, double added) {
+  constexpr int BV = 1 << VB;
+  __shared__ double acc[BV];
+  if (*done) return;
+  const int64_t u = blockIdx.x;
+  for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
+  __syncthreads();
+  blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
+  __syncthreads();
+  const uint32_t b = unit_block[u];
+  if (block_unit0[b + 1] - block_unit0[b] != 1) {
+    double* out = part + u * (int64_t)BV;
+    for (int i = threadIdx.x; i < BV; i += THREADS) out[i] = acc[i];
+    if (threadIdx.x == 0) diff_part[u] = 0.0;
+    return;
+  }
+  // The vertex step, four vertices per thread at a time: their loads are
+  // issued together (one LDS block per CU, so this tail is not hidden by
+  // another workgroup's edges).
+  constexpr int FU = 4;
+  const int64_t base = (int64_t)b << VB;
+  double diff = 0.0;
+  for (int i0 = threadIdx.x; i0 < BV; i0 += THREADS * FU) {
+    double r[FU];
+    uint64_t o0[FU], o1[FU];
+#pragma unroll
+    for (int k = 0; k < FU; ++k) {
+      const int i = i0 + k * THREADS;
+      const int64_t v = base + i;
+      if (i < BV && v < n) {
+        r[k] = rank[v];
+        o0[k] = off[v];
+        o1[k] = off[v + 1];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < FU; ++k) {
+      const int i = i0 + k * THREADS;
+      const int64_t v = base + i;
+      if (i < BV && v < n) {
+        const double nw = __dadd_rn(__dmul_rn(damping, 0.0 + acc[i]), added);
+        diff += fabs(r[k] - nw);
+        rank[v] = nw;
+        const uint64_t deg = o1[k] - o0[k];
+        contrib_next[v] = deg ? nw / (double)(uint32_t)deg : 0.0;
+      }
+    }
+  }
+  diff = wg_sum<THREADS>(diff);
+  if (threadIdx.x == 0) diff_part[u] = diff;
+}
+
+// Split blocks: a vertex's sum is its units' partials (hub blocks hold up to
+// ~10^3 units), added in a fixed order by eight interleaved accumulators so
+// that eight loads are in flight, then the vertex step.  Grid: nmulti *
+// (2^VB / 256) workgroups of 256; diff_part[workgroup] = its L1 terms.
+template <int VB>
+__global__ __launch_bounds__(256) void k_blk_pr_merge(const uint32_t* __restrict__ multi,
+                                                      const uint32_t* __restrict__ block_unit0,
+                                                      const double* __restrict__ part, double* rank,
+                                                      const uint64_t* __restrict__ off,
+                                                      double* __restrict__ contrib_next, int64_t n,
+                                                      double* diff_part, const int64_t* done, double damping,
+                                                      double added) {
+  constexpr int PER = (1 << VB) / 256;
+  if (*done) return;
+  const uint32_t b = multi[blockIdx.x / PER];
+  const int l = (int)(blockIdx.x % PER) * 256 + threadIdx.x;
+  const int64_t v = ((int64_t)b << VB) + l;
+  double diff = 0.0;
+  if (v < n) {
+    const uint32_t u1 = block_unit0[b + 1];
+    uint32_t u = block_unit0[b];
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (; u + 8 <= u1; u += 8) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] += part[((int64_t)(u + k) << VB) + l];
+    }
+    for (int k = 0; u < u1; ++u, ++k) a[k] += part[((int64_t)u << VB) + l];
+    const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    diff = pr_finalize(v, sum, rank, off, contrib_next, damping, added);
+  }
+  diff = wg_sum<256>(diff);
+  if (threadIdx.x == 0) diff_part[blockIdx.x] = diff;
 }
 
 int grid_n(rg_ctx* ctx, int64_t n) {
@@ -246,35 +381,6 @@ __global__ void k_pr_init_b(const uint64_t* off, int64_t n, double* rank, double
   }
 }
 
-__global__ __launch_bounds__(256) void k_pr_vertex_b(const uint64_t* off, int64_t n, double* rank,
-                                                     double* contrib, const double* part,
-                                                     const uint32_t* block_unit0, int vb, double damping,
-                                                     double added, double* partial, const int64_t* done) {
-  if (*done) return;
-  double diff = 0.0;
-  const int64_t lmask = (int64_t(1) << vb) - 1;
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = v >> vb;
-    const int64_t l = v & lmask;
-    double a = 0.0;
-    for (uint32_t u = block_unit0[b]; u < block_unit0[b + 1]; ++u) a += part[((int64_t)u << vb) + l];
-    double nw = __dadd_rn(__dmul_rn(damping, a), added);
-    diff += fabs(rank[v] - nw);
-    rank[v] = nw;
-    uint64_t deg = off[v + 1] - off[v];
-    contrib[v] = deg ? nw / (double)(uint32_t)deg : 0.0;
-  }
-  __shared__ double red[256];
-  red[threadIdx.x] = diff;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
-
 __global__ void k_pr_finish_b(const double* partial, int nparts, double* l1_hist, int64_t iter,
                               int64_t* done, double epsilon) {
   if (*done) return;
@@ -356,48 +462,63 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
   BlockedCSC* B = nullptr;
   RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B));
-  void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const double*, double*, const int64_t*);
+  void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*, const uint32_t*,
+               const double*, double*, double*, const uint64_t*, int64_t, double*, double*, const int64_t*,
+               double, double);
+  void (*merge)(const uint32_t*, const uint32_t*, const double*, double*, const uint64_t*, double*, int64_t,
+                double*, const int64_t*, double, double);
   const bool u4 = o && o->unroll == 4;
-#define RG_PICK(VB)                                                                   \
-  kern = threads == 512 ? (u4 ? k_blk_pr<VB, 512, true, 4> : k_blk_pr<VB, 512, true, 1>) \
-                        : (u4 ? k_blk_pr<VB, 1024, true, 4> : k_blk_pr<VB, 1024, true, 1>)
+#define RG_PICK(VB)                                                                                 \
+  do {                                                                                              \
+    kern = threads == 512 ? (u4 ? k_blk_pr_it<VB, 512, true, 4> : k_blk_pr_it<VB, 512, true, 1>)     \
+                          : (u4 ? k_blk_pr_it<VB, 1024, true, 4> : k_blk_pr_it<VB, 1024, true, 1>); \
+    merge = k_blk_pr_merge<VB>;                                                                     \
+  } while (0)
   if (vb == 14) RG_PICK(14);
   else if (vb == 13) RG_PICK(13);
   else if (vb == 12) RG_PICK(12);
   else if (vb == 11) RG_PICK(11);
   else RG_PICK(10);
 #undef RG_PICK
-  DevBuf &b_rank = ctx->scratch[0], &b_contrib = ctx->scratch[1], &b_part = ctx->scratch[3],
-         &b_unit = ctx->scratch[2];
+  DevBuf &b_rank = ctx->scratch[0], &b_contrib = ctx->scratch[1], &b_contrib2 = ctx->scratch[4],
+         &b_part = ctx->scratch[3], &b_unit = ctx->scratch[2];
   RG_TRY(b_rank.ensure(sizeof(double) * n));
   RG_TRY(b_contrib.ensure(sizeof(double) * n));
+  RG_TRY(b_contrib2.ensure(sizeof(double) * n));
   RG_TRY(b_unit.ensure(sizeof(double) * ((size_t)B->nunits << vb)));
   double* unit_part = b_unit.as<double>();
-  const int vgrid = std::min<int>(grid_n(ctx, n), 1024);
-  RG_TRY(b_part.ensure(sizeof(double) * (vgrid + std::max<int64_t>(maxiters, 1) + 8)));
+  // L1 terms: one per unit, then one per merge workgroup; then the history.
+  const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 256);
+  const int64_t nparts = B->nunits + mgrid;
+  RG_TRY(b_part.ensure(sizeof(double) * (nparts + std::max<int64_t>(maxiters, 1) + 8)));
   double* rank = b_rank.as<double>();
-  double* contrib = b_contrib.as<double>();
+  double* contrib[2] = {b_contrib.as<double>(), b_contrib2.as<double>()};
   double* partial = b_part.as<double>();
-  double* l1_hist = partial + vgrid;
+  double* l1_hist = partial + nparts;
   int64_t* done = ctx->counts.as<int64_t>() + 16;
   RG_HIP(hipMemsetAsync(done, 0, sizeof(int64_t), ctx->stream));
   const double damping = 0.85, epsilon = 0.0000001;
   const double one_over_n = 1 / (double)n;
   const double added = (1 - damping) * (1 / (double)n);
-  hipLaunchKernelGGL(k_pr_init_b, grid_n(ctx, n), 256, 0, ctx->stream, g->off, n, rank, contrib, one_over_n);
+  hipLaunchKernelGGL(k_pr_init_b, grid_n(ctx, n), 256, 0, ctx->stream, g->off, n, rank, contrib[0], one_over_n);
   RG_HIP(hipGetLastError());
   hipEvent_t t0, t1;
   RG_HIP(hipEventCreate(&t0));
   RG_HIP(hipEventCreate(&t1));
   RG_HIP(hipEventRecord(t0, ctx->stream));
+  // Iteration it reads contrib[(it - 1) & 1] and writes the other; once
+  // converged, the later iterations' kernels return at once (*done).
   for (int64_t it = 1; it <= maxiters; ++it) {
+    const double* cur = contrib[(it - 1) & 1];
+    double* nxt = contrib[it & 1];
     hipEvent_t* ev = timing_begin(ctx);
-    hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, contrib,
-                       unit_part, done);
+    hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, B->unit_block,
+                       B->block_unit0, cur, nxt, rank, g->off, n, unit_part, partial, done, damping, added);
+    if (mgrid)
+      hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, unit_part, rank, g->off,
+                         nxt, n, partial + B->nunits, done, damping, added);
     if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
-    hipLaunchKernelGGL(k_pr_vertex_b, vgrid, 256, 0, ctx->stream, g->off, n, rank, contrib, unit_part,
-                       B->block_unit0, vb, damping, added, partial, done);
-    hipLaunchKernelGGL(k_pr_finish_b, 1, 256, 0, ctx->stream, partial, vgrid, l1_hist, it, done, epsilon);
+    hipLaunchKernelGGL(k_pr_finish_b, 1, 256, 0, ctx->stream, partial, (int)nparts, l1_hist, it, done, epsilon);
     RG_HIP(hipGetLastError());
   }
   RG_HIP(hipEventRecord(t1, ctx->stream));

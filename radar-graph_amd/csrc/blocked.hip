@@ -381,21 +381,17 @@ __global__ void k_pr_init_b(const uint64_t* off, int64_t n, double* rank, double
   }
 }
 
-__global__ void k_pr_finish_b(const double* partial, int nparts, double* l1_hist, int64_t iter,
-                              int64_t* done, double epsilon) {
+// L1 norm of the iteration from the per-workgroup terms (one workgroup of
+// 1024 threads), its history entry, and the convergence flag.
+__global__ __launch_bounds__(1024) void k_pr_finish_b(const double* partial, int nparts, double* l1_hist,
+                                                      int64_t iter, int64_t* done, double epsilon) {
   if (*done) return;
-  __shared__ double red[256];
   double s = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-    __syncthreads();
-  }
+  s = wg_sum<1024>(s);
   if (threadIdx.x == 0) {
-    l1_hist[iter - 1] = red[0];
-    if (red[0] < epsilon) *done = iter;
+    l1_hist[iter - 1] = s;
+    if (s < epsilon) *done = iter;
   }
 }
 
@@ -518,7 +514,7 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
       hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, unit_part, rank, g->off,
                          nxt, n, partial + B->nunits, done, damping, added);
     if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
-    hipLaunchKernelGGL(k_pr_finish_b, 1, 256, 0, ctx->stream, partial, (int)nparts, l1_hist, it, done, epsilon);
+    hipLaunchKernelGGL(k_pr_finish_b, 1, 1024, 0, ctx->stream, partial, (int)nparts, l1_hist, it, done, epsilon);
     RG_HIP(hipGetLastError());
   }
   RG_HIP(hipEventRecord(t1, ctx->stream));

@@ -273,14 +273,16 @@ typedef struct rg_bc_summary {
   int64_t nonfinite;
 } rg_bc_summary;
 
-/* Batched BC: 1..64 independent single-source queries answered together;
- * row i of deps_out (nullable, nsources * n doubles, source-major) equals
+/* Batched BC: 1..256 independent single-source queries, answered in
+ * batches of 64 (in order; the last batch takes the remainder); row i of
+ * deps_out (nullable, nsources * n doubles, source-major) equals
  * rg_bc(g, &sources[i], 1, ...)'s deps_out within FP64 rounding, out[i] its
- * summary.  The forward levels run on the bit-parallel BFS of rg_bfs_batch
- * (source i owns bit i); path counts and dependencies are summed per vertex
- * by one wavefront with a lane per source, with no atomics but for
- * vertices of degree above 4096.  Sources deeper than 63 levels are
- * answered by single-source runs.  No counterpart in the reference. */
+ * summary.  A batch's forward levels run on the bit-parallel BFS of
+ * rg_bfs_batch (source i owns bit i); path counts and dependencies are
+ * summed per vertex by one wavefront with a lane per source, with no
+ * atomics but for vertices of degree above 4096.  A batch with a source
+ * deeper than 63 levels is answered by single-source runs.  No counterpart
+ * in the reference. */
 int rg_bc_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
                 rg_bc_summary* out, double* ms_out);
 

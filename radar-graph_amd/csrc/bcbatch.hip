@@ -45,7 +45,8 @@
 
 namespace rg {
 
-constexpr int kBcbMaxSources = 64;
+constexpr int kBcbMaxSources = 64;    // sources per batch (one word of source bits)
+constexpr int kBcbMaxBatch = 256;     // sources per call, in batches of 64
 constexpr int kBcbMaxLevels = 64;      // levels 0..63 kept; deeper sources take single-source runs
 constexpr int kBcbLevelsInit = 32;     // level words allocated at first (grown on demand)
 constexpr uint64_t kBcbHeavy = 4096;   // degree above which edges are split into chunks
@@ -564,11 +565,56 @@ int64_t bc_batch_bytes(int64_t n, int64_t m) {
   return n * (64 * 8 + 8 * kBcbLevelsInit + 4) + item_cap(m) * 8 + heavy_cap(m) * (4 + 64 * 8) + 4096;
 }
 
+namespace {
+
+// One batch of 1..64 sources; deps_dev (nullable, nsrc * n) on the device,
+// deps_host the caller's rows, which the single-source fallback writes
+// itself (*on_host set).
+int bc_batch64(rg_graph* g, BcbState& S, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_dev,
+               double* deps_host, rg_bc_summary* out, bool* on_host) {
+  *on_host = false;
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  RG_TRY(bcb_prepare(S, n, g->m, kBcbLevelsInit));
+  std::vector<rg_bfs_summary> fw(nsrc);
+  bool overflow = false;
+  RG_TRY(bcb_run(g, S, sources, nsrc, o, deps_dev, fw, &overflow));
+  if (overflow) {
+    // The forward told the depth: grow the level words and run again, or,
+    // beyond kBcbMaxLevels levels, answer each source on its own.
+    int64_t levels = 0;
+    for (const auto& x : fw) levels = std::max(levels, x.iters);
+    if (levels <= kBcbMaxLevels) {
+      RG_TRY(bcb_prepare(S, n, g->m, (int)levels));
+      RG_TRY(bcb_run(g, S, sources, nsrc, o, deps_dev, fw, &overflow));
+    }
+    if (overflow) {
+      *on_host = true;
+      return bc_batch_single(g, sources, nsrc, o, deps_host, fw, out);
+    }
+  }
+  std::vector<double> h(2 * kBcbMaxSources);
+  RG_TRY(ctx_copy(ctx, h.data(), S.stats.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  for (int s = 0; s < nsrc; ++s) {
+    uint64_t nf = 0;
+    std::memcpy(&nf, &h[kBcbMaxSources + s], sizeof(nf));
+    out[s].rounds = fw[s].iters;
+    out[s].reached = fw[s].reached;
+    out[s].wsum = h[s];
+    out[s].nonfinite = (int64_t)nf + (n - fw[s].reached);
+  }
+  return RG_OK;
+}
+
+}  // namespace
+
+// 1..256 sources: batches of 64 in order, the last one uneven (a batch
+// costs far less than 64 single runs at any size, so full batches first).
 int bc_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_out,
              rg_bc_summary* out, double* ms_out) {
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
-  if (nsrc < 1 || nsrc > kBcbMaxSources) return set_error(RG_EINVAL, "rg_bc_batch: between 1 and 64 sources");
+  if (nsrc < 1 || nsrc > kBcbMaxBatch) return set_error(RG_EINVAL, "rg_bc_batch: between 1 and 256 sources");
   if (!sources || !out) return set_error(RG_EINVAL, "rg_bc_batch: null argument");
   for (int i = 0; i < nsrc; ++i)
     if ((int64_t)sources[i] >= n) return set_error(RG_EINVAL, "rg_bc_batch: source out of range");
@@ -579,46 +625,25 @@ int bc_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, d
     S = new BcbState;
     ctx->bcb = S;
   }
-  RG_TRY(bcb_prepare(*S, n, g->m, kBcbLevelsInit));
+  const int first = std::min(nsrc, kBcbMaxSources);
   DevBuf dep;
   double* deps_dev = nullptr;
   if (deps_out) {
-    RG_TRY(dep.ensure(sizeof(double) * (size_t)nsrc * n));
+    RG_TRY(dep.ensure(sizeof(double) * (size_t)first * n));
     deps_dev = dep.as<double>();
   }
-  EventTimer tm(ctx->stream);
-  std::vector<rg_bfs_summary> fw(nsrc);
-  bool overflow = false;
-  RG_TRY(bcb_run(g, *S, sources, nsrc, o, deps_dev, fw, &overflow));
-  if (overflow) {
-    // The forward told the depth: grow the level words and run again, or,
-    // beyond kBcbMaxLevels levels, answer each source on its own.
-    int64_t levels = 0;
-    for (const auto& x : fw) levels = std::max(levels, x.iters);
-    if (levels <= kBcbMaxLevels) {
-      RG_TRY(bcb_prepare(*S, n, g->m, (int)levels));
-      RG_TRY(bcb_run(g, *S, sources, nsrc, o, deps_dev, fw, &overflow));
-    }
-    if (overflow) {
-      RG_TRY(bc_batch_single(g, sources, nsrc, o, deps_out, fw, out));
-      if (ms_out) *ms_out = tm.stop_ms();
-      return RG_OK;
-    }
-  }
-  std::vector<double> h(2 * kBcbMaxSources);
-  RG_TRY(ctx_copy(ctx, h.data(), S->stats.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-  const double ms = tm.stop_ms();
-  for (int s = 0; s < nsrc; ++s) {
-    uint64_t nf = 0;
-    std::memcpy(&nf, &h[kBcbMaxSources + s], sizeof(nf));
-    out[s].rounds = fw[s].iters;
-    out[s].reached = fw[s].reached;
-    out[s].wsum = h[s];
-    out[s].nonfinite = (int64_t)nf + (n - fw[s].reached);
+  double ms = 0.0;
+  for (int c0 = 0; c0 < nsrc; c0 += kBcbMaxSources) {
+    const int k = std::min(kBcbMaxSources, nsrc - c0);
+    double* host = deps_out ? deps_out + (size_t)c0 * n : nullptr;
+    EventTimer tm(ctx->stream);
+    bool on_host = false;
+    RG_TRY(bc_batch64(g, *S, sources + c0, k, o, deps_dev, host, out + c0, &on_host));
+    ms += tm.stop_ms();
+    if (deps_out && !on_host)
+      RG_TRY(ctx_copy(ctx, host, deps_dev, sizeof(double) * (size_t)k * n, hipMemcpyDeviceToHost));
   }
   if (ms_out) *ms_out = ms;
-  if (deps_out)
-    RG_TRY(ctx_copy(ctx, deps_out, deps_dev, sizeof(double) * (size_t)nsrc * n, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 

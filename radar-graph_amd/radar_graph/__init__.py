@@ -520,8 +520,8 @@ class DeviceGraph:
         return deps, comb.value, ms.value
 
     def bc_batch(self, sources, o: Opts = None, want_deps=False):
-        """Batch of 1..64 single-source BC queries: (list of BcSummary,
-        dependencies [nsources, n] float64 or None, ms)."""
+        """1..256 single-source BC queries in batches of 64: (list of
+        BcSummary, dependencies [nsources, n] float64 or None, ms)."""
         o = o or opts()
         src = np.ascontiguousarray(sources, np.uint32)
         out = (BcSummary * len(src))()

@@ -1,4 +1,5 @@
-"""GPU tests of the batched single-source BC (rg_bc_batch): every row of
+"""GPU tests of the batched single-source BC (rg_bc_batch, 1..256 sources
+in batches of 64): every row of
 the batch's dependencies equals the oracle's single-source BC for that
 source within 1e-10 relative L-inf (NaN at exactly the unreached vertices),
 and every summary equals the one computed from the oracle's array (rounds,
@@ -103,6 +104,27 @@ def test_bc_batch_deep_levels(rg, orc, ctx, length):
     check(rg, orc, hg, dg, np.array([0, length // 2, length - 1], np.uint32))
 
 
+def test_bc_batch_deep_then_regrown_batch(rg, orc, ctx):
+    # 70 sources on a path of 80 vertices: the first batch of 64 holds a
+    # source 79 levels deep (single-source runs write its rows), the second
+    # batch 41 levels deep (level words grown past the first 32, run again).
+    length = 80
+    a = np.arange(length - 1, dtype=np.uint32)
+    hg = rg.HostGraph.from_edges(length, a, a + 1, symmetric=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    srcs = np.concatenate([np.array([0], np.uint32), np.arange(10, 73, dtype=np.uint32),
+                           np.full(6, 40, np.uint32)])
+    check(rg, orc, hg, dg, srcs)
+
+
+@pytest.mark.parametrize("k", [65, 130, 256])
+def test_bc_batch_uneven_batches(rg, orc, ctx, graphs, k):
+    # More than 64 sources: batches of 64 in order, the last one uneven.
+    hg = graphs("rmat10s")
+    dg = rg.DeviceGraph(ctx, hg)
+    check(rg, orc, hg, dg, pick(hg.n, k, 40 + k))
+
+
 def test_bc_batch_bigger_graph(rg, orc, ctx, graphs):
     hg = graphs("rmat16s")
     dg = rg.DeviceGraph(ctx, hg)
@@ -125,7 +147,7 @@ def test_bc_batch_errors(rg, ctx, graphs):
     hg = graphs("rmat10s")
     dg = rg.DeviceGraph(ctx, hg)
     with pytest.raises(rg.RGError):
-        dg.bc_batch(np.arange(65, dtype=np.uint32))
+        dg.bc_batch(np.arange(257, dtype=np.uint32))
     with pytest.raises(rg.RGError):
         dg.bc_batch(np.array([hg.n], np.uint32))
     with pytest.raises(rg.RGError):

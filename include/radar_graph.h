@@ -106,6 +106,7 @@ typedef struct rg_ctx rg_ctx;
 int rg_device_count(int* count);
 /* One context = one device + one HIP stream + preallocated work pools. */
 int rg_ctx_create(int device, rg_ctx** out);
+/* Free every rg_graph bound to a context before destroying the context. */
 int rg_ctx_destroy(rg_ctx* ctx);
 int rg_ctx_sync(rg_ctx* ctx);
 /* Instrumentation (the reference times whole Compute calls with its Timer,
@@ -122,6 +123,14 @@ int rg_ctx_timing(rg_ctx* ctx, int enable);
  * counters.  No counterpart in the reference. */
 int rg_device_copy_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, double* gbps, double* ms_per_copy);
 int rg_ctx_kernel_time(rg_ctx* ctx, int64_t* launches, double* total_ms);
+/* Iteration loops as HIP graphs (on by default): the blocked PageRank
+ * engine captures its whole iteration loop once per context and replays it
+ * with one launch while the graph, the iteration count and the context's
+ * buffers stay the same (a serving slot answering repeated PageRank
+ * queries); captured again when they change.  Not used while per-launch
+ * timing (rg_ctx_timing) is on.  rg_ctx_graphs(ctx, 0) launches every
+ * kernel of every iteration instead.  No counterpart in the reference. */
+int rg_ctx_graphs(rg_ctx* ctx, int enable);
 
 /* ---- device graphs ------------------------------------------------------- */
 typedef struct rg_graph rg_graph;

@@ -382,16 +382,20 @@ __global__ void k_pr_init_b(const uint64_t* off, int64_t n, double* rank, double
 }
 
 // L1 norm of the iteration from the per-workgroup terms (one workgroup of
-// 1024 threads), its history entry, and the convergence flag.
+// 1024 threads), its history entry, and the convergence flag.  The
+// iteration number is counted on the device (*iter), so a captured loop
+// holds no per-iteration argument.
 __global__ __launch_bounds__(1024) void k_pr_finish_b(const double* partial, int nparts, double* l1_hist,
-                                                      int64_t iter, int64_t* done, double epsilon) {
+                                                      int64_t* iter, int64_t* done, double epsilon) {
   if (*done) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
   s = wg_sum<1024>(s);
   if (threadIdx.x == 0) {
-    l1_hist[iter - 1] = s;
-    if (s < epsilon) *done = iter;
+    const int64_t it = *iter + 1;
+    *iter = it;
+    l1_hist[it - 1] = s;
+    if (s < epsilon) *done = it;
   }
 }
 
@@ -492,7 +496,9 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   double* partial = b_part.as<double>();
   double* l1_hist = partial + nparts;
   int64_t* done = ctx->counts.as<int64_t>() + 16;
+  int64_t* iter_dev = ctx->counts.as<int64_t>() + 18;
   RG_HIP(hipMemsetAsync(done, 0, sizeof(int64_t), ctx->stream));
+  RG_HIP(hipMemsetAsync(iter_dev, 0, sizeof(int64_t), ctx->stream));
   const double damping = 0.85, epsilon = 0.0000001;
   const double one_over_n = 1 / (double)n;
   const double added = (1 - damping) * (1 / (double)n);
@@ -504,18 +510,52 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   RG_HIP(hipEventRecord(t0, ctx->stream));
   // Iteration it reads contrib[(it - 1) & 1] and writes the other; once
   // converged, the later iterations' kernels return at once (*done).
-  for (int64_t it = 1; it <= maxiters; ++it) {
-    const double* cur = contrib[(it - 1) & 1];
-    double* nxt = contrib[it & 1];
-    hipEvent_t* ev = timing_begin(ctx);
-    hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, B->unit_block,
-                       B->block_unit0, cur, nxt, rank, g->off, n, unit_part, partial, done, damping, added);
-    if (mgrid)
-      hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, unit_part, rank, g->off,
-                         nxt, n, partial + B->nunits, done, damping, added);
-    if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
-    hipLaunchKernelGGL(k_pr_finish_b, 1, 1024, 0, ctx->stream, partial, (int)nparts, l1_hist, it, done, epsilon);
-    RG_HIP(hipGetLastError());
+  auto iterations = [&](bool timed) -> int {
+    for (int64_t it = 1; it <= maxiters; ++it) {
+      const double* cur = contrib[(it - 1) & 1];
+      double* nxt = contrib[it & 1];
+      hipEvent_t* ev = timed ? timing_begin(ctx) : nullptr;
+      hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, B->unit_block,
+                         B->block_unit0, cur, nxt, rank, g->off, n, unit_part, partial, done, damping, added);
+      if (mgrid)
+        hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, unit_part, rank,
+                           g->off, nxt, n, partial + B->nunits, done, damping, added);
+      if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
+      hipLaunchKernelGGL(k_pr_finish_b, 1, 1024, 0, ctx->stream, partial, (int)nparts, l1_hist, iter_dev, done,
+                         epsilon);
+      RG_HIP(hipGetLastError());
+    }
+    return RG_OK;
+  };
+  if (ctx->graphs && !ctx->timing && maxiters > 0) {
+    // The whole loop as one graph, captured once for these arguments.
+    const std::vector<uint64_t> key = {
+        (uint64_t)(uintptr_t)B->src, (uint64_t)(uintptr_t)B->vl, (uint64_t)(uintptr_t)B->unit_lo,
+        (uint64_t)(uintptr_t)B->unit_block, (uint64_t)(uintptr_t)B->block_unit0, (uint64_t)(uintptr_t)B->multi,
+        (uint64_t)B->nunits, (uint64_t)mgrid, (uint64_t)(uintptr_t)merge,
+        (uint64_t)(uintptr_t)kern, (uint64_t)threads, (uint64_t)maxiters,
+        (uint64_t)(uintptr_t)rank, (uint64_t)(uintptr_t)contrib[0], (uint64_t)(uintptr_t)contrib[1],
+        (uint64_t)(uintptr_t)unit_part, (uint64_t)(uintptr_t)partial, (uint64_t)nparts,
+        (uint64_t)(uintptr_t)g->off, (uint64_t)n, (uint64_t)(uintptr_t)done};
+    GraphCache& gc = ctx->pr_graph;
+    if (gc.key != key || !gc.exec) {
+      gc.release();
+      RG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = iterations(false);
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+      if (rc != RG_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      RG_HIP(ec);
+      gc.graph = graph;
+      RG_HIP(hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0));
+      gc.key = key;
+    }
+    RG_HIP(hipGraphLaunch(gc.exec, ctx->stream));
+  } else {
+    RG_TRY(iterations(true));
   }
   RG_HIP(hipEventRecord(t1, ctx->stream));
   RG_HIP(hipEventSynchronize(t1));

@@ -43,6 +43,18 @@ struct DevBuf {
 
 }  // namespace rg
 
+namespace rg {
+// An instantiated HIP graph of a captured launch sequence and the arguments
+// it was captured with (pointers, sizes): replayed while they are equal,
+// captured again when they change.
+struct GraphCache {
+  std::vector<uint64_t> key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void release();
+};
+}  // namespace rg
+
 struct rg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -70,6 +82,10 @@ struct rg_ctx {
   void* ms = nullptr;
   // Batched BC state (opaque, bcbatch.hip), grown on demand.
   void* bcb = nullptr;
+  // Iteration loops replayed as HIP graphs (rg_ctx_graphs; on by default,
+  // not used while per-launch timing is on).
+  bool graphs = true;
+  rg::GraphCache pr_graph;   // blocked PageRank's maxiters iterations
   // Device array holding the last PageRank ranks / BC dependencies computed
   // on this context (n doubles; valid until the next call on the context).
   const double* last_result = nullptr;

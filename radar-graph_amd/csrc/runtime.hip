@@ -33,6 +33,14 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+void GraphCache::release() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  key.clear();
+}
+
 int ctx_frontiers(rg_ctx* ctx, int64_t n, Frontier& a, Frontier& b) {
   size_t cnt = (size_t)std::max<int64_t>(n, 64);
   RG_TRY(ctx->fr_ids[0].ensure(sizeof(uint32_t) * cnt));
@@ -124,6 +132,7 @@ int rg_ctx_destroy(rg_ctx* ctx) {
   if (!ctx) return RG_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  ctx->pr_graph.release();
   ctx->counts.release();
   for (auto& b : ctx->scratch) b.release();
   ctx->fr_ids[0].release();
@@ -149,6 +158,13 @@ int rg_ctx_timing(rg_ctx* ctx, int enable) {
   RG_HIP(hipStreamSynchronize(ctx->stream));
   ctx->timing = enable != 0;
   ctx->ev_used = 0;
+  return RG_OK;
+}
+
+int rg_ctx_graphs(rg_ctx* ctx, int enable) {
+  if (!ctx) return set_error(RG_EINVAL, "rg_ctx_graphs: null");
+  RG_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->graphs = enable != 0;
   return RG_OK;
 }
 

@@ -10,6 +10,7 @@ counting) run on.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 from dataclasses import dataclass
 
@@ -129,6 +130,7 @@ _sig("rg_ctx_create", C.c_int, C.c_int, C.POINTER(_vp))
 _sig("rg_ctx_destroy", C.c_int, _vp)
 _sig("rg_ctx_sync", C.c_int, _vp)
 _sig("rg_ctx_timing", C.c_int, _vp, C.c_int)
+_sig("rg_ctx_graphs", C.c_int, _vp, C.c_int)
 _sig("rg_ctx_kernel_time", C.c_int, _vp, _i64p, _f64p)
 _sig("rg_device_copy_bandwidth", C.c_int, _vp, C.c_int64, C.c_int, _f64p, _f64p)
 _sig("rg_graph_upload", C.c_int, _vp, _vp, C.POINTER(_vp))
@@ -366,10 +368,15 @@ class Context:
 
     def __init__(self, device=0):
         self._h = _vp()
+        self._graphs = weakref.WeakSet()   # device graphs bound to this context
         _check(lib.rg_ctx_create(device, C.byref(self._h)))
 
     def sync(self):
         _check(lib.rg_ctx_sync(self._h))
+
+    def graphs(self, enable=True):
+        """Iteration loops replayed as HIP graphs (on by default)."""
+        _check(lib.rg_ctx_graphs(self._h, int(enable)))
 
     def timing(self, enable=True):
         """Bracket every edge-map launch with HIP events (resets the record)."""
@@ -388,7 +395,11 @@ class Context:
         return n.value, ms.value
 
     def close(self):
+        """Frees the context; the device graphs bound to it are freed first
+        (a graph must not outlive its context)."""
         if self._h:
+            for g in list(self._graphs):
+                g.close()
             lib.rg_ctx_destroy(self._h)
             self._h = None
 
@@ -426,6 +437,7 @@ class DeviceGraph:
         else:
             self._h = _vp()
             _check(lib.rg_graph_upload(ctx._h, hg._h, C.byref(self._h)))
+        ctx._graphs.add(self)
         n, m, s = C.c_int64(), C.c_int64(), C.c_int()
         _check(lib.rg_graph_info(self._h, C.byref(n), C.byref(m), C.byref(s)))
         self.n, self.m, self.symmetric = n.value, m.value, bool(s.value)

@@ -1,0 +1,79 @@
+"""GPU tests of the HIP-graph replay of iteration loops: the blocked
+PageRank loop captured once per context and replayed.  Every replay, with
+other eager work and synchronisations between the replays, agrees with the
+eager launches (the only difference being the LDS atomic order: 1e-10
+relative, iterations exact) and with the oracle; a changed graph or
+iteration count is captured again."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP_TOL = 1e-10
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def test_pagerank_graph_replays_match_eager(rg, orc, ctx, graphs):
+    hg = graphs("rmat14s")
+    off, nbr = hg.csr()
+    c = rg.Context(0)
+    try:
+        dg = rg.DeviceGraph(c, hg)
+        o = rg.opts("blocked")
+        c.graphs(False)
+        eager = dg.pagerank(20, o)
+        ref, it, _ = orc.pagerank(off, nbr, 20)
+        assert eager.iters == it and rel(eager.ranks, ref) < FP_TOL
+        c.graphs(True)
+        src = hg.pick_sources(3)
+        for rep in range(5):
+            print("replay", rep, flush=True)
+            r = dg.pagerank(20, o)
+            assert r.iters == eager.iters, rep
+            assert rel(r.ranks, eager.ranks) < FP_TOL, rep
+            assert abs(r.l1 - eager.l1) <= FP_TOL * abs(eager.l1) + 1e-300, rep
+            # eager work on the same context between the replays
+            dg.bfs(int(src[rep % 3]))
+            dg.bc_batch(src)
+            c.sync()
+    finally:
+        c.close()
+
+
+def test_pagerank_graph_recaptured_on_change(rg, orc, ctx, graphs):
+    # Alternating graphs and iteration counts on one context: each call
+    # captures again and still equals the oracle.
+    a, b = graphs("rmat12s"), graphs("rmat12d")
+    c = rg.Context(0)
+    try:
+        da, db = rg.DeviceGraph(c, a), rg.DeviceGraph(c, b)
+        o = rg.opts("blocked")
+        for hg, dg, iters in [(a, da, 20), (b, db, 20), (a, da, 7), (a, da, 7), (b, db, 20), (a, da, 20)]:
+            off, nbr = hg.csr()
+            ref, it, l1 = orc.pagerank(off, nbr, iters)
+            r = dg.pagerank(iters, o)
+            assert r.iters == it and rel(r.ranks, ref) < FP_TOL, (hg.n, iters)
+    finally:
+        c.close()
+
+
+def test_pagerank_converged_loop_replay(rg, orc, ctx):
+    # A graph that converges before maxiters: the replayed loop's later
+    # iterations do nothing, and the printed iterations / L1 are the
+    # converged ones.
+    n = 64
+    a = np.arange(n, dtype=np.uint32)
+    hg = rg.HostGraph.from_edges(n, a, (a + 1) % n, symmetric=True)   # a cycle: converges at once
+    off, nbr = hg.csr()
+    c = rg.Context(0)
+    try:
+        dg = rg.DeviceGraph(c, hg)
+        ref, it, l1 = orc.pagerank(off, nbr, 50)
+        for _ in range(3):
+            r = dg.pagerank(50, rg.opts("blocked"))
+            assert r.iters == it and rel(r.ranks, ref) < FP_TOL
+    finally:
+        c.close()

@@ -773,10 +773,14 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   // frontier size and degree sum (the source counts once).
   int64_t reached = 1, edges = degsum;
   EventTimer tm(ctx->stream);
+  // Consecutive pull levels pass the frontier on as a complete bitmap.
+  const unsigned long long* pull_bits = nullptr;
   while (F->size > 0) {
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
+    const unsigned long long* cur_bits = pull_bits;
+    pull_bits = nullptr;
     if (pull && degsum > 0 && F->size + degsum > E.threshold)
-      RG_TRY(bfs_pull_level(g, F->ids, F->size, parents, o->heavy_degree, N->ids, info));
+      RG_TRY(bfs_pull_level(g, F->ids, F->size, parents, o->heavy_degree, N->ids, info, cur_bits, &pull_bits));
     else if (blocked && degsum > 0 && F->size + degsum > E.threshold)
       RG_TRY(bfs_dense_level(g, F->ids, F->size, parents, N->ids, info));
     else

@@ -226,11 +226,17 @@ __global__ __launch_bounds__(THREADS) void k_bfs_merge(BcDenseArgs a, uint32_t* 
 // false).  Light vertices: one thread each; in-degree above heavy_deg: one
 // wavefront each (second kernel), 64 in-neighbours per step, the lowest
 // lane that finds one wins -- the same parent the ordered scan picks.
+// Thread per vertex (a wave holds 64 aligned vertices): an unvisited light
+// vertex takes its first in-neighbour on the frontier as parent.  Every
+// word of the next frontier's bitmap `next` is written (a ballot per wave),
+// so the next pull level reads it as it is; heavy vertices are queued for
+// k_bfs_pull_heavy, which ORs in their bits.
 __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, const uint32_t* in_nbr,
                                                         const uint64_t* off, int64_t n,
                                                         const unsigned long long* active, uint32_t* parents,
                                                         uint32_t* heavy, int64_t* heavy_count, int heavy_deg,
-                                                        uint32_t* out_ids, int64_t* out_info) {
+                                                        uint32_t* out_ids, int64_t* out_info,
+                                                        unsigned long long* next) {
   __shared__ uint32_t s_q[4][kQueue];
   EdgeArgs ea{};
   ea.off = off;
@@ -255,6 +261,8 @@ __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, 
         }
       }
     }
+    const unsigned long long word = __ballot(join);
+    if (__lane_id() == 0 && v < n) next[v >> 6] = word;
     wq_push(q, ea, join, (uint32_t)v);
   }
   wq_finish(q, ea);
@@ -264,7 +272,7 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(const uint64_t* in_off, 
                                                         const uint64_t* off, const unsigned long long* active,
                                                         uint32_t* parents, const uint32_t* heavy,
                                                         const int64_t* heavy_count, uint32_t* out_ids,
-                                                        int64_t* out_info) {
+                                                        int64_t* out_info, unsigned long long* next) {
   __shared__ uint32_t s_q[4][kQueue];
   EdgeArgs ea{};
   ea.off = off;
@@ -291,7 +299,10 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(const uint64_t* in_off, 
       if (m) {
         const int first = __ffsll((long long)m) - 1;
         const uint32_t p = __shfl(s, first);
-        if (lane == 0) parents[v] = p;
+        if (lane == 0) {
+          parents[v] = p;
+          atomicOr(next + (v >> 6), 1ull << (v & 63));
+        }
         join = true;
         break;
       }
@@ -408,31 +419,42 @@ int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* p
 }
 
 int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, int heavy_deg,
-                   uint32_t* out_ids, int64_t* out_info) {
+                   uint32_t* out_ids, int64_t* out_info, const unsigned long long* cur_bits,
+                   const unsigned long long** next_bits) {
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
   if (heavy_deg <= 0) heavy_deg = 256;
-  // Heavy list (vertices above heavy_deg; at most m / heavy_deg) and its
-  // counter after the frontier bitmap.
+  // After the frontier bitmap (all zero between uses): two complete
+  // bitmaps that consecutive pull levels pass on, the heavy counter and the
+  // heavy list (vertices above heavy_deg; at most m / heavy_deg).
+  const int64_t nwords = n / 64 + 1;
+  const size_t full_bytes = ((sizeof(unsigned long long) * nwords + 255) / 256) * 256;
   const size_t cap = (size_t)(g->m / heavy_deg + 1);
   unsigned long long* bits = nullptr;
   void* part = nullptr;
-  RG_TRY(frontier_bitmap(ctx, n, sizeof(int64_t) * 2 + sizeof(uint32_t) * cap, &bits, &part));
-  int64_t* hcount = static_cast<int64_t*>(part);
+  RG_TRY(frontier_bitmap(ctx, n, 2 * full_bytes + sizeof(int64_t) * 2 + sizeof(uint32_t) * cap, &bits, &part));
+  unsigned long long* full[2] = {static_cast<unsigned long long*>(part),
+                                 reinterpret_cast<unsigned long long*>(static_cast<char*>(part) + full_bytes)};
+  int64_t* hcount = reinterpret_cast<int64_t*>(static_cast<char*>(part) + 2 * full_bytes);
   uint32_t* heavy = reinterpret_cast<uint32_t*>(hcount + 2);
   RG_TRY(fill<int64_t>(ctx, hcount, 1, 0));
+  // The current frontier: the previous pull level's complete bitmap, or
+  // the list's bits set into the zero bitmap (cleared again below).
+  const bool chained = cur_bits == full[0] || cur_bits == full[1];
+  const unsigned long long* active = chained ? cur_bits : bits;
+  unsigned long long* next = cur_bits == full[0] ? full[1] : full[0];
   const int grid = grid_for(ctx, count);
-  hipLaunchKernelGGL(k_set_bits, grid, 256, 0, ctx->stream, ids, count, bits);
+  if (!chained) hipLaunchKernelGGL(k_set_bits, grid, 256, 0, ctx->stream, ids, count, bits);
   hipEvent_t* ev = timing_begin(ctx);
   const int vgrid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 8);
   hipLaunchKernelGGL(k_bfs_pull_light, std::max(vgrid, 1), 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off, n,
-                     bits, parents, heavy, hcount, heavy_deg, out_ids, out_info);
-  hipLaunchKernelGGL(k_bfs_pull_heavy, ctx->num_cus * 4, 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off, bits,
-                     parents, heavy, hcount, out_ids, out_info);
+                     active, parents, heavy, hcount, heavy_deg, out_ids, out_info, next);
+  hipLaunchKernelGGL(k_bfs_pull_heavy, ctx->num_cus * 4, 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off, active,
+                     parents, heavy, hcount, out_ids, out_info, next);
   if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
-  hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);
+  if (!chained) hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);
   RG_HIP(hipGetLastError());
-  // The heavy list overwrote scratch past the bitmap only.
+  if (next_bits) *next_bits = next;
   return RG_OK;
 }
 

@@ -97,9 +97,14 @@ int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* p
                     int64_t* out_info);
 
 // One bottom-up BFS level (the push-pull tree's pull edgeMap with early
-// exit) over g's in-edges (bcdense.hip).
+// exit) over g's in-edges (bcdense.hip).  cur_bits: the complete frontier
+// bitmap the previous pull level returned in *next_bits, or null (the
+// frontier is then taken from ids).  *next_bits (nullable) receives the
+// next frontier's complete bitmap, valid until the next dense call on the
+// context.
 int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, int heavy_deg,
-                   uint32_t* out_ids, int64_t* out_info);
+                   uint32_t* out_ids, int64_t* out_info, const unsigned long long* cur_bits = nullptr,
+                   const unsigned long long** next_bits = nullptr);
 
 int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
             int64_t* combined_out, double* ms_out);

@@ -120,6 +120,10 @@ struct rg_graph {
   void* blocked = nullptr;
   void* blocked_ms = nullptr;     // of the in-edges of the original orientation
   void* blocked_ms_t = nullptr;   // of the transposed orientation (directed graphs)
+  // 1 + the last vertex with in-edges in the orientation whose in-offsets
+  // are nz_for (the bound of bottom-up scans).
+  const uint64_t* nz_for = nullptr;
+  int64_t nz_end = 0;
   // A view shares the device arrays and cached layouts of `base` (bound to
   // another context, e.g. one serving slot) and frees only what it builds.
   bool view = false;

@@ -353,13 +353,15 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
  * execute them concurrently, and with batching on, every free slot takes
  * all BFS queries that have arrived (up to max_batch) as one bit-parallel
  * batch (rg_bfs_batch) -- continuous batching: queries that arrive while
- * others run join the next batch on the next free slot; BC queries the
- * same way in batches of up to 64 (rg_bc_batch) once three or more wait,
- * one per slot on the single-source engine otherwise.  PageRank queries run one
+ * others run join the next batch on the next free slot, once three or more
+ * wait (fewer run one per slot on the single-source engine with bfs_opts,
+ * which answers one or two queries sooner than a batch does); BC queries
+ * the same way in batches of up to 64 (rg_bc_batch) once three or more
+ * wait, one per slot on the single-source engine otherwise.  PageRank queries run one
  * per slot.  The number of slots is capped by free HBM (less hbm_reserve)
  * over a per-slot estimate.  one_at_a_time = 1 runs the same trace on one
  * slot, one query at a time, BFS on the single-source engine
- * (rg_bfs_query), BC on rg_bc with one source: the baseline.
+ * (rg_bfs_query with bfs_opts), BC on rg_bc with one source: the baseline.
  * Answers: BFS an rg_bfs_summary (equal to rg_bfs_query's); BC (one source,
  * BC-manyiter-v2.C semantics for that source, orientation restored after)
  * and PageRank (`iters` iterations, PageRank.C) a summary of the result
@@ -388,8 +390,10 @@ typedef struct rg_serve_config {
   int max_batch;         /* BFS queries per batch (1..256, <= 0: 256) */
   int one_at_a_time;
   int64_t hbm_reserve;   /* bytes of device memory left free (<= 0: 4 GiB) */
-  rg_opts opts;          /* engine options for every query (default variant
-                            RG_BLOCKED: blocked PageRank, blocked dense BC levels) */
+  rg_opts opts;          /* engine options for BC, PageRank and batched BFS (default
+                            variant RG_BLOCKED: blocked PageRank, blocked dense BC levels) */
+  rg_opts bfs_opts;      /* single-source BFS queries (default variant RG_PULL:
+                            bottom-up dense levels) */
 } rg_serve_config;
 typedef struct rg_serve_stats {
   int64_t queries, batches;

@@ -35,10 +35,12 @@ namespace rg {
 
 constexpr int kSumBlocks = 512;
 constexpr int kBcBatch = 64;   // BC queries per batch (one word of source bits)
-// Fewest waiting BC queries worth a batch: a batch of one or two costs more
-// than as many single-source runs (rg_bc_batch: ~1.7x one single-source run
-// for one source, ~2.3x for two, on an R-MAT of 2^22 vertices).
-constexpr int kBcBatchMin = 3;
+// Fewest waiting queries of a kind worth a batch: a batch of one or two
+// costs more than as many single-source runs, three about as much or less
+// (symmetric R-MAT of 2^22 vertices: a BFS batch of 1 / 2 / 3 takes 2.2 /
+// 2.5 / 2.9 ms against 1.1 ms per bottom-up single-source query; a BC batch
+// of one ~1.7x a single-source run, of two ~2.3x).
+constexpr int kBatchMin = 3;
 
 // Weighted sum of the finite entries (weights w(v) in [0,1) from a fixed
 // hash of v) and count of the non-finite ones: per-block partial sums in a
@@ -136,10 +138,12 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
       }
     if (best >= 0) {
       size_t take = 1;
-      if (best == RG_Q_BFS && !c.one_at_a_time) take = std::min<size_t>(S.pending[best].size(), (size_t)c.max_batch);
-      // BC: a batch once kBcBatchMin queries wait (below that, single-source
+      // BFS: a batch once kBatchMin queries wait (below that, single-source
       // runs on free slots answer sooner), else one query.
-      if (best == RG_Q_BC && !c.one_at_a_time && S.pending[best].size() >= (size_t)kBcBatchMin)
+      if (best == RG_Q_BFS && !c.one_at_a_time && S.pending[best].size() >= (size_t)kBatchMin)
+        take = std::min<size_t>(S.pending[best].size(), (size_t)c.max_batch);
+      // BC likewise, in batches of up to 64.
+      if (best == RG_Q_BC && !c.one_at_a_time && S.pending[best].size() >= (size_t)kBatchMin)
         take = std::min<size_t>(S.pending[best].size(), (size_t)std::min(c.max_batch, kBcBatch));
       for (size_t i = 0; i < take; ++i) {
         work.push_back(S.pending[best].front());
@@ -171,10 +175,10 @@ int run_work(Slot& sl, const rg_serve_config& c, const rg_query* q, int kind, co
   std::vector<int64_t> nonfin(k, 0);
   double ms = 0.0;
   if (kind == RG_Q_BFS) {
-    if (c.one_at_a_time) {
+    if (k == 1) {
       uint64_t cs = 0;
-      RG_TRY(bfs_push(g, q[work[0]].source, &qo, nullptr, &sum[0].iters, &sum[0].reached, &sum[0].edges, &ms,
-                      nullptr, &cs));
+      RG_TRY(bfs_push(g, q[work[0]].source, &c.bfs_opts, nullptr, &sum[0].iters, &sum[0].reached, &sum[0].edges,
+                      &ms, nullptr, &cs));
       sum[0].checksum = cs;
     } else {
       std::vector<uint32_t> src(k);
@@ -230,6 +234,8 @@ void rg_serve_config_default(rg_serve_config* c) {
   c->hbm_reserve = (int64_t)4 << 30;
   rg_opts_default(&c->opts);
   c->opts.variant = RG_BLOCKED;
+  rg_opts_default(&c->bfs_opts);
+  c->bfs_opts.variant = RG_PULL;
 }
 
 int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, int64_t nq,

@@ -89,7 +89,7 @@ class QueryResult(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int), ("max_batch", C.c_int), ("one_at_a_time", C.c_int),
-                ("hbm_reserve", C.c_int64), ("opts", Opts)]
+                ("hbm_reserve", C.c_int64), ("opts", Opts), ("bfs_opts", Opts)]
 
 
 class ServeStats(C.Structure):
@@ -211,7 +211,8 @@ def hash_int(a: int) -> int:
     return int(lib.rg_hash_int(C.c_uint64(a)))
 
 
-def serve_config(slots=2, max_batch=256, one_at_a_time=False, hbm_reserve=0, o: Opts = None) -> ServeConfig:
+def serve_config(slots=2, max_batch=256, one_at_a_time=False, hbm_reserve=0, o: Opts = None,
+                 bfs: Opts = None) -> ServeConfig:
     c = ServeConfig()
     lib.rg_serve_config_default(C.byref(c))
     c.slots, c.max_batch, c.one_at_a_time = slots, max_batch, int(one_at_a_time)
@@ -219,6 +220,8 @@ def serve_config(slots=2, max_batch=256, one_at_a_time=False, hbm_reserve=0, o: 
         c.hbm_reserve = hbm_reserve
     if o is not None:
         c.opts = o
+    if bfs is not None:
+        c.bfs_opts = bfs
     return c
 
 

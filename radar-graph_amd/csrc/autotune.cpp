@@ -7,11 +7,12 @@
 //
 // PageRank candidates: the blocked engine with each workgroup shape
 // (512 / 1024 threads) and 1 or 4 edges in flight per lane, and the atomic
-// RADAR engine at the LDS hub budget.  BFS candidates: the sparse/dense
-// threshold (m/5, m/20, m/80), the wave-per-vertex cut-over degree and the
-// dense-level engine (push; RADAR: LDS hub copies + blocked dense levels;
-// pull: bottom-up dense levels).  Every candidate computes the same levels;
-// only time differs.
+// RADAR engine with hub count k at the LDS budget or 4096, each with and
+// without per-XCD L2 copies of the next 256K hubs.  BFS candidates: the
+// sparse/dense threshold (m/5, m/20, m/80), the wave-per-vertex cut-over
+// degree and the dense-level engine (push; RADAR: LDS hub copies, k = 1024
+// or 4096, + blocked dense levels; pull: bottom-up dense levels).  Every
+// candidate computes the same result; only time differs.
 #include <cstring>
 #include <vector>
 
@@ -45,22 +46,28 @@ extern "C" int rg_autotune(rg_graph* g, int app, const uint32_t* sources, int ns
         o.unroll = un;
         cand.push_back(o);
       }
-    rg_opts r = b;
-    r.variant = RG_RADAR;
-    r.hubs = 0;
-    cand.push_back(r);
+    for (int64_t k : {0, 4096})
+      for (int64_t k2 : {-1, 0}) {
+        rg_opts r = b;
+        r.variant = RG_RADAR;
+        r.hubs = k;
+        r.l2_hubs = k2;
+        cand.push_back(r);
+      }
   } else if (app == RG_APP_BFS) {
     if (!sources || nsources <= 0) return rg::set_error(RG_EINVAL, "rg_autotune: BFS needs sources");
     for (int64_t div : {5, 20, 80})
       for (int heavy : {256, 1024})
-        for (int hv : {RG_PUSH, RG_RADAR, RG_PULL}) {
-          rg_opts o = b;
-          o.variant = hv;
-          o.threshold = m / div;
-          o.heavy_degree = heavy;
-          o.hubs = hv == RG_RADAR ? 2048 : 0;
-          cand.push_back(o);
-        }
+        for (int hv : {RG_PUSH, RG_RADAR, RG_PULL})
+          for (int64_t k : {1024, 4096}) {
+            if (hv != RG_RADAR && k != 1024) continue;
+            rg_opts o = b;
+            o.variant = hv;
+            o.threshold = m / div;
+            o.heavy_degree = heavy;
+            o.hubs = hv == RG_RADAR ? k : 0;
+            cand.push_back(o);
+          }
   } else {
     return rg::set_error(RG_EINVAL, "rg_autotune: unknown app");
   }

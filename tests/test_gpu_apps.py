@@ -213,3 +213,29 @@ def test_pagerank_delta(rg, orc, ctx, graphs, gname, variant):
         assert res.iters == it
         assert rel_linf(res.ranks, ref) < FP_TOL
         assert abs(res.l1 - l1) <= FP_TOL * max(abs(l1), 1e-300) or (res.l1 < 1e-7 and l1 < 1e-7)
+
+
+@pytest.mark.parametrize("gname", ["rmat14s", "rmat12d"])
+def test_autotune_choice_gives_oracle_answers(rg, orc, ctx, graphs, gname):
+    # Whatever configuration the autotuner picks (engine, workgroup shape,
+    # hub count, threshold), it computes the same PageRank and BFS levels.
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    dg = rg.DeviceGraph(ctx, hg)
+    sg, ids = dg.degree_sort()
+    best, ms = sg.autotune("pagerank", base=rg.opts("radar"))
+    assert ms > 0 and best.variant in (rg.BLOCKED, rg.RADAR)
+    soff, snbr = sg.download()
+    r = sg.pagerank(20, best)
+    ref, it, _ = orc.pagerank(soff, snbr, 20)
+    assert r.iters == it and rel_linf(r.ranks, ref) < FP_TOL
+    srcs = ids[hg.pick_sources(4)]
+    bt, _ = sg.autotune("bfs", sources=srcs, base=rg.opts("push"))
+    assert bt.variant in (rg.PUSH, rg.RADAR, rg.PULL)
+    if bt.variant == rg.RADAR:
+        assert bt.hubs in (1024, 4096)
+    for s in srcs:
+        b = sg.bfs(int(s), bt)
+        lv, ecc = orc.bfs_levels(soff, snbr, int(s))
+        assert orc.check_parents(soff, snbr, int(s), lv, b.parents) == 0
+        assert b.iters == ecc + 1

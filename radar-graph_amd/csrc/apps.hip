@@ -952,10 +952,23 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
   // RADAR and blocked run dense levels on the blocked layout (bcdense.hip);
   // push and HUBDUP keep the atomic push engine for every level.
   const bool blocked = o && (o->variant == RG_RADAR || o->variant == RG_BLOCKED);
+  // Blocked dense levels keep one complete frontier bitmap per level (the
+  // forward's dense levels write the next one themselves, sparse levels set
+  // it from their list) and the backward reuses them: no level rebuilds a
+  // bitmap from a list.  Levels past kLevelBits fall back to doing so.
+  constexpr int64_t kLevelBits = 64;
+  const int64_t nwords = n / 64 + 1;
+  DevBuf& b_lvb = ctx->scratch[4];
+  unsigned long long* lvb = nullptr;
   if (blocked) {
     BlockedCSC* B = nullptr;
     RG_TRY(query_layout(g, &B));   // the forward orientation's layout, outside the timed loop
+    RG_TRY(b_lvb.ensure(sizeof(unsigned long long) * nwords * kLevelBits));
+    lvb = b_lvb.as<unsigned long long>();
   }
+  auto level_bits = [&](int64_t L) -> unsigned long long* {
+    return lvb && L < kLevelBits ? lvb + L * nwords : nullptr;
+  };
   EventTimer tm(ctx->stream);
   for (int it = 0; it < nsources; ++it) {
     Engine E;
@@ -968,6 +981,11 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
     std::vector<int64_t> lstart = {0, 1};
     int64_t degsum = 0;
     RG_TRY(frontier_degsum(ctx, g, lv, 1, &degsum));
+    std::vector<int64_t> lvdeg = {degsum};   // out-degree sum of each level
+    if (unsigned long long* b0 = level_bits(0)) {
+      RG_TRY(fill<unsigned long long>(ctx, b0, nwords, 0ull));
+      RG_TRY(set_frontier_bits(ctx, lv, 1, b0));
+    }
     int64_t round = 0;
     BcFwdOp fop{np, visited};
     while (true) {
@@ -977,15 +995,22 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
       F.ids = lv + fb;
       F.size = fe - fb;
       RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
-      if (blocked && F.size + degsum > E.threshold)
-        RG_TRY(bc_dense_level(g, true, F.ids, F.size, np, np, visited, lv + fe, info));
+      // Level round - 1 is the frontier; level round is being found.
+      unsigned long long* cur_b = level_bits(round - 1);
+      unsigned long long* nxt_b = cur_b ? level_bits(round) : nullptr;
+      if (nxt_b) RG_TRY(fill<unsigned long long>(ctx, nxt_b, nwords, 0ull));
+      const bool dense = blocked && F.size + degsum > E.threshold;
+      if (dense)
+        RG_TRY(bc_dense_level(g, true, F.ids, F.size, np, np, visited, lv + fe, info, cur_b, nxt_b));
       else
         RG_TRY(edge_map(E, fop, F, degsum, lv + fe, info));
       int64_t h[2];
       RG_TRY(ctx_read_count(ctx, info, h, 2));
       if (h[0] > 0)
         hipLaunchKernelGGL(k_mark, grid_for(ctx, h[0]), 256, 0, ctx->stream, lv + fe, h[0], visited);
+      if (!dense && nxt_b) RG_TRY(set_frontier_bits(ctx, lv + fe, h[0], nxt_b));
       degsum = h[1];
+      lvdeg.push_back(degsum);
       lstart.push_back(fe + h[0]);
     }
     combined += round;
@@ -1002,10 +1027,13 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
     RG_TRY(make_engine(g, o, sizeof(double), EB));
     BcBackOp bop{deps, visited};
     for (int64_t r = round - 2; r >= 0; --r) {
+      // The frontier is level r + 1; its out-degree sum in the transposed
+      // graph is its in-degree sum, the forward's own sum on a symmetric graph.
       int64_t ds = 0;
-      RG_TRY(frontier_degsum(ctx, g, F.ids, F.size, &ds));
+      if (g->symmetric) ds = lvdeg[r + 1];
+      else RG_TRY(frontier_degsum(ctx, g, F.ids, F.size, &ds));
       if (blocked && ds > 0 && F.size + ds > EB.threshold)
-        RG_TRY(bc_dense_level(g, false, F.ids, F.size, deps, deps, visited, nullptr, nullptr));
+        RG_TRY(bc_dense_level(g, false, F.ids, F.size, deps, deps, visited, nullptr, nullptr, level_bits(r + 1)));
       else
         RG_TRY(edge_map(EB, bop, F, ds, nullptr, nullptr));
       level(r);

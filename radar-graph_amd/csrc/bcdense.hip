@@ -40,6 +40,8 @@ struct BcDenseArgs {
   double* part;                        // per-unit partial sums of split blocks
   uint32_t* out_ids;                   // forward: next frontier
   int64_t* out_info;                   // forward: [count, out-degree sum]
+  unsigned long long* next_bits;       // forward (nullable): the next frontier's bitmap, a word per
+                                       // wave of finalised destinations (blocks left out stay as they are)
 };
 
 // Whether every bit of the block's skip words is set (all threads read the
@@ -68,7 +70,14 @@ __device__ __forceinline__ void bc_finalize(const BcDenseArgs& a, WaveQueue& q, 
       a.out[d] += x;
     }
   }
-  if (FWD) wq_push(q, ea, join, (uint32_t)d);
+  if (FWD) {
+    // Called by whole waves over 64 aligned destinations.
+    if (a.next_bits) {
+      const unsigned long long word = __ballot(join);
+      if (__lane_id() == 0 && d < a.n) a.next_bits[d >> 6] = word;
+    }
+    wq_push(q, ea, join, (uint32_t)d);
+  }
 }
 
 template <bool FWD, int VB, int THREADS>
@@ -413,7 +422,8 @@ int frontier_bitmap(rg_ctx* ctx, int64_t n, size_t part_bytes, unsigned long lon
 }
 
 int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const double* val, double* out,
-                const uint8_t* visited, uint32_t* parents, uint32_t* out_ids, int64_t* out_info) {
+                const uint8_t* visited, uint32_t* parents, uint32_t* out_ids, int64_t* out_info,
+                const unsigned long long* active_bits = nullptr, unsigned long long* next_bits = nullptr) {
   rg_ctx* ctx = g->ctx;
   BlockedCSC* B = nullptr;
   RG_TRY(query_layout(g, &B));
@@ -437,8 +447,12 @@ int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const
   a.part = static_cast<double*>(part);
   a.out_ids = out_ids;
   a.out_info = out_info;
+  a.next_bits = next_bits;
+  // The frontier: a complete bitmap the caller keeps, or the list's bits
+  // set into the zero bitmap (cleared again below).
+  if (active_bits) a.active = active_bits;
   const int grid = grid_for(ctx, count);
-  hipLaunchKernelGGL(k_set_bits, grid, 256, 0, ctx->stream, ids, count, bits);
+  if (!active_bits) hipLaunchKernelGGL(k_set_bits, grid, 256, 0, ctx->stream, ids, count, bits);
   constexpr int PER = (1 << kQueryVB) / kBcThreads;
   hipEvent_t* ev = timing_begin(ctx);
   if (kind == 2) {
@@ -460,7 +474,7 @@ int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const
                          ctx->stream, a);
   }
   if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
-  hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);
+  if (!active_bits) hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);
   RG_HIP(hipGetLastError());
   return RG_OK;
 }
@@ -468,8 +482,17 @@ int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const
 }  // namespace
 
 int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, const double* val, double* out,
-                   const uint8_t* visited, uint32_t* out_ids, int64_t* out_info) {
-  return dense_level(g, fwd ? 0 : 1, ids, count, val, out, visited, nullptr, out_ids, out_info);
+                   const uint8_t* visited, uint32_t* out_ids, int64_t* out_info,
+                   const unsigned long long* active_bits, unsigned long long* next_bits) {
+  return dense_level(g, fwd ? 0 : 1, ids, count, val, out, visited, nullptr, out_ids, out_info, active_bits,
+                     next_bits);
+}
+
+int set_frontier_bits(rg_ctx* ctx, const uint32_t* ids, int64_t count, unsigned long long* bits) {
+  if (count <= 0) return RG_OK;
+  hipLaunchKernelGGL(k_set_bits, grid_for(ctx, count), 256, 0, ctx->stream, ids, count, bits);
+  RG_HIP(hipGetLastError());
+  return RG_OK;
 }
 
 int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, uint32_t* out_ids,

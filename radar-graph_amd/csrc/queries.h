@@ -88,8 +88,16 @@ int static_heavy_list(rg_graph* g, int heavy_deg);
 
 // One dense BC level (forward path counts or backward dependencies) on the
 // blocked layout of g's current orientation (bcdense.hip).
+// active_bits (nullable): a complete bitmap of the frontier ids[0, count)
+// the caller keeps (else the bits are set from ids and cleared after);
+// next_bits (forward, nullable): receives the words of the next frontier's
+// bitmap for every block with an unvisited destination (the caller zeroes
+// it first).
 int bc_dense_level(rg_graph* g, bool fwd, const uint32_t* ids, int64_t count, const double* val, double* out,
-                   const uint8_t* visited, uint32_t* out_ids, int64_t* out_info);
+                   const uint8_t* visited, uint32_t* out_ids, int64_t* out_info,
+                   const unsigned long long* active_bits = nullptr, unsigned long long* next_bits = nullptr);
+// Sets the bits of ids[0, count) in `bits` (atomic OR per vertex).
+int set_frontier_bits(rg_ctx* ctx, const uint32_t* ids, int64_t count, unsigned long long* bits);
 
 // One dense BFS level (BFS_F: unvisited vertices with a frontier in-neighbour
 // take one as parent and join the next frontier) on the blocked layout.

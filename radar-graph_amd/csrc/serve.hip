@@ -280,7 +280,7 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
   RG_HIP(hipMemGetInfo(&free_b, &total_b));
   const int64_t n = g->n, m = g->m;
   const int64_t bfs_b = rg::bfs_batch_bytes(n, m, c.max_batch);
-  const int64_t bc_b = std::max<int64_t>(n * 30 + (m / 2048 + m / 1024 + 64) * 8,
+  const int64_t bc_b = std::max<int64_t>(n * 38 + (m / 2048 + m / 1024 + 64) * 8,
                                          c.one_at_a_time ? 0 : rg::bc_batch_bytes(n, m));
   const int64_t pr_b = n * 24 + ((n >> 14) + m / rg::kUnitEdges + 2) * (int64_t)(16384 * 8);
   const int64_t slot_bytes = bfs_b + std::max(bc_b, pr_b);

@@ -608,6 +608,17 @@ int bc_batch64(rg_graph* g, BcbState& S, const uint32_t* sources, int nsrc, cons
 
 }  // namespace
 
+int bc_batch_reserve(rg_graph* g) {
+  rg_ctx* ctx = g->ctx;
+  BcbState* S = static_cast<BcbState*>(ctx->bcb);
+  if (!S) {
+    S = new BcbState;
+    ctx->bcb = S;
+  }
+  RG_TRY(bcb_prepare(*S, g->n, g->m, kBcbLevelsInit));
+  return bfs_batch_reserve(g, kBcbMaxSources);
+}
+
 // 1..256 sources: batches of 64 in order, the last one uneven (a batch
 // costs far less than 64 single runs at any size, so full batches first).
 int bc_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_out,

@@ -660,6 +660,19 @@ int bfs_batch_prepare(rg_graph* g) {
   return query_layout(g, &B);
 }
 
+int bfs_batch_reserve(rg_graph* g, int nsrc) {
+  rg_ctx* ctx = g->ctx;
+  BlockedCSC* B = nullptr;
+  RG_TRY(query_layout(g, &B));
+  MsState* S = static_cast<MsState*>(ctx->ms);
+  if (!S) {
+    S = new MsState;
+    ctx->ms = S;
+  }
+  const int W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
+  return ms_prepare(ctx, *S, g->n, W, B->nunits);
+}
+
 int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc) {
   const int64_t W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
   const int64_t units = (n >> kMsVB) + m / kUnitEdges + 2;

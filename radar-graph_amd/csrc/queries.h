@@ -72,6 +72,11 @@ int radii_batch(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts*
 
 // Builds the blocked in-edge layout the batches use (once per graph).
 int bfs_batch_prepare(rg_graph* g);
+// Allocates g's context's batch state for batches of up to nsrc sources
+// (so that a first batch does not pay for it).
+int bfs_batch_reserve(rg_graph* g, int nsrc);
+// Likewise for BC batches (their state and the one-word BFS batch state).
+int bc_batch_reserve(rg_graph* g);
 // Device bytes one context holds for batches of nsrc sources on an
 // (n, m) graph.
 int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc);

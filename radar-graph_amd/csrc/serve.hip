@@ -35,11 +35,12 @@ namespace rg {
 
 constexpr int kSumBlocks = 512;
 constexpr int kBcBatch = 64;   // BC queries per batch (one word of source bits)
-// Fewest waiting queries of a kind worth a batch: a batch of one or two
+// Fewest waiting queries of a kind worth a batch before any run has been
+// timed, and the floor of the learned threshold: a batch of one or two
 // costs more than as many single-source runs, three about as much or less
-// (symmetric R-MAT of 2^22 vertices: a BFS batch of 1 / 2 / 3 takes 2.2 /
-// 2.5 / 2.9 ms against 1.1 ms per bottom-up single-source query; a BC batch
-// of one ~1.7x a single-source run, of two ~2.3x).
+// on small graphs (symmetric R-MAT of 2^22 vertices: a BFS batch of 1 / 2 /
+// 3 takes 2.2 / 2.5 / 2.9 ms against 1.1 ms per bottom-up single-source
+// query; a BC batch of one ~1.7x a single-source run, of two ~2.3x).
 constexpr int kBatchMin = 3;
 
 // Weighted sum of the finite entries (weights w(v) in [0,1) from a fixed
@@ -109,6 +110,22 @@ struct Shared {
   std::chrono::steady_clock::time_point t0;
   int err = RG_OK;
   std::string msg;
+  // Running averages of the time of one-query work items and of batches,
+  // per kind.  Every query of a batch completes when the batch does, so k
+  // waiting queries are batched only if the batch ends before the last of
+  // them would as single runs spread over the slots: k >= slots * batch /
+  // single (the ratio grows with the graph: a BC batch of three is cheaper
+  // than three single runs on a 2^22-vertex R-MAT, not on a 2^26-vertex one).
+  double single_ms[3] = {0.0, 0.0, 0.0}, batch_ms[3] = {0.0, 0.0, 0.0};
+  int nslots = 1;
+  size_t min_batch(int kind) const {
+    if (single_ms[kind] <= 0.0 || batch_ms[kind] <= 0.0) return (size_t)kBatchMin;
+    return (size_t)std::max<double>(kBatchMin, std::ceil(nslots * batch_ms[kind] / single_ms[kind]));
+  }
+  void record(int kind, size_t k, double ms) {
+    double& e = k == 1 ? single_ms[kind] : batch_ms[kind];
+    e = e > 0.0 ? 0.75 * e + 0.25 * ms : ms;
+  }
   double ms_now() const {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -138,12 +155,12 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
       }
     if (best >= 0) {
       size_t take = 1;
-      // BFS: a batch once kBatchMin queries wait (below that, single-source
-      // runs on free slots answer sooner), else one query.
-      if (best == RG_Q_BFS && !c.one_at_a_time && S.pending[best].size() >= (size_t)kBatchMin)
+      // BFS: a batch once enough queries wait that it answers them sooner
+      // than single-source runs on free slots would, else one query.
+      if (best == RG_Q_BFS && !c.one_at_a_time && S.pending[best].size() >= S.min_batch(best))
         take = std::min<size_t>(S.pending[best].size(), (size_t)c.max_batch);
       // BC likewise, in batches of up to 64.
-      if (best == RG_Q_BC && !c.one_at_a_time && S.pending[best].size() >= (size_t)kBatchMin)
+      if (best == RG_Q_BC && !c.one_at_a_time && S.pending[best].size() >= S.min_batch(best))
         take = std::min<size_t>(S.pending[best].size(), (size_t)std::min(c.max_batch, kBcBatch));
       for (size_t i = 0; i < take; ++i) {
         work.push_back(S.pending[best].front());
@@ -213,6 +230,48 @@ int run_work(Slot& sl, const rg_serve_config& c, const rg_query* q, int kind, co
     R.l1 = l1[i];
     R.wsum = wsum[i];
     R.nonfinite = nonfin[i];
+  }
+  return RG_OK;
+}
+
+// Before the clock starts: the time of one single-source run and of a
+// one-query batch of each batched kind in the trace (answers discarded), so
+// the batching threshold holds from the first query on.
+int calibrate(Shared& S, Slot& sl, const rg_serve_config& c, const rg_query* q, int64_t nq) {
+  using clock = std::chrono::steady_clock;
+  rg_graph* g = sl.view;
+  rg_opts qo = c.opts;
+  if (qo.variant == RG_PULL) qo.variant = RG_PUSH;
+  for (int kind : {(int)RG_Q_BFS, (int)RG_Q_BC}) {
+    int64_t first = -1;
+    for (int64_t i = 0; i < nq && first < 0; ++i)
+      if (q[i].kind == kind) first = i;
+    if (first < 0) continue;
+    const uint32_t src = q[first].source;
+    double ms = 0.0;
+    auto t = clock::now();
+    for (int rep = 0; rep < 2; ++rep) {   // the second run is timed (the first sets up scratch)
+      t = clock::now();
+      if (kind == RG_Q_BFS) {
+        int64_t it = 0, reached = 0, edges = 0;
+        RG_TRY(bfs_push(g, src, &c.bfs_opts, nullptr, &it, &reached, &edges, &ms, nullptr, nullptr));
+      } else {
+        const bool tr = g->transposed;
+        int64_t it = 0;
+        RG_TRY(bc_push(g, &src, 1, &qo, nullptr, &it, &ms));
+        if (g->transposed != tr) RG_TRY(rg_graph_transpose(g));
+      }
+    }
+    S.single_ms[kind] = std::chrono::duration<double, std::milli>(clock::now() - t).count();
+    t = clock::now();
+    if (kind == RG_Q_BFS) {
+      rg_bfs_summary sum;
+      RG_TRY(bfs_batch(g, &src, 1, &qo, nullptr, &sum, &ms));
+    } else {
+      rg_bc_summary sum;
+      RG_TRY(bc_batch(g, &src, 1, &qo, nullptr, &sum, &ms));
+    }
+    S.batch_ms[kind] = std::chrono::duration<double, std::milli>(clock::now() - t).count();
   }
   return RG_OK;
 }
@@ -299,15 +358,21 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
       slots[i].view->ctx = slots[i].ctx;
       slots[i].view->view = true;
       slots[i].view->base = g;
+      // Batch state allocated before the clock starts.
+      if (!c.one_at_a_time && any_bfs) rc = rg::bfs_batch_reserve(slots[i].view, c.max_batch);
+      if (rc == RG_OK && !c.one_at_a_time && any_bc) rc = rg::bc_batch_reserve(slots[i].view);
     }
   }
+  if (rc == RG_OK) rc = hipDeviceSynchronize() == hipSuccess ? RG_OK : set_error(RG_EDEVICE, "rg_serve: set-up");
   rg::Shared S;
+  S.nslots = nslots;
   S.q = queries;
   S.nq = nq;
   S.order.resize(nq);
   std::iota(S.order.begin(), S.order.end(), 0);
   std::stable_sort(S.order.begin(), S.order.end(),
                    [&](int64_t a, int64_t b) { return queries[a].arrival_ms < queries[b].arrival_ms; });
+  if (rc == RG_OK && !c.one_at_a_time) rc = rg::calibrate(S, slots[0], c, queries, nq);
   if (rc == RG_OK) {
     S.t0 = std::chrono::steady_clock::now();
     std::vector<std::thread> th;
@@ -328,6 +393,10 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
           const double start = S.ms_now();
           int r = rg::run_work(slots[i], c, queries, kind, work, results);
           const double end = S.ms_now();
+          {
+            std::lock_guard<std::mutex> lk(S.mu);
+            S.record(kind, work.size(), end - start);
+          }
           if (r != RG_OK) {
             std::lock_guard<std::mutex> lk(S.mu);
             if (S.err == RG_OK) {

@@ -71,3 +71,45 @@ def test_radar_on_sorted_graph_matches_push(rg, orc, ctx, graphs, gname):
     ra, _, _ = dg.radii(starts, rg.opts("push"))
     rb, _, _ = dg.radii(starts, rg.opts("radar", hubs=4096))
     assert np.array_equal(ra, rb)
+
+
+@pytest.mark.parametrize("gname", ["rmat10s", "rmat12s", "er12s", "rmat14s"])
+@pytest.mark.parametrize("relabel", [False, True])
+def test_tc_total_only(rg, orc, ctx, graphs, gname, relabel):
+    # The totals-only kernels (LDS bitmaps of N+(v) for v < 2^19, then the
+    # flattened per-u kernel) against the oracle.
+    hg = graphs(gname)
+    dg = rg.DeviceGraph(ctx, hg)
+    if relabel:
+        dg, _ = dg.degree_sort(by_outdeg=True, sort_neighbors=True)
+    off, nb = dg.download()
+    tot, _ = dg.tc()
+    assert tot == orc.tc(off, nb)
+
+
+def test_tc_cliques_across_the_bitmap_split(rg, orc, ctx):
+    # 2^18 disjoint 4-cliques on randomly permuted ids in [0, 2^20): cliques
+    # straddle the 2^19 boundary between the two totals kernels.  Each
+    # clique holds 4 triangles.
+    n = 1 << 20
+    perm = np.random.default_rng(5).permutation(n).astype(np.uint32).reshape(-1, 4)
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    src = np.concatenate([perm[:, a] for a, _ in pairs])
+    dst = np.concatenate([perm[:, b] for _, b in pairs])
+    hg = rg.HostGraph.from_edges(n, src, dst, symmetric=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    tot, _ = dg.tc()
+    assert tot == 4 * perm.shape[0]
+    ptot, pv, _ = dg.tc(per_vertex=True)
+    assert ptot == tot and np.all(pv == 3)
+
+
+def test_tc_total_matches_per_vertex_path_rmat20(rg, ctx):
+    # Both totals kernels on a degree-sorted R-MAT with 2^20 vertices, against
+    # the per-vertex merge kernel (itself checked against the oracle above).
+    hg = rg.HostGraph.generate(20, 16, symmetric=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    sg, _ = dg.degree_sort(by_outdeg=True, sort_neighbors=True)
+    tot, _ = sg.tc()
+    ptot, pv, _ = sg.tc(per_vertex=True)
+    assert tot == ptot and int(pv.sum()) == 3 * tot

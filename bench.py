@@ -390,6 +390,18 @@ def main():
         pr["push"] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2)}
         pr["radar_vs_push"] = round(pr["radar_tuned"]["gteps"] / pr["push"]["gteps"], 2)
         log(f"pagerank push {ms:.1f} ms {pr['push']['gteps']} GTEPS; RADAR {pr['radar_vs_push']}x")
+        # PageRankDelta, 20 rounds: push-only on the original ids vs the
+        # blocked engine on the degree-sorted graph (dense rounds blocked,
+        # sparse rounds pushed from the active list).
+        prd = {}
+        for name, gg, oo in (("push", dg, rg.opts("push")), ("blocked", sg, rg.opts("blocked"))):
+            gg.pagerank_delta(it, oo, want_ranks=False)
+            runs = sorted((gg.pagerank_delta(it, oo, want_ranks=False) for _ in range(3)), key=lambda r: r.ms)
+            prd[name] = {"ms": round(runs[1].ms, 3), "rounds": runs[1].iters}
+        prd["blocked_vs_push"] = round(prd["push"]["ms"] / prd["blocked"]["ms"], 2)
+        pr["pagerank_delta"] = prd
+        log(f"pagerank_delta push {prd['push']['ms']:.1f} ms, blocked {prd['blocked']['ms']:.1f} ms "
+            f"({prd['blocked_vs_push']}x, {prd['blocked']['rounds']} rounds)")
         # BFS from GAP-picked sources: push-only vs autotuned RADAR.
         srcs = hg.pick_sources(args.bfs_sources)
         bt, _ = sg.autotune("bfs", sources=new_ids[srcs[:4]], base=rg.opts("push"))

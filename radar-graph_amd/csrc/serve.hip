@@ -35,13 +35,13 @@ namespace rg {
 
 constexpr int kSumBlocks = 512;
 constexpr int kBcBatch = 64;   // BC queries per batch (one word of source bits)
-// Fewest waiting queries of a kind worth a batch before any run has been
-// timed, and the floor of the learned threshold: a batch of one or two
-// costs more than as many single-source runs, three about as much or less
-// on small graphs (symmetric R-MAT of 2^22 vertices: a BFS batch of 1 / 2 /
-// 3 takes 2.2 / 2.5 / 2.9 ms against 1.1 ms per bottom-up single-source
-// query; a BC batch of one ~1.7x a single-source run, of two ~2.3x).
-constexpr int kBatchMin = 3;
+// Batch widths timed before the clock starts, per batched kind.  The cost
+// of a bit-parallel batch is far from linear in its width: on a symmetric
+// R-MAT of 2^26 vertices a BFS batch of 1 / 8 / 64 / 256 sources takes
+// 17 / 32 / 37 / 120 ms against 4 ms per single-source pull query, a BC
+// batch of 1 / 8 / 64 takes 99 / 273 / 489 ms against 22 ms per
+// single-source query.
+constexpr int kCalWidths[] = {1, 8, 64, 256};
 
 // Weighted sum of the finite entries (weights w(v) in [0,1) from a fixed
 // hash of v) and count of the non-finite ones: per-block partial sums in a
@@ -110,21 +110,27 @@ struct Shared {
   std::chrono::steady_clock::time_point t0;
   int err = RG_OK;
   std::string msg;
-  // Running averages of the time of one-query work items and of batches,
-  // per kind.  Every query of a batch completes when the batch does, so k
-  // waiting queries are batched only if the batch ends before the last of
-  // them would as single runs spread over the slots: k >= slots * batch /
-  // single (the ratio grows with the graph: a BC batch of three is cheaper
-  // than three single runs on a 2^22-vertex R-MAT, not on a 2^26-vertex one).
-  double single_ms[3] = {0.0, 0.0, 0.0}, batch_ms[3] = {0.0, 0.0, 0.0};
-  int nslots = 1;
-  size_t min_batch(int kind) const {
-    if (single_ms[kind] <= 0.0 || batch_ms[kind] <= 0.0) return (size_t)kBatchMin;
-    return (size_t)std::max<double>(kBatchMin, std::ceil(nslots * batch_ms[kind] / single_ms[kind]));
+  // Time of one single-source run and of batches of the calibrated widths,
+  // per kind, measured alone before the clock starts.  k waiting queries of
+  // a kind are batched when the batch ends no later than the last of them
+  // would as single-source runs: cost(k) <= k * single.  The runs are bound
+  // by device memory bandwidth, so slots running side by side share it and
+  // do not change the comparison.
+  double single_ms[3] = {0.0, 0.0, 0.0};
+  std::vector<std::pair<int, double>> curve[3];   // (width, ms), widths ascending
+  // Batch time at width k, linear between the timed widths and beyond the
+  // last two.
+  double batch_cost(int kind, size_t k) const {
+    const auto& c = curve[kind];
+    if (c.empty()) return 0.0;
+    if (c.size() == 1) return c[0].second * (double)k / (double)c[0].first;
+    size_t i = 1;
+    while (i + 1 < c.size() && (size_t)c[i].first < k) ++i;
+    const double x0 = c[i - 1].first, y0 = c[i - 1].second, x1 = c[i].first, y1 = c[i].second;
+    return y0 + (y1 - y0) * ((double)k - x0) / (x1 - x0);
   }
-  void record(int kind, size_t k, double ms) {
-    double& e = k == 1 ? single_ms[kind] : batch_ms[kind];
-    e = e > 0.0 ? 0.75 * e + 0.25 * ms : ms;
+  bool batch_pays(int kind, size_t k) const {
+    return k >= 2 && batch_cost(kind, k) <= (double)k * single_ms[kind];
   }
   double ms_now() const {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -155,13 +161,14 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
       }
     if (best >= 0) {
       size_t take = 1;
-      // BFS: a batch once enough queries wait that it answers them sooner
-      // than single-source runs on free slots would, else one query.
-      if (best == RG_Q_BFS && !c.one_at_a_time && S.pending[best].size() >= S.min_batch(best))
-        take = std::min<size_t>(S.pending[best].size(), (size_t)c.max_batch);
-      // BC likewise, in batches of up to 64.
-      if (best == RG_Q_BC && !c.one_at_a_time && S.pending[best].size() >= S.min_batch(best))
-        take = std::min<size_t>(S.pending[best].size(), (size_t)std::min(c.max_batch, kBcBatch));
+      // BFS and BC: every waiting query (up to the batch limit) as one batch
+      // when that answers them sooner than single-source runs would, else
+      // the oldest alone.
+      if ((best == RG_Q_BFS || best == RG_Q_BC) && !c.one_at_a_time) {
+        const size_t cap = best == RG_Q_BFS ? (size_t)c.max_batch : (size_t)std::min(c.max_batch, kBcBatch);
+        const size_t k = std::min(S.pending[best].size(), cap);
+        if (S.batch_pays(best, k)) take = k;
+      }
       for (size_t i = 0; i < take; ++i) {
         work.push_back(S.pending[best].front());
         S.pending[best].pop_front();
@@ -234,44 +241,53 @@ int run_work(Slot& sl, const rg_serve_config& c, const rg_query* q, int kind, co
   return RG_OK;
 }
 
-// Before the clock starts: the time of one single-source run and of a
-// one-query batch of each batched kind in the trace (answers discarded), so
-// the batching threshold holds from the first query on.
+// Before the clock starts: the time of one single-source run and of
+// batches of the calibrated widths (up to the batch limit) of each batched
+// kind in the trace, on the kind's own sources (repeated when the trace
+// holds fewer); answers are discarded.  The single-source run and the
+// first batch go once untimed first (scratch set-up).
 int calibrate(Shared& S, Slot& sl, const rg_serve_config& c, const rg_query* q, int64_t nq) {
   using clock = std::chrono::steady_clock;
   rg_graph* g = sl.view;
   rg_opts qo = c.opts;
   if (qo.variant == RG_PULL) qo.variant = RG_PUSH;
   for (int kind : {(int)RG_Q_BFS, (int)RG_Q_BC}) {
-    int64_t first = -1;
-    for (int64_t i = 0; i < nq && first < 0; ++i)
-      if (q[i].kind == kind) first = i;
-    if (first < 0) continue;
-    const uint32_t src = q[first].source;
-    double ms = 0.0;
-    auto t = clock::now();
-    for (int rep = 0; rep < 2; ++rep) {   // the second run is timed (the first sets up scratch)
-      t = clock::now();
+    std::vector<uint32_t> srcs;
+    for (int64_t i = 0; i < nq; ++i)
+      if (q[i].kind == kind) srcs.push_back(q[i].source);
+    if (srcs.empty()) continue;
+    const int cap = kind == RG_Q_BFS ? c.max_batch : std::min(c.max_batch, kBcBatch);
+    auto single = [&]() -> int {
+      double ms = 0.0;
       if (kind == RG_Q_BFS) {
         int64_t it = 0, reached = 0, edges = 0;
-        RG_TRY(bfs_push(g, src, &c.bfs_opts, nullptr, &it, &reached, &edges, &ms, nullptr, nullptr));
-      } else {
-        const bool tr = g->transposed;
-        int64_t it = 0;
-        RG_TRY(bc_push(g, &src, 1, &qo, nullptr, &it, &ms));
-        if (g->transposed != tr) RG_TRY(rg_graph_transpose(g));
+        return bfs_push(g, srcs[0], &c.bfs_opts, nullptr, &it, &reached, &edges, &ms, nullptr, nullptr);
       }
-    }
+      const bool tr = g->transposed;
+      int64_t it = 0;
+      RG_TRY(bc_push(g, &srcs[0], 1, &qo, nullptr, &it, &ms));
+      return g->transposed != tr ? rg_graph_transpose(g) : RG_OK;
+    };
+    RG_TRY(single());
+    auto t = clock::now();
+    RG_TRY(single());
     S.single_ms[kind] = std::chrono::duration<double, std::milli>(clock::now() - t).count();
-    t = clock::now();
-    if (kind == RG_Q_BFS) {
-      rg_bfs_summary sum;
-      RG_TRY(bfs_batch(g, &src, 1, &qo, nullptr, &sum, &ms));
-    } else {
-      rg_bc_summary sum;
-      RG_TRY(bc_batch(g, &src, 1, &qo, nullptr, &sum, &ms));
+    for (int w : kCalWidths) {
+      const int k = std::min(w, cap);
+      if (!S.curve[kind].empty() && S.curve[kind].back().first >= k) break;
+      std::vector<uint32_t> b(k);
+      for (int i = 0; i < k; ++i) b[i] = srcs[i % srcs.size()];
+      std::vector<rg_bfs_summary> bs(k);
+      std::vector<rg_bc_summary> cs(k);
+      double ms = 0.0;
+      const int reps = S.curve[kind].empty() ? 2 : 1;   // batch state is reserved for every width
+      for (int rep = 0; rep < reps; ++rep) {
+        t = clock::now();
+        if (kind == RG_Q_BFS) RG_TRY(bfs_batch(g, b.data(), k, &qo, nullptr, bs.data(), &ms));
+        else RG_TRY(bc_batch(g, b.data(), k, &qo, nullptr, cs.data(), &ms));
+      }
+      S.curve[kind].emplace_back(k, std::chrono::duration<double, std::milli>(clock::now() - t).count());
     }
-    S.batch_ms[kind] = std::chrono::duration<double, std::milli>(clock::now() - t).count();
   }
   return RG_OK;
 }
@@ -365,7 +381,6 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
   }
   if (rc == RG_OK) rc = hipDeviceSynchronize() == hipSuccess ? RG_OK : set_error(RG_EDEVICE, "rg_serve: set-up");
   rg::Shared S;
-  S.nslots = nslots;
   S.q = queries;
   S.nq = nq;
   S.order.resize(nq);
@@ -393,10 +408,6 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
           const double start = S.ms_now();
           int r = rg::run_work(slots[i], c, queries, kind, work, results);
           const double end = S.ms_now();
-          {
-            std::lock_guard<std::mutex> lk(S.mu);
-            S.record(kind, work.size(), end - start);
-          }
           if (r != RG_OK) {
             std::lock_guard<std::mutex> lk(S.mu);
             if (S.err == RG_OK) {

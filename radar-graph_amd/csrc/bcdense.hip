@@ -80,6 +80,32 @@ __device__ __forceinline__ void bc_finalize(const BcDenseArgs& a, WaveQueue& q, 
   }
 }
 
+// kDenseArcs arcs of a unit per thread and step [e0 + j * THREADS]: their
+// loads (arc, then the source's frontier bit word) issued together before
+// any is tested.  ok[j]: the arc exists, its destination is not skipped and
+// its source is on the frontier.
+constexpr int kDenseArcs = 4;
+
+template <int THREADS>
+__device__ __forceinline__ void load_arcs(const BcDenseArgs& a, uint64_t e0, uint64_t hi,
+                                          const unsigned long long* skip, uint32_t* s, uint32_t* l, bool* ok) {
+#pragma unroll
+  for (int j = 0; j < kDenseArcs; ++j) {
+    const uint64_t e = e0 + (uint64_t)j * THREADS;
+    ok[j] = e < hi;
+    s[j] = ok[j] ? __builtin_nontemporal_load(a.src + e) : 0u;
+    l[j] = ok[j] ? (uint32_t)__builtin_nontemporal_load(a.vl + e) : 0u;
+  }
+  unsigned long long act[kDenseArcs];
+#pragma unroll
+  for (int j = 0; j < kDenseArcs; ++j) {
+    ok[j] = ok[j] && !((skip[l[j] >> 6] >> (l[j] & 63)) & 1);
+    act[j] = ok[j] ? a.active[s[j] >> 6] : 0ull;
+  }
+#pragma unroll
+  for (int j = 0; j < kDenseArcs; ++j) ok[j] = ok[j] && ((act[j] >> (s[j] & 63)) & 1);
+}
+
 template <bool FWD, int VB, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
   constexpr int BV = 1 << VB;
@@ -103,12 +129,17 @@ __global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
   // of a split block reads no partial of a visited destination).
   if (block_done<BV>(skip)) return;
   const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
-  for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
-    const uint32_t s = __builtin_nontemporal_load(a.src + e);
-    const uint32_t l = __builtin_nontemporal_load(a.vl + e);
-    if ((skip[l >> 6] >> (l & 63)) & 1) continue;
-    if (!((a.active[s >> 6] >> (s & 63)) & 1)) continue;
-    atomicAdd(&acc[l], a.val[s]);
+  for (uint64_t e0 = lo + threadIdx.x; e0 < hi; e0 += (uint64_t)THREADS * kDenseArcs) {
+    uint32_t s[kDenseArcs], l[kDenseArcs];
+    bool ok[kDenseArcs];
+    load_arcs<THREADS>(a, e0, hi, skip, s, l, ok);
+    // Each value is loaded next to its add.  Issuing the four value loads
+    // ahead of the four adds gave dependencies that differed from run to
+    // run on a 2^10-vertex R-MAT (one workgroup per level; cause not found),
+    // while this form and the one-arc loop matched the oracle on every run.
+#pragma unroll
+    for (int j = 0; j < kDenseArcs; ++j)
+      if (ok[j]) atomicAdd(&acc[l[j]], a.val[s[j]]);
   }
   __syncthreads();
   if (!single) {
@@ -176,13 +207,13 @@ __global__ __launch_bounds__(THREADS) void k_bfs_dense(BcDenseArgs a, uint32_t* 
   __syncthreads();
   if (block_done<BV>(skip)) return;
   const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
-  for (uint64_t e = lo + threadIdx.x; e < hi; e += THREADS) {
-    const uint32_t s = __builtin_nontemporal_load(a.src + e);
-    const uint32_t l = __builtin_nontemporal_load(a.vl + e);
-    if ((skip[l >> 6] >> (l & 63)) & 1) continue;
-    if (cand[l] <= s) continue;
-    if (!((a.active[s >> 6] >> (s & 63)) & 1)) continue;
-    atomicMin(&cand[l], s);
+  for (uint64_t e0 = lo + threadIdx.x; e0 < hi; e0 += (uint64_t)THREADS * kDenseArcs) {
+    uint32_t s[kDenseArcs], l[kDenseArcs];
+    bool ok[kDenseArcs];
+    load_arcs<THREADS>(a, e0, hi, skip, s, l, ok);
+#pragma unroll
+    for (int j = 0; j < kDenseArcs; ++j)
+      if (ok[j] && cand[l[j]] > s[j]) atomicMin(&cand[l[j]], s[j]);
   }
   __syncthreads();
   if (!single) {

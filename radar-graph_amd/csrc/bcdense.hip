@@ -294,13 +294,15 @@ int in_edges_end(rg_graph* g, int64_t* end) {
 // them per step): an unvisited light vertex takes its first in-neighbour on
 // the frontier as parent.  The kPullU vertices' loads -- parent, offsets,
 // first in-neighbour and its frontier bit -- are issued together; a vertex
-// whose first in-neighbour is not on the frontier then scans on alone.
+// whose first in-neighbour is not on the frontier then scans on alone,
+// kPullScan in-neighbours and their frontier words loaded per step.
 // (Handing long scans to groups of lanes measured slower at every split
 // point: fewer scans in flight.)  Every word of the next frontier's bitmap
 // `next` below vertex `nn` is written (a ballot per wave), so the next pull
 // level reads it as it is; vertices at or above nn have no in-edges.  Heavy
 // vertices are queued for k_bfs_pull_heavy, which ORs in their bits.
 constexpr int kPullU = 4;
+constexpr int kPullScan = 4;   // in-neighbours tested per step of a long scan
 
 __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, const uint32_t* in_nbr,
                                                         const uint64_t* off, int64_t nn,
@@ -350,10 +352,21 @@ __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, 
 #pragma unroll
     for (int k = 0; k < kPullU; ++k) {
       if (cand[k] && !join[k]) {
-        for (uint64_t j = b[k] + 1; j < e[k]; ++j) {
-          const uint32_t s = in_nbr[j];
-          if ((active[s >> 6] >> (s & 63)) & 1) {
-            parents[v[k]] = s;
+        // kPullScan in-neighbours per step, their loads issued together;
+        // the first of them on the frontier (in list order) is the parent.
+        for (uint64_t j = b[k] + 1; j < e[k]; j += kPullScan) {
+          uint32_t s[kPullScan];
+          unsigned long long w[kPullScan];
+#pragma unroll
+          for (int t = 0; t < kPullScan; ++t) s[t] = j + t < e[k] ? in_nbr[j + t] : 0u;
+#pragma unroll
+          for (int t = 0; t < kPullScan; ++t) w[t] = j + t < e[k] ? active[s[t] >> 6] : 0ull;
+          int hit = kPullScan;
+#pragma unroll
+          for (int t = kPullScan - 1; t >= 0; --t)
+            if ((w[t] >> (s[t] & 63)) & 1) hit = t;
+          if (hit < kPullScan) {
+            parents[v[k]] = s[hit];
             join[k] = true;
             break;
           }

@@ -338,10 +338,12 @@ def main():
     log(f"measured stream copy: {stream_gbs:.0f} GB/s (read + write)")
     dg = rg.DeviceGraph(ctx, hg)
     t0 = time.time()
-    sg, new_ids = dg.degree_sort(by_outdeg=False, sort_neighbors=False)
+    # Strict DegreeSorting (neighbour lists sorted by new id, hubs first):
+    # a bottom-up BFS level finds a frontier parent in fewer probes.
+    sg, new_ids = dg.degree_sort(by_outdeg=False, sort_neighbors=True)
     ctx.sync()
     t_sort = time.time() - t0
-    log(f"degree sort (GPU): {t_sort * 1e3:.1f} ms")
+    log(f"degree sort (GPU, sorted lists): {t_sort * 1e3:.1f} ms")
     it = args.iters
 
     # RADAR PageRank on the engine the autotuner picks for this graph.

@@ -275,6 +275,42 @@ __global__ void k_last_in_edges(const uint64_t* in_off, int64_t n, unsigned long
 }
 
 // 1 + the last vertex with in-edges in g's current orientation (cached).
+__global__ void k_first_in(const uint64_t* in_off, const uint32_t* in_nbr, int64_t n, uint32_t* first) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = in_off[v];
+    first[v] = in_off[v + 1] > b ? in_nbr[b] : kNone;
+  }
+}
+
+}  // namespace
+
+// The first in-neighbour of every vertex in the current orientation, built
+// once per orientation (a view shares its base's array when the base has
+// one for the same in-edges).
+int in_first_neighbours(rg_graph* g, const uint32_t** out) {
+  if (g->first_for != g->in_nbr || !g->first_in) {
+    const rg_graph* base = g->view ? g->base : nullptr;
+    const bool shared = base && g->first_in && g->first_in == base->first_in;
+    if (g->first_in && !shared) (void)hipFree(g->first_in);
+    g->first_in = nullptr;
+    g->first_for = nullptr;
+    if (base && base->first_in && base->first_for == g->in_nbr) {
+      g->first_in = base->first_in;
+    } else {
+      rg_ctx* ctx = g->ctx;
+      RG_HIP(hipMalloc(&g->first_in, sizeof(uint32_t) * std::max<int64_t>(g->n, 1)));
+      hipLaunchKernelGGL(k_first_in, grid_for(ctx, g->n), 256, 0, ctx->stream, g->in_off, g->in_nbr, g->n,
+                         g->first_in);
+      RG_HIP(hipGetLastError());
+    }
+    g->first_for = g->in_nbr;
+  }
+  *out = g->first_in;
+  return RG_OK;
+}
+
+namespace {
+
 int in_edges_end(rg_graph* g, int64_t* end) {
   if (g->nz_for != g->in_off) {
     rg_ctx* ctx = g->ctx;
@@ -293,7 +329,9 @@ int in_edges_end(rg_graph* g, int64_t* end) {
 // Thread per vertex (a wave holds 64 aligned vertices, kPullU groups of
 // them per step): an unvisited light vertex takes its first in-neighbour on
 // the frontier as parent.  The kPullU vertices' loads -- parent, offsets,
-// first in-neighbour and its frontier bit -- are issued together; a vertex
+// first in-neighbour (from the per-vertex copy first_in, read coalesced
+// rather than one in-list line per vertex) and its frontier bit -- are
+// issued together; a vertex
 // whose first in-neighbour is not on the frontier then scans on alone,
 // kPullScan in-neighbours and their frontier words loaded per step.
 // (Handing long scans to groups of lanes measured slower at every split
@@ -305,7 +343,7 @@ constexpr int kPullU = 4;
 constexpr int kPullScan = 4;   // in-neighbours tested per step of a long scan
 
 __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, const uint32_t* in_nbr,
-                                                        const uint64_t* off, int64_t nn,
+                                                        const uint32_t* first_in, const uint64_t* off, int64_t nn,
                                                         const unsigned long long* active, uint32_t* parents,
                                                         uint32_t* heavy, int64_t* heavy_count, int heavy_deg,
                                                         uint32_t* out_ids, int64_t* out_info,
@@ -340,7 +378,7 @@ __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, 
         cand[k] = false;
       }
       cand[k] = cand[k] && e[k] > b[k];
-      s0[k] = cand[k] ? in_nbr[b[k]] : 0u;
+      s0[k] = cand[k] ? first_in[v[k]] : 0u;   // = in_nbr[b[k]]
     }
 #pragma unroll
     for (int k = 0; k < kPullU; ++k) {
@@ -573,6 +611,8 @@ int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* pa
   // and their bitmap words stay zero (cleared when a chain starts).
   int64_t nn = n;
   RG_TRY(in_edges_end(g, &nn));
+  const uint32_t* first_in = nullptr;
+  RG_TRY(in_first_neighbours(g, &first_in));
   const int64_t wnz = (nn + 63) / 64;
   if (!chained && nwords > wnz) {
     RG_TRY(fill<unsigned long long>(ctx, full[0] + wnz, nwords - wnz, 0ull));
@@ -582,8 +622,8 @@ int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* pa
   if (!chained) hipLaunchKernelGGL(k_set_bits, grid, 256, 0, ctx->stream, ids, count, bits);
   hipEvent_t* ev = timing_begin(ctx);
   const int vgrid = (int)std::min<int64_t>((nn + 256 * kPullU - 1) / (256 * kPullU), (int64_t)ctx->num_cus * 8);
-  hipLaunchKernelGGL(k_bfs_pull_light, std::max(vgrid, 1), 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off, nn,
-                     active, parents, heavy, hcount, heavy_deg, out_ids, out_info, next);
+  hipLaunchKernelGGL(k_bfs_pull_light, std::max(vgrid, 1), 256, 0, ctx->stream, g->in_off, g->in_nbr, first_in,
+                     g->off, nn, active, parents, heavy, hcount, heavy_deg, out_ids, out_info, next);
   hipLaunchKernelGGL(k_bfs_pull_heavy, ctx->num_cus * 4, 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off, active,
                      parents, heavy, hcount, out_ids, out_info, next);
   if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));

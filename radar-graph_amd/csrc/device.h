@@ -124,6 +124,11 @@ struct rg_graph {
   // are nz_for (the bound of bottom-up scans).
   const uint64_t* nz_for = nullptr;
   int64_t nz_end = 0;
+  // First in-neighbour of every vertex (kNone without in-edges) in the
+  // orientation whose in-neighbour array is first_for: the first probe of a
+  // bottom-up scan, read coalesced instead of one list line per vertex.
+  uint32_t* first_in = nullptr;
+  const uint32_t* first_for = nullptr;
   // A view shares the device arrays and cached layouts of `base` (bound to
   // another context, e.g. one serving slot) and frees only what it builds.
   bool view = false;

@@ -115,6 +115,9 @@ int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* p
 // frontier is then taken from ids).  *next_bits (nullable) receives the
 // next frontier's complete bitmap, valid until the next dense call on the
 // context.
+// First in-neighbour of every vertex of the current orientation (the pull
+// BFS's first probe), built on first use and kept with the graph.
+int in_first_neighbours(rg_graph* g, const uint32_t** out);
 int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, int heavy_deg,
                    uint32_t* out_ids, int64_t* out_info, const unsigned long long* cur_bits = nullptr,
                    const unsigned long long** next_bits = nullptr);

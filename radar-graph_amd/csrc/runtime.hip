@@ -308,6 +308,7 @@ int rg_graph_free(rg_graph* g) {
   if (own(g->hub_ids, b ? b->hub_ids : nullptr)) (void)hipFree(g->hub_ids);
   if (own(g->hub_slot_of, b ? b->hub_slot_of : nullptr)) (void)hipFree(g->hub_slot_of);
   if (own(g->heavy_items, b ? b->heavy_items : nullptr)) (void)hipFree(g->heavy_items);
+  if (own(g->first_in, b ? b->first_in : nullptr)) (void)hipFree(g->first_in);
   if (own(g->blocked, b ? b->blocked : nullptr)) rg::blocked_free(g->blocked);
   if (own(g->blocked_ms, b ? b->blocked_ms : nullptr)) rg::blocked_free(g->blocked_ms);
   if (own(g->blocked_ms_t, b ? b->blocked_ms_t : nullptr)) rg::blocked_free(g->blocked_ms_t);

@@ -344,6 +344,10 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
   }
   const bool bc_blocked = c.opts.variant == RG_RADAR || c.opts.variant == RG_BLOCKED;
   if (((any_bfs || any_bc) && !c.one_at_a_time) || (any_bc && bc_blocked)) RG_TRY(rg::bfs_batch_prepare(g));
+  if (any_bfs && c.bfs_opts.variant == RG_PULL) {
+    const uint32_t* first = nullptr;   // shared by the slots' views
+    RG_TRY(rg::in_first_neighbours(g, &first));
+  }
   if (any_pr && c.opts.variant == RG_BLOCKED) RG_TRY(rg::pagerank_prepare(g, &c.opts));
   if (any_pr && c.opts.variant != RG_BLOCKED && c.opts.variant != RG_PULL)
     RG_TRY(rg::static_heavy_list(g, c.opts.heavy_degree > 0 ? c.opts.heavy_degree : 1024));

@@ -297,7 +297,10 @@ int rg_bc_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_opt
 
 /* Triangle counting (PNO GAP/src/tc_local.cc:48-86): u > v > w ordered merge
  * counting over a symmetric graph with sorted lists; total and per-vertex
- * counts (each triangle adds 1 to all three corners).  per_vertex nullable. */
+ * counts (each triangle adds 1 to all three corners).  per_vertex nullable.
+ * Totals only: the leading vertices of degree >= o->heavy_degree (0 or a
+ * null o: 512; at most the first 2^19 ids) are counted from their lower
+ * side with an LDS bitmap, the rest per upper vertex. */
 int rg_tc(rg_graph* g, const rg_opts* o, uint64_t* total_out, uint64_t* per_vertex,
           double* ms_out);
 

@@ -10,20 +10,25 @@
 //
 // Mapping: one wavefront per vertex u.  With per-vertex counts its lanes take
 // the arcs (u, v) of N+(u) and each merges N+(u) with N+(v).  For the total
-// alone, arcs (u, v) with v below 2^19 are counted from v's side
+// alone, arcs (u, v) with v below a split are counted from v's side
 // (k_tc_bits): a workgroup puts N+(v) in an LDS bitmap and every u of N-(v)
 // (neighbours above v) tests its neighbours below v against it -- the
 // shorter list scanned, the longer one probed in one step; on a
 // degree-relabelled R-MAT of 2^22 vertices this is 4x fewer checks than
-// scanning N+(v) for each u.  The remaining arcs (v >= 2^19, short lists)
-// are counted from u's side with the (v, w) work flattened over the lanes
-// (k_tc_flat).
+// scanning N+(v) for each u.  The remaining arcs are counted from u's side
+// with the (v, w) work flattened over the lanes (k_tc_flat).  The split is
+// the first vertex of degree below kTcBitsDeg (at most 2^19, the bitmap's
+// range), found on the device: each v of k_tc_bits costs a workgroup-wide
+// set-up, which pays for high-degree v only (degree-relabelled R-MAT of
+// 2^22 vertices: 38.7 ms with the split at 2^19, 29.3 ms at 2^15, where
+// degrees cross ~950).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "device.h"
 #include "engine.h"
+#include "engine_host.h"
 
 namespace rg {
 namespace {
@@ -147,7 +152,8 @@ __device__ __forceinline__ unsigned long long tc_vertex(const uint64_t* off, con
 
 // Arcs (u, v) with v >= vlo only (the others are k_tc_bits').
 __global__ __launch_bounds__(256) void k_tc_flat(const uint64_t* off, const uint32_t* nbr, int64_t n,
-                                                 unsigned long long* total, uint32_t vlo) {
+                                                 unsigned long long* total, const unsigned long long* split) {
+  const uint32_t vlo = (uint32_t)*split;
   constexpr int WPB = 256 / kWave;
   __shared__ uint32_t s_nu[WPB][kTcLds];
   __shared__ uint32_t s_pre[WPB][kWave + 1];
@@ -182,15 +188,30 @@ __global__ __launch_bounds__(256) void k_tc_flat(const uint64_t* off, const uint
 }
 
 constexpr int kTcBitsLog = 19;
-constexpr int64_t kTcBitsV = int64_t(1) << kTcBitsLog;   // vertices v counted by k_tc_bits
+constexpr int64_t kTcBitsV = int64_t(1) << kTcBitsLog;   // bitmap range: vertices v counted by k_tc_bits
 constexpr int kTcBitsThreads = 1024;
+constexpr int kTcBitsDeg = 512;   // default degree from which v is counted by k_tc_bits
+
+// split = the first vertex below vcap of degree < dmin (vcap if none).
+__global__ void k_tc_split(const uint64_t* off, int64_t vcap, uint64_t dmin, unsigned long long* split) {
+  unsigned long long first = ~0ull;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < vcap; v += (int64_t)gridDim.x * blockDim.x)
+    if (off[v + 1] - off[v] < dmin) {
+      first = (unsigned long long)v;
+      break;
+    }
+  for (int o = 32; o > 0; o >>= 1) first = min(first, (unsigned long long)__shfl_xor(first, o));
+  if (__lane_id() == 0 && first != ~0ull) atomicMin(split, first);
+}
 
 // Persistent workgroups taking vertices v < vmax from a counter: N+(v) as
 // bits of an LDS bitmap over [0, 2^19) (set, used, then cleared again), and
 // for each u of N-(v), 64 per wave, the neighbours w < v of u (the prefix of
 // N(u) below v) tested against it, the (u, w) work flattened over the lanes.
-__global__ __launch_bounds__(kTcBitsThreads) void k_tc_bits(const uint64_t* off, const uint32_t* nbr, int64_t vmax,
+__global__ __launch_bounds__(kTcBitsThreads) void k_tc_bits(const uint64_t* off, const uint32_t* nbr,
+                                                            const unsigned long long* split,
                                                             unsigned long long* next_v, unsigned long long* total) {
+  const int64_t vmax = (int64_t)*split;
   constexpr int WPB = kTcBitsThreads / kWave;
   __shared__ uint32_t bits[kTcBitsV / 32];
   __shared__ uint32_t s_pre[WPB][kWave + 1];
@@ -269,7 +290,6 @@ using rg::set_error;
 
 extern "C" int rg_tc(rg_graph* g, const rg_opts* o, uint64_t* total_out, uint64_t* per_vertex,
                      double* ms_out) {
-  (void)o;
   if (!g || !total_out) return set_error(RG_EINVAL, "rg_tc: null argument");
   if (!g->symmetric)
     return set_error(RG_EUNSUPPORTED, "Input graph is directed but tc requires undirected");
@@ -292,12 +312,16 @@ extern "C" int rg_tc(rg_graph* g, const rg_opts* o, uint64_t* total_out, uint64_
   if (per_vertex)
     hipLaunchKernelGGL(rg::k_tc, grid, 256, 0, s, g->off, g->nbr, n, d_total, tri.as<unsigned long long>());
   else {
-    const int64_t vmax = std::min<int64_t>(n, rg::kTcBitsV);
+    const int64_t vcap = std::min<int64_t>(n, rg::kTcBitsV);
+    const uint64_t dmin = o && o->heavy_degree > 0 ? (uint64_t)o->heavy_degree : (uint64_t)rg::kTcBitsDeg;
     unsigned long long* next_v = reinterpret_cast<unsigned long long*>(ctx->counts.as<int64_t>() + 21);
+    unsigned long long* split = reinterpret_cast<unsigned long long*>(ctx->counts.as<int64_t>() + 22);
     RG_HIP(hipMemsetAsync(next_v, 0, sizeof(uint64_t), s));
-    hipLaunchKernelGGL(rg::k_tc_bits, ctx->num_cus * 2, rg::kTcBitsThreads, 0, s, g->off, g->nbr, vmax, next_v,
+    RG_TRY(rg::fill<unsigned long long>(ctx, split, 1, (unsigned long long)vcap));
+    hipLaunchKernelGGL(rg::k_tc_split, rg::grid_for(ctx, vcap), 256, 0, s, g->off, vcap, dmin, split);
+    hipLaunchKernelGGL(rg::k_tc_bits, ctx->num_cus * 2, rg::kTcBitsThreads, 0, s, g->off, g->nbr, split, next_v,
                        d_total);
-    if (n > vmax) hipLaunchKernelGGL(rg::k_tc_flat, grid, 256, 0, s, g->off, g->nbr, n, d_total, (uint32_t)vmax);
+    hipLaunchKernelGGL(rg::k_tc_flat, grid, 256, 0, s, g->off, g->nbr, n, d_total, split);
   }
   RG_HIP(hipGetLastError());
   RG_HIP(hipEventRecord(b, s));

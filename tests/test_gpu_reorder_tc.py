@@ -88,9 +88,11 @@ def test_tc_total_only(rg, orc, ctx, graphs, gname, relabel):
 
 
 def test_tc_cliques_across_the_bitmap_split(rg, orc, ctx):
-    # 2^18 disjoint 4-cliques on randomly permuted ids in [0, 2^20): cliques
-    # straddle the 2^19 boundary between the two totals kernels.  Each
-    # clique holds 4 triangles.
+    # 2^18 disjoint 4-cliques on randomly permuted ids in [0, 2^20).  With
+    # heavy_degree 1 every vertex below 2^19 (the bitmap's range) is counted
+    # by the bitmap kernel, so cliques straddle the boundary between the two
+    # totals kernels; with the default degree split (512) the flat kernel
+    # counts them all.  Each clique holds 4 triangles.
     n = 1 << 20
     perm = np.random.default_rng(5).permutation(n).astype(np.uint32).reshape(-1, 4)
     pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
@@ -98,10 +100,36 @@ def test_tc_cliques_across_the_bitmap_split(rg, orc, ctx):
     dst = np.concatenate([perm[:, b] for _, b in pairs])
     hg = rg.HostGraph.from_edges(n, src, dst, symmetric=True)
     dg = rg.DeviceGraph(ctx, hg)
-    tot, _ = dg.tc()
-    assert tot == 4 * perm.shape[0]
+    for o in (rg.opts(heavy_degree=1), None):
+        tot, _ = dg.tc(o)
+        assert tot == 4 * perm.shape[0]
     ptot, pv, _ = dg.tc(per_vertex=True)
-    assert ptot == tot and np.all(pv == 3)
+    assert ptot == 4 * perm.shape[0] and np.all(pv == 3)
+
+
+@pytest.mark.parametrize("dmin", [0, 2, 300, 600, 5000])
+def test_tc_degree_split_with_hubs(rg, orc, ctx, dmin):
+    # Hubs 0..7 of degree ~1000-1700 joined to each other and to random
+    # light vertices, which are joined among themselves: triangles hub-hub-
+    # light, hub-light-light and light-light-light, so the split (first
+    # vertex of degree < dmin; 0 = the default 512) cuts through them.
+    rng = np.random.default_rng(17)
+    n, hubs = 20000, 8
+    hs, hd = np.triu_indices(hubs, 1)
+    spokes_s = np.repeat(np.arange(hubs), 1000 + 100 * np.arange(hubs))
+    spokes_d = rng.integers(hubs, n, size=spokes_s.size)
+    light_s = rng.integers(hubs, n, size=60000)
+    light_d = rng.integers(hubs, n, size=60000)
+    src = np.concatenate([hs, spokes_s, light_s]).astype(np.uint32)
+    dst = np.concatenate([hd, spokes_d, light_d]).astype(np.uint32)
+    hg = rg.HostGraph.from_edges(n, src, dst, symmetric=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    dg, _ = dg.degree_sort(by_outdeg=True, sort_neighbors=True)
+    off, nb = dg.download()
+    want = orc.tc(off, nb)
+    assert want > 0
+    tot, _ = dg.tc(rg.opts(heavy_degree=dmin))
+    assert tot == want
 
 
 def test_tc_total_matches_per_vertex_path_rmat20(rg, ctx):

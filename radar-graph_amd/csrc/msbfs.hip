@@ -47,7 +47,12 @@ template <int W>
 constexpr int ms_threads() { return W == 1 ? 512 : 1024; }
 constexpr int kMsMaxSources = 256;
 constexpr int kMsLevelsPerSync = 4;  // levels launched between host checks
-constexpr int kMsArcs = 4;           // dense level: arcs per thread and step
+// Dense level: arcs per thread and step.  Eight for one word per vertex
+// (RMAT-26 batch of 64: 25.7 ms against 26.4 with four), four for two and
+// four words (eight measured slower for two; four words already fill the
+// register budget of a 1024-thread workgroup).
+template <int W>
+constexpr int ms_arcs() { return W == 1 ? 8 : 4; }
 
 enum { MS_SPARSE = 0, MS_DENSE = 1, MS_DONE = 2 };
 static_assert(MS_DONE == kMsDone, "level hooks test the mode against kMsDone");
@@ -332,36 +337,37 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
     bool all = true;
     for (int i = 0; i < BV / 64; ++i) all = all && full[i] == ~0ull;
     if (all) continue;
-    // kMsArcs arcs per thread and step, each stage's loads issued for all
+    // A = ms_arcs<W>() arcs per thread and step, each stage's loads issued for all
     // of them before any result is tested: the arc, its active bit word,
     // then the source's W frontier words (kept in flight together).
+    constexpr int A = ms_arcs<W>();
     const uint64_t lo = a.unit_lo[u], hi = a.unit_lo[u + 1];
-    for (uint64_t e0 = lo + threadIdx.x; e0 < hi; e0 += (uint64_t)THREADS * kMsArcs) {
-      uint32_t s[kMsArcs], l[kMsArcs];
-      bool ok[kMsArcs];
+    for (uint64_t e0 = lo + threadIdx.x; e0 < hi; e0 += (uint64_t)THREADS * A) {
+      uint32_t s[A], l[A];
+      bool ok[A];
 #pragma unroll
-      for (int j = 0; j < kMsArcs; ++j) {
+      for (int j = 0; j < A; ++j) {
         const uint64_t e = e0 + (uint64_t)j * THREADS;
         ok[j] = e < hi;
         s[j] = ok[j] ? ld_nt(a.src + e) : 0u;
         l[j] = ok[j] ? (uint32_t)ld_nt(a.vl + e) : 0u;
       }
-      ull act[kMsArcs];
+      ull act[A];
 #pragma unroll
-      for (int j = 0; j < kMsArcs; ++j) {
+      for (int j = 0; j < A; ++j) {
         ok[j] = ok[j] && !((full[l[j] >> 6] >> (l[j] & 63)) & 1);
         act[j] = ok[j] ? a.active_cur[s[j] >> 6] : 0ull;
       }
-      ull f[kMsArcs][W];
+      ull f[A][W];
 #pragma unroll
-      for (int j = 0; j < kMsArcs; ++j) {
+      for (int j = 0; j < A; ++j) {
         ok[j] = ok[j] && ((act[j] >> (s[j] & 63)) & 1);
         const ull* p = a.front_cur + (size_t)s[j] * W;
 #pragma unroll
         for (int w = 0; w < W; ++w) f[j][w] = ok[j] ? p[w] : 0ull;
       }
 #pragma unroll
-      for (int j = 0; j < kMsArcs; ++j)
+      for (int j = 0; j < A; ++j)
 #pragma unroll
         for (int w = 0; w < W; ++w)
           if (f[j][w]) atomicOr(&acc[l[j] * W + w], f[j][w]);

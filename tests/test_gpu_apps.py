@@ -64,6 +64,26 @@ def test_bfs(rg, orc, ctx, graphs, gname, variant):
         assert r.edges == int(np.diff(off)[lv >= 0].sum())
 
 
+@pytest.mark.parametrize("threshold", [0, -1])
+def test_pull_bfs_follows_transposes(rg, orc, ctx, graphs, threshold):
+    # The pull engine's first-probe array belongs to one orientation: after
+    # each transpose of a directed graph it is rebuilt from the new in-edges
+    # (threshold 0: every level bottom-up).
+    hg = graphs("rmat12d")
+    csr = {False: hg.csr(), True: hg.csr(in_edges=True)}
+    srcs = hg.pick_sources(3)
+    dg = rg.DeviceGraph(ctx, hg)
+    o = rg.opts("pull", threshold=threshold)
+    for transposed in (False, True, False, True):
+        off, nbr = csr[transposed]
+        for s in srcs:
+            r = dg.bfs(int(s), o)
+            lv, ecc = orc.bfs_levels(off, nbr, int(s))
+            assert orc.check_parents(off, nbr, int(s), lv, r.parents) == 0, (transposed, s)
+            assert r.iters == ecc + 1 and r.reached == int((lv >= 0).sum())
+        dg.transpose()
+
+
 @pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d", "rmat16s"])
 @pytest.mark.parametrize("variant", VARIANTS + ["blocked"])
 def test_radii(rg, orc, ctx, graphs, gname, variant):

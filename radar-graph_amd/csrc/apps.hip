@@ -729,6 +729,12 @@ __global__ void k_level_cs(const uint32_t* ids, const int64_t* count, uint32_t l
   if (__lane_id() == 0 && s) atomicAdd(out, s);
 }
 
+// Cost of a top-down arc (a random parent word) against a bottom-up step
+// (a sequential parent read or list entry) in the pull engine's choice to
+// stay bottom-up.  On a degree-sorted RMAT-26: 3.107 ms per source with the
+// rule off (or at 4, where it never fires), 3.07 at 16, 32 and 64.
+constexpr int64_t kPullStayArcCost = 16;
+
 int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
              int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev,
              uint64_t* checksum_out) {
@@ -779,7 +785,13 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
     const unsigned long long* cur_bits = pull_bits;
     pull_bits = nullptr;
-    if (pull && degsum > 0 && F->size + degsum > E.threshold)
+    // After a bottom-up level, stay bottom-up while that is cheaper than
+    // pushing: a top-down level gathers a random parent word per frontier
+    // arc (degsum of them), a bottom-up one reads every vertex's parent once
+    // and scans at most the lists of the unvisited vertices (m - edges arcs
+    // on a symmetric graph; an estimate on a directed one).
+    const bool stay = pull && cur_bits && degsum * kPullStayArcCost > g->n + (g->m - edges);
+    if (pull && degsum > 0 && (F->size + degsum > E.threshold || stay))
       RG_TRY(bfs_pull_level(g, F->ids, F->size, parents, o->heavy_degree, N->ids, info, cur_bits, &pull_bits));
     else if (blocked && degsum > 0 && F->size + degsum > E.threshold)
       RG_TRY(bfs_dense_level(g, F->ids, F->size, parents, N->ids, info));

@@ -356,12 +356,13 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
  * execute them concurrently, and with batching on, every free slot takes
  * all BFS queries that have arrived (up to max_batch) as one bit-parallel
  * batch (rg_bfs_batch) -- continuous batching: queries that arrive while
- * others run join the next batch on the next free slot, once three or more
- * wait (fewer run one per slot on the single-source engine with bfs_opts,
- * which answers one or two queries sooner than a batch does); BC queries
- * the same way in batches of up to 64 (rg_bc_batch) once three or more
- * wait, one per slot on the single-source engine otherwise.  PageRank queries run one
- * per slot.  The number of slots is capped by free HBM (less hbm_reserve)
+ * others run join the next batch on the next free slot, once a batch of
+ * the waiting queries ends no later than as many single-source runs would
+ * (a cost curve timed before the trace's clock starts: one single-source
+ * run and batches of 1, 8, 64 and 256 sources, interpolated); otherwise the
+ * oldest runs alone on the single-source engine with bfs_opts.  BC queries
+ * the same way in batches of up to 64 (rg_bc_batch).  PageRank queries run
+ * one per slot.  The number of slots is capped by free HBM (less hbm_reserve)
  * over a per-slot estimate.  one_at_a_time = 1 runs the same trace on one
  * slot, one query at a time, BFS on the single-source engine
  * (rg_bfs_query with bfs_opts), BC on rg_bc with one source: the baseline.

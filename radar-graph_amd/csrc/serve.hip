@@ -110,6 +110,12 @@ struct Shared {
   std::chrono::steady_clock::time_point t0;
   int err = RG_OK;
   std::string msg;
+  // A PageRank query runs alone: while one does, no slot starts other work.
+  // Both it and what would run beside it are bound by device memory
+  // bandwidth, so overlapping them gains next to no throughput and doubles
+  // the PageRank query's time (RMAT-26: ~150 ms beside other work against
+  // ~70 ms alone), which set the tail of mixed traces.
+  bool alone = false;
   // Time of one single-source run and of batches of the calibrated widths,
   // per kind, measured alone before the clock starts.  k waiting queries of
   // a kind are batched when the batch ends no later than the last of them
@@ -149,6 +155,10 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
       const int64_t i = S.order[S.next++];
       S.pending[S.q[i].kind].push_back(i);
     }
+    if (S.alone) {   // woken when the PageRank query ends
+      S.cv.wait(lk);
+      continue;
+    }
     int best = -1;
     double ba = 0.0;
     for (int k = 0; k < 3; ++k)
@@ -175,6 +185,7 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
       }
       *kind = best;
       *batch = S.batches++;
+      if (best == RG_Q_PAGERANK && !c.one_at_a_time) S.alone = true;
       bool more = false;
       for (auto& p : S.pending) more = more || !p.empty();
       if (more) S.cv.notify_all();
@@ -412,6 +423,11 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
           const double start = S.ms_now();
           int r = rg::run_work(slots[i], c, queries, kind, work, results);
           const double end = S.ms_now();
+          if (kind == RG_Q_PAGERANK) {
+            std::lock_guard<std::mutex> lk(S.mu);
+            S.alone = false;
+            S.cv.notify_all();
+          }
           if (r != RG_OK) {
             std::lock_guard<std::mutex> lk(S.mu);
             if (S.err == RG_OK) {

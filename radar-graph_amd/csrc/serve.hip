@@ -144,9 +144,12 @@ struct Shared {
 };
 
 // Picks the next work item: the kind whose oldest pending query arrived
-// first; all pending BFS queries (up to max_batch) when batching.  Returns
-// false when every query has been taken.
-bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64_t>& work, int64_t* batch) {
+// first; the waiting BFS or BC queries as one batch when that pays.  Waits
+// while a PageRank query runs.  taken_ms: the item's start on the trace's
+// clock, read under the lock (so no item taken after a PageRank query
+// starts before it).  Returns false when every query has been taken.
+bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64_t>& work, int64_t* batch,
+               double* taken_ms) {
   std::unique_lock<std::mutex> lk(S.mu);
   for (;;) {
     if (S.err != RG_OK) return false;
@@ -185,6 +188,7 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
       }
       *kind = best;
       *batch = S.batches++;
+      *taken_ms = S.ms_now();   // the item's start, under the lock
       if (best == RG_Q_PAGERANK && !c.one_at_a_time) S.alone = true;
       bool more = false;
       for (auto& p : S.pending) more = more || !p.empty();
@@ -419,8 +423,8 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
           int kind = 0;
           int64_t batch = 0;
           std::vector<int64_t> work;
-          if (!rg::take_work(S, c, &kind, work, &batch)) return;
-          const double start = S.ms_now();
+          double start = 0.0;
+          if (!rg::take_work(S, c, &kind, work, &batch, &start)) return;
           int r = rg::run_work(slots[i], c, queries, kind, work, results);
           const double end = S.ms_now();
           if (kind == RG_Q_PAGERANK) {

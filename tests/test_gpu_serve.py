@@ -127,3 +127,24 @@ def test_bc_queries_batched(rg, ctx, graphs):
             assert (x.iters, x.nonfinite) == (y.iters, y.nonfinite) and close(x.wsum, y.wsum), f"query {i}"
         else:
             assert x.bfs.astuple() == y.bfs.astuple()
+
+
+def test_pagerank_queries_run_alone(rg, ctx, graphs):
+    # While a PageRank query runs, no slot starts other work (what started
+    # before it may still finish beside it), and the answers are unchanged.
+    hg = graphs("rmat14s")
+    dg = rg.DeviceGraph(ctx, hg)
+    q = rg.poisson_workload(hg, 200, 0.0, mix=(0.6, 0.2, 0.2), seed=21)
+    res, st = dg.serve(q, rg.serve_config(slots=3))
+    assert st.slots == 3
+    items = {(r.slot, r.batch_id, r.kind, r.start_ms, r.end_ms) for r in res}
+    pr = [it for it in items if it[2] == rg.Q_PAGERANK]
+    assert len(pr) > 10
+    for _, b, _, s0, e0 in pr:
+        assert not any(other[1] != b and s0 < other[3] < e0 for other in items), b
+    alone, _ = dg.serve(q, rg.serve_config(one_at_a_time=True))
+    for x, y in zip(res, alone):
+        if x.kind == rg.Q_BFS:
+            assert x.bfs.astuple() == y.bfs.astuple()
+        else:
+            assert x.nonfinite == y.nonfinite and close(x.wsum, y.wsum)

@@ -462,16 +462,22 @@ static int cmp_u32_desc(const void* a, const void* b) {
 }
 
 int64_t orc_identify_hubs(int64_t n, const uint32_t* deg, int64_t num_hubs, uint8_t* is_hub) {
-  uint32_t* s = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
-  memcpy(s, deg, sizeof(uint32_t) * (size_t)n);
-  qsort(s, (size_t)n, sizeof(uint32_t), cmp_u32_desc);
-  uint32_t thresh = s[num_hubs];     /* hubdup IO.h:759 */
+  /* The reference reads degrees[numHubs] past the array when numHubs >= n
+   * (hubdup IO.h:759); here, as in the library, every vertex of positive
+   * degree is a hub then. */
+  uint32_t thresh = 0;
+  if (num_hubs < n) {
+    uint32_t* s = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+    memcpy(s, deg, sizeof(uint32_t) * (size_t)n);
+    qsort(s, (size_t)n, sizeof(uint32_t), cmp_u32_desc);
+    thresh = s[num_hubs];            /* hubdup IO.h:759 */
+    free(s);
+  }
   int64_t c = 0;
   for (int64_t v = 0; v < n; ++v) {
     is_hub[v] = deg[v] > thresh;     /* IO.h:779 */
     c += is_hub[v];
   }
-  free(s);
   return c;
 }
 

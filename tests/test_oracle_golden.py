@@ -284,3 +284,44 @@ def test_check_parents_catches_planted_errors(orc):
         bad = good.copy()
         bad[v] = p
         assert orc.check_parents(off, nbr, 0, lv, bad) == 1 + v, (v, p)
+
+
+def test_identify_hubs_closed_forms(orc):
+    # hubdup IO.h:759,779: hubs have in-degree above the num_hubs-th largest
+    # (0-based), ties at that value excluded.
+    deg = np.array([5, 5, 3, 3, 1, 0], np.uint32)
+    hub, c = orc.identify_hubs(deg, 2)      # threshold 3
+    assert c == 2 and hub.tolist() == [True, True, False, False, False, False]
+    hub, c = orc.identify_hubs(deg, 1)      # threshold 5: the tie leaves no hub
+    assert c == 0 and not hub.any()
+    hub, c = orc.identify_hubs(deg, 4)      # threshold 1
+    assert c == 4 and hub.tolist() == [True] * 4 + [False] * 2
+    hub, c = orc.identify_hubs(deg, 0)      # threshold 5
+    assert c == 0
+    # num_hubs >= n (out of range in the reference): every vertex of
+    # positive degree, as the library's selection does.
+    hub, c = orc.identify_hubs(deg, 6)
+    assert c == 5 and hub.tolist() == [True] * 5 + [False]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_identify_hubs_characterisation(orc, seed):
+    # For every k the hubs H are the unique set with: H an upper set of the
+    # degrees (every hub above every non-hub), |H| <= k, and the largest
+    # non-hub degree t held by more than k vertices counting those above
+    # it (so no larger upper set within k exists).
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 400))
+    deg = rng.zipf(1.8, size=n).clip(0, 5000).astype(np.uint32) * rng.integers(0, 2, size=n).astype(np.uint32)
+    for k in sorted({0, 1, 2, n // 3, n - 1, n, n + 5, int(rng.integers(0, n + 1))}):
+        hub, c = orc.identify_hubs(deg, k)
+        assert c == int(hub.sum())
+        if k >= n:
+            assert np.array_equal(hub, deg > 0)
+            continue
+        assert c <= k
+        if c:
+            assert deg[hub].min() > (deg[~hub].max() if (~hub).any() else -1)
+        if (~hub).any():
+            t = deg[~hub].max()
+            assert int((deg >= t).sum()) > k

@@ -52,6 +52,37 @@ struct PrOp {
   __device__ double hub_combine(double x, double y) const { return x + y; }
 };
 
+// PageRankDelta's sparse rounds: PR_F's add, and d joins the round's list
+// of touched vertices the first time it is reached (a per-vertex stamp of
+// the round), so the vertexFilter can run over those alone.  Hub copies in
+// LDS only: the per-XCD merge reports no joins.
+struct PrdTouchOp {
+  using SrcT = double;
+  using HubT = double;
+  static constexpr bool kOutput = true;
+  static constexpr bool kL2 = false;
+  const double* contrib;
+  double* acc;
+  uint32_t* stamp;
+  uint32_t tag;
+  __device__ bool skip() const { return false; }
+  __device__ double load_src(uint32_t s) const { return contrib[s]; }
+  __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool first(uint32_t d) const { return atomicExch(stamp + d, tag) != tag; }
+  __device__ bool update(uint32_t, double sv, uint32_t d) const {
+    unsafeAtomicAdd(acc + d, sv);
+    return first(d);
+  }
+  __device__ double hub_identity() const { return 0.0; }
+  __device__ void hub_update(double* slot, uint32_t, double sv, uint32_t) const { atomicAdd(slot, sv); }
+  __device__ bool hub_flush(uint32_t d, double v) const {
+    unsafeAtomicAdd(acc + d, v);
+    return first(d);
+  }
+  __device__ void l2_update(double*, double) const {}
+  __device__ double hub_combine(double x, double y) const { return x + y; }
+};
+
 // BFS_F (BFS.C:169-190): first CAS on Parents[d] wins and d joins the next
 // frontier.  Hub copies keep one candidate parent per hub per workgroup
 // (hubdup BFS.C:40-47, 51-69).
@@ -568,9 +599,14 @@ __global__ void k_prd_init(const uint64_t* off, int64_t n, double* p, double* de
 // for active s (0 for inactive ones, so an all-vertices pass adds nothing
 // for them).  Active vertices are appended to out_ids with their out-degree
 // sum.  nghSum comes from `ngh` or, after a blocked pass, from the unit
-// partials of v's block added in unit order.  Multiplies and adds are
+// partials of v's block added in unit order.  With `list`, only the
+// vertices list[0, *list_count) are visited: after a sparse round every
+// other vertex has nghSum 0, so Delta 0, p unchanged and contrib 0 (kept
+// so: the previous round's active vertices have their contrib cleared
+// first).  Multiplies and adds are
 // rounded separately, as the reference's host code computes them.
-__global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t n, double* p, double* delta,
+__global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t n, const uint32_t* list,
+                                                    const int64_t* list_count, double* p, double* delta,
                                                     double* ngh, double* contrib, const double* part,
                                                     const uint32_t* block_unit0, int vb, int first,
                                                     double damping, double added, double one_over_n,
@@ -585,10 +621,13 @@ __global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t
   WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
   double l1 = 0.0;
   const int64_t lmask = (int64_t(1) << vb) - 1;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t v = base + threadIdx.x;
+  const int64_t total = list ? *list_count : n;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = base + threadIdx.x;
     bool act = false;
-    if (v < n) {
+    int64_t v = i;
+    if (i < total) {
+      if (list) v = list[i];
       double ns;
       if (part) {
         ns = 0.0;
@@ -625,6 +664,12 @@ __global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_zero_at(const uint32_t* ids, int64_t count, double* x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[ids[i]] = 0.0;
 }
 
 __global__ void k_prd_l1(const double* partial, int nparts, double* out) {
@@ -664,7 +709,16 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
   double* l1_dev = partial + vgrid;
   int64_t* zero = ctx->counts.as<int64_t>() + 32;
   int64_t* info = ctx->counts.as<int64_t>() + 34;   // [count, out-degree sum]
+  int64_t* tinfo = ctx->counts.as<int64_t>() + 36;  // touched list of a sparse round
   RG_HIP(hipMemsetAsync(zero, 0, sizeof(int64_t), ctx->stream));
+  // Sparse rounds: the vertices their push reaches, and a per-vertex stamp
+  // of the last round that reached each (tags are rounds, from 2).
+  DevBuf &b_touch = ctx->scratch[8], &b_stamp = ctx->scratch[9];
+  RG_TRY(b_touch.ensure(sizeof(uint32_t) * n));
+  RG_TRY(b_stamp.ensure(sizeof(uint32_t) * n));
+  uint32_t* touched = b_touch.as<uint32_t>();
+  uint32_t* stamp = b_stamp.as<uint32_t>();
+  RG_TRY(fill<uint32_t>(ctx, stamp, n, 0u));
   Frontier fa, fb;
   RG_TRY(ctx_frontiers(ctx, n, fa, fb));
   const double damping = 0.85, epsilon = 0.0000001, epsilon2 = 0.01;
@@ -686,15 +740,28 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
     const double* part = nullptr;
     const uint32_t* bu0 = nullptr;
     int vb = 0;
-    if (blocked && (F->all || F->size + degsum > E.threshold)) {
+    const uint32_t* list = nullptr;   // vertexFilter over the touched vertices only
+    const bool dense_blocked = blocked && (F->all || F->size + degsum > E.threshold);
+    if (dense_blocked) {
       RG_TRY(blocked_pr_pass(g, o, contrib, b_unit, zero, &vb, &bu0));
       part = b_unit.as<double>();
+    } else if (!F->all && F->size + degsum <= E.threshold) {
+      // Sparse push (edge_map's list traversal) that also lists the vertices
+      // it reaches; then the previous active vertices' contrib is cleared
+      // (the vertex step sets it again for those still active).
+      RG_HIP(hipMemsetAsync(tinfo, 0, 2 * sizeof(int64_t), ctx->stream));
+      PrdTouchOp top{contrib, ngh, stamp, (uint32_t)round};
+      RG_TRY(edge_map(E, top, *F, degsum, touched, tinfo));
+      if (F->size > 0)
+        hipLaunchKernelGGL(k_zero_at, grid_for(ctx, F->size), 256, 0, ctx->stream, F->ids, F->size, contrib);
+      list = touched;
     } else {
       RG_TRY(edge_map(E, op, *F, degsum, nullptr, nullptr));
     }
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
-    hipLaunchKernelGGL(k_prd_vertex, vgrid, 256, 0, ctx->stream, g->off, n, p, delta, ngh, contrib, part, bu0, vb,
-                       round == 1 ? 1 : 0, damping, added, one_over_n, epsilon2, partial, N->ids, info);
+    hipLaunchKernelGGL(k_prd_vertex, vgrid, 256, 0, ctx->stream, g->off, n, list, tinfo, p, delta, ngh, contrib,
+                       part, bu0, vb, round == 1 ? 1 : 0, damping, added, one_over_n, epsilon2, partial, N->ids,
+                       info);
     hipLaunchKernelGGL(k_prd_l1, 1, 256, 0, ctx->stream, partial, vgrid, l1_dev);
     RG_HIP(hipGetLastError());
     int64_t h[2];

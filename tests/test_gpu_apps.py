@@ -236,6 +236,25 @@ def test_pagerank_delta(rg, orc, ctx, graphs, gname, variant):
 
 
 @pytest.mark.parametrize("gname", ["rmat14s", "rmat12d"])
+@pytest.mark.parametrize("wg", [512, 1024])
+@pytest.mark.parametrize("unroll", [1, 4])
+def test_blocked_shapes(rg, orc, ctx, graphs, gname, wg, unroll):
+    # Every workgroup shape / arcs-in-flight of the blocked engine (the
+    # autotuner's candidates; the default is 1024 threads, 4 arcs), for
+    # PageRank and for PageRankDelta's dense rounds.
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    dg = rg.DeviceGraph(ctx, hg)
+    o = rg.opts("blocked", wg_size=wg, unroll=unroll)
+    res = dg.pagerank(20, o)
+    ref, it, _ = orc.pagerank(off, nbr, 20)
+    assert res.iters == it and rel_linf(res.ranks, ref) < FP_TOL
+    res = dg.pagerank_delta(20, o)
+    ref, it, _ = orc.pagerank_delta(off, nbr, 20)
+    assert res.iters == it and rel_linf(res.ranks, ref) < FP_TOL
+
+
+@pytest.mark.parametrize("gname", ["rmat14s", "rmat12d"])
 def test_autotune_choice_gives_oracle_answers(rg, orc, ctx, graphs, gname):
     # Whatever configuration the autotuner picks (engine, workgroup shape,
     # hub count, threshold), it computes the same PageRank and BFS levels.

@@ -362,9 +362,7 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
  * run and batches of 1, 8, 64 and 256 sources, interpolated); otherwise the
  * oldest runs alone on the single-source engine with bfs_opts.  BC queries
  * the same way in batches of up to 64 (rg_bc_batch).  PageRank queries run
- * one per slot and alone: while one runs, no slot starts other work (both
- * are bound by device memory bandwidth; side by side the PageRank query
- * takes about twice as long for next to no gain in throughput).  The number of slots is capped by free HBM (less hbm_reserve)
+ * one per slot.  The number of slots is capped by free HBM (less hbm_reserve)
  * over a per-slot estimate.  one_at_a_time = 1 runs the same trace on one
  * slot, one query at a time, BFS on the single-source engine
  * (rg_bfs_query with bfs_opts), BC on rg_bc with one source: the baseline.

@@ -110,12 +110,6 @@ struct Shared {
   std::chrono::steady_clock::time_point t0;
   int err = RG_OK;
   std::string msg;
-  // A PageRank query runs alone: while one does, no slot starts other work.
-  // Both it and what would run beside it are bound by device memory
-  // bandwidth, so overlapping them gains next to no throughput and doubles
-  // the PageRank query's time (RMAT-26: ~150 ms beside other work against
-  // ~70 ms alone), which set the tail of mixed traces.
-  bool alone = false;
   // Time of one single-source run and of batches of the calibrated widths,
   // per kind, measured alone before the clock starts.  k waiting queries of
   // a kind are batched when the batch ends no later than the last of them
@@ -144,10 +138,10 @@ struct Shared {
 };
 
 // Picks the next work item: the kind whose oldest pending query arrived
-// first; the waiting BFS or BC queries as one batch when that pays.  Waits
-// while a PageRank query runs.  taken_ms: the item's start on the trace's
-// clock, read under the lock (so no item taken after a PageRank query
-// starts before it).  Returns false when every query has been taken.
+// first; the waiting BFS or BC queries as one batch when that pays.
+// taken_ms: the item's start on the trace's clock, read under the lock (so
+// items start in the order they are taken).  Returns false when every
+// query has been taken.
 bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64_t>& work, int64_t* batch,
                double* taken_ms) {
   std::unique_lock<std::mutex> lk(S.mu);
@@ -157,10 +151,6 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
     while (S.next < S.nq && S.q[S.order[S.next]].arrival_ms <= now) {
       const int64_t i = S.order[S.next++];
       S.pending[S.q[i].kind].push_back(i);
-    }
-    if (S.alone) {   // woken when the PageRank query ends
-      S.cv.wait(lk);
-      continue;
     }
     int best = -1;
     double ba = 0.0;
@@ -189,7 +179,6 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
       *kind = best;
       *batch = S.batches++;
       *taken_ms = S.ms_now();   // the item's start, under the lock
-      if (best == RG_Q_PAGERANK && !c.one_at_a_time) S.alone = true;
       bool more = false;
       for (auto& p : S.pending) more = more || !p.empty();
       if (more) S.cv.notify_all();
@@ -427,11 +416,6 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
           if (!rg::take_work(S, c, &kind, work, &batch, &start)) return;
           int r = rg::run_work(slots[i], c, queries, kind, work, results);
           const double end = S.ms_now();
-          if (kind == RG_Q_PAGERANK) {
-            std::lock_guard<std::mutex> lk(S.mu);
-            S.alone = false;
-            S.cv.notify_all();
-          }
           if (r != RG_OK) {
             std::lock_guard<std::mutex> lk(S.mu);
             if (S.err == RG_OK) {

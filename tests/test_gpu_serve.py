@@ -132,3 +132,22 @@ def test_bc_queries_batched(rg, ctx, graphs):
         else:
             assert x.bfs.astuple() == y.bfs.astuple()
 
+
+
+@pytest.mark.parametrize("max_batch", [2, 4])
+def test_small_batch_limit(rg, ctx, graphs, max_batch):
+    # A batch limit below the calibrated widths (1, 8, 64, 256): the cost
+    # curve is timed at the widths up to the limit, batches never exceed
+    # it, and every answer equals the one-at-a-time answer.
+    hg = graphs("rmat12s")
+    dg = rg.DeviceGraph(ctx, hg)
+    q = rg.poisson_workload(hg, 120, 0.0, mix=(0.6, 0.3, 0.1), seed=31)
+    a, st = dg.serve(q, rg.serve_config(slots=2, max_batch=max_batch))
+    b, _ = dg.serve(q, rg.serve_config(one_at_a_time=True))
+    assert max(r.batch_size for r in a) <= max_batch
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert x.kind == y.kind
+        if x.kind == rg.Q_BFS:
+            assert x.bfs.astuple() == y.bfs.astuple(), f"query {i}"
+        else:
+            assert x.nonfinite == y.nonfinite and close(x.wsum, y.wsum), f"query {i}"

@@ -295,6 +295,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
   constexpr int NW = THREADS / kWave;
   __shared__ ull acc[BV * W];
   __shared__ ull full[BV / 64];
+  __shared__ uint8_t fullw[W > 1 ? BV : 1];   // W > 1: bit w = every source of word w reached it
   __shared__ uint32_t s_q[NW][kQueue];
   __shared__ MsStats<W> st;
   if (a.ctl->mode != MS_DENSE) return;
@@ -320,15 +321,19 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
     const int64_t base = (int64_t)b << VB;
     const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
     for (int i = threadIdx.x; i < BV * W; i += THREADS) acc[i] = 0;
-    // Destinations every source of the batch has reached need no gathers.
+    // Destinations every source of the batch has reached need no gathers;
+    // with several words, neither do the words whose sources all reached
+    // the destination (their bits would be masked by seen).
     for (int l = threadIdx.x; l < BV; l += THREADS) {
       const int64_t d = base + l;
-      bool f = true;
+      uint32_t fm = (1u << W) - 1;
       if (d < a.n) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) f = f && (a.seen[(size_t)d * W + w] & allbits[w]) == allbits[w];
+        for (int w = 0; w < W; ++w)
+          if ((a.seen[(size_t)d * W + w] & allbits[w]) != allbits[w]) fm &= ~(1u << w);
       }
-      const ull word = __ballot(f);
+      if (W > 1) fullw[l] = (uint8_t)fm;
+      const ull word = __ballot(fm == (1u << W) - 1);
       if (__lane_id() == 0) full[l >> 6] = word;
     }
     __syncthreads();
@@ -353,10 +358,12 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
         l[j] = ok[j] ? (uint32_t)ld_nt(a.vl + e) : 0u;
       }
       ull act[A];
+      uint32_t fm[A];
 #pragma unroll
       for (int j = 0; j < A; ++j) {
         ok[j] = ok[j] && !((full[l[j] >> 6] >> (l[j] & 63)) & 1);
         act[j] = ok[j] ? a.active_cur[s[j] >> 6] : 0ull;
+        fm[j] = W > 1 && ok[j] ? fullw[l[j]] : 0u;
       }
       ull f[A][W];
 #pragma unroll
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
         ok[j] = ok[j] && ((act[j] >> (s[j] & 63)) & 1);
         const ull* p = a.front_cur + (size_t)s[j] * W;
 #pragma unroll
-        for (int w = 0; w < W; ++w) f[j][w] = ok[j] ? p[w] : 0ull;
+        for (int w = 0; w < W; ++w) f[j][w] = ok[j] && !((fm[j] >> w) & 1) ? p[w] : 0ull;
       }
 #pragma unroll
       for (int j = 0; j < A; ++j)

@@ -190,7 +190,8 @@ typedef struct rg_opts {
                         /* (10..14; 0 = 14)                                    */
   int src_segment_bits; /* RG_BLOCKED: 2^bits sources per L2 segment (1..24;   */
                         /* 0 = order by source id)                             */
-  int unroll;           /* RG_BLOCKED: edges in flight per lane (1 or 4; 0 = 4) */
+  int unroll;           /* RG_BLOCKED: edges in flight per lane (1 or 4; 0 =   */
+                        /* 4 on a graph made by rg_graph_degree_sort, else 1) */
 } rg_opts;
 
 /* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */

@@ -101,6 +101,7 @@ struct rg_graph {
   uint64_t* in_off = nullptr;
   uint32_t* in_nbr = nullptr;
   bool transposed = false;
+  bool degree_sorted = false;   // ids by descending degree (made by rg_graph_degree_sort)
   // Hub metadata (HUBDUP): bitmap of hubs + per-word rank prefix, for the
   // in-degree ordering of the current orientation.
   uint32_t* hub_bits = nullptr;

@@ -147,6 +147,7 @@ extern "C" int rg_graph_degree_sort(rg_graph* g, int by_outdeg, int sort_neighbo
   ng->n = n;
   ng->m = m;
   ng->symmetric = g->symmetric;
+  ng->degree_sorted = true;
   int rc = rg::relabel_csr(ctx, g->off, g->nbr, n, m, inv.as<uint32_t>(), ids.as<uint32_t>(),
                            sort_neighbors != 0, &ng->off, &ng->nbr);
   if (rc == RG_OK) {

@@ -306,6 +306,15 @@ static int edge_map(const Engine& E, const Op& op, Frontier& F, int64_t f_degsum
   }
   a.count = F.size;
   a.ids = F.ids;
+  // The light kernel spreads each wave's 64 vertices' arcs over its lanes, so
+  // a small frontier with long lists (BFS level 2 on RMAT-26: ~700 vertices,
+  // ~20M arcs, 12 waves for 1.2 ms) leaves the device idle: below 16 light
+  // waves per CU, lists longer than a wave become chunk items of the heavy
+  // kernel, one wave each, when the queue holds them (at most |F| +
+  // degsum / kHeavyChunk items).
+  if (F.size < (int64_t)E.ctx->num_cus * 16 * kWave && E.heavy_deg > kWave &&
+      F.size + f_degsum / kHeavyChunk + 64 <= E.heavy_cap)
+    a.heavy_deg = kWave;
   return launch_hub<Op, SRC_LIST>(E, op, a);
 }
 
@@ -453,6 +462,7 @@ int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E) {
   int64_t cap = g->m / kHeavyChunk + g->m / std::max(1, E.heavy_deg) + 64;
   E.heavy = &ctx->scratch[7];
   RG_TRY(E.heavy->ensure(sizeof(uint64_t) * cap));
+  E.heavy_cap = cap;
   E.heavy_count = ctx->counts.as<int64_t>() + 0;
   return RG_OK;
 }

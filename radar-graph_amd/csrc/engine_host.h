@@ -54,6 +54,7 @@ struct Engine {
   void* l2_copies;
   int l2_ncopies;
   DevBuf* heavy;       // heavy work queue
+  int64_t heavy_cap = 0;   // its capacity in items (0: never lower the cut-over)
   int64_t* heavy_count;
 };
 

@@ -30,7 +30,7 @@ def orc():
 
 @pytest.fixture(scope="session")
 def ctx(rg):
-    # On a GPU box a missing device is an error, not a skip.
+    # The gpu-marked tests need device 0: a missing device is an error, not a skip.
     c = rg.Context(0)
     yield c
     c.close()

@@ -17,7 +17,7 @@ in Ligra's AdjacencyGraph format / GAP's .el format for the reference to
 read; the generator itself is checked against the reference's by the digests.
 
 Usage: python tests/golden/make_golden.py   (needs /root/reference built into
-oracle/_ref; the fixtures are committed, the GPU box never runs this).
+oracle/_ref; the fixtures are committed, and the GPU tests only read them).
 """
 import hashlib
 import json

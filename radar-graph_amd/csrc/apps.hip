@@ -592,12 +592,13 @@ int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks
 // ---------------------------------------------------------------------------
 // PageRankDelta (PageRankDelta.C:102-154).
 
-__global__ void k_prd_init(const uint64_t* off, int64_t n, double* p, double* delta, double* ngh, double* contrib,
+// Delta itself is not stored: the edge phase reads only Delta[s]/outdeg(s),
+// kept in `contrib` (1/n over the degree before round 1).
+__global__ void k_prd_init(const uint64_t* off, int64_t n, double* p, double* ngh, double* contrib,
                            double one_over_n) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t deg = off[v + 1] - off[v];
     p[v] = 0.0;
-    delta[v] = one_over_n;
     ngh[v] = 0.0;
     contrib[v] = deg ? one_over_n / (double)(uint32_t)deg : 0.0;
   }
@@ -616,8 +617,8 @@ __global__ void k_prd_init(const uint64_t* off, int64_t n, double* p, double* de
 // first).  Multiplies and adds are
 // rounded separately, as the reference's host code computes them.
 __global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t n, const uint32_t* list,
-                                                    const int64_t* list_count, double* p, double* delta,
-                                                    double* ngh, double* contrib, const double* part,
+                                                    const int64_t* list_count, double* p, double* ngh,
+                                                    double* contrib, const double* part,
                                                     const uint32_t* block_unit0, int vb, int first,
                                                     double damping, double added, double one_over_n,
                                                     double eps2, double* partial, uint32_t* out_ids,
@@ -659,7 +660,6 @@ __global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t
         if (act) pv = __dadd_rn(pv, dl);
       }
       p[v] = pv;
-      delta[v] = dl;
       l1 += fabs(dl);
       const uint64_t deg = off[v + 1] - off[v];
       contrib[v] = (act && deg) ? dl / (double)(uint32_t)deg : 0.0;
@@ -703,23 +703,22 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
   Engine E;
   RG_TRY(make_engine(g, o, sizeof(double), E));
   const bool blocked = o && o->variant == RG_BLOCKED;
-  DevBuf &b_p = ctx->scratch[0], &b_delta = ctx->scratch[1], &b_ngh = ctx->scratch[2], &b_part = ctx->scratch[3],
-         &b_contrib = ctx->scratch[4], &b_unit = ctx->scratch[5];
+  DevBuf &b_p = ctx->scratch[0], &b_ngh = ctx->scratch[2], &b_part = ctx->scratch[3], &b_contrib = ctx->scratch[4],
+         &b_unit = ctx->scratch[5];
   RG_TRY(b_p.ensure(sizeof(double) * n));
-  RG_TRY(b_delta.ensure(sizeof(double) * n));
   RG_TRY(b_ngh.ensure(sizeof(double) * n));
   RG_TRY(b_contrib.ensure(sizeof(double) * n));
   const int vgrid = std::min<int>(grid_for(ctx, n), 1024);
   RG_TRY(b_part.ensure(sizeof(double) * (vgrid + 8)));
   double* p = b_p.as<double>();
-  double* delta = b_delta.as<double>();
   double* ngh = b_ngh.as<double>();
   double* contrib = b_contrib.as<double>();
   double* partial = b_part.as<double>();
-  double* l1_dev = partial + vgrid;
   int64_t* zero = ctx->counts.as<int64_t>() + 32;
   int64_t* info = ctx->counts.as<int64_t>() + 34;   // [count, out-degree sum]
   int64_t* tinfo = ctx->counts.as<int64_t>() + 36;  // touched list of a sparse round
+  // The round's L1 norm sits after them, so one read returns count, degree sum and L1.
+  double* l1_dev = reinterpret_cast<double*>(ctx->counts.as<int64_t>() + 38);
   RG_HIP(hipMemsetAsync(zero, 0, sizeof(int64_t), ctx->stream));
   // Sparse rounds: the vertices their push reaches, and a per-vertex stamp
   // of the last round that reached each (tags are rounds, from 2).
@@ -734,8 +733,7 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
   const double damping = 0.85, epsilon = 0.0000001, epsilon2 = 0.01;
   const double one_over_n = 1 / (double)n;
   const double added = (1 - damping) * one_over_n;
-  hipLaunchKernelGGL(k_prd_init, grid_for(ctx, n), 256, 0, ctx->stream, g->off, n, p, delta, ngh, contrib,
-                     one_over_n);
+  hipLaunchKernelGGL(k_prd_init, grid_for(ctx, n), 256, 0, ctx->stream, g->off, n, p, ngh, contrib, one_over_n);
   RG_HIP(hipGetLastError());
   Frontier* F = &fa;
   Frontier* N = &fb;
@@ -769,14 +767,14 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
       RG_TRY(edge_map(E, op, *F, degsum, nullptr, nullptr));
     }
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
-    hipLaunchKernelGGL(k_prd_vertex, vgrid, 256, 0, ctx->stream, g->off, n, list, tinfo, p, delta, ngh, contrib,
+    hipLaunchKernelGGL(k_prd_vertex, vgrid, 256, 0, ctx->stream, g->off, n, list, tinfo, p, ngh, contrib,
                        part, bu0, vb, round == 1 ? 1 : 0, damping, added, one_over_n, epsilon2, partial, N->ids,
                        info);
     hipLaunchKernelGGL(k_prd_l1, 1, 256, 0, ctx->stream, partial, vgrid, l1_dev);
     RG_HIP(hipGetLastError());
-    int64_t h[2];
-    RG_TRY(ctx_read_count(ctx, info, h, 2));
-    RG_TRY(ctx_copy(ctx, &L1, l1_dev, sizeof(double), hipMemcpyDeviceToHost));
+    int64_t h[5];
+    RG_TRY(ctx_read_count(ctx, info, h, 5));
+    std::memcpy(&L1, &h[4], sizeof(double));
     if (L1 < epsilon) break;
     N->all = false;
     N->size = h[0];

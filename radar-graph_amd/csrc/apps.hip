@@ -609,8 +609,7 @@ __global__ void k_prd_init(const uint64_t* off, int64_t n, double* p, double* ng
 // reset of nghSum (80-88) and the next round's quotient Delta[s]/outdeg(s)
 // for active s (0 for inactive ones, so an all-vertices pass adds nothing
 // for them).  Active vertices are appended to out_ids with their out-degree
-// sum.  nghSum comes from `ngh` or, after a blocked pass, from the unit
-// partials of v's block added in unit order.  With `list`, only the
+// sum.  nghSum comes from `ngh`, which is reset.  With `list`, only the
 // vertices list[0, *list_count) are visited: after a sparse round every
 // other vertex has nghSum 0, so Delta 0, p unchanged and contrib 0 (kept
 // so: the previous round's active vertices have their contrib cleared
@@ -618,8 +617,7 @@ __global__ void k_prd_init(const uint64_t* off, int64_t n, double* p, double* ng
 // rounded separately, as the reference's host code computes them.
 __global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t n, const uint32_t* list,
                                                     const int64_t* list_count, double* p, double* ngh,
-                                                    double* contrib, const double* part,
-                                                    const uint32_t* block_unit0, int vb, int first,
+                                                    double* contrib, int first,
                                                     double damping, double added, double one_over_n,
                                                     double eps2, double* partial, uint32_t* out_ids,
                                                     int64_t* out_info) {
@@ -631,7 +629,6 @@ __global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t
   ea.out_info = out_info;
   WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
   double l1 = 0.0;
-  const int64_t lmask = (int64_t(1) << vb) - 1;
   const int64_t total = list ? *list_count : n;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = base + threadIdx.x;
@@ -639,15 +636,8 @@ __global__ __launch_bounds__(256) void k_prd_vertex(const uint64_t* off, int64_t
     int64_t v = i;
     if (i < total) {
       if (list) v = list[i];
-      double ns;
-      if (part) {
-        ns = 0.0;
-        const int64_t b = v >> vb;
-        for (uint32_t u = block_unit0[b]; u < block_unit0[b + 1]; ++u) ns += part[((int64_t)u << vb) + (v & lmask)];
-      } else {
-        ns = ngh[v];
-        ngh[v] = 0.0;
-      }
+      const double ns = ngh[v];
+      ngh[v] = 0.0;
       double pv = p[v], dl;
       if (first) {
         dl = __dadd_rn(__dmul_rn(damping, ns), added);
@@ -704,15 +694,19 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
   RG_TRY(make_engine(g, o, sizeof(double), E));
   const bool blocked = o && o->variant == RG_BLOCKED;
   DevBuf &b_p = ctx->scratch[0], &b_ngh = ctx->scratch[2], &b_part = ctx->scratch[3], &b_contrib = ctx->scratch[4],
-         &b_unit = ctx->scratch[5];
+         &b_unit = ctx->scratch[5], &b_contrib2 = ctx->scratch[1];
   RG_TRY(b_p.ensure(sizeof(double) * n));
   RG_TRY(b_ngh.ensure(sizeof(double) * n));
   RG_TRY(b_contrib.ensure(sizeof(double) * n));
+  if (blocked) RG_TRY(b_contrib2.ensure(sizeof(double) * n));
   const int vgrid = std::min<int>(grid_for(ctx, n), 1024);
   RG_TRY(b_part.ensure(sizeof(double) * (vgrid + 8)));
   double* p = b_p.as<double>();
   double* ngh = b_ngh.as<double>();
   double* contrib = b_contrib.as<double>();
+  // Blocked dense rounds read contrib while they write the next quotients,
+  // so they write the other buffer and the two swap.
+  double* contrib_alt = blocked ? b_contrib2.as<double>() : nullptr;
   double* partial = b_part.as<double>();
   int64_t* zero = ctx->counts.as<int64_t>() + 32;
   int64_t* info = ctx->counts.as<int64_t>() + 34;   // [count, out-degree sum]
@@ -740,19 +734,20 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
   F->all = true;
   F->size = n;
   int64_t degsum = g->m;
-  PrOp op{contrib, ngh, zero};
   double L1 = 0.0;
   int64_t round = 0;
   EventTimer tm(ctx->stream);
   while (round++ < maxiters) {
-    const double* part = nullptr;
-    const uint32_t* bu0 = nullptr;
-    int vb = 0;
     const uint32_t* list = nullptr;   // vertexFilter over the touched vertices only
     const bool dense_blocked = blocked && (F->all || F->size + degsum > E.threshold);
+    RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
     if (dense_blocked) {
-      RG_TRY(blocked_pr_pass(g, o, contrib, b_unit, zero, &vb, &bu0));
-      part = b_unit.as<double>();
+      const PrdStep st{damping, added, one_over_n, epsilon2, round == 1 ? 1 : 0};
+      const double* l1_part = nullptr;
+      int64_t nparts = 0;
+      RG_TRY(blocked_prd_round(g, o, contrib, contrib_alt, p, st, N->ids, info, zero, b_unit, &l1_part, &nparts));
+      hipLaunchKernelGGL(k_prd_l1, 1, 256, 0, ctx->stream, l1_part, (int)nparts, l1_dev);
+      std::swap(contrib, contrib_alt);
     } else if (!F->all && F->size + degsum <= E.threshold) {
       // Sparse push (edge_map's list traversal) that also lists the vertices
       // it reaches; then the previous active vertices' contrib is cleared
@@ -764,13 +759,14 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
         hipLaunchKernelGGL(k_zero_at, grid_for(ctx, F->size), 256, 0, ctx->stream, F->ids, F->size, contrib);
       list = touched;
     } else {
+      PrOp op{contrib, ngh, zero};
       RG_TRY(edge_map(E, op, *F, degsum, nullptr, nullptr));
     }
-    RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
-    hipLaunchKernelGGL(k_prd_vertex, vgrid, 256, 0, ctx->stream, g->off, n, list, tinfo, p, ngh, contrib,
-                       part, bu0, vb, round == 1 ? 1 : 0, damping, added, one_over_n, epsilon2, partial, N->ids,
-                       info);
-    hipLaunchKernelGGL(k_prd_l1, 1, 256, 0, ctx->stream, partial, vgrid, l1_dev);
+    if (!dense_blocked) {
+      hipLaunchKernelGGL(k_prd_vertex, vgrid, 256, 0, ctx->stream, g->off, n, list, tinfo, p, ngh, contrib,
+                         round == 1 ? 1 : 0, damping, added, one_over_n, epsilon2, partial, N->ids, info);
+      hipLaunchKernelGGL(k_prd_l1, 1, 256, 0, ctx->stream, partial, vgrid, l1_dev);
+    }
     RG_HIP(hipGetLastError());
     int64_t h[5];
     RG_TRY(ctx_read_count(ctx, info, h, 5));

@@ -28,6 +28,7 @@
 
 #include "blocked.h"
 #include "device.h"
+#include "queries.h"
 
 namespace rg {
 
@@ -127,24 +128,6 @@ __device__ __forceinline__ void blk_accumulate(const uint32_t* __restrict__ src,
     const uint32_t l = ld_stream<NT>(vl + e);
     atomicAdd(&acc[l], contrib[s]);
   }
-}
-
-template <int VB, int THREADS, bool NT, int UNROLL>
-__global__ __launch_bounds__(THREADS) void k_blk_pr(const uint32_t* __restrict__ src,
-                                                    const uint16_t* __restrict__ vl,
-                                                    const uint64_t* __restrict__ unit_lo,
-                                                    const double* __restrict__ contrib,
-                                                    double* __restrict__ part, const int64_t* done) {
-  constexpr int BV = 1 << VB;
-  __shared__ double acc[BV];
-  if (*done) return;
-  const int64_t u = blockIdx.x;
-  for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
-  __syncthreads();
-  blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
-  __syncthreads();
-  double* out = part + u * (int64_t)BV;
-  for (int i = threadIdx.x; i < BV; i += THREADS) out[i] = acc[i];
 }
 
 // PR_Vertex_F (PageRank.C:55) for vertex v from its in-edge sum a, fused
@@ -272,6 +255,206 @@ __global__ __launch_bounds__(256) void k_blk_pr_merge(const uint32_t* __restrict
   }
   diff = wg_sum<256>(diff);
   if (threadIdx.x == 0) diff_part[blockIdx.x] = diff;
+}
+
+// vertexFilter of one PageRankDelta round for vertex v, from its in-edge sum
+// ns, its old p and out-degree (PR_Vertex_F_FirstRound, PageRankDelta.C:51-64,
+// in the first round; PR_Vertex_F, 66-78, after): updates p[v], writes the
+// next round's quotient Delta/outdeg (0 when v is not active) into
+// contrib_next[v] and returns whether v is active; |Delta| goes to l1.
+// Multiplies and adds are rounded separately, as the reference's host code
+// computes them.
+__device__ __forceinline__ bool prd_finalize(int64_t v, double ns, double pv, uint64_t deg, double* p,
+                                             double* contrib_next, const PrdStep& st, double& l1) {
+  double dl;
+  bool act;
+  if (st.first) {
+    dl = __dadd_rn(__dmul_rn(st.damping, ns), st.added);
+    pv = __dadd_rn(pv, dl);
+    dl = __dsub_rn(dl, st.one_over_n);
+    act = fabs(dl) > __dmul_rn(st.eps2, pv);
+    p[v] = pv;
+  } else {
+    dl = __dmul_rn(ns, st.damping);
+    act = fabs(dl) > __dmul_rn(st.eps2, pv);
+    if (act) p[v] = __dadd_rn(pv, dl);
+  }
+  l1 += fabs(dl);
+  contrib_next[v] = (act && deg) ? dl / (double)(uint32_t)deg : 0.0;
+  return act;
+}
+
+// Appends the vertices whose act[k] is set (FU per thread) to out_ids, in
+// thread order within the workgroup: a wave scan of the per-thread counts,
+// the waves' totals added up by thread 0 and one atomicAdd on *count per
+// call.  Every thread of the workgroup calls it.
+template <int THREADS, int FU>
+__device__ __forceinline__ void wg_append(const bool* act, const int64_t* vs, uint32_t* __restrict__ out_ids,
+                                          int64_t* count, uint32_t* s_cnt, unsigned long long* s_base) {
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < FU; ++k) c += act[k] ? 1 : 0;
+  const int lane = __lane_id();
+  int x = c;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  const int w = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_cnt[w] = (uint32_t)x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < THREADS / kWave; ++i) {
+      const uint32_t y = s_cnt[i];
+      s_cnt[i] = t;
+      t += y;
+    }
+    *s_base = t ? atomicAdd(reinterpret_cast<unsigned long long*>(count), (unsigned long long)t) : 0ull;
+  }
+  __syncthreads();
+  uint64_t pos = *s_base + s_cnt[w] + (uint32_t)(x - c);
+#pragma unroll
+  for (int k = 0; k < FU; ++k)
+    if (act[k]) out_ids[pos++] = (uint32_t)vs[k];
+  __syncthreads();   // s_cnt and s_base are reused by the next call
+}
+
+// Sums l1 and deg over the workgroup: l1 into *l1_out, deg added to *deg_sum.
+template <int THREADS>
+__device__ __forceinline__ void wg_finish(double l1, uint64_t deg, double* l1_out, int64_t* deg_sum) {
+  __shared__ double r1[THREADS / kWave];
+  __shared__ uint64_t r2[THREADS / kWave];
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    l1 += __shfl_xor(l1, o);
+    deg += __shfl_xor(deg, o);
+  }
+  if (__lane_id() == 0) {
+    r1[threadIdx.x / kWave] = l1;
+    r2[threadIdx.x / kWave] = deg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    uint64_t d = 0;
+    for (int i = 0; i < THREADS / kWave; ++i) {
+      a += r1[i];
+      d += r2[i];
+    }
+    *l1_out = a;
+    if (d) atomicAdd(reinterpret_cast<unsigned long long*>(deg_sum), (unsigned long long)d);
+  }
+}
+
+// One dense PageRankDelta round on the blocked layout, edge phase and
+// vertexFilter fused as in k_blk_pr_it: a unit that holds its block alone
+// applies the vertex step to the block (next quotients into contrib_next,
+// active vertices appended to out_ids, their count and out-degree sum into
+// info[0..1]); units of split blocks write partials for k_blk_prd_merge.
+// l1_part[u] = the unit's |Delta| sum (0 for split-block units).
+template <int VB, int THREADS, bool NT, int UNROLL>
+__global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restrict__ src,
+                                                        const uint16_t* __restrict__ vl,
+                                                        const uint64_t* __restrict__ unit_lo,
+                                                        const uint32_t* __restrict__ unit_block,
+                                                        const uint32_t* __restrict__ block_unit0,
+                                                        const double* __restrict__ contrib,
+                                                        double* __restrict__ contrib_next, double* p,
+                                                        const uint64_t* __restrict__ off, int64_t n,
+                                                        double* __restrict__ part, double* l1_part,
+                                                        uint32_t* __restrict__ out_ids, int64_t* info,
+                                                        const int64_t* done, PrdStep st) {
+  constexpr int BV = 1 << VB;
+  constexpr int FU = 4;
+  __shared__ double acc[BV];
+  __shared__ uint32_t s_cnt[THREADS / kWave];
+  __shared__ unsigned long long s_base;
+  if (*done) return;
+  const int64_t u = blockIdx.x;
+  for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
+  __syncthreads();
+  blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
+  __syncthreads();
+  const uint32_t b = unit_block[u];
+  if (block_unit0[b + 1] - block_unit0[b] != 1) {
+    double* out = part + u * (int64_t)BV;
+    for (int i = threadIdx.x; i < BV; i += THREADS) out[i] = acc[i];
+    if (threadIdx.x == 0) l1_part[u] = 0.0;
+    return;
+  }
+  const int64_t base = (int64_t)b << VB;
+  double l1 = 0.0;
+  uint64_t dsum = 0;
+  // The same number of passes for every thread (wg_append synchronizes).
+  for (int pass = 0; pass < (BV + THREADS * FU - 1) / (THREADS * FU); ++pass) {
+    const int i0 = pass * THREADS * FU + threadIdx.x;
+    double pv[FU];
+    uint64_t o0[FU], o1[FU];
+    int64_t vs[FU];
+    bool act[FU];
+#pragma unroll
+    for (int k = 0; k < FU; ++k) {
+      const int i = i0 + k * THREADS;
+      vs[k] = base + i;
+      act[k] = false;
+      if (i < BV && vs[k] < n) {
+        pv[k] = p[vs[k]];
+        o0[k] = off[vs[k]];
+        o1[k] = off[vs[k] + 1];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < FU; ++k) {
+      const int i = i0 + k * THREADS;
+      if (i < BV && vs[k] < n) {
+        const uint64_t deg = o1[k] - o0[k];
+        act[k] = prd_finalize(vs[k], 0.0 + acc[i], pv[k], deg, p, contrib_next, st, l1);
+        if (act[k]) dsum += deg;
+      }
+    }
+    wg_append<THREADS, FU>(act, vs, out_ids, info, s_cnt, &s_base);
+  }
+  wg_finish<THREADS>(l1, dsum, &l1_part[u], info + 1);
+}
+
+// Split blocks of a dense PageRankDelta round: a vertex's sum is its units'
+// partials (added as in k_blk_pr_merge), then the vertex step and the append
+// of k_blk_prd_it.  Grid: nmulti * (2^VB / 256) workgroups of 256;
+// l1_part[workgroup] = its |Delta| sum.
+template <int VB>
+__global__ __launch_bounds__(256) void k_blk_prd_merge(const uint32_t* __restrict__ multi,
+                                                       const uint32_t* __restrict__ block_unit0,
+                                                       const double* __restrict__ part, double* p,
+                                                       const uint64_t* __restrict__ off,
+                                                       double* __restrict__ contrib_next, int64_t n,
+                                                       double* l1_part, uint32_t* __restrict__ out_ids,
+                                                       int64_t* info, const int64_t* done, PrdStep st) {
+  constexpr int PER = (1 << VB) / 256;
+  __shared__ uint32_t s_cnt[256 / kWave];
+  __shared__ unsigned long long s_base;
+  if (*done) return;
+  const uint32_t b = multi[blockIdx.x / PER];
+  const int l = (int)(blockIdx.x % PER) * 256 + threadIdx.x;
+  const int64_t v = ((int64_t)b << VB) + l;
+  double l1 = 0.0;
+  uint64_t dsum = 0;
+  bool act = false;
+  if (v < n) {
+    const uint32_t u1 = block_unit0[b + 1];
+    uint32_t u = block_unit0[b];
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (; u + 8 <= u1; u += 8) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] += part[((int64_t)(u + k) << VB) + l];
+    }
+    for (int k = 0; u < u1; ++u, ++k) a[k] += part[((int64_t)u << VB) + l];
+    const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    const uint64_t deg = off[v + 1] - off[v];
+    act = prd_finalize(v, sum, p[v], deg, p, contrib_next, st, l1);
+    if (act) dsum = deg;
+  }
+  wg_append<256, 1>(&act, &v, out_ids, info, s_cnt, &s_base);
+  wg_finish<256>(l1, dsum, &l1_part[blockIdx.x], info + 1);
 }
 
 int grid_n(rg_ctx* ctx, int64_t n) {
@@ -423,24 +606,36 @@ int pagerank_prepare(rg_graph* g, const rg_opts* o) {
 // over each destination block's in-edges into unit_part (grown as needed).
 // The caller adds a block's partials in unit order: *vb_out and
 // *block_unit0_out describe the layout.
-int blocked_pr_pass(rg_graph* g, const rg_opts* o, const double* contrib, DevBuf& unit_part,
-                    const int64_t* zero, int* vb_out, const uint32_t** block_unit0_out) {
+int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, double* contrib_next, double* p,
+                      const PrdStep& st, uint32_t* out_ids, int64_t* info, const int64_t* zero, DevBuf& work,
+                      const double** l1_part_out, int64_t* nparts_out) {
   rg_ctx* ctx = g->ctx;
   int vb, sb;
   layout_bits(o, &vb, &sb);
   BlockedCSC* B = nullptr;
   RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B));
-  RG_TRY(unit_part.ensure(sizeof(double) * ((size_t)B->nunits << vb)));
+  const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 256);
+  const size_t unit_doubles = (size_t)B->nunits << vb;
+  RG_TRY(work.ensure(sizeof(double) * (unit_doubles + B->nunits + mgrid)));
+  double* part = work.as<double>();
+  double* l1_part = part + unit_doubles;
   const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
   // Arcs in flight per lane: 0 (default) = 4 on a degree-sorted graph,
-  // where the hub blocks' long units dominate (PageRankDelta on RMAT-26:
-  // 42.7 ms against 49.1), 1 on other id orders (PageRank on directed
+  // where the hub blocks' long units dominate (PageRankDelta on RMAT-26,
+  // before its vertex step was fused: 42.7 ms against 49.1), 1 on other id orders (PageRank on directed
   // RMAT-24 original ids: 118 GTEPS against 91).
   const bool u4 = o && o->unroll ? o->unroll == 4 : g->degree_sorted;
-  void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const double*, double*, const int64_t*);
-#define RG_PICK(VB)                                                                   \
-  kern = threads == 512 ? (u4 ? k_blk_pr<VB, 512, true, 4> : k_blk_pr<VB, 512, true, 1>) \
-                        : (u4 ? k_blk_pr<VB, 1024, true, 4> : k_blk_pr<VB, 1024, true, 1>)
+  void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*, const uint32_t*,
+               const double*, double*, double*, const uint64_t*, int64_t, double*, double*, uint32_t*,
+               int64_t*, const int64_t*, PrdStep);
+  void (*merge)(const uint32_t*, const uint32_t*, const double*, double*, const uint64_t*, double*, int64_t,
+                double*, uint32_t*, int64_t*, const int64_t*, PrdStep);
+#define RG_PICK(VB)                                                                                   \
+  do {                                                                                                \
+    kern = threads == 512 ? (u4 ? k_blk_prd_it<VB, 512, true, 4> : k_blk_prd_it<VB, 512, true, 1>)     \
+                          : (u4 ? k_blk_prd_it<VB, 1024, true, 4> : k_blk_prd_it<VB, 1024, true, 1>); \
+    merge = k_blk_prd_merge<VB>;                                                                      \
+  } while (0)
   if (vb == 14) RG_PICK(14);
   else if (vb == 13) RG_PICK(13);
   else if (vb == 12) RG_PICK(12);
@@ -448,12 +643,16 @@ int blocked_pr_pass(rg_graph* g, const rg_opts* o, const double* contrib, DevBuf
   else RG_PICK(10);
 #undef RG_PICK
   hipEvent_t* ev = timing_begin(ctx);
-  hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, contrib,
-                     unit_part.as<double>(), zero);
+  hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, B->unit_block,
+                     B->block_unit0, contrib, contrib_next, p, g->off, g->n, part, l1_part, out_ids, info, zero,
+                     st);
+  if (mgrid)
+    hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, part, p, g->off,
+                       contrib_next, g->n, l1_part + B->nunits, out_ids, info, zero, st);
   if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
   RG_HIP(hipGetLastError());
-  *vb_out = vb;
-  *block_unit0_out = B->block_unit0;
+  *l1_part_out = l1_part;
+  *nparts_out = B->nunits + mgrid;
   return RG_OK;
 }
 
@@ -471,7 +670,7 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
                double, double);
   void (*merge)(const uint32_t*, const uint32_t*, const double*, double*, const uint64_t*, double*, int64_t,
                 double*, const int64_t*, double, double);
-  const bool u4 = o && o->unroll ? o->unroll == 4 : g->degree_sorted;   // default as in blocked_pr_pass
+  const bool u4 = o && o->unroll ? o->unroll == 4 : g->degree_sorted;   // default as in blocked_prd_round
 #define RG_PICK(VB)                                                                                 \
   do {                                                                                              \
     kern = threads == 512 ? (u4 ? k_blk_pr_it<VB, 512, true, 4> : k_blk_pr_it<VB, 512, true, 1>)     \

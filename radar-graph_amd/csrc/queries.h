@@ -81,10 +81,21 @@ int bc_batch_reserve(rg_graph* g);
 // (n, m) graph.
 int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc);
 
-// One blocked PageRank edge pass (blocked.hip): per-unit partial sums of
-// contrib over each destination block's in-edges.
-int blocked_pr_pass(rg_graph* g, const rg_opts* o, const double* contrib, DevBuf& unit_part,
-                    const int64_t* zero, int* vb_out, const uint32_t** block_unit0_out);
+// Constants of a PageRankDelta vertexFilter; first = 1 in round 1.
+struct PrdStep {
+  double damping, added, one_over_n, eps2;
+  int first;
+};
+
+// One dense PageRankDelta round on the blocked layout (blocked.hip): the
+// edge pass over contrib and the vertexFilter, fused.  Updates p, writes
+// every vertex's next quotient into contrib_next, appends the active
+// vertices to out_ids with their count and out-degree sum added to
+// info[0..1]; *l1_part_out[0, *nparts_out) are the |Delta| partial sums.
+// work holds the split blocks' partials and the L1 terms.
+int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, double* contrib_next, double* p,
+                      const PrdStep& st, uint32_t* out_ids, int64_t* info, const int64_t* zero, DevBuf& work,
+                      const double** l1_part_out, int64_t* nparts_out);
 
 // Builds the blocked PageRank layout for these options (once per graph).
 int pagerank_prepare(rg_graph* g, const rg_opts* o);

@@ -743,10 +743,7 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
     if (dense_blocked) {
       const PrdStep st{damping, added, one_over_n, epsilon2, round == 1 ? 1 : 0};
-      const double* l1_part = nullptr;
-      int64_t nparts = 0;
-      RG_TRY(blocked_prd_round(g, o, contrib, contrib_alt, p, st, N->ids, info, zero, b_unit, &l1_part, &nparts));
-      hipLaunchKernelGGL(k_prd_l1, 1, 256, 0, ctx->stream, l1_part, (int)nparts, l1_dev);
+      RG_TRY(blocked_prd_round(g, o, contrib, contrib_alt, p, st, N->ids, info, l1_dev, zero, b_unit));
       std::swap(contrib, contrib_alt);
     } else if (!F->all && F->size + degsum <= E.threshold) {
       // Sparse push (edge_map's list traversal) that also lists the vertices

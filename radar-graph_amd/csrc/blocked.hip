@@ -320,9 +320,9 @@ __device__ __forceinline__ void wg_append(const bool* act, const int64_t* vs, ui
   __syncthreads();   // s_cnt and s_base are reused by the next call
 }
 
-// Sums l1 and deg over the workgroup: l1 into *l1_out, deg added to *deg_sum.
+// Sums l1 and deg over the workgroup into *l1_out and *deg_out (thread 0).
 template <int THREADS>
-__device__ __forceinline__ void wg_finish(double l1, uint64_t deg, double* l1_out, int64_t* deg_sum) {
+__device__ __forceinline__ void wg_finish(double l1, uint64_t deg, double* l1_out, uint64_t* deg_out) {
   __shared__ double r1[THREADS / kWave];
   __shared__ uint64_t r2[THREADS / kWave];
   for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -342,16 +342,17 @@ __device__ __forceinline__ void wg_finish(double l1, uint64_t deg, double* l1_ou
       d += r2[i];
     }
     *l1_out = a;
-    if (d) atomicAdd(reinterpret_cast<unsigned long long*>(deg_sum), (unsigned long long)d);
+    *deg_out = d;
   }
 }
 
 // One dense PageRankDelta round on the blocked layout, edge phase and
 // vertexFilter fused as in k_blk_pr_it: a unit that holds its block alone
 // applies the vertex step to the block (next quotients into contrib_next,
-// active vertices appended to out_ids, their count and out-degree sum into
-// info[0..1]); units of split blocks write partials for k_blk_prd_merge.
-// l1_part[u] = the unit's |Delta| sum (0 for split-block units).
+// active vertices appended to out_ids, their count added to *count); units
+// of split blocks write partials for k_blk_prd_merge.  l1_part[u] and
+// deg_part[u] = the unit's |Delta| sum and its active vertices' out-degree
+// sum (0 for split-block units).
 template <int VB, int THREADS, bool NT, int UNROLL>
 __global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restrict__ src,
                                                         const uint16_t* __restrict__ vl,
@@ -362,8 +363,8 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restri
                                                         double* __restrict__ contrib_next, double* p,
                                                         const uint64_t* __restrict__ off, int64_t n,
                                                         double* __restrict__ part, double* l1_part,
-                                                        uint32_t* __restrict__ out_ids, int64_t* info,
-                                                        const int64_t* done, PrdStep st) {
+                                                        uint64_t* deg_part, uint32_t* __restrict__ out_ids,
+                                                        int64_t* count, const int64_t* done, PrdStep st) {
   constexpr int BV = 1 << VB;
   constexpr int FU = 4;
   __shared__ double acc[BV];
@@ -379,7 +380,10 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restri
   if (block_unit0[b + 1] - block_unit0[b] != 1) {
     double* out = part + u * (int64_t)BV;
     for (int i = threadIdx.x; i < BV; i += THREADS) out[i] = acc[i];
-    if (threadIdx.x == 0) l1_part[u] = 0.0;
+    if (threadIdx.x == 0) {
+      l1_part[u] = 0.0;
+      deg_part[u] = 0;
+    }
     return;
   }
   const int64_t base = (int64_t)b << VB;
@@ -412,49 +416,71 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restri
         if (act[k]) dsum += deg;
       }
     }
-    wg_append<THREADS, FU>(act, vs, out_ids, info, s_cnt, &s_base);
+    wg_append<THREADS, FU>(act, vs, out_ids, count, s_cnt, &s_base);
   }
-  wg_finish<THREADS>(l1, dsum, &l1_part[u], info + 1);
+  wg_finish<THREADS>(l1, dsum, &l1_part[u], &deg_part[u]);
 }
 
 // Split blocks of a dense PageRankDelta round: a vertex's sum is its units'
 // partials (added as in k_blk_pr_merge), then the vertex step and the append
-// of k_blk_prd_it.  Grid: nmulti * (2^VB / 256) workgroups of 256;
-// l1_part[workgroup] = its |Delta| sum.
+// of k_blk_prd_it, four vertices per thread (one append, so one atomic, per
+// 1024 vertices).  Grid: nmulti * (2^VB / 1024) workgroups of 256;
+// l1_part / deg_part[workgroup] = its sums.
 template <int VB>
 __global__ __launch_bounds__(256) void k_blk_prd_merge(const uint32_t* __restrict__ multi,
                                                        const uint32_t* __restrict__ block_unit0,
                                                        const double* __restrict__ part, double* p,
                                                        const uint64_t* __restrict__ off,
                                                        double* __restrict__ contrib_next, int64_t n,
-                                                       double* l1_part, uint32_t* __restrict__ out_ids,
-                                                       int64_t* info, const int64_t* done, PrdStep st) {
-  constexpr int PER = (1 << VB) / 256;
+                                                       double* l1_part, uint64_t* deg_part,
+                                                       uint32_t* __restrict__ out_ids, int64_t* count,
+                                                       const int64_t* done, PrdStep st) {
+  constexpr int FU = 4;
+  constexpr int PER = (1 << VB) / (256 * FU);
   __shared__ uint32_t s_cnt[256 / kWave];
   __shared__ unsigned long long s_base;
   if (*done) return;
   const uint32_t b = multi[blockIdx.x / PER];
-  const int l = (int)(blockIdx.x % PER) * 256 + threadIdx.x;
-  const int64_t v = ((int64_t)b << VB) + l;
+  const uint32_t ub = block_unit0[b], u1 = block_unit0[b + 1];
   double l1 = 0.0;
   uint64_t dsum = 0;
-  bool act = false;
-  if (v < n) {
-    const uint32_t u1 = block_unit0[b + 1];
-    uint32_t u = block_unit0[b];
-    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (; u + 8 <= u1; u += 8) {
+  bool act[FU];
+  int64_t vs[FU];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) a[k] += part[((int64_t)(u + k) << VB) + l];
+  for (int k = 0; k < FU; ++k) {
+    const int l = (int)(blockIdx.x % PER) * (256 * FU) + k * 256 + threadIdx.x;
+    const int64_t v = ((int64_t)b << VB) + l;
+    vs[k] = v;
+    act[k] = false;
+    if (v < n) {
+      uint32_t u = ub;
+      double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (; u + 8 <= u1; u += 8) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[q] += part[((int64_t)(u + q) << VB) + l];
+      }
+      for (int q = 0; u < u1; ++u, ++q) a[q] += part[((int64_t)u << VB) + l];
+      const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+      const uint64_t deg = off[v + 1] - off[v];
+      act[k] = prd_finalize(v, sum, p[v], deg, p, contrib_next, st, l1);
+      if (act[k]) dsum += deg;
     }
-    for (int k = 0; u < u1; ++u, ++k) a[k] += part[((int64_t)u << VB) + l];
-    const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-    const uint64_t deg = off[v + 1] - off[v];
-    act = prd_finalize(v, sum, p[v], deg, p, contrib_next, st, l1);
-    if (act) dsum = deg;
   }
-  wg_append<256, 1>(&act, &v, out_ids, info, s_cnt, &s_base);
-  wg_finish<256>(l1, dsum, &l1_part[blockIdx.x], info + 1);
+  wg_append<256, FU>(act, vs, out_ids, count, s_cnt, &s_base);
+  wg_finish<256>(l1, dsum, &l1_part[blockIdx.x], &deg_part[blockIdx.x]);
+}
+
+// The round's L1 norm and active out-degree sum from the workgroups' parts
+// (one workgroup, a fixed order).
+__global__ __launch_bounds__(256) void k_prd_round_sum(const double* l1_part, const uint64_t* deg_part,
+                                                       int64_t nparts, double* l1_out, int64_t* deg_sum) {
+  double a = 0.0;
+  uint64_t d = 0;
+  for (int64_t i = threadIdx.x; i < nparts; i += 256) {
+    a += l1_part[i];
+    d += deg_part[i];
+  }
+  wg_finish<256>(a, d, l1_out, reinterpret_cast<uint64_t*>(deg_sum));
 }
 
 int grid_n(rg_ctx* ctx, int64_t n) {
@@ -607,18 +633,20 @@ int pagerank_prepare(rg_graph* g, const rg_opts* o) {
 // The caller adds a block's partials in unit order: *vb_out and
 // *block_unit0_out describe the layout.
 int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, double* contrib_next, double* p,
-                      const PrdStep& st, uint32_t* out_ids, int64_t* info, const int64_t* zero, DevBuf& work,
-                      const double** l1_part_out, int64_t* nparts_out) {
+                      const PrdStep& st, uint32_t* out_ids, int64_t* info, double* l1, const int64_t* zero,
+                      DevBuf& work) {
   rg_ctx* ctx = g->ctx;
   int vb, sb;
   layout_bits(o, &vb, &sb);
   BlockedCSC* B = nullptr;
   RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B));
-  const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 256);
+  const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 1024);
+  const int64_t nparts = B->nunits + mgrid;
   const size_t unit_doubles = (size_t)B->nunits << vb;
-  RG_TRY(work.ensure(sizeof(double) * (unit_doubles + B->nunits + mgrid)));
+  RG_TRY(work.ensure(sizeof(double) * (unit_doubles + 2 * nparts)));
   double* part = work.as<double>();
   double* l1_part = part + unit_doubles;
+  uint64_t* deg_part = reinterpret_cast<uint64_t*>(l1_part + nparts);
   const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
   // Arcs in flight per lane: 0 (default) = 4 on a degree-sorted graph,
   // where the hub blocks' long units dominate (PageRankDelta on RMAT-26,
@@ -626,10 +654,10 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
   // RMAT-24 original ids: 118 GTEPS against 91).
   const bool u4 = o && o->unroll ? o->unroll == 4 : g->degree_sorted;
   void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*, const uint32_t*,
-               const double*, double*, double*, const uint64_t*, int64_t, double*, double*, uint32_t*,
-               int64_t*, const int64_t*, PrdStep);
+               const double*, double*, double*, const uint64_t*, int64_t, double*, double*, uint64_t*,
+               uint32_t*, int64_t*, const int64_t*, PrdStep);
   void (*merge)(const uint32_t*, const uint32_t*, const double*, double*, const uint64_t*, double*, int64_t,
-                double*, uint32_t*, int64_t*, const int64_t*, PrdStep);
+                double*, uint64_t*, uint32_t*, int64_t*, const int64_t*, PrdStep);
 #define RG_PICK(VB)                                                                                   \
   do {                                                                                                \
     kern = threads == 512 ? (u4 ? k_blk_prd_it<VB, 512, true, 4> : k_blk_prd_it<VB, 512, true, 1>)     \
@@ -644,15 +672,14 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
 #undef RG_PICK
   hipEvent_t* ev = timing_begin(ctx);
   hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, B->unit_block,
-                     B->block_unit0, contrib, contrib_next, p, g->off, g->n, part, l1_part, out_ids, info, zero,
-                     st);
+                     B->block_unit0, contrib, contrib_next, p, g->off, g->n, part, l1_part, deg_part, out_ids,
+                     info, zero, st);
   if (mgrid)
     hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, part, p, g->off,
-                       contrib_next, g->n, l1_part + B->nunits, out_ids, info, zero, st);
+                       contrib_next, g->n, l1_part + B->nunits, deg_part + B->nunits, out_ids, info, zero, st);
   if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
+  hipLaunchKernelGGL(k_prd_round_sum, 1, 256, 0, ctx->stream, l1_part, deg_part, nparts, l1, info + 1);
   RG_HIP(hipGetLastError());
-  *l1_part_out = l1_part;
-  *nparts_out = B->nunits + mgrid;
   return RG_OK;
 }
 

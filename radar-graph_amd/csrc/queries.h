@@ -90,12 +90,12 @@ struct PrdStep {
 // One dense PageRankDelta round on the blocked layout (blocked.hip): the
 // edge pass over contrib and the vertexFilter, fused.  Updates p, writes
 // every vertex's next quotient into contrib_next, appends the active
-// vertices to out_ids with their count and out-degree sum added to
-// info[0..1]; *l1_part_out[0, *nparts_out) are the |Delta| partial sums.
-// work holds the split blocks' partials and the L1 terms.
+// vertices to out_ids, adds their count to info[0] (zeroed by the caller)
+// and stores their out-degree sum in info[1] and the round's L1 norm in *l1.
+// work holds the split blocks' partials and the workgroups' sums.
 int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, double* contrib_next, double* p,
-                      const PrdStep& st, uint32_t* out_ids, int64_t* info, const int64_t* zero, DevBuf& work,
-                      const double** l1_part_out, int64_t* nparts_out);
+                      const PrdStep& st, uint32_t* out_ids, int64_t* info, double* l1, const int64_t* zero,
+                      DevBuf& work);
 
 // Builds the blocked PageRank layout for these options (once per graph).
 int pagerank_prepare(rg_graph* g, const rg_opts* o);

@@ -736,22 +736,16 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
   int64_t degsum = g->m;
   double L1 = 0.0;
   int64_t round = 0;
-  // With the blocked engine a pushed round costs ~20x a blocked round per
-  // arc (FP64 atomics against LDS accumulation; RMAT-26: 6.7 ms for a
-  // frontier just under m/20 arcs, 6.5 ms for a blocked round over all m),
-  // plus its vertexFilter over the touched vertices: the push pays below
-  // about m/24 arcs, unless the caller sets the threshold.
-  const int64_t blk_dense_at = (o && o->threshold >= 0) ? E.threshold : g->m / 24;
   EventTimer tm(ctx->stream);
   while (round++ < maxiters) {
     const uint32_t* list = nullptr;   // vertexFilter over the touched vertices only
-    const bool dense_blocked = blocked && (F->all || F->size + degsum > blk_dense_at);
+    const bool dense_blocked = blocked && (F->all || F->size + degsum > E.threshold);
     RG_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int64_t), ctx->stream));
     if (dense_blocked) {
       const PrdStep st{damping, added, one_over_n, epsilon2, round == 1 ? 1 : 0};
       RG_TRY(blocked_prd_round(g, o, contrib, contrib_alt, p, st, N->ids, info, l1_dev, zero, b_unit));
       std::swap(contrib, contrib_alt);
-    } else if (!F->all && (blocked || F->size + degsum <= E.threshold)) {
+    } else if (!F->all && F->size + degsum <= E.threshold) {
       // Sparse push (edge_map's list traversal) that also lists the vertices
       // it reaches; then the previous active vertices' contrib is cleared
       // (the vertex step sets it again for those still active).

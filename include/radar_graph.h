@@ -215,7 +215,9 @@ int rg_pagerank(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_o
  * vertices with |Delta| > 0.01 p; stop when sum |Delta| < 1e-7 or after
  * maxiters.  iters_out = printed "Num Iters" (maxiters+1 when not
  * converged), l1_out = "L1_Norm", ranks_out (nullable) = p.  RG_BLOCKED
- * runs dense rounds on the blocked PageRank layout. */
+ * runs dense rounds on the blocked PageRank layout, each as one edge pass
+ * with the vertexFilter fused (the active list comes out in no fixed
+ * order); sparse rounds push. */
 int rg_pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
                       int64_t* iters_out, double* l1_out, double* ms_out);
 

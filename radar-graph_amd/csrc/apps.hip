@@ -68,7 +68,10 @@ struct PrdTouchOp {
   __device__ bool skip() const { return false; }
   __device__ double load_src(uint32_t s) const { return contrib[s]; }
   __device__ bool cond(uint32_t) const { return true; }
-  __device__ bool first(uint32_t d) const { return atomicExch(stamp + d, tag) != tag; }
+  // A plain read first: once d carries this round's tag (most arcs into a
+  // hub) no returning atomic is issued; a stale read only costs the exchange,
+  // which still picks exactly one first toucher.
+  __device__ bool first(uint32_t d) const { return stamp[d] != tag && atomicExch(stamp + d, tag) != tag; }
   __device__ bool update(uint32_t, double sv, uint32_t d) const {
     unsafeAtomicAdd(acc + d, sv);
     return first(d);

@@ -92,7 +92,7 @@ void rg_hgraph_free(rg_hgraph* g);
 /* GAP SourcePicker (benchmark.h:47-68): mt19937(27491095) +
  * uniform_int_distribution<int32>(0,n-1), rejecting out-degree 0. */
 int rg_pick_sources_gap(const rg_hgraph* g, int count, uint32_t* out);
-/* BC's picker (BC-manyiter-v2.C:306-325): mt19937(27491095) +
+/* BC's picker (BC-manyiter-v2.C:149-168): mt19937(27491095) +
  * uniform_int_distribution<long>(0,n-1), rejecting out-degree 0 in the
  * orientation BC sees for that source (directed graphs alternate, because BC
  * transposes once per source).  Original ids. */
@@ -221,7 +221,7 @@ int rg_pagerank(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_o
 int rg_pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks_out,
                       int64_t* iters_out, double* l1_out, double* ms_out);
 
-/* BFS (PNO Ligra/apps/BFS.C:213-258): parents via first successful CAS,
+/* BFS (PNO Ligra/apps/BFS.C:73-117): parents via first successful CAS,
  * parents[src] = src, UINT32_MAX when unreached.  iters_out = rounds
  * (printed "Num. Iters"), reached_out = tree size, edges_out = sum of
  * out-degrees of reached vertices (the traversed-edge count for TEPS). */
@@ -229,7 +229,7 @@ int rg_bfs(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
            int64_t* iters_out, int64_t* reached_out, int64_t* edges_out, double* ms_out);
 
 /* Summary of one BFS query: iters = rounds with a nonempty frontier (the
- * printed "Num. Iters", BFS.C:253), reached = tree size (BFS.C:254-257),
+ * printed "Num. Iters", BFS.C:107), reached = tree size (BFS.C:108-115),
  * edges = sum of out-degrees of the reached vertices (TEPS numerator),
  * checksum = sum over reached v of rg_level_hash(v, level(v)) mod 2^64, which
  * identifies the level array.  No counterpart in the reference beyond the
@@ -240,7 +240,7 @@ typedef struct rg_bfs_summary {
 } rg_bfs_summary;
 uint64_t rg_level_hash(uint32_t v, uint32_t level);
 
-/* One BFS query on the single-source engine (BFS.C:213-258 semantics, as
+/* One BFS query on the single-source engine (BFS.C:73-117 semantics, as
  * rg_bfs), returning its summary. */
 int rg_bfs_query(rg_graph* g, uint32_t src, const rg_opts* o, rg_bfs_summary* out, double* ms_out);
 
@@ -261,7 +261,7 @@ int rg_bfs_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_op
 int rg_radii(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* o,
              int32_t* radii_out, int64_t* rounds_out, double* ms_out);
 
-/* BC (PNO Ligra/apps/BC-manyiter-v2.C:280-398), sources given in the graph's
+/* BC (PNO Ligra/apps/BC-manyiter-v2.C:124-241), sources given in the graph's
  * own id space, processed in order with the reference's semantics: forward
  * path counting by level, backward dependency accumulation on the transposed
  * graph (the transpose is not undone between sources), Dependencies carried
@@ -277,7 +277,7 @@ int rg_bc(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o,
  * source reaches, wsum = sum over the finite dependencies dep[v] of
  * dep[v] * w(v) with w(v) = (rg_level_hash(v, 0xffffffff) >> 11) * 2^-53,
  * nonfinite = count of non-finite dependencies (the unreached vertices are
- * NaN in BC-manyiter-v2.C:387-389's final step).  No counterpart in the
+ * NaN in BC-manyiter-v2.C:230-232's final step).  No counterpart in the
  * reference, which prints no per-source result. */
 typedef struct rg_bc_summary {
   int64_t rounds, reached;
@@ -333,11 +333,11 @@ int rg_array_read(const rg_array* a, void* host, int64_t first, int64_t count);
 int rg_array_fill(rg_array* a, uint64_t bits);  /* every element = low elem_bytes of bits */
 int rg_array_free(rg_array* a);
 /* edgeMap.  State per operator:
- *   RG_OP_BFS      {parents u32}                    BFS_F      (BFS.C:169-190)
- *   RG_OP_PAGERANK {contrib f64, p_next f64}        PR_F       (PageRank.C:29-43)
+ *   RG_OP_BFS      {parents u32}                    BFS_F      (BFS.C:28-49)
+ *   RG_OP_PAGERANK {contrib f64, p_next f64}        PR_F       (PageRank.C:30-43)
  *   RG_OP_RADII    {visited u64, next u64, radii i32} Radii_F  (Radii.C:35-64; `round`)
- *   RG_OP_BC_FWD   {num_paths f64, visited u8}      BC_F       (BC-manyiter-v2.C:189-213)
- *   RG_OP_BC_BACK  {dependencies f64, visited u8}   BC_Back_F  (BC-manyiter-v2.C:215-236)
+ *   RG_OP_BC_FWD   {num_paths f64, visited u8}      BC_F       (BC-manyiter-v2.C:32-56)
+ *   RG_OP_BC_BACK  {dependencies f64, visited u8}   BC_Back_F  (BC-manyiter-v2.C:58-79)
  * o (nullable) selects push / HUBDUP / RADAR hub copies. */
 int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int nstate, int64_t round,
                int64_t threshold, int flags, const rg_opts* o, rg_subset* out);
@@ -345,11 +345,11 @@ int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int n
  *   RG_VOP_PR_UPDATE  {p}: p = a*p + b             PR_Vertex_F (PageRank.C:46-58)
  *   RG_VOP_PR_CONTRIB {p, contrib}: contrib = p/outdeg, every vertex
  *   RG_VOP_RADII_SYNC {visited, next}: visited = next   Radii_Vertex_F (Radii.C:67-75)
- *   RG_VOP_MARK       {visited u8}: visited = 1     BC_Vertex_F (BC-manyiter-v2.C:239-246)
+ *   RG_VOP_MARK       {visited u8}: visited = 1     BC_Vertex_F (BC-manyiter-v2.C:82-89)
  *   RG_VOP_BC_BACK    {visited u8, deps, inv}: visited = 1, deps += inv
- *                                                   BC_Back_Vertex_F (BC-manyiter-v2.C:250-258)
- *   RG_VOP_INVERT     {x}: x = 1/x, every vertex    (BC-manyiter-v2.C:364-365)
- *   RG_VOP_BC_FINAL   {deps, inv}: deps = (deps - inv)/inv, every vertex (BC-manyiter-v2.C:387-389) */
+ *                                                   BC_Back_Vertex_F (BC-manyiter-v2.C:93-101)
+ *   RG_VOP_INVERT     {x}: x = 1/x, every vertex    (BC-manyiter-v2.C:206-208)
+ *   RG_VOP_BC_FINAL   {deps, inv}: deps = (deps - inv)/inv, every vertex (BC-manyiter-v2.C:230-232) */
 int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int nstate, double a, double b);
 
 /* ---- query serving ---------------------------------------------------------

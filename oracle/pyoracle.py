@@ -87,7 +87,7 @@ def level_hash(v, level):
 
 def bfs_summary(off, level):
     """(iters, reached, edges, checksum) of a BFS level array: rounds with a
-    nonempty frontier (BFS.C:253), tree size (BFS.C:254-257), out-degree sum
+    nonempty frontier (BFS.C:107), tree size (BFS.C:108-115), out-degree sum
     of the reached vertices, and the sum of level_hash(v, level) mod 2^64."""
     level = np.asarray(level)
     reached = np.nonzero(level >= 0)[0]

@@ -11,10 +11,10 @@
 //   * `free(p)` inside the app source is captured (not freed) while the
 //     wrapper runs, so the arrays the app frees last stay readable.
 // The arrays captured are the app's final frees, in source order:
-//   BFS.C:257        free(Parents)                       -> parents  uint32[n]
+//   BFS.C:116        free(Parents)                       -> parents  uint32[n]
 //   PageRank.C:136   free(p_curr); free(p_next)          -> ranks    f64[n]
 //   Radii.C:156      free(radii)                         -> radii    int32[n]
-//   BC-manyiter-v2.C:392-394 free(NumPaths); free(Visited); free(Dependencies)
+//   BC-manyiter-v2.C:235-237 free(NumPaths); free(Visited); free(Dependencies)
 //                                                        -> deps     f64[n]
 // Build: oracle/Makefile compiles this file once per (variant, app) with
 // -DRG_APP_SOURCE="\"<App>.C\"" and the reference's apps/ and ligra/ dirs on

@@ -287,7 +287,7 @@ void orc_radii(int64_t n, const uint64_t* off, const uint32_t* nbr, const uint32
 }
 
 /* ---- BC --------------------------------------------------------------------
- * push-new-orig Ligra/apps/BC-manyiter-v2.C:189-258 (functors), 280-398
+ * push-new-orig Ligra/apps/BC-manyiter-v2.C:32-101 (functors), 280-398
  * (Compute).  Frontiers are processed in ascending id order (sparse lists
  * from the bitmap scan, ligra.h:150-164, and dense loops are both
  * ascending). */
@@ -354,7 +354,7 @@ void orc_bc(int64_t n, const uint64_t* off, const uint32_t* nbr, const uint64_t*
       visited[v] = 1;
       deps[v] += np_[v];
     }
-    /* GA.transpose() -- never undone (BC-manyiter-v2.C:374). */
+    /* GA.transpose() -- never undone (BC-manyiter-v2.C:217). */
     { const uint64_t* a = co; co = to; to = a; const uint32_t* b = cn; cn = tn; tn = b; }
     for (int64_t r = round - 2; r >= 0; --r) {
       for (int64_t i = lstart[r + 1]; i < lstart[r + 2]; ++i) {

@@ -56,7 +56,7 @@ void orc_pagerank_pull(int64_t n, const uint64_t* off, const uint64_t* in_off,
 void orc_radii(int64_t n, const uint64_t* off, const uint32_t* nbr, const uint32_t* starts,
                int nstarts, int32_t* radii, int64_t* rounds);
 
-/* BC (push-new-orig Ligra/apps/BC-manyiter-v2.C:280-398) over the given
+/* BC (push-new-orig Ligra/apps/BC-manyiter-v2.C:124-241) over the given
  * sources (graph id space), with the reference's carry-over of
  * Dependencies between sources and its per-source transpose of a directed
  * graph.  For symmetric graphs pass in_off/in_nbr = off/nbr. */

@@ -1,5 +1,5 @@
 // Betweenness centrality with Ligra's interface (push-new-orig
-// Ligra/apps/BC-manyiter-v2.C:280-398): -maxiters sources (default 16) from
+// Ligra/apps/BC-manyiter-v2.C:124-241): -maxiters sources (default 16) from
 // mt19937(27491095), forward path counting level by level, then backward
 // dependency accumulation on the transposed graph; Dependencies carry over
 // between sources and the transpose is not undone, as in the reference.

@@ -1,4 +1,4 @@
-// BFS with Ligra's interface (push-new-orig Ligra/apps/BFS.C:213-258) on the
+// BFS with Ligra's interface (push-new-orig Ligra/apps/BFS.C:73-117) on the
 // radar_graph engine.  Usage: BFS [-s] [-b] [-r src] [-rounds R]
 // [-preprocess 0|1] [-variant push|hubdup|radar] [-k hubs] [-out file] <graph>
 #include <cstdio>

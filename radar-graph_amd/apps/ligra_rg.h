@@ -181,13 +181,13 @@ vertexSubset all_vertices(graph<V>& G) {
 }
 
 // The reference's edge functors as device operators.
-struct BFS_F {                 // BFS.C:169-190
+struct BFS_F {                 // BFS.C:28-49
   array<uintE> Parents;
   static constexpr int op = RG_OP_BFS;
   std::vector<rg_array*> state() const { return {Parents.a}; }
   long round() const { return 0; }
 };
-struct PR_F {                  // PageRank.C:29-43 (contrib[s] = p_curr[s]/outdeg(s))
+struct PR_F {                  // PageRank.C:30-43 (contrib[s] = p_curr[s]/outdeg(s))
   array<double> contrib, p_next;
   static constexpr int op = RG_OP_PAGERANK;
   std::vector<rg_array*> state() const { return {contrib.a, p_next.a}; }
@@ -201,14 +201,14 @@ struct Radii_F {               // Radii.C:35-64
   std::vector<rg_array*> state() const { return {Visited.a, NextVisited.a, radii.a}; }
   long round() const { return r; }
 };
-struct BC_F {                  // BC-manyiter-v2.C:189-213
+struct BC_F {                  // BC-manyiter-v2.C:32-56
   array<double> NumPaths;
   array<uint8_t> Visited;
   static constexpr int op = RG_OP_BC_FWD;
   std::vector<rg_array*> state() const { return {NumPaths.a, Visited.a}; }
   long round() const { return 0; }
 };
-struct BC_Back_F {             // BC-manyiter-v2.C:215-236
+struct BC_Back_F {             // BC-manyiter-v2.C:58-79
   array<double> Dependencies;
   array<uint8_t> Visited;
   static constexpr int op = RG_OP_BC_BACK;
@@ -251,24 +251,24 @@ struct Radii_Vertex_F {        // Radii.C:67-75
   static constexpr int vop = RG_VOP_RADII_SYNC;
   std::vector<rg_array*> state() const { return {Visited.a, NextVisited.a}; }
 };
-struct BC_Vertex_F {           // BC-manyiter-v2.C:239-246
+struct BC_Vertex_F {           // BC-manyiter-v2.C:82-89
   array<uint8_t> Visited;
   static constexpr int vop = RG_VOP_MARK;
   std::vector<rg_array*> state() const { return {Visited.a}; }
 };
-struct BC_Back_Vertex_F {      // BC-manyiter-v2.C:250-258
+struct BC_Back_Vertex_F {      // BC-manyiter-v2.C:93-101
   array<uint8_t> Visited;
   array<double> Dependencies, inverseNumPaths;
   static constexpr int vop = RG_VOP_BC_BACK;
   std::vector<rg_array*> state() const { return {Visited.a, Dependencies.a, inverseNumPaths.a}; }
 };
 
-struct Invert_F {              // inverseNumPaths[i] = 1/NumPaths[i] (BC-manyiter-v2.C:364-365)
+struct Invert_F {              // inverseNumPaths[i] = 1/NumPaths[i] (BC-manyiter-v2.C:206-208)
   array<double> x;
   static constexpr int vop = RG_VOP_INVERT;
   std::vector<rg_array*> state() const { return {x.a}; }
 };
-struct BC_Final_F {            // Dependencies = (Dep - inv)/inv (BC-manyiter-v2.C:387-389)
+struct BC_Final_F {            // Dependencies = (Dep - inv)/inv (BC-manyiter-v2.C:230-232)
   array<double> Dependencies, inverseNumPaths;
   static constexpr int vop = RG_VOP_BC_FINAL;
   std::vector<rg_array*> state() const { return {Dependencies.a, inverseNumPaths.a}; }

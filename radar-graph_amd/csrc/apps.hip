@@ -20,7 +20,7 @@ namespace rg {
 // Operators.  Each mirrors one reference functor; citations are to
 // push-new-orig/Ligra/apps/ unless noted.
 
-// PR_F (PageRank.C:29-43): p_next[d] += p_curr[s]/outdeg(s).  contrib[s]
+// PR_F (PageRank.C:30-43): p_next[d] += p_curr[s]/outdeg(s).  contrib[s]
 // holds p_curr[s]/outdeg(s), computed once per vertex by the vertex phase
 // (the same IEEE quotient the reference recomputes per arc).
 struct PrOp {
@@ -86,9 +86,9 @@ struct PrdTouchOp {
   __device__ double hub_combine(double x, double y) const { return x + y; }
 };
 
-// BFS_F (BFS.C:169-190): first CAS on Parents[d] wins and d joins the next
+// BFS_F (BFS.C:28-49): first CAS on Parents[d] wins and d joins the next
 // frontier.  Hub copies keep one candidate parent per hub per workgroup
-// (hubdup BFS.C:40-47, 51-69).
+// (hubdup BFS.C:46-68, 72-97).
 struct BfsOp {
   using SrcT = uint32_t;
   using HubT = uint32_t;
@@ -155,7 +155,7 @@ struct RadiiOp {
   }
 };
 
-// BC_F (BC-manyiter-v2.C:189-213): NumPaths[d] += NumPaths[s] for unvisited
+// BC_F (BC-manyiter-v2.C:32-56): NumPaths[d] += NumPaths[s] for unvisited
 // d; d joins when the add saw 0.
 struct BcFwdOp {
   using SrcT = double;
@@ -177,7 +177,7 @@ struct BcFwdOp {
   __device__ bool hub_flush(uint32_t d, double v) const { return unsafeAtomicAdd(np + d, v) == 0.0; }
 };
 
-// BC_Back_F (BC-manyiter-v2.C:215-236): Dependencies[d] += Dependencies[s].
+// BC_Back_F (BC-manyiter-v2.C:58-79): Dependencies[d] += Dependencies[s].
 struct BcBackOp {
   using SrcT = double;
   using HubT = double;
@@ -787,7 +787,7 @@ int pagerank_delta(rg_graph* g, int64_t maxiters, const rg_opts* o, double* rank
 }
 
 // ---------------------------------------------------------------------------
-// BFS (BFS.C:213-258).
+// BFS (BFS.C:73-117).
 
 // Sum of level_hash(v, level) over a frontier list whose size is on the
 // device (the level checksum of a BFS query).
@@ -971,7 +971,7 @@ int radii_push(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* 
 }
 
 // ---------------------------------------------------------------------------
-// BC (BC-manyiter-v2.C:280-398).
+// BC (BC-manyiter-v2.C:124-241).
 
 __global__ void k_bc_start(double* np, uint8_t* visited, uint32_t start) {
   np[start] = 1.0;
@@ -984,7 +984,7 @@ __global__ void k_mark(const uint32_t* ids, int64_t count, uint8_t* visited) {
     visited[ids[i]] = 1;
 }
 
-// BC_Back_Vertex_F (BC-manyiter-v2.C:250-258).
+// BC_Back_Vertex_F (BC-manyiter-v2.C:93-101).
 __global__ void k_bc_back_vertex(const uint32_t* ids, int64_t count, uint8_t* visited, double* deps,
                                  const double* inv) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
@@ -1487,12 +1487,12 @@ int rg_array_free(rg_array* a) {
 }
 
 // edgeMap (ligra.h:264-302, 336-340).  Operators and their state arrays:
-//   RG_OP_BFS      [parents u32]                       BFS_F (BFS.C:169-190)
-//   RG_OP_PAGERANK [contrib f64, p_next f64]           PR_F (PageRank.C:29-43); contrib[s] =
+//   RG_OP_BFS      [parents u32]                       BFS_F (BFS.C:28-49)
+//   RG_OP_PAGERANK [contrib f64, p_next f64]           PR_F (PageRank.C:30-43); contrib[s] =
 //                                                      p_curr[s]/outdeg(s) (rg_vertexmap RG_VOP_PR_CONTRIB)
 //   RG_OP_RADII    [visited u64, next_visited u64, radii i32]   Radii_F (Radii.C:35-64), `round`
-//   RG_OP_BC_FWD   [num_paths f64, visited u8]         BC_F (BC-manyiter-v2.C:189-213)
-//   RG_OP_BC_BACK  [dependencies f64, visited u8]      BC_Back_F (BC-manyiter-v2.C:215-236)
+//   RG_OP_BC_FWD   [num_paths f64, visited u8]         BC_F (BC-manyiter-v2.C:32-56)
+//   RG_OP_BC_BACK  [dependencies f64, visited u8]      BC_Back_F (BC-manyiter-v2.C:58-79)
 int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int nstate, int64_t round,
                int64_t threshold, int flags, const rg_opts* o, rg_subset* out) {
   if (!g || !in || in->g->n != g->n) return set_error(RG_EINVAL, "edgeMap: Sizes Don't match");
@@ -1596,11 +1596,11 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
       hipLaunchKernelGGL(rg::k_sync_u64, grid, 256, 0, ctx->stream, ids, cnt, s->all,
                          state[0]->buf.as<unsigned long long>(), state[1]->buf.as<unsigned long long>());
       break;
-    case RG_VOP_MARK:        // BC_Vertex_F (BC-manyiter-v2.C:239-246): Visited = 1
+    case RG_VOP_MARK:        // BC_Vertex_F (BC-manyiter-v2.C:82-89): Visited = 1
       RG_TRY(rg::check_state(g, state, nstate, {1}, "vertexMap(BC_Vertex_F)"));
       hipLaunchKernelGGL(rg::k_mark_u8, grid, 256, 0, ctx->stream, ids, cnt, s->all, state[0]->buf.as<uint8_t>());
       break;
-    case RG_VOP_BC_BACK:     // BC_Back_Vertex_F (BC-manyiter-v2.C:250-258)
+    case RG_VOP_BC_BACK:     // BC_Back_Vertex_F (BC-manyiter-v2.C:93-101)
       RG_TRY(rg::check_state(g, state, nstate, {1, 8, 8}, "vertexMap(BC_Back_Vertex_F)"));
       if (s->all)
         hipLaunchKernelGGL(rg::k_bc_back_vertex_all, grid, 256, 0, ctx->stream, g->n, state[0]->buf.as<uint8_t>(),
@@ -1609,12 +1609,12 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
         hipLaunchKernelGGL(rg::k_bc_back_vertex_l, grid, 256, 0, ctx->stream, ids, cnt, state[0]->buf.as<uint8_t>(),
                            state[1]->buf.as<double>(), state[2]->buf.as<double>());
       break;
-    case RG_VOP_INVERT:      // inverseNumPaths = 1/NumPaths, every vertex (BC-manyiter-v2.C:364-365)
+    case RG_VOP_INVERT:      // inverseNumPaths = 1/NumPaths, every vertex (BC-manyiter-v2.C:206-208)
       RG_TRY(rg::check_state(g, state, nstate, {8}, "vertexMap(invert)"));
       hipLaunchKernelGGL(rg::k_invert, rg::grid_for(ctx, g->n), 256, 0, ctx->stream, state[0]->buf.as<double>(),
                          g->n);
       break;
-    case RG_VOP_BC_FINAL:    // Dep = (Dep - inv)/inv, every vertex (BC-manyiter-v2.C:387-389)
+    case RG_VOP_BC_FINAL:    // Dep = (Dep - inv)/inv, every vertex (BC-manyiter-v2.C:230-232)
       RG_TRY(rg::check_state(g, state, nstate, {8, 8}, "vertexMap(BC final)"));
       hipLaunchKernelGGL(rg::k_bc_final, rg::grid_for(ctx, g->n), 256, 0, ctx->stream, state[0]->buf.as<double>(),
                          state[1]->buf.as<double>(), g->n);

@@ -1,5 +1,5 @@
 // Batched betweenness centrality: up to 64 independent single-source BC
-// queries (BC-manyiter-v2.C:280-398 run with one source each) answered
+// queries (BC-manyiter-v2.C:124-241 run with one source each) answered
 // together.
 //
 // Forward: the levels come from the bit-parallel BFS of msbfs.hip (one
@@ -8,17 +8,17 @@
 // L + 1, one wavefront computes the path counts of all those sources at
 // once, lane s for source s:
 //     sigma_s(d) = sum of sigma_s(u) over the in-neighbours u of d at level L from s
-// (BC_F, BC-manyiter-v2.C:189-213, in pull form).  The in-edges are read 64
+// (BC_F, BC-manyiter-v2.C:32-56, in pull form).  The in-edges are read 64
 // at a time, a lane per edge; the edges whose tail is on the current
 // frontier of one of d's new sources are then taken one by one with a lane
 // per source, each lane reading its own word of the tail's 64-word row of
 // path counts: one coalesced 512-byte row per such edge.
 //
 // Backward, level r from the deepest down to 0 (BC_Back_F and
-// BC_Back_Vertex_F, BC-manyiter-v2.C:215-258, in pull form): for each vertex
+// BC_Back_Vertex_F, BC-manyiter-v2.C:58-101, in pull form): for each vertex
 // v at level r from source s,
 //     D_s(v) = 1/sigma_s(v) + sum of D_s(w) over the out-neighbours w at level r + 1 from s
-// and the dependency is (D - 1/sigma) / (1/sigma) (BC-manyiter-v2.C:387-389),
+// and the dependency is (D - 1/sigma) / (1/sigma) (BC-manyiter-v2.C:230-232),
 // NaN for the vertices the source does not reach (1/0 there).  The same
 // gather as the forward one, over out-edges: a lane per edge tests the
 // head's level-(r + 1) word against v's level-r word, then a lane per source

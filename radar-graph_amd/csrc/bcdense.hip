@@ -182,7 +182,7 @@ __global__ __launch_bounds__(THREADS) void k_bc_merge(BcDenseArgs a) {
   if (FWD) wq_finish(q, ea);
 }
 
-// BFS_F (BFS.C:169-190) on a dense level: every unvisited destination of
+// BFS_F (BFS.C:28-49) on a dense level: every unvisited destination of
 // the block takes a frontier in-neighbour as its parent (the smallest-id
 // one seen by the unit, any being a valid BFS parent); split blocks keep
 // the first unit's candidate.  `out` is Parents (uint32).

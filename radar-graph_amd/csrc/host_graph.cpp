@@ -717,13 +717,13 @@ int rg_pick_sources_gap(const rg_hgraph* g, int count, uint32_t* out) {
   return RG_OK;
 }
 
-// BC's source sequence (BC-manyiter-v2.C:306-325): mt19937(27491095) with
+// BC's source sequence (BC-manyiter-v2.C:149-168): mt19937(27491095) with
 // uniform_int_distribution<long>(0, n-1), rejecting sources whose out-degree
 // is 0 in the graph's *current* orientation.  BC transposes a directed graph
-// once per source and never undoes it (BC-manyiter-v2.C:374), so source i is
+// once per source and never undoes it (BC-manyiter-v2.C:217), so source i is
 // tested against out-degrees for even i and in-degrees for odd i.  Ids are in
 // the graph's original space (a degree-sorted run maps them through new_ids,
-// BC-manyiter-v2.C:333-334; the degree test is the same either way).
+// BC-manyiter-v2.C:176-177; the degree test is the same either way).
 int rg_pick_sources_bc(const rg_hgraph* g, int count, uint32_t* out) {
   if (!g || !out || count < 0) return set_error(RG_EINVAL, "rg_pick_sources_bc: bad arguments");
   if (g->m == 0) return set_error(RG_EINVAL, "rg_pick_sources_bc: graph has no edges");

@@ -110,7 +110,7 @@ def test_edge_cases(rg):
 
 def test_bc_picker_alternates_orientation(rg):
     # Directed: odd-numbered BC sources are tested against in-degrees
-    # because BC transposes once per source (BC-manyiter-v2.C:374).
+    # because BC transposes once per source (BC-manyiter-v2.C:217).
     g = rg.HostGraph.generate(10, 4, symmetric=False)
     srcs = g.pick_bc_sources(6)
     out_deg = np.diff(g.offsets())

@@ -1,8 +1,9 @@
 """GPU tests of the HIP-graph replay of iteration loops: the blocked
 PageRank loop captured once per context and replayed.  Every replay, with
 other eager work and synchronisations between the replays, agrees with the
-eager launches (the only difference being the LDS atomic order: 1e-10
-relative, iterations exact) and with the oracle; a changed graph or
+eager launches (the only difference being the LDS atomic order: ranks
+within 1e-10 relative L-inf, L1 within 1e-10 of the rank sum, iterations
+exact) and with the oracle; a changed graph or
 iteration count is captured again."""
 import numpy as np
 import pytest
@@ -34,7 +35,11 @@ def test_pagerank_graph_replays_match_eager(rg, orc, ctx, graphs):
             r = dg.pagerank(20, o)
             assert r.iters == eager.iters, rep
             assert rel(r.ranks, eager.ranks) < FP_TOL, rep
-            assert abs(r.l1 - eager.l1) <= FP_TOL * abs(eager.l1) + 1e-300, rep
+            # L1 sums |p_curr - p_next|, differences of nearly equal ranks
+            # whose last bits follow the LDS atomic order, so eager and
+            # replay agree absolutely, to the ranks' error summed over n,
+            # not relative to L1 itself.
+            assert abs(r.l1 - eager.l1) <= FP_TOL * float(np.sum(eager.ranks)), rep
             # eager work on the same context between the replays
             dg.bfs(int(src[rep % 3]))
             dg.bc_batch(src)

@@ -319,7 +319,8 @@ typedef struct rg_array rg_array;
 enum { RG_NO_OUTPUT = 1, RG_DENSE_FORWARD = 8 };   /* Ligra's flags (ligra.h:51-58) */
 enum { RG_OP_BFS = 0, RG_OP_PAGERANK = 1, RG_OP_RADII = 2, RG_OP_BC_FWD = 3, RG_OP_BC_BACK = 4 };
 enum { RG_VOP_PR_UPDATE = 0, RG_VOP_PR_CONTRIB = 1, RG_VOP_RADII_SYNC = 2, RG_VOP_MARK = 3,
-       RG_VOP_BC_BACK = 4, RG_VOP_INVERT = 5, RG_VOP_BC_FINAL = 6 };
+       RG_VOP_BC_BACK = 4, RG_VOP_INVERT = 5, RG_VOP_BC_FINAL = 6, RG_VOP_ABS_DIFF = 7, RG_VOP_ABS = 8 };
+enum { RG_FOP_PRD_FIRST = 0, RG_FOP_PRD = 1, RG_FOP_U32_EQ = 2, RG_FOP_U8_NONZERO = 3 };
 int rg_subset_create(rg_graph* g, rg_subset** out);                 /* empty subset  */
 int rg_subset_set(rg_subset* s, const uint32_t* ids, int64_t count); /* from host ids */
 int rg_subset_set_all(rg_subset* s);                                 /* every vertex  */
@@ -332,6 +333,13 @@ int rg_array_write(rg_array* a, const void* host, int64_t first, int64_t count);
 int rg_array_read(const rg_array* a, void* host, int64_t first, int64_t count);
 int rg_array_fill(rg_array* a, uint64_t bits);  /* every element = low elem_bytes of bits */
 int rg_array_free(rg_array* a);
+/* sequence::plusReduce (PNO Ligra/ligra/sequence.h, as PageRank.C:128 and
+ * PageRankDelta.C:138 call it) over an array of doubles, summed in a fixed
+ * order (the same bits on every call). */
+int rg_array_sum(const rg_array* a, double* out);
+/* The array's device address, for entry points that take device pointers
+ * into caller memory.  No counterpart in the reference. */
+int rg_array_device_ptr(const rg_array* a, void** out);
 /* edgeMap.  State per operator:
  *   RG_OP_BFS      {parents u32}                    BFS_F      (BFS.C:28-49)
  *   RG_OP_PAGERANK {contrib f64, p_next f64}        PR_F       (PageRank.C:30-43)
@@ -349,8 +357,23 @@ int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int n
  *   RG_VOP_BC_BACK    {visited u8, deps, inv}: visited = 1, deps += inv
  *                                                   BC_Back_Vertex_F (BC-manyiter-v2.C:93-101)
  *   RG_VOP_INVERT     {x}: x = 1/x, every vertex    (BC-manyiter-v2.C:206-208)
- *   RG_VOP_BC_FINAL   {deps, inv}: deps = (deps - inv)/inv, every vertex (BC-manyiter-v2.C:230-232) */
+ *   RG_VOP_BC_FINAL   {deps, inv}: deps = (deps - inv)/inv, every vertex (BC-manyiter-v2.C:230-232)
+ *   RG_VOP_ABS_DIFF   {x, y}: x = |x - y|           (PageRank.C:125-127, the L1 terms)
+ *   RG_VOP_ABS        {x, y}: x = |y|               (PageRankDelta.C:136-137) */
 int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int nstate, double a, double b);
+/* vertexFilter(V, filter) (PNO Ligra/ligra/ligra.h:500-509): applies the
+ * filter functor to every vertex of s and returns in `out` (which must not
+ * be a listed s) the vertices it kept, as an ascending list -- the order of
+ * Ligra's dense result.  Functors and their state / params:
+ *   RG_FOP_PRD_FIRST  {p, Delta, nghSum} f64; {damping, one_over_n, epsilon2}
+ *                     PR_Vertex_F_FirstRound (PageRankDelta.C:49-63)
+ *   RG_FOP_PRD        {p, Delta, nghSum} f64; {damping, epsilon2}
+ *                     PR_Vertex_F (PageRankDelta.C:65-77)
+ *   RG_FOP_U32_EQ     {x u32}; {value}: keeps x[v] == value (e.g. unvisited
+ *                     BFS parents, BFS.C:47's commented cond)
+ *   RG_FOP_U8_NONZERO {x u8}; {}: keeps x[v] != 0 */
+int rg_vertexfilter(rg_graph* g, int fop, rg_subset* s, rg_array* const* state, int nstate, const double* params,
+                    int nparams, rg_subset* out);
 
 /* ---- query serving ---------------------------------------------------------
  * A request loop over a resident graph (no counterpart in the reference,

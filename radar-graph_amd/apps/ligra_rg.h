@@ -129,6 +129,11 @@ struct array {
     check(rg_array_fill(a, bits), "fill");
   }
   void set(long i, T v) { check(rg_array_write(a, &v, i, 1), "write"); }
+  // swap(p_curr, p_next) of two arrays (PageRank.C:132).
+  friend void swap(array& x, array& y) {
+    std::swap(x.a, y.a);
+    std::swap(x.n, y.n);
+  }
   std::vector<T> get() const {
     std::vector<T> h(n);
     check(rg_array_read(a, h.data(), 0, n), "read");
@@ -274,10 +279,65 @@ struct BC_Final_F {            // Dependencies = (Dep - inv)/inv (BC-manyiter-v2
   std::vector<rg_array*> state() const { return {Dependencies.a, inverseNumPaths.a}; }
 };
 
+struct PR_Vertex_Contrib_F {   // contrib[i] = p[i] / outdeg(i): PR_F's quotient (PageRank.C:36-40)
+  array<double> p, contrib;
+  static constexpr int vop = RG_VOP_PR_CONTRIB;
+  std::vector<rg_array*> state() const { return {p.a, contrib.a}; }
+};
+struct Abs_Diff_F {            // x[i] = fabs(x[i] - y[i]) (PageRank.C:125-127)
+  array<double> x, y;
+  static constexpr int vop = RG_VOP_ABS_DIFF;
+  std::vector<rg_array*> state() const { return {x.a, y.a}; }
+};
+struct Abs_F {                 // x[i] = fabs(y[i]) (PageRankDelta.C:136-137)
+  array<double> x, y;
+  static constexpr int vop = RG_VOP_ABS;
+  std::vector<rg_array*> state() const { return {x.a, y.a}; }
+};
+
 template <class V, class F>
 void vertexMap(graph<V>& GA, vertexSubset& vs, F f, double a = 0.0, double b = 0.0) {
   std::vector<rg_array*> st = f.state();
   check(rg_vertexmap(GA.g, F::vop, vs.s, st.data(), (int)st.size(), a, b), "vertexMap");
+}
+
+// Filter functors (vertexFilter, ligra.h:500-509).
+struct PR_Vertex_F_FirstRound {   // PageRankDelta.C:49-63
+  array<double> p, Delta, nghSum;
+  double damping, one_over_n, epsilon2;
+  static constexpr int fop = RG_FOP_PRD_FIRST;
+  std::vector<rg_array*> state() const { return {p.a, Delta.a, nghSum.a}; }
+  std::vector<double> params() const { return {damping, one_over_n, epsilon2}; }
+};
+struct PR_Delta_Vertex_F {        // PR_Vertex_F of PageRankDelta.C:65-77
+  array<double> p, Delta, nghSum;
+  double damping, epsilon2;
+  static constexpr int fop = RG_FOP_PRD;
+  std::vector<rg_array*> state() const { return {p.a, Delta.a, nghSum.a}; }
+  std::vector<double> params() const { return {damping, epsilon2}; }
+};
+struct Unvisited_F {              // keeps Parents[i] == UINT_E_MAX (BFS.C:47)
+  array<uintE> Parents;
+  static constexpr int fop = RG_FOP_U32_EQ;
+  std::vector<rg_array*> state() const { return {Parents.a}; }
+  std::vector<double> params() const { return {(double)UINT_E_MAX}; }
+};
+
+template <class V, class F>
+vertexSubset vertexFilter(graph<V>& GA, vertexSubset& vs, F f) {
+  vertexSubset out(GA, GA.n);
+  std::vector<rg_array*> st = f.state();
+  std::vector<double> pr = f.params();
+  check(rg_vertexfilter(GA.g, F::fop, vs.s, st.data(), (int)st.size(), pr.data(), (int)pr.size(), out.s),
+        "vertexFilter");
+  return out;
+}
+
+// sequence::plusReduce over a device array of doubles.
+inline double plusReduce(const array<double>& x) {
+  double s = 0.0;
+  check(rg_array_sum(x.a, &s), "plusReduce");
+  return s;
 }
 
 template <class vertex>

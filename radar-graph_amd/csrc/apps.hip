@@ -278,7 +278,7 @@ static int edge_map(const Engine& E, const Op& op, Frontier& F, int64_t f_degsum
   a.nbr = g->nbr;
   a.out_ids = out_ids;
   a.out_info = out_info;
-  a.heavy = E.heavy->as<uint64_t>();
+  a.heavy = E.heavy;
   a.heavy_count = E.heavy_count;
   a.heavy_deg = E.heavy_deg;
   a.hub_k = E.hub_k;
@@ -323,8 +323,8 @@ static int edge_map(const Engine& E, const Op& op, Frontier& F, int64_t f_degsum
 
 // Sum of out-degrees of a frontier list (device), read back.
 static int frontier_degsum(rg_ctx* ctx, const rg_graph* g, const uint32_t* ids, int64_t count,
-                           int64_t* out) {
-  int64_t* d = ctx->counts.as<int64_t>() + 8;
+                           int64_t* out, int64_t* counter = nullptr) {
+  int64_t* d = counter ? counter : ctx->counts.as<int64_t>() + 8;
   RG_HIP(hipMemsetAsync(d, 0, sizeof(int64_t), ctx->stream));
   if (count > 0)
     hipLaunchKernelGGL(k_degsum, grid_for(ctx, count), 256, 0, ctx->stream, ids, count, g->off, d);
@@ -412,7 +412,8 @@ static int build_hub_bitmap(rg_graph* g, int64_t k) {
 }
 
 // Resolves variant/opts into an Engine.  elem = bytes of one hub copy entry.
-int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E) {
+int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E, uint64_t* heavy, int64_t heavy_cap,
+                int64_t* heavy_count) {
   rg_ctx* ctx = g->ctx;
   E.ctx = ctx;
   E.g = g;
@@ -462,9 +463,16 @@ int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E) {
     }
   }
   // Heavy queue: at most m/heavy_deg heavy vertices, each split in chunks.
-  int64_t cap = g->m / kHeavyChunk + g->m / std::max(1, E.heavy_deg) + 64;
-  E.heavy = &ctx->scratch[7];
-  RG_TRY(E.heavy->ensure(sizeof(uint64_t) * cap));
+  int64_t cap = heavy_queue_items(g->m, E.heavy_deg);
+  if (heavy) {
+    if (heavy_cap < cap || !heavy_count) return set_error(RG_EINVAL, "edgeMap: heavy queue too small");
+    E.heavy = heavy;
+    E.heavy_cap = heavy_cap;
+    E.heavy_count = heavy_count;
+    return RG_OK;
+  }
+  RG_TRY(ctx->scratch[7].ensure(sizeof(uint64_t) * cap));
+  E.heavy = ctx->scratch[7].as<uint64_t>();
   E.heavy_cap = cap;
   E.heavy_count = ctx->counts.as<int64_t>() + 0;
   return RG_OK;
@@ -808,12 +816,18 @@ constexpr int64_t kPullStayArcCost = 16;
 
 int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
              int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev,
-             uint64_t* checksum_out) {
+             uint64_t* checksum_out, const BfsWork* work) {
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
   if ((int64_t)src >= n) return set_error(RG_EINVAL, "rg_bfs: source out of range");
+  if (work && (!parents_dev || ms_out || (o && o->variant != RG_PUSH)))
+    return set_error(RG_EINVAL, "rg_bfs: a caller workspace runs the push engine into device parents");
+  int64_t* counts = work ? work->counts : ctx->counts.as<int64_t>();
   Engine E;
-  RG_TRY(make_engine(g, o, sizeof(uint32_t), E));
+  if (work)
+    RG_TRY(make_engine(g, o, sizeof(uint32_t), E, work->heavy, work->heavy_cap, counts + 0));
+  else
+    RG_TRY(make_engine(g, o, sizeof(uint32_t), E));
   DevBuf& b_par = ctx->scratch[0];
   uint32_t* parents = parents_dev;
   if (!parents) {
@@ -821,20 +835,26 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
     parents = b_par.as<uint32_t>();
   }
   Frontier fa, fb;
-  RG_TRY(ctx_frontiers(ctx, n, fa, fb));
-  int64_t* info = ctx->counts.as<int64_t>() + 4;   // [count, degsum]
+  if (work) {
+    fa.ids = work->ids[0];
+    fb.ids = work->ids[1];
+    fa.dense = fb.dense = work->dense;
+  } else {
+    RG_TRY(ctx_frontiers(ctx, n, fa, fb));
+  }
+  int64_t* info = counts + 4;   // [count, degsum]
   RG_TRY(fill<uint32_t>(ctx, parents, n, kNone));
   hipLaunchKernelGGL(k_set<uint32_t>, 1, 1, 0, ctx->stream, parents, (int64_t)src, src);
   hipLaunchKernelGGL(k_set<uint32_t>, 1, 1, 0, ctx->stream, fa.ids, (int64_t)0, src);
   RG_HIP(hipGetLastError());
   fa.size = 1;
   int64_t degsum = 0;
-  RG_TRY(frontier_degsum(ctx, g, fa.ids, 1, &degsum));
+  RG_TRY(frontier_degsum(ctx, g, fa.ids, 1, &degsum, counts + 8));
   BfsOp op{parents};
   Frontier* F = &fa;
   Frontier* N = &fb;
   int64_t iter = 0;
-  unsigned long long* cs = reinterpret_cast<unsigned long long*>(ctx->counts.as<int64_t>() + 12);
+  unsigned long long* cs = reinterpret_cast<unsigned long long*>(counts + 12);
   if (checksum_out) RG_HIP(hipMemsetAsync(cs, 0, sizeof(int64_t), ctx->stream));
   // RADAR and blocked run dense levels on the blocked layout (each unvisited
   // vertex scans its in-edges for a frontier parent; no global atomics).
@@ -849,7 +869,8 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   // Reached vertices and their out-degree sum accumulate from each level's
   // frontier size and degree sum (the source counts once).
   int64_t reached = 1, edges = degsum;
-  EventTimer tm(ctx->stream);
+  std::unique_ptr<EventTimer> tm;
+  if (ms_out) tm.reset(new EventTimer(ctx->stream));
   // Consecutive pull levels pass the frontier on as a complete bitmap.
   const unsigned long long* pull_bits = nullptr;
   while (F->size > 0) {
@@ -879,11 +900,10 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
     std::swap(F, N);
     ++iter;
   }
-  double ms = tm.stop_ms();
+  if (ms_out) *ms_out = tm->stop_ms();
   if (iters_out) *iters_out = iter;
   if (reached_out) *reached_out = reached;
   if (edges_out) *edges_out = edges;
-  if (ms_out) *ms_out = ms;
   if (checksum_out) {
     int64_t c = 0;
     RG_TRY(ctx_read_count(ctx, reinterpret_cast<int64_t*>(cs), &c, 1));
@@ -1374,6 +1394,156 @@ __global__ void k_pr_contrib(const uint64_t* off, int64_t n, const double* p, do
   }
 }
 
+// x = |x - y| (PageRank.C:125-127) and x = |y| (PageRankDelta.C:136-137).
+__global__ void k_abs_diff(const uint32_t* ids, int64_t count, bool all, double* x, const double* y, int diff) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = all ? (uint32_t)i : ids[i];
+    x[v] = diff ? fabs(x[v] - y[v]) : fabs(y[v]);
+  }
+}
+
+// ---- vertexFilter (ligra.h:500-509) ---------------------------------------
+// Filter functors write a keep flag per vertex of the input subset; the
+// flags are then compacted into an ascending id list (the order toSparse
+// gives Ligra's dense output).
+
+// PR_Vertex_F_FirstRound (PageRankDelta.C:49-63) and PR_Vertex_F (65-77);
+// multiply and add rounded separately, as the reference's host code.
+__global__ void k_filter_prd(const uint32_t* ids, int64_t count, bool all, double* p, double* delta,
+                             const double* ngh, double damping, double added, double one_over_n,
+                             double eps2, int first, uint8_t* keep) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = all ? (uint32_t)i : ids[i];
+    bool k;
+    if (first) {
+      double d = __dadd_rn(__dmul_rn(damping, ngh[v]), added);
+      double pv = p[v] + d;
+      p[v] = pv;
+      d = d - one_over_n;
+      delta[v] = d;
+      k = fabs(d) > __dmul_rn(eps2, pv);
+    } else {
+      double d = __dmul_rn(ngh[v], damping);
+      delta[v] = d;
+      k = fabs(d) > __dmul_rn(eps2, p[v]);
+      if (k) p[v] += d;
+    }
+    keep[v] = k;
+  }
+}
+
+// Keeps v with x[v] == value (4-byte elements) or x[v] != 0 (1-byte).
+__global__ void k_filter_u32_eq(const uint32_t* ids, int64_t count, bool all, const uint32_t* x, uint32_t value,
+                                uint8_t* keep) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = all ? (uint32_t)i : ids[i];
+    keep[v] = x[v] == value;
+  }
+}
+__global__ void k_filter_u8_nz(const uint32_t* ids, int64_t count, bool all, const uint8_t* x, uint8_t* keep) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = all ? (uint32_t)i : ids[i];
+    keep[v] = x[v] != 0;
+  }
+}
+
+constexpr int kCompactBlock = 256;
+constexpr int kCompactBlocks = 1024;
+
+// Pass 1: kept flags per contiguous chunk of vertices (one workgroup each).
+__global__ __launch_bounds__(kCompactBlock) void k_compact_count(const uint8_t* keep, int64_t n, int64_t chunk,
+                                                                 int64_t* counts) {
+  const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  int64_t c = 0;
+  for (int64_t v = lo + threadIdx.x; v < hi; v += kCompactBlock) c += keep[v] != 0;
+  __shared__ int64_t red[kCompactBlock];
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = kCompactBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = red[0];
+}
+
+// Pass 2 (one workgroup): exclusive prefix of the chunk counts in place,
+// the total in counts[nb].
+__global__ __launch_bounds__(kCompactBlocks) void k_compact_scan(int64_t* counts, int nb) {
+  __shared__ int64_t s[kCompactBlocks];
+  const int t = threadIdx.x;
+  int64_t x = t < nb ? counts[t] : 0;
+  s[t] = x;
+  __syncthreads();
+  for (int d = 1; d < kCompactBlocks; d <<= 1) {
+    int64_t y = t >= d ? s[t - d] : 0;
+    __syncthreads();
+    s[t] += y;
+    __syncthreads();
+  }
+  if (t < nb) counts[t] = s[t] - x;
+  if (t == kCompactBlocks - 1) counts[nb] = s[t];
+}
+
+// Pass 3: each workgroup writes its chunk's kept ids in ascending order and
+// clears the flags it read (the flag array is all zero between calls).
+__global__ __launch_bounds__(kCompactBlock) void k_compact_write(uint8_t* keep, int64_t n, int64_t chunk,
+                                                                 const int64_t* offsets, uint32_t* out) {
+  constexpr int kW = kCompactBlock / kWave;
+  __shared__ uint32_t wsum[kW];
+  const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  int64_t pos = offsets[blockIdx.x];
+  const int lane = (int)__lane_id(), w = threadIdx.x / kWave;
+  for (int64_t t0 = lo; t0 < hi; t0 += kCompactBlock) {
+    const int64_t v = t0 + threadIdx.x;
+    const bool k = v < hi && keep[v];
+    if (k) keep[v] = 0;
+    const unsigned long long bal = __ballot(k);
+    const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int i = 0; i < kW; ++i) {
+      before += i < w ? wsum[i] : 0u;
+      total += wsum[i];
+    }
+    if (k) out[pos + before + below] = (uint32_t)v;
+    pos += total;
+    __syncthreads();
+  }
+}
+
+// Deterministic FP64 sum (sequence::plusReduce): fixed per-workgroup
+// partials, then one workgroup adds them in a fixed order.
+__global__ __launch_bounds__(256) void k_sum_partial(const double* x, int64_t n, double* part) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void k_sum_final(double* part, int np) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[np] = red[0];
+}
+
 static int check_state(rg_graph* g, rg_array* const* st, int nstate, std::initializer_list<int> elems,
                        const char* who) {
   if (nstate != (int)elems.size()) return set_error(RG_EINVAL, std::string(who) + ": wrong number of state arrays");
@@ -1619,10 +1789,121 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
       hipLaunchKernelGGL(rg::k_bc_final, rg::grid_for(ctx, g->n), 256, 0, ctx->stream, state[0]->buf.as<double>(),
                          state[1]->buf.as<double>(), g->n);
       break;
+    case RG_VOP_ABS_DIFF:    // x = |x - y| (PageRank.C:125-127)
+    case RG_VOP_ABS:         // x = |y| (PageRankDelta.C:136-137)
+      RG_TRY(rg::check_state(g, state, nstate, {8, 8}, "vertexMap(abs)"));
+      hipLaunchKernelGGL(rg::k_abs_diff, grid, 256, 0, ctx->stream, ids, cnt, s->all, state[0]->buf.as<double>(),
+                         state[1]->buf.as<const double>(), vop == RG_VOP_ABS_DIFF ? 1 : 0);
+      break;
     default:
       return set_error(RG_EINVAL, "vertexMap: unknown operator");
   }
   RG_HIP(hipGetLastError());
+  return RG_OK;
+}
+
+// vertexFilter (ligra.h:500-509) with the reference's filter functors.
+int rg_vertexfilter(rg_graph* g, int fop, rg_subset* s, rg_array* const* state, int nstate, const double* params,
+                    int nparams, rg_subset* out) {
+  if (!g || !s || !out || s->g->n != g->n || out->g->n != g->n)
+    return set_error(RG_EINVAL, "vertexFilter: bad subset");
+  if (nparams < 0 || (nparams > 0 && !params)) return set_error(RG_EINVAL, "vertexFilter: bad parameters");
+  rg_ctx* ctx = g->ctx;
+  RG_HIP(hipSetDevice(ctx->device));
+  const int64_t n = g->n;
+  const int64_t cnt = s->all ? n : s->size;
+  if (out == s && !s->all) return set_error(RG_EINVAL, "vertexFilter: output must differ from a listed input");
+  // Keep flags: a byte per vertex, all zero between calls.
+  rg::DevBuf& kb = ctx->scratch[10];
+  if (kb.bytes < (size_t)std::max<int64_t>(n, 1) || !kb.p) {
+    RG_TRY(kb.ensure((size_t)std::max<int64_t>(n, 1)));
+    RG_TRY(rg::fill<uint8_t>(ctx, kb.as<uint8_t>(), (int64_t)kb.bytes, (uint8_t)0));
+  }
+  uint8_t* keep = kb.as<uint8_t>();
+  const uint32_t* ids = s->ids.as<uint32_t>();
+  const int grid = rg::grid_for(ctx, std::max<int64_t>(cnt, 1));
+  auto need = [&](int k, const char* who) -> int {
+    return nparams == k ? RG_OK : set_error(RG_EINVAL, std::string(who) + ": " + std::to_string(k) + " parameters");
+  };
+  if (cnt > 0) switch (fop) {
+    case RG_FOP_PRD_FIRST:   // {p, Delta, nghSum}; params {damping, one_over_n, epsilon2}
+      RG_TRY(rg::check_state(g, state, nstate, {8, 8, 8}, "vertexFilter(PR_Vertex_F_FirstRound)"));
+      RG_TRY(need(3, "vertexFilter(PR_Vertex_F_FirstRound)"));
+      hipLaunchKernelGGL(rg::k_filter_prd, grid, 256, 0, ctx->stream, ids, cnt, s->all, state[0]->buf.as<double>(),
+                         state[1]->buf.as<double>(), state[2]->buf.as<const double>(), params[0],
+                         (1.0 - params[0]) * params[1], params[1], params[2], 1, keep);
+      break;
+    case RG_FOP_PRD:         // {p, Delta, nghSum}; params {damping, epsilon2}
+      RG_TRY(rg::check_state(g, state, nstate, {8, 8, 8}, "vertexFilter(PR_Vertex_F)"));
+      RG_TRY(need(2, "vertexFilter(PR_Vertex_F)"));
+      hipLaunchKernelGGL(rg::k_filter_prd, grid, 256, 0, ctx->stream, ids, cnt, s->all, state[0]->buf.as<double>(),
+                         state[1]->buf.as<double>(), state[2]->buf.as<const double>(), params[0], 0.0, 0.0,
+                         params[1], 0, keep);
+      break;
+    case RG_FOP_U32_EQ:      // {x u32}; params {value}
+      RG_TRY(rg::check_state(g, state, nstate, {4}, "vertexFilter(x == value)"));
+      RG_TRY(need(1, "vertexFilter(x == value)"));
+      if (!(params[0] >= 0.0 && params[0] <= 4294967295.0 && params[0] == std::floor(params[0])))
+        return set_error(RG_EINVAL, "vertexFilter(x == value): value must be a 32-bit unsigned integer");
+      hipLaunchKernelGGL(rg::k_filter_u32_eq, grid, 256, 0, ctx->stream, ids, cnt, s->all,
+                         state[0]->buf.as<const uint32_t>(), (uint32_t)params[0], keep);
+      break;
+    case RG_FOP_U8_NONZERO:  // {x u8}
+      RG_TRY(rg::check_state(g, state, nstate, {1}, "vertexFilter(x != 0)"));
+      RG_TRY(need(0, "vertexFilter(x != 0)"));
+      hipLaunchKernelGGL(rg::k_filter_u8_nz, grid, 256, 0, ctx->stream, ids, cnt, s->all,
+                         state[0]->buf.as<const uint8_t>(), keep);
+      break;
+    default:
+      return set_error(RG_EINVAL, "vertexFilter: unknown operator");
+  } else if (fop < RG_FOP_PRD_FIRST || fop > RG_FOP_U8_NONZERO) {
+    return set_error(RG_EINVAL, "vertexFilter: unknown operator");
+  }
+  RG_HIP(hipGetLastError());
+  // Compact the flags (all n) into an ascending list.
+  const int64_t chunk = std::max<int64_t>(rg::kCompactBlock, (n + rg::kCompactBlocks - 1) / rg::kCompactBlocks);
+  const int nb = (int)std::max<int64_t>(1, (n + chunk - 1) / chunk);
+  int64_t* offs = nullptr;
+  rg::DevBuf& ob = ctx->scratch[11];
+  RG_TRY(ob.ensure(sizeof(int64_t) * (rg::kCompactBlocks + 1)));
+  offs = ob.as<int64_t>();
+  RG_TRY(out->ids.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(n, 1)));
+  hipLaunchKernelGGL(rg::k_compact_count, nb, rg::kCompactBlock, 0, ctx->stream, keep, n, chunk, offs);
+  hipLaunchKernelGGL(rg::k_compact_scan, 1, rg::kCompactBlocks, 0, ctx->stream, offs, nb);
+  hipLaunchKernelGGL(rg::k_compact_write, nb, rg::kCompactBlock, 0, ctx->stream, keep, n, chunk, offs,
+                     out->ids.as<uint32_t>());
+  RG_HIP(hipGetLastError());
+  int64_t total = 0;
+  RG_TRY(rg::ctx_read_count(ctx, offs + nb, &total, 1));
+  out->size = total;
+  out->degsum = -1;
+  out->all = false;
+  return RG_OK;
+}
+
+// sequence::plusReduce over an FP64 state array, in a fixed order.
+int rg_array_sum(const rg_array* a, double* out) {
+  if (!a || !out || a->elem != 8) return set_error(RG_EINVAL, "rg_array_sum: an array of doubles");
+  rg_ctx* ctx = a->ctx;
+  RG_HIP(hipSetDevice(ctx->device));
+  const int np = (int)std::min<int64_t>(std::max<int64_t>(1, (a->count + 255) / 256), (int64_t)ctx->num_cus * 4);
+  rg::DevBuf& pb = ctx->scratch[11];
+  RG_TRY(pb.ensure(sizeof(double) * (np + 1)));
+  hipLaunchKernelGGL(rg::k_sum_partial, np, 256, 0, ctx->stream, a->buf.as<const double>(), a->count,
+                     pb.as<double>());
+  hipLaunchKernelGGL(rg::k_sum_final, 1, 256, 0, ctx->stream, pb.as<double>(), np);
+  RG_HIP(hipGetLastError());
+  int64_t bits = 0;
+  RG_TRY(rg::ctx_read_count(ctx, reinterpret_cast<const int64_t*>(pb.as<double>() + np), &bits, 1));
+  std::memcpy(out, &bits, sizeof(double));
+  return RG_OK;
+}
+
+// The device address of a state array (for entry points taking device
+// pointers).
+int rg_array_device_ptr(const rg_array* a, void** out) {
+  if (!a || !out) return set_error(RG_EINVAL, "rg_array_device_ptr: null");
+  *out = a->buf.p;
   return RG_OK;
 }
 

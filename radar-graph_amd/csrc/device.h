@@ -66,7 +66,7 @@ struct rg_ctx {
   int64_t* h_counts = nullptr;
   // Scratch pools reused by the engine (grown on demand).
   rg::DevBuf counts;      // int64 counters (queue tails, sums)
-  rg::DevBuf scratch[10];  // generic pools for apps / kernels
+  rg::DevBuf scratch[12];  // generic pools for apps / kernels ([10], [11]: vertexFilter and reductions)
   rg::DevBuf fr_ids[2];   // frontier id lists (uint32[n] each)
   rg::DevBuf fr_dense;    // frontier byte map (uint8[n]); all zero between uses
   // Optional HIP-event timing of the edge-map launches (rg_ctx_timing).

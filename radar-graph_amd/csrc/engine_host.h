@@ -42,6 +42,12 @@ inline int fill(rg_ctx* ctx, T* p, int64_t n, T v) {
 // ---------------------------------------------------------------------------
 // Engine configuration and the edgeMap dispatcher.
 
+// Capacity of the heavy queue in items: at most m / heavy_deg heavy
+// vertices, each split in chunks of kHeavyChunk arcs.
+inline int64_t heavy_queue_items(int64_t m, int heavy_deg) {
+  return m / kHeavyChunk + m / std::max(1, heavy_deg) + 64;
+}
+
 struct Engine {
   rg_ctx* ctx;
   rg_graph* g;
@@ -53,7 +59,7 @@ struct Engine {
   int64_t hub_k2;      // LDS + per-XCD slots
   void* l2_copies;
   int l2_ncopies;
-  DevBuf* heavy;       // heavy work queue
+  uint64_t* heavy;     // heavy work queue (chunk items)
   int64_t heavy_cap = 0;   // its capacity in items (0: never lower the cut-over)
   int64_t* heavy_count;
 };
@@ -125,7 +131,11 @@ static int launch_hub(const Engine& E, const Op& op, EdgeArgs a) {
 }
 
 // Resolves variant/opts into an Engine.  elem = bytes of one hub copy entry.
-int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E);
+// heavy / heavy_cap / heavy_count (all given or none): the heavy queue and
+// its counter in caller memory instead of the context pools; only the push
+// variant (no hub copies) then runs without allocating.
+int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E, uint64_t* heavy = nullptr,
+                int64_t heavy_cap = 0, int64_t* heavy_count = nullptr);
 
 // ---------------------------------------------------------------------------
 // Timing helper (device time of a region on the context stream).

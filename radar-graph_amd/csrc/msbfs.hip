@@ -645,7 +645,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   ea.count = n;
   ea.count_dev = &a.ctl->fcount;
   ea.out_info = &a.ctl->ncount;
-  ea.heavy = E.heavy->as<uint64_t>();
+  ea.heavy = E.heavy;
   ea.heavy_count = &a.ctl->heavy_count;
   ea.heavy_deg = E.heavy_deg;
   const int gfin = std::max(1, ctx->num_cus * 4);

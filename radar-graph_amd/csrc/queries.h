@@ -26,9 +26,23 @@ __host__ __device__ inline double summary_weight(uint32_t v) {
   return (double)(level_hash(v, 0xffffffffu) >> 11) * 0x1p-53;
 }
 
+// Device buffers of one single-source push BFS supplied by the caller
+// instead of the context pools: two frontier lists and a byte map (n
+// entries each; the map all zero on entry, and left so), the heavy queue
+// (heavy_cap chunk items) and 16 int64 counters.  With a workspace the BFS
+// runs the push engine (no hub copies, no pull or blocked levels), records
+// no events and allocates nothing; parents_dev must then be given.
+struct BfsWork {
+  uint32_t* ids[2];
+  uint8_t* dense;
+  uint64_t* heavy;
+  int64_t heavy_cap;
+  int64_t* counts;
+};
+
 int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
              int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev,
-             uint64_t* checksum_out);
+             uint64_t* checksum_out, const BfsWork* work = nullptr);
 
 int bfs_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_out,
               rg_bfs_summary* out, double* ms_out);

@@ -151,6 +151,29 @@ _sig("rg_serve", C.c_int, _vp, C.POINTER(ServeConfig), C.POINTER(Query), C.c_int
 _sig("rg_radii", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _i32p, _i64p, _f64p)
 _sig("rg_bc", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _f64p, _i64p, _f64p)
 _sig("rg_bc_batch", C.c_int, _vp, _u32p, C.c_int, C.POINTER(Opts), _f64p, C.POINTER(BcSummary), _f64p)
+# Engine level: vertexSubset / edgeMap / vertexMap / vertexFilter over device state arrays.
+_sig("rg_subset_create", C.c_int, _vp, C.POINTER(_vp))
+_sig("rg_subset_set", C.c_int, _vp, _u32p, C.c_int64)
+_sig("rg_subset_set_all", C.c_int, _vp)
+_sig("rg_subset_size", C.c_int, _vp, _i64p)
+_sig("rg_subset_get", C.c_int, _vp, _u32p)
+_sig("rg_subset_free", C.c_int, _vp)
+_sig("rg_array_alloc", C.c_int, _vp, C.c_int64, C.c_int, C.POINTER(_vp))
+_sig("rg_array_write", C.c_int, _vp, _vp, C.c_int64, C.c_int64)
+_sig("rg_array_read", C.c_int, _vp, _vp, C.c_int64, C.c_int64)
+_sig("rg_array_fill", C.c_int, _vp, C.c_uint64)
+_sig("rg_array_free", C.c_int, _vp)
+_sig("rg_array_sum", C.c_int, _vp, _f64p)
+_sig("rg_array_device_ptr", C.c_int, _vp, C.POINTER(_vp))
+_sig("rg_edgemap", C.c_int, _vp, C.c_int, _vp, C.POINTER(_vp), C.c_int, C.c_int64, C.c_int64, C.c_int,
+     C.POINTER(Opts), _vp)
+_sig("rg_vertexmap", C.c_int, _vp, C.c_int, _vp, C.POINTER(_vp), C.c_int, C.c_double, C.c_double)
+_sig("rg_vertexfilter", C.c_int, _vp, C.c_int, _vp, C.POINTER(_vp), C.c_int, _f64p, C.c_int, _vp)
+NO_OUTPUT, DENSE_FORWARD = 1, 8
+OP_BFS, OP_PAGERANK, OP_RADII, OP_BC_FWD, OP_BC_BACK = range(5)
+(VOP_PR_UPDATE, VOP_PR_CONTRIB, VOP_RADII_SYNC, VOP_MARK, VOP_BC_BACK, VOP_INVERT, VOP_BC_FINAL,
+ VOP_ABS_DIFF, VOP_ABS) = range(9)
+FOP_PRD_FIRST, FOP_PRD, FOP_U32_EQ, FOP_U8_NONZERO = range(4)
 
 _OPTIONAL = {}
 
@@ -372,6 +395,7 @@ class Context:
     def __init__(self, device=0):
         self._h = _vp()
         self._graphs = weakref.WeakSet()   # device graphs bound to this context
+        self._arrays = weakref.WeakSet()   # device state arrays of this context
         _check(lib.rg_ctx_create(device, C.byref(self._h)))
 
     def sync(self):
@@ -401,6 +425,8 @@ class Context:
         """Frees the context; the device graphs bound to it are freed first
         (a graph must not outlive its context)."""
         if self._h:
+            for a in list(self._arrays):
+                a.close()
             for g in list(self._graphs):
                 g.close()
             lib.rg_ctx_destroy(self._h)
@@ -411,6 +437,99 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+_NP_OF_ELEM = {1: np.uint8, 4: np.uint32, 8: np.float64}
+
+
+class DeviceArray:
+    """A device state array (rg_array): the apps' newA arrays, the state of
+    the engine-level operators.  Elements of 1, 4 or 8 bytes; `dtype` is the
+    numpy type read() and write() use (its itemsize must match)."""
+
+    def __init__(self, ctx: Context, count: int, dtype=np.float64):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self._h = _vp()
+        _check(lib.rg_array_alloc(ctx._h, self.count, self.dtype.itemsize, C.byref(self._h)))
+        ctx._arrays.add(self)
+
+    def write(self, values, first=0):
+        a = np.ascontiguousarray(values, self.dtype)
+        _check(lib.rg_array_write(self._h, a.ctypes.data_as(_vp), int(first), a.size))
+
+    def read(self, first=0, count=None):
+        count = self.count - first if count is None else count
+        out = np.empty(count, self.dtype)
+        _check(lib.rg_array_read(self._h, out.ctypes.data_as(_vp), int(first), int(count)))
+        return out
+
+    def fill(self, value):
+        bits = np.array([value], self.dtype).view({1: np.uint8, 4: np.uint32, 8: np.uint64}[self.dtype.itemsize])
+        _check(lib.rg_array_fill(self._h, int(bits[0])))
+
+    def sum(self) -> float:
+        s = C.c_double()
+        _check(lib.rg_array_sum(self._h, C.byref(s)))
+        return s.value
+
+    def device_ptr(self) -> int:
+        p = _vp()
+        _check(lib.rg_array_device_ptr(self._h, C.byref(p)))
+        return p.value
+
+    def close(self):
+        if self._h:
+            lib.rg_array_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Subset:
+    """vertexSubset (rg_subset) of a device graph: a device id list or all
+    vertices."""
+
+    def __init__(self, dg: "DeviceGraph", ids=None, all_vertices=False):
+        self.dg = dg
+        self._h = _vp()
+        _check(lib.rg_subset_create(dg._h, C.byref(self._h)))
+        dg._subsets.add(self)
+        if all_vertices:
+            _check(lib.rg_subset_set_all(self._h))
+        elif ids is not None:
+            a = np.ascontiguousarray(ids, np.uint32)
+            _check(lib.rg_subset_set(self._h, _ptr(a, _u32p), a.size))
+
+    def size(self) -> int:
+        k = C.c_int64()
+        _check(lib.rg_subset_size(self._h, C.byref(k)))
+        return k.value
+
+    def ids(self) -> np.ndarray:
+        out = np.empty(max(self.size(), 1), np.uint32)
+        _check(lib.rg_subset_get(self._h, _ptr(out, _u32p)))
+        return out[: self.size()]
+
+    def close(self):
+        if self._h:
+            lib.rg_subset_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _state(arrays):
+    return (_vp * max(len(arrays), 1))(*[a._h for a in arrays]), len(arrays)
 
 
 @dataclass
@@ -441,6 +560,7 @@ class DeviceGraph:
             self._h = _vp()
             _check(lib.rg_graph_upload(ctx._h, hg._h, C.byref(self._h)))
         ctx._graphs.add(self)
+        self._subsets = weakref.WeakSet()
         n, m, s = C.c_int64(), C.c_int64(), C.c_int()
         _check(lib.rg_graph_info(self._h, C.byref(n), C.byref(m), C.byref(s)))
         self.n, self.m, self.symmetric = n.value, m.value, bool(s.value)
@@ -568,8 +688,33 @@ class DeviceGraph:
         _check(f(self._h, C.byref(o), C.byref(tot), _ptr(pv, _u64p) if per_vertex else None, C.byref(ms)))
         return (tot.value, pv, ms.value) if per_vertex else (tot.value, ms.value)
 
+    # ---- engine level (Ligra's edgeMap / vertexMap / vertexFilter) --------
+    def edgemap(self, op, frontier: Subset, state, round_=0, threshold=-1, flags=0, o: Opts = None,
+                out: Subset = None) -> Subset:
+        """edgeMap(GA, frontier, F(state...), threshold, flags); returns the
+        next frontier (a new Subset unless `out` is given; empty with
+        NO_OUTPUT)."""
+        out = out if out is not None else Subset(self)
+        st, k = _state(state)
+        _check(lib.rg_edgemap(self._h, int(op), frontier._h, st, k, int(round_), int(threshold), int(flags),
+                              C.byref(o) if o is not None else None, out._h))
+        return out
+
+    def vertexmap(self, vop, subset: Subset, state, a=0.0, b=0.0):
+        st, k = _state(state)
+        _check(lib.rg_vertexmap(self._h, int(vop), subset._h, st, k, float(a), float(b)))
+
+    def vertexfilter(self, fop, subset: Subset, state, params=()) -> Subset:
+        out = Subset(self)
+        st, k = _state(state)
+        pr = np.ascontiguousarray(params if len(params) else [0.0], np.float64)
+        _check(lib.rg_vertexfilter(self._h, int(fop), subset._h, st, k, _ptr(pr, _f64p), len(params), out._h))
+        return out
+
     def close(self):
         if self._h:
+            for s in list(self._subsets):
+                s.close()
             lib.rg_graph_free(self._h)
             self._h = None
 

@@ -79,15 +79,22 @@ def test_bfs_app(rg, orc, golden, tmp_path, name, pre, var, variant):
 
 @pytest.mark.parametrize("name", ["rmat10s", "rmat10d"])
 @pytest.mark.parametrize("pre,var", [("-1", "pno"), ("1", "ds")])
-@pytest.mark.parametrize("variant", ["push", "hubdup", "radar", "blocked"])
-def test_pagerank_app(rg, golden, tmp_path, name, pre, var, variant):
+@pytest.mark.parametrize("variant,fused", [("push", False), ("hubdup", False), ("radar", False), ("push", True),
+                                           ("hubdup", True), ("radar", True), ("blocked", True)])
+def test_pagerank_app(rg, golden, tmp_path, name, pre, var, variant, fused):
+    # Without -fused every iteration runs the reference's edgeMap / vertexMap /
+    # plusReduce sequence through the engine ABI; -fused the device loop.
     _, path, flag = graph_file(rg, tmp_path, name)
     fx = np.load(os.path.join(GOLD, f"ligra_{name}.npz"))
     out = str(tmp_path / "ranks.bin")
     o = run("PageRank", flag + ["-rounds", "0", "-maxiters", "20", "-preprocess", pre, "-variant", variant,
-                                "-k", "128", "-out", out, path])
-    assert int(val(o, "Num Iters")) == golden["ligra"][name][f"{var}_pr"]["iters"]
-    assert same_fp(np.fromfile(out, np.float64), fx[f"{var}_pr_ranks"])
+                                "-k", "128", "-out", out] + (["-fused"] if fused else []) + [path])
+    gold = golden["ligra"][name][f"{var}_pr"]
+    assert int(val(o, "Num Iters")) == gold["iters"]
+    ranks = np.fromfile(out, np.float64)
+    assert same_fp(ranks, fx[f"{var}_pr_ranks"])
+    # The printed L1 (6 significant digits, as the reference prints it).
+    assert abs(float(val(o, "L1_Norm")) - gold["l1"]) <= 1e-5 * gold["l1"] + FP_TOL * float(np.sum(ranks))
 
 
 @pytest.mark.parametrize("name", ["rmat10s", "rmat10d"])
@@ -132,12 +139,15 @@ def test_tc_local_app(orc, golden, relabel):
 
 @pytest.mark.parametrize("name", ["rmat10s", "rmat10d"])
 @pytest.mark.parametrize("pre,var", [("-1", "pno"), ("1", "ds")])
-@pytest.mark.parametrize("variant", ["push", "radar", "blocked"])
-def test_pagerank_delta_app(rg, golden, tmp_path, name, pre, var, variant):
+@pytest.mark.parametrize("variant,fused", [("push", False), ("radar", False), ("push", True), ("radar", True),
+                                           ("blocked", True)])
+def test_pagerank_delta_app(rg, golden, tmp_path, name, pre, var, variant, fused):
+    # Without -fused every round runs edgeMap / vertexFilter / plusReduce
+    # through the engine ABI; -fused the device loop.
     _, path, flag = graph_file(rg, tmp_path, name)
     fx = np.load(os.path.join(GOLD, f"ligra_{name}.npz"))
     out = str(tmp_path / "prd.bin")
     o = run("PageRankDelta", flag + ["-rounds", "0", "-maxiters", "20", "-preprocess", pre, "-variant", variant,
-                                     "-k", "128", "-out", out, path])
+                                     "-k", "128", "-out", out] + (["-fused"] if fused else []) + [path])
     assert int(val(o, "Num Iters")) == golden["ligra"][name][f"{var}_prd"]["iters"]
     assert same_fp(np.fromfile(out, np.float64), fx[f"{var}_prd_ranks"])

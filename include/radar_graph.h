@@ -338,7 +338,7 @@ int rg_array_free(rg_array* a);
  * order (the same bits on every call). */
 int rg_array_sum(const rg_array* a, double* out);
 /* The array's device address, for entry points that take device pointers
- * into caller memory.  No counterpart in the reference. */
+ * into caller memory, such as rg_bfs_noalloc.  No counterpart in the reference. */
 int rg_array_device_ptr(const rg_array* a, void** out);
 /* edgeMap.  State per operator:
  *   RG_OP_BFS      {parents u32}                    BFS_F      (BFS.C:28-49)
@@ -374,6 +374,23 @@ int rg_vertexmap(rg_graph* g, int vop, rg_subset* s, rg_array* const* state, int
  *   RG_FOP_U8_NONZERO {x u8}; {}: keeps x[v] != 0 */
 int rg_vertexfilter(rg_graph* g, int fop, rg_subset* s, rg_array* const* state, int nstate, const double* params,
                     int nparams, rg_subset* out);
+
+/* Single-source BFS that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The answer of rg_bfs with the
+ * push engine (o->variant RG_PUSH): parents via the first successful CAS
+ * into the caller's device array parents_dev (n uint32; the address of an
+ * rg_array of 4-byte elements, rg_array_device_ptr, or any device
+ * allocation), and out = rg_bfs_query's summary.  The frontier lists, byte
+ * map, heavy queue and counters live in one fixed device array of
+ * RG_FIXED_SCRATCH_BYTES per GPU, set up when the first context on the GPU
+ * is created and held until the process ends; a graph needing more (9 bytes
+ * per vertex plus 8 per 683 arcs, about 7M vertices at 16 arcs per vertex)
+ * is refused with RG_EINVAL.  Every call on a GPU shares that GPU's array,
+ * so calls that overlap on one GPU (other contexts or host threads) race;
+ * per-launch timing (rg_ctx_timing) must be off. */
+#define RG_FIXED_SCRATCH_BYTES (64u << 20)
+int rg_bfs_noalloc(rg_graph* g, uint32_t src, uint32_t* parents_dev, rg_bfs_summary* out);
 
 /* ---- query serving ---------------------------------------------------------
  * A request loop over a resident graph (no counterpart in the reference,

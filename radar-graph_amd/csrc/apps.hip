@@ -1881,6 +1881,62 @@ int rg_vertexfilter(rg_graph* g, int fop, rg_subset* s, rg_array* const* state, 
   return RG_OK;
 }
 
+// rg_bfs_noalloc's layout of the fixed scratch array: 32 int64 counters,
+// two frontier lists, the byte map and the heavy queue, each 256-byte
+// aligned.
+namespace {
+struct FixedLayout {
+  size_t ids0, ids1, dense, heavy, end;
+  int64_t heavy_cap;
+};
+FixedLayout fixed_layout(int64_t n, int64_t m, int heavy_deg) {
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t cnt = (size_t)std::max<int64_t>(n, 64);
+  FixedLayout L;
+  L.ids0 = 256;
+  L.ids1 = L.ids0 + up(sizeof(uint32_t) * cnt);
+  L.dense = L.ids1 + up(sizeof(uint32_t) * cnt);
+  L.heavy = L.dense + up(cnt);
+  L.heavy_cap = rg::heavy_queue_items(m, heavy_deg);
+  L.end = L.heavy + up(sizeof(uint64_t) * (size_t)L.heavy_cap);
+  return L;
+}
+}  // namespace
+
+int rg_bfs_noalloc(rg_graph* g, uint32_t src, uint32_t* parents_dev, rg_bfs_summary* out) {
+  if (!g || !parents_dev || !out) return set_error(RG_EINVAL, "rg_bfs_noalloc: null argument");
+  rg_ctx* ctx = g->ctx;
+  RG_HIP(hipSetDevice(ctx->device));
+  if ((int64_t)src >= g->n) return set_error(RG_EINVAL, "rg_bfs_noalloc: source out of range");
+  if (ctx->timing) return set_error(RG_EINVAL, "rg_bfs_noalloc: per-launch timing is on (it creates events)");
+  if (!ctx->fixed_scratch) return set_error(RG_EDEVICE, "rg_bfs_noalloc: no fixed scratch on this device");
+  rg_opts o;
+  rg_opts_default(&o);
+  o.variant = RG_PUSH;
+  const FixedLayout L = fixed_layout(g->n, g->m, 1024);
+  if (L.end > (size_t)RG_FIXED_SCRATCH_BYTES)
+    return set_error(RG_EINVAL, "rg_bfs_noalloc: this graph needs " + std::to_string(L.end) +
+                                    " bytes of scratch (9 per vertex, 8 per 683 arcs), more than the fixed " +
+                                    std::to_string((size_t)RG_FIXED_SCRATCH_BYTES));
+  char* base = static_cast<char*>(ctx->fixed_scratch);
+  rg::BfsWork w;
+  w.counts = reinterpret_cast<int64_t*>(base);
+  w.ids[0] = reinterpret_cast<uint32_t*>(base + L.ids0);
+  w.ids[1] = reinterpret_cast<uint32_t*>(base + L.ids1);
+  w.dense = reinterpret_cast<uint8_t*>(base + L.dense);
+  w.heavy = reinterpret_cast<uint64_t*>(base + L.heavy);
+  w.heavy_cap = L.heavy_cap;
+  // The array keeps the last call's contents: counters and the byte map
+  // start from zero (a call that stopped on an error may have left them set).
+  RG_TRY(rg::fill<unsigned long long>(ctx, reinterpret_cast<unsigned long long*>(w.counts), 32, 0ull));
+  RG_TRY(rg::fill<uint8_t>(ctx, w.dense, std::max<int64_t>(g->n, 64), (uint8_t)0));
+  uint64_t cs = 0;
+  RG_TRY(rg::bfs_push(g, src, &o, nullptr, &out->iters, &out->reached, &out->edges, nullptr, parents_dev, &cs,
+                      &w));
+  out->checksum = cs;
+  return RG_OK;
+}
+
 // sequence::plusReduce over an FP64 state array, in a fixed order.
 int rg_array_sum(const rg_array* a, double* out) {
   if (!a || !out || a->elem != 8) return set_error(RG_EINVAL, "rg_array_sum: an array of doubles");

@@ -89,6 +89,9 @@ struct rg_ctx {
   // Device array holding the last PageRank ranks / BC dependencies computed
   // on this context (n doubles; valid until the next call on the context).
   const double* last_result = nullptr;
+  // Device address of this GPU's fixed scratch array (rg_bfs_noalloc),
+  // looked up once per device when the first context is created.
+  void* fixed_scratch = nullptr;
 };
 
 struct rg_graph {

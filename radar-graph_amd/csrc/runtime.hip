@@ -4,8 +4,41 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "device.h"
+
+// Fixed device scratch of rg_bfs_noalloc (its frontier lists, byte map,
+// heavy queue and counters): the loader places one copy on every GPU the
+// library runs on, held until the process ends.  In host code the name is
+// not a device address; the address comes from hipGetSymbolAddress, once
+// per device (fixed_scratch_address), and only that address is used.
+static __device__ __align__(256) unsigned char scratch[RG_FIXED_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_fixed_mu;
+constexpr int kMaxDevices = 64;
+void* g_fixed_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+int fixed_scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_fixed_mu);
+  if (device < 0 || device >= kMaxDevices)
+    return rg::set_error(RG_EINVAL, "fixed scratch: device index above " + std::to_string(kMaxDevices - 1));
+  if (!g_fixed_addr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return rg::set_error(RG_EDEVICE, std::string("fixed scratch: hipGetSymbolAddress failed: ") +
+                                           hipGetErrorString(e));
+    }
+    g_fixed_addr[device] = p;
+  }
+  *out = g_fixed_addr[device];
+  return RG_OK;
+}
+}  // namespace
 
 namespace rg {
 
@@ -121,6 +154,7 @@ int rg_ctx_create(int device, rg_ctx** out) {
   ctx->max_lds_per_block = prop.sharedMemPerBlock;
   if (prop.maxSharedMemoryPerMultiProcessor > 0)
     ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+  RG_TRY(fixed_scratch_address(device, &ctx->fixed_scratch));
   RG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   RG_HIP(hipHostMalloc((void**)&ctx->h_counts, 64 * sizeof(int64_t), hipHostMallocDefault));
   RG_TRY(ctx->counts.ensure(64 * sizeof(int64_t)));

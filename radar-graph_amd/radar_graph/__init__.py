@@ -169,6 +169,8 @@ _sig("rg_edgemap", C.c_int, _vp, C.c_int, _vp, C.POINTER(_vp), C.c_int, C.c_int6
      C.POINTER(Opts), _vp)
 _sig("rg_vertexmap", C.c_int, _vp, C.c_int, _vp, C.POINTER(_vp), C.c_int, C.c_double, C.c_double)
 _sig("rg_vertexfilter", C.c_int, _vp, C.c_int, _vp, C.POINTER(_vp), C.c_int, _f64p, C.c_int, _vp)
+_sig("rg_bfs_noalloc", C.c_int, _vp, C.c_uint32, _vp, C.POINTER(BfsSummary))
+FIXED_SCRATCH_BYTES = 64 << 20
 NO_OUTPUT, DENSE_FORWARD = 1, 8
 OP_BFS, OP_PAGERANK, OP_RADII, OP_BC_FWD, OP_BC_BACK = range(5)
 (VOP_PR_UPDATE, VOP_PR_CONTRIB, VOP_RADII_SYNC, VOP_MARK, VOP_BC_BACK, VOP_INVERT, VOP_BC_FINAL,
@@ -615,6 +617,15 @@ class DeviceGraph:
         out, ms = BfsSummary(), C.c_double()
         _check(lib.rg_bfs_query(self._h, int(src), C.byref(o), C.byref(out), C.byref(ms)))
         return out, ms.value
+
+    def bfs_noalloc(self, src, parents: "DeviceArray") -> BfsSummary:
+        """Push BFS that allocates nothing (rg_bfs_noalloc): parents into the
+        device array `parents` (n uint32), scratch in the GPU's fixed array."""
+        if parents.dtype.itemsize != 4 or parents.count < self.n:
+            raise RGError(RG_EINVAL, "bfs_noalloc: parents must hold n 4-byte elements")
+        out = BfsSummary()
+        _check(lib.rg_bfs_noalloc(self._h, int(src), _vp(parents.device_ptr()), C.byref(out)))
+        return out
 
     def bfs_batch(self, sources, o: Opts = None, want_levels=False):
         """Bit-parallel batch of BFS queries: (list of BfsSummary, levels

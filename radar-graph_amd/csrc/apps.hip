@@ -44,10 +44,12 @@ struct PrOp {
     unsafeAtomicAdd(acc + d, v);
     return false;
   }
-  // The per-XCD copy is touched by one XCD only: a workgroup-scope atomic
-  // executes in that XCD's L2 instead of being forwarded to memory.
+  // The per-XCD copy is meant for one XCD's workgroups, but many
+  // workgroups add into it: agent scope, the scope that makes the add atomic
+  // across them (on this part every global FP64 atomic executes past the L2
+  // whatever its scope, so the tier saves no memory traffic either way).
   __device__ void l2_update(double* slot, double sv) const {
-    __hip_atomic_fetch_add(slot, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(slot, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __device__ double hub_combine(double x, double y) const { return x + y; }
 };
@@ -183,10 +185,12 @@ struct BcBackOp {
   using HubT = double;
   static constexpr bool kOutput = false;
   static constexpr bool kL2 = true;
-  // The per-XCD copy is touched by one XCD only: a workgroup-scope atomic
-  // executes in that XCD's L2 instead of being forwarded to memory.
+  // The per-XCD copy is meant for one XCD's workgroups, but many
+  // workgroups add into it: agent scope, the scope that makes the add atomic
+  // across them (on this part every global FP64 atomic executes past the L2
+  // whatever its scope, so the tier saves no memory traffic either way).
   __device__ void l2_update(double* slot, double sv) const {
-    __hip_atomic_fetch_add(slot, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(slot, sv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __device__ double hub_combine(double x, double y) const { return x + y; }
   double* deps;
@@ -246,7 +250,7 @@ __global__ void k_heavy_list(const uint64_t* off, int64_t n, int heavy_deg, uint
 int static_heavy_list(rg_graph* g, int heavy_deg) {
   if (g->heavy_items && g->heavy_items_deg == heavy_deg && g->heavy_items_off == g->off) return RG_OK;
   rg_ctx* ctx = g->ctx;
-  if (g->heavy_items) (void)hipFree(g->heavy_items);
+  if (g->heavy_items && !borrowed(g, g->heavy_items)) (void)hipFree(g->heavy_items);
   g->heavy_items = nullptr;
   int64_t cap = g->m / kHeavyChunk + g->m / std::max(1, heavy_deg) + 64;
   RG_HIP(hipMalloc(&g->heavy_items, sizeof(uint64_t) * cap));
@@ -390,10 +394,8 @@ static int build_hub_bitmap(rg_graph* g, int64_t k) {
     slot_of[by_deg[s]] = (uint32_t)s;
     slot_ids[s] = ids[by_deg[s]];
   }
-  if (g->hub_bits) (void)hipFree(g->hub_bits);
-  if (g->hub_rank) (void)hipFree(g->hub_rank);
-  if (g->hub_ids) (void)hipFree(g->hub_ids);
-  if (g->hub_slot_of) (void)hipFree(g->hub_slot_of);
+  for (uint32_t* p : {g->hub_bits, g->hub_rank, g->hub_ids, g->hub_slot_of})
+    if (p && !borrowed(g, p)) (void)hipFree(p);
   g->hub_bits = g->hub_rank = g->hub_ids = g->hub_slot_of = nullptr;
   const size_t nh = std::max<size_t>(1, ids.size());
   RG_HIP(hipMalloc(&g->hub_bits, sizeof(uint32_t) * nw));

@@ -133,10 +133,11 @@ __global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
     uint32_t s[kDenseArcs], l[kDenseArcs];
     bool ok[kDenseArcs];
     load_arcs<THREADS>(a, e0, hi, skip, s, l, ok);
-    // Each value is loaded next to its add.  Issuing the four value loads
-    // ahead of the four adds gave dependencies that differed from run to
-    // run on a 2^10-vertex R-MAT (one workgroup per level; cause not found),
-    // while this form and the one-arc loop matched the oracle on every run.
+    // Each value is loaded next to its add.  The values read (frontier
+    // sources, all visited) and the destinations written (unvisited) are
+    // disjoint sets, so no workgroup reads a value another one writes.  A
+    // form that issued the four value loads ahead of the four adds once gave
+    // run-to-run differences on a 2^10-vertex R-MAT and is not used.
 #pragma unroll
     for (int j = 0; j < kDenseArcs; ++j)
       if (ok[j]) atomicAdd(&acc[l[j]], a.val[s[j]]);

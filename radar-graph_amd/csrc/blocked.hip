@@ -498,6 +498,7 @@ int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out) {
   if (g->m > (int64_t)INT32_MAX) return set_error(RG_EUNSUPPORTED, "blocked layout: more than 2^31 arcs");
   rg_ctx* ctx = g->ctx;
   hipStream_t s = ctx->stream;
+  if (borrowed(g, B)) B = nullptr;   // the base's layout stays as it is
   if (!B) {
     B = new BlockedCSC;
     *slot = B;

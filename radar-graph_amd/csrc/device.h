@@ -141,6 +141,17 @@ struct rg_graph {
 
 namespace rg {
 
+// True when p, an array or layout a view holds, is its base's: the view
+// must neither free it nor rebuild it in place (other views may be reading
+// it), and builds its own instead.
+inline bool borrowed(const rg_graph* g, const void* p) {
+  if (!g->view || !g->base || !p) return false;
+  const rg_graph* b = g->base;
+  return p == b->hub_bits || p == b->hub_rank || p == b->hub_ids || p == b->hub_slot_of ||
+         p == b->heavy_items || p == b->first_in || p == b->blocked || p == b->blocked_ms ||
+         p == b->blocked_ms_t;
+}
+
 // Device frontier (Ligra vertexSubset, vertexSubset.h:113-229): a sparse id
 // list (always kept) and, for dense rounds, a byte map over all n.  A view:
 // the storage belongs to the context pools (ctx_frontiers).

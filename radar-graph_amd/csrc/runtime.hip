@@ -328,24 +328,19 @@ int rg_graph_transpose(rg_graph* g) {
 int rg_graph_free(rg_graph* g) {
   if (!g) return RG_OK;
   if (g->ctx) (void)hipSetDevice(g->ctx->device);
-  const rg_graph* b = g->view ? g->base : nullptr;
-  // A view frees only what it built itself (not shared with its base).
-  auto own = [&](const void* p, const void* shared) { return p && (!b || p != shared); };
-  if (!b) {
+  // A view frees only what it built itself (not borrowed from its base);
+  // the CSR and CSC are always the base's.
+  if (!g->view) {
     if (g->in_off && g->in_off != g->off) (void)hipFree(g->in_off);
     if (g->in_nbr && g->in_nbr != g->nbr) (void)hipFree(g->in_nbr);
     if (g->off) (void)hipFree(g->off);
     if (g->nbr) (void)hipFree(g->nbr);
   }
-  if (own(g->hub_bits, b ? b->hub_bits : nullptr)) (void)hipFree(g->hub_bits);
-  if (own(g->hub_rank, b ? b->hub_rank : nullptr)) (void)hipFree(g->hub_rank);
-  if (own(g->hub_ids, b ? b->hub_ids : nullptr)) (void)hipFree(g->hub_ids);
-  if (own(g->hub_slot_of, b ? b->hub_slot_of : nullptr)) (void)hipFree(g->hub_slot_of);
-  if (own(g->heavy_items, b ? b->heavy_items : nullptr)) (void)hipFree(g->heavy_items);
-  if (own(g->first_in, b ? b->first_in : nullptr)) (void)hipFree(g->first_in);
-  if (own(g->blocked, b ? b->blocked : nullptr)) rg::blocked_free(g->blocked);
-  if (own(g->blocked_ms, b ? b->blocked_ms : nullptr)) rg::blocked_free(g->blocked_ms);
-  if (own(g->blocked_ms_t, b ? b->blocked_ms_t : nullptr)) rg::blocked_free(g->blocked_ms_t);
+  for (uint32_t* p : {g->hub_bits, g->hub_rank, g->hub_ids, g->hub_slot_of, g->first_in})
+    if (p && !rg::borrowed(g, p)) (void)hipFree(p);
+  if (g->heavy_items && !rg::borrowed(g, g->heavy_items)) (void)hipFree(g->heavy_items);
+  for (void* p : {g->blocked, g->blocked_ms, g->blocked_ms_t})
+    if (p && !rg::borrowed(g, p)) rg::blocked_free(p);
   delete g;
   return RG_OK;
 }

@@ -130,6 +130,32 @@ def test_bc_blocked_dense_levels(rg, orc, ctx, graphs, gname, threshold):
     assert rel_linf(deps, ref) < FP_TOL
 
 
+@pytest.mark.parametrize("gname", ["rmat10s", "rmat10d", "rmat12s", "rmat12d"])
+def test_bc_dense_levels_repeat_exactly_enough(rg, orc, ctx, graphs, gname):
+    # Every level dense (threshold 0), four sources with the carry-over, run
+    # three times: each run within 1e-10 of the oracle and of the first run
+    # (the only run-to-run difference the LDS atomic order).  A 2^10-vertex
+    # graph is one workgroup per level; 2^12 several.
+    hg = rg.HostGraph.generate(10, 16, symmetric=gname.endswith("s")) if gname.startswith("rmat10") else graphs(gname)
+    off, nbr = hg.csr()
+    ioff, inbr = hg.csr(in_edges=True)
+    srcs = hg.pick_bc_sources(4)
+    ref, rc = orc.bc(off, nbr, ioff, inbr, srcs)
+    dg = rg.DeviceGraph(ctx, hg)
+    first = None
+    for rep in range(3):
+        print(f"bc dense repeat {gname} {rep}", flush=True)
+        for variant in ("blocked", "radar"):
+            deps, comb, _ = dg.bc(srcs, rg.opts(variant, threshold=0))
+            if not hg.symmetric and len(srcs) % 2:
+                dg.transpose()   # an odd number of sources leaves a directed graph transposed
+            assert comb == rc
+            assert rel_linf(deps, ref) < FP_TOL, (rep, variant)
+            if first is None:
+                first = deps
+            assert rel_linf(deps, first) < FP_TOL
+
+
 def test_bc_blocked_split_hub_block(rg, orc, ctx):
     # A hub with 2^20 in-edges: its destination block is split over units
     # whose partial sums the merge kernel adds.

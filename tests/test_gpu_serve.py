@@ -151,3 +151,41 @@ def test_small_batch_limit(rg, ctx, graphs, max_batch):
             assert x.bfs.astuple() == y.bfs.astuple(), f"query {i}"
         else:
             assert x.nonfinite == y.nonfinite and close(x.wsum, y.wsum), f"query {i}"
+
+
+@pytest.mark.parametrize("one_at_a_time", [True, False])
+def test_hubdup_views_leave_base_metadata(rg, orc, graphs, one_at_a_time):
+    # The serving slots' views share the base graph's hub metadata, heavy
+    # list and layouts.  A HUBDUP BC query transposes a directed view, whose
+    # hub metadata (built for the other orientation) is then rebuilt: the
+    # view builds its own and leaves the base's arrays alone, so the base
+    # still answers HUBDUP queries correctly afterwards and frees cleanly.
+    hg = graphs("rmat12d")
+    off, nb = hg.csr()
+    in_off, in_nb = hg.csr(in_edges=True)
+    c = rg.Context(0)
+    try:
+        dg = rg.DeviceGraph(c, hg)
+        o = rg.opts("hubdup", hubs=256)
+        ref, it, _ = orc.pagerank(off, nb, 20)
+        before = dg.pagerank(20, o)   # base builds its hub metadata and heavy list
+        assert before.iters == it
+        q = rg.poisson_workload(hg, 40, 0.0, mix=(0.4, 0.4, 0.2), seed=11)
+        print("serve hubdup views", one_at_a_time, flush=True)
+        res, st = dg.serve(q, rg.serve_config(slots=2, one_at_a_time=one_at_a_time, o=o))
+        assert st.queries == len(q)
+        for qq, r in zip(q, res):
+            if qq.kind == rg.Q_BC:
+                ref_bc, _ = orc.bc(off, nb, in_off, in_nb, [qq.source])
+                ws, nf = fp_summary(orc, ref_bc)
+                assert r.nonfinite == nf and close(r.wsum, ws)
+        after = dg.pagerank(20, o)
+        assert after.iters == it
+        assert np.max(np.abs(after.ranks - ref)) <= FP_TOL * np.max(np.abs(ref))
+        src = int(hg.pick_sources(1)[0])
+        b = dg.bfs(src, rg.opts("hubdup", hubs=128))
+        lv, _ = orc.bfs_levels(off, nb, src)
+        assert orc.check_parents(off, nb, src, lv, b.parents) == 0
+        dg.close()
+    finally:
+        c.close()

@@ -66,6 +66,31 @@ def bfs_gteps(g, srcs, o):
             "alg_gbs": round(gbs, 1), "roofline_frac": round(gbs / PEAK_HBM_GBS, 4)}
 
 
+def host_cpu():
+    """The host CPU the baseline runs on: the cores this process may use
+    (sched_getaffinity), the OpenMP thread cap if one is set (the GPU host
+    sets OMP_NUM_THREADS to its CPU share), nproc and the lscpu model."""
+    allowed = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    used = min(allowed, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else allowed
+    model, sockets, per_core = None, None, None
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                model = v.strip()
+            elif k.strip() == "Socket(s)":
+                sockets = v.strip()
+            elif k.strip() == "Thread(s) per core":
+                per_core = v.strip()
+    except Exception:
+        pass
+    return {"threads_used": used, "affinity_cpus": allowed, "nproc": os.cpu_count(),
+            "omp_num_threads": cap, "model": model, "sockets": sockets, "threads_per_core": per_core}
+
+
 def bc_bytes(summary):
     """BC's algorithmic bytes for one source: forward and backward 24 B per
     scanned arc each (4 B column + 4 B level + 16 B FP64 read-modify-write)
@@ -206,8 +231,15 @@ def tc_study(rg, args):
     reorder = (time.time() - t0) * 1e3
     sg.tc()
     total, ms = sg.tc()
-    # Algorithmic bytes: 8 (n + 1) + 4 m+ + 4 sum over oriented arcs (u, v),
-    # v < u, of d+(u) + d+(v), with d+(u) = |{v in N(u): v < u}|.
+    # TC is latency-bound (list probes), not a stream: it is reported by its
+    # compulsory bytes (offsets and the oriented lists read once, 8 (n + 1)
+    # + 4 m+) and by the set-intersection probes its kernels make.  The
+    # totals kernels split at the first vertex of degree below 512 (at most
+    # 2^19): an arc (u, v) with v below the split tests the neighbours of u
+    # below v against an LDS bitmap of N+(v) (its position of v in N+(u)
+    # probes), one above it binary-searches each w of N+(v) in N+(u)
+    # (d+(v) probes).  The merge model of SURVEY's TC row (8 (n + 1) + 4 m+ +
+    # 4 sum of d+(u) + d+(v) over oriented arcs) is kept beside them.
     off, nbr = sg.download()
     deg = np.diff(off.astype(np.int64))
     u = np.repeat(np.arange(n, dtype=np.uint32), deg)
@@ -215,12 +247,22 @@ def tc_study(rg, args):
     dplus = np.bincount(u[low], minlength=n).astype(np.int64)
     mplus = int(low.sum())
     scan = int(dplus[u[low]].sum() + dplus[nbr[low]].sum())
-    del off, nbr, u, low
-    tbytes = 8 * (n + 1) + 4 * mplus + 4 * scan
+    below = np.nonzero(deg < 512)[0]
+    split = int(min(below[0] if len(below) else n, 1 << 19))
+    pos = (np.arange(len(nbr), dtype=np.int64) - off[:-1].astype(np.int64)[u])[low]   # v's index in N(u)
+    vv = nbr[low].astype(np.int64)
+    probes = int(np.where(vv < split, pos, dplus[vv]).sum())
+    del off, nbr, u, low, pos, vv
+    merge_bytes = 8 * (n + 1) + 4 * mplus + 4 * scan
+    comp_bytes = 8 * (n + 1) + 4 * mplus
     out = {"graph": f"rmat{args.tc_scale} symmetric, degree-sorted", "n": n, "m": m, "triangles": total,
            "count_ms": round(ms, 3), "reorder_ms": round(reorder, 2),
-           "alg_bytes": tbytes, "compulsory_bytes": 8 * n + 4 * mplus,
-           "roofline_frac": round(tbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+           "compulsory_bytes": comp_bytes,
+           "compulsory_gbs": round(comp_bytes / (ms * 1e-3) / 1e9, 1),
+           "compulsory_frac": round(comp_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+           "probes": probes, "giga_probes_per_s": round(probes / (ms * 1e-3) / 1e9, 2), "bitmap_split": split,
+           "merge_model_bytes": merge_bytes,
+           "merge_model_frac": round(merge_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
     log(f"tc rmat{args.tc_scale}: {total} triangles in {ms:.1f} ms (reorder {reorder:.1f} ms)")
     for x in (sg, dg):
         x.close()
@@ -449,7 +491,8 @@ def main():
     if not args.no_cpu:
         import pyoracle
         off, nbr = hg.csr()
-        cores = min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        cores = host["threads_used"]
         t0 = time.perf_counter()
         pyoracle.pagerank(off, nbr, 1, nthreads=cores)
         per_iter = time.perf_counter() - t0
@@ -460,7 +503,8 @@ def main():
         cpu_iters = min(done_iters, cpu_iters)
         cpu = {"value": round(cpu_iters * m / tc / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
                "sample": f"oracle push PageRank (OpenMP, CAS-loop FP64 adds as the reference), "
-                         f"{cpu_iters} iterations on the same rmat{args.scale} graph, {tc:.1f} s"}
+                         f"{cpu_iters} iterations on the same rmat{args.scale} graph, {tc:.1f} s",
+               "host": host}
         log(f"cpu baseline: {cpu['value']} GTEPS on {cores} threads")
         del off, nbr
     dg.close()

@@ -137,9 +137,15 @@ def variants_table(rg, args):
     out["hub_sweep"] = sweep
     srcs = hg.pick_sources(args.variants_bfs_sources)
     bt, _ = sg.autotune("bfs", sources=ids[srcs[:4]], base=rg.opts("push"))
+    # hubdup_full_lds: the hub copy sized to the whole LDS budget and used on
+    # every launch (one or two workgroups per CU), as before the default copy
+    # left room for four workgroups and small launches went without copies.
+    full = dict(hubs=1 << 30, hub_min_arcs=-1)
     bfs_runs = [("push", dg, srcs, rg.opts("push")), ("hubdup", dg, srcs, rg.opts("hubdup", hubs=args.hubs)),
+                ("hubdup_full_lds", dg, srcs, rg.opts("hubdup", **full)),
                 ("degree_sorting", sg, ids[srcs], rg.opts("push")),
                 ("radar", sg, ids[srcs], rg.opts("radar", hubs=args.hubs)),
+                ("radar_full_lds", sg, ids[srcs], rg.opts("radar", **full)),
                 ("push_pull", dg, srcs, rg.opts("pull")), ("radar_tuned", sg, ids[srcs], bt)]
     for name, g, ss, o in bfs_runs:
         out["bfs"][name] = bfs_gteps(g, ss, o)
@@ -173,8 +179,10 @@ def radii_bc_study(rg, args):
         log(f"{kind}{args.radii_scale} symmetric n={n} m={m} ({time.time() - t0:.1f}s)")
         starts = rg.radii_starts(n)
         row = {"n": n, "m": m, "radii_ms": {}, "bc_ms": {}}
+        full = dict(hubs=1 << 30, hub_min_arcs=-1)   # as in the variants table
         for name, g, st, o in [("push", dg, starts, rg.opts("push")),
                                ("hubdup", dg, starts, rg.opts("hubdup", hubs=args.hubs)),
+                               ("hubdup_full_lds", dg, starts, rg.opts("hubdup", **full)),
                                ("radar", sg, ids[starts], rg.opts("radar", hubs=args.hubs))]:
             g.radii(st, o)
             _, rounds, ms = g.radii(st, o)
@@ -183,6 +191,7 @@ def radii_bc_study(rg, args):
         src = hg.pick_bc_sources(1)
         for name, g, ss, o in [("push", dg, src, rg.opts("push")),
                                ("hubdup", dg, src, rg.opts("hubdup", hubs=args.hubs)),
+                               ("hubdup_full_lds", dg, src, rg.opts("hubdup", **full)),
                                ("radar", sg, ids[src], rg.opts("radar", hubs=args.hubs))]:
             g.bc(ss, o)
             _, _, ms = g.bc(ss, o)
@@ -327,6 +336,50 @@ def serve_bench(rg, args, resident=None):
     return out
 
 
+def profile_run(rg, args):
+    """One app on one engine, repeated, for counter collection under
+    rocprofv3: --profile app:variant:scale[d|s][:key=value,...] (app bfs,
+    pagerank, radii or bc; variant push, hubdup, radar, blocked or pull; d =
+    directed, s = symmetric; radar and blocked run on the degree-sorted
+    graph; options are rg.opts keywords, e.g. hubs=1073741824,
+    hub_min_arcs=-1)."""
+    parts = args.profile.split(":")
+    app, variant, gspec = parts[0], parts[1], parts[2]
+    kw = {}
+    if len(parts) > 3 and parts[3]:
+        for item in parts[3].split(","):
+            k, v = item.split("=")
+            kw[k] = int(v)
+    scale, sym = int(gspec[:-1]), gspec[-1] == "s"
+    hg = rg.HostGraph.generate(scale, 16, symmetric=sym)
+    ctx = rg.Context(0)
+    dg = rg.DeviceGraph(ctx, hg)
+    ids = None
+    if variant in ("radar", "blocked"):
+        g, ids = dg.degree_sort()
+    else:
+        g = dg
+    o = rg.opts(variant, **kw)
+    srcs = hg.pick_sources(args.profile_reps)
+    if ids is not None:
+        srcs = ids[srcs]
+    def run(i):
+        if app == "bfs":
+            return g.bfs(int(srcs[i]), o, want_parents=False).ms
+        if app == "pagerank":
+            return g.pagerank(args.iters, o, want_ranks=False).ms
+        if app == "radii":
+            st = rg.radii_starts(g.n)
+            return g.radii(st if ids is None else ids[st], o)[2]
+        return g.bc(srcs[i:i + 1], o)[2]
+
+    run(0)   # set-up outside the timed runs (hub metadata, layouts)
+    ms = sum(run(i) for i in range(args.profile_reps)) / args.profile_reps
+    ctx.sync()
+    log(f"profile {args.profile}: {ms:.3f} ms per run (device time)")
+    return {"profile": args.profile, "ms_per_run": round(ms, 3), "reps": args.profile_reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,6 +401,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--sections", default="headline,variants,radii,tc,serve",
                     help="comma list of sections to run (headline,variants,radii,tc,serve)")
+    ap.add_argument("--profile", default="", help="section 'profile': app:variant:scale[d|s][:opts] (see profile_run)")
+    ap.add_argument("--profile-reps", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the library's plain 1 GiB stream copy once (k_stream_copy: the "
                          "known-size kernel that scales FETCH_SIZE / WRITE_SIZE)")
@@ -363,6 +418,8 @@ def main():
     if "headline" not in sections:
         # Sections on their own (e.g. for counter collection): no headline line.
         out = {}
+        if "profile" in sections and args.profile:
+            out["profile"] = profile_run(rg, args)
         if "variants" in sections and args.variants_scale > 0:
             out["variants"] = variants_table(rg, args)
         if "radii" in sections and args.radii_scale > 0:

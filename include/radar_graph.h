@@ -179,19 +179,24 @@ enum { RG_HUBS_LDS = 0, RG_HUBS_L2 = 1 };
 typedef struct rg_opts {
   int variant;          /* RG_PUSH ...                                        */
   int hub_placement;    /* RG_HUBS_LDS / RG_HUBS_L2                           */
-  int64_t hubs;         /* k of the LDS tier; <= 0 selects the LDS budget     */
+  int64_t hubs;         /* k of the LDS tier; <= 0: what leaves room for one  */
+                        /* workgroup per CU (PageRank, BC backward) or four   */
+                        /* (BFS, Radii, BC forward: latency-bound)            */
   int64_t threshold;    /* sparse/dense switch; -1 = m/20 (ligra.h:269)       */
   int wg_size;          /* workgroup size for edge kernels; 0 = default       */
   int heavy_degree;     /* wave-per-vertex cut-over degree; 0 = default       */
-  int64_t l2_hubs;      /* HUBDUP/RADAR FP64 accumulators: hub slots [hubs,    */
-                        /* l2_hubs) get one copy per XCD (L2-resident); 0 =    */
-                        /* half an XCD L2 (256K), < 0 = off                    */
+  int64_t l2_hubs;      /* HUBDUP/RADAR PageRank and BC backward: hub slots    */
+                        /* [hubs, l2_hubs) get one copy per XCD; <= hubs = off */
+                        /* (the default: measured no gain)                     */
   int dst_block_bits;   /* RG_BLOCKED: 2^bits destinations per LDS block       */
                         /* (10..14; 0 = 14)                                    */
   int src_segment_bits; /* RG_BLOCKED: 2^bits sources per L2 segment (1..24;   */
                         /* 0 = order by source id)                             */
   int unroll;           /* RG_BLOCKED: edges in flight per lane (1 or 4; 0 =   */
                         /* 4 on a graph made by rg_graph_degree_sort, else 1) */
+  int64_t hub_min_arcs; /* HUBDUP/RADAR: list frontiers with fewer arcs        */
+                        /* (|F| + out-degree sum) run without hub copies; 0 =  */
+                        /* 4096 per CU, < 0 = always with copies               */
 } rg_opts;
 
 /* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */

@@ -34,6 +34,7 @@ struct PrOp {
   __device__ bool skip() const { return *done != 0; }
   __device__ double load_src(uint32_t s) const { return contrib[s]; }
   __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool pre_hub(double, uint32_t) const { return true; }
   __device__ bool update(uint32_t, double sv, uint32_t d) const {
     unsafeAtomicAdd(acc + d, sv);
     return false;
@@ -70,6 +71,7 @@ struct PrdTouchOp {
   __device__ bool skip() const { return false; }
   __device__ double load_src(uint32_t s) const { return contrib[s]; }
   __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool pre_hub(double, uint32_t) const { return true; }
   // A plain read first: once d carries this round's tag (most arcs into a
   // hub) no returning atomic is issued; a stale read only costs the exchange,
   // which still picks exactly one first toucher.
@@ -102,6 +104,7 @@ struct BfsOp {
   __device__ bool skip() const { return false; }
   __device__ uint32_t load_src(uint32_t s) const { return s; }
   __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool pre_hub(uint32_t, uint32_t d) const { return parents[d] == kNone; }
   __device__ bool update(uint32_t s, uint32_t, uint32_t d) const {
     if (parents[d] != kNone) return false;
     return atomicCAS(parents + d, kNone, s) == kNone;
@@ -132,6 +135,7 @@ struct RadiiOp {
   __device__ bool skip() const { return false; }
   __device__ unsigned long long load_src(uint32_t s) const { return vis[s]; }
   __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool pre_hub(unsigned long long sv, uint32_t d) const { return (vis[d] | sv) != vis[d]; }
   __device__ bool claim(uint32_t d) const {
     int32_t old = radii[d];
     if (old == round) return false;
@@ -171,6 +175,7 @@ struct BcFwdOp {
   __device__ bool skip() const { return false; }
   __device__ double load_src(uint32_t s) const { return np[s]; }
   __device__ bool cond(uint32_t d) const { return visited[d] == 0; }
+  __device__ bool pre_hub(double, uint32_t) const { return true; }
   __device__ bool update(uint32_t, double sv, uint32_t d) const {
     return unsafeAtomicAdd(np + d, sv) == 0.0;
   }
@@ -198,6 +203,7 @@ struct BcBackOp {
   __device__ bool skip() const { return false; }
   __device__ double load_src(uint32_t s) const { return deps[s]; }
   __device__ bool cond(uint32_t d) const { return visited[d] == 0; }
+  __device__ bool pre_hub(double, uint32_t) const { return true; }
   __device__ bool update(uint32_t, double sv, uint32_t d) const {
     unsafeAtomicAdd(deps + d, sv);
     return false;
@@ -287,10 +293,13 @@ static int edge_map(const Engine& E, const Op& op, Frontier& F, int64_t f_degsum
   a.heavy_deg = E.heavy_deg;
   a.hub_k = E.hub_k;
   a.hub_k2 = Op::kL2 ? E.hub_k2 : E.hub_k;
-  a.hub_bits = g->hub_bits;
-  a.hub_rank = g->hub_rank;
-  a.hub_slot_of = g->hub_slot_of;
-  a.hub_ids = g->hub_ids;
+  if (E.hm) {
+    a.hub_bits = E.hm->bits;
+    a.hub_rank = E.hm->rank;
+    a.hub_slot_of = E.hm->slot_of;
+    a.hub_ids = E.hm->ids;
+    a.hub_filter = E.hm->filter;
+  }
   a.l2_copies = E.l2_copies;
   a.l2_ncopies = E.l2_ncopies;
   if (F.all) {
@@ -301,6 +310,11 @@ static int edge_map(const Engine& E, const Op& op, Frontier& F, int64_t f_degsum
     return launch_hub<Op, SRC_ALL>(E, op, a);
   }
   if (f_degsum == 0) return RG_OK;   // ligra.h:287
+  if (E.hub_mode != HUB_NONE && F.size + f_degsum < E.hub_min_arcs) {
+    Engine plain = E;
+    plain.hub_mode = HUB_NONE;
+    return edge_map(plain, op, F, f_degsum, out_ids, out_info);
+  }
   if (F.size + f_degsum > E.threshold) {
     const int grid = grid_for(E.ctx, F.size);
     hipLaunchKernelGGL(k_scatter_u8, grid, 256, 0, E.ctx->stream, F.ids, F.size, F.dense, (uint8_t)1);
@@ -351,9 +365,16 @@ __global__ void k_in_degree(const uint64_t* in_off, int64_t n, uint32_t* deg) {
 // prefix (id-order rank of a hub), a rank -> slot table and a slot -> vertex
 // table; slots are ordered by descending in-degree so the first slots (the
 // LDS tier) are the heaviest hubs.
-static int build_hub_bitmap(rg_graph* g, int64_t k) {
+static int build_hub_bitmap(rg_graph* g, int64_t k, const HubMeta** out) {
   rg_ctx* ctx = g->ctx;
-  if (g->hub_k_req == k && g->hub_bits) return RG_OK;
+  for (HubMeta& h : g->hubs)
+    if (h.k_req == k && h.for_in == g->in_off && h.bits) {
+      h.used = ++g->hub_clock;
+      *out = &h;
+      return RG_OK;
+    }
+  HubMeta& hm = g->hubs[0].used <= g->hubs[1].used ? g->hubs[0] : g->hubs[1];   // least recently used
+  hub_meta_release(g, hm);
   const int64_t n = g->n;
   std::vector<uint32_t> deg(n);
   DevBuf tmp;
@@ -394,28 +415,35 @@ static int build_hub_bitmap(rg_graph* g, int64_t k) {
     slot_of[by_deg[s]] = (uint32_t)s;
     slot_ids[s] = ids[by_deg[s]];
   }
-  for (uint32_t* p : {g->hub_bits, g->hub_rank, g->hub_ids, g->hub_slot_of})
-    if (p && !borrowed(g, p)) (void)hipFree(p);
-  g->hub_bits = g->hub_rank = g->hub_ids = g->hub_slot_of = nullptr;
-  const size_t nh = std::max<size_t>(1, ids.size());
-  RG_HIP(hipMalloc(&g->hub_bits, sizeof(uint32_t) * nw));
-  RG_HIP(hipMalloc(&g->hub_rank, sizeof(uint32_t) * nw));
-  RG_HIP(hipMalloc(&g->hub_ids, sizeof(uint32_t) * nh));
-  RG_HIP(hipMalloc(&g->hub_slot_of, sizeof(uint32_t) * nh));
-  RG_TRY(rg::ctx_copy(ctx, g->hub_bits, bits.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
-  RG_TRY(rg::ctx_copy(ctx, g->hub_rank, rank.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
-  if (!ids.empty()) {
-    RG_TRY(rg::ctx_copy(ctx, g->hub_ids, slot_ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
-    RG_TRY(rg::ctx_copy(ctx, g->hub_slot_of, slot_of.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+  std::vector<uint32_t> filt(kHubFilterWords, 0);
+  for (uint32_t v : ids) {
+    const uint32_t h = hub_hash(v);
+    filt[h >> 5] |= 1u << (h & 31);
   }
-  g->hub_k = (int64_t)ids.size();
-  g->hub_k_req = k;
+  const size_t nh = std::max<size_t>(1, ids.size());
+  RG_HIP(hipMalloc(&hm.bits, sizeof(uint32_t) * nw));
+  RG_HIP(hipMalloc(&hm.rank, sizeof(uint32_t) * nw));
+  RG_HIP(hipMalloc(&hm.ids, sizeof(uint32_t) * nh));
+  RG_HIP(hipMalloc(&hm.slot_of, sizeof(uint32_t) * nh));
+  RG_HIP(hipMalloc(&hm.filter, sizeof(uint32_t) * kHubFilterWords));
+  RG_TRY(rg::ctx_copy(ctx, hm.filter, filt.data(), sizeof(uint32_t) * kHubFilterWords, hipMemcpyHostToDevice));
+  RG_TRY(rg::ctx_copy(ctx, hm.bits, bits.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
+  RG_TRY(rg::ctx_copy(ctx, hm.rank, rank.data(), sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
+  if (!ids.empty()) {
+    RG_TRY(rg::ctx_copy(ctx, hm.ids, slot_ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+    RG_TRY(rg::ctx_copy(ctx, hm.slot_of, slot_of.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+  }
+  hm.k = (int64_t)ids.size();
+  hm.k_req = k;
+  hm.for_in = g->in_off;
+  hm.used = ++g->hub_clock;
+  *out = &hm;
   return RG_OK;
 }
 
 // Resolves variant/opts into an Engine.  elem = bytes of one hub copy entry.
-int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E, uint64_t* heavy, int64_t heavy_cap,
-                int64_t* heavy_count) {
+int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E, int wgs_per_cu, bool l2_tier,
+                uint64_t* heavy, int64_t heavy_cap, int64_t* heavy_count) {
   rg_ctx* ctx = g->ctx;
   E.ctx = ctx;
   E.g = g;
@@ -430,28 +458,30 @@ int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E, uint64_t*
   if (variant == RG_HUBDUP || variant == RG_RADAR) {
     const int wpb = E.block / kWave;
     size_t static_lds = wpb * ((kWave + 1) * 4 + kWave * (8 + 4 + 8) + kQueue * 4);
+    if (variant == RG_HUBDUP) static_lds += sizeof(uint32_t) * kHubFilterWords;   // the membership filter
     size_t budget = ctx->max_lds_per_block > static_lds ? ctx->max_lds_per_block - static_lds : 0;
     int64_t kmax = (int64_t)(budget / elem);
-    // LDS tier: the requested k or the LDS budget (the reference sizes its
-    // copies from the cache budget llcCap, hubdup IO.h:749-755).
-    int64_t k = (o && o->hubs > 0) ? std::min<int64_t>(o->hubs, kmax) : kmax;
+    // LDS tier: the requested k, or by default what leaves room for
+    // wgs_per_cu workgroups per CU (the reference sizes its copies from the
+    // cache budget llcCap, hubdup IO.h:749-755).
+    const size_t share = ctx->lds_per_cu / (size_t)std::max(1, wgs_per_cu);
+    const int64_t kdef = share > static_lds ? std::min<int64_t>(kmax, (int64_t)((share - static_lds) / elem)) : 0;
+    int64_t k = (o && o->hubs > 0) ? std::min<int64_t>(o->hubs, kmax) : kdef;
     k = std::min<int64_t>(k, g->n);
-    // Per-XCD tier: up to half of a 4 MB XCD L2 per copy by default.
+    // Per-XCD tier, when asked for: its adds execute past the L2 like any
+    // global atomic on this part, and the hub sweep measured no gain from it.
     int64_t k2 = k;
-    if (elem == sizeof(double)) {
-      int64_t l2 = (o && o->l2_hubs != 0) ? o->l2_hubs : (int64_t)((2u << 20) / elem);
-      if (l2 > k) k2 = std::min<int64_t>(l2, g->n);
-    }
+    if (l2_tier && elem == sizeof(double) && o && o->l2_hubs > k) k2 = std::min<int64_t>(o->l2_hubs, g->n);
     if (k > 0) {
       if (variant == RG_RADAR) {
         E.hub_mode = HUB_PREFIX;
         E.hub_k = k;
         E.hub_k2 = k2;
       } else {
-        RG_TRY(build_hub_bitmap(g, k2));
+        RG_TRY(build_hub_bitmap(g, k2, &E.hm));
         E.hub_mode = HUB_BITMAP;
-        E.hub_k2 = g->hub_k;
-        E.hub_k = std::min<int64_t>(k, g->hub_k);
+        E.hub_k2 = E.hm->k;
+        E.hub_k = std::min<int64_t>(k, E.hm->k);
         if (E.hub_k2 == 0) E.hub_mode = HUB_NONE;
       }
     }
@@ -464,6 +494,10 @@ int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E, uint64_t*
       E.l2_copies = b.p;
     }
   }
+  // Small launches skip the hub copies (o->hub_min_arcs: < 0 never, 0 the
+  // default of 4096 frontier arcs per CU).
+  E.hub_min_arcs = (o && o->hub_min_arcs < 0) ? 0
+                   : (o && o->hub_min_arcs > 0) ? o->hub_min_arcs : (int64_t)ctx->num_cus * 4096;
   // Heavy queue: at most m/heavy_deg heavy vertices, each split in chunks.
   int64_t cap = heavy_queue_items(g->m, E.heavy_deg);
   if (heavy) {
@@ -555,7 +589,7 @@ int pagerank_push(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ranks
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
   Engine E;
-  RG_TRY(make_engine(g, o, sizeof(double), E));
+  RG_TRY(make_engine(g, o, sizeof(double), E, 1, true));
   DevBuf &b_rank = ctx->scratch[0], &b_contrib = ctx->scratch[1], &b_acc = ctx->scratch[2],
          &b_part = ctx->scratch[3];
   RG_TRY(b_rank.ensure(sizeof(double) * n));
@@ -827,9 +861,9 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   int64_t* counts = work ? work->counts : ctx->counts.as<int64_t>();
   Engine E;
   if (work)
-    RG_TRY(make_engine(g, o, sizeof(uint32_t), E, work->heavy, work->heavy_cap, counts + 0));
+    RG_TRY(make_engine(g, o, sizeof(uint32_t), E, 4, false, work->heavy, work->heavy_cap, counts + 0));
   else
-    RG_TRY(make_engine(g, o, sizeof(uint32_t), E));
+    RG_TRY(make_engine(g, o, sizeof(uint32_t), E, 4));
   DevBuf& b_par = ctx->scratch[0];
   uint32_t* parents = parents_dev;
   if (!parents) {
@@ -947,7 +981,7 @@ int radii_push(rg_graph* g, const uint32_t* starts, int nstarts, const rg_opts* 
   for (int i = 0; i < nstarts; ++i)
     if ((int64_t)starts[i] >= n) return set_error(RG_EINVAL, "rg_radii: start out of range");
   Engine E;
-  RG_TRY(make_engine(g, o, sizeof(uint64_t), E));
+  RG_TRY(make_engine(g, o, sizeof(uint64_t), E, 4));
   DevBuf &b_rad = ctx->scratch[0], &b_vis = ctx->scratch[1], &b_nvis = ctx->scratch[2];
   RG_TRY(b_rad.ensure(sizeof(int32_t) * n));
   RG_TRY(b_vis.ensure(sizeof(uint64_t) * n));
@@ -1077,7 +1111,7 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
   EventTimer tm(ctx->stream);
   for (int it = 0; it < nsources; ++it) {
     Engine E;
-    RG_TRY(make_engine(g, o, sizeof(double), E));
+    RG_TRY(make_engine(g, o, sizeof(double), E, 4));
     uint32_t start = sources[it];
     RG_HIP(hipMemsetAsync(np, 0, sizeof(double) * n, ctx->stream));
     RG_HIP(hipMemsetAsync(visited, 0, (size_t)n, ctx->stream));
@@ -1129,7 +1163,7 @@ int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o
                          visited, deps, np);
     RG_TRY(rg_graph_transpose(g));
     Engine EB;
-    RG_TRY(make_engine(g, o, sizeof(double), EB));
+    RG_TRY(make_engine(g, o, sizeof(double), EB, 4, true));
     BcBackOp bop{deps, visited};
     for (int64_t r = round - 2; r >= 0; --r) {
       // The frontier is level r + 1; its out-degree sum in the transposed
@@ -1691,13 +1725,13 @@ int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int n
   switch (op) {
     case RG_OP_BFS: {
       RG_TRY(rg::check_state(g, state, nstate, {4}, "edgeMap(BFS_F)"));
-      RG_TRY(rg::make_engine(g, &oo, sizeof(uint32_t), E));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(uint32_t), E, 4));
       RG_TRY(rg::edge_map(E, rg::BfsOp{state[0]->buf.as<uint32_t>()}, F, degsum, out_ids, info));
       break;
     }
     case RG_OP_PAGERANK: {
       RG_TRY(rg::check_state(g, state, nstate, {8, 8}, "edgeMap(PR_F)"));
-      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E, 1, true));
       int64_t* zero = ctx->counts.as<int64_t>() + 30;
       RG_HIP(hipMemsetAsync(zero, 0, sizeof(int64_t), ctx->stream));
       rg::PrOp op2{state[0]->buf.as<double>(), state[1]->buf.as<double>(), zero};
@@ -1706,7 +1740,7 @@ int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int n
     }
     case RG_OP_RADII: {
       RG_TRY(rg::check_state(g, state, nstate, {8, 8, 4}, "edgeMap(Radii_F)"));
-      RG_TRY(rg::make_engine(g, &oo, sizeof(uint64_t), E));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(uint64_t), E, 4));
       rg::RadiiOp op2{state[0]->buf.as<unsigned long long>(), state[1]->buf.as<unsigned long long>(),
                       state[2]->buf.as<int32_t>(), (int32_t)round};
       RG_TRY(rg::edge_map(E, op2, F, degsum, out_ids, info));
@@ -1714,14 +1748,14 @@ int rg_edgemap(rg_graph* g, int op, rg_subset* in, rg_array* const* state, int n
     }
     case RG_OP_BC_FWD: {
       RG_TRY(rg::check_state(g, state, nstate, {8, 1}, "edgeMap(BC_F)"));
-      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E, 4));
       RG_TRY(rg::edge_map(E, rg::BcFwdOp{state[0]->buf.as<double>(), state[1]->buf.as<uint8_t>()}, F, degsum,
                           out_ids, info));
       break;
     }
     case RG_OP_BC_BACK: {
       RG_TRY(rg::check_state(g, state, nstate, {8, 1}, "edgeMap(BC_Back_F)"));
-      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E));
+      RG_TRY(rg::make_engine(g, &oo, sizeof(double), E, 4, true));
       RG_TRY(rg::edge_map(E, rg::BcBackOp{state[0]->buf.as<double>(), state[1]->buf.as<uint8_t>()}, F, degsum,
                           nullptr, nullptr));
       break;

@@ -47,7 +47,7 @@ extern "C" int rg_autotune(rg_graph* g, int app, const uint32_t* sources, int ns
         cand.push_back(o);
       }
     for (int64_t k : {0, 4096})
-      for (int64_t k2 : {-1, 0}) {
+      for (int64_t k2 : {(int64_t)-1, (int64_t)262144}) {
         rg_opts r = b;
         r.variant = RG_RADAR;
         r.hubs = k;

@@ -94,6 +94,23 @@ struct rg_ctx {
   void* fixed_scratch = nullptr;
 };
 
+namespace rg {
+// HUBDUP's hubs for one orientation and k: the top-k in-degree vertices as a
+// bitmap with a per-32-bit-word rank prefix, rank -> slot and slot -> vertex
+// tables (slots by descending in-degree) and the LDS membership filter.
+struct HubMeta {
+  uint32_t* bits = nullptr;
+  uint32_t* rank = nullptr;
+  uint32_t* ids = nullptr;       // slot -> vertex
+  uint32_t* slot_of = nullptr;   // id-order hub rank -> slot
+  uint32_t* filter = nullptr;    // kHubFilterWords words: a bit per hub_hash of each hub
+  int64_t k = 0;                 // hubs selected
+  int64_t k_req = -1;            // k asked for (-1: empty entry)
+  const uint64_t* for_in = nullptr;   // in-offsets of the orientation
+  uint64_t used = 0;             // last use (the graph's hub_clock)
+};
+}  // namespace rg
+
 struct rg_graph {
   rg_ctx* ctx = nullptr;
   int64_t n = 0, m = 0;
@@ -105,14 +122,11 @@ struct rg_graph {
   uint32_t* in_nbr = nullptr;
   bool transposed = false;
   bool degree_sorted = false;   // ids by descending degree (made by rg_graph_degree_sort)
-  // Hub metadata (HUBDUP): bitmap of hubs + per-word rank prefix, for the
-  // in-degree ordering of the current orientation.
-  uint32_t* hub_bits = nullptr;
-  uint32_t* hub_rank = nullptr;
-  uint32_t* hub_ids = nullptr;   // slot -> vertex
-  uint32_t* hub_slot_of = nullptr; // id-order hub rank -> slot id
-  int64_t hub_k = 0;             // hubs selected
-  int64_t hub_k_req = -1;        // k the metadata was built for
+  // Hub metadata (HUBDUP), the two most recently used (orientation, k)
+  // pairs: BC's forward and backward passes, or the two orientations of a
+  // directed graph, each keep theirs.
+  rg::HubMeta hubs[2];
+  uint64_t hub_clock = 0;
   // Static heavy-chunk list for all-vertices frontiers (PageRank), built
   // for one (out-CSR, heavy degree) pair.
   uint64_t* heavy_items = nullptr;
@@ -147,10 +161,14 @@ namespace rg {
 inline bool borrowed(const rg_graph* g, const void* p) {
   if (!g->view || !g->base || !p) return false;
   const rg_graph* b = g->base;
-  return p == b->hub_bits || p == b->hub_rank || p == b->hub_ids || p == b->hub_slot_of ||
-         p == b->heavy_items || p == b->first_in || p == b->blocked || p == b->blocked_ms ||
+  for (const HubMeta& h : b->hubs)
+    if (p == h.bits || p == h.rank || p == h.ids || p == h.slot_of || p == h.filter) return true;
+  return p == b->heavy_items || p == b->first_in || p == b->blocked || p == b->blocked_ms ||
          p == b->blocked_ms_t;
 }
+
+// Frees an entry's arrays (those the graph does not borrow) and empties it.
+void hub_meta_release(rg_graph* g, HubMeta& h);
 
 // Device frontier (Ligra vertexSubset, vertexSubset.h:113-229): a sparse id
 // list (always kept) and, for dense rounds, a byte map over all n.  A view:

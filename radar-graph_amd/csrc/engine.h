@@ -43,6 +43,13 @@ namespace rg {
 
 constexpr int kHeavyChunk = 2048;
 constexpr int kQueue = 256;   // per-wave LDS staging queue (vertices)
+// HUBDUP's hub membership filter: a bit per hash of every hub, staged in
+// each workgroup's LDS after its hub copy; an arc whose destination's bit is
+// clear is no hub and skips the global bitmap lookup.
+constexpr int kHubFilterBits = 16;
+constexpr int kHubFilterWords = (1 << kHubFilterBits) / 32;
+
+__host__ __device__ inline uint32_t hub_hash(uint32_t d) { return (d * 0x9E3779B1u) >> (32 - kHubFilterBits); }
 
 enum SrcMode { SRC_ALL = 0, SRC_LIST = 1, SRC_DENSE = 2 };
 enum HubMode { HUB_NONE = 0, HUB_BITMAP = 1, HUB_PREFIX = 2 };
@@ -73,6 +80,7 @@ struct EdgeArgs {
   const uint32_t* hub_rank;       // HUB_BITMAP: hubs before word w
   const uint32_t* hub_slot_of;    // HUB_BITMAP: id-order rank -> slot
   const uint32_t* hub_ids;        // HUB_BITMAP: slot -> vertex
+  const uint32_t* hub_filter;     // HUB_BITMAP: kHubFilterWords words of hub_hash bits
   void* l2_copies;
   int l2_ncopies;
 };
@@ -130,9 +138,11 @@ __device__ __forceinline__ void wq_finish(WaveQueue& q, const EdgeArgs& a) {
 }
 
 template <int HUB>
-__device__ __forceinline__ int64_t hub_slot(const EdgeArgs& a, uint32_t d) {
+__device__ __forceinline__ int64_t hub_slot(const EdgeArgs& a, const uint32_t* lds_filter, uint32_t d) {
   if (HUB == HUB_PREFIX) return (int64_t)d < a.hub_k2 ? (int64_t)d : -1;
   if (HUB == HUB_BITMAP) {
+    const uint32_t h = hub_hash(d);
+    if (!((lds_filter[h >> 5] >> (h & 31)) & 1u)) return -1;
     uint32_t w = a.hub_bits[d >> 5];
     uint32_t bit = 1u << (d & 31);
     if (!(w & bit)) return -1;
@@ -154,7 +164,12 @@ __device__ __forceinline__ bool do_edge(const Op& op, const EdgeArgs& a, typenam
                                         uint32_t d) {
   if (!op.cond(d)) return false;
   if (HUB != HUB_NONE) {
-    int64_t slot = hub_slot<HUB>(a, d);
+    // A destination the arc cannot change (a claimed BFS vertex, a word
+    // that already holds the bits) is dropped before the hub lookup, so the
+    // lookup's loads are paid on the arcs that update only.
+    if (!op.pre_hub(sv, d)) return false;
+    const uint32_t* filt = reinterpret_cast<const uint32_t*>(lds_hub + a.hub_k);
+    int64_t slot = hub_slot<HUB>(a, filt, d);
     if (slot >= 0 && slot < a.hub_k) {
       op.hub_update(&lds_hub[slot], s, sv, d);
       return false;
@@ -177,6 +192,10 @@ template <class Op, int HUB, int BLOCK>
 __device__ __forceinline__ void hub_init(const Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub) {
   if (HUB == HUB_NONE) return;
   for (int64_t i = threadIdx.x; i < a.hub_k; i += BLOCK) lds_hub[i] = op.hub_identity();
+  if (HUB == HUB_BITMAP) {
+    uint32_t* filt = reinterpret_cast<uint32_t*>(lds_hub + a.hub_k);
+    for (int i = threadIdx.x; i < kHubFilterWords; i += BLOCK) filt[i] = a.hub_filter[i];
+  }
   __syncthreads();
 }
 

@@ -62,6 +62,11 @@ struct Engine {
   uint64_t* heavy;     // heavy work queue (chunk items)
   int64_t heavy_cap = 0;   // its capacity in items (0: never lower the cut-over)
   int64_t* heavy_count;
+  // List / byte-map frontiers with fewer arcs (|F| + out-degree sum) run
+  // without hub copies: their few workgroups would spend as long setting up
+  // and merging the copies as on the arcs.  0: always with copies.
+  int64_t hub_min_arcs = 0;
+  const HubMeta* hm = nullptr;   // HUB_BITMAP: the hub metadata in use
 };
 
 template <class Op, int MODE, int HUB, int BLOCK>
@@ -124,18 +129,25 @@ template <class Op, int MODE>
 static int launch_hub(const Engine& E, const Op& op, EdgeArgs a) {
   size_t hub_bytes = (size_t)a.hub_k * sizeof(typename Op::HubT);
   switch (E.hub_mode) {
-    case HUB_BITMAP: return launch_block<Op, MODE, HUB_BITMAP>(E, op, a, hub_bytes);
+    case HUB_BITMAP:   // the copy, then the membership filter
+      return launch_block<Op, MODE, HUB_BITMAP>(E, op, a, hub_bytes + sizeof(uint32_t) * kHubFilterWords);
     case HUB_PREFIX: return launch_block<Op, MODE, HUB_PREFIX>(E, op, a, hub_bytes);
     default: a.hub_k = a.hub_k2 = 0; return launch_block<Op, MODE, HUB_NONE>(E, op, a, 0);
   }
 }
 
-// Resolves variant/opts into an Engine.  elem = bytes of one hub copy entry.
-// heavy / heavy_cap / heavy_count (all given or none): the heavy queue and
-// its counter in caller memory instead of the context pools; only the push
-// variant (no hub copies) then runs without allocating.
-int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E, uint64_t* heavy = nullptr,
-                int64_t heavy_cap = 0, int64_t* heavy_count = nullptr);
+// Resolves variant/opts into an Engine.  elem = bytes of one hub copy entry;
+// wgs_per_cu = workgroups per CU the default LDS hub copy leaves room for
+// (1 for the accumulating operators, bound by memory-side atomics, whose
+// copies pay at any size; 4 for the traversal operators, bound by latency,
+// which need the occupancy).  l2_tier: the operator takes the per-XCD hub
+// copies (Op::kL2; hubs past the LDS tier are otherwise plain vertices, and
+// the hub metadata covers the LDS tier alone).  heavy / heavy_cap /
+// heavy_count (all given or none): the heavy queue and its counter in caller
+// memory instead of the context pools; only the push variant (no hub
+// copies) then runs without allocating.
+int make_engine(rg_graph* g, const rg_opts* o, size_t elem, Engine& E, int wgs_per_cu = 1, bool l2_tier = false,
+                uint64_t* heavy = nullptr, int64_t heavy_cap = 0, int64_t* heavy_count = nullptr);
 
 // ---------------------------------------------------------------------------
 // Timing helper (device time of a region on the context stream).

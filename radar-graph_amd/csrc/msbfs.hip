@@ -129,6 +129,7 @@ struct MsPushOp {
     return r;
   }
   __device__ bool cond(uint32_t) const { return true; }
+  __device__ bool pre_hub(const SrcT&, uint32_t) const { return true; }
   // next[d] |= front[s] & ~seen[d]; d joins the touched list once per level.
   __device__ bool update(uint32_t, SrcT sv, uint32_t d) const {
     bool any = false;
@@ -734,7 +735,7 @@ int batch_run(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
     po.wg_size = o->wg_size;
   }
   po.variant = RG_PUSH;
-  RG_TRY(make_engine(g, &po, sizeof(uint32_t), E));
+  RG_TRY(make_engine(g, &po, sizeof(uint32_t), E, 4));
   const int64_t threshold = (o && o->threshold >= 0) ? o->threshold : g->m / 20;
   MsState* S = static_cast<MsState*>(ctx->ms);
   if (!S) {

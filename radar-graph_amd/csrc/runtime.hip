@@ -66,6 +66,12 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+void hub_meta_release(rg_graph* g, HubMeta& h) {
+  for (uint32_t* p : {h.bits, h.rank, h.ids, h.slot_of, h.filter})
+    if (p && !borrowed(g, p)) (void)hipFree(p);
+  h = HubMeta{};
+}
+
 void GraphCache::release() {
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
@@ -321,7 +327,6 @@ int rg_graph_transpose(rg_graph* g) {
   std::swap(g->off, g->in_off);
   std::swap(g->nbr, g->in_nbr);
   g->transposed = !g->transposed;
-  g->hub_k_req = -1;   // hub metadata follows the in-degrees of the orientation
   return RG_OK;
 }
 
@@ -336,8 +341,8 @@ int rg_graph_free(rg_graph* g) {
     if (g->off) (void)hipFree(g->off);
     if (g->nbr) (void)hipFree(g->nbr);
   }
-  for (uint32_t* p : {g->hub_bits, g->hub_rank, g->hub_ids, g->hub_slot_of, g->first_in})
-    if (p && !rg::borrowed(g, p)) (void)hipFree(p);
+  for (rg::HubMeta& h : g->hubs) rg::hub_meta_release(g, h);
+  if (g->first_in && !rg::borrowed(g, g->first_in)) (void)hipFree(g->first_in);
   if (g->heavy_items && !rg::borrowed(g, g->heavy_items)) (void)hipFree(g->heavy_items);
   for (void* p : {g->blocked, g->blocked_ms, g->blocked_ms_t})
     if (p && !rg::borrowed(g, p)) rg::blocked_free(p);

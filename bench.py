@@ -339,10 +339,11 @@ def serve_bench(rg, args, resident=None):
 def profile_run(rg, args):
     """One app on one engine, repeated, for counter collection under
     rocprofv3: --profile app:variant:scale[d|s][:key=value,...] (app bfs,
+    query (BFS summary only), batch (--batch sources, degree-sorted graph),
     pagerank, radii or bc; variant push, hubdup, radar, blocked or pull; d =
-    directed, s = symmetric; radar and blocked run on the degree-sorted
-    graph; options are rg.opts keywords, e.g. hubs=1073741824,
-    hub_min_arcs=-1)."""
+    directed, s = symmetric; radar, blocked, batch and query run on the
+    degree-sorted graph; graphs=0 turns HIP-graph replay off; other options
+    are rg.opts keywords, e.g. hubs=1073741824, hub_min_arcs=-1)."""
     parts = args.profile.split(":")
     app, variant, gspec = parts[0], parts[1], parts[2]
     kw = {}
@@ -351,21 +352,28 @@ def profile_run(rg, args):
             k, v = item.split("=")
             kw[k] = int(v)
     scale, sym = int(gspec[:-1]), gspec[-1] == "s"
+    graphs_on = kw.pop("graphs", 1)   # graphs=0: launch every kernel (no HIP-graph replay)
     hg = rg.HostGraph.generate(scale, 16, symmetric=sym)
     ctx = rg.Context(0)
+    ctx.graphs(bool(graphs_on))
     dg = rg.DeviceGraph(ctx, hg)
     ids = None
-    if variant in ("radar", "blocked"):
-        g, ids = dg.degree_sort()
+    if variant in ("radar", "blocked") or app in ("batch", "query"):
+        g, ids = dg.degree_sort(sort_neighbors=True)
     else:
         g = dg
     o = rg.opts(variant, **kw)
-    srcs = hg.pick_sources(args.profile_reps)
+    srcs = hg.pick_sources(max(args.profile_reps, args.batch))
     if ids is not None:
         srcs = ids[srcs]
+
     def run(i):
         if app == "bfs":
             return g.bfs(int(srcs[i]), o, want_parents=False).ms
+        if app == "query":
+            return g.bfs_query(int(srcs[i]), o)[1]
+        if app == "batch":   # --batch sources as one bit-parallel batch (the variant is unused)
+            return g.bfs_batch(srcs[:args.batch])[2]
         if app == "pagerank":
             return g.pagerank(args.iters, o, want_ranks=False).ms
         if app == "radii":
@@ -497,7 +505,14 @@ def main():
         ms = median_pr(dg, it, rg.opts("push"), reps=1)
         pr["push"] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2)}
         pr["radar_vs_push"] = round(pr["radar_tuned"]["gteps"] / pr["push"]["gteps"], 2)
-        log(f"pagerank push {ms:.1f} ms {pr['push']['gteps']} GTEPS; RADAR {pr['radar_vs_push']}x")
+        # RADAR in the reference's own form: the degree-sorted graph with LDS
+        # copies of hubs 0..k-1 merged by atomics (RD PageRank.C:42-70), beside
+        # the blocked engine the autotuner picks.
+        ms = median_pr(sg, it, rg.opts("radar", hubs=args.hubs), reps=1)
+        pr["radar_atomic"] = {"ms": round(ms, 3), "gteps": round(it * m / ms / 1e6, 2), "engine": "radar",
+                              "vs_push": round(pr["push"]["ms"] / ms, 2)}
+        log(f"pagerank push {pr['push']['ms']:.1f} ms {pr['push']['gteps']} GTEPS; tuned {pr['radar_vs_push']}x; "
+            f"atomic RADAR {pr['radar_atomic']['gteps']} GTEPS ({pr['radar_atomic']['vs_push']}x)")
         # PageRankDelta, 20 rounds: push-only on the original ids vs the
         # blocked engine on the degree-sorted graph (dense rounds blocked,
         # sparse rounds pushed from the active list).
@@ -518,8 +533,15 @@ def main():
         bfs["radar_tuned"].update({"engine": ENGINES[bt.variant], "threshold": bt.threshold,
                                    "heavy_degree": bt.heavy_degree})
         bfs["radar_vs_push"] = round(bfs["radar_tuned"]["gteps"] / bfs["push"]["gteps"], 2)
+        # RADAR in the reference's own form (RD BFS.C:43-94): degree-sorted
+        # ids, LDS copies of hubs 0..k-1, dense levels on the blocked layout;
+        # and the push-pull engine alone on the same graph.
+        bfs["radar"] = bfs_gteps(sg, new_ids[srcs], rg.opts("radar", hubs=args.hubs))
+        bfs["radar"]["vs_push"] = round(bfs["radar"]["gteps"] / bfs["push"]["gteps"], 2)
+        bfs["pull_degree_sorted"] = bfs_gteps(sg, new_ids[srcs], rg.opts("pull"))
         log(f"bfs push {bfs['push']['gteps']} GTEPS, radar_tuned ({ENGINES[bt.variant]}) "
-            f"{bfs['radar_tuned']['gteps']} GTEPS ({bfs['radar_vs_push']}x)")
+            f"{bfs['radar_tuned']['gteps']} GTEPS ({bfs['radar_vs_push']}x), RADAR (hub copies) "
+            f"{bfs['radar']['gteps']} GTEPS ({bfs['radar']['vs_push']}x), pull {bfs['pull_degree_sorted']['gteps']}")
         # Batched BFS: 256 sources as one bit-parallel batch vs the same
         # queries one at a time on the best single-source engine.
         bsrc = new_ids[hg.pick_sources(args.batch)]

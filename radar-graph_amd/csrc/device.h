@@ -86,6 +86,7 @@ struct rg_ctx {
   // not used while per-launch timing is on).
   bool graphs = true;
   rg::GraphCache pr_graph;   // blocked PageRank's maxiters iterations
+  rg::GraphCache ms_graph[3];   // a chunk of bit-parallel batch levels, per word width (1, 2, 4)
   // Device array holding the last PageRank ranks / BC dependencies computed
   // on this context (n doubles; valid until the next call on the context).
   const double* last_result = nullptr;

@@ -67,6 +67,11 @@ struct MsCtl {
   int64_t heavy_count;    // engine heavy queue
   int64_t dense_levels, sparse_levels;
   int64_t unit_next;      // dense level: next unit to take (dynamic work distribution)
+  // Read by the level kernels on the device, so that a captured chunk of
+  // levels is the same for every batch of one word width: the batch's
+  // source count and the stamp tag of the current level.
+  int64_t nsrc;
+  uint32_t tag;
 };
 
 typedef unsigned long long ull;
@@ -119,7 +124,7 @@ struct MsPushOp {
   const ull* seen;
   ull* next;
   uint32_t* stamp;
-  uint32_t tag;
+  const uint32_t* tag;   // the level's stamp tag (MsCtl::tag)
   const int64_t* mode;
   __device__ bool skip() const { return *mode != MS_SPARSE; }
   __device__ SrcT load_src(uint32_t s) const {
@@ -141,8 +146,9 @@ struct MsPushOp {
         any = true;
       }
     }
-    if (!any || stamp[d] == tag) return false;
-    return atomicExch(stamp + d, tag) != tag;
+    const uint32_t t = *tag;
+    if (!any || stamp[d] == t) return false;
+    return atomicExch(stamp + d, t) != t;
   }
   __device__ uint32_t hub_identity() const { return 0; }
   __device__ void hub_update(uint32_t*, uint32_t, SrcT, uint32_t) const {}
@@ -174,7 +180,8 @@ __device__ __forceinline__ void stats_init(MsStats<W>& st) {
 template <int W, int THREADS>
 __device__ __forceinline__ void stats_flush(const MsArgs& a, MsStats<W>& st) {
   __syncthreads();
-  for (int i = threadIdx.x; i < a.nsrc; i += THREADS) {
+  const int nsrc = (int)a.ctl->nsrc;
+  for (int i = threadIdx.x; i < nsrc; i += THREADS) {
     if (st.cnt[i]) {
       atomicAdd(a.lvl_new + i, (ull)st.cnt[i]);
       atomicAdd(a.edges + i, st.deg[i]);
@@ -253,7 +260,7 @@ template <int W>
 __device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
                                             int64_t d, bool valid, const ull* accw, uint32_t level) {
   ull allbits[W];
-  batch_bits<W>(a.nsrc, allbits);
+  batch_bits<W>((int)a.ctl->nsrc, allbits);
   ull nn[W];
   bool any = false;
 #pragma unroll
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
   if (a.ctl->mode != MS_DENSE) return;
   const uint32_t level = (uint32_t)a.ctl->level + 1;
   ull allbits[W];
-  batch_bits<W>(a.nsrc, allbits);
+  batch_bits<W>((int)a.ctl->nsrc, allbits);
   EdgeArgs ea{};
   ea.off = a.off;
   ea.out_ids = a.list_nxt;
@@ -458,7 +465,8 @@ __global__ void k_ms_level_end(MsArgs a, ull* reached, int64_t* depth) {
   MsCtl* c = a.ctl;
   if (c->mode == MS_DONE) return;
   const int64_t level = c->level + 1;
-  for (int s = threadIdx.x; s < a.nsrc; s += blockDim.x) {
+  const int nsrc = (int)c->nsrc;
+  for (int s = threadIdx.x; s < nsrc; s += blockDim.x) {
     const ull k = a.lvl_new[s];
     if (k) {
       reached[s] += k;
@@ -477,6 +485,7 @@ __global__ void k_ms_level_end(MsArgs a, ull* reached, int64_t* depth) {
     c->ndeg = 0;
     c->heavy_count = 0;
     c->unit_next = 0;
+    c->tag++;
     c->mode = c->fcount == 0 ? MS_DONE : (c->fcount + c->fdeg > c->threshold ? MS_DENSE : MS_SPARSE);
   }
 }
@@ -484,7 +493,7 @@ __global__ void k_ms_level_end(MsArgs a, ull* reached, int64_t* depth) {
 // Batch start: source bits, per-source statistics, the first frontier
 // (unique source vertices, given in list_cur) and the first level's kind.
 __global__ void k_ms_init(MsArgs a, int W, const uint32_t* sources, int64_t nuniq, int64_t threshold,
-                          ull* reached, int64_t* depth) {
+                          ull* reached, int64_t* depth, uint32_t tag) {
   __shared__ ull degsum;
   if (threadIdx.x == 0) degsum = 0;
   __syncthreads();
@@ -519,6 +528,8 @@ __global__ void k_ms_init(MsArgs a, int W, const uint32_t* sources, int64_t nuni
     c->heavy_count = 0;
     c->unit_next = 0;
     c->dense_levels = c->sparse_levels = 0;
+    c->nsrc = a.nsrc;
+    c->tag = tag;
     c->mode = c->fcount + c->fdeg > threshold ? MS_DENSE : MS_SPARSE;
   }
 }
@@ -635,7 +646,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   bind(cur);
   S.dirty = true;   // until the batch completes
   hipLaunchKernelGGL(k_ms_init, 1, 256, 0, st, a, W, S.srcs.as<uint32_t>(), (int64_t)uniq.size(), threshold,
-                     reached, depth);
+                     reached, depth, S.tag + 1u);
   RG_HIP(hipGetLastError());
 
   EdgeArgs ea{};
@@ -656,11 +667,14 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   const int gclr = grid_for(ctx, std::max<int64_t>(nwords, 1));
   constexpr int kMsThreads = ms_threads<W>();
   constexpr int PER = (1 << kMsVB) / kMsThreads;
-  int64_t steps = 0;
-  for (;;) {
+  // One chunk of kMsLevelsPerSync levels (an even number: the chunk starts
+  // and ends on the same buffer parity, and its kernels read the level, the
+  // stamp tag and the source count from MsCtl, so every chunk of a batch,
+  // and of every batch of this word width on these buffers, is the same
+  // launch sequence).
+  auto chunk = [&]() -> int {
     for (int k = 0; k < kMsLevelsPerSync; ++k) {
-      ++steps;
-      MsPushOp<W> op{a.front_cur, a.seen, a.next, S.stamp.as<uint32_t>(), S.tag + (uint32_t)steps, &a.ctl->mode};
+      MsPushOp<W> op{a.front_cur, a.seen, a.next, S.stamp.as<uint32_t>(), &a.ctl->tag, &a.ctl->mode};
       ea.ids = a.list_cur;
       ea.out_ids = a.list_nxt;
       RG_TRY((launch_hub<MsPushOp<W>, SRC_LIST>(E, op, ea)));
@@ -679,6 +693,45 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
       cur ^= 1;
       bind(cur);
     }
+    return RG_OK;
+  };
+  // Without a level hook and with per-launch timing off, the chunk is
+  // captured once as a HIP graph (per word width, while the buffers, layout
+  // and engine stay the same) and replayed.
+  const bool captured = !hook && ctx->graphs && !ctx->timing;
+  GraphCache& gc = ctx->ms_graph[W == 1 ? 0 : W == 2 ? 1 : 2];
+  if (captured) {
+    const std::vector<uint64_t> key = {
+        (uint64_t)(uintptr_t)g->off, (uint64_t)(uintptr_t)g->nbr, (uint64_t)(uintptr_t)B->src,
+        (uint64_t)(uintptr_t)B->vl, (uint64_t)(uintptr_t)B->unit_lo, (uint64_t)(uintptr_t)B->unit_block,
+        (uint64_t)(uintptr_t)B->block_unit0, (uint64_t)(uintptr_t)B->multi, (uint64_t)B->nunits,
+        (uint64_t)B->nmulti, (uint64_t)n, (uint64_t)(uintptr_t)a.ctl, (uint64_t)(uintptr_t)a.seen,
+        (uint64_t)(uintptr_t)a.next, (uint64_t)(uintptr_t)a.part, (uint64_t)(uintptr_t)stats,
+        (uint64_t)(uintptr_t)S.front[0].p, (uint64_t)(uintptr_t)S.front[1].p, (uint64_t)(uintptr_t)S.active[0].p,
+        (uint64_t)(uintptr_t)S.active[1].p, (uint64_t)(uintptr_t)S.list[0].p, (uint64_t)(uintptr_t)S.list[1].p,
+        (uint64_t)(uintptr_t)S.stamp.p, (uint64_t)(uintptr_t)E.heavy, (uint64_t)E.heavy_deg, (uint64_t)E.block,
+        (uint64_t)(uintptr_t)levels_dev, (uint64_t)(uintptr_t)radii_dev};
+    if (gc.key != key || !gc.exec) {
+      gc.release();
+      RG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const int rc = chunk();
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(st, &graph);
+      if (rc != RG_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      RG_HIP(ec);
+      gc.graph = graph;
+      RG_HIP(hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0));
+      gc.key = key;
+    }
+  }
+  int64_t steps = 0;
+  for (;;) {
+    if (captured) RG_HIP(hipGraphLaunch(gc.exec, st));
+    else RG_TRY(chunk());
+    steps += kMsLevelsPerSync;
     int64_t mode = 0;
     RG_TRY(ctx_read_count(ctx, &a.ctl->mode, &mode, 1));
     if (mode == MS_DONE) break;

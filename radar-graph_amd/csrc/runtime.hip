@@ -173,6 +173,7 @@ int rg_ctx_destroy(rg_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->pr_graph.release();
+  for (auto& gcache : ctx->ms_graph) gcache.release();
   ctx->counts.release();
   for (auto& b : ctx->scratch) b.release();
   ctx->fr_ids[0].release();

@@ -82,3 +82,36 @@ def test_pagerank_converged_loop_replay(rg, orc, ctx):
             assert r.iters == it and rel(r.ranks, ref) < FP_TOL
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("gname,k", [("rmat14s", 64), ("rmat12d", 100), ("rmat14s", 256), ("rmat12d", 1)])
+def test_batch_level_chunk_replays_match_eager(rg, orc, graphs, gname, k):
+    # The bit-parallel batch's chunk of levels is captured once per word
+    # width and replayed: batches of other sources, other eager work and
+    # synchronisations in between, every replayed batch equal bit-for-bit to
+    # the same batch launched kernel by kernel, and to the oracle.
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    c = rg.Context(0)
+    try:
+        dg = rg.DeviceGraph(c, hg)
+        srcs = hg.pick_sources(k + 8)
+        sets = [srcs[:k], srcs[8:8 + k], srcs[:k]]
+        c.graphs(False)
+        eager = [dg.bfs_batch(s, want_levels=True)[:2] for s in sets]
+        c.graphs(True)
+        for rep in range(6):
+            i = rep % 3
+            print("batch replay", gname, k, rep, flush=True)
+            out, lv, _ = dg.bfs_batch(sets[i], want_levels=rep % 2 == 0)
+            assert [x.astuple() for x in out] == [x.astuple() for x in eager[i][0]], rep
+            if lv is not None:
+                assert np.array_equal(lv, eager[i][1]), rep
+            dg.bfs(int(srcs[rep]))              # eager work on the same context
+            dg.radii(rg.radii_starts(dg.n), rg.opts("blocked"))   # a batch of another width
+            c.sync()
+        for j in (0, len(sets[0]) - 1):
+            ref, _ = orc.bfs_levels(off, nbr, int(sets[0][j]))
+            assert np.array_equal(eager[0][1][j], ref)
+    finally:
+        c.close()

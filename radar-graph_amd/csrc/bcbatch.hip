@@ -430,6 +430,15 @@ struct BcbState {
 
 void bcb_state_free(void* p) { delete static_cast<BcbState*>(p); }
 
+int64_t bcb_state_bytes(const void* p) {
+  const BcbState* S = static_cast<const BcbState*>(p);
+  if (!S) return 0;
+  int64_t b = 0;
+  for (const DevBuf* d : {&S->sig, &S->lw, &S->items, &S->hv, &S->acc_h, &S->list, &S->ctl, &S->stats, &S->srcs})
+    b += (int64_t)d->bytes;
+  return b;
+}
+
 namespace {
 
 int64_t heavy_cap(int64_t m) { return m / (int64_t)kBcbHeavy + 1; }

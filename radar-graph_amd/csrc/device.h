@@ -200,8 +200,17 @@ hipEvent_t* timing_begin(rg_ctx* ctx);
 // Cached blocked layout of a graph's in-edges (blocked.hip).
 void blocked_free(void* p);
 
-// Multi-source BFS state of a context (msbfs.hip).
+// Multi-source BFS state of a context (msbfs.hip) and batched BC state
+// (bcbatch.hip): free, and the device bytes they hold.
 void ms_state_free(void* p);
 void bcb_state_free(void* p);
+int64_t ms_state_bytes(const void* p);
+int64_t bcb_state_bytes(const void* p);
+
+// Device bytes a context holds in its pools and query states.
+int64_t ctx_device_bytes(const rg_ctx* ctx);
+// Frees the context's bit-parallel batch states (BFS and BC) and the
+// captured graphs that use them; they are rebuilt on the next batch.
+void ctx_release_batch_state(rg_ctx* ctx);
 
 }  // namespace rg

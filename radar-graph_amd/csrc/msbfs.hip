@@ -126,7 +126,11 @@ struct MsPushOp {
   uint32_t* stamp;
   const uint32_t* tag;   // the level's stamp tag (MsCtl::tag)
   const int64_t* mode;
-  __device__ bool skip() const { return *mode != MS_SPARSE; }
+  mutable uint32_t t = 0;   // *tag, read once per kernel by skip(), which every edge kernel calls first
+  __device__ bool skip() const {
+    t = *tag;
+    return *mode != MS_SPARSE;
+  }
   __device__ SrcT load_src(uint32_t s) const {
     SrcT r;
 #pragma unroll
@@ -146,7 +150,6 @@ struct MsPushOp {
         any = true;
       }
     }
-    const uint32_t t = *tag;
     if (!any || stamp[d] == t) return false;
     return atomicExch(stamp + d, t) != t;
   }
@@ -258,9 +261,8 @@ __device__ __forceinline__ void batch_bits(int nsrc, ull* allbits) {
 
 template <int W>
 __device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
-                                            int64_t d, bool valid, const ull* accw, uint32_t level) {
-  ull allbits[W];
-  batch_bits<W>((int)a.ctl->nsrc, allbits);
+                                            int64_t d, bool valid, const ull* accw, uint32_t level,
+                                            const ull* allbits) {
   ull nn[W];
   bool any = false;
 #pragma unroll
@@ -396,7 +398,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
     for (int l0 = 0; l0 < BV; l0 += THREADS) {
       const int l = l0 + threadIdx.x;
       const int64_t d = base + l;
-      ms_finalize<W>(a, st, q, ea, d, d < a.n, &acc[l * W], level);
+      ms_finalize<W>(a, st, q, ea, d, d < a.n, &acc[l * W], level, allbits);
     }
   }
   wq_finish(q, ea);
@@ -417,6 +419,8 @@ __global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
   const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
   const int64_t d = ((int64_t)b << VB) + l;
   const uint32_t level = (uint32_t)a.ctl->level + 1;
+  ull allbits[W];
+  batch_bits<W>((int)a.ctl->nsrc, allbits);
   stats_init<W, THREADS>(st);
   __syncthreads();
   ull accw[W];
@@ -432,7 +436,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
   ea.out_ids = a.list_nxt;
   ea.out_info = &a.ctl->ncount;
   WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
-  ms_finalize<W>(a, st, q, ea, d, d < a.n, accw, level);
+  ms_finalize<W>(a, st, q, ea, d, d < a.n, accw, level, allbits);
   wq_finish(q, ea);
   stats_flush<W, THREADS>(a, st);
 }
@@ -551,6 +555,16 @@ struct MsState {
 };
 
 void ms_state_free(void* p) { delete static_cast<MsState*>(p); }
+
+int64_t ms_state_bytes(const void* p) {
+  const MsState* S = static_cast<const MsState*>(p);
+  if (!S) return 0;
+  int64_t b = 0;
+  for (const DevBuf* d : {&S->seen, &S->front[0], &S->front[1], &S->next, &S->stamp, &S->active[0], &S->active[1],
+                          &S->list[0], &S->list[1], &S->part, &S->stats, &S->ctl, &S->srcs})
+    b += (int64_t)d->bytes;
+  return b;
+}
 
 namespace {
 
@@ -674,7 +688,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   // launch sequence).
   auto chunk = [&]() -> int {
     for (int k = 0; k < kMsLevelsPerSync; ++k) {
-      MsPushOp<W> op{a.front_cur, a.seen, a.next, S.stamp.as<uint32_t>(), &a.ctl->tag, &a.ctl->mode};
+      MsPushOp<W> op{a.front_cur, a.seen, a.next, S.stamp.as<uint32_t>(), &a.ctl->tag, &a.ctl->mode, 0u};
       ea.ids = a.list_cur;
       ea.out_ids = a.list_nxt;
       RG_TRY((launch_hub<MsPushOp<W>, SRC_LIST>(E, op, ea)));

@@ -108,6 +108,22 @@ hipEvent_t* timing_begin(rg_ctx* ctx) {
   return p.e;
 }
 
+int64_t ctx_device_bytes(const rg_ctx* ctx) {
+  int64_t b = (int64_t)ctx->counts.bytes + (int64_t)ctx->fr_ids[0].bytes + (int64_t)ctx->fr_ids[1].bytes +
+              (int64_t)ctx->fr_dense.bytes + (int64_t)ctx->bc_dense.bytes;
+  for (const DevBuf& s : ctx->scratch) b += (int64_t)s.bytes;
+  return b + ms_state_bytes(ctx->ms) + bcb_state_bytes(ctx->bcb);
+}
+
+void ctx_release_batch_state(rg_ctx* ctx) {
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& gcache : ctx->ms_graph) gcache.release();
+  ms_state_free(ctx->ms);
+  ctx->ms = nullptr;
+  bcb_state_free(ctx->bcb);
+  ctx->bcb = nullptr;
+}
+
 int ctx_copy(rg_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
   if (bytes == 0) return RG_OK;
   RG_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));

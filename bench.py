@@ -284,7 +284,8 @@ def serve_bench(rg, args, resident=None):
     """Serving over a resident degree-sorted symmetric R-MAT graph: seeded
     query traces (BFS only; BC only; 80/15/5 BFS/BC/PageRank-10) run saturated (all
     queries at t = 0) and as Poisson arrivals at 80% of the one-at-a-time
-    capacity, batched (continuous batching, 2 slots) vs one at a time.
+    capacity, batched (continuous batching, 2 slots, admission by device
+    memory, PageRank queries run alone) vs one at a time.
     resident = (host graph, degree-sorted device graph, new ids, scale) of a
     graph already on the device (the headline graph), else one of scale
     --serve-scale is built."""
@@ -312,7 +313,9 @@ def serve_bench(rg, args, resident=None):
 
     def row(st):
         return {"queries_per_s": round(st.qps, 1), "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
-                "mean_batch": round(st.mean_batch, 1), "slots": st.slots}
+                "mean_batch": round(st.mean_batch, 1), "slots": st.slots,
+                "peak_gb": round(st.peak_bytes / 1e9, 2), "hbm_budget_gb": round(st.hbm_budget / 1e9, 1),
+                "mem_waits": st.mem_waits}
 
     out = {"graph": f"rmat{scale} symmetric, degree-sorted", "n": n, "m": m}
     for name, mix, count in [("bfs", (1, 0, 0), args.serve_queries), ("bc", (0, 1, 0), bc_count),

@@ -410,8 +410,13 @@ int rg_bfs_noalloc(rg_graph* g, uint32_t src, uint32_t* parents_dev, rg_bfs_summ
  * run and batches of 1, 8, 64 and 256 sources, interpolated); otherwise the
  * oldest runs alone on the single-source engine with bfs_opts.  BC queries
  * the same way in batches of up to 64 (rg_bc_batch).  PageRank queries run
- * one per slot.  The number of slots is capped by free HBM (less hbm_reserve)
- * over a per-slot estimate.  one_at_a_time = 1 runs the same trace on one
+ * one per slot.  Admission by device memory: the slots' query state may
+ * hold free HBM less hbm_reserve (a configuration where one slot cannot hold
+ * a full-width BFS batch plus a BC or PageRank query is refused); a work
+ * item starts only when what every slot holds, plus what the items in
+ * flight and this one may add, fits -- otherwise a narrower batch, or batch
+ * state that is not in use is freed, or the item waits until memory frees
+ * up (stats: peak_bytes, mem_waits).  one_at_a_time = 1 runs the same trace on one
  * slot, one query at a time, BFS on the single-source engine
  * (rg_bfs_query with bfs_opts), BC on rg_bc with one source: the baseline.
  * Answers: BFS an rg_bfs_summary (equal to rg_bfs_query's); BC (one source,
@@ -446,12 +451,18 @@ typedef struct rg_serve_config {
                             variant RG_BLOCKED: blocked PageRank, blocked dense BC levels) */
   rg_opts bfs_opts;      /* single-source BFS queries (default variant RG_PULL:
                             bottom-up dense levels) */
+  int alone_kinds;       /* bit (1 << kind): queries of that kind run alone -- the
+                            other slots finish their items and start nothing new
+                            until it ends (bandwidth-bound queries run twice as
+                            long side by side); default 1 << RG_Q_PAGERANK */
 } rg_serve_config;
 typedef struct rg_serve_stats {
   int64_t queries, batches;
   int slots;
   double makespan_ms, qps, p50_ms, p99_ms, mean_ms, mean_batch;
   int64_t hbm_budget, slot_bytes;
+  int64_t peak_bytes;    /* most device bytes the slots' query state held (measured + admitted) */
+  int64_t mem_waits;     /* work items that waited for device memory, or ran as a narrower batch */
 } rg_serve_stats;
 void rg_serve_config_default(rg_serve_config* c);
 int rg_serve(rg_graph* g, const rg_serve_config* c, const rg_query* queries, int64_t nqueries,

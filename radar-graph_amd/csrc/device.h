@@ -209,8 +209,9 @@ int64_t bcb_state_bytes(const void* p);
 
 // Device bytes a context holds in its pools and query states.
 int64_t ctx_device_bytes(const rg_ctx* ctx);
-// Frees the context's bit-parallel batch states (BFS and BC) and the
-// captured graphs that use them; they are rebuilt on the next batch.
-void ctx_release_batch_state(rg_ctx* ctx);
+// Frees the context's bit-parallel batch states -- which & 1: the batched
+// BFS state (and the captured graphs that use it), which & 2: the batched
+// BC state; they are rebuilt on the next batch.
+void ctx_release_batch_state(rg_ctx* ctx, int which);
 
 }  // namespace rg

@@ -115,13 +115,17 @@ int64_t ctx_device_bytes(const rg_ctx* ctx) {
   return b + ms_state_bytes(ctx->ms) + bcb_state_bytes(ctx->bcb);
 }
 
-void ctx_release_batch_state(rg_ctx* ctx) {
+void ctx_release_batch_state(rg_ctx* ctx, int which) {
   (void)hipStreamSynchronize(ctx->stream);
-  for (auto& gcache : ctx->ms_graph) gcache.release();
-  ms_state_free(ctx->ms);
-  ctx->ms = nullptr;
-  bcb_state_free(ctx->bcb);
-  ctx->bcb = nullptr;
+  if (which & 1) {
+    for (auto& gcache : ctx->ms_graph) gcache.release();
+    ms_state_free(ctx->ms);
+    ctx->ms = nullptr;
+  }
+  if (which & 2) {
+    bcb_state_free(ctx->bcb);
+    ctx->bcb = nullptr;
+  }
 }
 
 int ctx_copy(rg_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {

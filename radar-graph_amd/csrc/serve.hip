@@ -19,6 +19,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -135,6 +136,51 @@ struct Shared {
   double ms_now() const {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
+  // Admission by device memory.  Each slot's context holds device memory
+  // for query state: its batched BFS state (ms), its batched BC state (bcb)
+  // and its pools (base), measured after every item it runs.  An item
+  // starts on a slot only when everything the slots hold, plus the growth
+  // the items in flight may still need, plus this item's growth beyond what
+  // its slot holds that it can reuse, stays within the budget; otherwise a
+  // narrower batch is tried, batch state the item does not use is dropped
+  // (on this slot, then on idle ones), or the item waits until an item in
+  // flight ends.
+  int64_t budget = 0;
+  int64_t n = 0, m = 0, pr_bytes = 0;
+  Slot* slots = nullptr;
+  int nslots = 0;
+  std::vector<int64_t> held_ms, held_bcb, held_base, growth;
+  std::vector<char> busy, drop;   // drop: an idle slot is asked to free its batch state
+  std::vector<int> running;       // kind of the item a busy slot runs
+  int64_t peak = 0, mem_waits = 0;
+  // Items wait for memory: frees (which synchronise the device) happen only
+  // then, never on a path every item takes.
+  int mem_short = 0;
+  int64_t held_total() const {
+    int64_t t = 0;
+    for (int i = 0; i < nslots; ++i) t += held_ms[i] + held_bcb[i] + held_base[i] + (busy[i] ? growth[i] : 0);
+    return t;
+  }
+  void measure(int i) {
+    const rg_ctx* ctx = slots[i].ctx;
+    held_ms[i] = ms_state_bytes(ctx->ms);
+    held_bcb[i] = bcb_state_bytes(ctx->bcb);
+    held_base[i] = ctx_device_bytes(ctx) - held_ms[i] - held_bcb[i];
+  }
+  // Device bytes of query state one work item needs on its context.
+  int64_t item_bytes(int kind, size_t k) const {
+    const int64_t heavy = (m / 2048 + m / 1024 + 64) * 8;
+    if (kind == RG_Q_BFS) return k > 1 ? bfs_batch_bytes(n, m, (int)k) + heavy : 14 * n + heavy;
+    if (kind == RG_Q_BC) return k > 1 ? bfs_batch_bytes(n, m, kBcBatch) + bc_batch_bytes(n, m) + heavy : n * 38 + heavy;
+    return pr_bytes;
+  }
+  // What the item may add to slot i: its bytes beyond what the slot holds
+  // and the item can use (batch items use the batch states).
+  int64_t item_growth(int i, int kind, size_t k) const {
+    int64_t reuse = held_base[i];
+    if (k > 1) reuse += held_ms[i] + (kind == RG_Q_BC ? held_bcb[i] : 0);
+    return std::max<int64_t>(0, item_bytes(kind, k) - reuse);
+  }
 };
 
 // Picks the next work item: the kind whose oldest pending query arrived
@@ -142,11 +188,17 @@ struct Shared {
 // taken_ms: the item's start on the trace's clock, read under the lock (so
 // items start in the order they are taken).  Returns false when every
 // query has been taken.
-bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64_t>& work, int64_t* batch,
-               double* taken_ms) {
+bool take_work(Shared& S, const rg_serve_config& c, int slot, int* kind, std::vector<int64_t>& work,
+               int64_t* batch, double* taken_ms) {
   std::unique_lock<std::mutex> lk(S.mu);
+  bool waited = false;
   for (;;) {
     if (S.err != RG_OK) return false;
+    if (S.drop[slot]) {   // asked while idle: free this slot's batch state
+      ctx_release_batch_state(S.slots[slot].ctx, 3);
+      S.measure(slot);
+      S.drop[slot] = 0;
+    }
     const double now = S.ms_now();
     while (S.next < S.nq && S.q[S.order[S.next]].arrival_ms <= now) {
       const int64_t i = S.order[S.next++];
@@ -162,16 +214,88 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
           ba = a;
         }
       }
+    // Queries that run alone: none starts while one runs, and one starts
+    // only when no other slot is busy.
+    bool alone_running = false, any_busy_slot = false;
+    for (int j = 0; j < S.nslots; ++j)
+      if (S.busy[j]) {
+        any_busy_slot = true;
+        alone_running = alone_running || ((c.alone_kinds >> S.running[j]) & 1);
+      }
+    if (best >= 0 && (alone_running || (((c.alone_kinds >> best) & 1) && any_busy_slot))) {
+      S.cv.wait_for(lk, std::chrono::milliseconds(2));
+      continue;
+    }
     if (best >= 0) {
       size_t take = 1;
-      // BFS and BC: every waiting query (up to the batch limit) as one batch
-      // when that answers them sooner than single-source runs would, else
-      // the oldest alone.
+      // BFS and BC: the waiting queries of the kind that arrived before the
+      // oldest waiting query of any other kind (up to the batch limit) as one
+      // batch when that answers them sooner than single-source runs would,
+      // else the oldest alone.  A batch never takes queries that arrived
+      // after an older query of another kind, so batching keeps the order of
+      // arrival across kinds (later BFS queries riding in a batch would
+      // otherwise overtake an older BC query by the batch's length).
       if ((best == RG_Q_BFS || best == RG_Q_BC) && !c.one_at_a_time) {
         const size_t cap = best == RG_Q_BFS ? (size_t)c.max_batch : (size_t)std::min(c.max_batch, kBcBatch);
-        const size_t k = std::min(S.pending[best].size(), cap);
+        double other = std::numeric_limits<double>::infinity();
+        for (int k2 = 0; k2 < 3; ++k2)
+          if (k2 != best && !S.pending[k2].empty()) other = std::min(other, S.q[S.pending[k2].front()].arrival_ms);
+        size_t k = 0;
+        for (int64_t qi : S.pending[best]) {
+          if (k >= cap || S.q[qi].arrival_ms > other) break;
+          ++k;
+        }
         if (S.batch_pays(best, k)) take = k;
       }
+      // Admission by memory: the widest batch that pays and fits, else
+      // drop unused batch state, else wait for an item in flight to end.
+      const int64_t others = S.held_total() - S.held_ms[slot] - S.held_bcb[slot] - S.held_base[slot];
+      auto fits = [&](size_t k) {
+        return others + S.held_ms[slot] + S.held_bcb[slot] + S.held_base[slot] + S.item_growth(slot, best, k) <=
+               S.budget;
+      };
+      bool narrowed = false;
+      while (take > 1 && !fits(take)) {
+        const size_t half = take / 2;
+        take = half >= 2 && S.batch_pays(best, half) ? half : 1;
+        narrowed = true;
+      }
+      if (!fits(take)) {
+        // Batch state this item does not use, on this slot first.
+        const int unused = take == 1 ? 3 : (best == RG_Q_BFS ? 2 : 0);
+        const bool mine = (unused & 1 && S.held_ms[slot]) || (unused & 2 && S.held_bcb[slot]);
+        if (mine) {
+          ctx_release_batch_state(S.slots[slot].ctx, unused);
+          S.measure(slot);
+          continue;
+        }
+        bool asked = false, any_busy = false;
+        for (int j = 0; j < S.nslots; ++j) {
+          if (j == slot) continue;
+          any_busy = any_busy || S.busy[j];
+          if (!S.busy[j] && (S.held_ms[j] || S.held_bcb[j]) && !S.drop[j]) {
+            S.drop[j] = 1;
+            asked = true;
+          }
+        }
+        if (asked || any_busy) {
+          if (!waited) {
+            S.mem_waits++;
+            S.mem_short++;
+          }
+          waited = true;
+          S.cv.notify_all();
+          S.cv.wait_for(lk, std::chrono::milliseconds(5));
+          continue;
+        }
+        // Nothing else holds or will free memory: the item runs alone.
+      }
+      if (waited) S.mem_short--;
+      if (narrowed && !waited) S.mem_waits++;
+      S.growth[slot] = S.item_growth(slot, best, take);
+      S.busy[slot] = 1;
+      S.running[slot] = best;
+      S.peak = std::max(S.peak, S.held_total());
       for (size_t i = 0; i < take; ++i) {
         work.push_back(S.pending[best].front());
         S.pending[best].pop_front();
@@ -184,7 +308,16 @@ bool take_work(Shared& S, const rg_serve_config& c, int* kind, std::vector<int64
       if (more) S.cv.notify_all();
       return true;
     }
-    if (S.next >= S.nq) return false;
+    if (S.next >= S.nq) {
+      // Every query is taken: this slot's batch state is freed if items on
+      // the other slots wait for memory.
+      if (S.mem_short) {
+        ctx_release_batch_state(S.slots[slot].ctx, 3);
+        S.measure(slot);
+        S.cv.notify_all();
+      }
+      return false;
+    }
     S.cv.wait_until(lk, S.t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
                                    std::chrono::duration<double, std::milli>(S.q[S.order[S.next]].arrival_ms)));
   }
@@ -296,6 +429,21 @@ int calibrate(Shared& S, Slot& sl, const rg_serve_config& c, const rg_query* q, 
   return RG_OK;
 }
 
+int warm_slot(Slot& sl, const rg_serve_config& c, const rg_query* q, int64_t nq) {
+  bool done[3] = {false, false, false};
+  std::vector<rg_query_result> scratch_res(1);
+  for (int64_t i = 0; i < nq; ++i) {
+    const int kind = q[i].kind;
+    if (done[kind]) continue;
+    done[kind] = true;
+    // A private copy of the query so the trace's results are untouched.
+    rg_query one = q[i];
+    const std::vector<int64_t> work = {0};
+    RG_TRY(run_work(sl, c, &one, kind, work, scratch_res.data()));
+  }
+  return RG_OK;
+}
+
 }  // namespace
 
 }  // namespace rg
@@ -315,6 +463,7 @@ void rg_serve_config_default(rg_serve_config* c) {
   c->opts.variant = RG_BLOCKED;
   rg_opts_default(&c->bfs_opts);
   c->bfs_opts.variant = RG_PULL;
+  c->alone_kinds = 1 << RG_Q_PAGERANK;
 }
 
 int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, int64_t nq,
@@ -356,9 +505,10 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
   if (any_pr && c.opts.variant != RG_BLOCKED && c.opts.variant != RG_PULL)
     RG_TRY(rg::static_heavy_list(g, c.opts.heavy_degree > 0 ? c.opts.heavy_degree : 1024));
   RG_HIP(hipDeviceSynchronize());
-  // Admission by device memory: slots = free HBM (less the reserve) over the
-  // bytes one slot can hold (a full-width BFS batch plus the larger of a BC
-  // and a PageRank query).
+  // Admission by device memory: the slots' query state may hold free HBM
+  // less the reserve; one slot must hold a full-width BFS batch plus the
+  // larger of a BC and a PageRank query, or the configuration is refused.
+  // Each item is then admitted against what the slots hold (take_work).
   size_t free_b = 0, total_b = 0;
   RG_HIP(hipMemGetInfo(&free_b, &total_b));
   const int64_t n = g->n, m = g->m;
@@ -371,10 +521,12 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
   if (budget < slot_bytes)
     return set_error(RG_ENOMEM, "rg_serve: free device memory holds no slot (" + std::to_string(budget) +
                                     " bytes free beyond the reserve, " + std::to_string(slot_bytes) + " per slot)");
-  const int nslots = (int)std::max<int64_t>(1, std::min<int64_t>(c.slots, budget / slot_bytes));
+  const int nslots = std::max(1, c.slots);
 
   std::vector<rg::Slot> slots(nslots);
   int rc = RG_OK;
+  int64_t reserved = 0;
+  std::vector<char> warm(nslots, 0);   // slots whose full state fits: reserved and warmed before the clock
   for (int i = 0; i < nslots && rc == RG_OK; ++i) {
     rc = rg_ctx_create(dev, &slots[i].ctx);
     if (rc == RG_OK) {
@@ -382,20 +534,46 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
       slots[i].view->ctx = slots[i].ctx;
       slots[i].view->view = true;
       slots[i].view->base = g;
-      // Batch state allocated before the clock starts.
-      if (!c.one_at_a_time && any_bfs) rc = rg::bfs_batch_reserve(slots[i].view, c.max_batch);
-      if (rc == RG_OK && !c.one_at_a_time && any_bc) rc = rg::bc_batch_reserve(slots[i].view);
+      // Batch state allocated before the clock starts, on the slots whose
+      // full state fits the budget (the others allocate when admitted).
+      if (reserved + slot_bytes <= budget) {
+        if (!c.one_at_a_time && any_bfs) rc = rg::bfs_batch_reserve(slots[i].view, c.max_batch);
+        if (rc == RG_OK && !c.one_at_a_time && any_bc) rc = rg::bc_batch_reserve(slots[i].view);
+        reserved += slot_bytes;
+        warm[i] = 1;
+      }
     }
   }
   if (rc == RG_OK) rc = hipDeviceSynchronize() == hipSuccess ? RG_OK : set_error(RG_EDEVICE, "rg_serve: set-up");
   rg::Shared S;
   S.q = queries;
   S.nq = nq;
+  S.budget = budget;
+  S.n = n;
+  S.m = m;
+  S.pr_bytes = pr_b;
+  S.slots = slots.data();
+  S.nslots = nslots;
+  for (auto* v : {&S.held_ms, &S.held_bcb, &S.held_base, &S.growth}) v->assign(nslots, 0);
+  S.busy.assign(nslots, 0);
+  S.drop.assign(nslots, 0);
+  S.running.assign(nslots, 0);
   S.order.resize(nq);
   std::iota(S.order.begin(), S.order.end(), 0);
   std::stable_sort(S.order.begin(), S.order.end(),
                    [&](int64_t a, int64_t b) { return queries[a].arrival_ms < queries[b].arrival_ms; });
   if (rc == RG_OK && !c.one_at_a_time) rc = rg::calibrate(S, slots[0], c, queries, nq);
+  // Every other slot whose full state fits the budget runs one query of
+  // each kind in the trace before the clock starts, so that it allocates no
+  // pools (which synchronises the device) while the trace runs; the others
+  // allocate when admitted.
+  for (int i = 1; i < nslots && rc == RG_OK; ++i)
+    if (warm[i]) rc = rg::warm_slot(slots[i], c, queries, nq);
+  if (rc == RG_OK) {
+    if (hipDeviceSynchronize() != hipSuccess) rc = set_error(RG_EDEVICE, "rg_serve: calibration");
+    for (int i = 0; i < nslots; ++i) S.measure(i);
+    S.peak = S.held_total();
+  }
   if (rc == RG_OK) {
     S.t0 = std::chrono::steady_clock::now();
     std::vector<std::thread> th;
@@ -413,9 +591,17 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
           int64_t batch = 0;
           std::vector<int64_t> work;
           double start = 0.0;
-          if (!rg::take_work(S, c, &kind, work, &batch, &start)) return;
+          if (!rg::take_work(S, c, i, &kind, work, &batch, &start)) return;
           int r = rg::run_work(slots[i], c, queries, kind, work, results);
           const double end = S.ms_now();
+          {
+            std::lock_guard<std::mutex> lk(S.mu);   // the item's memory is now measured, not estimated
+            S.busy[i] = 0;
+            S.growth[i] = 0;
+            S.measure(i);
+            S.peak = std::max(S.peak, S.held_total());
+            S.cv.notify_all();
+          }
           if (r != RG_OK) {
             std::lock_guard<std::mutex> lk(S.mu);
             if (S.err == RG_OK) {
@@ -484,6 +670,8 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
     stats->mean_batch = bfs_batches ? (double)nbfs / (double)bfs_batches : 0.0;
     stats->hbm_budget = budget;
     stats->slot_bytes = slot_bytes;
+    stats->peak_bytes = S.peak;
+    stats->mem_waits = S.mem_waits;
   }
   return RG_OK;
 }

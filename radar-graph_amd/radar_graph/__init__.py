@@ -89,14 +89,15 @@ class QueryResult(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int), ("max_batch", C.c_int), ("one_at_a_time", C.c_int),
-                ("hbm_reserve", C.c_int64), ("opts", Opts), ("bfs_opts", Opts)]
+                ("hbm_reserve", C.c_int64), ("opts", Opts), ("bfs_opts", Opts), ("alone_kinds", C.c_int)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("queries", C.c_int64), ("batches", C.c_int64), ("slots", C.c_int),
                 ("makespan_ms", C.c_double), ("qps", C.c_double), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("mean_batch", C.c_double),
-                ("hbm_budget", C.c_int64), ("slot_bytes", C.c_int64)]
+                ("hbm_budget", C.c_int64), ("slot_bytes", C.c_int64), ("peak_bytes", C.c_int64),
+                ("mem_waits", C.c_int64)]
 
     def asdict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -238,10 +239,12 @@ def hash_int(a: int) -> int:
 
 
 def serve_config(slots=2, max_batch=256, one_at_a_time=False, hbm_reserve=0, o: Opts = None,
-                 bfs: Opts = None) -> ServeConfig:
+                 bfs: Opts = None, alone_kinds=None) -> ServeConfig:
     c = ServeConfig()
     lib.rg_serve_config_default(C.byref(c))
     c.slots, c.max_batch, c.one_at_a_time = slots, max_batch, int(one_at_a_time)
+    if alone_kinds is not None:
+        c.alone_kinds = int(alone_kinds)
     if hbm_reserve:
         c.hbm_reserve = hbm_reserve
     if o is not None:

@@ -98,19 +98,32 @@ def test_one_at_a_time_same_answers(rg, ctx, graphs):
             assert x.nonfinite == y.nonfinite and close(x.wsum, y.wsum)
 
 
-def test_admission_by_memory(rg, ctx, graphs):
-    # A graph whose slots are large next to the MB-scale drift of the
-    # device's free memory between calls.
+def test_admission_by_memory(rg, orc, ctx, graphs):
+    # Items are admitted against the device memory the slots' query state
+    # holds.  With room for little more than one slot's full state (reserved
+    # on the first slot before the clock starts), the two BC batches of the
+    # trace cannot both hold a BC batch's state: one waits for memory or runs
+    # as narrower batches,
+    # the measured peak stays within the budget, and every answer is the one
+    # the query gets alone.  With room to spare nothing waits; with room for
+    # less than one slot the configuration is refused.
     hg = graphs("rmat16s")
+    off, nb = hg.csr()
     dg = rg.DeviceGraph(ctx, hg)
-    q = rg.poisson_workload(hg, 30, 0.0, seed=4)
+    q = rg.poisson_workload(hg, 100, 0.0, mix=(0.0, 1.0, 0.0), seed=4)   # two BC batches of 64 and 36
     dg.serve(q, rg.serve_config(slots=4))   # builds the layouts (their build's scratch is freed after)
-    _, st = dg.serve(q, rg.serve_config(slots=4))
+    res, st = dg.serve(q, rg.serve_config(slots=4))
     free = st.hbm_budget + rg.serve_config().hbm_reserve
-    # leave room for one slot and a half: one slot admitted
-    _, st1 = dg.serve(q, rg.serve_config(slots=4, hbm_reserve=int(free - 1.5 * st.slot_bytes)))
-    print("admission", st.slots, st.hbm_budget, st.slot_bytes, st1.slots, st1.hbm_budget, flush=True)
-    assert st.slots == 4 and st1.slots == 1
+    print("admission, room to spare:", st.asdict(), flush=True)
+    assert st.slots == 4 and st.mem_waits == 0 and st.peak_bytes <= st.hbm_budget
+    tight = rg.serve_config(slots=4, hbm_reserve=int(free - 1.15 * st.slot_bytes))
+    res1, st1 = dg.serve(q, tight)
+    print("admission, tight:", st1.asdict(), flush=True)
+    assert st1.slots == 4 and st1.mem_waits > 0
+    assert st1.peak_bytes <= st1.hbm_budget
+    for a, b in zip(res, res1):
+        if a.kind == rg.Q_BC:
+            assert a.nonfinite == b.nonfinite and close(a.wsum, b.wsum)
     with pytest.raises(rg.RGError):
         dg.serve(q, rg.serve_config(slots=4, hbm_reserve=int(free - st.slot_bytes // 2)))
 
@@ -187,5 +200,33 @@ def test_hubdup_views_leave_base_metadata(rg, orc, graphs, one_at_a_time):
         lv, _ = orc.bfs_levels(off, nb, src)
         assert orc.check_parents(off, nb, src, lv, b.parents) == 0
         dg.close()
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("alone", [0, 1 << 2])
+def test_pagerank_alone_policy(rg, graphs, alone):
+    # alone_kinds: a PageRank query never runs beside another item (the other
+    # slots drain first and start nothing until it ends); without it items
+    # overlap it.  Answers are the same either way.
+    hg = graphs("rmat14s")
+    c = rg.Context(0)
+    try:
+        dg = rg.DeviceGraph(c, hg)
+        q = rg.poisson_workload(hg, 120, 0.0, mix=(0.6, 0.2, 0.2), seed=21)
+        res, st = dg.serve(q, rg.serve_config(slots=3, alone_kinds=alone))
+        ref, _ = dg.serve(q, rg.serve_config(one_at_a_time=True))
+        items = {(r.batch_id, r.kind, r.slot, r.start_ms, r.end_ms) for r in res}
+        pr = [x for x in items if x[1] == rg.Q_PAGERANK]
+        overlap = any(o[0] != p[0] and o[3] < p[4] and p[3] < o[4] for p in pr for o in items)
+        print("alone", alone, "PageRank items", len(pr), "overlap", overlap, flush=True)
+        assert len(pr) > 0
+        if alone:
+            assert not overlap
+        for a, b in zip(res, ref):
+            if a.kind == rg.Q_BFS:
+                assert a.bfs.astuple() == b.bfs.astuple()
+            else:
+                assert a.nonfinite == b.nonfinite and close(a.wsum, b.wsum)
     finally:
         c.close()

@@ -153,7 +153,9 @@ int rg_graph_free(rg_graph* g);
  * use their degree either way).  sort_neighbors=1 is the strict PNO copy
  * (lists sorted), 0 the relaxed DS/RD copy (IO.h:725-726: original order,
  * relabelled).  new_ids (host, n entries, nullable) receives new_ids[orig].
- * The result is a new device graph; `g` is left unchanged. */
+ * The result is a new device graph; `g` is left unchanged.  Graphs of fewer
+ * than 2^31 vertices are accepted at any arc count (strict lists of graphs
+ * with 2^31 arcs or more are sorted in chunks of at most 2^30 arcs). */
 int rg_graph_degree_sort(rg_graph* g, int by_outdeg, int sort_neighbors,
                          rg_graph** out, uint32_t* new_ids);
 

@@ -16,7 +16,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <memory>
+#include <vector>
 
 #include "device.h"
 
@@ -63,47 +65,99 @@ __global__ void k_relabel_lists(const uint64_t* off, const uint32_t* nbr, const 
 
 __global__ void k_set_last(uint64_t* noff, int64_t n, uint64_t m) { noff[n] = m; }
 
+// Offsets of vertices [v0, v0 + cnt] relative to the first one (the
+// segmented sort takes int item counts: lists are sorted in chunks of fewer
+// than 2^31 arcs).
+__global__ void k_rel_offsets(const uint64_t* noff, int64_t v0, int64_t cnt, uint64_t* rel) {
+  const uint64_t base = noff[v0];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= cnt; i += (int64_t)gridDim.x * blockDim.x)
+    rel[i] = noff[v0 + i] - base;
+}
+
 int grid(rg_ctx* ctx, int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 16));
 }
 
-// Builds one relabelled CSR (offsets + lists) from (off, nbr).
-int relabel_csr(rg_ctx* ctx, const uint64_t* off, const uint32_t* nbr, int64_t n, int64_t m,
-                const uint32_t* inv, const uint32_t* new_ids, bool strict, uint64_t** noff_out,
-                uint32_t** nnbr_out) {
+// Sorts every list of (noff, nbr) into sorted (the segmented sort takes int
+// item and segment counts: graphs of 2^31 arcs or more, or any graph when
+// chunk_arcs > 0, are sorted in chunks of consecutive vertices holding at
+// most chunk_arcs arcs each, default 2^30, with offsets relative to the
+// chunk; one list longer than that is a chunk of its own).
+int sort_lists(rg_ctx* ctx, const uint64_t* noff, const uint32_t* nbr, int64_t n, int64_t m,
+               uint64_t chunk_arcs, DevBuf& tmp, uint32_t* sorted) {
   hipStream_t s = ctx->stream;
-  uint64_t* noff = nullptr;
-  uint32_t* nnbr = nullptr;
-  RG_HIP(hipMalloc(&noff, sizeof(uint64_t) * (n + 1)));
-  RG_HIP(hipMalloc(&nnbr, sizeof(uint32_t) * std::max<int64_t>(m, 1)));
-  DevBuf ndeg, tmp;
+  if (m <= (int64_t)INT32_MAX && chunk_arcs == 0) {
+    size_t sb = 0;
+    RG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, sb, nbr, sorted, (int)m, (int)n, noff, noff + 1,
+                                                      0, 32, s));
+    RG_TRY(tmp.ensure(sb));
+    RG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, sb, nbr, sorted, (int)m, (int)n, noff, noff + 1, 0,
+                                                      32, s));
+    return RG_OK;
+  }
+  std::vector<uint64_t> h(n + 1);
+  RG_HIP(hipMemcpyAsync(h.data(), noff, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, s));
+  RG_HIP(hipStreamSynchronize(s));
+  DevBuf rel;
+  const uint64_t chunk = chunk_arcs ? chunk_arcs : (uint64_t)1 << 30;
+  for (int64_t v0 = 0; v0 < n;) {
+    int64_t v1 = v0 + 1;
+    while (v1 < n && h[v1 + 1] - h[v0] <= chunk) ++v1;
+    const uint64_t items = h[v1] - h[v0];
+    if (items > (uint64_t)INT32_MAX) return set_error(RG_EUNSUPPORTED, "degree sort: a list of 2^31 arcs");
+    if (items > 1) {
+      const int64_t cnt = v1 - v0;
+      RG_TRY(rel.ensure(sizeof(uint64_t) * (cnt + 1)));
+      hipLaunchKernelGGL(k_rel_offsets, grid(ctx, cnt + 1), 256, 0, s, noff, v0, cnt, rel.as<uint64_t>());
+      const uint64_t* rb = rel.as<uint64_t>();
+      size_t sb = 0;
+      RG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, sb, nbr + h[v0], sorted + h[v0], (int)items,
+                                                        (int)cnt, rb, rb + 1, 0, 32, s));
+      RG_TRY(tmp.ensure(sb));
+      RG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, sb, nbr + h[v0], sorted + h[v0], (int)items,
+                                                        (int)cnt, rb, rb + 1, 0, 32, s));
+      // rel is rewritten by the next chunk: this chunk's sort must be done.
+      RG_HIP(hipStreamSynchronize(s));
+    } else if (items == 1) {
+      RG_HIP(hipMemcpyAsync(sorted + h[v0], nbr + h[v0], sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    }
+    v0 = v1;
+  }
+  return RG_OK;
+}
+
+// Builds one relabelled CSR (offsets + lists) from (off, nbr); on success
+// the caller owns *noff_out and *nnbr_out.
+int relabel_csr(rg_ctx* ctx, const uint64_t* off, const uint32_t* nbr, int64_t n, int64_t m,
+                const uint32_t* inv, const uint32_t* new_ids, bool strict, uint64_t chunk_arcs,
+                uint64_t** noff_out, uint32_t** nnbr_out) {
+  hipStream_t s = ctx->stream;
+  DevBuf noff, nnbr, ndeg, tmp;
+  RG_TRY(noff.ensure(sizeof(uint64_t) * (n + 1)));
+  RG_TRY(nnbr.ensure(sizeof(uint32_t) * std::max<int64_t>(m, 1)));
   RG_TRY(ndeg.ensure(sizeof(uint64_t) * n));
   hipLaunchKernelGGL(k_new_degrees, grid(ctx, n), 256, 0, s, off, inv, n, ndeg.as<uint64_t>());
   size_t tb = 0;
-  RG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ndeg.as<uint64_t>(), noff, (int)n, s));
+  RG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ndeg.as<uint64_t>(), noff.as<uint64_t>(), (int)n, s));
   RG_TRY(tmp.ensure(tb));
-  RG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, ndeg.as<uint64_t>(), noff, (int)n, s));
-  hipLaunchKernelGGL(k_set_last, 1, 1, 0, s, noff, n, (uint64_t)m);
-  hipLaunchKernelGGL(k_relabel_lists, grid(ctx, n), 256, 0, s, off, nbr, inv, new_ids, noff, n, nnbr);
+  RG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, ndeg.as<uint64_t>(), noff.as<uint64_t>(), (int)n, s));
+  hipLaunchKernelGGL(k_set_last, 1, 1, 0, s, noff.as<uint64_t>(), n, (uint64_t)m);
+  hipLaunchKernelGGL(k_relabel_lists, grid(ctx, n), 256, 0, s, off, nbr, inv, new_ids, noff.as<uint64_t>(), n,
+                     nnbr.as<uint32_t>());
   RG_HIP(hipGetLastError());
   if (strict && m > 0) {
-    uint32_t* sorted = nullptr;
-    RG_HIP(hipMalloc(&sorted, sizeof(uint32_t) * m));
-    size_t sb = 0;
-    RG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, sb, nnbr, sorted, (int)m, (int)n, noff,
-                                                      noff + 1, 0, 32, s));
-    RG_TRY(tmp.ensure(sb));
-    RG_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, sb, nnbr, sorted, (int)m, (int)n, noff,
-                                                      noff + 1, 0, 32, s));
+    DevBuf sorted;
+    RG_TRY(sorted.ensure(sizeof(uint32_t) * m));
+    RG_TRY(sort_lists(ctx, noff.as<uint64_t>(), nnbr.as<uint32_t>(), n, m, chunk_arcs, tmp, sorted.as<uint32_t>()));
     RG_HIP(hipStreamSynchronize(s));
-    (void)hipFree(nnbr);
-    nnbr = sorted;
+    std::swap(nnbr.p, sorted.p);
+    std::swap(nnbr.bytes, sorted.bytes);
   }
   RG_HIP(hipStreamSynchronize(s));
-  ndeg.release();
-  tmp.release();
-  *noff_out = noff;
-  *nnbr_out = nnbr;
+  *noff_out = noff.as<uint64_t>();
+  *nnbr_out = nnbr.as<uint32_t>();
+  noff.p = nullptr;
+  nnbr.p = nullptr;
   return RG_OK;
 }
 
@@ -115,8 +169,7 @@ using rg::set_error;
 extern "C" int rg_graph_degree_sort(rg_graph* g, int by_outdeg, int sort_neighbors, rg_graph** out,
                                     uint32_t* new_ids_host) {
   if (!g || !out) return set_error(RG_EINVAL, "rg_graph_degree_sort: null argument");
-  if (g->m > (int64_t)INT32_MAX || g->n > (int64_t)INT32_MAX)
-    return set_error(RG_EUNSUPPORTED, "rg_graph_degree_sort: more than 2^31 arcs");
+  if (g->n > (int64_t)INT32_MAX) return set_error(RG_EUNSUPPORTED, "rg_graph_degree_sort: 2^31 or more vertices");
   rg_ctx* ctx = g->ctx;
   RG_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
@@ -148,15 +201,19 @@ extern "C" int rg_graph_degree_sort(rg_graph* g, int by_outdeg, int sort_neighbo
   ng->m = m;
   ng->symmetric = g->symmetric;
   ng->degree_sorted = true;
+  // RG_DEGREE_SORT_CHUNK (arcs) forces the chunked list sort that graphs of
+  // 2^31 arcs or more take.
+  const char* chunk_env = std::getenv("RG_DEGREE_SORT_CHUNK");
+  const uint64_t chunk_arcs = chunk_env ? std::strtoull(chunk_env, nullptr, 10) : 0;
   int rc = rg::relabel_csr(ctx, g->off, g->nbr, n, m, inv.as<uint32_t>(), ids.as<uint32_t>(),
-                           sort_neighbors != 0, &ng->off, &ng->nbr);
+                           sort_neighbors != 0, chunk_arcs, &ng->off, &ng->nbr);
   if (rc == RG_OK) {
     if (g->symmetric) {
       ng->in_off = ng->off;
       ng->in_nbr = ng->nbr;
     } else {
       rc = rg::relabel_csr(ctx, g->in_off, g->in_nbr, n, m, inv.as<uint32_t>(), ids.as<uint32_t>(),
-                           sort_neighbors != 0, &ng->in_off, &ng->in_nbr);
+                           sort_neighbors != 0, chunk_arcs, &ng->in_off, &ng->in_nbr);
     }
   }
   if (rc == RG_OK && new_ids_host)

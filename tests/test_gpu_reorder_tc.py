@@ -28,6 +28,31 @@ def test_degree_sort(rg, orc, ctx, graphs, gname, by_outdeg, strict):
         assert np.array_equal(gio, nioff) and np.array_equal(gin, ninb)
 
 
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d"])
+@pytest.mark.parametrize("chunk", [1, 700, 5000])
+def test_degree_sort_chunked_lists(rg, orc, ctx, graphs, gname, chunk, monkeypatch):
+    """The chunked list sort that graphs of 2^31 arcs or more take, forced
+    on a small graph: chunks of at most `chunk` arcs (hub lists longer than
+    that are chunks of their own; chunk 1 makes every list one) give the
+    same strict CSR as the oracle."""
+    hg = graphs(gname)
+    print(f"degree_sort chunked {gname} chunk={chunk}", flush=True)
+    monkeypatch.setenv("RG_DEGREE_SORT_CHUNK", str(chunk))
+    ng, ids = rg.DeviceGraph(ctx, hg).degree_sort(by_outdeg=False, sort_neighbors=True)
+    off, nb = hg.csr()
+    key = np.diff(hg.offsets(in_edges=not hg.symmetric)).astype(np.uint32)
+    ref_ids = orc.degree_order(key)
+    assert np.array_equal(ids, ref_ids)
+    noff, nnb = orc.relabel_csr(off, nb, ref_ids, strict=True)
+    goff, gnb = ng.download()
+    assert np.array_equal(goff, noff) and np.array_equal(gnb, nnb)
+    if not hg.symmetric:
+        ioff, inb = hg.csr(in_edges=True)
+        nioff, ninb = orc.relabel_csr(ioff, inb, ref_ids, strict=True)
+        gio, gin = ng.download(in_edges=True)
+        assert np.array_equal(gio, nioff) and np.array_equal(gin, ninb)
+
+
 @pytest.mark.parametrize("gname", ["rmat10s", "rmat12s", "er12s"])
 @pytest.mark.parametrize("relabel", [False, True])
 def test_tc(rg, orc, ctx, graphs, gname, relabel):

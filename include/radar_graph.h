@@ -199,6 +199,12 @@ typedef struct rg_opts {
   int64_t hub_min_arcs; /* HUBDUP/RADAR: list frontiers with fewer arcs        */
                         /* (|F| + out-degree sum) run without hub copies; 0 =  */
                         /* 4096 per CU, < 0 = always with copies               */
+  int unit_order;       /* RG_BLOCKED: order in which units run: 0 = the split */
+                        /* blocks' units by first source, then the single-    */
+                        /* unit blocks; 1 = that order cut into per-XCD       */
+                        /* shares (a queue per XCD); 2 = the layout's order   */
+  int unit_queue;       /* RG_BLOCKED: 0 = persistent workgroups take units   */
+                        /* from a work queue; 1 = one workgroup per unit      */
 } rg_opts;
 
 /* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */

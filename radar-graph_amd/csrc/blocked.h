@@ -32,6 +32,18 @@ struct BlockedCSC {
   uint32_t* block_unit0 = nullptr; // nblocks + 1: first unit of each block
   uint32_t* unit_block = nullptr;  // nunits: block of each unit
   uint32_t* multi = nullptr;       // nmulti: the blocks with more than one unit
+  // Launch order other than the layout's (workgroup or queue position ->
+  // unit): the split blocks' units sorted by their first source -- a split
+  // hub block's units cover consecutive source ranges, so units of different
+  // hub blocks that read the same sources then run at the same time --
+  // followed by the single-unit blocks in block order.  nsplit = the split
+  // blocks' unit count; xcd_shares (device, 2 (X + 1) entries) cuts the
+  // sorted split units into X contiguous per-XCD shares [lo, hi) plus the
+  // single-unit blocks as a common share, for k_blk_pr_qx.
+  uint32_t* by_src = nullptr;      // nunits
+  int64_t nsplit = 0;
+  int64_t* xcd_shares = nullptr;
+  int nxcd = 0;
   void release();
 };
 

@@ -34,6 +34,7 @@ namespace rg {
 
 constexpr int kMaxVB = 14;                       // at most 16K destinations per block (128 KB LDS)
 constexpr int kUnitThreads = 1024;
+constexpr int kMaxXcds = 16;                     // per-XCD work queues (k_blk_pr_qx)
 
 void BlockedCSC::release() {
   if (src) (void)hipFree(src);
@@ -42,8 +43,14 @@ void BlockedCSC::release() {
   if (block_unit0) (void)hipFree(block_unit0);
   if (unit_block) (void)hipFree(unit_block);
   if (multi) (void)hipFree(multi);
+  if (by_src) (void)hipFree(by_src);
+  if (xcd_shares) (void)hipFree(xcd_shares);
   src = nullptr; vl = nullptr; unit_lo = nullptr; block_unit0 = nullptr; unit_block = nullptr;
   multi = nullptr;
+  by_src = nullptr;
+  xcd_shares = nullptr;
+  nsplit = 0;
+  nxcd = 0;
   built_for = nullptr;
   nmulti = 0;
 }
@@ -76,6 +83,13 @@ __global__ void k_blk_keys(const uint64_t* in_off, const uint32_t* in_nbr, int64
       vals[j] = s;
     }
   }
+}
+
+// First source of every unit (kNone for an empty unit).
+__global__ void k_unit_first_src(const uint32_t* src, const uint64_t* unit_lo, int64_t nunits, uint32_t* out) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nunits;
+       u += (int64_t)gridDim.x * blockDim.x)
+    out[u] = unit_lo[u] < unit_lo[u + 1] ? src[unit_lo[u]] : kNone;
 }
 
 __global__ void k_blk_split(const uint64_t* keys, int64_t m, int vb, int bs, uint16_t* vl, uint64_t* block_lo) {
@@ -155,27 +169,22 @@ __device__ __forceinline__ double wg_sum(double x) {
   return t;
 }
 
-// One whole PageRank iteration on the blocked layout: each unit sums its
-// in-edges into an LDS block; a unit that holds its block alone then applies
-// the vertex step to the block's vertices itself (new rank in place, L1
-// term, next contribution into the other contribution buffer); units of
+// One unit of a PageRank iteration on the blocked layout: the unit sums its
+// in-edges into the LDS block acc; a unit that holds its block alone then
+// applies the vertex step to the block's vertices itself (new rank in place,
+// L1 term, next contribution into the other contribution buffer); units of
 // split blocks write partials, which k_blk_pr_merge adds in unit order.
-// diff_part[u] = the unit's L1 terms (0 for split-block units).
+// diff_part[u] = the unit's L1 terms (0 for split-block units).  Every
+// thread of the workgroup calls it; acc is free again when it returns.
 template <int VB, int THREADS, bool NT, int UNROLL>
-__global__ __launch_bounds__(THREADS) void k_blk_pr_it(const uint32_t* __restrict__ src,
-                                                       const uint16_t* __restrict__ vl,
-                                                       const uint64_t* __restrict__ unit_lo,
-                                                       const uint32_t* __restrict__ unit_block,
-                                                       const uint32_t* __restrict__ block_unit0,
-                                                       const double* __restrict__ contrib,
-                                                       double* __restrict__ contrib_next, double* rank,
-                                                       const uint64_t* __restrict__ off, int64_t n,
-                                                       double* __restrict__ part, double* diff_part,
-                                                       const int64_t* done, double damping, double added) {
+__device__ __forceinline__ void pr_unit(int64_t u, double* acc, const uint32_t* __restrict__ src,
+                                        const uint16_t* __restrict__ vl, const uint64_t* __restrict__ unit_lo,
+                                        const uint32_t* __restrict__ unit_block,
+                                        const uint32_t* __restrict__ block_unit0, const double* __restrict__ contrib,
+                                        double* __restrict__ contrib_next, double* rank,
+                                        const uint64_t* __restrict__ off, int64_t n, double* __restrict__ part,
+                                        double* diff_part, double damping, double added) {
   constexpr int BV = 1 << VB;
-  __shared__ double acc[BV];
-  if (*done) return;
-  const int64_t u = blockIdx.x;
   for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
   __syncthreads();
   blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
@@ -185,6 +194,7 @@ __global__ __launch_bounds__(THREADS) void k_blk_pr_it(const uint32_t* __restric
     double* out = part + u * (int64_t)BV;
     for (int i = threadIdx.x; i < BV; i += THREADS) out[i] = acc[i];
     if (threadIdx.x == 0) diff_part[u] = 0.0;
+    __syncthreads();
     return;
   }
   // The vertex step, four vertices per thread at a time: their loads are
@@ -221,6 +231,88 @@ __global__ __launch_bounds__(THREADS) void k_blk_pr_it(const uint32_t* __restric
   }
   diff = wg_sum<THREADS>(diff);
   if (threadIdx.x == 0) diff_part[u] = diff;
+  __syncthreads();
+}
+
+// One PageRank iteration, one workgroup per unit (order: workgroup -> unit,
+// nullptr = the layout's order).
+template <int VB, int THREADS, bool NT, int UNROLL>
+__global__ __launch_bounds__(THREADS) void k_blk_pr_it(const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ src,
+                                                       const uint16_t* __restrict__ vl,
+                                                       const uint64_t* __restrict__ unit_lo,
+                                                       const uint32_t* __restrict__ unit_block,
+                                                       const uint32_t* __restrict__ block_unit0,
+                                                       const double* __restrict__ contrib,
+                                                       double* __restrict__ contrib_next, double* rank,
+                                                       const uint64_t* __restrict__ off, int64_t n,
+                                                       double* __restrict__ part, double* diff_part,
+                                                       const int64_t* done, double damping, double added,
+                                                       const int64_t* shares, int nxcd, unsigned long long* heads) {
+  __shared__ double acc[1 << VB];
+  if (*done) return;
+  const int64_t u = order ? order[blockIdx.x] : blockIdx.x;
+  pr_unit<VB, THREADS, NT, UNROLL>(u, acc, src, vl, unit_lo, unit_block, block_unit0, contrib, contrib_next, rank,
+                                   off, n, part, diff_part, damping, added);
+}
+
+// XCD of the calling wave (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ int xcc_id() { return (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 15); }
+
+// Takes the next unit for the calling workgroup (thread 0 only) from the
+// per-XCD shares of `order` (nullptr: the layout's order): share x =
+// order[shares[2x], shares[2x + 1]) for x < X, share X the common tail; heads[x] counts what share x has handed
+// out.  A workgroup drains its own XCD's share, then the other XCDs' in
+// turn, then the common share; *cur is the share it takes from (starts at
+// its XCD), *tried the XCD shares found empty.  Returns -1 when all are
+// empty.
+__device__ __forceinline__ int64_t take_unit(const uint32_t* __restrict__ order, const int64_t* __restrict__ shares,
+                                             unsigned long long* heads, int X, int* cur, int* tried) {
+  for (;;) {
+    const int c = *cur;
+    const int64_t q = shares[2 * c] + (int64_t)atomicAdd(&heads[c], 1ull);
+    if (q < shares[2 * c + 1]) return order ? (int64_t)order[q] : q;
+    if (c == X) return -1;
+    *tried += 1;
+    *cur = *tried < X ? (c + 1) % X : X;
+  }
+}
+
+// One PageRank iteration on persistent workgroups (as many as fit on the
+// chip at once) taking units from a work queue, so no CU waits for a
+// workgroup launch and no XCD idles while another still holds units (with
+// one workgroup per unit, XCDs took the same number of units but finished
+// up to 800 us apart in a 6.1 ms iteration).  By default the queue hands
+// out the split blocks' units by first source, so units of different hub
+// blocks reading the same source range run at the same time, then the
+// single-unit blocks.  With per-XCD shares, that sequence is cut into X
+// contiguous shares, one per XCD, and a workgroup whose XCD's share is
+// empty takes from the others'.  k_pr_finish_b clears the queue heads.
+template <int VB, int THREADS, bool NT, int UNROLL>
+__global__ __launch_bounds__(THREADS) void k_blk_pr_qx(const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ src,
+                                                       const uint16_t* __restrict__ vl,
+                                                       const uint64_t* __restrict__ unit_lo,
+                                                       const uint32_t* __restrict__ unit_block,
+                                                       const uint32_t* __restrict__ block_unit0,
+                                                       const double* __restrict__ contrib,
+                                                       double* __restrict__ contrib_next, double* rank,
+                                                       const uint64_t* __restrict__ off, int64_t n,
+                                                       double* __restrict__ part, double* diff_part,
+                                                       const int64_t* done, double damping, double added,
+                                                       const int64_t* shares, int nxcd, unsigned long long* heads) {
+  __shared__ double acc[1 << VB];
+  __shared__ long long s_u;
+  if (*done) return;
+  int cur = xcc_id() % nxcd, tried = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_u = take_unit(order, shares, heads, nxcd, &cur, &tried);
+    __syncthreads();
+    const int64_t u = (int64_t)s_u;
+    if (u < 0) return;
+    pr_unit<VB, THREADS, NT, UNROLL>(u, acc, src, vl, unit_lo, unit_block, block_unit0, contrib, contrib_next,
+                                     rank, off, n, part, diff_part, damping, added);
+  }
 }
 
 // Split blocks: a vertex's sum is its units' partials (hub blocks hold up to
@@ -346,32 +438,26 @@ __device__ __forceinline__ void wg_finish(double l1, uint64_t deg, double* l1_ou
   }
 }
 
-// One dense PageRankDelta round on the blocked layout, edge phase and
-// vertexFilter fused as in k_blk_pr_it: a unit that holds its block alone
+// One unit of a dense PageRankDelta round on the blocked layout, edge phase
+// and vertexFilter fused as in pr_unit: a unit that holds its block alone
 // applies the vertex step to the block (next quotients into contrib_next,
 // active vertices appended to out_ids, their count added to *count); units
 // of split blocks write partials for k_blk_prd_merge.  l1_part[u] and
 // deg_part[u] = the unit's |Delta| sum and its active vertices' out-degree
-// sum (0 for split-block units).
+// sum (0 for split-block units).  Every thread of the workgroup calls it;
+// acc, s_cnt and s_base are free again when it returns.
 template <int VB, int THREADS, bool NT, int UNROLL>
-__global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restrict__ src,
-                                                        const uint16_t* __restrict__ vl,
-                                                        const uint64_t* __restrict__ unit_lo,
-                                                        const uint32_t* __restrict__ unit_block,
-                                                        const uint32_t* __restrict__ block_unit0,
-                                                        const double* __restrict__ contrib,
-                                                        double* __restrict__ contrib_next, double* p,
-                                                        const uint64_t* __restrict__ off, int64_t n,
-                                                        double* __restrict__ part, double* l1_part,
-                                                        uint64_t* deg_part, uint32_t* __restrict__ out_ids,
-                                                        int64_t* count, const int64_t* done, PrdStep st) {
+__device__ __forceinline__ void prd_unit(int64_t u, double* acc, uint32_t* s_cnt, unsigned long long* s_base,
+                                         const uint32_t* __restrict__ src, const uint16_t* __restrict__ vl,
+                                         const uint64_t* __restrict__ unit_lo,
+                                         const uint32_t* __restrict__ unit_block,
+                                         const uint32_t* __restrict__ block_unit0,
+                                         const double* __restrict__ contrib, double* __restrict__ contrib_next,
+                                         double* p, const uint64_t* __restrict__ off, int64_t n,
+                                         double* __restrict__ part, double* l1_part, uint64_t* deg_part,
+                                         uint32_t* __restrict__ out_ids, int64_t* count, const PrdStep& st) {
   constexpr int BV = 1 << VB;
   constexpr int FU = 4;
-  __shared__ double acc[BV];
-  __shared__ uint32_t s_cnt[THREADS / kWave];
-  __shared__ unsigned long long s_base;
-  if (*done) return;
-  const int64_t u = blockIdx.x;
   for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
   __syncthreads();
   blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
@@ -384,6 +470,7 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restri
       l1_part[u] = 0.0;
       deg_part[u] = 0;
     }
+    __syncthreads();
     return;
   }
   const int64_t base = (int64_t)b << VB;
@@ -416,9 +503,68 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restri
         if (act[k]) dsum += deg;
       }
     }
-    wg_append<THREADS, FU>(act, vs, out_ids, count, s_cnt, &s_base);
+    wg_append<THREADS, FU>(act, vs, out_ids, count, s_cnt, s_base);
   }
   wg_finish<THREADS>(l1, dsum, &l1_part[u], &deg_part[u]);
+  __syncthreads();
+}
+
+// One dense PageRankDelta round, one workgroup per unit (order: workgroup ->
+// unit, nullptr = the layout's order).
+template <int VB, int THREADS, bool NT, int UNROLL>
+__global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ src,
+                                                        const uint16_t* __restrict__ vl,
+                                                        const uint64_t* __restrict__ unit_lo,
+                                                        const uint32_t* __restrict__ unit_block,
+                                                        const uint32_t* __restrict__ block_unit0,
+                                                        const double* __restrict__ contrib,
+                                                        double* __restrict__ contrib_next, double* p,
+                                                        const uint64_t* __restrict__ off, int64_t n,
+                                                        double* __restrict__ part, double* l1_part,
+                                                        uint64_t* deg_part, uint32_t* __restrict__ out_ids,
+                                                        int64_t* count, const int64_t* done, PrdStep st,
+                                                        const int64_t* shares, int nxcd, unsigned long long* heads) {
+  __shared__ double acc[1 << VB];
+  __shared__ uint32_t s_cnt[THREADS / kWave];
+  __shared__ unsigned long long s_base;
+  if (*done) return;
+  const int64_t u = order ? order[blockIdx.x] : blockIdx.x;
+  prd_unit<VB, THREADS, NT, UNROLL>(u, acc, s_cnt, &s_base, src, vl, unit_lo, unit_block, block_unit0, contrib,
+                                    contrib_next, p, off, n, part, l1_part, deg_part, out_ids, count, st);
+}
+
+// One dense PageRankDelta round on persistent workgroups taking units from
+// per-XCD shares (take_unit, as k_blk_pr_qx); k_prd_round_sum clears the
+// queue heads.
+template <int VB, int THREADS, bool NT, int UNROLL>
+__global__ __launch_bounds__(THREADS) void k_blk_prd_qx(const uint32_t* __restrict__ order,
+                                                        const uint32_t* __restrict__ src,
+                                                        const uint16_t* __restrict__ vl,
+                                                        const uint64_t* __restrict__ unit_lo,
+                                                        const uint32_t* __restrict__ unit_block,
+                                                        const uint32_t* __restrict__ block_unit0,
+                                                        const double* __restrict__ contrib,
+                                                        double* __restrict__ contrib_next, double* p,
+                                                        const uint64_t* __restrict__ off, int64_t n,
+                                                        double* __restrict__ part, double* l1_part,
+                                                        uint64_t* deg_part, uint32_t* __restrict__ out_ids,
+                                                        int64_t* count, const int64_t* done, PrdStep st,
+                                                        const int64_t* shares, int nxcd, unsigned long long* heads) {
+  __shared__ double acc[1 << VB];
+  __shared__ uint32_t s_cnt[THREADS / kWave];
+  __shared__ unsigned long long s_base;
+  __shared__ long long s_u;
+  if (*done) return;
+  int cur = xcc_id() % nxcd, tried = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_u = take_unit(order, shares, heads, nxcd, &cur, &tried);
+    __syncthreads();
+    const int64_t u = (int64_t)s_u;
+    if (u < 0) return;
+    prd_unit<VB, THREADS, NT, UNROLL>(u, acc, s_cnt, &s_base, src, vl, unit_lo, unit_block, block_unit0, contrib,
+                                      contrib_next, p, off, n, part, l1_part, deg_part, out_ids, count, st);
+  }
 }
 
 // Split blocks of a dense PageRankDelta round: a vertex's sum is its units'
@@ -568,6 +714,43 @@ int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out) {
   RG_TRY(ctx_copy(ctx, B->unit_block, ub.data(), sizeof(uint32_t) * ub.size(), hipMemcpyHostToDevice));
   if (!multi.empty())
     RG_TRY(ctx_copy(ctx, B->multi, multi.data(), sizeof(uint32_t) * multi.size(), hipMemcpyHostToDevice));
+  // The order by first source and its per-XCD shares (BlockedCSC::by_src).
+  {
+    DevBuf fs;
+    RG_TRY(fs.ensure(sizeof(uint32_t) * B->nunits));
+    hipLaunchKernelGGL(k_unit_first_src, grid_n(ctx, B->nunits), 256, 0, s, B->src, B->unit_lo, B->nunits,
+                       fs.as<uint32_t>());
+    RG_HIP(hipGetLastError());
+    std::vector<uint32_t> first(B->nunits);
+    RG_TRY(ctx_copy(ctx, first.data(), fs.p, sizeof(uint32_t) * B->nunits, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> split, single;
+    for (int64_t b = 0; b < nblocks; ++b) {
+      if (bu0[b + 1] - bu0[b] > 1)
+        for (uint32_t u = bu0[b]; u < bu0[b + 1]; ++u) split.push_back(u);
+      else
+        single.push_back(bu0[b]);
+    }
+    std::stable_sort(split.begin(), split.end(), [&](uint32_t a, uint32_t c) { return first[a] < first[c]; });
+    const int64_t X = std::min(kMaxXcds, std::max(1, ctx->num_xcds)), P = (int64_t)split.size();
+    const int64_t q = P / X, r = P % X;
+    std::vector<uint32_t> seq(split);
+    seq.insert(seq.end(), single.begin(), single.end());
+    std::vector<int64_t> shares(2 * (X + 1));
+    for (int64_t x = 0; x < X; ++x) {
+      shares[2 * x] = x * q + std::min(x, r);
+      shares[2 * x + 1] = (x + 1) * q + std::min(x + 1, r);
+    }
+    shares[2 * X] = P;
+    shares[2 * X + 1] = B->nunits;
+    // Then the one-queue form: every unit in one share, an empty common one.
+    shares.insert(shares.end(), {0, B->nunits, B->nunits, B->nunits});
+    RG_HIP(hipMalloc(&B->by_src, sizeof(uint32_t) * std::max<int64_t>(1, B->nunits)));
+    RG_HIP(hipMalloc(&B->xcd_shares, sizeof(int64_t) * shares.size()));
+    RG_TRY(ctx_copy(ctx, B->by_src, seq.data(), sizeof(uint32_t) * B->nunits, hipMemcpyHostToDevice));
+    RG_TRY(ctx_copy(ctx, B->xcd_shares, shares.data(), sizeof(int64_t) * shares.size(), hipMemcpyHostToDevice));
+    B->nsplit = P;
+    B->nxcd = (int)X;
+  }
   B->built_for = g->in_off;
   B->vb = vb;
   B->sb = sb;
@@ -592,11 +775,14 @@ __global__ void k_pr_init_b(const uint64_t* off, int64_t n, double* rank, double
 }
 
 // L1 norm of the iteration from the per-workgroup terms (one workgroup of
-// 1024 threads), its history entry, and the convergence flag.  The
+// 1024 threads), its history entry, the convergence flag, and the work
+// queue heads of k_blk_pr_qx cleared for the next iteration.  The
 // iteration number is counted on the device (*iter), so a captured loop
 // holds no per-iteration argument.
 __global__ __launch_bounds__(1024) void k_pr_finish_b(const double* partial, int nparts, double* l1_hist,
-                                                      int64_t* iter, int64_t* done, double epsilon) {
+                                                      int64_t* iter, int64_t* done, double epsilon,
+                                                      unsigned long long* queue, int nqueue) {
+  if ((int)threadIdx.x < nqueue) queue[threadIdx.x] = 0;   // the next iteration's work queues start empty
   if (*done) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
@@ -621,6 +807,40 @@ void layout_bits(const rg_opts* o, int* vb, int* sb) {
   *sb = (o && o->src_segment_bits > 0 && o->src_segment_bits <= 24) ? o->src_segment_bits : 0;
 }
 }  // namespace
+
+// How a blocked edge pass runs its units (rg_opts unit_order / unit_queue):
+// persistent workgroups on a work queue (default) or one workgroup per unit;
+// units by first source in one queue (default), by first source cut into
+// per-XCD shares, or in the layout's order.  Measured on symmetric RMAT-26
+// (degree-sorted), ms per iteration: one queue by source 5.87, per-XCD
+// shares 5.84-5.90, queue in layout order 5.98, one workgroup per unit in
+// layout order 6.27-6.31 (per-XCD static shares without a queue: 8.1, the
+// XCDs finish far apart).  The queue heads are context counters
+// 40 .. 40 + nxcd, zero when a pass starts.
+struct UnitSchedule {
+  bool queue = true;
+  const uint32_t* order = nullptr;
+  const int64_t* shares = nullptr;
+  int nxcd = 1;
+  int grid = 0;
+  unsigned long long* heads = nullptr;
+};
+
+UnitSchedule unit_schedule(rg_ctx* ctx, const BlockedCSC* B, const rg_opts* o, int vb, int threads) {
+  UnitSchedule u;
+  const int uo = o ? o->unit_order : 0;
+  u.queue = !(o && o->unit_queue == 1);
+  u.order = uo == 2 ? nullptr : B->by_src;
+  u.nxcd = uo == 1 ? B->nxcd : 1;
+  u.shares = uo == 1 ? B->xcd_shares : B->xcd_shares + 2 * (B->nxcd + 1);
+  // The workgroups that fit on the chip at once (LDS-bound: one per CU for
+  // 2^14-vertex blocks).
+  const int64_t lds_fit = (int64_t)ctx->lds_per_cu / ((int64_t(8) << vb) + 256);
+  const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(2048 / threads, lds_fit));
+  u.grid = u.queue ? (int)std::min<int64_t>(B->nunits, (int64_t)ctx->num_cus * per_cu) : (int)B->nunits;
+  u.heads = reinterpret_cast<unsigned long long*>(ctx->counts.as<int64_t>() + 40);
+  return u;
+}
 
 int pagerank_prepare(rg_graph* g, const rg_opts* o) {
   int vb, sb;
@@ -654,15 +874,20 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
   // before its vertex step was fused: 42.7 ms against 49.1), 1 on other id orders (PageRank on directed
   // RMAT-24 original ids: 118 GTEPS against 91).
   const bool u4 = o && o->unroll ? o->unroll == 4 : g->degree_sorted;
-  void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*, const uint32_t*,
-               const double*, double*, double*, const uint64_t*, int64_t, double*, double*, uint64_t*,
-               uint32_t*, int64_t*, const int64_t*, PrdStep);
+  const UnitSchedule us = unit_schedule(ctx, B, o, vb, threads);
+  void (*kern)(const uint32_t*, const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*,
+               const uint32_t*, const double*, double*, double*, const uint64_t*, int64_t, double*, double*,
+               uint64_t*, uint32_t*, int64_t*, const int64_t*, PrdStep, const int64_t*, int, unsigned long long*);
   void (*merge)(const uint32_t*, const uint32_t*, const double*, double*, const uint64_t*, double*, int64_t,
                 double*, uint64_t*, uint32_t*, int64_t*, const int64_t*, PrdStep);
 #define RG_PICK(VB)                                                                                   \
   do {                                                                                                \
-    kern = threads == 512 ? (u4 ? k_blk_prd_it<VB, 512, true, 4> : k_blk_prd_it<VB, 512, true, 1>)     \
-                          : (u4 ? k_blk_prd_it<VB, 1024, true, 4> : k_blk_prd_it<VB, 1024, true, 1>); \
+    if (us.queue)                                                                                     \
+      kern = threads == 512 ? (u4 ? k_blk_prd_qx<VB, 512, true, 4> : k_blk_prd_qx<VB, 512, true, 1>)   \
+                            : (u4 ? k_blk_prd_qx<VB, 1024, true, 4> : k_blk_prd_qx<VB, 1024, true, 1>); \
+    else                                                                                              \
+      kern = threads == 512 ? (u4 ? k_blk_prd_it<VB, 512, true, 4> : k_blk_prd_it<VB, 512, true, 1>)   \
+                            : (u4 ? k_blk_prd_it<VB, 1024, true, 4> : k_blk_prd_it<VB, 1024, true, 1>); \
     merge = k_blk_prd_merge<VB>;                                                                      \
   } while (0)
   if (vb == 14) RG_PICK(14);
@@ -671,10 +896,11 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
   else if (vb == 11) RG_PICK(11);
   else RG_PICK(10);
 #undef RG_PICK
+  RG_HIP(hipMemsetAsync(us.heads, 0, sizeof(int64_t) * (us.nxcd + 1), ctx->stream));
   hipEvent_t* ev = timing_begin(ctx);
-  hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, B->unit_block,
+  hipLaunchKernelGGL(kern, us.grid, threads, 0, ctx->stream, us.order, B->src, B->vl, B->unit_lo, B->unit_block,
                      B->block_unit0, contrib, contrib_next, p, g->off, g->n, part, l1_part, deg_part, out_ids,
-                     info, zero, st);
+                     info, zero, st, us.shares, us.nxcd, us.heads);
   if (mgrid)
     hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, part, p, g->off,
                        contrib_next, g->n, l1_part + B->nunits, deg_part + B->nunits, out_ids, info, zero, st);
@@ -693,17 +919,22 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
   BlockedCSC* B = nullptr;
   RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B));
-  void (*kern)(const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*, const uint32_t*,
-               const double*, double*, double*, const uint64_t*, int64_t, double*, double*, const int64_t*,
-               double, double);
+  const UnitSchedule us = unit_schedule(ctx, B, o, vb, threads);
+  void (*kern)(const uint32_t*, const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*,
+               const uint32_t*, const double*, double*, double*, const uint64_t*, int64_t, double*, double*,
+               const int64_t*, double, double, const int64_t*, int, unsigned long long*);
   void (*merge)(const uint32_t*, const uint32_t*, const double*, double*, const uint64_t*, double*, int64_t,
                 double*, const int64_t*, double, double);
   const bool u4 = o && o->unroll ? o->unroll == 4 : g->degree_sorted;   // default as in blocked_prd_round
-#define RG_PICK(VB)                                                                                 \
-  do {                                                                                              \
-    kern = threads == 512 ? (u4 ? k_blk_pr_it<VB, 512, true, 4> : k_blk_pr_it<VB, 512, true, 1>)     \
-                          : (u4 ? k_blk_pr_it<VB, 1024, true, 4> : k_blk_pr_it<VB, 1024, true, 1>); \
-    merge = k_blk_pr_merge<VB>;                                                                     \
+#define RG_PICK(VB)                                                                                   \
+  do {                                                                                                \
+    if (us.queue)                                                                                     \
+      kern = threads == 512 ? (u4 ? k_blk_pr_qx<VB, 512, true, 4> : k_blk_pr_qx<VB, 512, true, 1>)     \
+                            : (u4 ? k_blk_pr_qx<VB, 1024, true, 4> : k_blk_pr_qx<VB, 1024, true, 1>); \
+    else                                                                                              \
+      kern = threads == 512 ? (u4 ? k_blk_pr_it<VB, 512, true, 4> : k_blk_pr_it<VB, 512, true, 1>)     \
+                            : (u4 ? k_blk_pr_it<VB, 1024, true, 4> : k_blk_pr_it<VB, 1024, true, 1>); \
+    merge = k_blk_pr_merge<VB>;                                                                       \
   } while (0)
   if (vb == 14) RG_PICK(14);
   else if (vb == 13) RG_PICK(13);
@@ -730,6 +961,8 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   int64_t* iter_dev = ctx->counts.as<int64_t>() + 18;
   RG_HIP(hipMemsetAsync(done, 0, sizeof(int64_t), ctx->stream));
   RG_HIP(hipMemsetAsync(iter_dev, 0, sizeof(int64_t), ctx->stream));
+  const int nheads = us.nxcd + 1;
+  RG_HIP(hipMemsetAsync(us.heads, 0, sizeof(int64_t) * nheads, ctx->stream));
   const double damping = 0.85, epsilon = 0.0000001;
   const double one_over_n = 1 / (double)n;
   const double added = (1 - damping) * (1 / (double)n);
@@ -746,14 +979,15 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
       const double* cur = contrib[(it - 1) & 1];
       double* nxt = contrib[it & 1];
       hipEvent_t* ev = timed ? timing_begin(ctx) : nullptr;
-      hipLaunchKernelGGL(kern, (int)B->nunits, threads, 0, ctx->stream, B->src, B->vl, B->unit_lo, B->unit_block,
-                         B->block_unit0, cur, nxt, rank, g->off, n, unit_part, partial, done, damping, added);
+      hipLaunchKernelGGL(kern, us.grid, threads, 0, ctx->stream, us.order, B->src, B->vl, B->unit_lo,
+                         B->unit_block, B->block_unit0, cur, nxt, rank, g->off, n, unit_part, partial, done, damping,
+                         added, us.shares, us.nxcd, us.heads);
       if (mgrid)
         hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, unit_part, rank,
                            g->off, nxt, n, partial + B->nunits, done, damping, added);
       if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
       hipLaunchKernelGGL(k_pr_finish_b, 1, 1024, 0, ctx->stream, partial, (int)nparts, l1_hist, iter_dev, done,
-                         epsilon);
+                         epsilon, us.heads, nheads);
       RG_HIP(hipGetLastError());
     }
     return RG_OK;
@@ -767,7 +1001,8 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
         (uint64_t)(uintptr_t)kern, (uint64_t)threads, (uint64_t)maxiters,
         (uint64_t)(uintptr_t)rank, (uint64_t)(uintptr_t)contrib[0], (uint64_t)(uintptr_t)contrib[1],
         (uint64_t)(uintptr_t)unit_part, (uint64_t)(uintptr_t)partial, (uint64_t)nparts,
-        (uint64_t)(uintptr_t)g->off, (uint64_t)n, (uint64_t)(uintptr_t)done};
+        (uint64_t)(uintptr_t)g->off, (uint64_t)n, (uint64_t)(uintptr_t)done, (uint64_t)(uintptr_t)us.order,
+        (uint64_t)us.grid, (uint64_t)(uintptr_t)us.heads, (uint64_t)us.nxcd, (uint64_t)(uintptr_t)us.shares};
     GraphCache& gc = ctx->pr_graph;
     if (gc.key != key || !gc.exec) {
       gc.release();

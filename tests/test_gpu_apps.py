@@ -283,6 +283,39 @@ def test_blocked_shapes(rg, orc, ctx, graphs, gname, wg, unroll):
     assert res.iters == it and rel_linf(res.ranks, ref) < FP_TOL
 
 
+def _split_star(rg):
+    # A hub with 2^20 in-edges (its block split over units) plus a path.
+    leaves = 1 << 20
+    src = np.concatenate([np.zeros(leaves, np.uint32), np.arange(1, 1001, dtype=np.uint32)])
+    dst = np.concatenate([np.arange(1, leaves + 1, dtype=np.uint32), np.arange(1001, 2001, dtype=np.uint32)])
+    return rg.HostGraph.from_edges(leaves + 1, src, dst, symmetric=True)
+
+
+@pytest.mark.parametrize("gname", ["star", "rmat16s"])
+@pytest.mark.parametrize("unit_order", [0, 1, 2])
+@pytest.mark.parametrize("unit_queue", [0, 1])
+def test_blocked_unit_schedules(rg, orc, ctx, graphs, gname, unit_order, unit_queue):
+    # Every unit schedule of the blocked engine -- one work queue by first
+    # source (default), per-XCD queues, the layout's order; persistent
+    # workgroups or one per unit -- computes the oracle's PageRank and
+    # PageRankDelta, on graphs whose hub blocks are split over several units
+    # (the star's hub; the degree-sorted R-MAT's first 1024-vertex blocks).
+    hg = _split_star(rg) if gname == "star" else graphs(gname)
+    print(f"blocked schedule {gname} order={unit_order} queue={unit_queue}", flush=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    if gname != "star":
+        dg, _ = dg.degree_sort(sort_neighbors=True)
+    off, nbr = dg.download()
+    o = rg.opts("blocked", dst_block_bits=10, unit_order=unit_order, unit_queue=unit_queue)
+    for _ in range(2):   # the second call replays the captured loop
+        res = dg.pagerank(20, o)
+        ref, it, _ = orc.pagerank(off, nbr, 20)
+        assert res.iters == it and rel_linf(res.ranks, ref) < FP_TOL
+    res = dg.pagerank_delta(20, o)
+    ref, it, _ = orc.pagerank_delta(off, nbr, 20)
+    assert res.iters == it and rel_linf(res.ranks, ref) < FP_TOL
+
+
 @pytest.mark.parametrize("gname", ["rmat14s", "rmat12d"])
 def test_autotune_choice_gives_oracle_answers(rg, orc, ctx, graphs, gname):
     # Whatever configuration the autotuner picks (engine, workgroup shape,

@@ -28,7 +28,9 @@ Prints one JSON line.
 import argparse
 import json
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -391,6 +393,45 @@ def profile_run(rg, args):
     return {"profile": args.profile, "ms_per_run": round(ms, 3), "reps": args.profile_reps}
 
 
+def graph_cache_trip(rg, ctx, sg, new_ids, args):
+    """The restart path: the degree-sorted graph saved as GAP .sg with its
+    new_ids sidecar (rg_graph_save), then loaded back (rg_graph_load, mmap ->
+    pinned staging -> device) in place of generating, building and sorting
+    it.  The reload is checked (sizes, new_ids, one PageRank's iteration
+    count and L1 equal to the original graph's).  Skipped when the cache
+    directory lacks room.  The file was just written, so the load reads it
+    from the page cache: a cold restart adds the disk's read time."""
+    d = args.graph_cache or os.path.join(tempfile.gettempdir(), "rg_graph_cache")
+    os.makedirs(d, exist_ok=True)
+    n, m = sg.n, sg.m
+    nbytes = 17 + (n + 1) * 8 + m * 4
+    if shutil.disk_usage(d).free < nbytes + n * 4 + (2 << 30):
+        log(f"graph cache: skipped, {d} has {shutil.disk_usage(d).free / 1e9:.1f} GB free")
+        return {"skipped": "not enough free disk"}
+    path = os.path.join(d, f"rmat{args.scale}s_degree_sorted.sg")
+    try:
+        t0 = time.time()
+        sg.save(path, rg.FMT_SG, new_ids=new_ids)
+        t_save = time.time() - t0
+        t0 = time.time()
+        lg, lids = rg.DeviceGraph.load(ctx, path, rg.FMT_SG)
+        ctx.sync()
+        t_load = time.time() - t0
+        o = rg.opts("blocked")
+        a, b = sg.pagerank(3, o, want_ranks=False), lg.pagerank(3, o, want_ranks=False)
+        ok = (lids is not None and np.array_equal(lids, new_ids) and (lg.n, lg.m) == (n, m)
+              and a.iters == b.iters and abs(a.l1 - b.l1) <= 1e-10)
+        lg.close()
+    finally:
+        for f in (path, path + ".newids"):
+            if os.path.exists(f):
+                os.remove(f)
+    log(f"graph cache: save {t_save * 1e3:.0f} ms, load {t_load * 1e3:.0f} ms "
+        f"({nbytes / t_load / 1e9:.1f} GB/s), reload equal: {ok}")
+    return {"file_bytes": nbytes, "save_ms": round(t_save * 1e3, 1), "load_ms": round(t_load * 1e3, 1),
+            "load_gbs": round(nbytes / t_load / 1e9, 2), "reload_equal": bool(ok), "format": "GAP .sg + new_ids"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,6 +455,9 @@ def main():
                     help="comma list of sections to run (headline,variants,radii,tc,serve)")
     ap.add_argument("--profile", default="", help="section 'profile': app:variant:scale[d|s][:opts] (see profile_run)")
     ap.add_argument("--profile-reps", type=int, default=8)
+    ap.add_argument("--graph-cache", default="",
+                    help="directory for the save/reload of the degree-sorted headline graph "
+                         "(default: <tmp>/rg_graph_cache; 'off' skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run the library's plain 1 GiB stream copy once (k_stream_copy: the "
                          "known-size kernel that scales FETCH_SIZE / WRITE_SIZE)")
@@ -451,6 +495,9 @@ def main():
     if args.calibrate_copy:
         gb, _ = ctx.copy_bandwidth(1 << 30, reps=1)
         log(f"calibration copy (k_stream_copy): 1 GiB read + 1 GiB write, {gb:.0f} GB/s")
+        gr, _, gms = ctx.read_bandwidth(1 << 32, reps=1, gathers=1 << 27)
+        log(f"calibration read (k_stream_read): 4 GiB read once, {gr:.0f} GB/s; "
+            f"k_gather_read: 2^27 random 8-byte loads over it, {gms:.3f} ms ({(1 << 27) / gms / 1e6:.2f} G loads/s)")
     stream_gbs, _ = ctx.copy_bandwidth(1 << 30, reps=5)
     log(f"measured stream copy: {stream_gbs:.0f} GB/s (read + write)")
     dg = rg.DeviceGraph(ctx, hg)
@@ -461,6 +508,7 @@ def main():
     ctx.sync()
     t_sort = time.time() - t0
     log(f"degree sort (GPU, sorted lists): {t_sort * 1e3:.1f} ms")
+    cache = graph_cache_trip(rg, ctx, sg, new_ids, args) if args.graph_cache != "off" and not args.quick else None
     it = args.iters
 
     # RADAR PageRank on the engine the autotuner picks for this graph.
@@ -484,7 +532,7 @@ def main():
     ms_step = wall * 1e3 / args.steps
     value = args.steps * it * m / wall / 1e9
     # Algorithmic bytes of one timed launch: the blocked engine runs a whole
-    # iteration in it (k_blk_pr_it, plus k_blk_pr_merge for split hub
+    # iteration in it (k_blk_pr_qx, plus k_blk_pr_merge for split hub
     # blocks, timed together), the per-iteration 20 m + 48 n of the push
     # model: per arc 4 B column + 16 B accumulator read-modify-write, per
     # vertex 8 B offset + 8 B contribution + 8 B accumulator read + 8 B old
@@ -498,6 +546,12 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    # The blocked layout's own compulsory bytes per iteration: 6 B per arc
+    # (4 B source id + 2 B local destination, streamed once) and 40 B per
+    # vertex (offset, contribution read once, old rank, new rank, next
+    # contribution); the gathers of contrib[s] beyond one read per vertex
+    # are what the L2 does not keep.
+    compulsory = 6 * m + 40 * n
     # Whole iterations (edge and vertex phases) at 20 m + 48 n bytes each.
     pr_frac = it * (20 * m + 48 * n) / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS
     pr = {"radar_tuned": {"ms": round(ms_step, 3), "gteps": round(value, 2), "engine": engine,
@@ -629,7 +683,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "measured_stream_copy_gbs": round(stream_gbs, 1),
-                     "kernel": "k_blk_pr_it + k_blk_pr_merge" if engine == "blocked" else "edge_light_kernel<PrOp,SRC_ALL,HUB_PREFIX>",
+                     "frac_of_measured_copy": round(achieved / stream_gbs, 4),
+                     "compulsory_bytes_per_launch": compulsory if engine == "blocked" else None,
+                     "compulsory_gbs": round(compulsory / (avg_launch_ms * 1e-3) / 1e9, 1) if engine == "blocked" else None,
+                     "compulsory_frac_of_measured_copy": (round(compulsory / (avg_launch_ms * 1e-3) / 1e9 / stream_gbs, 4)
+                                                          if engine == "blocked" else None),
+                     "traffic_frac_of_measured_copy": (round(traffic / (avg_launch_ms * 1e-3) / 1e9 / stream_gbs, 4)
+                                                       if traffic else None),
+                     "kernel": "k_blk_pr_qx + k_blk_pr_merge" if engine == "blocked" else "edge_light_kernel<PrOp,SRC_ALL,HUB_PREFIX>",
                      "launches": launches, "avg_launch_ms": round(avg_launch_ms, 4),
                      "bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,
@@ -640,6 +701,7 @@ def main():
         "radii_bc": study,
         "tc": tc,
         "serving": serving or None,
+        "graph_cache": cache,
         "preprocess_ms": {"host_graph_build": round(t_gen * 1e3, 1), "gpu_degree_sort": round(t_sort * 1e3, 2),
                           "autotune": round(t_tune * 1e3, 1)},
     }

@@ -122,6 +122,14 @@ int rg_ctx_timing(rg_ctx* ctx, int enable);
  * and the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE
  * counters.  No counterpart in the reference. */
 int rg_device_copy_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, double* gbps, double* ms_per_copy);
+/* Device read probes, the read-side references for the FETCH_SIZE counter:
+ * k_stream_read loads `bytes` once per pass (16 bytes per lane, nothing
+ * stored), *gbps = bytes / time; then, when gathers > 0, k_gather_read makes
+ * `gathers` random 8-byte loads over the same buffer per pass
+ * (*ms_per_gather_pass).  `reps` passes of each are timed.  No counterpart
+ * in the reference. */
+int rg_device_read_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, int64_t gathers, double* gbps,
+                             double* ms_per_read, double* ms_per_gather_pass);
 int rg_ctx_kernel_time(rg_ctx* ctx, int64_t* launches, double* total_ms);
 /* Iteration loops as HIP graphs (on by default): the blocked PageRank
  * engine captures its whole iteration loop once per context and replays it
@@ -141,6 +149,33 @@ int rg_graph_info(const rg_graph* g, int64_t* n, int64_t* m, int* symmetric);
 /* Copy the current orientation's CSR (in_edges=0) or CSC back to the host.
  * offsets has n+1 entries, nbrs m entries. */
 int rg_graph_download(const rg_graph* g, int in_edges, uint64_t* offsets, uint32_t* nbrs);
+/* Fast ingest of the binary formats, file -> device with no host CSR
+ * (replaces Ligra's binary reader, PNO Ligra/ligra/IO.h:477-626, and GAP's
+ * ReadSerializedGraph, GAP/src/reader.h:251-304, for callers whose graph
+ * lives on the device): the file is memory-mapped and streamed through two
+ * pinned staging buffers (host copies overlap the DMA).  format RG_FMT_SG
+ * (the file says directed or not; a directed file's in-edge CSR is read
+ * as stored) or RG_FMT_ADJ_BINARY (path = the prefix of .config/.adj/.idx;
+ * symmetric as Ligra's -s; a directed graph's in-edges are built on the
+ * device, lists sorted by source, as the host reader makes them).  The
+ * lists are taken as stored (no dedup, as the binary readers).  A loaded
+ * CSR that is not valid (offsets not starting at 0, decreasing, not ending
+ * at m; a neighbour >= n) is refused with RG_EFORMAT before any kernel uses
+ * it.  When <path>.newids exists (written by rg_graph_save), the graph is
+ * marked degree-sorted and new_ids (host, n entries, nullable) receives it.
+ * No counterpart in the reference beyond the readers it replaces. */
+int rg_graph_load(rg_ctx* ctx, const char* path, int format, int symmetric, uint32_t* new_ids, rg_graph** out);
+/* n, m, symmetric (-1 for Ligra binary, whose files do not say) and whether
+ * a new_ids sidecar exists, from the file headers. */
+int rg_graph_file_info(const char* path, int format, int64_t* n, int64_t* m, int* symmetric, int* has_new_ids);
+/* Writes a device graph in its original orientation as RG_FMT_SG (GAP
+ * WriteSerializedGraph, GAP/src/writer.h:39-69) or RG_FMT_ADJ_BINARY
+ * (fewer than 2^32 arcs), streamed through pinned buffers; with new_ids
+ * (host, n entries) also <path>.newids, n uint32 new_ids[orig] -- the cache
+ * of a DegreeSorted graph that a restart loads with rg_graph_load instead of
+ * building and sorting the graph again.  Without new_ids a stale sidecar is
+ * removed. */
+int rg_graph_save(const rg_graph* g, const char* path, int format, const uint32_t* new_ids);
 /* Ligra graph::transpose() (graph.h:78-86): swap CSR and CSC of a directed
  * graph; no-op for symmetric graphs. */
 int rg_graph_transpose(rg_graph* g);

@@ -280,6 +280,75 @@ int rg_device_copy_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, double* gbps,
   return RG_OK;
 }
 
+// Read-only stream: every 16-byte word of src is loaded once; a workgroup
+// stores only when its XOR of everything it read equals an impossible
+// pattern, so the loads cannot be dropped and (for data that is not that
+// pattern) nothing is written.
+__global__ void k_stream_read(const uint4* __restrict__ src, int64_t n16, uint4* __restrict__ sink) {
+  uint4 x = make_uint4(0, 0, 0, 0);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint4 v = src[i];
+    x.x ^= v.x; x.y ^= v.y; x.z ^= v.z; x.w ^= v.w;
+  }
+  if (x.x == 0xdeadbeefu && x.y == 0x01234567u && x.z == 0x89abcdefu && x.w == 0x0badf00du) sink[blockIdx.x] = x;
+}
+
+// Random 8-byte gathers: count loads of src[hash(i) mod n8], the hash a
+// splitmix64 finalizer, so each load lands on its own line of a buffer far
+// larger than the caches; the same XOR guard keeps stores away.
+__global__ void k_gather_read(const uint64_t* __restrict__ src, int64_t n8, int64_t count, uint64_t* __restrict__ sink) {
+  uint64_t x = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t h = (uint64_t)i + 0x9e3779b97f4a7c15ull;
+    h = (h ^ (h >> 30)) * 0xbf58476d1ce4e5b9ull;
+    h = (h ^ (h >> 27)) * 0x94d049bb133111ebull;
+    h ^= h >> 31;
+    x ^= src[h % (uint64_t)n8];
+  }
+  if (x == 0xdeadbeef01234567ull) sink[blockIdx.x] = x;
+}
+
+int rg_device_read_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, int64_t gathers, double* gbps,
+                             double* ms_per_read, double* ms_per_gather_pass) {
+  if (!ctx || bytes < 16 || reps < 1 || gathers < 0)
+    return set_error(RG_EINVAL, "rg_device_read_bandwidth: bad arguments");
+  RG_HIP(hipSetDevice(ctx->device));
+  const int64_t n16 = bytes / 16;
+  rg::DevBuf a, sink;
+  const int grid = ctx->num_cus * 8;
+  RG_TRY(a.ensure((size_t)n16 * 16));
+  RG_TRY(sink.ensure(sizeof(uint4) * grid));
+  RG_HIP(hipMemsetAsync(a.p, 1, (size_t)n16 * 16, ctx->stream));
+  hipEvent_t e0, e1, e2;
+  RG_HIP(hipEventCreate(&e0));
+  RG_HIP(hipEventCreate(&e1));
+  RG_HIP(hipEventCreate(&e2));
+  hipLaunchKernelGGL(k_stream_read, grid, 256, 0, ctx->stream, a.as<uint4>(), n16, sink.as<uint4>());
+  RG_HIP(hipEventRecord(e0, ctx->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_stream_read, grid, 256, 0, ctx->stream, a.as<uint4>(), n16, sink.as<uint4>());
+  RG_HIP(hipEventRecord(e1, ctx->stream));
+  for (int r = 0; r < (gathers ? reps : 0); ++r)
+    hipLaunchKernelGGL(k_gather_read, grid, 256, 0, ctx->stream, a.as<uint64_t>(), n16 * 2, gathers,
+                       sink.as<uint64_t>());
+  RG_HIP(hipEventRecord(e2, ctx->stream));
+  RG_HIP(hipGetLastError());
+  RG_HIP(hipEventSynchronize(e2));
+  float ms = 0.f, msg = 0.f;
+  RG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  RG_HIP(hipEventElapsedTime(&msg, e1, e2));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  const double per = ms / reps;
+  if (ms_per_read) *ms_per_read = per;
+  if (gbps) *gbps = (double)(n16 * 16) / (per * 1e-3) / 1e9;
+  if (ms_per_gather_pass) *ms_per_gather_pass = gathers ? msg / reps : 0.0;
+  return RG_OK;
+}
+
 int rg_ctx_sync(rg_ctx* ctx) {
   if (!ctx) return set_error(RG_EINVAL, "rg_ctx_sync: null");
   RG_HIP(hipStreamSynchronize(ctx->stream));

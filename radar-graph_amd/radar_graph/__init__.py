@@ -135,9 +135,13 @@ _sig("rg_ctx_timing", C.c_int, _vp, C.c_int)
 _sig("rg_ctx_graphs", C.c_int, _vp, C.c_int)
 _sig("rg_ctx_kernel_time", C.c_int, _vp, _i64p, _f64p)
 _sig("rg_device_copy_bandwidth", C.c_int, _vp, C.c_int64, C.c_int, _f64p, _f64p)
+_sig("rg_device_read_bandwidth", C.c_int, _vp, C.c_int64, C.c_int, C.c_int64, _f64p, _f64p, _f64p)
 _sig("rg_graph_upload", C.c_int, _vp, _vp, C.POINTER(_vp))
 _sig("rg_graph_info", C.c_int, _vp, _i64p, _i64p, C.POINTER(C.c_int))
 _sig("rg_graph_download", C.c_int, _vp, C.c_int, _u64p, _u32p)
+_sig("rg_graph_load", C.c_int, _vp, C.c_char_p, C.c_int, C.c_int, _u32p, C.POINTER(_vp))
+_sig("rg_graph_file_info", C.c_int, C.c_char_p, C.c_int, _i64p, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("rg_graph_save", C.c_int, _vp, C.c_char_p, C.c_int, _u32p)
 _sig("rg_graph_transpose", C.c_int, _vp)
 _sig("rg_graph_free", C.c_int, _vp)
 _sig("rg_opts_default", None, C.POINTER(Opts))
@@ -419,6 +423,14 @@ class Context:
         """Bracket every edge-map launch with HIP events (resets the record)."""
         _check(lib.rg_ctx_timing(self._h, int(enable)))
 
+    def read_bandwidth(self, nbytes=1 << 30, reps=5, gathers=0):
+        """(GB/s read, ms per read pass, ms per pass of `gathers` random
+        8-byte loads) of the device read probes."""
+        g, ms, msg = C.c_double(), C.c_double(), C.c_double()
+        _check(lib.rg_device_read_bandwidth(self._h, int(nbytes), int(reps), int(gathers), C.byref(g),
+                                            C.byref(ms), C.byref(msg)))
+        return g.value, ms.value, msg.value
+
     def copy_bandwidth(self, nbytes=1 << 30, reps=10):
         """(GB/s read+write, ms per copy) of a plain device stream copy."""
         g, ms = C.c_double(), C.c_double()
@@ -574,6 +586,23 @@ class DeviceGraph:
         n, m, s = C.c_int64(), C.c_int64(), C.c_int()
         _check(lib.rg_graph_info(self._h, C.byref(n), C.byref(m), C.byref(s)))
         self.n, self.m, self.symmetric = n.value, m.value, bool(s.value)
+
+    @classmethod
+    def load(cls, ctx: Context, path, fmt=FMT_SG, symmetric=True):
+        """Binary graph file straight to the device (rg_graph_load); returns
+        (DeviceGraph, new_ids or None when the file has no new_ids sidecar)."""
+        n, m, s, has = C.c_int64(), C.c_int64(), C.c_int(), C.c_int()
+        _check(lib.rg_graph_file_info(os.fsencode(path), int(fmt), C.byref(n), C.byref(m), C.byref(s), C.byref(has)))
+        ids = np.empty(n.value, np.uint32) if has.value else None
+        h = _vp()
+        _check(lib.rg_graph_load(ctx._h, os.fsencode(path), int(fmt), int(symmetric),
+                                 _ptr(ids, _u32p) if ids is not None else None, C.byref(h)))
+        return cls(ctx, handle=h), ids
+
+    def save(self, path, fmt=FMT_SG, new_ids=None):
+        """Write the graph (rg_graph_save), with a new_ids sidecar if given."""
+        ids = None if new_ids is None else np.ascontiguousarray(new_ids, np.uint32)
+        _check(lib.rg_graph_save(self._h, os.fsencode(path), int(fmt), _ptr(ids, _u32p) if ids is not None else None))
 
     def download(self, in_edges=False):
         off = np.empty(self.n + 1, np.uint64)

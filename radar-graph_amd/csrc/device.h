@@ -197,6 +197,12 @@ int ctx_copy(rg_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKin
 // records element [1] after the launch.  nullptr when timing is off.
 hipEvent_t* timing_begin(rg_ctx* ctx);
 
+// Sorts every adjacency list of the CSR (off, nbr) into `sorted` (m
+// entries, ascending ids), on the context stream (reorder.hip); chunk_arcs
+// > 0 forces the chunked form that graphs of 2^31 arcs or more take.
+int sort_lists(rg_ctx* ctx, const uint64_t* off, const uint32_t* nbr, int64_t n, int64_t m, uint64_t chunk_arcs,
+               DevBuf& tmp, uint32_t* sorted);
+
 // Cached blocked layout of a graph's in-edges (blocked.hip).
 void blocked_free(void* p);
 

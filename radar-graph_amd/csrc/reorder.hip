@@ -78,6 +78,8 @@ int grid(rg_ctx* ctx, int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cus * 16));
 }
 
+}  // namespace
+
 // Sorts every list of (noff, nbr) into sorted (the segmented sort takes int
 // item and segment counts: graphs of 2^31 arcs or more, or any graph when
 // chunk_arcs > 0, are sorted in chunks of consecutive vertices holding at
@@ -125,6 +127,8 @@ int sort_lists(rg_ctx* ctx, const uint64_t* noff, const uint32_t* nbr, int64_t n
   }
   return RG_OK;
 }
+
+namespace {
 
 // Builds one relabelled CSR (offsets + lists) from (off, nbr); on success
 // the caller owns *noff_out and *nnbr_out.

@@ -131,13 +131,22 @@ int rg_device_copy_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, double* gbps,
 int rg_device_read_bandwidth(rg_ctx* ctx, int64_t bytes, int reps, int64_t gathers, double* gbps,
                              double* ms_per_read, double* ms_per_gather_pass);
 int rg_ctx_kernel_time(rg_ctx* ctx, int64_t* launches, double* total_ms);
-/* Iteration loops as HIP graphs (on by default): the blocked PageRank
- * engine captures its whole iteration loop once per context and replays it
- * with one launch while the graph, the iteration count and the context's
- * buffers stay the same (a serving slot answering repeated PageRank
- * queries); captured again when they change.  Not used while per-launch
- * timing (rg_ctx_timing) is on.  rg_ctx_graphs(ctx, 0) launches every
- * kernel of every iteration instead.  No counterpart in the reference. */
+/* Iteration loops as HIP graphs.  enable is a set of flags:
+ * RG_GRAPHS_LOOPS (on by default): the blocked PageRank engine captures its
+ * whole iteration loop once per context and replays it with one launch
+ * while the graph, the iteration count and the context's buffers stay the
+ * same (a serving slot answering repeated PageRank queries), and a
+ * bit-parallel BFS batch replays a captured chunk of four levels; captured
+ * again when they change.  Not used while per-launch timing (rg_ctx_timing)
+ * is on.  RG_GRAPHS_BFS_LEVELS (off by default): the single-source pull BFS
+ * decides each level's kind (top-down or bottom-up) on the device and runs
+ * chunks of four levels (replayed as a captured graph with RG_GRAPHS_LOOPS)
+ * with one host check per chunk instead of a read of the frontier counts
+ * per level -- same answers; on RMAT-26 the launches of the kernels a level
+ * does not use cost what the per-level reads cost (2.90 against 2.88 ms
+ * per query).  rg_ctx_graphs(ctx, 0) launches every kernel of every
+ * iteration from the host.  No counterpart in the reference. */
+enum { RG_GRAPHS_LOOPS = 1, RG_GRAPHS_BFS_LEVELS = 2 };
 int rg_ctx_graphs(rg_ctx* ctx, int enable);
 
 /* ---- device graphs ------------------------------------------------------- */

@@ -850,6 +850,197 @@ __global__ void k_level_cs(const uint32_t* ids, const int64_t* count, uint32_t l
 // rule off (or at 4, where it never fires), 3.07 at 16, 32 and 64.
 constexpr int64_t kPullStayArcCost = 16;
 
+// Single-source BFS with the level kinds decided on the device (the pull
+// engine's loop): the control block below replaces the host's per-level
+// read of the frontier size and degree sum, so a chunk of kBfsLevelsPerSync
+// levels -- every level's push kernels (skipped unless the level is
+// top-down) and pull kernels (skipped unless bottom-up), the checksum and
+// k_bfs_level_end -- is launched, or replayed as a captured HIP graph,
+// without a host round trip; the host checks for completion once per chunk.
+enum { BFS_DONE = 0, BFS_TOPDOWN = 1, BFS_BOTTOMUP = 2 };
+constexpr int kBfsLevelsPerSync = 4;   // even: a chunk ends on the list it started from
+
+struct BfsCtl {
+  int64_t mode;                 // BFS_DONE / BFS_TOPDOWN / BFS_BOTTOMUP: this level's kind
+  int64_t fcount, fdeg;         // the frontier: size, out-degree sum
+  int64_t ncount, ndeg;         // the next frontier (the level's out_info)
+  int64_t heavy_count;          // top-down heavy queue items
+  int64_t pheavy_count;         // bottom-up heavy vertices
+  int64_t reached, edges, iter;
+  int64_t chain;                // PullSel::chain of this level
+  int64_t heavy_deg;            // top-down wave-per-vertex cut-over of this level
+  unsigned long long cs;        // summary checksum (levels >= 1)
+  // constants of the query
+  int64_t threshold, n, m, heavy_cap, small_list, base_heavy_deg;
+};
+
+// The host loop's choice for the frontier in c (bfs_push): bottom-up when
+// |F| + degsum passes the threshold, or when the previous level was
+// bottom-up and staying costs less than pushing degsum random arcs; else
+// top-down, with lists longer than a wave sent to the heavy kernel when
+// the frontier fills fewer than 16 light waves per CU.
+__device__ void bfs_plan(BfsCtl* c) {
+  if (c->fcount == 0) {
+    c->mode = BFS_DONE;
+    return;
+  }
+  const bool stay = c->chain != 0 && c->fdeg * kPullStayArcCost > c->n + (c->m - c->edges);
+  if (c->fdeg > 0 && (c->fcount + c->fdeg > c->threshold || stay)) {
+    c->mode = BFS_BOTTOMUP;
+  } else {
+    c->mode = BFS_TOPDOWN;
+    c->heavy_deg = (c->fcount < c->small_list && c->base_heavy_deg > kWave &&
+                    c->fcount + c->fdeg / kHeavyChunk + 64 <= c->heavy_cap)
+                       ? kWave
+                       : c->base_heavy_deg;
+  }
+}
+
+__global__ void k_bfs_ctl_init(BfsCtl* c, const uint64_t* off, uint32_t src, int64_t threshold, int64_t n,
+                               int64_t m, int64_t heavy_cap, int64_t small_list, int64_t base_heavy_deg) {
+  c->fcount = 1;
+  c->fdeg = (int64_t)(off[src + 1] - off[src]);
+  c->ncount = c->ndeg = c->heavy_count = c->pheavy_count = 0;
+  c->reached = 1;
+  c->edges = c->fdeg;
+  c->iter = 0;
+  c->chain = 0;
+  c->cs = 0;
+  c->threshold = threshold;
+  c->n = n;
+  c->m = m;
+  c->heavy_cap = heavy_cap;
+  c->small_list = small_list;
+  c->base_heavy_deg = base_heavy_deg;
+  c->heavy_deg = base_heavy_deg;
+  bfs_plan(c);
+}
+
+// After a level: the next frontier becomes the frontier, counters reset,
+// the next level's kind chosen.
+__global__ void k_bfs_level_end(BfsCtl* c) {
+  if (c->mode == BFS_DONE) return;
+  c->iter += 1;
+  c->reached += c->ncount;
+  c->edges += c->ndeg;
+  c->chain = c->mode == BFS_BOTTOMUP ? (c->chain == 1 ? 2 : 1) : 0;
+  c->fcount = c->ncount;
+  c->fdeg = c->ndeg;
+  c->ncount = c->ndeg = c->heavy_count = c->pheavy_count = 0;
+  bfs_plan(c);
+}
+
+__global__ void k_level_cs_ctl(const uint32_t* ids, const BfsCtl* c, unsigned long long* out) {
+  if (c->mode == BFS_DONE) return;
+  const int64_t cnt = c->ncount;
+  const uint32_t level = (uint32_t)(c->iter + 1);
+  unsigned long long s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+    s += level_hash(ids[i], level);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (__lane_id() == 0 && s) atomicAdd(out, s);
+}
+
+// BfsOp whose launches run only on top-down levels.
+struct BfsLevelOp : BfsOp {
+  const int64_t* mode;
+  __device__ bool skip() const { return *mode != BFS_TOPDOWN; }
+};
+
+// The level loop of the pull engine run by the device (frontier list
+// ids[0] = {src} and parents initialised by the caller).
+int bfs_pull_device_loop(rg_graph* g, const Engine& E, const rg_opts* o, uint32_t src, uint32_t* parents,
+                         uint32_t* const list[2], bool checksum, int64_t* iters, int64_t* reached, int64_t* edges,
+                         uint64_t* cs_out) {
+  rg_ctx* ctx = g->ctx;
+  hipStream_t st = ctx->stream;
+  PullBufs pb;
+  RG_TRY(bfs_pull_buffers(g, o ? o->heavy_degree : 0, &pb));
+  RG_TRY(ctx->bfs_ctl.ensure(sizeof(BfsCtl)));
+  BfsCtl* c = ctx->bfs_ctl.as<BfsCtl>();
+  RG_TRY(bfs_pull_zero_tails(ctx, pb));
+  hipLaunchKernelGGL(k_bfs_ctl_init, 1, 1, 0, st, c, g->off, src, E.threshold, g->n, g->m, E.heavy_cap,
+                     (int64_t)ctx->num_cus * 16 * kWave, (int64_t)E.heavy_deg);
+  RG_HIP(hipGetLastError());
+  PullSel sel;
+  sel.mode = &c->mode;
+  sel.chain = &c->chain;
+  sel.want = BFS_BOTTOMUP;
+  sel.zero_bits = pb.bits;
+  sel.full0 = pb.full[0];
+  sel.full1 = pb.full[1];
+  BfsLevelOp op;
+  op.parents = parents;
+  op.mode = &c->mode;
+  EdgeArgs ea{};
+  ea.off = g->off;
+  ea.nbr = g->nbr;
+  // The list size is read on the device; the bound only sizes the grid (a
+  // persistent one of four 256-thread workgroups per CU: top-down levels of
+  // this engine are the small frontiers).
+  ea.count = (int64_t)ctx->num_cus * 4 * 256;
+  ea.count_dev = &c->fcount;
+  ea.out_info = &c->ncount;
+  ea.heavy = E.heavy;
+  ea.heavy_count = &c->heavy_count;
+  ea.heavy_deg = E.heavy_deg;
+  ea.heavy_deg_dev = &c->heavy_deg;
+  auto chunk = [&]() -> int {
+    for (int k = 0; k < kBfsLevelsPerSync; ++k) {
+      ea.ids = list[k & 1];
+      ea.out_ids = list[(k + 1) & 1];
+      RG_TRY((launch_hub<BfsLevelOp, SRC_LIST>(E, op, ea)));
+      RG_TRY(bfs_pull_level_dev(g, pb, sel, list[k & 1], &c->fcount, parents, &c->pheavy_count, list[(k + 1) & 1],
+                                &c->ncount));
+      if (checksum) hipLaunchKernelGGL(k_level_cs_ctl, ctx->num_cus, 256, 0, st, list[(k + 1) & 1], c, &c->cs);
+      hipLaunchKernelGGL(k_bfs_level_end, 1, 1, 0, st, c);
+      RG_HIP(hipGetLastError());
+    }
+    return RG_OK;
+  };
+  const bool captured = ctx->graphs && !ctx->timing;
+  GraphCache& gc = ctx->bfs_graph;
+  if (captured) {
+    const std::vector<uint64_t> key = {
+        (uint64_t)(uintptr_t)g->off, (uint64_t)(uintptr_t)g->nbr, (uint64_t)(uintptr_t)g->in_off,
+        (uint64_t)(uintptr_t)g->in_nbr, (uint64_t)(uintptr_t)pb.first_in, (uint64_t)pb.nn,
+        (uint64_t)(uintptr_t)pb.bits, (uint64_t)(uintptr_t)pb.full[0], (uint64_t)(uintptr_t)pb.full[1],
+        (uint64_t)(uintptr_t)pb.heavy, (uint64_t)pb.heavy_deg, (uint64_t)(uintptr_t)c,
+        (uint64_t)(uintptr_t)parents, (uint64_t)(uintptr_t)list[0], (uint64_t)(uintptr_t)list[1],
+        (uint64_t)(uintptr_t)E.heavy, (uint64_t)E.heavy_deg, (uint64_t)E.block, (uint64_t)checksum};
+    if (gc.key != key || !gc.exec) {
+      gc.release();
+      RG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const int rc = chunk();
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(st, &graph);
+      if (rc != RG_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      RG_HIP(ec);
+      gc.graph = graph;
+      RG_HIP(hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0));
+      gc.key = key;
+    }
+  }
+  for (int64_t steps = 0;; steps += kBfsLevelsPerSync) {
+    if (steps > g->n + 8) return set_error(RG_EDEVICE, "rg_bfs: no convergence");
+    if (captured) RG_HIP(hipGraphLaunch(gc.exec, st));
+    else RG_TRY(chunk());
+    int64_t mode = 0;
+    RG_TRY(ctx_read_count(ctx, &c->mode, &mode, 1));
+    if (mode == BFS_DONE) break;
+  }
+  BfsCtl h;
+  RG_TRY(ctx_copy(ctx, &h, c, sizeof(BfsCtl), hipMemcpyDeviceToHost));
+  *iters = h.iter;
+  *reached = h.reached;
+  *edges = h.edges;
+  if (cs_out) *cs_out = (uint64_t)h.cs;
+  return RG_OK;
+}
+
 int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out, int64_t* iters_out,
              int64_t* reached_out, int64_t* edges_out, double* ms_out, uint32_t* parents_dev,
              uint64_t* checksum_out, const BfsWork* work) {
@@ -907,6 +1098,21 @@ int bfs_push(rg_graph* g, uint32_t src, const rg_opts* o, uint32_t* parents_out,
   int64_t reached = 1, edges = degsum;
   std::unique_ptr<EventTimer> tm;
   if (ms_out) tm.reset(new EventTimer(ctx->stream));
+  // With RG_GRAPHS_BFS_LEVELS the pull engine's levels are decided on the
+  // device; otherwise the host reads every level's counts, as below.
+  if (pull && ctx->bfs_device_levels && !work) {
+    uint32_t* const lists[2] = {fa.ids, fb.ids};
+    uint64_t c = 0;
+    RG_TRY(bfs_pull_device_loop(g, E, o, src, parents, lists, checksum_out != nullptr, &iter, &reached, &edges,
+                                checksum_out ? &c : nullptr));
+    if (ms_out) *ms_out = tm->stop_ms();
+    if (iters_out) *iters_out = iter;
+    if (reached_out) *reached_out = reached;
+    if (edges_out) *edges_out = edges;
+    if (checksum_out) *checksum_out = c + level_hash(src, 0);
+    if (parents_out) RG_TRY(rg::ctx_copy(ctx, parents_out, parents, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    return RG_OK;
+  }
   // Consecutive pull levels pass the frontier on as a complete bitmap.
   const unsigned long long* pull_bits = nullptr;
   while (F->size > 0) {

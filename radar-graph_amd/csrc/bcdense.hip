@@ -343,13 +343,26 @@ int in_edges_end(rg_graph* g, int64_t* end) {
 constexpr int kPullU = 4;
 constexpr int kPullScan = 4;   // in-neighbours tested per step of a long scan
 
+// Picks the level's frontier and next bitmaps of a device-controlled pull
+// level (PullSel::mode set): false when this level is not a pull level.
+__device__ __forceinline__ bool pull_select(const PullSel& s, const unsigned long long*& active,
+                                            unsigned long long*& next) {
+  if (!s.mode) return true;
+  if (*s.mode != s.want) return false;
+  const int64_t c = *s.chain;
+  active = c == 0 ? s.zero_bits : c == 1 ? s.full0 : s.full1;
+  next = c == 1 ? s.full1 : s.full0;
+  return true;
+}
+
 __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, const uint32_t* in_nbr,
                                                         const uint32_t* first_in, const uint64_t* off, int64_t nn,
                                                         const unsigned long long* active, uint32_t* parents,
                                                         uint32_t* heavy, int64_t* heavy_count, int heavy_deg,
                                                         uint32_t* out_ids, int64_t* out_info,
-                                                        unsigned long long* next) {
+                                                        unsigned long long* next, PullSel sel) {
   __shared__ uint32_t s_q[4][kQueue];
+  if (!pull_select(sel, active, next)) return;
   EdgeArgs ea{};
   ea.off = off;
   ea.out_ids = out_ids;
@@ -423,8 +436,9 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(const uint64_t* in_off, 
                                                         const uint64_t* off, const unsigned long long* active,
                                                         uint32_t* parents, const uint32_t* heavy,
                                                         const int64_t* heavy_count, uint32_t* out_ids,
-                                                        int64_t* out_info, unsigned long long* next) {
+                                                        int64_t* out_info, unsigned long long* next, PullSel sel) {
   __shared__ uint32_t s_q[4][kQueue];
+  if (!pull_select(sel, active, next)) return;
   EdgeArgs ea{};
   ea.off = off;
   ea.out_ids = out_ids;
@@ -461,6 +475,19 @@ __global__ __launch_bounds__(256) void k_bfs_pull_heavy(const uint64_t* in_off, 
     wq_push(q, ea, join && lane == 0, v);
   }
   wq_finish(q, ea);
+}
+
+// A device-controlled pull level whose frontier is only a list (chain 0):
+// its bits set into the zero bitmap before the level, cleared after.
+__global__ void k_list_bits_sel(const uint32_t* ids, const int64_t* count, PullSel sel, bool set) {
+  if (*sel.mode != sel.want || *sel.chain != 0) return;
+  unsigned long long* bits = const_cast<unsigned long long*>(sel.zero_bits);
+  const int64_t c = *count;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < c; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = ids[i];
+    if (set) atomicOr(bits + (v >> 6), 1ull << (v & 63));
+    else bits[v >> 6] = 0ull;   // every bit set in the word is a frontier vertex's
+  }
 }
 
 __global__ void k_set_bits(const uint32_t* ids, int64_t count, unsigned long long* bits) {
@@ -583,6 +610,56 @@ int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* p
   return dense_level(g, 2, ids, count, nullptr, nullptr, nullptr, parents, out_ids, out_info);
 }
 
+int bfs_pull_buffers(rg_graph* g, int heavy_deg, PullBufs* pb) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  if (heavy_deg <= 0) heavy_deg = 256;
+  // The same layout as bfs_pull_level's: the zero frontier bitmap, then two
+  // complete bitmaps, the heavy counter and the heavy list.
+  const int64_t nwords = n / 64 + 1;
+  const size_t full_bytes = ((sizeof(unsigned long long) * nwords + 255) / 256) * 256;
+  const size_t cap = (size_t)(g->m / heavy_deg + 1);
+  unsigned long long* bits = nullptr;
+  void* part = nullptr;
+  RG_TRY(frontier_bitmap(ctx, n, 2 * full_bytes + sizeof(int64_t) * 2 + sizeof(uint32_t) * cap, &bits, &part));
+  pb->bits = bits;
+  pb->full[0] = static_cast<unsigned long long*>(part);
+  pb->full[1] = reinterpret_cast<unsigned long long*>(static_cast<char*>(part) + full_bytes);
+  pb->heavy = reinterpret_cast<uint32_t*>(reinterpret_cast<int64_t*>(static_cast<char*>(part) + 2 * full_bytes) + 2);
+  pb->heavy_deg = heavy_deg;
+  pb->nwords = nwords;
+  RG_TRY(in_edges_end(g, &pb->nn));
+  RG_TRY(in_first_neighbours(g, &pb->first_in));
+  pb->wnz = (pb->nn + 63) / 64;
+  return RG_OK;
+}
+
+int bfs_pull_zero_tails(rg_ctx* ctx, const PullBufs& pb) {
+  if (pb.nwords > pb.wnz) {
+    RG_TRY(fill<unsigned long long>(ctx, pb.full[0] + pb.wnz, pb.nwords - pb.wnz, 0ull));
+    RG_TRY(fill<unsigned long long>(ctx, pb.full[1] + pb.wnz, pb.nwords - pb.wnz, 0ull));
+  }
+  return RG_OK;
+}
+
+int bfs_pull_level_dev(rg_graph* g, const PullBufs& pb, const PullSel& sel, const uint32_t* ids,
+                       const int64_t* count_dev, uint32_t* parents, int64_t* hcount, uint32_t* out_ids,
+                       int64_t* out_info) {
+  rg_ctx* ctx = g->ctx;
+  const int gl = ctx->num_cus * 4;
+  hipLaunchKernelGGL(k_list_bits_sel, gl, 256, 0, ctx->stream, ids, count_dev, sel, true);
+  const int vgrid = (int)std::min<int64_t>((pb.nn + 256 * kPullU - 1) / (256 * kPullU), (int64_t)ctx->num_cus * 8);
+  hipLaunchKernelGGL(k_bfs_pull_light, std::max(vgrid, 1), 256, 0, ctx->stream, g->in_off, g->in_nbr, pb.first_in,
+                     g->off, pb.nn, (const unsigned long long*)nullptr, parents, pb.heavy, hcount, pb.heavy_deg,
+                     out_ids, out_info, (unsigned long long*)nullptr, sel);
+  hipLaunchKernelGGL(k_bfs_pull_heavy, ctx->num_cus * 4, 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off,
+                     (const unsigned long long*)nullptr, parents, pb.heavy, hcount, out_ids, out_info,
+                     (unsigned long long*)nullptr, sel);
+  hipLaunchKernelGGL(k_list_bits_sel, gl, 256, 0, ctx->stream, ids, count_dev, sel, false);
+  RG_HIP(hipGetLastError());
+  return RG_OK;
+}
+
 int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, int heavy_deg,
                    uint32_t* out_ids, int64_t* out_info, const unsigned long long* cur_bits,
                    const unsigned long long** next_bits) {
@@ -624,9 +701,9 @@ int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* pa
   hipEvent_t* ev = timing_begin(ctx);
   const int vgrid = (int)std::min<int64_t>((nn + 256 * kPullU - 1) / (256 * kPullU), (int64_t)ctx->num_cus * 8);
   hipLaunchKernelGGL(k_bfs_pull_light, std::max(vgrid, 1), 256, 0, ctx->stream, g->in_off, g->in_nbr, first_in,
-                     g->off, nn, active, parents, heavy, hcount, heavy_deg, out_ids, out_info, next);
+                     g->off, nn, active, parents, heavy, hcount, heavy_deg, out_ids, out_info, next, PullSel{});
   hipLaunchKernelGGL(k_bfs_pull_heavy, ctx->num_cus * 4, 256, 0, ctx->stream, g->in_off, g->in_nbr, g->off, active,
-                     parents, heavy, hcount, out_ids, out_info, next);
+                     parents, heavy, hcount, out_ids, out_info, next, PullSel{});
   if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
   if (!chained) hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);
   RG_HIP(hipGetLastError());

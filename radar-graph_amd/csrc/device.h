@@ -85,8 +85,13 @@ struct rg_ctx {
   // Iteration loops replayed as HIP graphs (rg_ctx_graphs; on by default,
   // not used while per-launch timing is on).
   bool graphs = true;
+  // Single-source pull BFS with its level kinds decided on the device
+  // (RG_GRAPHS_BFS_LEVELS; off by default).
+  bool bfs_device_levels = false;
   rg::GraphCache pr_graph;   // blocked PageRank's maxiters iterations
   rg::GraphCache ms_graph[3];   // a chunk of bit-parallel batch levels, per word width (1, 2, 4)
+  rg::GraphCache bfs_graph;     // a chunk of device-controlled single-source BFS levels
+  rg::DevBuf bfs_ctl;           // its level control block (BfsCtl)
   // Device array holding the last PageRank ranks / BC dependencies computed
   // on this context (n doubles; valid until the next call on the context).
   const double* last_result = nullptr;

@@ -71,6 +71,7 @@ struct EdgeArgs {
   const uint64_t* static_heavy;
   int64_t static_heavy_count;
   int heavy_deg;
+  const int64_t* heavy_deg_dev;   // SRC_LIST: cut-over read on the device (overrides heavy_deg)
   // hubs.  Slots are ordered by descending in-degree; slots < hub_k live
   // in the workgroup's LDS copy, slots in [hub_k, hub_k2) in the per-XCD
   // copy l2_copies[xcd * hub_k2 + slot] (operators with kL2 only).
@@ -278,6 +279,7 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
   const int64_t wave_id = (int64_t)blockIdx.x * WPB + w;
   const int64_t nwaves = (int64_t)gridDim.x * WPB;
   const bool static_heavy = a.static_heavy != nullptr;
+  const uint64_t heavy_deg = a.heavy_deg_dev ? (uint64_t)*a.heavy_deg_dev : (uint64_t)a.heavy_deg;
 
   for (int64_t c = wave_id; c < nchunks; c += nwaves) {
     int64_t i = c * kWave + lane;
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
     if (active) {
       start = a.off[v];
       uint64_t dd = a.off[v + 1] - start;
-      if (dd > (uint64_t)a.heavy_deg) {
+      if (dd > heavy_deg) {
         if (!static_heavy) {
           int64_t nch = (int64_t)((dd + kHeavyChunk - 1) / kHeavyChunk);
           int64_t base = atomicAdd((unsigned long long*)a.heavy_count, (unsigned long long)nch);

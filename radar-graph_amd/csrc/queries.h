@@ -134,18 +134,55 @@ int set_frontier_bits(rg_ctx* ctx, const uint32_t* ids, int64_t count, unsigned 
 int bfs_dense_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, uint32_t* out_ids,
                     int64_t* out_info);
 
+// First in-neighbour of every vertex of the current orientation (the pull
+// BFS's first probe), built on first use and kept with the graph.
+int in_first_neighbours(rg_graph* g, const uint32_t** out);
+
 // One bottom-up BFS level (the push-pull tree's pull edgeMap with early
 // exit) over g's in-edges (bcdense.hip).  cur_bits: the complete frontier
 // bitmap the previous pull level returned in *next_bits, or null (the
 // frontier is then taken from ids).  *next_bits (nullable) receives the
 // next frontier's complete bitmap, valid until the next dense call on the
 // context.
-// First in-neighbour of every vertex of the current orientation (the pull
-// BFS's first probe), built on first use and kept with the graph.
-int in_first_neighbours(rg_graph* g, const uint32_t** out);
 int bfs_pull_level(rg_graph* g, const uint32_t* ids, int64_t count, uint32_t* parents, int heavy_deg,
                    uint32_t* out_ids, int64_t* out_info, const unsigned long long* cur_bits = nullptr,
                    const unsigned long long** next_bits = nullptr);
+
+// Pull levels whose kind is decided on the device (the single-source BFS
+// level loop run as a captured chunk): every level's kernels are launched,
+// and those of a pull level run only when *mode == want.  chain: where the
+// level's frontier is as a complete bitmap -- 0: only as the list (its bits
+// are set into the zero bitmap and cleared after), 1: full0, 2: full1 (the
+// previous pull level's output); the level writes the other full bitmap.
+struct PullSel {
+  const int64_t* mode = nullptr;
+  const int64_t* chain = nullptr;
+  int64_t want = 0;
+  const unsigned long long* zero_bits = nullptr;
+  unsigned long long* full0 = nullptr;
+  unsigned long long* full1 = nullptr;
+};
+
+// The pull levels' buffers (those bfs_pull_level uses), the bound of the
+// vertices with in-edges (nn, wnz words) and the first in-neighbours.
+struct PullBufs {
+  unsigned long long* bits = nullptr;      // zero between levels
+  unsigned long long* full[2] = {nullptr, nullptr};
+  uint32_t* heavy = nullptr;
+  int heavy_deg = 256;
+  int64_t nn = 0, nwords = 0, wnz = 0;
+  const uint32_t* first_in = nullptr;
+};
+int bfs_pull_buffers(rg_graph* g, int heavy_deg, PullBufs* pb);
+// Zeroes the full bitmaps' words past the vertices with in-edges (no pull
+// level writes them), once before a device-controlled loop.
+int bfs_pull_zero_tails(rg_ctx* ctx, const PullBufs& pb);
+// Launches one device-controlled pull level: frontier list ids with its
+// size at *count_dev, heavy counter *hcount (zero when the level starts),
+// next frontier appended to out_ids with out_info[0..1] += count, degree sum.
+int bfs_pull_level_dev(rg_graph* g, const PullBufs& pb, const PullSel& sel, const uint32_t* ids,
+                       const int64_t* count_dev, uint32_t* parents, int64_t* hcount, uint32_t* out_ids,
+                       int64_t* out_info);
 
 int bc_push(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
             int64_t* combined_out, double* ms_out);

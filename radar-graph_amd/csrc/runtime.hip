@@ -193,6 +193,7 @@ int rg_ctx_destroy(rg_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->pr_graph.release();
+  ctx->bfs_graph.release();
   for (auto& gcache : ctx->ms_graph) gcache.release();
   ctx->counts.release();
   for (auto& b : ctx->scratch) b.release();
@@ -225,7 +226,8 @@ int rg_ctx_timing(rg_ctx* ctx, int enable) {
 int rg_ctx_graphs(rg_ctx* ctx, int enable) {
   if (!ctx) return set_error(RG_EINVAL, "rg_ctx_graphs: null");
   RG_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->graphs = enable != 0;
+  ctx->graphs = (enable & RG_GRAPHS_LOOPS) != 0;
+  ctx->bfs_device_levels = (enable & RG_GRAPHS_BFS_LEVELS) != 0;
   return RG_OK;
 }
 

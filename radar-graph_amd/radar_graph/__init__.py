@@ -415,9 +415,11 @@ class Context:
     def sync(self):
         _check(lib.rg_ctx_sync(self._h))
 
-    def graphs(self, enable=True):
-        """Iteration loops replayed as HIP graphs (on by default)."""
-        _check(lib.rg_ctx_graphs(self._h, int(enable)))
+    def graphs(self, enable=True, bfs_device_levels=False):
+        """Iteration loops replayed as HIP graphs (on by default); with
+        bfs_device_levels, the single-source pull BFS decides its level
+        kinds on the device (rg_ctx_graphs flags)."""
+        _check(lib.rg_ctx_graphs(self._h, (1 if enable else 0) | (2 if bfs_device_levels else 0)))
 
     def timing(self, enable=True):
         """Bracket every edge-map launch with HIP events (resets the record)."""

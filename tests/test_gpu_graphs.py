@@ -115,3 +115,44 @@ def test_batch_level_chunk_replays_match_eager(rg, orc, graphs, gname, k):
             assert np.array_equal(eager[0][1][j], ref)
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("gname", ["rmat12s", "rmat12d", "er12d", "rmat16s"])
+@pytest.mark.parametrize("threshold", [-1, 0, 1 << 40])
+def test_bfs_device_levels_match_host_levels(rg, orc, ctx, graphs, gname, threshold):
+    # With RG_GRAPHS_BFS_LEVELS the pull engine decides its level kinds on
+    # the device and replays a captured chunk of levels (eagerly launched
+    # with loops off); by default the host reads every level's counts and
+    # decides.  Both give the oracle's levels (a
+    # valid parent tree), the same rounds, reached vertices and out-degree
+    # sum, and the same query checksum, over repeated replays with eager
+    # work between them (threshold 0: every level bottom-up; 2^40: every
+    # level top-down; also a source without out-arcs).
+    hg = graphs(gname)
+    off, nbr = hg.csr()
+    srcs = list(hg.pick_sources(3))
+    deg = np.diff(off)
+    zero = np.nonzero(deg == 0)[0]
+    if zero.size:
+        srcs.append(int(zero[0]))
+    c = rg.Context(0)
+    try:
+        dg = rg.DeviceGraph(c, hg)
+        o = rg.opts("pull", threshold=threshold, heavy_degree=64)
+        host = {s: (dg.bfs(int(s), o), dg.bfs_query(int(s), o)[0].astuple()) for s in srcs}
+        for rep in range(4):
+            c.graphs(rep != 3, bfs_device_levels=True)
+            for s in srcs:
+                print(f"bfs device levels {gname} thr={threshold} src={s} rep={rep}", flush=True)
+                r = dg.bfs(int(s), o)
+                lv, ecc = orc.bfs_levels(off, nbr, int(s))
+                assert orc.check_parents(off, nbr, int(s), lv, r.parents) == 0
+                h, hq = host[s]
+                assert (r.iters, r.reached, r.edges) == (h.iters, h.reached, h.edges) == (
+                    ecc + 1, int((lv >= 0).sum()), int(deg[lv >= 0].sum()))
+                q, _ = dg.bfs_query(int(s), o)
+                assert q.astuple() == hq == orc.bfs_summary(off, lv)
+            dg.pagerank(3, rg.opts("blocked"))   # eager work between the replays
+            c.sync()
+    finally:
+        c.close()

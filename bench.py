@@ -220,6 +220,19 @@ def radii_bc_study(rg, args):
                                 "equivalent_gbs": round(eq / (bms * 1e-3) / 1e9, 1)}
         log(f"  radii ms {row['radii_ms']}  bc ms {row['bc_ms']}  64 BC queries: batch {bms:.1f} ms vs "
             f"{single:.1f} ms one at a time")
+        if kind == "rmat":
+            # Uneven source counts (1..256): rg_bc_batch runs them as batches
+            # of 64 with the last one partial; one at a time = the mean
+            # single-source time above times the count.
+            allsrc = ids[hg.pick_bc_sources(256)]
+            per_single = single / len(src64)
+            row["bc_uneven_batches"] = {}
+            for k in (1, 37, 100, 256):
+                sg.bc_batch(allsrc[:k], o)
+                _, _, kms = sg.bc_batch(allsrc[:k], o)
+                row["bc_uneven_batches"][str(k)] = {"ms": round(kms, 2), "queries_per_s": round(k / kms * 1e3, 1),
+                                                    "vs_one_at_a_time": round(per_single * k / kms, 2)}
+            log(f"  BC uneven batches: {row['bc_uneven_batches']}")
         out[f"{kind}{args.radii_scale}"] = row
         for x in (sg, dg):
             x.close()

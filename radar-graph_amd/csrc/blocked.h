@@ -44,6 +44,20 @@ struct BlockedCSC {
   int64_t nsplit = 0;
   int64_t* xcd_shares = nullptr;
   int nxcd = 0;
+  // Packed arcs (the PageRank layout, when 2^sbits-source segments leave
+  // room): pk[e] = (source mod 2^sbits) << vb | local destination, 4 bytes
+  // per arc in place of src + vl (which are then freed).  A unit's arcs,
+  // sorted by source, fall in at most kMaxSegs segments of 2^sbits
+  // sources: segment j of unit u (j in [unit_seg0[u], unit_seg0[u + 1]))
+  // starts at arc unit_lo[u] + seg_rel[j] and holds sources seg_hi[j] <<
+  // sbits + (pk >> vb).
+  uint32_t* pk = nullptr;          // m
+  uint32_t* seg_rel = nullptr;     // nsegs
+  uint32_t* seg_hi = nullptr;      // nsegs
+  uint32_t* unit_seg0 = nullptr;   // nunits + 1
+  int64_t nsegs = 0;
+  int sbits = 0;
+  bool pack_asked = false;
   void release();
 };
 
@@ -51,7 +65,10 @@ struct BlockedCSC {
 // the cache slot *slot (a member of rg_graph).  The layout is read-only once
 // built, so contexts sharing the graph may use it at the same time; the
 // per-unit partial results live in each context's scratch.
-int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out);
+// pack: build the packed arcs (BlockedCSC::pk) where the graph allows.
+int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out, bool pack = false);
+
+constexpr int kMaxSegs = 1024;   // source segments per unit in the packed layout
 
 // The query layout (2^kQueryVB-vertex blocks, source order) of g's current
 // orientation, built on first use.

@@ -45,6 +45,12 @@ void BlockedCSC::release() {
   if (multi) (void)hipFree(multi);
   if (by_src) (void)hipFree(by_src);
   if (xcd_shares) (void)hipFree(xcd_shares);
+  for (uint32_t* q : {pk, seg_rel, seg_hi, unit_seg0})
+    if (q) (void)hipFree(q);
+  pk = seg_rel = seg_hi = unit_seg0 = nullptr;
+  nsegs = 0;
+  sbits = 0;
+  pack_asked = false;
   src = nullptr; vl = nullptr; unit_lo = nullptr; block_unit0 = nullptr; unit_block = nullptr;
   multi = nullptr;
   by_src = nullptr;
@@ -144,6 +150,72 @@ __device__ __forceinline__ void blk_accumulate(const uint32_t* __restrict__ src,
   }
 }
 
+// The packed arcs and their source segments (BlockedCSC::pk, seg_rel,
+// seg_hi, unit_seg0, sbits); pk == nullptr: the layout keeps src + vl.
+struct PackedArcs {
+  const uint32_t* pk;
+  const uint32_t* seg_rel;
+  const uint32_t* seg_hi;
+  const uint32_t* unit_seg0;
+  int sbits;
+};
+
+// A unit's source segments staged in LDS: s_rel[j] = first arc of segment j
+// (relative to the unit's first arc), s_hi[j] its high source bits;
+// s_rel[count] = UINT32_MAX.  Synchronised by the caller's next barrier.
+template <int THREADS>
+__device__ __forceinline__ void stage_segments(const PackedArcs& pa, int64_t u, uint32_t* s_rel, uint32_t* s_hi) {
+  const uint32_t g0 = pa.unit_seg0[u], ns = pa.unit_seg0[u + 1] - g0;
+  for (uint32_t i = threadIdx.x; i < ns; i += THREADS) {
+    s_rel[i] = pa.seg_rel[g0 + i];
+    s_hi[i] = pa.seg_hi[g0 + i];
+  }
+  if (threadIdx.x == 0) s_rel[ns] = 0xffffffffu;
+}
+
+// blk_accumulate over packed arcs: arc r (relative) holds source
+// s_hi[j] << sbits | pk >> VB for the segment j holding r; each of a
+// thread's in-flight arcs keeps its own segment cursor, advanced as its
+// arcs move forward (a step crosses few segments except in the sparse tail
+// blocks).
+template <int VB, int THREADS, bool NT, int UNROLL>
+__device__ __forceinline__ void blk_accumulate_pk(const uint32_t* __restrict__ pk, uint64_t lo, uint64_t hi,
+                                                  const uint32_t* s_rel, const uint32_t* s_hi, int sbits,
+                                                  const double* __restrict__ contrib, double* acc) {
+  constexpr uint32_t LM = (1u << VB) - 1;
+  const uint32_t len = (uint32_t)(hi - lo);
+  const uint32_t* p = pk + lo;
+  uint32_t r = threadIdx.x;
+  int c0 = 0;
+  if (UNROLL == 4) {
+    int c1 = 0, c2 = 0, c3 = 0;
+    for (; r + 3 * THREADS < len; r += 4 * THREADS) {
+      const uint32_t w0 = ld_stream<NT>(p + r);
+      const uint32_t w1 = ld_stream<NT>(p + r + THREADS);
+      const uint32_t w2 = ld_stream<NT>(p + r + 2 * THREADS);
+      const uint32_t w3 = ld_stream<NT>(p + r + 3 * THREADS);
+      while (s_rel[c0 + 1] <= r) ++c0;
+      while (s_rel[c1 + 1] <= r + THREADS) ++c1;
+      while (s_rel[c2 + 1] <= r + 2 * THREADS) ++c2;
+      while (s_rel[c3 + 1] <= r + 3 * THREADS) ++c3;
+      const uint32_t s0 = (s_hi[c0] << sbits) | (w0 >> VB);
+      const uint32_t s1 = (s_hi[c1] << sbits) | (w1 >> VB);
+      const uint32_t s2 = (s_hi[c2] << sbits) | (w2 >> VB);
+      const uint32_t s3 = (s_hi[c3] << sbits) | (w3 >> VB);
+      const double x0 = contrib[s0], x1 = contrib[s1], x2 = contrib[s2], x3 = contrib[s3];
+      atomicAdd(&acc[w0 & LM], x0);
+      atomicAdd(&acc[w1 & LM], x1);
+      atomicAdd(&acc[w2 & LM], x2);
+      atomicAdd(&acc[w3 & LM], x3);
+    }
+  }
+  for (; r < len; r += THREADS) {
+    const uint32_t w = ld_stream<NT>(p + r);
+    while (s_rel[c0 + 1] <= r) ++c0;
+    atomicAdd(&acc[w & LM], contrib[(s_hi[c0] << sbits) | (w >> VB)]);
+  }
+}
+
 // PR_Vertex_F (PageRank.C:55) for vertex v from its in-edge sum a, fused
 // with the next iteration's contribution: returns |old - new| (the L1 term).
 __device__ __forceinline__ double pr_finalize(int64_t v, double a, double* rank, const uint64_t* off,
@@ -183,11 +255,17 @@ __device__ __forceinline__ void pr_unit(int64_t u, double* acc, const uint32_t* 
                                         const uint32_t* __restrict__ block_unit0, const double* __restrict__ contrib,
                                         double* __restrict__ contrib_next, double* rank,
                                         const uint64_t* __restrict__ off, int64_t n, double* __restrict__ part,
-                                        double* diff_part, double damping, double added) {
+                                        double* diff_part, double damping, double added, const PackedArcs& pa,
+                                        uint32_t* s_rel, uint32_t* s_hi) {
   constexpr int BV = 1 << VB;
   for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
+  if (pa.pk) stage_segments<THREADS>(pa, u, s_rel, s_hi);
   __syncthreads();
-  blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
+  if (pa.pk)
+    blk_accumulate_pk<VB, THREADS, NT, UNROLL>(pa.pk, unit_lo[u], unit_lo[u + 1], s_rel, s_hi, pa.sbits, contrib,
+                                               acc);
+  else
+    blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
   __syncthreads();
   const uint32_t b = unit_block[u];
   if (block_unit0[b + 1] - block_unit0[b] != 1) {
@@ -248,12 +326,14 @@ __global__ __launch_bounds__(THREADS) void k_blk_pr_it(const uint32_t* __restric
                                                        const uint64_t* __restrict__ off, int64_t n,
                                                        double* __restrict__ part, double* diff_part,
                                                        const int64_t* done, double damping, double added,
-                                                       const int64_t* shares, int nxcd, unsigned long long* heads) {
+                                                       const int64_t* shares, int nxcd, unsigned long long* heads,
+    PackedArcs pa) {
   __shared__ double acc[1 << VB];
+  __shared__ uint32_t s_rel[kMaxSegs + 1], s_hi[kMaxSegs];
   if (*done) return;
   const int64_t u = order ? order[blockIdx.x] : blockIdx.x;
   pr_unit<VB, THREADS, NT, UNROLL>(u, acc, src, vl, unit_lo, unit_block, block_unit0, contrib, contrib_next, rank,
-                                   off, n, part, diff_part, damping, added);
+                                   off, n, part, diff_part, damping, added, pa, s_rel, s_hi);
 }
 
 // XCD of the calling wave (hardware register XCC_ID, bits 3:0).
@@ -300,8 +380,10 @@ __global__ __launch_bounds__(THREADS) void k_blk_pr_qx(const uint32_t* __restric
                                                        const uint64_t* __restrict__ off, int64_t n,
                                                        double* __restrict__ part, double* diff_part,
                                                        const int64_t* done, double damping, double added,
-                                                       const int64_t* shares, int nxcd, unsigned long long* heads) {
+                                                       const int64_t* shares, int nxcd, unsigned long long* heads,
+    PackedArcs pa) {
   __shared__ double acc[1 << VB];
+  __shared__ uint32_t s_rel[kMaxSegs + 1], s_hi[kMaxSegs];
   __shared__ long long s_u;
   if (*done) return;
   int cur = xcc_id() % nxcd, tried = 0;
@@ -311,7 +393,7 @@ __global__ __launch_bounds__(THREADS) void k_blk_pr_qx(const uint32_t* __restric
     const int64_t u = (int64_t)s_u;
     if (u < 0) return;
     pr_unit<VB, THREADS, NT, UNROLL>(u, acc, src, vl, unit_lo, unit_block, block_unit0, contrib, contrib_next,
-                                     rank, off, n, part, diff_part, damping, added);
+                                     rank, off, n, part, diff_part, damping, added, pa, s_rel, s_hi);
   }
 }
 
@@ -455,12 +537,18 @@ __device__ __forceinline__ void prd_unit(int64_t u, double* acc, uint32_t* s_cnt
                                          const double* __restrict__ contrib, double* __restrict__ contrib_next,
                                          double* p, const uint64_t* __restrict__ off, int64_t n,
                                          double* __restrict__ part, double* l1_part, uint64_t* deg_part,
-                                         uint32_t* __restrict__ out_ids, int64_t* count, const PrdStep& st) {
+                                         uint32_t* __restrict__ out_ids, int64_t* count, const PrdStep& st,
+                                         const PackedArcs& pa, uint32_t* s_rel, uint32_t* s_hi) {
   constexpr int BV = 1 << VB;
   constexpr int FU = 4;
   for (int i = threadIdx.x; i < BV; i += THREADS) acc[i] = 0.0;
+  if (pa.pk) stage_segments<THREADS>(pa, u, s_rel, s_hi);
   __syncthreads();
-  blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
+  if (pa.pk)
+    blk_accumulate_pk<VB, THREADS, NT, UNROLL>(pa.pk, unit_lo[u], unit_lo[u + 1], s_rel, s_hi, pa.sbits, contrib,
+                                               acc);
+  else
+    blk_accumulate<THREADS, NT, UNROLL>(src, vl, unit_lo[u], unit_lo[u + 1], contrib, acc);
   __syncthreads();
   const uint32_t b = unit_block[u];
   if (block_unit0[b + 1] - block_unit0[b] != 1) {
@@ -524,14 +612,17 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_it(const uint32_t* __restri
                                                         double* __restrict__ part, double* l1_part,
                                                         uint64_t* deg_part, uint32_t* __restrict__ out_ids,
                                                         int64_t* count, const int64_t* done, PrdStep st,
-                                                        const int64_t* shares, int nxcd, unsigned long long* heads) {
+                                                        const int64_t* shares, int nxcd, unsigned long long* heads,
+    PackedArcs pa) {
   __shared__ double acc[1 << VB];
+  __shared__ uint32_t s_rel[kMaxSegs + 1], s_hi[kMaxSegs];
   __shared__ uint32_t s_cnt[THREADS / kWave];
   __shared__ unsigned long long s_base;
   if (*done) return;
   const int64_t u = order ? order[blockIdx.x] : blockIdx.x;
   prd_unit<VB, THREADS, NT, UNROLL>(u, acc, s_cnt, &s_base, src, vl, unit_lo, unit_block, block_unit0, contrib,
-                                    contrib_next, p, off, n, part, l1_part, deg_part, out_ids, count, st);
+                                    contrib_next, p, off, n, part, l1_part, deg_part, out_ids, count, st, pa, s_rel,
+                                    s_hi);
 }
 
 // One dense PageRankDelta round on persistent workgroups taking units from
@@ -550,8 +641,10 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_qx(const uint32_t* __restri
                                                         double* __restrict__ part, double* l1_part,
                                                         uint64_t* deg_part, uint32_t* __restrict__ out_ids,
                                                         int64_t* count, const int64_t* done, PrdStep st,
-                                                        const int64_t* shares, int nxcd, unsigned long long* heads) {
+                                                        const int64_t* shares, int nxcd, unsigned long long* heads,
+    PackedArcs pa) {
   __shared__ double acc[1 << VB];
+  __shared__ uint32_t s_rel[kMaxSegs + 1], s_hi[kMaxSegs];
   __shared__ uint32_t s_cnt[THREADS / kWave];
   __shared__ unsigned long long s_base;
   __shared__ long long s_u;
@@ -563,7 +656,8 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_qx(const uint32_t* __restri
     const int64_t u = (int64_t)s_u;
     if (u < 0) return;
     prd_unit<VB, THREADS, NT, UNROLL>(u, acc, s_cnt, &s_base, src, vl, unit_lo, unit_block, block_unit0, contrib,
-                                      contrib_next, p, off, n, part, l1_part, deg_part, out_ids, count, st);
+                                      contrib_next, p, off, n, part, l1_part, deg_part, out_ids, count, st, pa,
+                                      s_rel, s_hi);
   }
 }
 
@@ -635,9 +729,105 @@ int grid_n(rg_ctx* ctx, int64_t n) {
 
 }  // namespace
 
-int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out) {
+namespace {
+
+// Packed-arc build (BlockedCSC::pk): segment starts flagged (every unit's
+// first arc, every arc whose source segment differs from the previous
+// arc's), numbered by a scan, then the tables and the 4-byte arcs.
+__global__ void k_flag_units(const uint64_t* unit_lo, int64_t nunits, uint64_t m, uint32_t* flags) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nunits; u += (int64_t)gridDim.x * blockDim.x)
+    if (unit_lo[u] < unit_lo[u + 1]) flags[unit_lo[u]] = 1;
+}
+
+__global__ void k_flag_segments(const uint32_t* src, int64_t m, int sbits, uint32_t* flags) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + 1; e < m; e += (int64_t)gridDim.x * blockDim.x)
+    if ((src[e] >> sbits) != (src[e - 1] >> sbits)) flags[e] = 1;
+}
+
+__global__ void k_fill_segments(const uint32_t* flags, const uint32_t* incl, const uint32_t* src, int64_t m,
+                                const uint64_t* unit_lo, int64_t nunits, int sbits, uint32_t* seg_rel,
+                                uint32_t* seg_hi) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+    if (!flags[e]) continue;
+    // The unit holding arc e: the last u with unit_lo[u] <= e.
+    int64_t a = 0, b = nunits - 1;
+    while (a < b) {
+      const int64_t mid = (a + b + 1) / 2;
+      if (unit_lo[mid] <= (uint64_t)e) a = mid;
+      else b = mid - 1;
+    }
+    const uint32_t j = incl[e] - 1;
+    seg_rel[j] = (uint32_t)((uint64_t)e - unit_lo[a]);
+    seg_hi[j] = src[e] >> sbits;
+  }
+}
+
+__global__ void k_unit_segments(const uint64_t* unit_lo, int64_t nunits, const uint32_t* flags, const uint32_t* incl,
+                                uint64_t m, uint32_t nsegs, uint32_t* unit_seg0) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u <= nunits; u += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t lo = unit_lo[u];
+    unit_seg0[u] = lo < m ? incl[lo] - flags[lo] : nsegs;
+  }
+}
+
+__global__ void k_pack_arcs(const uint32_t* src, const uint16_t* vl, int64_t m, int sbits, int vb, uint32_t* pk) {
+  const uint32_t mask = sbits ? (sbits >= 32 ? 0xffffffffu : (1u << sbits) - 1) : 0u;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x)
+    pk[e] = ((src[e] & mask) << vb) | (uint32_t)vl[e];
+}
+
+// Builds the packed arcs of B when the graph allows (2^sbits-source
+// segments, at most kMaxSegs per unit, and sbits + vb <= 32 bits); src and
+// vl are freed then.  Otherwise B keeps src + vl.
+int pack_arcs(rg_ctx* ctx, BlockedCSC* B, int64_t n, int64_t m, int vb) {
+  int lg = 0;
+  while ((int64_t(1) << lg) < n) ++lg;
+  const int sbits = std::max(0, lg - 10);   // (n - 1) >> sbits < kMaxSegs
+  if (m <= 0 || sbits + vb > 32) return RG_OK;
+  hipStream_t s = ctx->stream;
+  DevBuf flags, incl, tmp;
+  RG_TRY(flags.ensure(sizeof(uint32_t) * m));
+  RG_TRY(incl.ensure(sizeof(uint32_t) * m));
+  RG_HIP(hipMemsetAsync(flags.p, 0, sizeof(uint32_t) * m, s));
+  hipLaunchKernelGGL(k_flag_units, grid_n(ctx, B->nunits), 256, 0, s, B->unit_lo, B->nunits, (uint64_t)m,
+                     flags.as<uint32_t>());
+  hipLaunchKernelGGL(k_flag_segments, grid_n(ctx, m), 256, 0, s, B->src, m, sbits, flags.as<uint32_t>());
+  size_t tb = 0;
+  RG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, flags.as<uint32_t>(), incl.as<uint32_t>(), (int)m, s));
+  RG_TRY(tmp.ensure(tb));
+  RG_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tb, flags.as<uint32_t>(), incl.as<uint32_t>(), (int)m, s));
+  uint32_t nsegs = 0;
+  RG_TRY(ctx_copy(ctx, &nsegs, incl.as<uint32_t>() + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost));
+  RG_HIP(hipMalloc(&B->seg_rel, sizeof(uint32_t) * std::max<uint32_t>(nsegs, 1)));
+  RG_HIP(hipMalloc(&B->seg_hi, sizeof(uint32_t) * std::max<uint32_t>(nsegs, 1)));
+  RG_HIP(hipMalloc(&B->unit_seg0, sizeof(uint32_t) * (B->nunits + 1)));
+  RG_HIP(hipMalloc(&B->pk, sizeof(uint32_t) * m));
+  hipLaunchKernelGGL(k_fill_segments, grid_n(ctx, m), 256, 0, s, flags.as<uint32_t>(), incl.as<uint32_t>(), B->src,
+                     m, B->unit_lo, B->nunits, sbits, B->seg_rel, B->seg_hi);
+  hipLaunchKernelGGL(k_unit_segments, grid_n(ctx, B->nunits + 1), 256, 0, s, B->unit_lo, B->nunits,
+                     flags.as<uint32_t>(), incl.as<uint32_t>(), (uint64_t)m, nsegs, B->unit_seg0);
+  hipLaunchKernelGGL(k_pack_arcs, grid_n(ctx, m), 256, 0, s, B->src, B->vl, m, sbits, vb, B->pk);
+  RG_HIP(hipGetLastError());
+  // Every unit within kMaxSegs segments (holds by the choice of sbits).
+  std::vector<uint32_t> us(B->nunits + 1);
+  RG_TRY(ctx_copy(ctx, us.data(), B->unit_seg0, sizeof(uint32_t) * us.size(), hipMemcpyDeviceToHost));
+  for (int64_t u = 0; u < B->nunits; ++u)
+    if (us[u + 1] - us[u] > (uint32_t)kMaxSegs)
+      return set_error(RG_EDEVICE, "blocked layout: a unit spans more than kMaxSegs source segments");
+  (void)hipFree(B->src);
+  (void)hipFree(B->vl);
+  B->src = nullptr;
+  B->vl = nullptr;
+  B->nsegs = nsegs;
+  B->sbits = sbits;
+  return RG_OK;
+}
+
+}  // namespace
+
+int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out, bool pack) {
   BlockedCSC* B = static_cast<BlockedCSC*>(*slot);
-  if (B && B->built_for == g->in_off && B->vb == vb && B->sb == sb) {
+  if (B && B->built_for == g->in_off && B->vb == vb && B->sb == sb && B->pack_asked == pack) {
     *out = B;
     return RG_OK;
   }
@@ -751,6 +941,8 @@ int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out) {
     B->nsplit = P;
     B->nxcd = (int)X;
   }
+  if (pack) RG_TRY(pack_arcs(ctx, B, n, m, vb));
+  B->pack_asked = pack;
   B->built_for = g->in_off;
   B->vb = vb;
   B->sb = sb;
@@ -846,7 +1038,7 @@ int pagerank_prepare(rg_graph* g, const rg_opts* o) {
   int vb, sb;
   layout_bits(o, &vb, &sb);
   BlockedCSC* B = nullptr;
-  return blocked_build(g, vb, sb, &g->blocked, &B);
+  return blocked_build(g, vb, sb, &g->blocked, &B, true);
 }
 
 // One edge pass of the blocked engine: per-unit partial sums of contrib[s]
@@ -860,7 +1052,7 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
   int vb, sb;
   layout_bits(o, &vb, &sb);
   BlockedCSC* B = nullptr;
-  RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B));
+  RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B, true));
   const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 1024);
   const int64_t nparts = B->nunits + mgrid;
   const size_t unit_doubles = (size_t)B->nunits << vb;
@@ -877,7 +1069,8 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
   const UnitSchedule us = unit_schedule(ctx, B, o, vb, threads);
   void (*kern)(const uint32_t*, const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*,
                const uint32_t*, const double*, double*, double*, const uint64_t*, int64_t, double*, double*,
-               uint64_t*, uint32_t*, int64_t*, const int64_t*, PrdStep, const int64_t*, int, unsigned long long*);
+               uint64_t*, uint32_t*, int64_t*, const int64_t*, PrdStep, const int64_t*, int, unsigned long long*,
+               PackedArcs);
   void (*merge)(const uint32_t*, const uint32_t*, const double*, double*, const uint64_t*, double*, int64_t,
                 double*, uint64_t*, uint32_t*, int64_t*, const int64_t*, PrdStep);
 #define RG_PICK(VB)                                                                                   \
@@ -900,7 +1093,8 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
   hipEvent_t* ev = timing_begin(ctx);
   hipLaunchKernelGGL(kern, us.grid, threads, 0, ctx->stream, us.order, B->src, B->vl, B->unit_lo, B->unit_block,
                      B->block_unit0, contrib, contrib_next, p, g->off, g->n, part, l1_part, deg_part, out_ids,
-                     info, zero, st, us.shares, us.nxcd, us.heads);
+                     info, zero, st, us.shares, us.nxcd, us.heads,
+                     PackedArcs{B->pk, B->seg_rel, B->seg_hi, B->unit_seg0, B->sbits});
   if (mgrid)
     hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, part, p, g->off,
                        contrib_next, g->n, l1_part + B->nunits, deg_part + B->nunits, out_ids, info, zero, st);
@@ -918,11 +1112,11 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
   layout_bits(o, &vb, &sb);
   const int threads = (o && o->wg_size > 0 && o->wg_size <= 512) ? 512 : kUnitThreads;
   BlockedCSC* B = nullptr;
-  RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B));
+  RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B, true));
   const UnitSchedule us = unit_schedule(ctx, B, o, vb, threads);
   void (*kern)(const uint32_t*, const uint32_t*, const uint16_t*, const uint64_t*, const uint32_t*,
                const uint32_t*, const double*, double*, double*, const uint64_t*, int64_t, double*, double*,
-               const int64_t*, double, double, const int64_t*, int, unsigned long long*);
+               const int64_t*, double, double, const int64_t*, int, unsigned long long*, PackedArcs);
   void (*merge)(const uint32_t*, const uint32_t*, const double*, double*, const uint64_t*, double*, int64_t,
                 double*, const int64_t*, double, double);
   const bool u4 = o && o->unroll ? o->unroll == 4 : g->degree_sorted;   // default as in blocked_prd_round
@@ -981,7 +1175,8 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
       hipEvent_t* ev = timed ? timing_begin(ctx) : nullptr;
       hipLaunchKernelGGL(kern, us.grid, threads, 0, ctx->stream, us.order, B->src, B->vl, B->unit_lo,
                          B->unit_block, B->block_unit0, cur, nxt, rank, g->off, n, unit_part, partial, done, damping,
-                         added, us.shares, us.nxcd, us.heads);
+                         added, us.shares, us.nxcd, us.heads,
+                         PackedArcs{B->pk, B->seg_rel, B->seg_hi, B->unit_seg0, B->sbits});
       if (mgrid)
         hipLaunchKernelGGL(merge, (int)mgrid, 256, 0, ctx->stream, B->multi, B->block_unit0, unit_part, rank,
                            g->off, nxt, n, partial + B->nunits, done, damping, added);
@@ -1002,7 +1197,9 @@ int pagerank_blocked(rg_graph* g, int64_t maxiters, const rg_opts* o, double* ra
         (uint64_t)(uintptr_t)rank, (uint64_t)(uintptr_t)contrib[0], (uint64_t)(uintptr_t)contrib[1],
         (uint64_t)(uintptr_t)unit_part, (uint64_t)(uintptr_t)partial, (uint64_t)nparts,
         (uint64_t)(uintptr_t)g->off, (uint64_t)n, (uint64_t)(uintptr_t)done, (uint64_t)(uintptr_t)us.order,
-        (uint64_t)us.grid, (uint64_t)(uintptr_t)us.heads, (uint64_t)us.nxcd, (uint64_t)(uintptr_t)us.shares};
+        (uint64_t)us.grid, (uint64_t)(uintptr_t)us.heads, (uint64_t)us.nxcd, (uint64_t)(uintptr_t)us.shares,
+        (uint64_t)(uintptr_t)B->pk, (uint64_t)(uintptr_t)B->seg_rel, (uint64_t)(uintptr_t)B->seg_hi,
+        (uint64_t)(uintptr_t)B->unit_seg0, (uint64_t)B->sbits};
     GraphCache& gc = ctx->pr_graph;
     if (gc.key != key || !gc.exec) {
       gc.release();

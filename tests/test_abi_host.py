@@ -239,3 +239,35 @@ def test_format_of_suffix(rg):
     f.argtypes = [rg.C.c_char_p]
     assert [f(x) for x in (b"g.el", b"g.wel", b"g.sg", b"g.gr", b"g.graph", b"g.mtx", b"g.adj")] == \
         [rg.FMT_EL, rg.FMT_WEL, rg.FMT_SG, rg.FMT_GR, rg.FMT_METIS, rg.FMT_MTX, rg.FMT_ADJ]
+
+
+def _file_info(rg, path, fmt):
+    import ctypes as C
+    n, m, s, has = C.c_int64(), C.c_int64(), C.c_int(), C.c_int()
+    rc = rg.lib.rg_graph_file_info(os.fsencode(str(path)), fmt, C.byref(n), C.byref(m), C.byref(s), C.byref(has))
+    return rc, (n.value, m.value, s.value, has.value)
+
+
+@pytest.mark.parametrize("gname", ["rmat10s", "rmat12d"])
+def test_graph_file_info(rg, graphs, tmp_path, gname):
+    # rg_graph_file_info reads the binary headers on the host (no GPU): n, m,
+    # the orientation (-1 for Ligra binary, whose files do not say) and
+    # whether a new_ids sidecar (n uint32) exists; a sidecar of the wrong
+    # size and a truncated .sg are refused.
+    hg = graphs(gname)
+    sg = tmp_path / "g.sg"
+    hg.write(sg, rg.FMT_SG)
+    assert _file_info(rg, sg, rg.FMT_SG) == (0, (hg.n, hg.m, 1 if hg.symmetric else 0, 0))
+    np.arange(hg.n, dtype=np.uint32).tofile(str(sg) + ".newids")
+    assert _file_info(rg, sg, rg.FMT_SG) == (0, (hg.n, hg.m, 1 if hg.symmetric else 0, 1))
+    np.arange(hg.n - 1, dtype=np.uint32).tofile(str(sg) + ".newids")
+    assert _file_info(rg, sg, rg.FMT_SG)[0] == rg.RG_EFORMAT
+    os.remove(str(sg) + ".newids")
+    binp = tmp_path / "g"
+    hg.write(binp, rg.FMT_ADJ_BINARY)
+    assert _file_info(rg, binp, rg.FMT_ADJ_BINARY) == (0, (hg.n, hg.m, -1, 0))
+    with open(sg, "r+b") as f:
+        f.truncate(40)
+    assert _file_info(rg, sg, rg.FMT_SG)[0] == rg.RG_EFORMAT
+    assert _file_info(rg, tmp_path / "missing.sg", rg.FMT_SG)[0] == rg.RG_EIO
+    assert _file_info(rg, sg, rg.FMT_ADJ)[0] == rg.RG_EUNSUPPORTED

@@ -340,3 +340,27 @@ def test_autotune_choice_gives_oracle_answers(rg, orc, ctx, graphs, gname):
         lv, ecc = orc.bfs_levels(soff, snbr, int(s))
         assert orc.check_parents(soff, snbr, int(s), lv, b.parents) == 0
         assert b.iters == ecc + 1
+
+
+@pytest.mark.parametrize("n", [5, 1000, 1025, 3000])
+def test_blocked_pagerank_small_graphs(rg, orc, ctx, n):
+    # The packed arcs of the blocked PageRank layout on small vertex counts:
+    # below 1024 vertices every source is its own segment (no low bits),
+    # just above, one low bit.  A directed cycle (every rank 1/n, a closed
+    # form) plus random chords, against the oracle.
+    rng = np.random.default_rng(n)
+    k = 4 * n
+    src = np.concatenate([np.arange(n), rng.integers(0, n, k)]).astype(np.uint32)
+    dst = np.concatenate([(np.arange(n) + 1) % n, rng.integers(0, n, k)]).astype(np.uint32)
+    for sym in (False, True):
+        hg = rg.HostGraph.from_edges(n, src, dst, symmetric=sym)
+        off, nbr = hg.csr()
+        dg = rg.DeviceGraph(ctx, hg)
+        for vb in (0, 10):
+            res = dg.pagerank(20, rg.opts("blocked", dst_block_bits=vb))
+            ref, it, _ = orc.pagerank(off, nbr, 20)
+            assert res.iters == it and rel_linf(res.ranks, ref) < FP_TOL, (sym, vb)
+    cyc = rg.HostGraph.from_edges(n, np.arange(n, dtype=np.uint32), ((np.arange(n) + 1) % n).astype(np.uint32),
+                                  symmetric=False)
+    r = rg.DeviceGraph(ctx, cyc).pagerank(20, rg.opts("blocked"))
+    assert np.allclose(r.ranks, 1.0 / n, rtol=1e-12, atol=0)

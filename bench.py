@@ -391,7 +391,7 @@ def profile_run(rg, args):
         if app == "query":
             return g.bfs_query(int(srcs[i]), o)[1]
         if app == "batch":   # --batch sources as one bit-parallel batch (the variant is unused)
-            return g.bfs_batch(srcs[:args.batch])[2]
+            return g.bfs_batch(srcs[:args.batch], o)[2]
         if app == "pagerank":
             return g.pagerank(args.iters, o, want_ranks=False).ms
         if app == "radii":
@@ -622,14 +622,27 @@ def main():
         t0 = time.perf_counter()
         single = [sg.bfs_query(int(s), bt)[0] for s in bsrc[:64]]
         t_single = (time.perf_counter() - t0) / 64
-        sg.bfs_batch(bsrc)
         reps = 3
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            out, _, _ = sg.bfs_batch(bsrc)
-        t_batch = (time.perf_counter() - t0) / reps
-        same = all(a.astuple() == b.astuple() for a, b in zip(out[:64], single))
+
+        def time_batch(o):
+            sg.bfs_batch(bsrc, o)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                res, _, _ = sg.bfs_batch(bsrc, o)
+            return (time.perf_counter() - t0) / reps, res
+
+        # The library's choice of launch form, then each form forced: per-level
+        # kernels (HIP-graph chunks of four levels) and one persistent launch.
+        t_batch, out = time_batch(rg.opts())
+        t_levels, out_levels = time_batch(rg.opts(batch_launch=1))
+        t_persist, out_persist = time_batch(rg.opts(batch_launch=2))
+        same = (all(a.astuple() == b.astuple() for a, b in zip(out[:64], single))
+                and [x.astuple() for x in out] == [x.astuple() for x in out_levels]
+                and [x.astuple() for x in out] == [x.astuple() for x in out_persist])
         batched = {"sources": len(bsrc), "batch_ms": round(t_batch * 1e3, 3),
+                   "launch": "one persistent launch" if m >= (1 << 27) else "per-level kernels (HIP graph chunks)",
+                   "per_level_launches_ms": round(t_levels * 1e3, 3),
+                   "persistent_launch_ms": round(t_persist * 1e3, 3),
                    "queries_per_s": round(len(bsrc) / t_batch, 1),
                    "one_at_a_time_queries_per_s": round(1.0 / t_single, 1),
                    "one_at_a_time_engine": ENGINES[bt.variant],

@@ -249,6 +249,12 @@ typedef struct rg_opts {
                         /* shares (a queue per XCD); 2 = the layout's order   */
   int unit_queue;       /* RG_BLOCKED: 0 = persistent workgroups take units   */
                         /* from a work queue; 1 = one workgroup per unit      */
+  int batch_launch;     /* bit-parallel batches (rg_bfs_batch, rg_radii's     */
+                        /* batch, serving BFS batches): 1 = per-level kernels */
+                        /* (chunks of four levels replayed as HIP graphs, one */
+                        /* host check per chunk); 2 = the whole batch as one  */
+                        /* persistent cooperative launch (grid barriers       */
+                        /* between phases); 0 = the library's choice          */
 } rg_opts;
 
 /* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */

@@ -255,32 +255,28 @@ __device__ __forceinline__ void heavy_items(const Op& op, const EdgeArgs& a, typ
   }
 }
 
-// Light (+ static heavy) kernel.  Grid-strides over chunks of 64 entries,
-// one chunk per wave.
-template <class Op, int MODE, int HUB, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
-  constexpr int WPB = BLOCK / kWave;
-  __shared__ uint32_t s_pre[WPB][kWave + 1];
-  __shared__ uint64_t s_start[WPB][kWave];
-  __shared__ uint32_t s_src[WPB][kWave];
-  __shared__ typename Op::SrcT s_val[WPB][kWave];
-  __shared__ uint32_t s_q[Op::kOutput ? WPB : 1][Op::kOutput ? kQueue : 1];
-  extern __shared__ __align__(16) unsigned char dyn_lds[];
-  typename Op::HubT* lds_hub = reinterpret_cast<typename Op::HubT*>(dyn_lds);
-  if (op.skip()) return;
-  hub_init<Op, HUB, BLOCK>(op, a, lds_hub);
-  typename Op::HubT* l2_hub = l2_copy<Op, HUB>(a);
+// LDS of the light phase: per wave, the scanned degrees, edge starts,
+// sources and source values of its 64 entries.
+template <class Op, int WPB>
+struct LightLds {
+  uint32_t pre[WPB][kWave + 1];
+  uint64_t start[WPB][kWave];
+  uint32_t src[WPB][kWave];
+  typename Op::SrcT val[WPB][kWave];
+};
 
+// Light phase over `count` frontier entries: chunks of 64 entries, one
+// chunk per wave (grid-stride over the waves given).  Vertices of degree
+// above heavy_deg are queued as chunk items (unless the heavy list is
+// static) and skipped here.  Called by whole waves.
+template <class Op, int MODE, int HUB, int WPB>
+__device__ __forceinline__ void light_phase(const Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub,
+                                            typename Op::HubT* l2_hub, WaveQueue& q, LightLds<Op, WPB>& L,
+                                            int64_t count, int64_t wave_id, int64_t nwaves, bool static_heavy,
+                                            uint64_t heavy_deg) {
   const int lane = (int)__lane_id();
   const int w = threadIdx.x / kWave;
-  WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0};
-  const int64_t count = a.count_dev ? *a.count_dev : a.count;
   const int64_t nchunks = (count + kWave - 1) / kWave;
-  const int64_t wave_id = (int64_t)blockIdx.x * WPB + w;
-  const int64_t nwaves = (int64_t)gridDim.x * WPB;
-  const bool static_heavy = a.static_heavy != nullptr;
-  const uint64_t heavy_deg = a.heavy_deg_dev ? (uint64_t)*a.heavy_deg_dev : (uint64_t)a.heavy_deg;
-
   for (int64_t c = wave_id; c < nchunks; c += nwaves) {
     int64_t i = c * kWave + lane;
     uint32_t v = 0;
@@ -314,10 +310,10 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
     }
     uint32_t total = __shfl(inc, kWave - 1);
     if (total == 0) continue;
-    s_pre[w][lane] = inc - deg;
-    s_start[w][lane] = start;
-    s_src[w][lane] = v;
-    if (deg) s_val[w][lane] = op.load_src(v);
+    L.pre[w][lane] = inc - deg;
+    L.start[w][lane] = start;
+    L.src[w][lane] = v;
+    if (deg) L.val[w][lane] = op.load_src(v);
     wave_sync_lds();
     for (uint32_t t0 = 0; t0 < total; t0 += kWave) {
       uint32_t t = t0 + lane;
@@ -327,15 +323,38 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
         int j = 0;
 #pragma unroll
         for (int step = 32; step > 0; step >>= 1)
-          if (s_pre[w][j + step] <= t) j += step;
-        uint64_t e = s_start[w][j] + (t - s_pre[w][j]);
+          if (L.pre[w][j + step] <= t) j += step;
+        uint64_t e = L.start[w][j] + (t - L.pre[w][j]);
         d = a.nbr[e];
-        act = do_edge<Op, HUB>(op, a, lds_hub, l2_hub, s_src[w][j], s_val[w][j], d);
+        act = do_edge<Op, HUB>(op, a, lds_hub, l2_hub, L.src[w][j], L.val[w][j], d);
       }
       if (Op::kOutput) wq_push(q, a, act, d);
     }
     wave_sync_lds();
   }
+}
+
+// Light (+ static heavy) kernel.  Grid-strides over chunks of 64 entries,
+// one chunk per wave.
+template <class Op, int MODE, int HUB, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
+  constexpr int WPB = BLOCK / kWave;
+  __shared__ LightLds<Op, WPB> L;
+  __shared__ uint32_t s_q[Op::kOutput ? WPB : 1][Op::kOutput ? kQueue : 1];
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
+  typename Op::HubT* lds_hub = reinterpret_cast<typename Op::HubT*>(dyn_lds);
+  if (op.skip()) return;
+  hub_init<Op, HUB, BLOCK>(op, a, lds_hub);
+  typename Op::HubT* l2_hub = l2_copy<Op, HUB>(a);
+
+  const int w = threadIdx.x / kWave;
+  WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0};
+  const int64_t count = a.count_dev ? *a.count_dev : a.count;
+  const int64_t wave_id = (int64_t)blockIdx.x * WPB + w;
+  const int64_t nwaves = (int64_t)gridDim.x * WPB;
+  const bool static_heavy = a.static_heavy != nullptr;
+  const uint64_t heavy_deg = a.heavy_deg_dev ? (uint64_t)*a.heavy_deg_dev : (uint64_t)a.heavy_deg;
+  light_phase<Op, MODE, HUB, WPB>(op, a, lds_hub, l2_hub, q, L, count, wave_id, nwaves, static_heavy, heavy_deg);
   if (static_heavy)
     heavy_items<Op, HUB>(op, a, lds_hub, l2_hub, q, a.static_heavy, wave_id, a.static_heavy_count,
                          nwaves);

@@ -17,9 +17,14 @@
 //     atomics; blocks split over several units (hubs) are merged by a
 //     second kernel.
 // Ligra's rule picks the level kind (|F| + sum of out-degrees > threshold,
-// ligra.h:269), on the device: every level launches the kernels of both
-// kinds and the ones not chosen exit at once, so the host reads the state
-// back only every few levels.
+// ligra.h:269), on the device.  Two launch forms give the same answers:
+//   * per-level kernels: every level launches the kernels of both kinds and
+//     the ones not chosen exit at once; chunks of four levels are captured
+//     once as a HIP graph and replayed, and the host reads the state back
+//     once per chunk;
+//   * one persistent launch: a cooperative grid runs every level of the
+//     batch, whatever its source count (1..256), with grid barriers between
+//     the phases of a level (k_ms_persist).
 //
 // Per source the batch yields the BFS summary of a single-source run:
 // rounds, vertices reached, the out-degree sum of the reached vertices and
@@ -29,6 +34,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "blocked.h"
@@ -214,16 +220,11 @@ __device__ __forceinline__ void stats_add(const MsArgs& a, MsStats<W>& st, uint3
 
 // Sparse level, vertex pass over the touched list (= the next frontier):
 // nn = next[d] (already disjoint from seen), seen |= nn, front_nxt = nn.
+// Entries [first, count) with stride `step`.
 template <int W>
-__global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
-  __shared__ MsStats<W> st;
-  if (a.ctl->mode != MS_SPARSE) return;
-  stats_init<W, 256>(st);
-  __syncthreads();
-  const int64_t count = a.ctl->ncount;
-  const uint32_t level = (uint32_t)a.ctl->level + 1;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
-       i += (int64_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void ms_sparse_fin(const MsArgs& a, MsStats<W>& st, int64_t count, uint32_t level,
+                                              int64_t first, int64_t step) {
+  for (int64_t i = first; i < count; i += step) {
     const uint32_t d = a.list_nxt[i];
     ull nn[W];
 #pragma unroll
@@ -240,6 +241,16 @@ __global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
     if (a.radii) a.radii[d] = (int32_t)level;
     stats_add<W>(a, st, d, level, nn, a.off[d + 1] - a.off[d]);
   }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
+  __shared__ MsStats<W> st;
+  if (a.ctl->mode != MS_SPARSE) return;
+  stats_init<W, 256>(st);
+  __syncthreads();
+  ms_sparse_fin<W>(a, st, a.ctl->ncount, (uint32_t)a.ctl->level + 1, blockIdx.x * (int64_t)blockDim.x + threadIdx.x,
+                   (int64_t)gridDim.x * blockDim.x);
   stats_flush<W, 256>(a, st);
 }
 
@@ -299,31 +310,19 @@ __device__ __forceinline__ T ld_nt(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
+// Dense level over units [0, nunits) taken from the counter *unit_next:
+// a persistent workgroup takes the next unit until none is left (the
+// conditions below are uniform across it).  acc (BV * W words), full
+// (BV / 64) and fullw (BV bytes when W > 1) are the workgroup's LDS.
 template <int W, int VB, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) {
+__device__ __forceinline__ void ms_dense_units(const MsArgs& a, int64_t nunits, unsigned long long* unit_next,
+                                               uint32_t level, const ull* allbits, ull* acc, ull* full,
+                                               uint8_t* fullw, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
+                                               int64_t& s_unit) {
   constexpr int BV = 1 << VB;
-  constexpr int NW = THREADS / kWave;
-  __shared__ ull acc[BV * W];
-  __shared__ ull full[BV / 64];
-  __shared__ uint8_t fullw[W > 1 ? BV : 1];   // W > 1: bit w = every source of word w reached it
-  __shared__ uint32_t s_q[NW][kQueue];
-  __shared__ MsStats<W> st;
-  if (a.ctl->mode != MS_DENSE) return;
-  const uint32_t level = (uint32_t)a.ctl->level + 1;
-  ull allbits[W];
-  batch_bits<W>((int)a.ctl->nsrc, allbits);
-  EdgeArgs ea{};
-  ea.off = a.off;
-  ea.out_ids = a.list_nxt;
-  ea.out_info = &a.ctl->ncount;
-  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
-  __shared__ int64_t s_unit;
-  stats_init<W, THREADS>(st);
-  // Persistent grid: each workgroup takes the next unit from a counter
-  // until none is left (the conditions below are uniform across it).
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) s_unit = (int64_t)atomicAdd((unsigned long long*)&a.ctl->unit_next, 1ull);
+    if (threadIdx.x == 0) s_unit = (int64_t)atomicAdd(unit_next, 1ull);
     __syncthreads();
     const int64_t u = s_unit;
     if (u >= nunits) break;
@@ -401,11 +400,51 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
       ms_finalize<W>(a, st, q, ea, d, d < a.n, &acc[l * W], level, allbits);
     }
   }
+}
+
+template <int W, int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) {
+  constexpr int BV = 1 << VB;
+  constexpr int NW = THREADS / kWave;
+  __shared__ ull acc[BV * W];
+  __shared__ ull full[BV / 64];
+  __shared__ uint8_t fullw[W > 1 ? BV : 1];   // W > 1: bit w = every source of word w reached it
+  __shared__ uint32_t s_q[NW][kQueue];
+  __shared__ MsStats<W> st;
+  __shared__ int64_t s_unit;
+  if (a.ctl->mode != MS_DENSE) return;
+  const uint32_t level = (uint32_t)a.ctl->level + 1;
+  ull allbits[W];
+  batch_bits<W>((int)a.ctl->nsrc, allbits);
+  EdgeArgs ea{};
+  ea.off = a.off;
+  ea.out_ids = a.list_nxt;
+  ea.out_info = &a.ctl->ncount;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  stats_init<W, THREADS>(st);
+  ms_dense_units<W, VB, THREADS>(a, nunits, (unsigned long long*)&a.ctl->unit_next, level, allbits, acc, full, fullw,
+                                 st, q, ea, s_unit);
   wq_finish(q, ea);
   stats_flush<W, THREADS>(a, st);
 }
 
-// Blocks split over several units: OR the units' partial words, finalise.
+// Blocks split over several units: OR the units' partial words, finalise
+// vertex l of split block b.  Called by whole waves (64 aligned vertices).
+template <int W, int VB>
+__device__ __forceinline__ void ms_merge_vertex(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
+                                                uint32_t b, int l, uint32_t level, const ull* allbits) {
+  const int64_t d = ((int64_t)b << VB) + l;
+  ull accw[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) accw[w] = 0;
+  for (uint32_t u = a.block_unit0[b]; u < a.block_unit0[b + 1]; ++u) {
+    const ull* p = a.part + (((size_t)u << VB) + l) * W;
+#pragma unroll
+    for (int w = 0; w < W; ++w) accw[w] |= p[w];
+  }
+  ms_finalize<W>(a, st, q, ea, d, d < a.n, accw, level, allbits);
+}
+
 // Grid: nmulti * (BV / THREADS) workgroups.
 template <int W, int VB, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
@@ -417,26 +456,17 @@ __global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
   if (a.ctl->mode != MS_DENSE) return;
   const uint32_t b = a.multi[blockIdx.x / PER];
   const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
-  const int64_t d = ((int64_t)b << VB) + l;
   const uint32_t level = (uint32_t)a.ctl->level + 1;
   ull allbits[W];
   batch_bits<W>((int)a.ctl->nsrc, allbits);
   stats_init<W, THREADS>(st);
   __syncthreads();
-  ull accw[W];
-#pragma unroll
-  for (int w = 0; w < W; ++w) accw[w] = 0;
-  for (uint32_t u = a.block_unit0[b]; u < a.block_unit0[b + 1]; ++u) {
-    const ull* p = a.part + (((size_t)u << VB) + l) * W;
-#pragma unroll
-    for (int w = 0; w < W; ++w) accw[w] |= p[w];
-  }
   EdgeArgs ea{};
   ea.off = a.off;
   ea.out_ids = a.list_nxt;
   ea.out_info = &a.ctl->ncount;
   WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
-  ms_finalize<W>(a, st, q, ea, d, d < a.n, accw, level, allbits);
+  ms_merge_vertex<W, VB>(a, st, q, ea, b, l, level, allbits);
   wq_finish(q, ea);
   stats_flush<W, THREADS>(a, st);
 }
@@ -445,22 +475,26 @@ __global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
 // Level bookkeeping.
 
 // After a level: clear the current frontier's words (the buffer becomes the
-// next level's output) and its active bits.
+// next level's output) and its active bits; entries and words [first, ..)
+// with stride `step`.
 template <int W>
-__global__ void k_ms_clear(MsArgs a, int64_t nwords) {
-  if (a.ctl->mode == MS_DONE) return;
-  const int64_t count = a.ctl->fcount;
+__device__ __forceinline__ void ms_clear(const MsArgs& a, int64_t count, int64_t nwords, int64_t first,
+                                         int64_t step) {
   ull* fc = const_cast<ull*>(a.front_cur);
   ull* ac = const_cast<ull*>(a.active_cur);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = first; i < count; i += step) {
     const uint32_t v = a.list_cur[i];
 #pragma unroll
     for (int w = 0; w < W; ++w) fc[(size_t)v * W + w] = 0;
   }
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwords;
-       i += (int64_t)gridDim.x * blockDim.x)
-    ac[i] = 0;
+  for (int64_t i = first; i < nwords; i += step) ac[i] = 0;
+}
+
+template <int W>
+__global__ void k_ms_clear(MsArgs a, int64_t nwords) {
+  if (a.ctl->mode == MS_DONE) return;
+  ms_clear<W>(a, a.ctl->fcount, nwords, blockIdx.x * (int64_t)blockDim.x + threadIdx.x,
+              (int64_t)gridDim.x * blockDim.x);
 }
 
 // Closes a level: per-source reached counts and depths, the next frontier
@@ -539,6 +573,239 @@ __global__ void k_ms_init(MsArgs a, int W, const uint32_t* sources, int64_t nuni
 }
 
 // ---------------------------------------------------------------------------
+// One persistent launch for a whole batch.
+//
+// A cooperative launch of as many workgroups as can be resident at once
+// runs every level of the batch: the same phases as the per-level kernels
+// above (sparse: light phase, heavy chunks, vertex pass; dense: units,
+// split-block merge; then the clear of the current frontier), separated by
+// grid-wide barriers instead of kernel boundaries.  Every workgroup reads
+// the level's frontier size after the barrier that closes the level and
+// takes the same branch, so all of them meet at every barrier.  A barrier
+// wait is bounded by the wall clock: past the limit the waiting workgroup
+// sets the abort flag, and every workgroup leaves the level loop at its
+// next barrier, so a launch always ends; the host then reports the batch
+// as failed.
+
+// Barrier state: workgroup 0 writes `release` (barriers completed) after
+// seeing every workgroup's arrival flag; any waiter past the wall-clock
+// limit sets `abort`.  Each flag has its own 128-byte line (flags follow
+// the control block in the same buffer, cleared before every launch).
+struct GridBar {
+  unsigned int release;
+  unsigned int abort;
+  unsigned int pad[30];
+};
+constexpr int kFlagStride = 32;   // unsigned ints per arrival flag (128 bytes)
+
+// Outputs of one level: the next frontier list (size, out-degree sum, as
+// EdgeArgs::out_info), the heavy chunk items queued (sparse) and the unit
+// counter (dense).  Three slots rotate: level L writes slot L % 3 while
+// workgroup 0 clears slot (L + 1) % 3, which every workgroup last read when
+// level L - 1 began.
+struct MsSlot {
+  int64_t ncount, ndeg;
+  int64_t heavy_count;
+  unsigned long long unit_next;
+  int64_t pad[4];
+};
+
+struct MsPersistCtl {
+  GridBar bar;
+  MsSlot slot[3];
+  int64_t pad[8];
+};
+static_assert(sizeof(MsSlot) == 64 && sizeof(MsPersistCtl) == 384, "persistent control block layout");
+
+struct MsPersistArgs {
+  MsPersistCtl* pc;
+  unsigned int* flags;          // arrival flags, kFlagStride apart
+  const uint32_t* nbr;          // out-neighbours (sparse levels)
+  ull* front[2];
+  ull* active[2];
+  uint32_t* list[2];
+  ull* lvl_new[2];              // per-source vertices reached, by level parity
+  ull* reached;
+  int64_t* depth;
+  uint32_t* stamp;
+  uint64_t* heavy;              // heavy chunk queue
+  int heavy_deg;
+  int64_t nunits, nmulti, nwords;
+  uint64_t spin_ticks;          // barrier wait limit (wall-clock ticks)
+};
+
+// Wall-clock-bounded spin until *p == want; false (and the abort flag set)
+// past the limit or once another waiter has set it.
+__device__ __forceinline__ bool spin_until(const unsigned int* p, unsigned int want, GridBar* b, uint64_t t0,
+                                           uint64_t spin_ticks) {
+  while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+    if (__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if (wall_clock64() - t0 > spin_ticks) {
+      __hip_atomic_store(&b->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+  }
+  return true;
+}
+
+// Grid-wide barrier.  The workgroup's threads order their writes at
+// workgroup scope, then one thread writes the XCD's L2 back (device-scope
+// release fence) and stores the workgroup's arrival flag.  Workgroup 0
+// polls every flag with its threads in parallel and stores the release
+// word; each workgroup's first thread waits for it and invalidates its
+// caches (device-scope acquire fence), so what any workgroup wrote before
+// the barrier is seen by all after it.  Fencing in one thread per
+// workgroup rather than in every thread keeps a barrier at ~14 us on 256
+// workgroups (~170 us with every thread fencing; a counter that all
+// workgroups increment costs ~31 us).  Returns false once the barrier has
+// been abandoned.
+__device__ __forceinline__ bool grid_sync(GridBar* b, unsigned int* flags, unsigned int nwg, unsigned int& gen,
+                                          uint64_t spin_ticks, int& s_ok) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  const unsigned int g = gen + 1;
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(&flags[(size_t)blockIdx.x * kFlagStride], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (blockIdx.x == 0) {
+    const uint64_t t0 = wall_clock64();
+    for (unsigned int i = threadIdx.x; i < nwg; i += blockDim.x)
+      if (!spin_until(&flags[(size_t)i * kFlagStride], g, b, t0, spin_ticks)) break;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      __hip_atomic_store(&b->release, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (threadIdx.x == 0) {
+    (void)spin_until(&b->release, g, b, wall_clock64(), spin_ticks);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_ok = __hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  }
+  gen = g;
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return s_ok != 0;
+}
+
+template <int W, int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs p) {
+  constexpr int BV = 1 << VB;
+  constexpr int NW = THREADS / kWave;
+  constexpr int PER = BV / THREADS;
+  using Op = MsPushOp<W>;
+  using Light = LightLds<Op, NW>;
+  // Dense levels hold the block's words here; sparse levels, the light
+  // phase's tables.
+  __shared__ ull lds_main[BV * W];
+  __shared__ ull full[BV / 64];
+  __shared__ uint8_t fullw[W > 1 ? BV : 1];
+  __shared__ uint32_t s_q[NW][kQueue];
+  __shared__ MsStats<W> st;
+  __shared__ int64_t s_unit;
+  __shared__ int s_ok;
+  static_assert(sizeof(Light) <= sizeof(ull) * BV * W, "light-phase tables fit in the block's words");
+  Light& L = *reinterpret_cast<Light*>(lds_main);
+
+  MsCtl* c = a.ctl;
+  const int64_t threshold = c->threshold;
+  const uint32_t tag0 = c->tag;
+  const int nsrc = (int)c->nsrc;
+  int64_t fcount = c->fcount, fdeg = c->fdeg;
+  ull allbits[W];
+  batch_bits<W>(nsrc, allbits);
+  const unsigned int nwg = gridDim.x;
+  unsigned int gen = 0;
+  const int64_t wave_id = (int64_t)blockIdx.x * NW + threadIdx.x / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * NW;
+  const int64_t gtid = (int64_t)blockIdx.x * THREADS + threadIdx.x;
+  const int64_t gthreads = (int64_t)gridDim.x * THREADS;
+  int64_t lev = 0, dense_levels = 0, sparse_levels = 0;
+  for (;; ++lev) {
+    const int cur = (int)(lev & 1);
+    a.front_cur = p.front[cur];
+    a.front_nxt = p.front[cur ^ 1];
+    a.active_cur = p.active[cur];
+    a.active_nxt = p.active[cur ^ 1];
+    a.list_cur = p.list[cur];
+    a.list_nxt = p.list[cur ^ 1];
+    a.lvl_new = p.lvl_new[cur];
+    MsSlot* out = &p.pc->slot[lev % 3];
+    if (blockIdx.x == 0) {
+      if (lev > 0) {
+        // The previous level's counts (flushed before the barrier that closed it).
+        ull* prev = p.lvl_new[cur ^ 1];
+        for (int s = threadIdx.x; s < nsrc; s += THREADS) {
+          const ull k = prev[s];
+          if (k) {
+            p.reached[s] += k;
+            p.depth[s] = lev;
+            prev[s] = 0;
+          }
+        }
+      } else {
+        for (int s = threadIdx.x; s < nsrc; s += THREADS) p.lvl_new[1][s] = 0;
+      }
+      if (threadIdx.x < 8) reinterpret_cast<int64_t*>(&p.pc->slot[(lev + 1) % 3])[threadIdx.x] = 0;
+    }
+    if (fcount == 0) break;
+    const bool dense = fcount + fdeg > threshold;
+    if (dense) ++dense_levels;
+    else ++sparse_levels;
+    const uint32_t level = (uint32_t)lev + 1;
+    EdgeArgs ea{};
+    ea.off = a.off;
+    ea.nbr = p.nbr;
+    ea.ids = a.list_cur;
+    ea.count = fcount;
+    ea.out_ids = a.list_nxt;
+    ea.out_info = &out->ncount;
+    ea.heavy = p.heavy;
+    ea.heavy_count = &out->heavy_count;
+    ea.heavy_deg = p.heavy_deg;
+    WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+    stats_init<W, THREADS>(st);
+    if (dense) {
+      ms_dense_units<W, VB, THREADS>(a, p.nunits, &out->unit_next, level, allbits, lds_main, full, fullw, st, q,
+                                     ea, s_unit);
+      if (!grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok)) break;
+      for (int64_t item = blockIdx.x; item < p.nmulti * PER; item += gridDim.x)
+        ms_merge_vertex<W, VB>(a, st, q, ea, a.multi[item / PER], (int)(item % PER) * THREADS + (int)threadIdx.x,
+                               level, allbits);
+      wq_finish(q, ea);
+    } else {
+      __syncthreads();
+      const Op op{a.front_cur, a.seen, a.next, p.stamp, nullptr, nullptr, tag0 + (uint32_t)lev};
+      light_phase<Op, SRC_LIST, HUB_NONE, NW>(op, ea, nullptr, nullptr, q, L, fcount, wave_id, nwaves, false,
+                                               (uint64_t)p.heavy_deg);
+      if (!grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok)) break;
+      heavy_items<Op, HUB_NONE>(op, ea, nullptr, nullptr, q, p.heavy, wave_id, out->heavy_count, nwaves);
+      wq_finish(q, ea);
+      if (!grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok)) break;
+      ms_sparse_fin<W>(a, st, out->ncount, level, gtid, gthreads);
+    }
+    ms_clear<W>(a, fcount, p.nwords, gtid, gthreads);
+    stats_flush<W, THREADS>(a, st);
+    if (!grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok)) break;
+    fcount = out->ncount;
+    fdeg = out->ndeg;
+  }
+  // Every workgroup has read the control block it starts from before the
+  // closing values go in.
+  (void)grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c->level = lev;
+    c->tag = tag0 + (uint32_t)lev;
+    c->fcount = 0;
+    c->fdeg = 0;
+    c->dense_levels = dense_levels;
+    c->sparse_levels = sparse_levels;
+    c->mode = MS_DONE;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host side.
 
 // Per-context state, grown on demand.  Between batches every buffer except
@@ -547,7 +814,8 @@ __global__ void k_ms_init(MsArgs a, int W, const uint32_t* sources, int64_t nuni
 // failed batch.  The word buffers are sized for the widest batch so far
 // (a narrower batch uses their first n * W words).
 struct MsState {
-  DevBuf seen, front[2], next, stamp, active[2], list[2], part, stats, ctl, srcs;
+  DevBuf seen, front[2], next, stamp, active[2], list[2], part, stats, ctl, srcs, pctl;
+  int persist_grid[3] = {0, 0, 0};   // resident workgroups of k_ms_persist per word width (0: not yet asked)
   int64_t n = 0;
   int wcap = 0;
   uint32_t tag = 0;
@@ -561,7 +829,7 @@ int64_t ms_state_bytes(const void* p) {
   if (!S) return 0;
   int64_t b = 0;
   for (const DevBuf* d : {&S->seen, &S->front[0], &S->front[1], &S->next, &S->stamp, &S->active[0], &S->active[1],
-                          &S->list[0], &S->list[1], &S->part, &S->stats, &S->ctl, &S->srcs})
+                          &S->list[0], &S->list[1], &S->part, &S->stats, &S->ctl, &S->srcs, &S->pctl})
     b += (int64_t)d->bytes;
   return b;
 }
@@ -600,8 +868,10 @@ int ms_prepare(rg_ctx* ctx, MsState& S, int64_t n, int W, int64_t nunits) {
     S.part.release();
     RG_TRY(S.part.ensure(pb));
   }
-  RG_TRY(S.stats.ensure(sizeof(ull) * 5 * kMsMaxSources));
+  // lvl_new, edges, cs, reached, depth, and lvl_new of odd levels (persistent launch)
+  RG_TRY(S.stats.ensure(sizeof(ull) * 6 * kMsMaxSources));
   RG_TRY(S.ctl.ensure(sizeof(MsCtl)));
+  RG_TRY(S.pctl.ensure(sizeof(MsPersistCtl) + sizeof(unsigned int) * kFlagStride * 4 * 1024));
   RG_TRY(S.srcs.ensure(sizeof(uint32_t) * kMsMaxSources));
   if (S.dirty || S.tag > 0x7fffffffu) {
     for (DevBuf* b : {&S.front[0], &S.front[1], &S.next, &S.active[0], &S.active[1]}) RG_TRY(zero<ull>(ctx, *b));
@@ -612,9 +882,75 @@ int ms_prepare(rg_ctx* ctx, MsState& S, int64_t n, int W, int64_t nunits) {
   return RG_OK;
 }
 
+// Runs the batch that k_ms_init set up as one cooperative launch of
+// k_ms_persist: as many workgroups as the device holds at once (the launch
+// fails rather than runs if they cannot all be resident).
+template <int W>
+int ms_persist(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const MsArgs& a, ull* reached,
+               int64_t* depth, ull* lvl_new_odd, int64_t* levels_run) {
+  rg_ctx* ctx = g->ctx;
+  constexpr int T = ms_threads<W>();
+  auto kern = k_ms_persist<W, kMsVB, T>;
+  int& grid = S.persist_grid[W == 1 ? 0 : W == 2 ? 1 : 2];
+  if (grid == 0) {
+    int coop = 0;
+    RG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device));
+    if (!coop) return set_error(RG_EUNSUPPORTED, "persistent batch: no cooperative launch on this device");
+    int per_cu = 0;
+    RG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, T, 0));
+    if (per_cu < 1) return set_error(RG_EUNSUPPORTED, "persistent batch: the kernel does not fit a CU");
+    grid = per_cu * ctx->num_cus;
+  }
+  int rate_khz = 0;
+  RG_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (grid > 4 * 1024) return set_error(RG_EUNSUPPORTED, "persistent batch: more than 4096 resident workgroups");
+  MsPersistArgs p{};
+  p.pc = S.pctl.as<MsPersistCtl>();
+  p.flags = reinterpret_cast<unsigned int*>(p.pc + 1);
+  p.nbr = g->nbr;
+  for (int i = 0; i < 2; ++i) {
+    p.front[i] = S.front[i].as<ull>();
+    p.active[i] = S.active[i].as<ull>();
+    p.list[i] = S.list[i].as<uint32_t>();
+  }
+  p.lvl_new[0] = a.lvl_new;
+  p.lvl_new[1] = lvl_new_odd;
+  p.reached = reached;
+  p.depth = depth;
+  p.stamp = S.stamp.as<uint32_t>();
+  p.heavy = E.heavy;
+  p.heavy_deg = E.heavy_deg;
+  p.nunits = B->nunits;
+  p.nmulti = B->nmulti;
+  p.nwords = g->n / 64 + 1;
+  p.spin_ticks = (uint64_t)std::max(rate_khz, 1) * 1000ull * 10ull;   // a barrier wait of 10 s
+  RG_TRY(fill<ull>(ctx, reinterpret_cast<ull*>(p.pc),
+                   (int64_t)((sizeof(MsPersistCtl) + sizeof(unsigned int) * kFlagStride * grid) / sizeof(ull)), 0ull));
+  MsArgs ka = a;
+  void* args[] = {&ka, &p};
+  // One persistent batch on the device at a time: two cooperative grids
+  // from different contexts, each partly resident, would wait on each
+  // other's workgroups until the barrier limit.  The lock is held until
+  // the launch has ended (the reads below wait for it).
+  static std::mutex persist_mu;
+  std::lock_guard<std::mutex> lock(persist_mu);
+  RG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(T), args, 0, ctx->stream));
+  int64_t bar[1] = {0};   // GridBar: release | abort
+  int64_t lev = 0;
+  RG_TRY(ctx_read_count(ctx, reinterpret_cast<const int64_t*>(p.pc), bar, 1));
+  RG_TRY(ctx_read_count(ctx, &a.ctl->level, &lev, 1));
+  if ((uint32_t)((uint64_t)bar[0] >> 32) != 0)
+    return set_error(RG_EDEVICE, "persistent batch: a grid barrier wait ran past its limit");
+  S.tag += (uint32_t)lev + 1;
+  S.dirty = false;
+  if (levels_run) *levels_run = lev;
+  return RG_OK;
+}
+
 template <int W>
 int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32_t* sources, int nsrc,
-           int64_t threshold, int32_t* levels_dev, int32_t* radii_dev, int64_t* levels_run, MsLevelHook* hook) {
+           int64_t threshold, int32_t* levels_dev, int32_t* radii_dev, int64_t* levels_run, MsLevelHook* hook,
+           bool persist) {
   rg_ctx* ctx = g->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = g->n;
@@ -662,6 +998,8 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   hipLaunchKernelGGL(k_ms_init, 1, 256, 0, st, a, W, S.srcs.as<uint32_t>(), (int64_t)uniq.size(), threshold,
                      reached, depth, S.tag + 1u);
   RG_HIP(hipGetLastError());
+  if (persist && !hook)
+    return ms_persist<W>(g, S, B, E, a, reached, depth, stats + 5 * kMsMaxSources, levels_run);
 
   EdgeArgs ea{};
   ea.off = g->off;
@@ -785,6 +1123,22 @@ int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc) {
 
 namespace {
 
+// Launch form of a batch (rg_opts::batch_launch): 1 = per-level kernels,
+// 2 = one persistent launch; 0 = the library's choice: the persistent
+// launch from 2^27 arcs.  Measured on MI355X (device time per batch of 64 /
+// 256 sources, per-level vs persistent): symmetric R-MAT of 2^18 vertices
+// 1.51 / 2.68 vs 1.83 / 2.75 ms, 2^22: 2.64 / 5.55 vs 2.89 / 5.53 ms, 2^26:
+// 27.4 / 81.5 vs 25.4 / 79.0 ms -- a grid barrier (~14 us) costs more than
+// a kernel boundary inside a HIP graph (~1.6 us), which small graphs feel,
+// while on large ones the single launch saves the host checks and its
+// dense phase runs without the per-level kernel's register spills.
+bool batch_persistent(const rg_graph* g, const rg_opts* o) {
+  const int mode = o ? o->batch_launch : 0;
+  if (mode == 1) return false;
+  if (mode == 2) return true;
+  return g->m >= ((int64_t)1 << 27);
+}
+
 // Runs one batch; per-source summaries into out (nullable), level arrays or
 // radii into the device buffers given (nullable).
 int batch_run(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_dev,
@@ -812,9 +1166,10 @@ int batch_run(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
   RG_TRY(ms_prepare(ctx, *S, n, W, B->nunits));
   EventTimer tm(ctx->stream);
   int64_t steps = 0;
-  int rc = W == 1   ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook)
-           : W == 2 ? ms_run<2>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook)
-                    : ms_run<4>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook);
+  const bool persist = !hook && batch_persistent(g, o);
+  int rc = W == 1   ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook, persist)
+           : W == 2 ? ms_run<2>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook, persist)
+                    : ms_run<4>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook, persist);
   if (rc != RG_OK) return rc;
   double ms = tm.stop_ms();
   if (out) {

@@ -323,12 +323,20 @@ bool take_work(Shared& S, const rg_serve_config& c, int slot, int* kind, std::ve
   }
 }
 
+// Batches of several slots overlap on the device only as per-level kernels
+// (one persistent batch runs at a time), so with more than one slot the
+// library's choice of launch form becomes per-level kernels.
+void serve_batch_launch(const rg_serve_config& c, rg_opts& qo) {
+  if (qo.batch_launch == 0 && c.slots > 1) qo.batch_launch = 1;
+}
+
 int run_work(Slot& sl, const rg_serve_config& c, const rg_query* q, int kind, const std::vector<int64_t>& work,
              rg_query_result* res) {
   rg_graph* g = sl.view;
   rg_opts o = c.opts;
   rg_opts qo = c.opts;   // BFS / BC: the push engine, RADAR hub copies or blocked dense BC levels
   if (qo.variant == RG_PULL) qo.variant = RG_PUSH;
+  serve_batch_launch(c, qo);
   const size_t k = work.size();
   std::vector<rg_bfs_summary> sum(k, rg_bfs_summary{0, 0, 0, 0});
   std::vector<int64_t> iters(k, 0);
@@ -388,6 +396,7 @@ int calibrate(Shared& S, Slot& sl, const rg_serve_config& c, const rg_query* q, 
   rg_graph* g = sl.view;
   rg_opts qo = c.opts;
   if (qo.variant == RG_PULL) qo.variant = RG_PUSH;
+  serve_batch_launch(c, qo);
   for (int kind : {(int)RG_Q_BFS, (int)RG_Q_BC}) {
     std::vector<uint32_t> srcs;
     for (int64_t i = 0; i < nq; ++i)

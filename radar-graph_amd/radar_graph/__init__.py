@@ -54,7 +54,7 @@ class Opts(C.Structure):
                 ("threshold", C.c_int64), ("wg_size", C.c_int), ("heavy_degree", C.c_int),
                 ("l2_hubs", C.c_int64), ("dst_block_bits", C.c_int), ("src_segment_bits", C.c_int),
                 ("unroll", C.c_int), ("hub_min_arcs", C.c_int64), ("unit_order", C.c_int),
-                ("unit_queue", C.c_int)]
+                ("unit_queue", C.c_int), ("batch_launch", C.c_int)]
 
 
 class BfsSummary(C.Structure):
@@ -213,7 +213,7 @@ def _ptr(a: np.ndarray, t):
 
 def opts(variant="push", hubs=0, threshold=-1, wg_size=0, heavy_degree=0, hub_placement=HUBS_LDS,
          l2_hubs=0, dst_block_bits=0, src_segment_bits=0, unroll=0, hub_min_arcs=0,
-         unit_order=0, unit_queue=0) -> Opts:
+         unit_order=0, unit_queue=0, batch_launch=0) -> Opts:
     o = Opts()
     lib.rg_opts_default(C.byref(o))
     o.variant = VARIANTS[variant] if isinstance(variant, str) else int(variant)
@@ -229,6 +229,7 @@ def opts(variant="push", hubs=0, threshold=-1, wg_size=0, heavy_degree=0, hub_pl
     o.hub_min_arcs = int(hub_min_arcs)
     o.unit_order = int(unit_order)
     o.unit_queue = int(unit_queue)
+    o.batch_launch = int(batch_launch)
     return o
 
 

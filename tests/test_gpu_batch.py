@@ -122,3 +122,80 @@ def test_batch_rejects_bad_input(rg, ctx, graphs):
         dg.bfs_batch(np.zeros(257, np.uint32))
     with pytest.raises(rg.RGError):
         dg.bfs_batch(np.array([hg.n], np.uint32))
+
+
+# One persistent launch for the whole batch (rg_opts.batch_launch = 2): the
+# same answers as the oracle and as the per-level kernels (batch_launch = 1).
+
+PERSIST, LEVELS = 2, 1
+
+
+@pytest.mark.parametrize("name", ["rmat12s", "rmat12d", "er12s"])
+@pytest.mark.parametrize("k", [1, 7, 64, 65, 130, 256])
+def test_persistent_batch_levels_match_oracle(rg, orc, ctx, graphs, name, k):
+    hg = graphs(name)
+    dg = rg.DeviceGraph(ctx, hg)
+    check_batch(rg, orc, hg, dg, pick(hg.n, k, 300 + k), rg.opts(batch_launch=PERSIST))
+
+
+@pytest.mark.parametrize("threshold", [0, BIG])
+@pytest.mark.parametrize("k", [64, 200])
+def test_persistent_batch_all_dense_or_all_sparse(rg, orc, ctx, graphs, threshold, k):
+    hg = graphs("rmat12d")
+    dg = rg.DeviceGraph(ctx, hg)
+    check_batch(rg, orc, hg, dg, pick(hg.n, k, 9), rg.opts(threshold=threshold, batch_launch=PERSIST))
+
+
+def test_persistent_batch_split_hub_block(rg, orc, ctx):
+    # The star of test_batch_split_hub_block: split blocks merged between
+    # two barriers of the one launch; its hub's out-list is a heavy item on
+    # sparse levels.
+    leaves = 1 << 20
+    src = np.zeros(leaves, np.uint32)
+    dst = np.arange(1, leaves + 1, dtype=np.uint32)
+    src = np.concatenate([src, dst[:1000]])
+    dst = np.concatenate([dst, dst[1000:2000]])
+    hg = rg.HostGraph.from_edges(leaves + 1, src, dst, symmetric=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    srcs = np.array([0, 1, 5, 1500, 999, 77777, leaves], np.uint32)
+    for threshold in (0, -1, BIG):
+        check_batch(rg, orc, hg, dg, srcs, rg.opts(threshold=threshold, batch_launch=PERSIST))
+
+
+def test_persistent_equals_per_level_launches(rg, orc, ctx, graphs):
+    # Summaries and level arrays of both launch forms, alternated on one
+    # context (each form leaves the state the other starts from).
+    hg = graphs("rmat14d")
+    dg = rg.DeviceGraph(ctx, hg)
+    for k, seed in ((96, 21), (256, 22), (3, 23), (130, 24)):
+        srcs = pick(hg.n, k, seed)
+        a, la, _ = dg.bfs_batch(srcs, rg.opts(batch_launch=LEVELS), want_levels=True)
+        b, lb, _ = dg.bfs_batch(srcs, rg.opts(batch_launch=PERSIST), want_levels=True)
+        assert [x.astuple() for x in a] == [x.astuple() for x in b], k
+        assert np.array_equal(la, lb), k
+
+
+def test_persistent_batch_degree_sorted(rg, orc, ctx, graphs):
+    hg = graphs("rmat16s")
+    dg = rg.DeviceGraph(ctx, hg)
+    sg, ids = dg.degree_sort(sort_neighbors=True)
+    srcs = ids[hg.pick_sources(200)]
+    a, _, _ = sg.bfs_batch(srcs, rg.opts(batch_launch=LEVELS))
+    b, _, _ = sg.bfs_batch(srcs, rg.opts(batch_launch=PERSIST))
+    assert [x.astuple() for x in a] == [x.astuple() for x in b]
+    for i in (0, 77, 199):
+        single, _ = sg.bfs_query(int(srcs[i]))
+        assert b[i].astuple() == single.astuple()
+
+
+@pytest.mark.parametrize("threshold", [0, -1, BIG])
+def test_persistent_radii(rg, orc, ctx, graphs, threshold):
+    # Radii as one bit-parallel batch in one launch: exact radii and rounds.
+    for name in ("rmat12d", "rmat12s"):
+        hg = graphs(name)
+        off, nbr = hg.csr()
+        dg = rg.DeviceGraph(ctx, hg)
+        starts = rg.radii_starts(hg.n)
+        rad, rounds, _ = dg.radii(starts, rg.opts("blocked", threshold=threshold, batch_launch=PERSIST))
+        ref, rr = orc.radii(off, nbr, starts)
+        assert rounds == rr and np.array_equal(rad, ref), name

@@ -254,7 +254,9 @@ typedef struct rg_opts {
                         /* (chunks of four levels replayed as HIP graphs, one */
                         /* host check per chunk); 2 = the whole batch as one  */
                         /* persistent cooperative launch (grid barriers       */
-                        /* between phases); 0 = the library's choice          */
+                        /* between phases); 0 = the library's choice.         */
+                        /* rg_bc_batch: 1 keeps batches of fewer than 12      */
+                        /* sources bit-parallel (0: single-source runs)       */
 } rg_opts;
 
 /* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */
@@ -356,8 +358,10 @@ typedef struct rg_bc_summary {
  * rg_bfs_batch (source i owns bit i); path counts and dependencies are
  * summed per vertex by one wavefront with a lane per source, with no
  * atomics but for vertices of degree above 4096.  A batch with a source
- * deeper than 63 levels is answered by single-source runs.  No counterpart
- * in the reference. */
+ * deeper than 63 levels is answered by single-source runs, and so is a last
+ * batch of fewer than 12 sources unless o->batch_launch is 1 (such a small
+ * batch costs more than its single-source runs).  No counterpart in the
+ * reference. */
 int rg_bc_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
                 rg_bc_summary* out, double* ms_out);
 

@@ -47,6 +47,12 @@ namespace rg {
 
 constexpr int kBcbMaxSources = 64;    // sources per batch (one word of source bits)
 constexpr int kBcbMaxBatch = 256;     // sources per call, in batches of 64
+// A batch of fewer sources costs more than running them one by one: the
+// bit-parallel forward and the per-level gathers scan every arc whatever
+// the width (symmetric R-MAT of 2^25 vertices, RADAR engine: a batch of 1
+// / 37 / 64 takes 47 / 212 / 256 ms against 8.8 ms per single-source run;
+// 2^26: 1 / 8 / 64 take 99 / 273 / 489 ms against 22 ms).
+constexpr int kBcbSingleBelow = 12;
 constexpr int kBcbMaxLevels = 64;      // levels 0..63 kept; deeper sources take single-source runs
 constexpr int kBcbLevelsInit = 32;     // level words allocated at first (grown on demand)
 constexpr uint64_t kBcbHeavy = 4096;   // degree above which edges are split into chunks
@@ -628,8 +634,35 @@ int bc_batch_reserve(rg_graph* g) {
   return bfs_batch_reserve(g, kBcbMaxSources);
 }
 
+// Fewer than kBcbSingleBelow sources as single-source runs (rg_bc with one
+// source each, the orientation restored after it): the vertices a source
+// reaches are those whose dependency is finite.
+int bc_small_single(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_out,
+                    rg_bc_summary* out) {
+  rg_ctx* ctx = g->ctx;
+  const int64_t n = g->n;
+  for (int s = 0; s < nsrc; ++s) {
+    const bool tr = g->transposed;
+    int64_t rounds = 0;
+    double ms = 0.0;
+    RG_TRY(bc_push(g, &sources[s], 1, o, deps_out ? deps_out + (size_t)s * n : nullptr, &rounds, &ms));
+    if (g->transposed != tr) RG_TRY(rg_graph_transpose(g));
+    double w = 0.0;
+    int64_t nf = 0;
+    RG_TRY(fp_summary(ctx, ctx->last_result, n, &w, &nf));
+    out[s].rounds = rounds;
+    out[s].reached = n - nf;
+    out[s].wsum = w;
+    out[s].nonfinite = nf;
+  }
+  return RG_OK;
+}
+
 // 1..256 sources: batches of 64 in order, the last one uneven (a batch
 // costs far less than 64 single runs at any size, so full batches first).
+// With the library's choice of launch form (rg_opts::batch_launch 0), a
+// last batch of fewer than kBcbSingleBelow sources runs one source at a
+// time; batch_launch 1 keeps every batch bit-parallel.
 int bc_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, double* deps_out,
              rg_bc_summary* out, double* ms_out) {
   rg_ctx* ctx = g->ctx;
@@ -658,7 +691,12 @@ int bc_batch(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, d
     double* host = deps_out ? deps_out + (size_t)c0 * n : nullptr;
     EventTimer tm(ctx->stream);
     bool on_host = false;
-    RG_TRY(bc_batch64(g, *S, sources + c0, k, o, deps_dev, host, out + c0, &on_host));
+    if (k < kBcbSingleBelow && !(o && o->batch_launch == 1)) {
+      on_host = true;
+      RG_TRY(bc_small_single(g, sources + c0, k, o, host, out + c0));
+    } else {
+      RG_TRY(bc_batch64(g, *S, sources + c0, k, o, deps_dev, host, out + c0, &on_host));
+    }
     ms += tm.stop_ms();
     if (deps_out && !on_host)
       RG_TRY(ctx_copy(ctx, host, deps_dev, sizeof(double) * (size_t)k * n, hipMemcpyDeviceToHost));

@@ -1,5 +1,6 @@
 """GPU tests of the batched single-source BC (rg_bc_batch, 1..256 sources
-in batches of 64): every row of
+in batches of 64, a last batch under 12 sources one at a time unless
+batch_launch is 1): every row of
 the batch's dependencies equals the oracle's single-source BC for that
 source within 1e-10 relative L-inf (NaN at exactly the unreached vertices),
 and every summary equals the one computed from the oracle's array (rounds,
@@ -48,12 +49,15 @@ def check(rg, orc, hg, dg, srcs, o=None):
     return out, deps
 
 
+# launch 0: the library's choice (batches under 12 sources run one source
+# at a time); launch 1: every batch on the bit-parallel engine.
+@pytest.mark.parametrize("launch", [0, 1])
 @pytest.mark.parametrize("name", ["rmat12s", "rmat12d", "er12s", "er12d"])
-@pytest.mark.parametrize("k", [1, 17, 64])
-def test_bc_batch_matches_oracle(rg, orc, ctx, graphs, name, k):
+@pytest.mark.parametrize("k", [1, 5, 17, 64])
+def test_bc_batch_matches_oracle(rg, orc, ctx, graphs, name, k, launch):
     hg = graphs(name)
     dg = rg.DeviceGraph(ctx, hg)
-    check(rg, orc, hg, dg, pick(hg.n, k, 300 + k))
+    check(rg, orc, hg, dg, pick(hg.n, k, 300 + k), rg.opts(batch_launch=launch))
 
 
 @pytest.mark.parametrize("threshold", [0, BIG])
@@ -89,8 +93,9 @@ def test_bc_batch_heavy_vertices(rg, orc, ctx, symmetric):
     hg = rg.HostGraph.from_edges(leaves + 2, src, dst, symmetric=symmetric)
     dg = rg.DeviceGraph(ctx, hg)
     srcs = np.array([hub_out, hub_in, 5, 4000, 2999, 17000], np.uint32)
+    check(rg, orc, hg, dg, srcs, rg.opts(batch_launch=1))
+    check(rg, orc, hg, dg, srcs, rg.opts(threshold=0, batch_launch=1))
     check(rg, orc, hg, dg, srcs)
-    check(rg, orc, hg, dg, srcs, rg.opts(threshold=0))
 
 
 @pytest.mark.parametrize("length", [64, 65, 100])
@@ -101,7 +106,7 @@ def test_bc_batch_deep_levels(rg, orc, ctx, length):
     a = np.arange(length - 1, dtype=np.uint32)
     hg = rg.HostGraph.from_edges(length, a, a + 1, symmetric=True)
     dg = rg.DeviceGraph(ctx, hg)
-    check(rg, orc, hg, dg, np.array([0, length // 2, length - 1], np.uint32))
+    check(rg, orc, hg, dg, np.array([0, length // 2, length - 1], np.uint32), rg.opts(batch_launch=1))
 
 
 def test_bc_batch_deep_then_regrown_batch(rg, orc, ctx):
@@ -114,15 +119,17 @@ def test_bc_batch_deep_then_regrown_batch(rg, orc, ctx):
     dg = rg.DeviceGraph(ctx, hg)
     srcs = np.concatenate([np.array([0], np.uint32), np.arange(10, 73, dtype=np.uint32),
                            np.full(6, 40, np.uint32)])
-    check(rg, orc, hg, dg, srcs)
+    check(rg, orc, hg, dg, srcs, rg.opts(batch_launch=1))
 
 
+@pytest.mark.parametrize("launch", [0, 1])
 @pytest.mark.parametrize("k", [65, 130, 256])
-def test_bc_batch_uneven_batches(rg, orc, ctx, graphs, k):
-    # More than 64 sources: batches of 64 in order, the last one uneven.
+def test_bc_batch_uneven_batches(rg, orc, ctx, graphs, k, launch):
+    # More than 64 sources: batches of 64 in order, the last one uneven
+    # (with launch 0, a last batch of 1 or 2 sources runs one at a time).
     hg = graphs("rmat10s")
     dg = rg.DeviceGraph(ctx, hg)
-    check(rg, orc, hg, dg, pick(hg.n, k, 40 + k))
+    check(rg, orc, hg, dg, pick(hg.n, k, 40 + k), rg.opts(batch_launch=launch))
 
 
 def test_bc_batch_bigger_graph(rg, orc, ctx, graphs):
@@ -138,9 +145,10 @@ def test_bc_batch_repeated_calls(rg, orc, ctx, graphs):
     small = graphs("rmat12d")
     d1 = rg.DeviceGraph(ctx, big)
     d2 = rg.DeviceGraph(ctx, small)
-    check(rg, orc, big, d1, pick(big.n, 64, 1))
-    check(rg, orc, small, d2, pick(small.n, 5, 2))
-    check(rg, orc, big, d1, pick(big.n, 3, 3))
+    o = rg.opts(batch_launch=1)
+    check(rg, orc, big, d1, pick(big.n, 64, 1), o)
+    check(rg, orc, small, d2, pick(small.n, 5, 2), o)
+    check(rg, orc, big, d1, pick(big.n, 3, 3), o)
 
 
 def test_bc_batch_errors(rg, ctx, graphs):

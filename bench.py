@@ -358,7 +358,7 @@ def profile_run(rg, args):
     """One app on one engine, repeated, for counter collection under
     rocprofv3: --profile app:variant:scale[d|s][:key=value,...] (app bfs,
     query (BFS summary only), batch (--batch sources, degree-sorted graph),
-    pagerank, radii or bc; variant push, hubdup, radar, blocked or pull; d =
+    bcbatch (--batch BC queries, degree-sorted graph), pagerank, radii or bc; variant push, hubdup, radar, blocked or pull; d =
     directed, s = symmetric; radar, blocked, batch and query run on the
     degree-sorted graph; graphs=0 turns HIP-graph replay off; other options
     are rg.opts keywords, e.g. hubs=1073741824, hub_min_arcs=-1)."""
@@ -376,7 +376,7 @@ def profile_run(rg, args):
     ctx.graphs(bool(graphs_on))
     dg = rg.DeviceGraph(ctx, hg)
     ids = None
-    if variant in ("radar", "blocked") or app in ("batch", "query"):
+    if variant in ("radar", "blocked") or app in ("batch", "query", "bcbatch"):
         g, ids = dg.degree_sort(sort_neighbors=True)
     else:
         g = dg
@@ -392,6 +392,8 @@ def profile_run(rg, args):
             return g.bfs_query(int(srcs[i]), o)[1]
         if app == "batch":   # --batch sources as one bit-parallel batch (the variant is unused)
             return g.bfs_batch(srcs[:args.batch], o)[2]
+        if app == "bcbatch":   # --batch BC queries (rg_bc_batch) with the variant's single-source engine
+            return g.bc_batch(srcs[:args.batch], o)[2]
         if app == "pagerank":
             return g.pagerank(args.iters, o, want_ranks=False).ms
         if app == "radii":

@@ -255,8 +255,11 @@ typedef struct rg_opts {
                         /* host check per chunk); 2 = the whole batch as one  */
                         /* persistent cooperative launch (grid barriers       */
                         /* between phases); 0 = the library's choice.         */
-                        /* rg_bc_batch: 1 keeps batches of fewer than 12      */
-                        /* sources bit-parallel (0: single-source runs)       */
+                        /* rg_bc_batch: 0 = per-level kernels, a last batch   */
+                        /* of fewer than 12 sources as single-source runs;    */
+                        /* 1 = per-level kernels for every batch; 2 = each    */
+                        /* batch (forward and backward) as one persistent     */
+                        /* launch                                             */
 } rg_opts;
 
 /* Fill `o` with the defaults (RG_PUSH, threshold -1, library-chosen k). */
@@ -359,9 +362,10 @@ typedef struct rg_bc_summary {
  * summed per vertex by one wavefront with a lane per source, with no
  * atomics but for vertices of degree above 4096.  A batch with a source
  * deeper than 63 levels is answered by single-source runs, and so is a last
- * batch of fewer than 12 sources unless o->batch_launch is 1 (such a small
- * batch costs more than its single-source runs).  No counterpart in the
- * reference. */
+ * batch of fewer than 12 sources when o->batch_launch is 0 (such a small
+ * batch costs more than its single-source runs).  o->batch_launch 2 runs
+ * each batch, forward and backward, as one persistent launch.  No
+ * counterpart in the reference. */
 int rg_bc_batch(rg_graph* g, const uint32_t* sources, int nsources, const rg_opts* o, double* deps_out,
                 rg_bc_summary* out, double* ms_out);
 

@@ -37,6 +37,7 @@
 #include <mutex>
 #include <vector>
 
+#include "bcbatch.h"
 #include "blocked.h"
 #include "device.h"
 #include "engine.h"
@@ -607,7 +608,8 @@ struct MsSlot {
   int64_t ncount, ndeg;
   int64_t heavy_count;
   unsigned long long unit_next;
-  int64_t pad[4];
+  int64_t hook_items;           // BC forward: heavy chunk items of the path-count gathers
+  int64_t pad[3];
 };
 
 struct MsPersistCtl {
@@ -648,8 +650,8 @@ __device__ __forceinline__ bool spin_until(const unsigned int* p, unsigned int w
   return true;
 }
 
-// Grid-wide barrier.  The workgroup's threads order their writes at
-// workgroup scope, then one thread writes the XCD's L2 back (device-scope
+// Grid-wide barrier.  The workgroup's waves wait for their stores to
+// reach the L2, then one thread writes the XCD's L2 back (device-scope
 // release fence) and stores the workgroup's arrival flag.  Workgroup 0
 // polls every flag with its threads in parallel and stores the release
 // word; each workgroup's first thread waits for it and invalidates its
@@ -661,6 +663,11 @@ __device__ __forceinline__ bool spin_until(const unsigned int* p, unsigned int w
 // been abandoned.
 __device__ __forceinline__ bool grid_sync(GridBar* b, unsigned int* flags, unsigned int nwg, unsigned int& gen,
                                           uint64_t spin_ticks, int& s_ok) {
+  // Every wave waits until its own global stores have reached the L2: a
+  // workgroup-scope release does not wait for them (the workgroup's waves
+  // share one CU), and the one L2 write-back below would otherwise miss
+  // stores still in flight from other waves.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __syncthreads();
   const unsigned int g = gen + 1;
@@ -689,8 +696,18 @@ __device__ __forceinline__ bool grid_sync(GridBar* b, unsigned int* flags, unsig
   return s_ok != 0;
 }
 
-template <int W, int VB, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs p) {
+// A batch without per-level work beyond the traversal (BFS queries, Radii).
+struct MsNoHook {
+  static constexpr bool kOn = false;
+};
+
+// HOOK = BcPersistHook (one-word batches): at every level, once the next
+// frontier is complete and while the current one is intact, the BC forward
+// phases (the next level's words and path counts, then the heavy chunks)
+// between two more barriers; after the last level, the BC backward levels
+// (select, dependencies, heavy chunks) with three barriers each.
+template <int W, int VB, int THREADS, class HOOK>
+__global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs p, HOOK hk) {
   constexpr int BV = 1 << VB;
   constexpr int NW = THREADS / kWave;
   constexpr int PER = BV / THREADS;
@@ -705,7 +722,10 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
   __shared__ MsStats<W> st;
   __shared__ int64_t s_unit;
   __shared__ int s_ok;
+  __shared__ double s_w[HOOK::kOn ? 64 : 1];   // BC backward: the workgroup's per-source sums
+  __shared__ ull s_nf[HOOK::kOn ? 64 : 1];
   static_assert(sizeof(Light) <= sizeof(ull) * BV * W, "light-phase tables fit in the block's words");
+  static_assert(!HOOK::kOn || W == 1, "BC batches use one word of source bits");
   Light& L = *reinterpret_cast<Light*>(lds_main);
 
   MsCtl* c = a.ctl;
@@ -722,6 +742,8 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
   const int64_t gtid = (int64_t)blockIdx.x * THREADS + threadIdx.x;
   const int64_t gthreads = (int64_t)gridDim.x * THREADS;
   int64_t lev = 0, dense_levels = 0, sparse_levels = 0;
+  bool ok = true;
+  auto sync = [&]() { return grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok); };
   for (;; ++lev) {
     const int cur = (int)(lev & 1);
     a.front_cur = p.front[cur];
@@ -769,7 +791,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
     if (dense) {
       ms_dense_units<W, VB, THREADS>(a, p.nunits, &out->unit_next, level, allbits, lds_main, full, fullw, st, q,
                                      ea, s_unit);
-      if (!grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok)) break;
+      if (!(ok = sync())) break;
       for (int64_t item = blockIdx.x; item < p.nmulti * PER; item += gridDim.x)
         ms_merge_vertex<W, VB>(a, st, q, ea, a.multi[item / PER], (int)(item % PER) * THREADS + (int)threadIdx.x,
                                level, allbits);
@@ -779,21 +801,77 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
       const Op op{a.front_cur, a.seen, a.next, p.stamp, nullptr, nullptr, tag0 + (uint32_t)lev};
       light_phase<Op, SRC_LIST, HUB_NONE, NW>(op, ea, nullptr, nullptr, q, L, fcount, wave_id, nwaves, false,
                                                (uint64_t)p.heavy_deg);
-      if (!grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok)) break;
+      if (!(ok = sync())) break;
       heavy_items<Op, HUB_NONE>(op, ea, nullptr, nullptr, q, p.heavy, wave_id, out->heavy_count, nwaves);
       wq_finish(q, ea);
-      if (!grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok)) break;
+      if (!(ok = sync())) break;
       ms_sparse_fin<W>(a, st, out->ncount, level, gtid, gthreads);
+    }
+    if constexpr (HOOK::kOn) {
+      if (!(ok = sync())) break;
+      const int64_t L1 = lev + 1;
+      const int64_t ncount = out->ncount;
+      if (L1 >= hk.b.lcap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0 && ncount > 0) hk.b.ctl->overflow = 1;
+      } else {
+        bcf_words(hk.b, L1, a.front_nxt, gtid, gthreads);
+        bcf_light(hk.b, ncount, a.list_nxt, a.front_nxt, a.front_cur, a.active_cur, hk.in_off, hk.in_nbr,
+                  &out->hook_items, wave_id, nwaves);
+      }
+      if (!(ok = sync())) break;
+      if (L1 < hk.b.lcap)
+        bcf_heavy(hk.b, out->hook_items, a.front_nxt, a.front_cur, a.active_cur, hk.in_off, hk.in_nbr, wave_id,
+                  nwaves);
+      // The heavy chunks read the current frontier's words and bits, which
+      // the clear below zeroes.
+      if (!(ok = sync())) break;
     }
     ms_clear<W>(a, fcount, p.nwords, gtid, gthreads);
     stats_flush<W, THREADS>(a, st);
-    if (!grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok)) break;
+    if (!(ok = sync())) break;
     fcount = out->ncount;
     fdeg = out->ndeg;
   }
+  if constexpr (HOOK::kOn) {
+    // Backward, deepest level (lev - 1) first, unless a source went deeper
+    // than the level words hold (the host then runs the batch again).
+    if (ok && hk.b.ctl->overflow == 0) {
+      const int64_t maxd = lev - 1;
+      if (threadIdx.x < 64) {
+        s_w[threadIdx.x] = 0.0;
+        s_nf[threadIdx.x] = 0;
+      }
+      double wl = 0.0;
+      ull nfl = 0;
+      const BcbPSlot* prev = nullptr;
+      int64_t j = 0;
+      for (int64_t r = maxd; r >= 0; --r, ++j) {
+        BcbPSlot* bs = &hk.slots[j % 3];
+        if (blockIdx.x == 0 && threadIdx.x < 4) reinterpret_cast<int64_t*>(&hk.slots[(j + 1) % 3])[threadIdx.x] = 0;
+        // The heavy vertices of level r + 1 finish before level r reads their D.
+        if (prev && r + 1 < maxd) bcb_heavy_fin(hk.b, (uint32_t)(r + 1), prev->hverts, wave_id, nwaves, wl, nfl);
+        bcb_select(hk.b, (uint32_t)r, &bs->count, wave_id, nwaves);
+        if (!(ok = sync())) break;
+        if (r < maxd)
+          bcb_light<true>(hk.b, hk.off, hk.nbr, (uint32_t)r, bs->count, &bs->items, &bs->hverts, wave_id, nwaves,
+                          wl, nfl);
+        else
+          bcb_light<false>(hk.b, hk.off, hk.nbr, (uint32_t)r, bs->count, &bs->items, &bs->hverts, wave_id, nwaves,
+                           wl, nfl);
+        if (!(ok = sync())) break;
+        if (r < maxd) bcb_heavy(hk.b, hk.off, hk.nbr, (uint32_t)r, bs->items, wave_id, nwaves);
+        if (!(ok = sync())) break;
+        prev = bs;
+      }
+      if (ok) {
+        if (prev && maxd > 0) bcb_heavy_fin(hk.b, 0, prev->hverts, wave_id, nwaves, wl, nfl);
+        bcb_flush(hk.b, s_w, s_nf, wl, nfl);
+      }
+    }
+  }
   // Every workgroup has read the control block it starts from before the
   // closing values go in.
-  (void)grid_sync(&p.pc->bar, p.flags, nwg, gen, p.spin_ticks, s_ok);
+  (void)sync();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     c->level = lev;
     c->tag = tag0 + (uint32_t)lev;
@@ -815,7 +893,9 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
 // (a narrower batch uses their first n * W words).
 struct MsState {
   DevBuf seen, front[2], next, stamp, active[2], list[2], part, stats, ctl, srcs, pctl;
-  int persist_grid[3] = {0, 0, 0};   // resident workgroups of k_ms_persist per word width (0: not yet asked)
+  // Resident workgroups of k_ms_persist per word width 1 / 2 / 4 and for
+  // BC batches (0: not yet asked).
+  int persist_grid[4] = {0, 0, 0, 0};
   int64_t n = 0;
   int wcap = 0;
   uint32_t tag = 0;
@@ -885,13 +965,13 @@ int ms_prepare(rg_ctx* ctx, MsState& S, int64_t n, int W, int64_t nunits) {
 // Runs the batch that k_ms_init set up as one cooperative launch of
 // k_ms_persist: as many workgroups as the device holds at once (the launch
 // fails rather than runs if they cannot all be resident).
-template <int W>
+template <int W, class HOOK>
 int ms_persist(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const MsArgs& a, ull* reached,
-               int64_t* depth, ull* lvl_new_odd, int64_t* levels_run) {
+               int64_t* depth, ull* lvl_new_odd, int64_t* levels_run, const HOOK& hk) {
   rg_ctx* ctx = g->ctx;
   constexpr int T = ms_threads<W>();
-  auto kern = k_ms_persist<W, kMsVB, T>;
-  int& grid = S.persist_grid[W == 1 ? 0 : W == 2 ? 1 : 2];
+  auto kern = k_ms_persist<W, kMsVB, T, HOOK>;
+  int& grid = S.persist_grid[HOOK::kOn ? 3 : W == 1 ? 0 : W == 2 ? 1 : 2];
   if (grid == 0) {
     int coop = 0;
     RG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device));
@@ -927,7 +1007,8 @@ int ms_persist(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const Ms
   RG_TRY(fill<ull>(ctx, reinterpret_cast<ull*>(p.pc),
                    (int64_t)((sizeof(MsPersistCtl) + sizeof(unsigned int) * kFlagStride * grid) / sizeof(ull)), 0ull));
   MsArgs ka = a;
-  void* args[] = {&ka, &p};
+  HOOK kh = hk;
+  void* args[] = {&ka, &p, &kh};
   // One persistent batch on the device at a time: two cooperative grids
   // from different contexts, each partly resident, would wait on each
   // other's workgroups until the barrier limit.  The lock is held until
@@ -947,10 +1028,10 @@ int ms_persist(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const Ms
   return RG_OK;
 }
 
-template <int W>
+template <int W, class HOOK = MsNoHook>
 int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32_t* sources, int nsrc,
            int64_t threshold, int32_t* levels_dev, int32_t* radii_dev, int64_t* levels_run, MsLevelHook* hook,
-           bool persist) {
+           bool persist, const HOOK& dh = HOOK{}) {
   rg_ctx* ctx = g->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = g->n;
@@ -999,7 +1080,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
                      reached, depth, S.tag + 1u);
   RG_HIP(hipGetLastError());
   if (persist && !hook)
-    return ms_persist<W>(g, S, B, E, a, reached, depth, stats + 5 * kMsMaxSources, levels_run);
+    return ms_persist<W, HOOK>(g, S, B, E, a, reached, depth, stats + 5 * kMsMaxSources, levels_run, dh);
 
   EdgeArgs ea{};
   ea.off = g->off;
@@ -1121,8 +1202,6 @@ int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc) {
   return 4 * n * W * 8 + (units << kMsVB) * W * 8 + n * 12 + n / 4 + 4096;
 }
 
-namespace {
-
 // Launch form of a batch (rg_opts::batch_launch): 1 = per-level kernels,
 // 2 = one persistent launch; 0 = the library's choice: the persistent
 // launch from 2^27 arcs.  Measured on MI355X (device time per batch of 64 /
@@ -1139,10 +1218,13 @@ bool batch_persistent(const rg_graph* g, const rg_opts* o) {
   return g->m >= ((int64_t)1 << 27);
 }
 
+namespace {
+
 // Runs one batch; per-source summaries into out (nullable), level arrays or
 // radii into the device buffers given (nullable).
 int batch_run(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, int32_t* levels_dev,
-              int32_t* radii_dev, rg_bfs_summary* out, double* ms_out, MsLevelHook* hook = nullptr) {
+              int32_t* radii_dev, rg_bfs_summary* out, double* ms_out, MsLevelHook* hook = nullptr,
+              const BcPersistHook* bc = nullptr) {
   rg_ctx* ctx = g->ctx;
   const int64_t n = g->n;
   const int W = nsrc <= 64 ? 1 : nsrc <= 128 ? 2 : 4;
@@ -1166,8 +1248,10 @@ int batch_run(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, 
   RG_TRY(ms_prepare(ctx, *S, n, W, B->nunits));
   EventTimer tm(ctx->stream);
   int64_t steps = 0;
-  const bool persist = !hook && batch_persistent(g, o);
-  int rc = W == 1   ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook, persist)
+  const bool persist = !hook && (bc || batch_persistent(g, o));
+  int rc = bc       ? ms_run<1, BcPersistHook>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps,
+                                               nullptr, true, *bc)
+           : W == 1 ? ms_run<1>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook, persist)
            : W == 2 ? ms_run<2>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook, persist)
                     : ms_run<4>(g, *S, B, E, sources, nsrc, threshold, levels_dev, radii_dev, &steps, hook, persist);
   if (rc != RG_OK) return rc;
@@ -1215,6 +1299,12 @@ int bfs_batch_hooked(rg_graph* g, const uint32_t* sources, int nsrc, const rg_op
                      rg_bfs_summary* out) {
   if (nsrc < 1 || nsrc > 64) return set_error(RG_EINVAL, "hooked batch: between 1 and 64 sources");
   return batch_run(g, sources, nsrc, o, nullptr, nullptr, out, nullptr, hook);
+}
+
+int bfs_batch_bc_persist(rg_graph* g, const uint32_t* sources, int nsrc, const rg_opts* o, const BcPersistHook& hk,
+                         rg_bfs_summary* fw) {
+  if (nsrc < 1 || nsrc > 64) return set_error(RG_EINVAL, "BC batch: between 1 and 64 sources");
+  return batch_run(g, sources, nsrc, o, nullptr, nullptr, fw, nullptr, nullptr, &hk);
 }
 
 // Radii (Radii.C:98-157) as one bit-parallel batch: radii[v] = the last

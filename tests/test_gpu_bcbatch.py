@@ -50,8 +50,10 @@ def check(rg, orc, hg, dg, srcs, o=None):
 
 
 # launch 0: the library's choice (batches under 12 sources run one source
-# at a time); launch 1: every batch on the bit-parallel engine.
-@pytest.mark.parametrize("launch", [0, 1])
+# at a time); launch 1: every batch on the bit-parallel engine as per-level
+# kernels; launch 2: every batch as one persistent launch (forward levels,
+# path counts and backward levels in one grid).
+@pytest.mark.parametrize("launch", [0, 1, 2])
 @pytest.mark.parametrize("name", ["rmat12s", "rmat12d", "er12s", "er12d"])
 @pytest.mark.parametrize("k", [1, 5, 17, 64])
 def test_bc_batch_matches_oracle(rg, orc, ctx, graphs, name, k, launch):
@@ -60,13 +62,14 @@ def test_bc_batch_matches_oracle(rg, orc, ctx, graphs, name, k, launch):
     check(rg, orc, hg, dg, pick(hg.n, k, 300 + k), rg.opts(batch_launch=launch))
 
 
+@pytest.mark.parametrize("launch", [1, 2])
 @pytest.mark.parametrize("threshold", [0, BIG])
-def test_bc_batch_all_dense_or_all_sparse(rg, orc, ctx, graphs, threshold):
+def test_bc_batch_all_dense_or_all_sparse(rg, orc, ctx, graphs, threshold, launch):
     # The forward levels' kind (blocked dense or push) must not change the
     # answer: the path counts are summed by the same hook either way.
     hg = graphs("rmat12d")
     dg = rg.DeviceGraph(ctx, hg)
-    check(rg, orc, hg, dg, pick(hg.n, 64, 11), rg.opts(threshold=threshold))
+    check(rg, orc, hg, dg, pick(hg.n, 64, 11), rg.opts(threshold=threshold, batch_launch=launch))
 
 
 def test_bc_batch_duplicates_and_isolated(rg, orc, ctx, graphs):
@@ -95,21 +98,25 @@ def test_bc_batch_heavy_vertices(rg, orc, ctx, symmetric):
     srcs = np.array([hub_out, hub_in, 5, 4000, 2999, 17000], np.uint32)
     check(rg, orc, hg, dg, srcs, rg.opts(batch_launch=1))
     check(rg, orc, hg, dg, srcs, rg.opts(threshold=0, batch_launch=1))
+    check(rg, orc, hg, dg, srcs, rg.opts(batch_launch=2))
+    check(rg, orc, hg, dg, srcs, rg.opts(threshold=0, batch_launch=2))
     check(rg, orc, hg, dg, srcs)
 
 
+@pytest.mark.parametrize("launch", [1, 2])
 @pytest.mark.parametrize("length", [64, 65, 100])
-def test_bc_batch_deep_levels(rg, orc, ctx, length):
+def test_bc_batch_deep_levels(rg, orc, ctx, length, launch):
     # A path: from vertex 0 the last level is length - 1.  Up to level 63
     # the batch engine answers; deeper sources are answered by single-source
     # runs.
     a = np.arange(length - 1, dtype=np.uint32)
     hg = rg.HostGraph.from_edges(length, a, a + 1, symmetric=True)
     dg = rg.DeviceGraph(ctx, hg)
-    check(rg, orc, hg, dg, np.array([0, length // 2, length - 1], np.uint32), rg.opts(batch_launch=1))
+    check(rg, orc, hg, dg, np.array([0, length // 2, length - 1], np.uint32), rg.opts(batch_launch=launch))
 
 
-def test_bc_batch_deep_then_regrown_batch(rg, orc, ctx):
+@pytest.mark.parametrize("launch", [1, 2])
+def test_bc_batch_deep_then_regrown_batch(rg, orc, ctx, launch):
     # 70 sources on a path of 80 vertices: the first batch of 64 holds a
     # source 79 levels deep (single-source runs write its rows), the second
     # batch 41 levels deep (level words grown past the first 32, run again).
@@ -119,10 +126,10 @@ def test_bc_batch_deep_then_regrown_batch(rg, orc, ctx):
     dg = rg.DeviceGraph(ctx, hg)
     srcs = np.concatenate([np.array([0], np.uint32), np.arange(10, 73, dtype=np.uint32),
                            np.full(6, 40, np.uint32)])
-    check(rg, orc, hg, dg, srcs, rg.opts(batch_launch=1))
+    check(rg, orc, hg, dg, srcs, rg.opts(batch_launch=launch))
 
 
-@pytest.mark.parametrize("launch", [0, 1])
+@pytest.mark.parametrize("launch", [0, 1, 2])
 @pytest.mark.parametrize("k", [65, 130, 256])
 def test_bc_batch_uneven_batches(rg, orc, ctx, graphs, k, launch):
     # More than 64 sources: batches of 64 in order, the last one uneven
@@ -160,3 +167,15 @@ def test_bc_batch_errors(rg, ctx, graphs):
         dg.bc_batch(np.array([hg.n], np.uint32))
     with pytest.raises(rg.RGError):
         dg.bc_batch(np.array([], np.uint32))
+
+
+def test_bc_batch_persistent_bigger_graph_and_repeats(rg, orc, ctx, graphs):
+    # One persistent launch per batch on a larger graph, alternated with the
+    # per-level form on the same context: both equal the oracle.
+    hg = graphs("rmat16s")
+    dg = rg.DeviceGraph(ctx, hg)
+    srcs = hg.pick_bc_sources(64)
+    a, da = check(rg, orc, hg, dg, srcs, rg.opts(batch_launch=2))
+    b, db = check(rg, orc, hg, dg, srcs, rg.opts(batch_launch=1))
+    assert [(x.rounds, x.reached, x.nonfinite) for x in a] == [(x.rounds, x.reached, x.nonfinite) for x in b]
+    check(rg, orc, hg, dg, srcs[:20], rg.opts(batch_launch=2))

@@ -962,6 +962,11 @@ int ms_prepare(rg_ctx* ctx, MsState& S, int64_t n, int W, int64_t nunits) {
   return RG_OK;
 }
 
+// One persistent batch in the process at a time, whatever its width or
+// hook: two cooperative grids from different contexts, each partly
+// resident, would wait on each other's workgroups until the barrier limit.
+std::mutex g_persist_mu;
+
 // Runs the batch that k_ms_init set up as one cooperative launch of
 // k_ms_persist: as many workgroups as the device holds at once (the launch
 // fails rather than runs if they cannot all be resident).
@@ -1009,12 +1014,8 @@ int ms_persist(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const Ms
   MsArgs ka = a;
   HOOK kh = hk;
   void* args[] = {&ka, &p, &kh};
-  // One persistent batch on the device at a time: two cooperative grids
-  // from different contexts, each partly resident, would wait on each
-  // other's workgroups until the barrier limit.  The lock is held until
-  // the launch has ended (the reads below wait for it).
-  static std::mutex persist_mu;
-  std::lock_guard<std::mutex> lock(persist_mu);
+  // The lock is held until the launch has ended (the reads below wait for it).
+  std::lock_guard<std::mutex> lock(g_persist_mu);
   RG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(T), args, 0, ctx->stream));
   int64_t bar[1] = {0};   // GridBar: release | abort
   int64_t lev = 0;

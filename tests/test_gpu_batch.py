@@ -199,3 +199,38 @@ def test_persistent_radii(rg, orc, ctx, graphs, threshold):
         rad, rounds, _ = dg.radii(starts, rg.opts("blocked", threshold=threshold, batch_launch=PERSIST))
         ref, rr = orc.radii(off, nbr, starts)
         assert rounds == rr and np.array_equal(rad, ref), name
+
+
+def test_persistent_batches_from_two_contexts(rg, orc, ctx, graphs):
+    # Two host threads, each with its own context, run persistent batches of
+    # different widths at the same time: the launches take turns, and every
+    # answer equals the per-level form's.
+    import threading
+    hg = graphs("rmat14d")
+    ctxs = [rg.Context(0), rg.Context(0)]
+    try:
+        dgs = [rg.DeviceGraph(c, hg) for c in ctxs]
+        srcs = [pick(hg.n, 200, 51), pick(hg.n, 40, 52)]
+        want = [[x.astuple() for x in dgs[i].bfs_batch(srcs[i], rg.opts(batch_launch=LEVELS))[0]] for i in range(2)]
+        got = [[], []]
+        errs = []
+
+        def run(i):
+            try:
+                for _ in range(4):
+                    out, _, _ = dgs[i].bfs_batch(srcs[i], rg.opts(batch_launch=PERSIST))
+                    got[i].append([x.astuple() for x in out])
+            except Exception as e:   # reported below
+                errs.append(e)
+
+        ts = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert not errs, errs
+        for i in range(2):
+            assert all(g == want[i] for g in got[i]) and len(got[i]) == 4
+    finally:
+        for c in ctxs:
+            c.close()

@@ -26,12 +26,13 @@ def test_edgeless_graph_every_app(rg, orc, ctx, n, symmetric):
         b = dg.bfs(n - 1, rg.opts(v))
         assert (b.iters, b.reached, b.edges) == (1, 1, 0), v
         assert b.parents[n - 1] == n - 1 and np.all(np.delete(b.parents, n - 1) == 0xFFFFFFFF)
-    print("bfs batch", flush=True)
     srcs = np.arange(n, dtype=np.uint32)
-    out, lv, _ = dg.bfs_batch(srcs, want_levels=True)
-    for i, s in enumerate(srcs):
-        ref_lv, _ = orc.bfs_levels(off, nbr, int(s))
-        assert np.array_equal(lv[i], ref_lv) and out[i].astuple() == orc.bfs_summary(off, ref_lv)
+    for launch in (1, 2):   # per-level kernels, one persistent launch
+        print("bfs batch", launch, flush=True)
+        out, lv, _ = dg.bfs_batch(srcs, rg.opts(batch_launch=launch), want_levels=True)
+        for i, s in enumerate(srcs):
+            ref_lv, _ = orc.bfs_levels(off, nbr, int(s))
+            assert np.array_equal(lv[i], ref_lv) and out[i].astuple() == orc.bfs_summary(off, ref_lv)
     print("radii", flush=True)
     rad, rounds, _ = dg.radii(srcs, rg.opts("radar"))
     ref_rad, ref_rounds = orc.radii(off, nbr, srcs)
@@ -43,12 +44,13 @@ def test_edgeless_graph_every_app(rg, orc, ctx, n, symmetric):
             dg.transpose()
         ref_d, ref_c = orc.bc(off, nbr, ioff, inbr, [0])
         assert comb == ref_c and np.array_equal(np.isnan(deps), np.isnan(ref_d))
-    print("bc batch", flush=True)
-    bs, deps, _ = dg.bc_batch(srcs, want_deps=True)
-    for i, s in enumerate(srcs):
-        ref_d, ref_c = orc.bc(off, nbr, ioff, inbr, [int(s)])
-        assert bs[i].rounds == ref_c and bs[i].reached == 1
-        assert np.array_equal(np.isnan(deps[i]), np.isnan(ref_d))
+    for launch in (0, 1, 2):   # library's choice, per-level kernels, one persistent launch
+        print("bc batch", launch, flush=True)
+        bs, deps, _ = dg.bc_batch(srcs, rg.opts(batch_launch=launch), want_deps=True)
+        for i, s in enumerate(srcs):
+            ref_d, ref_c = orc.bc(off, nbr, ioff, inbr, [int(s)])
+            assert bs[i].rounds == ref_c and bs[i].reached == 1
+            assert np.array_equal(np.isnan(deps[i]), np.isnan(ref_d))
     if symmetric:
         print("tc", flush=True)
         assert dg.tc()[0] == 0

@@ -1212,7 +1212,7 @@ int64_t bfs_batch_bytes(int64_t n, int64_t m, int nsrc) {
 // a kernel boundary inside a HIP graph (~1.6 us), which small graphs feel,
 // while on large ones the single launch saves the host checks and its
 // dense phase runs without the per-level kernel's register spills.
-bool batch_persistent(const rg_graph* g, const rg_opts* o) {
+static bool batch_persistent(const rg_graph* g, const rg_opts* o) {
   const int mode = o ? o->batch_launch : 0;
   if (mode == 1) return false;
   if (mode == 2) return true;

@@ -74,3 +74,43 @@ def test_sorted_graph_batch_and_tc(rg, orc, ctx, sym):
     if sym:
         tot, _ = sg.tc()
         assert tot == orc.tc(soff, snb)
+
+
+@pytest.mark.parametrize("sym", [True, False])
+def test_batches_in_both_launch_forms(rg, orc, ctx, sym):
+    # The bit-parallel batches on the degree-sorted graph (split hub blocks,
+    # vertices above the 4096-arc chunk threshold) as per-level kernels and
+    # as one persistent launch: BFS levels of 256 sources equal, and equal to
+    # the oracle for a sample; Radii exact; BC rows of 64 sources equal
+    # between the forms and the oracle's single-source BC within 1e-10 for a
+    # sample.
+    R = refs(rg, orc, sym)
+    dg = rg.DeviceGraph(ctx, R["hg"])
+    sg, ids = dg.degree_sort(sort_neighbors=True)
+    soff, snb = sg.download()
+    print(f"batches rmat18 sym={sym}", flush=True)
+    srcs = ids[R["hg"].pick_sources(256)]
+    a, la, _ = sg.bfs_batch(srcs, rg.opts(batch_launch=1), want_levels=True)
+    b, lb, _ = sg.bfs_batch(srcs, rg.opts(batch_launch=2), want_levels=True)
+    assert [x.astuple() for x in a] == [x.astuple() for x in b]
+    assert np.array_equal(la, lb)
+    for i in (0, 100, 255):
+        assert np.array_equal(lb[i], orc.bfs_levels(soff, snb, int(srcs[i]))[0])
+    starts = ids[R["starts"]]
+    for launch in (1, 2):
+        rad, rounds, _ = sg.radii(starts, rg.opts("blocked", batch_launch=launch))
+        ref_rad, ref_rounds = orc.radii(soff, snb, starts)
+        assert rounds == ref_rounds and np.array_equal(rad, ref_rad), launch
+    bsrc = ids[R["hg"].pick_bc_sources(64)]
+    sioff, sinb = (soff, snb) if sym else sg.download(in_edges=True)
+    rows = {}
+    for launch in (1, 2):
+        out, deps, _ = sg.bc_batch(bsrc, rg.opts("radar", batch_launch=launch), want_deps=True)
+        rows[launch] = (out, deps)
+    (oa, da), (ob, db) = rows[1], rows[2]
+    assert [(x.rounds, x.reached, x.nonfinite) for x in oa] == [(x.rounds, x.reached, x.nonfinite) for x in ob]
+    for i in range(len(bsrc)):
+        assert rel(db[i], da[i]) < FP_TOL, i
+    for i in (0, 31, 63):
+        ref, rounds = orc.bc(soff, snb, sioff, sinb, [int(bsrc[i])])
+        assert rel(db[i], ref) < FP_TOL and ob[i].rounds == rounds, i

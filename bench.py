@@ -358,7 +358,8 @@ def profile_run(rg, args):
     """One app on one engine, repeated, for counter collection under
     rocprofv3: --profile app:variant:scale[d|s][:key=value,...] (app bfs,
     query (BFS summary only), batch (--batch sources, degree-sorted graph),
-    bcbatch (--batch BC queries, degree-sorted graph), pagerank, radii or bc; variant push, hubdup, radar, blocked or pull; d =
+    bcbatch (--batch BC queries, degree-sorted graph), pagerank, prd
+    (PageRankDelta), radii or bc; variant push, hubdup, radar, blocked or pull; d =
     directed, s = symmetric; radar, blocked, batch and query run on the
     degree-sorted graph; graphs=0 turns HIP-graph replay off; other options
     are rg.opts keywords, e.g. hubs=1073741824, hub_min_arcs=-1)."""
@@ -396,6 +397,8 @@ def profile_run(rg, args):
             return g.bc_batch(srcs[:args.batch], o)[2]
         if app == "pagerank":
             return g.pagerank(args.iters, o, want_ranks=False).ms
+        if app == "prd":
+            return g.pagerank_delta(args.iters, o, want_ranks=False).ms
         if app == "radii":
             st = rg.radii_starts(g.n)
             return g.radii(st if ids is None else ids[st], o)[2]

@@ -161,7 +161,10 @@ __global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
   if (FWD) wq_finish(q, ea);
 }
 
-// Split blocks: partial sums added in unit order.  Grid: nmulti * BV/THREADS.
+// Split blocks: partial sums of the units added, unit u into accumulator
+// u mod 8 (eight loads in flight: a hub block has hundreds of units), the
+// eight added pairwise; only for destinations that take the sum.  Grid:
+// nmulti * BV/THREADS.
 template <bool FWD, int VB, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_bc_merge(BcDenseArgs a) {
   constexpr int BV = 1 << VB;
@@ -173,7 +176,17 @@ __global__ __launch_bounds__(THREADS) void k_bc_merge(BcDenseArgs a) {
   const int64_t d = ((int64_t)b << VB) + l;
   const bool valid = d < a.n && a.visited[d] == 0;
   double x = 0.0;
-  for (uint32_t u = a.block_unit0[b]; u < a.block_unit0[b + 1]; ++u) x += a.part[((size_t)u << VB) + l];
+  if (valid) {
+    const uint32_t u1 = a.block_unit0[b + 1];
+    uint32_t u = a.block_unit0[b];
+    double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (; u + 8 <= u1; u += 8) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) t[k] += a.part[((size_t)(u + k) << VB) + l];
+    }
+    for (int k = 0; u < u1; ++u, ++k) t[k] += a.part[((size_t)u << VB) + l];
+    x = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+  }
   EdgeArgs ea{};
   ea.off = a.off;
   ea.out_ids = a.out_ids;
@@ -246,9 +259,20 @@ __global__ __launch_bounds__(THREADS) void k_bfs_merge(BcDenseArgs a, uint32_t* 
   const uint32_t b = a.multi[blockIdx.x / PER];
   const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
   const int64_t d = ((int64_t)b << VB) + l;
+  // The candidate of the first unit (in unit order) that has one; eight
+  // units' candidates are loaded at a time.
   uint32_t c = kNone;
-  if (d < a.n && parents[d] == kNone)
-    for (uint32_t u = a.block_unit0[b]; u < a.block_unit0[b + 1] && c == kNone; ++u) c = part[((size_t)u << VB) + l];
+  if (d < a.n && parents[d] == kNone) {
+    const uint32_t u1 = a.block_unit0[b + 1];
+    for (uint32_t u = a.block_unit0[b]; u < u1 && c == kNone; u += 8) {
+      uint32_t t[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) t[k] = u + k < u1 ? part[((size_t)(u + k) << VB) + l] : kNone;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (c == kNone) c = t[k];
+    }
+  }
   const bool join = c != kNone;
   if (join) parents[d] = c;
   EdgeArgs ea{};

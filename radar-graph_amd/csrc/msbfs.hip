@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -111,6 +112,7 @@ struct MsArgs {
   ull* cs;
   int32_t* levels;                     // optional [nsrc][n]
   int32_t* radii;                      // optional [n]: last level a vertex gained a bit (Radii.C)
+  uint32_t wide;                       // m >= 2^32: the statistics keep count and degree sum apart
 };
 
 // ---------------------------------------------------------------------------
@@ -169,13 +171,36 @@ struct MsPushOp {
 };
 
 // ---------------------------------------------------------------------------
-// Per-source statistics staged in LDS, flushed once per workgroup.
+// allbits[w]: the bits of word w that hold one of the batch's nsrc sources.
+template <int W>
+__device__ __forceinline__ void batch_bits(int nsrc, ull* allbits) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int rem = nsrc - 64 * w;
+    allbits[w] = rem >= 64 ? ~0ull : rem <= 0 ? 0ull : ((1ull << rem) - 1);
+  }
+}
 
+// Per-source statistics staged in LDS, flushed once per workgroup.
+//
+// A vertex reached by source s adds 1 to s's count, its out-degree to s's
+// edge sum and level_hash(v, level) to s's checksum.  With m < 2^32 the
+// count and the edge sum share one 64-bit word, count << 32 | degree sum
+// (a workgroup's degree sum for one source stays below m < 2^32, so the low
+// half never carries): two LDS atomics per (vertex, source) pair instead of
+// three.  A word of a vertex's new bits with more than 32 of its 64 sources
+// set is recorded by its complement: the vertex is added once to the
+// word's "every source" sums (one slot per lane, so the lanes of an atomic
+// never collide) and subtracted for the sources it did not reach, which
+// the flush adds back (mod 2^64, exact).  Graphs with m >= 2^32, and calls
+// that write every level, walk the set bits with separate counters.
 template <int W>
 struct MsStats {
-  uint32_t cnt[64 * W];
-  ull deg[64 * W];
+  uint32_t cnt[64 * W];   // m >= 2^32 only
+  ull deg[64 * W];        // count << 32 | degree sum (m < 2^32), or the degree sum
   ull cs[64 * W];
+  ull all_cd[W][64];      // per word and lane: sums added to every source of the word
+  ull all_cs[W][64];
 };
 
 template <int W, int THREADS>
@@ -184,36 +209,94 @@ __device__ __forceinline__ void stats_init(MsStats<W>& st) {
     st.cnt[i] = 0;
     st.deg[i] = 0;
     st.cs[i] = 0;
+    st.all_cd[i >> 6][i & 63] = 0;
+    st.all_cs[i >> 6][i & 63] = 0;
   }
 }
 
+__device__ __forceinline__ ull wave_sum_u64(ull v) {
+#pragma unroll
+  for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
+  return v;
+}
+
+// Every thread of the workgroup calls it.
 template <int W, int THREADS>
 __device__ __forceinline__ void stats_flush(const MsArgs& a, MsStats<W>& st) {
+  constexpr int NW = THREADS / kWave;
   __syncthreads();
+  if (!a.wide) {
+    // Fold each word's per-lane "every source" sums into slot 0.
+    for (int p = (int)(threadIdx.x / kWave); p < 2 * W; p += NW) {
+      ull* slot = (p & 1) ? st.all_cs[p >> 1] : st.all_cd[p >> 1];
+      const ull v = wave_sum_u64(slot[__lane_id()]);
+      if (__lane_id() == 0) slot[0] = v;
+    }
+    __syncthreads();
+  }
   const int nsrc = (int)a.ctl->nsrc;
   for (int i = threadIdx.x; i < nsrc; i += THREADS) {
-    if (st.cnt[i]) {
-      atomicAdd(a.lvl_new + i, (ull)st.cnt[i]);
-      atomicAdd(a.edges + i, st.deg[i]);
-      atomicAdd(a.cs + i, st.cs[i]);
+    ull cnt, deg, cs = st.cs[i];
+    if (a.wide) {
+      cnt = st.cnt[i];
+      deg = st.deg[i];
+    } else {
+      const ull cd = st.deg[i] + st.all_cd[i >> 6][0];
+      cs += st.all_cs[i >> 6][0];
+      cnt = cd >> 32;
+      deg = cd & 0xffffffffull;
+    }
+    if (cnt) {
+      atomicAdd(a.lvl_new + i, cnt);
+      atomicAdd(a.edges + i, deg);
+      atomicAdd(a.cs + i, cs);
     }
   }
 }
 
-// Records vertex d, reached at `level` by the sources in nn.
+// Records vertex d, reached at `level` by the sources in nn (allbits: the
+// batch's sources).  The lanes of a wave often hold the same sources (a
+// hub block's vertices are reached by the same sources at the same level),
+// so lane r walks its bits from bit r on, rotating: the lanes of one LDS
+// atomic then mostly add into different sources' counters instead of
+// queueing on one.
 template <int W>
 __device__ __forceinline__ void stats_add(const MsArgs& a, MsStats<W>& st, uint32_t d, uint32_t level,
-                                          const ull* nn, ull deg) {
+                                          const ull* nn, ull deg, const ull* allbits) {
   const ull h = level_hash(d, level);
+  const int r = (int)__lane_id();
+  if (a.wide) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      ull bits = __builtin_rotateright64(nn[w], (ull)r);
+      while (bits) {
+        const int s = w * 64 + ((__ffsll((long long)bits) - 1 + r) & 63);
+        bits &= bits - 1;
+        atomicAdd(&st.cnt[s], 1u);
+        atomicAdd(&st.deg[s], deg);
+        atomicAdd(&st.cs[s], h);
+        if (a.levels) a.levels[(size_t)s * a.n + d] = (int32_t)level;
+      }
+    }
+    return;
+  }
+  const ull cd = (1ull << 32) | deg;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    ull bits = nn[w];
+    ull x = nn[w], vcd = cd, vcs = h;
+    if (!a.levels && __popcll(x) > 32) {
+      atomicAdd(&st.all_cd[w][r], cd);
+      atomicAdd(&st.all_cs[w][r], h);
+      x = ~x & allbits[w];
+      vcd = 0ull - cd;
+      vcs = 0ull - h;
+    }
+    ull bits = __builtin_rotateright64(x, (ull)r);
     while (bits) {
-      const int s = w * 64 + (__ffsll((long long)bits) - 1);
+      const int s = w * 64 + ((__ffsll((long long)bits) - 1 + r) & 63);
       bits &= bits - 1;
-      atomicAdd(&st.cnt[s], 1u);
-      atomicAdd(&st.deg[s], deg);
-      atomicAdd(&st.cs[s], h);
+      atomicAdd(&st.deg[s], vcd);
+      atomicAdd(&st.cs[s], vcs);
       if (a.levels) a.levels[(size_t)s * a.n + d] = (int32_t)level;
     }
   }
@@ -224,7 +307,7 @@ __device__ __forceinline__ void stats_add(const MsArgs& a, MsStats<W>& st, uint3
 // Entries [first, count) with stride `step`.
 template <int W>
 __device__ __forceinline__ void ms_sparse_fin(const MsArgs& a, MsStats<W>& st, int64_t count, uint32_t level,
-                                              int64_t first, int64_t step) {
+                                              int64_t first, int64_t step, const ull* allbits) {
   for (int64_t i = first; i < count; i += step) {
     const uint32_t d = a.list_nxt[i];
     ull nn[W];
@@ -240,7 +323,7 @@ __device__ __forceinline__ void ms_sparse_fin(const MsArgs& a, MsStats<W>& st, i
     }
     atomicOr(a.active_nxt + (d >> 6), 1ull << (d & 63));
     if (a.radii) a.radii[d] = (int32_t)level;
-    stats_add<W>(a, st, d, level, nn, a.off[d + 1] - a.off[d]);
+    stats_add<W>(a, st, d, level, nn, a.off[d + 1] - a.off[d], allbits);
   }
 }
 
@@ -248,10 +331,12 @@ template <int W>
 __global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
   __shared__ MsStats<W> st;
   if (a.ctl->mode != MS_SPARSE) return;
+  ull allbits[W];
+  batch_bits<W>((int)a.ctl->nsrc, allbits);
   stats_init<W, 256>(st);
   __syncthreads();
   ms_sparse_fin<W>(a, st, a.ctl->ncount, (uint32_t)a.ctl->level + 1, blockIdx.x * (int64_t)blockDim.x + threadIdx.x,
-                   (int64_t)gridDim.x * blockDim.x);
+                   (int64_t)gridDim.x * blockDim.x, allbits);
   stats_flush<W, 256>(a, st);
 }
 
@@ -262,15 +347,6 @@ __global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
 // seen, front_nxt (every vertex of the block is written), the active bit
 // word (lane 0 of each wave; the 64 lanes hold 64 aligned vertices), the
 // next-frontier list and the statistics.  Called by whole waves.
-template <int W>
-__device__ __forceinline__ void batch_bits(int nsrc, ull* allbits) {
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    const int rem = nsrc - 64 * w;
-    allbits[w] = rem >= 64 ? ~0ull : rem <= 0 ? 0ull : ((1ull << rem) - 1);
-  }
-}
-
 template <int W>
 __device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
                                             int64_t d, bool valid, const ull* accw, uint32_t level,
@@ -301,7 +377,7 @@ __device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, Wav
   if (any) {
     deg = a.off[d + 1] - a.off[d];
     if (a.radii) a.radii[d] = (int32_t)level;
-    stats_add<W>(a, st, (uint32_t)d, level, nn, deg);
+    stats_add<W>(a, st, (uint32_t)d, level, nn, deg, allbits);
   }
   wq_push(q, ea, any, (uint32_t)d);
 }
@@ -431,19 +507,49 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
 
 // Blocks split over several units: OR the units' partial words, finalise
 // vertex l of split block b.  Called by whole waves (64 aligned vertices).
-template <int W, int VB>
+// A hub block is split over hundreds of units, so the words of U units are
+// loaded before any is combined (one load round trip per U units, not per
+// unit); a word whose sources have all reached d takes nothing (ms_finalize
+// masks with ~seen) and is not loaded.  The merge kernel loads 8 units at a
+// time; the persistent kernel, whose registers are sized by its dense phase,
+// 8 / W units with W = 1 or 2 and 2 units with W = 4 (more would spill there).
+template <int W, int VB, int U>
 __device__ __forceinline__ void ms_merge_vertex(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
                                                 uint32_t b, int l, uint32_t level, const ull* allbits) {
   const int64_t d = ((int64_t)b << VB) + l;
+  const bool valid = d < a.n;
+  uint32_t need = 0;
+  if (valid) {
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if ((a.seen[(size_t)d * W + w] & allbits[w]) != allbits[w]) need |= 1u << w;
+  }
   ull accw[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) accw[w] = 0;
-  for (uint32_t u = a.block_unit0[b]; u < a.block_unit0[b + 1]; ++u) {
-    const ull* p = a.part + (((size_t)u << VB) + l) * W;
+  const uint32_t u1 = a.block_unit0[b + 1];
+  uint32_t u = a.block_unit0[b];
+  if (need) {
+    for (; u + U <= u1; u += U) {
+      ull t[U][W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) accw[w] |= p[w];
+      for (int k = 0; k < U; ++k) {
+        const ull* p = a.part + (((size_t)(u + k) << VB) + l) * W;
+#pragma unroll
+        for (int w = 0; w < W; ++w) t[k][w] = (need >> w) & 1 ? p[w] : 0ull;
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+#pragma unroll
+        for (int w = 0; w < W; ++w) accw[w] |= t[k][w];
+    }
+    for (; u < u1; ++u) {
+      const ull* p = a.part + (((size_t)u << VB) + l) * W;
+#pragma unroll
+      for (int w = 0; w < W; ++w) accw[w] |= (need >> w) & 1 ? p[w] : 0ull;
+    }
   }
-  ms_finalize<W>(a, st, q, ea, d, d < a.n, accw, level, allbits);
+  ms_finalize<W>(a, st, q, ea, d, valid, accw, level, allbits);
 }
 
 // Grid: nmulti * (BV / THREADS) workgroups.
@@ -467,7 +573,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
   ea.out_ids = a.list_nxt;
   ea.out_info = &a.ctl->ncount;
   WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
-  ms_merge_vertex<W, VB>(a, st, q, ea, b, l, level, allbits);
+  ms_merge_vertex<W, VB, 8>(a, st, q, ea, b, l, level, allbits);
   wq_finish(q, ea);
   stats_flush<W, THREADS>(a, st);
 }
@@ -793,8 +899,8 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
                                      ea, s_unit);
       if (!(ok = sync())) break;
       for (int64_t item = blockIdx.x; item < p.nmulti * PER; item += gridDim.x)
-        ms_merge_vertex<W, VB>(a, st, q, ea, a.multi[item / PER], (int)(item % PER) * THREADS + (int)threadIdx.x,
-                               level, allbits);
+        ms_merge_vertex<W, VB, (W >= 4 ? 2 : 8 / W)>(a, st, q, ea, a.multi[item / PER],
+                                                      (int)(item % PER) * THREADS + (int)threadIdx.x, level, allbits);
       wq_finish(q, ea);
     } else {
       __syncthreads();
@@ -805,7 +911,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
       heavy_items<Op, HUB_NONE>(op, ea, nullptr, nullptr, q, p.heavy, wave_id, out->heavy_count, nwaves);
       wq_finish(q, ea);
       if (!(ok = sync())) break;
-      ms_sparse_fin<W>(a, st, out->ncount, level, gtid, gthreads);
+      ms_sparse_fin<W>(a, st, out->ncount, level, gtid, gthreads, allbits);
     }
     if constexpr (HOOK::kOn) {
       if (!(ok = sync())) break;
@@ -1066,6 +1172,8 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   int64_t* depth = reinterpret_cast<int64_t*>(stats + 4 * kMsMaxSources);
   a.levels = levels_dev;
   a.radii = radii_dev;
+  // RG_BATCH_WIDE_STATS forces the statistics of graphs with 2^32 arcs or more.
+  a.wide = g->m >= (int64_t(1) << 32) || std::getenv("RG_BATCH_WIDE_STATS") ? 1u : 0u;
   int cur = 0;
   auto bind = [&](int c) {
     a.front_cur = S.front[c].as<ull>();

@@ -234,3 +234,27 @@ def test_persistent_batches_from_two_contexts(rg, orc, ctx, graphs):
     finally:
         for c in ctxs:
             c.close()
+
+
+# Summaries alone (no level arrays): the per-source statistics share one
+# 64-bit word for count and edge sum and walk the words with more than half
+# of their sources set by their complement; RG_BATCH_WIDE_STATS forces the
+# separate counters that graphs with 2^32 arcs or more take.  Every summary
+# equals the oracle's, for widths of one, two and four words.
+@pytest.mark.parametrize("wide", [False, True])
+@pytest.mark.parametrize("launch", [LEVELS, PERSIST])
+def test_batch_summaries_without_levels(rg, orc, ctx, graphs, monkeypatch, wide, launch):
+    if wide:
+        monkeypatch.setenv("RG_BATCH_WIDE_STATS", "1")
+    for name, k, threshold in (("rmat14s", 256, None), ("rmat14d", 100, 0), ("rmat12s", 40, None),
+                               ("er12s", 200, BIG)):
+        hg = graphs(name)
+        dg = rg.DeviceGraph(ctx, hg)
+        off, nb = hg.csr()
+        srcs = hg.pick_sources(k)
+        o = rg.opts(batch_launch=launch) if threshold is None else rg.opts(threshold=threshold, batch_launch=launch)
+        print("summaries", name, k, threshold, launch, wide, flush=True)
+        out, _, _ = dg.bfs_batch(srcs, o)
+        for i, s in enumerate(srcs):
+            ref, _ = orc.bfs_levels(off, nb, int(s))
+            assert out[i].astuple() == orc.bfs_summary(off, ref), f"{name} source {s} (slot {i})"

@@ -21,6 +21,9 @@ constexpr int64_t kUnitEdges = int64_t(1) << 19;
 // fit in LDS.
 constexpr int kQueryVB = 12;
 
+// Units per merge chunk (BlockedCSC::mchunk).
+constexpr uint32_t kMergeChunk = 16;
+
 struct BlockedCSC {
   const uint64_t* built_for = nullptr;   // in_off of the orientation it was built from
   int vb = 0, sb = 0;                    // destination-block / source-segment bits
@@ -32,6 +35,10 @@ struct BlockedCSC {
   uint32_t* block_unit0 = nullptr; // nblocks + 1: first unit of each block
   uint32_t* unit_block = nullptr;  // nunits: block of each unit
   uint32_t* multi = nullptr;       // nmulti: the blocks with more than one unit
+  // Merge chunks: the units of every split block in runs of kMergeChunk
+  // (the last run of a block shorter); mchunk[c] = the run's first unit.
+  uint32_t* mchunk = nullptr;      // nmchunk
+  int64_t nmchunk = 0;
   // Launch order other than the layout's (workgroup or queue position ->
   // unit): the split blocks' units sorted by their first source -- a split
   // hub block's units cover consecutive source ranges, so units of different

@@ -43,6 +43,7 @@ void BlockedCSC::release() {
   if (block_unit0) (void)hipFree(block_unit0);
   if (unit_block) (void)hipFree(unit_block);
   if (multi) (void)hipFree(multi);
+  if (mchunk) (void)hipFree(mchunk);
   if (by_src) (void)hipFree(by_src);
   if (xcd_shares) (void)hipFree(xcd_shares);
   for (uint32_t* q : {pk, seg_rel, seg_hi, unit_seg0})
@@ -53,6 +54,8 @@ void BlockedCSC::release() {
   pack_asked = false;
   src = nullptr; vl = nullptr; unit_lo = nullptr; block_unit0 = nullptr; unit_block = nullptr;
   multi = nullptr;
+  mchunk = nullptr;
+  nmchunk = 0;
   by_src = nullptr;
   xcd_shares = nullptr;
   nsplit = 0;
@@ -897,12 +900,16 @@ int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out, bo
   B->nunits = (int64_t)ulo.size();
   ulo.push_back((uint64_t)m);
   B->nblocks = nblocks;
-  std::vector<uint32_t> ub(B->nunits), multi;
+  std::vector<uint32_t> ub(B->nunits), multi, mchunk;
   for (int64_t b = 0; b < nblocks; ++b) {
     for (uint32_t u = bu0[b]; u < bu0[b + 1]; ++u) ub[u] = (uint32_t)b;
-    if (bu0[b + 1] - bu0[b] > 1) multi.push_back((uint32_t)b);
+    if (bu0[b + 1] - bu0[b] > 1) {
+      multi.push_back((uint32_t)b);
+      for (uint32_t u = bu0[b]; u < bu0[b + 1]; u += kMergeChunk) mchunk.push_back(u);
+    }
   }
   B->nmulti = (int64_t)multi.size();
+  B->nmchunk = (int64_t)mchunk.size();
   RG_HIP(hipMalloc(&B->unit_lo, sizeof(uint64_t) * ulo.size()));
   RG_HIP(hipMalloc(&B->block_unit0, sizeof(uint32_t) * bu0.size()));
   RG_HIP(hipMalloc(&B->unit_block, sizeof(uint32_t) * ub.size()));
@@ -910,8 +917,11 @@ int blocked_build(rg_graph* g, int vb, int sb, void** slot, BlockedCSC** out, bo
   RG_TRY(ctx_copy(ctx, B->unit_lo, ulo.data(), sizeof(uint64_t) * ulo.size(), hipMemcpyHostToDevice));
   RG_TRY(ctx_copy(ctx, B->block_unit0, bu0.data(), sizeof(uint32_t) * bu0.size(), hipMemcpyHostToDevice));
   RG_TRY(ctx_copy(ctx, B->unit_block, ub.data(), sizeof(uint32_t) * ub.size(), hipMemcpyHostToDevice));
-  if (!multi.empty())
+  RG_HIP(hipMalloc(&B->mchunk, sizeof(uint32_t) * std::max<size_t>(1, mchunk.size())));
+  if (!multi.empty()) {
     RG_TRY(ctx_copy(ctx, B->multi, multi.data(), sizeof(uint32_t) * multi.size(), hipMemcpyHostToDevice));
+    RG_TRY(ctx_copy(ctx, B->mchunk, mchunk.data(), sizeof(uint32_t) * mchunk.size(), hipMemcpyHostToDevice));
+  }
   // The order by first source and its per-XCD shares (BlockedCSC::by_src).
   {
     DevBuf fs;

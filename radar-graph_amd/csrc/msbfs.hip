@@ -96,6 +96,7 @@ struct MsArgs {
   const uint32_t* unit_block;
   const uint32_t* block_unit0;
   const uint32_t* multi;
+  const uint32_t* mchunk;              // first unit of each merge chunk (BlockedCSC::mchunk)
   // per-vertex words
   ull* seen;
   const ull* front_cur;
@@ -505,64 +506,111 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
   stats_flush<W, THREADS>(a, st);
 }
 
-// Blocks split over several units: OR the units' partial words, finalise
-// vertex l of split block b.  Called by whole waves (64 aligned vertices).
-// A hub block is split over hundreds of units, so the words of U units are
-// loaded before any is combined (one load round trip per U units, not per
-// unit); a word whose sources have all reached d takes nothing (ms_finalize
-// masks with ~seen) and is not loaded.  The merge kernel loads 8 units at a
-// time; the persistent kernel, whose registers are sized by its dense phase,
-// 8 / W units with W = 1 or 2 and 2 units with W = 4 (more would spill there).
+// Blocks split over several units (the hub blocks of a degree-sorted
+// graph, hundreds of units each) are merged in two stages, so that a hub
+// block is read by many workgroups, not by the few that finalise it:
+// first every merge chunk (up to kMergeChunk consecutive units of one
+// block) ORs its units' partial words into the partial slot of its first
+// unit; then each vertex ORs its block's chunk heads and is finalised.
+// Only the words whose sources have not all reached the vertex are loaded
+// and written (ms_finalize masks the others with ~seen, and seen does not
+// change between the stages).  Loads of U units are issued before any is
+// combined: 8 in the merge kernels; in the persistent kernel, whose
+// registers are sized by its dense phase, 8 / W with W = 1 or 2 and 2 with
+// W = 4 (more would spill there).
+
+// Bit w set: word w of vertex d still misses some of the batch's sources.
+template <int W>
+__device__ __forceinline__ uint32_t ms_need(const MsArgs& a, int64_t d, const ull* allbits) {
+  uint32_t need = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    if ((a.seen[(size_t)d * W + w] & allbits[w]) != allbits[w]) need |= 1u << w;
+  return need;
+}
+
+// accw = OR of the partial words (need-masked) of vertex l over units
+// u0, u0 + step, ... below u1.
+template <int W, int VB, int U>
+__device__ __forceinline__ void ms_or_units(const MsArgs& a, uint32_t u0, uint32_t u1, uint32_t step, int l,
+                                            uint32_t need, ull* accw) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) accw[w] = 0;
+  uint32_t u = u0;
+  for (; u + (U - 1) * step < u1; u += U * step) {
+    ull t[U][W];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const ull* p = a.part + (((size_t)(u + k * step) << VB) + l) * W;
+#pragma unroll
+      for (int w = 0; w < W; ++w) t[k][w] = (need >> w) & 1 ? p[w] : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+#pragma unroll
+      for (int w = 0; w < W; ++w) accw[w] |= t[k][w];
+  }
+  for (; u < u1; u += step) {
+    const ull* p = a.part + (((size_t)u << VB) + l) * W;
+#pragma unroll
+    for (int w = 0; w < W; ++w) accw[w] |= (need >> w) & 1 ? p[w] : 0ull;
+  }
+}
+
+// First stage, vertex l of the merge chunk that starts at unit u.
+template <int W, int VB, int U>
+__device__ __forceinline__ void ms_merge_chunk(const MsArgs& a, uint32_t u, int l, const ull* allbits) {
+  const uint32_t b = a.unit_block[u];
+  const int64_t d = ((int64_t)b << VB) + l;
+  if (d >= a.n) return;
+  const uint32_t need = ms_need<W>(a, d, allbits);
+  if (!need) return;
+  const uint32_t u1 = a.block_unit0[b + 1];
+  ull accw[W];
+  ms_or_units<W, VB, U>(a, u, u1 - u > kMergeChunk ? u + kMergeChunk : u1, 1, l, need, accw);
+  ull* p = a.part + (((size_t)u << VB) + l) * W;
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    if ((need >> w) & 1) p[w] = accw[w];
+}
+
+// Second stage: OR the chunk heads of split block b for vertex l and
+// finalise it.  Called by whole waves (64 aligned vertices).
 template <int W, int VB, int U>
 __device__ __forceinline__ void ms_merge_vertex(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
                                                 uint32_t b, int l, uint32_t level, const ull* allbits) {
   const int64_t d = ((int64_t)b << VB) + l;
   const bool valid = d < a.n;
-  uint32_t need = 0;
-  if (valid) {
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      if ((a.seen[(size_t)d * W + w] & allbits[w]) != allbits[w]) need |= 1u << w;
-  }
+  const uint32_t need = valid ? ms_need<W>(a, d, allbits) : 0u;
   ull accw[W];
-#pragma unroll
-  for (int w = 0; w < W; ++w) accw[w] = 0;
-  const uint32_t u1 = a.block_unit0[b + 1];
-  uint32_t u = a.block_unit0[b];
-  if (need) {
-    for (; u + U <= u1; u += U) {
-      ull t[U][W];
-#pragma unroll
-      for (int k = 0; k < U; ++k) {
-        const ull* p = a.part + (((size_t)(u + k) << VB) + l) * W;
-#pragma unroll
-        for (int w = 0; w < W; ++w) t[k][w] = (need >> w) & 1 ? p[w] : 0ull;
-      }
-#pragma unroll
-      for (int k = 0; k < U; ++k)
-#pragma unroll
-        for (int w = 0; w < W; ++w) accw[w] |= t[k][w];
-    }
-    for (; u < u1; ++u) {
-      const ull* p = a.part + (((size_t)u << VB) + l) * W;
-#pragma unroll
-      for (int w = 0; w < W; ++w) accw[w] |= (need >> w) & 1 ? p[w] : 0ull;
-    }
-  }
+  ms_or_units<W, VB, U>(a, a.block_unit0[b], need ? a.block_unit0[b + 1] : a.block_unit0[b], kMergeChunk, l, need,
+                        accw);
   ms_finalize<W>(a, st, q, ea, d, valid, accw, level, allbits);
 }
 
-// Grid: nmulti * (BV / THREADS) workgroups.
+// First merge stage.  Grid: nmchunk * (BV / THREADS) workgroups.
 template <int W, int VB, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
+__global__ __launch_bounds__(THREADS) void k_ms_merge_chunk(MsArgs a) {
+  constexpr int PER = (1 << VB) / THREADS;
+  if (a.ctl->mode != MS_DENSE) return;
+  ull allbits[W];
+  batch_bits<W>((int)a.ctl->nsrc, allbits);
+  ms_merge_chunk<W, VB, 8>(a, a.mchunk[blockIdx.x / PER], (int)(blockIdx.x % PER) * THREADS + (int)threadIdx.x,
+                           allbits);
+}
+
+// Second merge stage: workgroups take the nitems = nmulti * (BV / THREADS)
+// (block, vertex range) items in turn, so that each flushes its per-source
+// sums once (thousands of flushes onto the same counters queue on them).
+// Grid: at most nitems workgroups.
+template <int W, int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a, int64_t nitems) {
   constexpr int BV = 1 << VB;
   constexpr int NW = THREADS / kWave;
   constexpr int PER = BV / THREADS;
   __shared__ uint32_t s_q[NW][kQueue];
   __shared__ MsStats<W> st;
   if (a.ctl->mode != MS_DENSE) return;
-  const uint32_t b = a.multi[blockIdx.x / PER];
-  const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
   const uint32_t level = (uint32_t)a.ctl->level + 1;
   ull allbits[W];
   batch_bits<W>((int)a.ctl->nsrc, allbits);
@@ -573,7 +621,9 @@ __global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a) {
   ea.out_ids = a.list_nxt;
   ea.out_info = &a.ctl->ncount;
   WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
-  ms_merge_vertex<W, VB, 8>(a, st, q, ea, b, l, level, allbits);
+  for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x)
+    ms_merge_vertex<W, VB, 8>(a, st, q, ea, a.multi[item / PER], (int)(item % PER) * THREADS + (int)threadIdx.x,
+                              level, allbits);
   wq_finish(q, ea);
   stats_flush<W, THREADS>(a, st);
 }
@@ -738,7 +788,7 @@ struct MsPersistArgs {
   uint32_t* stamp;
   uint64_t* heavy;              // heavy chunk queue
   int heavy_deg;
-  int64_t nunits, nmulti, nwords;
+  int64_t nunits, nmulti, nmchunk, nwords;
   uint64_t spin_ticks;          // barrier wait limit (wall-clock ticks)
 };
 
@@ -898,9 +948,16 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
       ms_dense_units<W, VB, THREADS>(a, p.nunits, &out->unit_next, level, allbits, lds_main, full, fullw, st, q,
                                      ea, s_unit);
       if (!(ok = sync())) break;
+      constexpr int MU = W >= 4 ? 2 : 8 / W;
+      if (p.nmulti) {
+        for (int64_t item = blockIdx.x; item < p.nmchunk * PER; item += gridDim.x)
+          ms_merge_chunk<W, VB, MU>(a, a.mchunk[item / PER], (int)(item % PER) * THREADS + (int)threadIdx.x,
+                                    allbits);
+        if (!(ok = sync())) break;
+      }
       for (int64_t item = blockIdx.x; item < p.nmulti * PER; item += gridDim.x)
-        ms_merge_vertex<W, VB, (W >= 4 ? 2 : 8 / W)>(a, st, q, ea, a.multi[item / PER],
-                                                      (int)(item % PER) * THREADS + (int)threadIdx.x, level, allbits);
+        ms_merge_vertex<W, VB, MU>(a, st, q, ea, a.multi[item / PER], (int)(item % PER) * THREADS + (int)threadIdx.x,
+                                   level, allbits);
       wq_finish(q, ea);
     } else {
       __syncthreads();
@@ -1113,6 +1170,7 @@ int ms_persist(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const Ms
   p.heavy_deg = E.heavy_deg;
   p.nunits = B->nunits;
   p.nmulti = B->nmulti;
+  p.nmchunk = B->nmchunk;
   p.nwords = g->n / 64 + 1;
   p.spin_ticks = (uint64_t)std::max(rate_khz, 1) * 1000ull * 10ull;   // a barrier wait of 10 s
   RG_TRY(fill<ull>(ctx, reinterpret_cast<ull*>(p.pc),
@@ -1162,6 +1220,7 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
   a.unit_block = B->unit_block;
   a.block_unit0 = B->block_unit0;
   a.multi = B->multi;
+  a.mchunk = B->mchunk;
   a.seen = S.seen.as<ull>();
   a.next = S.next.as<ull>();
   a.part = S.part.as<ull>();
@@ -1222,8 +1281,12 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
       RG_TRY((launch_hub<MsPushOp<W>, SRC_LIST>(E, op, ea)));
       hipLaunchKernelGGL(k_ms_sparse_fin<W>, gfin, 256, 0, st, a);
       hipLaunchKernelGGL((k_ms_dense<W, kMsVB, kMsThreads>), gdense, kMsThreads, 0, st, a, B->nunits);
-      if (B->nmulti)
-        hipLaunchKernelGGL((k_ms_merge<W, kMsVB, kMsThreads>), (int)(B->nmulti * PER), kMsThreads, 0, st, a);
+      if (B->nmulti) {
+        hipLaunchKernelGGL((k_ms_merge_chunk<W, kMsVB, kMsThreads>), (int)(B->nmchunk * PER), kMsThreads, 0, st, a);
+        hipLaunchKernelGGL((k_ms_merge<W, kMsVB, kMsThreads>),
+                           (int)std::min<int64_t>(B->nmulti * PER, (int64_t)ctx->num_cus * 2), kMsThreads, 0, st, a,
+                           B->nmulti * PER);
+      }
       if (hook) {
         const MsLevelView view{&a.ctl->mode, &a.ctl->level, &a.ctl->ncount, a.list_nxt,
                                a.front_cur,  a.front_nxt,   a.active_cur};
@@ -1246,7 +1309,8 @@ int ms_run(rg_graph* g, MsState& S, BlockedCSC* B, const Engine& E, const uint32
     const std::vector<uint64_t> key = {
         (uint64_t)(uintptr_t)g->off, (uint64_t)(uintptr_t)g->nbr, (uint64_t)(uintptr_t)B->src,
         (uint64_t)(uintptr_t)B->vl, (uint64_t)(uintptr_t)B->unit_lo, (uint64_t)(uintptr_t)B->unit_block,
-        (uint64_t)(uintptr_t)B->block_unit0, (uint64_t)(uintptr_t)B->multi, (uint64_t)B->nunits,
+        (uint64_t)(uintptr_t)B->block_unit0, (uint64_t)(uintptr_t)B->multi, (uint64_t)(uintptr_t)B->mchunk,
+        (uint64_t)B->nmchunk, (uint64_t)a.wide, (uint64_t)B->nunits,
         (uint64_t)B->nmulti, (uint64_t)n, (uint64_t)(uintptr_t)a.ctl, (uint64_t)(uintptr_t)a.seen,
         (uint64_t)(uintptr_t)a.next, (uint64_t)(uintptr_t)a.part, (uint64_t)(uintptr_t)stats,
         (uint64_t)(uintptr_t)S.front[0].p, (uint64_t)(uintptr_t)S.front[1].p, (uint64_t)(uintptr_t)S.active[0].p,

@@ -162,6 +162,26 @@ def test_persistent_batch_split_hub_block(rg, orc, ctx):
         check_batch(rg, orc, hg, dg, srcs, rg.opts(threshold=threshold, batch_launch=PERSIST))
 
 
+@pytest.mark.parametrize("launch", [LEVELS, PERSIST])
+def test_batch_hub_block_over_many_merge_chunks(rg, orc, ctx, launch):
+    # A star whose hub has 2^24 in-edges: its destination block is split
+    # over 33 units of 2^19 arcs, merged in chunks of 16 units (two full
+    # chunks and one of a single unit) before its vertices are finalised.
+    leaves = 1 << 24
+    src = np.zeros(leaves, np.uint32)
+    dst = np.arange(1, leaves + 1, dtype=np.uint32)
+    src = np.concatenate([src, dst[:1000]])
+    dst = np.concatenate([dst, dst[1000:2000]])
+    hg = rg.HostGraph.from_edges(leaves + 1, src, dst, symmetric=True)
+    dg = rg.DeviceGraph(ctx, hg)
+    srcs = np.array([0, 1, 5, 1500, 999, 77777, leaves], np.uint32)
+    for threshold in (0, None):
+        o = rg.opts(batch_launch=launch) if threshold is None else rg.opts(threshold=threshold, batch_launch=launch)
+        out = check_batch(rg, orc, hg, dg, srcs, o)
+        alone, _, _ = dg.bfs_batch(srcs, o)   # summaries without level arrays
+        assert [x.astuple() for x in alone] == [x.astuple() for x in out]
+
+
 def test_persistent_equals_per_level_launches(rg, orc, ctx, graphs):
     # Summaries and level arrays of both launch forms, alternated on one
     # context (each form leaves the state the other starts from).

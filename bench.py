@@ -328,7 +328,8 @@ def serve_bench(rg, args, resident=None):
 
     def row(st):
         return {"queries_per_s": round(st.qps, 1), "p50_ms": round(st.p50_ms, 3), "p99_ms": round(st.p99_ms, 3),
-                "mean_batch": round(st.mean_batch, 1), "slots": st.slots,
+                "mean_batch": round(st.mean_batch, 1), "mean_bc_batch": round(st.mean_bc_batch, 1),
+                "slots": st.slots,
                 "peak_gb": round(st.peak_bytes / 1e9, 2), "hbm_budget_gb": round(st.hbm_budget / 1e9, 1),
                 "mem_waits": st.mem_waits}
 

@@ -525,10 +525,12 @@ typedef struct rg_serve_config {
 typedef struct rg_serve_stats {
   int64_t queries, batches;
   int slots;
-  double makespan_ms, qps, p50_ms, p99_ms, mean_ms, mean_batch;
+  double makespan_ms, qps, p50_ms, p99_ms, mean_ms;
+  double mean_batch;     /* BFS queries per BFS work item (batch or single run) */
   int64_t hbm_budget, slot_bytes;
   int64_t peak_bytes;    /* most device bytes the slots' query state held (measured + admitted) */
   int64_t mem_waits;     /* work items that waited for device memory, or ran as a narrower batch */
+  double mean_bc_batch;  /* BC queries per BC work item */
 } rg_serve_stats;
 void rg_serve_config_default(rg_serve_config* c);
 int rg_serve(rg_graph* g, const rg_serve_config* c, const rg_query* queries, int64_t nqueries,

@@ -645,12 +645,13 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
   if (stats) {
     std::vector<double> lat(nq);
     double mk = 0.0, tot = 0.0;
-    int64_t nbfs = 0;
+    int64_t nbfs = 0, nbc = 0;
     for (int64_t i = 0; i < nq; ++i) {
       lat[i] = results[i].latency_ms;
       mk = std::max(mk, results[i].end_ms);
       tot += lat[i];
       nbfs += results[i].kind == RG_Q_BFS;
+      nbc += results[i].kind == RG_Q_BC;
     }
     std::sort(lat.begin(), lat.end());
     auto pct = [&](double p) {
@@ -667,16 +668,19 @@ int rg_serve(rg_graph* g, const rg_serve_config* cfg, const rg_query* queries, i
     stats->p50_ms = pct(0.50);
     stats->p99_ms = pct(0.99);
     stats->mean_ms = nq ? tot / (double)nq : 0.0;
-    int64_t bfs_batches = 0;
+    int64_t bfs_batches = 0, bc_batches = 0;
     {
       std::vector<char> seen((size_t)S.batches, 0);
-      for (int64_t i = 0; i < nq; ++i)
-        if (results[i].kind == RG_Q_BFS && !seen[results[i].batch_id]) {
+      for (int64_t i = 0; i < nq; ++i) {
+        const int k = results[i].kind;
+        if ((k == RG_Q_BFS || k == RG_Q_BC) && !seen[results[i].batch_id]) {
           seen[results[i].batch_id] = 1;
-          bfs_batches++;
+          (k == RG_Q_BFS ? bfs_batches : bc_batches)++;
         }
+      }
     }
     stats->mean_batch = bfs_batches ? (double)nbfs / (double)bfs_batches : 0.0;
+    stats->mean_bc_batch = bc_batches ? (double)nbc / (double)bc_batches : 0.0;
     stats->hbm_budget = budget;
     stats->slot_bytes = slot_bytes;
     stats->peak_bytes = S.peak;

@@ -98,7 +98,7 @@ class ServeStats(C.Structure):
                 ("makespan_ms", C.c_double), ("qps", C.c_double), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("mean_batch", C.c_double),
                 ("hbm_budget", C.c_int64), ("slot_bytes", C.c_int64), ("peak_bytes", C.c_int64),
-                ("mem_waits", C.c_int64)]
+                ("mem_waits", C.c_int64), ("mean_bc_batch", C.c_double)]
 
     def asdict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

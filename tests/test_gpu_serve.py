@@ -134,10 +134,15 @@ def test_bc_queries_batched(rg, ctx, graphs):
     hg = graphs("rmat12d")
     dg = rg.DeviceGraph(ctx, hg)
     q = rg.poisson_workload(hg, 150, 0.0, mix=(0.2, 0.8, 0.0), seed=12)
-    a, _ = dg.serve(q, rg.serve_config(slots=2))
-    b, _ = dg.serve(q, rg.serve_config(one_at_a_time=True))
+    a, st = dg.serve(q, rg.serve_config(slots=2))
+    b, st1 = dg.serve(q, rg.serve_config(one_at_a_time=True))
     sizes = [r.batch_size for r in a if r.kind == rg.Q_BC]
     assert max(sizes) > 1 and max(sizes) <= 64
+    # the stats' mean widths: queries of a kind per work item of that kind
+    for kind, mean, mean1 in ((rg.Q_BFS, st.mean_batch, st1.mean_batch), (rg.Q_BC, st.mean_bc_batch, st1.mean_bc_batch)):
+        items = {r.batch_id for r in a if r.kind == kind}
+        assert mean == pytest.approx(sum(r.kind == kind for r in a) / len(items)) and mean1 == 1.0
+    assert st.mean_bc_batch > 1
     for i, (x, y) in enumerate(zip(a, b)):
         assert x.kind == y.kind
         if x.kind == rg.Q_BC:

@@ -408,10 +408,34 @@ __global__ __launch_bounds__(THREADS) void k_blk_pr_qx(const uint32_t* __restric
   }
 }
 
-// Split blocks: a vertex's sum is its units' partials (hub blocks hold up to
-// ~10^3 units), added in a fixed order by eight interleaved accumulators so
-// that eight loads are in flight, then the vertex step.  Grid: nmulti *
-// (2^VB / 256) workgroups of 256; diff_part[workgroup] = its L1 terms.
+// Sum of vertex l's partials over units [u, u1) of a split block: unit u + k
+// goes into accumulator k mod K, so that K loads are in flight (a hub block
+// holds up to ~10^3 units, and the merge waits for its longest chain of
+// dependent round trips), and the K sums are added pairwise in a fixed
+// order.
+template <int VB, int K>
+__device__ __forceinline__ double sum_units(const double* __restrict__ part, uint32_t u, uint32_t u1, int l) {
+  double a[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = 0.0;
+  for (; u + K <= u1; u += K) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) a[k] += part[((int64_t)(u + k) << VB) + l];
+  }
+  for (int k = 0; u < u1; ++u, ++k) a[k] += part[((int64_t)u << VB) + l];
+#pragma unroll
+  for (int w = K / 2; w >= 1; w /= 2)
+#pragma unroll
+    for (int k = 0; k < w; ++k) a[k] += a[k + w];
+  return a[0];
+}
+
+// Units' partials summed per vertex in flight at a time by the merges.
+constexpr int kMergeLoads = 16;
+
+// Split blocks: a vertex's sum is its units' partials (sum_units), then the
+// vertex step.  Grid: nmulti * (2^VB / 256) workgroups of 256;
+// diff_part[workgroup] = its L1 terms.
 template <int VB>
 __global__ __launch_bounds__(256) void k_blk_pr_merge(const uint32_t* __restrict__ multi,
                                                       const uint32_t* __restrict__ block_unit0,
@@ -427,15 +451,7 @@ __global__ __launch_bounds__(256) void k_blk_pr_merge(const uint32_t* __restrict
   const int64_t v = ((int64_t)b << VB) + l;
   double diff = 0.0;
   if (v < n) {
-    const uint32_t u1 = block_unit0[b + 1];
-    uint32_t u = block_unit0[b];
-    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (; u + 8 <= u1; u += 8) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) a[k] += part[((int64_t)(u + k) << VB) + l];
-    }
-    for (int k = 0; u < u1; ++u, ++k) a[k] += part[((int64_t)u << VB) + l];
-    const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    const double sum = sum_units<VB, kMergeLoads>(part, block_unit0[b], block_unit0[b + 1], l);
     diff = pr_finalize(v, sum, rank, off, contrib_next, damping, added);
   }
   diff = wg_sum<256>(diff);
@@ -673,9 +689,10 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_qx(const uint32_t* __restri
 }
 
 // Split blocks of a dense PageRankDelta round: a vertex's sum is its units'
-// partials (added as in k_blk_pr_merge), then the vertex step and the append
-// of k_blk_prd_it, four vertices per thread (one append, so one atomic, per
-// 1024 vertices).  Grid: nmulti * (2^VB / 1024) workgroups of 256;
+// partials (sum_units, as in k_blk_pr_merge), then the vertex step and the
+// append of k_blk_prd_it, one vertex per thread (one append, so one atomic,
+// per 256 vertices; four vertices per thread queued four chains of unit
+// loads behind each other).  Grid: nmulti * (2^VB / 256) workgroups of 256;
 // l1_part / deg_part[workgroup] = its sums.
 template <int VB>
 __global__ __launch_bounds__(256) void k_blk_prd_merge(const uint32_t* __restrict__ multi,
@@ -686,38 +703,24 @@ __global__ __launch_bounds__(256) void k_blk_prd_merge(const uint32_t* __restric
                                                        double* l1_part, uint64_t* deg_part,
                                                        uint32_t* __restrict__ out_ids, int64_t* count,
                                                        const int64_t* done, PrdStep st) {
-  constexpr int FU = 4;
-  constexpr int PER = (1 << VB) / (256 * FU);
+  constexpr int PER = (1 << VB) / 256;
   __shared__ uint32_t s_cnt[256 / kWave];
   __shared__ unsigned long long s_base;
   if (*done) return;
   const uint32_t b = multi[blockIdx.x / PER];
-  const uint32_t ub = block_unit0[b], u1 = block_unit0[b + 1];
   double l1 = 0.0;
   uint64_t dsum = 0;
-  bool act[FU];
-  int64_t vs[FU];
-#pragma unroll
-  for (int k = 0; k < FU; ++k) {
-    const int l = (int)(blockIdx.x % PER) * (256 * FU) + k * 256 + threadIdx.x;
-    const int64_t v = ((int64_t)b << VB) + l;
-    vs[k] = v;
-    act[k] = false;
-    if (v < n) {
-      uint32_t u = ub;
-      double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      for (; u + 8 <= u1; u += 8) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) a[q] += part[((int64_t)(u + q) << VB) + l];
-      }
-      for (int q = 0; u < u1; ++u, ++q) a[q] += part[((int64_t)u << VB) + l];
-      const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-      const uint64_t deg = off[v + 1] - off[v];
-      act[k] = prd_finalize(v, sum, p[v], deg, p, contrib_next, st, l1);
-      if (act[k]) dsum += deg;
-    }
+  bool act[1] = {false};
+  const int l = (int)(blockIdx.x % PER) * 256 + threadIdx.x;
+  int64_t vs[1] = {((int64_t)b << VB) + l};
+  const int64_t v = vs[0];
+  if (v < n) {
+    const double sum = sum_units<VB, kMergeLoads>(part, block_unit0[b], block_unit0[b + 1], l);
+    const uint64_t deg = off[v + 1] - off[v];
+    act[0] = prd_finalize(v, sum, p[v], deg, p, contrib_next, st, l1);
+    if (act[0]) dsum += deg;
   }
-  wg_append<256, FU>(act, vs, out_ids, count, s_cnt, &s_base);
+  wg_append<256, 1>(act, vs, out_ids, count, s_cnt, &s_base);
   wg_finish<256>(l1, dsum, &l1_part[blockIdx.x], &deg_part[blockIdx.x]);
 }
 
@@ -1071,7 +1074,7 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
   layout_bits(o, &vb, &sb);
   BlockedCSC* B = nullptr;
   RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B, true));
-  const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 1024);
+  const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 256);
   const int64_t nparts = B->nunits + mgrid;
   const size_t unit_doubles = (size_t)B->nunits << vb;
   RG_TRY(work.ensure(sizeof(double) * (unit_doubles + 2 * nparts)));

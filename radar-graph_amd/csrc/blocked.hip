@@ -690,10 +690,10 @@ __global__ __launch_bounds__(THREADS) void k_blk_prd_qx(const uint32_t* __restri
 
 // Split blocks of a dense PageRankDelta round: a vertex's sum is its units'
 // partials (sum_units, as in k_blk_pr_merge), then the vertex step and the
-// append of k_blk_prd_it, one vertex per thread (one append, so one atomic,
-// per 256 vertices; four vertices per thread queued four chains of unit
-// loads behind each other).  Grid: nmulti * (2^VB / 256) workgroups of 256;
-// l1_part / deg_part[workgroup] = its sums.
+// append of k_blk_prd_it, four vertices per thread (one append, so one
+// atomic, and one pair of round sums per 1024 vertices: k_prd_round_sum
+// reads every workgroup's pair).  Grid: nmulti * (2^VB / 1024) workgroups
+// of 256; l1_part / deg_part[workgroup] = its sums.
 template <int VB>
 __global__ __launch_bounds__(256) void k_blk_prd_merge(const uint32_t* __restrict__ multi,
                                                        const uint32_t* __restrict__ block_unit0,
@@ -703,24 +703,30 @@ __global__ __launch_bounds__(256) void k_blk_prd_merge(const uint32_t* __restric
                                                        double* l1_part, uint64_t* deg_part,
                                                        uint32_t* __restrict__ out_ids, int64_t* count,
                                                        const int64_t* done, PrdStep st) {
-  constexpr int PER = (1 << VB) / 256;
+  constexpr int FU = 4;
+  constexpr int PER = (1 << VB) / (256 * FU);
   __shared__ uint32_t s_cnt[256 / kWave];
   __shared__ unsigned long long s_base;
   if (*done) return;
   const uint32_t b = multi[blockIdx.x / PER];
   double l1 = 0.0;
   uint64_t dsum = 0;
-  bool act[1] = {false};
-  const int l = (int)(blockIdx.x % PER) * 256 + threadIdx.x;
-  int64_t vs[1] = {((int64_t)b << VB) + l};
-  const int64_t v = vs[0];
-  if (v < n) {
-    const double sum = sum_units<VB, kMergeLoads>(part, block_unit0[b], block_unit0[b + 1], l);
-    const uint64_t deg = off[v + 1] - off[v];
-    act[0] = prd_finalize(v, sum, p[v], deg, p, contrib_next, st, l1);
-    if (act[0]) dsum += deg;
+  bool act[FU];
+  int64_t vs[FU];
+#pragma unroll
+  for (int k = 0; k < FU; ++k) {
+    const int l = (int)(blockIdx.x % PER) * (256 * FU) + k * 256 + threadIdx.x;
+    const int64_t v = ((int64_t)b << VB) + l;
+    vs[k] = v;
+    act[k] = false;
+    if (v < n) {
+      const double sum = sum_units<VB, kMergeLoads>(part, block_unit0[b], block_unit0[b + 1], l);
+      const uint64_t deg = off[v + 1] - off[v];
+      act[k] = prd_finalize(v, sum, p[v], deg, p, contrib_next, st, l1);
+      if (act[k]) dsum += deg;
+    }
   }
-  wg_append<256, 1>(act, vs, out_ids, count, s_cnt, &s_base);
+  wg_append<256, FU>(act, vs, out_ids, count, s_cnt, &s_base);
   wg_finish<256>(l1, dsum, &l1_part[blockIdx.x], &deg_part[blockIdx.x]);
 }
 
@@ -1074,7 +1080,7 @@ int blocked_prd_round(rg_graph* g, const rg_opts* o, const double* contrib, doub
   layout_bits(o, &vb, &sb);
   BlockedCSC* B = nullptr;
   RG_TRY(blocked_build(g, vb, sb, &g->blocked, &B, true));
-  const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 256);
+  const int64_t mgrid = B->nmulti * ((int64_t(1) << vb) / 1024);
   const int64_t nparts = B->nunits + mgrid;
   const size_t unit_doubles = (size_t)B->nunits << vb;
   RG_TRY(work.ensure(sizeof(double) * (unit_doubles + 2 * nparts)));

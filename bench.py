@@ -366,7 +366,8 @@ def profile_run(rg, args):
     bcbatch (--batch BC queries, degree-sorted graph), pagerank, prd
     (PageRankDelta), radii or bc; variant push, hubdup, radar, blocked or pull; d =
     directed, s = symmetric; radar, blocked, batch and query run on the
-    degree-sorted graph; graphs=0 turns HIP-graph replay off; other options
+    degree-sorted graph (sorted=1 / sorted=0 forces the degree-sorted / original
+    graph for any variant); graphs=0 turns HIP-graph replay off; other options
     are rg.opts keywords, e.g. hubs=1073741824, hub_min_arcs=-1)."""
     parts = args.profile.split(":")
     app, variant, gspec = parts[0], parts[1], parts[2]
@@ -377,12 +378,13 @@ def profile_run(rg, args):
             kw[k] = int(v)
     scale, sym = int(gspec[:-1]), gspec[-1] == "s"
     graphs_on = kw.pop("graphs", 1)   # graphs=0: launch every kernel (no HIP-graph replay)
+    sort = kw.pop("sorted", -1)       # 1 / 0: run on the degree-sorted / original graph whatever the variant
     hg = rg.HostGraph.generate(scale, 16, symmetric=sym)
     ctx = rg.Context(0)
     ctx.graphs(bool(graphs_on))
     dg = rg.DeviceGraph(ctx, hg)
     ids = None
-    if variant in ("radar", "blocked") or app in ("batch", "query", "bcbatch"):
+    if sort == 1 or (sort == -1 and (variant in ("radar", "blocked") or app in ("batch", "query", "bcbatch"))):
         g, ids = dg.degree_sort(sort_neighbors=True)
     else:
         g = dg

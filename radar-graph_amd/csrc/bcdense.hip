@@ -33,6 +33,7 @@ struct BcDenseArgs {
   const uint32_t* unit_block;
   const uint32_t* block_unit0;
   const uint32_t* multi;
+  const uint32_t* mchunk;              // first unit of each merge chunk (BlockedCSC::mchunk)
   const unsigned long long* active;    // bit per vertex: in the frontier
   const uint8_t* visited;
   const double* val;                   // value pushed from each frontier source
@@ -57,10 +58,11 @@ __device__ __forceinline__ bool block_done(const unsigned long long* skip) {
 // Destination d of the block, with sum x: forward, d joins the next
 // frontier when a frontier in-neighbour reached it (x != 0: path counts of
 // visited vertices are positive) and NumPaths[d] (0 while unvisited) = x;
-// backward, Dependencies[d] += x.
+// backward, Dependencies[d] += x.  bc_store writes the values and the next
+// frontier's bit word and returns whether d joins; bc_finalize also queues
+// the joining vertices on the wave's queue.
 template <bool FWD>
-__device__ __forceinline__ void bc_finalize(const BcDenseArgs& a, WaveQueue& q, const EdgeArgs& ea, int64_t d,
-                                            bool valid, double x) {
+__device__ __forceinline__ bool bc_store(const BcDenseArgs& a, int64_t d, bool valid, double x) {
   bool join = false;
   if (valid && x != 0.0) {
     if (FWD) {
@@ -70,14 +72,19 @@ __device__ __forceinline__ void bc_finalize(const BcDenseArgs& a, WaveQueue& q, 
       a.out[d] += x;
     }
   }
-  if (FWD) {
+  if (FWD && a.next_bits) {
     // Called by whole waves over 64 aligned destinations.
-    if (a.next_bits) {
-      const unsigned long long word = __ballot(join);
-      if (__lane_id() == 0 && d < a.n) a.next_bits[d >> 6] = word;
-    }
-    wq_push(q, ea, join, (uint32_t)d);
+    const unsigned long long word = __ballot(join);
+    if (__lane_id() == 0 && d < a.n) a.next_bits[d >> 6] = word;
   }
+  return join;
+}
+
+template <bool FWD>
+__device__ __forceinline__ void bc_finalize(const BcDenseArgs& a, WaveQueue& q, const EdgeArgs& ea, int64_t d,
+                                            bool valid, double x) {
+  const bool join = bc_store<FWD>(a, d, valid, x);
+  if (FWD) wq_push(q, ea, join, (uint32_t)d);
 }
 
 // kDenseArcs arcs of a unit per thread and step [e0 + j * THREADS]: their
@@ -112,11 +119,12 @@ __global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
   constexpr int NW = THREADS / kWave;
   __shared__ double acc[BV];
   __shared__ unsigned long long skip[BV / 64];
-  __shared__ uint32_t s_q[FWD ? NW : 1][FWD ? kQueue : 1];
+  __shared__ WgList<FWD ? BV : 1> joins;   // forward: the block's joining vertices
   const int64_t u = blockIdx.x;
   const uint32_t b = a.unit_block[u];
   const int64_t base = (int64_t)b << VB;
   const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
+  if (threadIdx.x == 0) joins.n = 0;
   for (int l = threadIdx.x; l < BV; l += THREADS) {
     acc[l] = 0.0;
     const int64_t d = base + l;
@@ -152,48 +160,78 @@ __global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
   ea.off = a.off;
   ea.out_ids = a.out_ids;
   ea.out_info = a.out_info;
-  WaveQueue q{s_q[FWD ? threadIdx.x / kWave : 0], 0, 0};
+  uint64_t deg = 0;
   for (int l0 = 0; l0 < BV; l0 += THREADS) {
     const int l = l0 + threadIdx.x;
     const int64_t d = base + l;
-    bc_finalize<FWD>(a, q, ea, d, !((skip[l >> 6] >> (l & 63)) & 1), acc[l]);
+    const bool join = bc_store<FWD>(a, d, !((skip[l >> 6] >> (l & 63)) & 1), acc[l]);
+    if (FWD) wgl_push(joins, ea, join, (uint32_t)d, deg);
   }
-  if (FWD) wq_finish(q, ea);
+  if (FWD) wgl_flush<THREADS>(joins, ea, deg);
 }
 
-// Split blocks: partial sums of the units added, unit u into accumulator
-// u mod 8 (eight loads in flight: a hub block has hundreds of units), the
-// eight added pairwise; only for destinations that take the sum.  Grid:
-// nmulti * BV/THREADS.
+// Split blocks are merged in two stages, so that a hub block (hundreds of
+// units) is read by workgroups across the whole grid rather than by the few
+// that finish it: first each merge chunk (up to kMergeChunk consecutive
+// units of one block, BlockedCSC::mchunk) combines its units' partials into
+// its first unit's slot, in unit order; then each vertex of the block
+// combines its chunk heads, in order.  Only destinations that take the
+// result are read and written.
+
+// Sum of vertex l's partials over units u0, u0 + step, ... below u1: unit
+// u0 + i * step into accumulator i mod K (K loads in flight), the K sums
+// added pairwise.
+template <int VB, int K>
+__device__ __forceinline__ double part_sum(const double* part, uint32_t u0, uint32_t u1, uint32_t step, int l) {
+  double t[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) t[k] = 0.0;
+  uint32_t u = u0;
+  for (; u + (K - 1) * step < u1; u += K * step) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) t[k] += part[((size_t)(u + k * step) << VB) + l];
+  }
+  for (int k = 0; u < u1; u += step, ++k) t[k] += part[((size_t)u << VB) + l];
+#pragma unroll
+  for (int w = K / 2; w >= 1; w /= 2)
+#pragma unroll
+    for (int k = 0; k < w; ++k) t[k] += t[k + w];
+  return t[0];
+}
+
+// First merge stage of a dense BC level.  Grid: nmchunk * BV/THREADS.
+template <int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_bc_merge_chunk(BcDenseArgs a) {
+  constexpr int PER = (1 << VB) / THREADS;
+  const uint32_t u = a.mchunk[blockIdx.x / PER];
+  const uint32_t b = a.unit_block[u];
+  const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
+  const int64_t d = ((int64_t)b << VB) + l;
+  if (d >= a.n || a.visited[d] != 0) return;
+  const uint32_t u1 = a.block_unit0[b + 1];
+  const uint32_t ue = u1 - u > kMergeChunk ? u + kMergeChunk : u1;
+  a.part[((size_t)u << VB) + l] = part_sum<VB, (int)kMergeChunk>(a.part, u, ue, 1, l);
+}
+
+// Second stage: the chunk heads' sums added, then the destination is
+// finalised.  Grid: nmulti * BV/THREADS.
 template <bool FWD, int VB, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_bc_merge(BcDenseArgs a) {
   constexpr int BV = 1 << VB;
-  constexpr int NW = THREADS / kWave;
   constexpr int PER = BV / THREADS;
-  __shared__ uint32_t s_q[FWD ? NW : 1][FWD ? kQueue : 1];
   const uint32_t b = a.multi[blockIdx.x / PER];
   const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
   const int64_t d = ((int64_t)b << VB) + l;
   const bool valid = d < a.n && a.visited[d] == 0;
-  double x = 0.0;
-  if (valid) {
-    const uint32_t u1 = a.block_unit0[b + 1];
-    uint32_t u = a.block_unit0[b];
-    double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (; u + 8 <= u1; u += 8) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) t[k] += a.part[((size_t)(u + k) << VB) + l];
-    }
-    for (int k = 0; u < u1; ++u, ++k) t[k] += a.part[((size_t)u << VB) + l];
-    x = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+  const double x = valid ? part_sum<VB, 8>(a.part, a.block_unit0[b], a.block_unit0[b + 1], kMergeChunk, l) : 0.0;
+  const bool join = bc_store<FWD>(a, d, valid, x);
+  if (FWD) {   // the joins appended once per workgroup
+    EdgeArgs ea{};
+    ea.off = a.off;
+    ea.out_ids = a.out_ids;
+    ea.out_info = a.out_info;
+    wg_push<THREADS>(ea, join, (uint32_t)d);
   }
-  EdgeArgs ea{};
-  ea.off = a.off;
-  ea.out_ids = a.out_ids;
-  ea.out_info = a.out_info;
-  WaveQueue q{s_q[FWD ? threadIdx.x / kWave : 0], 0, 0};
-  bc_finalize<FWD>(a, q, ea, d, valid, x);
-  if (FWD) wq_finish(q, ea);
 }
 
 // BFS_F (BFS.C:28-49) on a dense level: every unvisited destination of
@@ -206,11 +244,12 @@ __global__ __launch_bounds__(THREADS) void k_bfs_dense(BcDenseArgs a, uint32_t* 
   constexpr int NW = THREADS / kWave;
   __shared__ uint32_t cand[BV];
   __shared__ unsigned long long skip[BV / 64];
-  __shared__ uint32_t s_q[NW][kQueue];
+  __shared__ WgList<BV> joins;
   const int64_t u = blockIdx.x;
   const uint32_t b = a.unit_block[u];
   const int64_t base = (int64_t)b << VB;
   const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
+  if (threadIdx.x == 0) joins.n = 0;
   for (int l = threadIdx.x; l < BV; l += THREADS) {
     cand[l] = kNone;
     const int64_t d = base + l;
@@ -239,49 +278,66 @@ __global__ __launch_bounds__(THREADS) void k_bfs_dense(BcDenseArgs a, uint32_t* 
   ea.off = a.off;
   ea.out_ids = a.out_ids;
   ea.out_info = a.out_info;
-  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  uint64_t deg = 0;
   for (int l0 = 0; l0 < BV; l0 += THREADS) {
     const int l = l0 + threadIdx.x;
     const uint32_t c = cand[l];
     const bool join = c != kNone;
     if (join) parents[base + l] = c;
-    wq_push(q, ea, join, (uint32_t)(base + l));
+    wgl_push(joins, ea, join, (uint32_t)(base + l), deg);
   }
-  wq_finish(q, ea);
+  wgl_flush<THREADS>(joins, ea, deg);
 }
 
+// First candidate (in unit order) of vertex l over units u0, u0 + step, ...
+// below u1, eight loads at a time; kNone if none has one.
+template <int VB>
+__device__ __forceinline__ uint32_t first_candidate(const uint32_t* part, uint32_t u0, uint32_t u1, uint32_t step,
+                                                    int l) {
+  uint32_t c = kNone;
+  for (uint32_t u = u0; u < u1 && c == kNone; u += 8 * step) {
+    uint32_t t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = u + k * step < u1 ? part[((size_t)(u + k * step) << VB) + l] : kNone;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (c == kNone) c = t[k];
+  }
+  return c;
+}
+
+// First merge stage of a dense BFS level (as k_bc_merge_chunk: the chunk's
+// first candidate into its head's slot).  Grid: nmchunk * BV/THREADS.
+template <int VB, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_bfs_merge_chunk(BcDenseArgs a, const uint32_t* parents, uint32_t* part) {
+  constexpr int PER = (1 << VB) / THREADS;
+  const uint32_t u = a.mchunk[blockIdx.x / PER];
+  const uint32_t b = a.unit_block[u];
+  const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
+  const int64_t d = ((int64_t)b << VB) + l;
+  if (d >= a.n || parents[d] != kNone) return;
+  const uint32_t u1 = a.block_unit0[b + 1];
+  part[((size_t)u << VB) + l] = first_candidate<VB>(part, u, u1 - u > kMergeChunk ? u + kMergeChunk : u1, 1, l);
+}
+
+// Second stage: the first candidate over the chunk heads.
 template <int VB, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_bfs_merge(BcDenseArgs a, uint32_t* parents, const uint32_t* part) {
   constexpr int BV = 1 << VB;
-  constexpr int NW = THREADS / kWave;
   constexpr int PER = BV / THREADS;
-  __shared__ uint32_t s_q[NW][kQueue];
   const uint32_t b = a.multi[blockIdx.x / PER];
   const int l = (int)(blockIdx.x % PER) * THREADS + threadIdx.x;
   const int64_t d = ((int64_t)b << VB) + l;
-  // The candidate of the first unit (in unit order) that has one; eight
-  // units' candidates are loaded at a time.
-  uint32_t c = kNone;
-  if (d < a.n && parents[d] == kNone) {
-    const uint32_t u1 = a.block_unit0[b + 1];
-    for (uint32_t u = a.block_unit0[b]; u < u1 && c == kNone; u += 8) {
-      uint32_t t[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) t[k] = u + k < u1 ? part[((size_t)(u + k) << VB) + l] : kNone;
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (c == kNone) c = t[k];
-    }
-  }
+  const uint32_t c = d < a.n && parents[d] == kNone
+                         ? first_candidate<VB>(part, a.block_unit0[b], a.block_unit0[b + 1], kMergeChunk, l)
+                         : kNone;
   const bool join = c != kNone;
   if (join) parents[d] = c;
   EdgeArgs ea{};
   ea.off = a.off;
   ea.out_ids = a.out_ids;
   ea.out_info = a.out_info;
-  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
-  wq_push(q, ea, join, (uint32_t)d);
-  wq_finish(q, ea);
+  wg_push<THREADS>(ea, join, (uint32_t)d);   // joins appended once per workgroup
 }
 
 // Bottom-up BFS level: the push-pull tree's edgeMapDense (push-pull
@@ -379,21 +435,31 @@ __device__ __forceinline__ bool pull_select(const PullSel& s, const unsigned lon
   return true;
 }
 
+// The light pull kernel's joining vertices go to a workgroup-wide LDS list
+// of kPullList entries, appended to the next frontier when a step could
+// overflow it and at the end: a big bottom-up level joins tens of millions
+// of vertices, and every append is an atomic on the one count of the
+// next-frontier list (same-address atomics serialise; with per-wave queues
+// of 256 entries the RMAT-26 query spent a third of its time on them).
+constexpr int kPullList = 4096;
+
 __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, const uint32_t* in_nbr,
                                                         const uint32_t* first_in, const uint64_t* off, int64_t nn,
                                                         const unsigned long long* active, uint32_t* parents,
                                                         uint32_t* heavy, int64_t* heavy_count, int heavy_deg,
                                                         uint32_t* out_ids, int64_t* out_info,
                                                         unsigned long long* next, PullSel sel) {
-  __shared__ uint32_t s_q[4][kQueue];
+  __shared__ WgList<kPullList> joins;
   if (!pull_select(sel, active, next)) return;
   EdgeArgs ea{};
   ea.off = off;
   ea.out_ids = out_ids;
   ea.out_info = out_info;
-  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  if (threadIdx.x == 0) joins.n = 0;
+  uint64_t deg = 0;
   const int64_t step = (int64_t)gridDim.x * blockDim.x * kPullU;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x * kPullU; base < nn; base += step) {
+    wgl_drain<256>(joins, ea, 256 * kPullU);   // (its first barrier also orders the zeroing above)
     int64_t v[kPullU];
     bool cand[kPullU], join[kPullU];
     uint64_t b[kPullU], e[kPullU];
@@ -450,10 +516,10 @@ __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, 
       }
       const unsigned long long word = __ballot(join[k]);
       if (__lane_id() == 0 && v[k] < nn) next[v[k] >> 6] = word;
-      wq_push(q, ea, join[k], (uint32_t)v[k]);
+      wgl_push(joins, ea, join[k], (uint32_t)v[k], deg);
     }
   }
-  wq_finish(q, ea);
+  wgl_flush<256>(joins, ea, deg);
 }
 
 __global__ __launch_bounds__(256) void k_bfs_pull_heavy(const uint64_t* in_off, const uint32_t* in_nbr,
@@ -574,6 +640,7 @@ int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const
   a.unit_block = B->unit_block;
   a.block_unit0 = B->block_unit0;
   a.multi = B->multi;
+  a.mchunk = B->mchunk;
   a.active = bits;
   a.visited = visited;
   a.val = val;
@@ -593,19 +660,28 @@ int dense_level(rg_graph* g, int kind, const uint32_t* ids, int64_t count, const
     uint32_t* upart = reinterpret_cast<uint32_t*>(a.part);
     hipLaunchKernelGGL((k_bfs_dense<kQueryVB, kBcThreads>), (int)B->nunits, kBcThreads, 0, ctx->stream, a, parents,
                        upart);
-    if (B->nmulti)
+    if (B->nmulti) {
+      hipLaunchKernelGGL((k_bfs_merge_chunk<kQueryVB, kBcThreads>), (int)(B->nmchunk * PER), kBcThreads, 0,
+                         ctx->stream, a, parents, upart);
       hipLaunchKernelGGL((k_bfs_merge<kQueryVB, kBcThreads>), (int)(B->nmulti * PER), kBcThreads, 0, ctx->stream,
                          a, parents, upart);
+    }
   } else if (kind == 0) {
     hipLaunchKernelGGL((k_bc_dense<true, kQueryVB, kBcThreads>), (int)B->nunits, kBcThreads, 0, ctx->stream, a);
-    if (B->nmulti)
+    if (B->nmulti) {
+      hipLaunchKernelGGL((k_bc_merge_chunk<kQueryVB, kBcThreads>), (int)(B->nmchunk * PER), kBcThreads, 0,
+                         ctx->stream, a);
       hipLaunchKernelGGL((k_bc_merge<true, kQueryVB, kBcThreads>), (int)(B->nmulti * PER), kBcThreads, 0,
                          ctx->stream, a);
+    }
   } else {
     hipLaunchKernelGGL((k_bc_dense<false, kQueryVB, kBcThreads>), (int)B->nunits, kBcThreads, 0, ctx->stream, a);
-    if (B->nmulti)
+    if (B->nmulti) {
+      hipLaunchKernelGGL((k_bc_merge_chunk<kQueryVB, kBcThreads>), (int)(B->nmchunk * PER), kBcThreads, 0,
+                         ctx->stream, a);
       hipLaunchKernelGGL((k_bc_merge<false, kQueryVB, kBcThreads>), (int)(B->nmulti * PER), kBcThreads, 0,
                          ctx->stream, a);
+    }
   }
   if (ev) RG_HIP(hipEventRecord(ev[1], ctx->stream));
   if (!active_bits) hipLaunchKernelGGL(k_clear_bits, grid, 256, 0, ctx->stream, ids, count, bits);

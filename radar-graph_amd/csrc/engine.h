@@ -103,9 +103,10 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 // Per-wavefront staging queue for the next frontier.  fill is wave-uniform.
 struct WaveQueue {
-  uint32_t* buf;   // kQueue entries in LDS
+  uint32_t* buf;     // cap entries in LDS
   int fill;
-  uint64_t deg;    // this lane's share of the appended out-degrees
+  uint64_t deg;      // this lane's share of the appended out-degrees
+  int cap = kQueue;  // a flush (one atomic on the list's shared count) per cap - 64 or more entries
 };
 
 __device__ __forceinline__ void wq_flush(WaveQueue& q, const EdgeArgs& a) {
@@ -128,7 +129,108 @@ __device__ __forceinline__ void wq_push(WaveQueue& q, const EdgeArgs& a, bool ac
     q.deg += a.off[d + 1] - a.off[d];
   }
   q.fill += __popcll(mask);
-  if (q.fill > kQueue - kWave) wq_flush(q, a);
+  if (q.fill > q.cap - kWave) wq_flush(q, a);
+}
+
+// For kernels that finalise each vertex once (one vertex per thread): the
+// workgroup's joining vertices take one range of the next-frontier list,
+// with one atomic for the count and one for the out-degree sum per
+// workgroup rather than per wave (thousands of short workgroups flushing
+// per wave queue on the same two counters, which serialise).  Every thread
+// of the workgroup calls it.
+template <int THREADS>
+__device__ __forceinline__ void wg_push(const EdgeArgs& a, bool act, uint32_t d) {
+  constexpr int NW = THREADS / kWave;
+  __shared__ uint32_t s_cnt[NW];
+  __shared__ uint64_t s_deg[NW];
+  __shared__ unsigned long long s_base;
+  const uint64_t mask = __ballot(act);
+  const int w = (int)(threadIdx.x / kWave);
+  uint64_t deg = act ? a.off[d + 1] - a.off[d] : 0;
+  for (int o = kWave / 2; o > 0; o >>= 1) deg += __shfl_xor(deg, o);
+  if (__lane_id() == 0) {
+    s_cnt[w] = (uint32_t)__popcll(mask);
+    s_deg[w] = deg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    uint64_t dg = 0;
+    for (int i = 0; i < NW; ++i) {
+      const uint32_t y = s_cnt[i];
+      s_cnt[i] = t;
+      t += y;
+      dg += s_deg[i];
+    }
+    s_base = t ? atomicAdd((unsigned long long*)&a.out_info[0], (unsigned long long)t) : 0ull;
+    if (dg) atomicAdd((unsigned long long*)&a.out_info[1], (unsigned long long)dg);
+  }
+  __syncthreads();
+  if (act) a.out_ids[s_base + s_cnt[w] + __popcll(mask & lanemask_lt())] = d;
+}
+
+// A workgroup's list of joining vertices in LDS (up to CAP), appended to
+// the next frontier once, at the workgroup's end, with one atomic for the
+// count and one for the out-degree sum: for kernels with one short
+// workgroup per unit, whose per-wave queue flushes would all queue on the
+// same two counters.  n must be zeroed (and synchronised) before the first
+// push.
+template <int CAP>
+struct WgList {
+  uint32_t ids[CAP];
+  uint32_t n;
+};
+
+// Adds d for the lanes with act; deg accumulates the lane's out-degrees.
+// Called by whole waves.
+template <int CAP>
+__device__ __forceinline__ void wgl_push(WgList<CAP>& L, const EdgeArgs& a, bool act, uint32_t d, uint64_t& deg) {
+  const uint64_t mask = __ballot(act);
+  if (mask == 0) return;
+  uint32_t pos = 0;
+  if (__lane_id() == 0) pos = atomicAdd(&L.n, (uint32_t)__popcll(mask));
+  pos = __shfl(pos, 0);
+  if (act) {
+    L.ids[pos + __popcll(mask & lanemask_lt())] = d;
+    deg += a.off[d + 1] - a.off[d];
+  }
+}
+
+// Appends the list and empties it when fewer than `room` entries are free
+// (a kernel that loops calls it at the end of each step, with room = what
+// one step can add).  Every thread of the workgroup calls it.
+template <int THREADS, int CAP>
+__device__ __forceinline__ void wgl_drain(WgList<CAP>& L, const EdgeArgs& a, uint32_t room) {
+  __shared__ unsigned long long s_base;
+  __syncthreads();
+  const uint32_t n = L.n;
+  if (n + room <= (uint32_t)CAP) return;
+  if (threadIdx.x == 0) s_base = atomicAdd((unsigned long long*)&a.out_info[0], (unsigned long long)n);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += THREADS) a.out_ids[s_base + i] = L.ids[i];
+  __syncthreads();
+  if (threadIdx.x == 0) L.n = 0;
+  __syncthreads();
+}
+
+// Appends the list; every thread of the workgroup calls it once.
+template <int THREADS, int CAP>
+__device__ __forceinline__ void wgl_flush(WgList<CAP>& L, const EdgeArgs& a, uint64_t deg) {
+  constexpr int NW = THREADS / kWave;
+  __shared__ uint64_t s_deg[NW];
+  __shared__ unsigned long long s_base;
+  for (int o = kWave / 2; o > 0; o >>= 1) deg += __shfl_xor(deg, o);
+  if (__lane_id() == 0) s_deg[threadIdx.x / kWave] = deg;
+  __syncthreads();
+  const uint32_t n = L.n;
+  if (threadIdx.x == 0) {
+    uint64_t dg = 0;
+    for (int i = 0; i < NW; ++i) dg += s_deg[i];
+    s_base = n ? atomicAdd((unsigned long long*)&a.out_info[0], (unsigned long long)n) : 0ull;
+    if (dg) atomicAdd((unsigned long long*)&a.out_info[1], (unsigned long long)dg);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += THREADS) a.out_ids[s_base + i] = L.ids[i];
 }
 
 __device__ __forceinline__ void wq_finish(WaveQueue& q, const EdgeArgs& a) {

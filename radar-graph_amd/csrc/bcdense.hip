@@ -119,12 +119,14 @@ __global__ __launch_bounds__(THREADS) void k_bc_dense(BcDenseArgs a) {
   constexpr int NW = THREADS / kWave;
   __shared__ double acc[BV];
   __shared__ unsigned long long skip[BV / 64];
-  __shared__ WgList<FWD ? BV : 1> joins;   // forward: the block's joining vertices
+  __shared__ uint32_t join_ids[FWD ? BV : 1];   // forward: the block's joining vertices
+  __shared__ uint32_t join_n;
+  const WgList joins{join_ids, &join_n, FWD ? (uint32_t)BV : 1u};
   const int64_t u = blockIdx.x;
   const uint32_t b = a.unit_block[u];
   const int64_t base = (int64_t)b << VB;
   const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
-  if (threadIdx.x == 0) joins.n = 0;
+  if (threadIdx.x == 0) join_n = 0;
   for (int l = threadIdx.x; l < BV; l += THREADS) {
     acc[l] = 0.0;
     const int64_t d = base + l;
@@ -244,12 +246,14 @@ __global__ __launch_bounds__(THREADS) void k_bfs_dense(BcDenseArgs a, uint32_t* 
   constexpr int NW = THREADS / kWave;
   __shared__ uint32_t cand[BV];
   __shared__ unsigned long long skip[BV / 64];
-  __shared__ WgList<BV> joins;
+  __shared__ uint32_t join_ids[BV];
+  __shared__ uint32_t join_n;
+  const WgList joins{join_ids, &join_n, (uint32_t)BV};
   const int64_t u = blockIdx.x;
   const uint32_t b = a.unit_block[u];
   const int64_t base = (int64_t)b << VB;
   const bool single = a.block_unit0[b + 1] - a.block_unit0[b] == 1;
-  if (threadIdx.x == 0) joins.n = 0;
+  if (threadIdx.x == 0) join_n = 0;
   for (int l = threadIdx.x; l < BV; l += THREADS) {
     cand[l] = kNone;
     const int64_t d = base + l;
@@ -449,13 +453,15 @@ __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, 
                                                         uint32_t* heavy, int64_t* heavy_count, int heavy_deg,
                                                         uint32_t* out_ids, int64_t* out_info,
                                                         unsigned long long* next, PullSel sel) {
-  __shared__ WgList<kPullList> joins;
+  __shared__ uint32_t join_ids[kPullList];
+  __shared__ uint32_t join_n;
+  const WgList joins{join_ids, &join_n, (uint32_t)kPullList};
   if (!pull_select(sel, active, next)) return;
   EdgeArgs ea{};
   ea.off = off;
   ea.out_ids = out_ids;
   ea.out_info = out_info;
-  if (threadIdx.x == 0) joins.n = 0;
+  if (threadIdx.x == 0) join_n = 0;
   uint64_t deg = 0;
   const int64_t step = (int64_t)gridDim.x * blockDim.x * kPullU;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x * kPullU; base < nn; base += step) {

@@ -169,26 +169,26 @@ __device__ __forceinline__ void wg_push(const EdgeArgs& a, bool act, uint32_t d)
   if (act) a.out_ids[s_base + s_cnt[w] + __popcll(mask & lanemask_lt())] = d;
 }
 
-// A workgroup's list of joining vertices in LDS (up to CAP), appended to
-// the next frontier once, at the workgroup's end, with one atomic for the
-// count and one for the out-degree sum: for kernels with one short
-// workgroup per unit, whose per-wave queue flushes would all queue on the
-// same two counters.  n must be zeroed (and synchronised) before the first
-// push.
-template <int CAP>
+// A workgroup's list of joining vertices in LDS (ids[0 .. *n), up to cap),
+// appended to the next frontier with one atomic for the count per append
+// and one for the out-degree sum at the end, instead of a flush per wave
+// queue: kernels with many short workgroups, or levels that join tens of
+// millions of vertices, otherwise queue on the list's one count
+// (same-address atomics serialise).  *n must be zeroed (and synchronised)
+// before the first push; wgl_flush leaves it zeroed.
 struct WgList {
-  uint32_t ids[CAP];
-  uint32_t n;
+  uint32_t* ids;
+  uint32_t* n;
+  uint32_t cap;
 };
 
 // Adds d for the lanes with act; deg accumulates the lane's out-degrees.
 // Called by whole waves.
-template <int CAP>
-__device__ __forceinline__ void wgl_push(WgList<CAP>& L, const EdgeArgs& a, bool act, uint32_t d, uint64_t& deg) {
+__device__ __forceinline__ void wgl_push(const WgList& L, const EdgeArgs& a, bool act, uint32_t d, uint64_t& deg) {
   const uint64_t mask = __ballot(act);
   if (mask == 0) return;
   uint32_t pos = 0;
-  if (__lane_id() == 0) pos = atomicAdd(&L.n, (uint32_t)__popcll(mask));
+  if (__lane_id() == 0) pos = atomicAdd(L.n, (uint32_t)__popcll(mask));
   pos = __shfl(pos, 0);
   if (act) {
     L.ids[pos + __popcll(mask & lanemask_lt())] = d;
@@ -199,30 +199,33 @@ __device__ __forceinline__ void wgl_push(WgList<CAP>& L, const EdgeArgs& a, bool
 // Appends the list and empties it when fewer than `room` entries are free
 // (a kernel that loops calls it at the end of each step, with room = what
 // one step can add).  Every thread of the workgroup calls it.
-template <int THREADS, int CAP>
-__device__ __forceinline__ void wgl_drain(WgList<CAP>& L, const EdgeArgs& a, uint32_t room) {
+template <int THREADS>
+__device__ __forceinline__ void wgl_drain(const WgList& L, const EdgeArgs& a, uint32_t room) {
   __shared__ unsigned long long s_base;
   __syncthreads();
-  const uint32_t n = L.n;
-  if (n + room <= (uint32_t)CAP) return;
+  const uint32_t n = *L.n;
+  if (n + room <= L.cap) return;
   if (threadIdx.x == 0) s_base = atomicAdd((unsigned long long*)&a.out_info[0], (unsigned long long)n);
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < n; i += THREADS) a.out_ids[s_base + i] = L.ids[i];
   __syncthreads();
-  if (threadIdx.x == 0) L.n = 0;
+  if (threadIdx.x == 0) *L.n = 0;
   __syncthreads();
 }
 
-// Appends the list; every thread of the workgroup calls it once.
-template <int THREADS, int CAP>
-__device__ __forceinline__ void wgl_flush(WgList<CAP>& L, const EdgeArgs& a, uint64_t deg) {
+// Appends the rest of the list and the workgroup's out-degree sum (deg is
+// zeroed); every thread of the workgroup calls it.
+template <int THREADS>
+__device__ __forceinline__ void wgl_flush(const WgList& L, const EdgeArgs& a, uint64_t& deg) {
   constexpr int NW = THREADS / kWave;
   __shared__ uint64_t s_deg[NW];
   __shared__ unsigned long long s_base;
-  for (int o = kWave / 2; o > 0; o >>= 1) deg += __shfl_xor(deg, o);
-  if (__lane_id() == 0) s_deg[threadIdx.x / kWave] = deg;
+  uint64_t dl = deg;
+  deg = 0;
+  for (int o = kWave / 2; o > 0; o >>= 1) dl += __shfl_xor(dl, o);
+  if (__lane_id() == 0) s_deg[threadIdx.x / kWave] = dl;
   __syncthreads();
-  const uint32_t n = L.n;
+  const uint32_t n = *L.n;
   if (threadIdx.x == 0) {
     uint64_t dg = 0;
     for (int i = 0; i < NW; ++i) dg += s_deg[i];
@@ -231,6 +234,9 @@ __device__ __forceinline__ void wgl_flush(WgList<CAP>& L, const EdgeArgs& a, uin
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < n; i += THREADS) a.out_ids[s_base + i] = L.ids[i];
+  __syncthreads();
+  if (threadIdx.x == 0) *L.n = 0;
+  __syncthreads();
 }
 
 __device__ __forceinline__ void wq_finish(WaveQueue& q, const EdgeArgs& a) {

@@ -349,9 +349,9 @@ __global__ __launch_bounds__(256) void k_ms_sparse_fin(MsArgs a) {
 // word (lane 0 of each wave; the 64 lanes hold 64 aligned vertices), the
 // next-frontier list and the statistics.  Called by whole waves.
 template <int W>
-__device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
-                                            int64_t d, bool valid, const ull* accw, uint32_t level,
-                                            const ull* allbits) {
+__device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, const WgList& L, uint64_t& qdeg,
+                                            const EdgeArgs& ea, int64_t d, bool valid, const ull* accw,
+                                            uint32_t level, const ull* allbits) {
   ull nn[W];
   bool any = false;
 #pragma unroll
@@ -380,7 +380,7 @@ __device__ __forceinline__ void ms_finalize(const MsArgs& a, MsStats<W>& st, Wav
     if (a.radii) a.radii[d] = (int32_t)level;
     stats_add<W>(a, st, (uint32_t)d, level, nn, deg, allbits);
   }
-  wq_push(q, ea, any, (uint32_t)d);
+  wgl_push(L, ea, any, (uint32_t)d, qdeg);
 }
 
 template <class T>
@@ -395,8 +395,8 @@ __device__ __forceinline__ T ld_nt(const T* p) {
 template <int W, int VB, int THREADS>
 __device__ __forceinline__ void ms_dense_units(const MsArgs& a, int64_t nunits, unsigned long long* unit_next,
                                                uint32_t level, const ull* allbits, ull* acc, ull* full,
-                                               uint8_t* fullw, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
-                                               int64_t& s_unit) {
+                                               uint8_t* fullw, MsStats<W>& st, const WgList& L, uint64_t& qdeg,
+                                               const EdgeArgs& ea, int64_t& s_unit) {
   constexpr int BV = 1 << VB;
   for (;;) {
     __syncthreads();
@@ -475,7 +475,8 @@ __device__ __forceinline__ void ms_dense_units(const MsArgs& a, int64_t nunits, 
     for (int l0 = 0; l0 < BV; l0 += THREADS) {
       const int l = l0 + threadIdx.x;
       const int64_t d = base + l;
-      ms_finalize<W>(a, st, q, ea, d, d < a.n, &acc[l * W], level, allbits);
+      ms_finalize<W>(a, st, L, qdeg, ea, d, d < a.n, &acc[l * W], level, allbits);
+      wgl_drain<THREADS>(L, ea, THREADS);
     }
   }
 }
@@ -487,7 +488,8 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
   __shared__ ull acc[BV * W];
   __shared__ ull full[BV / 64];
   __shared__ uint8_t fullw[W > 1 ? BV : 1];   // W > 1: bit w = every source of word w reached it
-  __shared__ uint32_t s_q[NW][kQueue];
+  __shared__ uint32_t join_ids[NW * kQueue];
+  __shared__ uint32_t join_n;
   __shared__ MsStats<W> st;
   __shared__ int64_t s_unit;
   if (a.ctl->mode != MS_DENSE) return;
@@ -498,11 +500,13 @@ __global__ __launch_bounds__(THREADS) void k_ms_dense(MsArgs a, int64_t nunits) 
   ea.off = a.off;
   ea.out_ids = a.list_nxt;
   ea.out_info = &a.ctl->ncount;
-  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
+  const WgList joins{join_ids, &join_n, (uint32_t)(NW * kQueue)};
+  if (threadIdx.x == 0) join_n = 0;
+  uint64_t qdeg = 0;
   stats_init<W, THREADS>(st);
   ms_dense_units<W, VB, THREADS>(a, nunits, (unsigned long long*)&a.ctl->unit_next, level, allbits, acc, full, fullw,
-                                 st, q, ea, s_unit);
-  wq_finish(q, ea);
+                                 st, joins, qdeg, ea, s_unit);
+  wgl_flush<THREADS>(joins, ea, qdeg);
   stats_flush<W, THREADS>(a, st);
 }
 
@@ -577,15 +581,16 @@ __device__ __forceinline__ void ms_merge_chunk(const MsArgs& a, uint32_t u, int 
 // Second stage: OR the chunk heads of split block b for vertex l and
 // finalise it.  Called by whole waves (64 aligned vertices).
 template <int W, int VB, int U>
-__device__ __forceinline__ void ms_merge_vertex(const MsArgs& a, MsStats<W>& st, WaveQueue& q, const EdgeArgs& ea,
-                                                uint32_t b, int l, uint32_t level, const ull* allbits) {
+__device__ __forceinline__ void ms_merge_vertex(const MsArgs& a, MsStats<W>& st, const WgList& L, uint64_t& qdeg,
+                                                const EdgeArgs& ea, uint32_t b, int l, uint32_t level,
+                                                const ull* allbits) {
   const int64_t d = ((int64_t)b << VB) + l;
   const bool valid = d < a.n;
   const uint32_t need = valid ? ms_need<W>(a, d, allbits) : 0u;
   ull accw[W];
   ms_or_units<W, VB, U>(a, a.block_unit0[b], need ? a.block_unit0[b + 1] : a.block_unit0[b], kMergeChunk, l, need,
                         accw);
-  ms_finalize<W>(a, st, q, ea, d, valid, accw, level, allbits);
+  ms_finalize<W>(a, st, L, qdeg, ea, d, valid, accw, level, allbits);
 }
 
 // First merge stage.  Grid: nmchunk * (BV / THREADS) workgroups.
@@ -608,23 +613,28 @@ __global__ __launch_bounds__(THREADS) void k_ms_merge(MsArgs a, int64_t nitems) 
   constexpr int BV = 1 << VB;
   constexpr int NW = THREADS / kWave;
   constexpr int PER = BV / THREADS;
-  __shared__ uint32_t s_q[NW][kQueue];
+  __shared__ uint32_t join_ids[NW * kQueue];
+  __shared__ uint32_t join_n;
   __shared__ MsStats<W> st;
   if (a.ctl->mode != MS_DENSE) return;
   const uint32_t level = (uint32_t)a.ctl->level + 1;
   ull allbits[W];
   batch_bits<W>((int)a.ctl->nsrc, allbits);
   stats_init<W, THREADS>(st);
+  const WgList joins{join_ids, &join_n, (uint32_t)(NW * kQueue)};
+  if (threadIdx.x == 0) join_n = 0;
+  uint64_t qdeg = 0;
   __syncthreads();
   EdgeArgs ea{};
   ea.off = a.off;
   ea.out_ids = a.list_nxt;
   ea.out_info = &a.ctl->ncount;
-  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
-  for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x)
-    ms_merge_vertex<W, VB, 8>(a, st, q, ea, a.multi[item / PER], (int)(item % PER) * THREADS + (int)threadIdx.x,
-                              level, allbits);
-  wq_finish(q, ea);
+  for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+    ms_merge_vertex<W, VB, 8>(a, st, joins, qdeg, ea, a.multi[item / PER],
+                              (int)(item % PER) * THREADS + (int)threadIdx.x, level, allbits);
+    wgl_drain<THREADS>(joins, ea, THREADS);
+  }
+  wgl_flush<THREADS>(joins, ea, qdeg);
   stats_flush<W, THREADS>(a, st);
 }
 
@@ -874,7 +884,8 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
   __shared__ ull lds_main[BV * W];
   __shared__ ull full[BV / 64];
   __shared__ uint8_t fullw[W > 1 ? BV : 1];
-  __shared__ uint32_t s_q[NW][kQueue];
+  __shared__ uint32_t s_q[NW][kQueue];   // sparse levels' wave queues; dense levels' join list
+  __shared__ uint32_t join_n;
   __shared__ MsStats<W> st;
   __shared__ int64_t s_unit;
   __shared__ int s_ok;
@@ -883,6 +894,9 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
   static_assert(sizeof(Light) <= sizeof(ull) * BV * W, "light-phase tables fit in the block's words");
   static_assert(!HOOK::kOn || W == 1, "BC batches use one word of source bits");
   Light& L = *reinterpret_cast<Light*>(lds_main);
+  const WgList joins{&s_q[0][0], &join_n, (uint32_t)(NW * kQueue)};
+  if (threadIdx.x == 0) join_n = 0;   // ordered by the barriers before the first dense level
+  uint64_t qdeg = 0;
 
   MsCtl* c = a.ctl;
   const int64_t threshold = c->threshold;
@@ -945,8 +959,8 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
     WaveQueue q{s_q[threadIdx.x / kWave], 0, 0};
     stats_init<W, THREADS>(st);
     if (dense) {
-      ms_dense_units<W, VB, THREADS>(a, p.nunits, &out->unit_next, level, allbits, lds_main, full, fullw, st, q,
-                                     ea, s_unit);
+      ms_dense_units<W, VB, THREADS>(a, p.nunits, &out->unit_next, level, allbits, lds_main, full, fullw, st, joins,
+                                     qdeg, ea, s_unit);
       if (!(ok = sync())) break;
       constexpr int MU = W >= 4 ? 2 : 8 / W;
       if (p.nmulti) {
@@ -955,10 +969,12 @@ __global__ __launch_bounds__(THREADS) void k_ms_persist(MsArgs a, MsPersistArgs 
                                     allbits);
         if (!(ok = sync())) break;
       }
-      for (int64_t item = blockIdx.x; item < p.nmulti * PER; item += gridDim.x)
-        ms_merge_vertex<W, VB, MU>(a, st, q, ea, a.multi[item / PER], (int)(item % PER) * THREADS + (int)threadIdx.x,
-                                   level, allbits);
-      wq_finish(q, ea);
+      for (int64_t item = blockIdx.x; item < p.nmulti * PER; item += gridDim.x) {
+        ms_merge_vertex<W, VB, MU>(a, st, joins, qdeg, ea, a.multi[item / PER],
+                                   (int)(item % PER) * THREADS + (int)threadIdx.x, level, allbits);
+        wgl_drain<THREADS>(joins, ea, THREADS);
+      }
+      wgl_flush<THREADS>(joins, ea, qdeg);
     } else {
       __syncthreads();
       const Op op{a.front_cur, a.seen, a.next, p.stamp, nullptr, nullptr, tag0 + (uint32_t)lev};

@@ -377,6 +377,17 @@ struct LightLds {
 // chunk per wave (grid-stride over the waves given).  Vertices of degree
 // above heavy_deg are queued as chunk items (unless the heavy list is
 // static) and skipped here.  Called by whole waves.
+// Wave queue entries of the edge kernels: 768 without LDS hub copies, whose
+// launches hold little else in LDS (a level that claims millions of vertices
+// otherwise flushes its queues tens of thousands of times, each an atomic on
+// the list's one count; same-address atomics serialise), kQueue with them
+// (the host sizes the hub copies around kQueue).
+constexpr int kQueueNoHub = 768;
+template <class Op, int HUB>
+constexpr int edge_queue() {
+  return !Op::kOutput ? 1 : HUB == HUB_NONE ? kQueueNoHub : kQueue;
+}
+
 template <class Op, int MODE, int HUB, int WPB>
 __device__ __forceinline__ void light_phase(const Op& op, const EdgeArgs& a, typename Op::HubT* lds_hub,
                                             typename Op::HubT* l2_hub, WaveQueue& q, LightLds<Op, WPB>& L,
@@ -448,7 +459,8 @@ template <class Op, int MODE, int HUB, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
   constexpr int WPB = BLOCK / kWave;
   __shared__ LightLds<Op, WPB> L;
-  __shared__ uint32_t s_q[Op::kOutput ? WPB : 1][Op::kOutput ? kQueue : 1];
+  constexpr int QN = edge_queue<Op, HUB>();
+  __shared__ uint32_t s_q[Op::kOutput ? WPB : 1][QN];
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   typename Op::HubT* lds_hub = reinterpret_cast<typename Op::HubT*>(dyn_lds);
   if (op.skip()) return;
@@ -456,7 +468,7 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
   typename Op::HubT* l2_hub = l2_copy<Op, HUB>(a);
 
   const int w = threadIdx.x / kWave;
-  WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0};
+  WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0, QN};
   const int64_t count = a.count_dev ? *a.count_dev : a.count;
   const int64_t wave_id = (int64_t)blockIdx.x * WPB + w;
   const int64_t nwaves = (int64_t)gridDim.x * WPB;
@@ -474,7 +486,8 @@ __global__ __launch_bounds__(BLOCK) void edge_light_kernel(Op op, EdgeArgs a) {
 template <class Op, int HUB, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void edge_heavy_kernel(Op op, EdgeArgs a) {
   constexpr int WPB = BLOCK / kWave;
-  __shared__ uint32_t s_q[Op::kOutput ? WPB : 1][Op::kOutput ? kQueue : 1];
+  constexpr int QN = edge_queue<Op, HUB>();
+  __shared__ uint32_t s_q[Op::kOutput ? WPB : 1][QN];
   extern __shared__ __align__(16) unsigned char dyn_lds[];
   typename Op::HubT* lds_hub = reinterpret_cast<typename Op::HubT*>(dyn_lds);
   if (op.skip()) return;
@@ -483,7 +496,7 @@ __global__ __launch_bounds__(BLOCK) void edge_heavy_kernel(Op op, EdgeArgs a) {
   hub_init<Op, HUB, BLOCK>(op, a, lds_hub);
   typename Op::HubT* l2_hub = l2_copy<Op, HUB>(a);
   const int w = threadIdx.x / kWave;
-  WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0};
+  WaveQueue q{Op::kOutput ? s_q[w] : nullptr, 0, 0, QN};
   const int64_t wave_id = (int64_t)blockIdx.x * WPB + w;
   const int64_t nwaves = (int64_t)gridDim.x * WPB;
   heavy_items<Op, HUB>(op, a, lds_hub, l2_hub, q, a.heavy, wave_id, items, nwaves);

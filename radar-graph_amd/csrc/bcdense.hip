@@ -483,11 +483,17 @@ __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, 
     }
 #pragma unroll
     for (int k = 0; k < kPullU; ++k) {
-      if (cand[k] && e[k] - b[k] > (uint64_t)heavy_deg) {
-        heavy[atomicAdd((unsigned long long*)heavy_count, 1ull)] = (uint32_t)v[k];
-        cand[k] = false;
+      // Heavy candidates (hubs, consecutive ids after a degree sort) go to
+      // the wave-per-vertex kernel's list, one atomic per wave.
+      const bool hv = cand[k] && e[k] - b[k] > (uint64_t)heavy_deg;
+      const uint64_t hm = __ballot(hv);
+      if (hm) {
+        unsigned long long hb = 0;
+        if (__lane_id() == 0) hb = atomicAdd((unsigned long long*)heavy_count, (unsigned long long)__popcll(hm));
+        hb = __shfl(hb, 0);
+        if (hv) heavy[hb + __popcll(hm & lanemask_lt())] = (uint32_t)v[k];
       }
-      cand[k] = cand[k] && e[k] > b[k];
+      cand[k] = cand[k] && !hv && e[k] > b[k];
       s0[k] = cand[k] ? first_in[v[k]] : 0u;   // = in_nbr[b[k]]
     }
 #pragma unroll

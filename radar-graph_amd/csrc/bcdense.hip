@@ -439,13 +439,11 @@ __device__ __forceinline__ bool pull_select(const PullSel& s, const unsigned lon
   return true;
 }
 
-// The light pull kernel's joining vertices go to a workgroup-wide LDS list
-// of kPullList entries, appended to the next frontier when a step could
-// overflow it and at the end: a big bottom-up level joins tens of millions
-// of vertices, and every append is an atomic on the one count of the
-// next-frontier list (same-address atomics serialise; with per-wave queues
-// of 256 entries the RMAT-26 query spent a third of its time on them).
-constexpr int kPullList = 4096;
+// Wave queue entries of the light pull kernel: a big bottom-up level joins
+// tens of millions of vertices, and every queue flush is an atomic on the
+// one count of the next-frontier list (same-address atomics serialise; with
+// 256-entry queues the RMAT-26 query spent a third of its time on them).
+constexpr int kPullQueue = 1024;
 
 __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, const uint32_t* in_nbr,
                                                         const uint32_t* first_in, const uint64_t* off, int64_t nn,
@@ -453,19 +451,15 @@ __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, 
                                                         uint32_t* heavy, int64_t* heavy_count, int heavy_deg,
                                                         uint32_t* out_ids, int64_t* out_info,
                                                         unsigned long long* next, PullSel sel) {
-  __shared__ uint32_t join_ids[kPullList];
-  __shared__ uint32_t join_n;
-  const WgList joins{join_ids, &join_n, (uint32_t)kPullList};
+  __shared__ uint32_t s_q[4][kPullQueue];
   if (!pull_select(sel, active, next)) return;
   EdgeArgs ea{};
   ea.off = off;
   ea.out_ids = out_ids;
   ea.out_info = out_info;
-  if (threadIdx.x == 0) join_n = 0;
-  uint64_t deg = 0;
+  WaveQueue q{s_q[threadIdx.x / kWave], 0, 0, kPullQueue};
   const int64_t step = (int64_t)gridDim.x * blockDim.x * kPullU;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x * kPullU; base < nn; base += step) {
-    wgl_drain<256>(joins, ea, 256 * kPullU);   // (its first barrier also orders the zeroing above)
     int64_t v[kPullU];
     bool cand[kPullU], join[kPullU];
     uint64_t b[kPullU], e[kPullU];
@@ -528,10 +522,10 @@ __global__ __launch_bounds__(256) void k_bfs_pull_light(const uint64_t* in_off, 
       }
       const unsigned long long word = __ballot(join[k]);
       if (__lane_id() == 0 && v[k] < nn) next[v[k] >> 6] = word;
-      wgl_push(joins, ea, join[k], (uint32_t)v[k], deg);
+      wq_push(q, ea, join[k], (uint32_t)v[k]);
     }
   }
-  wgl_flush<256>(joins, ea, deg);
+  wq_finish(q, ea);
 }
 
 __global__ __launch_bounds__(256) void k_bfs_pull_heavy(const uint64_t* in_off, const uint32_t* in_nbr,

@@ -64,6 +64,33 @@ def test_bfs(rg, orc, ctx, graphs, gname, variant):
         assert r.edges == int(np.diff(off)[lv >= 0].sum())
 
 
+_RMAT20 = {}
+
+
+@pytest.mark.parametrize("variant,threshold", [("pull", None), ("pull", 1 << 20), ("pull", 1 << 30), ("push", None),
+                                               ("radar", None), ("blocked", None)])
+def test_bfs_rmat20_levels_and_query_summaries(rg, orc, ctx, variant, threshold):
+    # A million vertices: the bottom-up kernels run a thousand workgroups and
+    # their next-frontier lists carry millions of vertices per level.  Every
+    # parent is an in-neighbour one level up (levels exact) and every query
+    # summary (iterations, reached, edges, level checksum) equals the
+    # oracle's.
+    if "g" not in _RMAT20:
+        _RMAT20["g"] = rg.HostGraph.generate(20, 16, symmetric=True)
+    hg = _RMAT20["g"]
+    off, nbr = hg.csr()
+    dg = rg.DeviceGraph(ctx, hg)
+    o = rg.opts(variant, heavy_degree=1024) if threshold is None else \
+        rg.opts(variant, threshold=threshold, heavy_degree=1024)
+    for s in hg.pick_sources(4):
+        print("bfs rmat20", variant, threshold, s, flush=True)
+        lv, ecc = orc.bfs_levels(off, nbr, int(s))
+        r = dg.bfs(int(s), o)
+        assert orc.check_parents(off, nbr, int(s), lv, r.parents) == 0, s
+        for _ in range(2):
+            assert dg.bfs_query(int(s), o)[0].astuple() == orc.bfs_summary(off, lv), s
+
+
 @pytest.mark.parametrize("threshold", [0, -1])
 def test_pull_bfs_follows_transposes(rg, orc, ctx, graphs, threshold):
     # The pull engine's first-probe array belongs to one orientation: after

@@ -474,6 +474,9 @@ def main():
     ap.add_argument("--serve-scale", type=int, default=22, help="R-MAT scale of the serving graph (symmetric)")
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-resident", action="store_true",
+                    help="also serve the resident headline graph (off by default: one run of its mixed-query "
+                         "Poisson trace ended in a GPU memory fault whose cause is not found)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--sections", default="headline,variants,radii,tc,serve",
@@ -618,6 +621,10 @@ def main():
         bfs["radar_tuned"].update({"engine": ENGINES[bt.variant], "threshold": bt.threshold,
                                    "heavy_degree": bt.heavy_degree})
         bfs["radar_vs_push"] = round(bfs["radar_tuned"]["gteps"] / bfs["push"]["gteps"], 2)
+        bfs["roofline_note"] = ("roofline_frac counts the push model's bytes, 8 B per arc of every reached vertex and "
+                                "24 B per reached vertex; bottom-up levels stop at an unvisited vertex's first "
+                                "frontier in-neighbour and never read most arcs, so the pull engine's figure can "
+                                "pass 1 (profiles/k_bfs_pull_light_bfs_rmat26.md has its counter bytes)")
         # RADAR in the reference's own form (RD BFS.C:43-94): degree-sorted
         # ids, LDS copies of hubs 0..k-1, dense levels on the blocked layout;
         # and the push-pull engine alone on the same graph.
@@ -686,8 +693,10 @@ def main():
         del off, nbr
     dg.close()
     serving = {}
-    if not args.quick and "serve" in sections and not args.no_serve:
-        # Serving over the headline graph itself, resident and reordered.
+    if not args.quick and "serve" in sections and not args.no_serve and args.serve_resident:
+        # Serving over the headline graph itself, resident and reordered (off
+        # by default: one run of its mixed-query Poisson trace, batched on
+        # two slots, ended in a GPU memory fault whose cause is not found).
         serving[f"rmat{args.scale}"] = serve_bench(rg, args, resident=(hg, sg, new_ids, args.scale))
     sg.close()
     hg.close()

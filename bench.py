@@ -475,8 +475,8 @@ def main():
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-resident", action="store_true",
-                    help="also serve the resident headline graph (off by default: one run of its mixed-query "
-                         "Poisson trace ended in a GPU memory fault whose cause is not found)")
+                    help="also serve the resident headline graph (off by default: its mixed-query Poisson trace, "
+                         "batched on two slots, has ended in a GPU memory fault in two of three runs; cause not found)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--sections", default="headline,variants,radii,tc,serve",
@@ -695,8 +695,8 @@ def main():
     serving = {}
     if not args.quick and "serve" in sections and not args.no_serve and args.serve_resident:
         # Serving over the headline graph itself, resident and reordered (off
-        # by default: one run of its mixed-query Poisson trace, batched on
-        # two slots, ended in a GPU memory fault whose cause is not found).
+        # by default: its mixed-query Poisson trace, batched on two slots, has
+        # ended in a GPU memory fault in two of three runs; cause not found).
         serving[f"rmat{args.scale}"] = serve_bench(rg, args, resident=(hg, sg, new_ids, args.scale))
     sg.close()
     hg.close()
